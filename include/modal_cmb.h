@@ -1,0 +1,168 @@
+/*
+ * modal_cmb.h -- C ABI of the MI355X (gfx950) Gamma / shape-projection library.
+ *
+ * Plain C types only (pointers, sizes, status codes).  Two layers:
+ *
+ *  1. An explicit-state API over an opaque context (one GPU, one stream).
+ *     Every call returns an int status (MCMB_OK == 0, negative on error) and
+ *     never exits; mcmb_last_error() gives the message of the calling
+ *     thread's last failure.
+ *  2. The reference's own entry points (same names, argument meaning and
+ *     array layouts as PetarSuman/Modal-CMB), implemented as shims over a
+ *     process-global context bound with mcmb_legacy_bind().  Each declaration
+ *     cites the reference prototype it replaces.
+ *
+ * Array layouts follow the reference: row-major FP64, n x n Gram matrices,
+ * basis tables [point][pmax+1], mode orders [mode][3].
+ */
+#ifndef MODAL_CMB_H
+#define MODAL_CMB_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define MCMB_OK            0
+#define MCMB_ERR_ARG      -1   /* bad argument or size                          */
+#define MCMB_ERR_HIP      -2   /* HIP runtime failure (message has the detail)  */
+#define MCMB_ERR_NOT_SPD  -3   /* Cholesky pivot <= 0: Gram not positive definite */
+#define MCMB_ERR_ORDER    -4   /* bad mode-order table                          */
+#define MCMB_ERR_IO       -5   /* file missing, short or unwritable             */
+#define MCMB_ERR_NOMEM    -6   /* device or host allocation failed              */
+#define MCMB_ERR_STATE    -7   /* required setup call not made                  */
+
+/* precision of the tetrapyd Gram contraction */
+#define MCMB_FP64   0   /* FP64 MFMA, FP64 accumulation                          */
+#define MCMB_MIXED  1   /* BF16 hi+lo split operands on BF16 MFMA (3 products),
+                           FP32 chunk accumulation, FP64 cross-chunk sums, FP64
+                           diagonal lo*lo correction                              */
+
+/* geometric factor h^2 */
+#define MCMB_H2_EXACT      0  /* (2l1+1)(2l2+1)(2l3+1)/(4pi) (l1 l2 l3;0 0 0)^2   */
+#define MCMB_H2_ASYMPTOTIC 1  /* calculate_geometric.c:59-92                     */
+
+typedef struct mcmb_ctx mcmb_ctx;
+
+/* tetrapyd_tools.h:31-37 */
+typedef struct { int i_bgn; int j_bgn; int k_bgn; int loops; } tetrapyd_limits;
+/* global.h:32-36 */
+typedef struct { double a1; double a2; double a3; } shape_params;
+
+const char* mcmb_last_error(void);
+const char* mcmb_status_string(int status);
+
+/* Create a context on HIP device `device` (own non-blocking stream). */
+int mcmb_create(mcmb_ctx** ctx, int device);
+int mcmb_destroy(mcmb_ctx* ctx);
+/* Block until all work queued on the context's stream has finished. */
+int mcmb_sync(mcmb_ctx* ctx);
+
+/* ---- late-time (l-space) Gram --------------------------------------------- */
+/* Load the 1D basis q[l][0..ldq) for l = 0..lmax (row l is multipole l; only
+ * l in [lmin,lmax] is read), per-leg weight v[l] (= 1/C_l for the config
+ * weight h^2/(C_l1 C_l2 C_l3)), and the mode order table order[n][3]
+ * (r,s,t < ldq).  Replaces the state create_basis_late / set_terms_late
+ * leave behind (fixed_data.c:272-1986).  Tables are copied to the device. */
+int mcmb_lspace_set(mcmb_ctx* ctx, int lmin, int lmax, int ldq, const double* q,
+                    const double* v, int nmodes, const int* order, int h2_mode);
+
+/* Gamma_mn = sum_{l1<=l2<=l3 in T(lmin,lmax)} permsix h^2 v1 v2 v3 Q_m Q_n
+ * (the tetrapyd contraction of calculate_orthol_3D, orthogonalisation_late.c:
+ * 338-618, and of calculate_gamma_TTT_3D_nested, calculate_gamma_3d.c:274-440,
+ * without the 6 dphi^2 scaling).  Writes the full symmetric n x n matrix to
+ * gamma_host (row-major) if non-NULL, and/or to the device buffer gamma_dev
+ * (n x n doubles) if non-NULL. */
+int mcmb_gamma_tet(mcmb_ctx* ctx, int precision, double* gamma_host, double* gamma_dev);
+
+/* The same Gram through the Gauss-Legendre factorisation (calculate_orthol,
+ * orthogonalisation_late.c:239-336) with npt nodes (npt <= 0: 3 lmax/2 + 1). */
+int mcmb_gamma_gl(mcmb_ctx* ctx, int npt, double* gamma_host, double* gamma_dev);
+
+/* lmin, lmax, mode count and a generation number that changes on every
+ * successful mcmb_lspace_set (any output pointer may be NULL). */
+int mcmb_lspace_info(mcmb_ctx* ctx, int* lmin, int* lmax, int* nmodes, long* generation);
+
+/* Number of tetrapyd points / rows of the current l-space setup. */
+long mcmb_lspace_npoints(mcmb_ctx* ctx);
+
+/* Expand the device point enumeration (rows of T in walk order) into
+ * (i,j,k) triples; writes min(cap, N) triples, returns N or a negative status.
+ * For bit-exact parity with get_ijk_next_XXX (tetrapyd_tools.c:104-142). */
+long mcmb_lspace_points(mcmb_ctx* ctx, int* ijk, long cap);
+
+/* Per-point weights w_p (walk order) computed on the device, up to cap. */
+long mcmb_lspace_weights(mcmb_ctx* ctx, double* w, long cap);
+
+/* ---- FP64 orthonormalisation on the GPU (ortho_prim_mpi.c:198-284) ------- */
+/* norm = sqrt(diag); C = D^-1 Gamma D^-1 = L L^T; ortho = symmetrised D L,
+ * lambda = symmetrised (D L)^-1 (update_ortho/update_lambda layouts,
+ * fixed_data.c:3285-3295, 3576-3586).  Host arrays, n x n.  Returns
+ * MCMB_ERR_NOT_SPD (and the failing pivot in *bad_pivot if non-NULL) when C is
+ * not positive definite in FP64. */
+int mcmb_orthonormalise(mcmb_ctx* ctx, int n, const double* gamma, double* ortho,
+                        double* lambda, double* norm, int* bad_pivot);
+/* beta = Gamma^-1 alpha for nrhs right-hand sides (column j = alpha + j*n),
+ * with the symmetrised D L factor from mcmb_orthonormalise
+ * (gsl_linalg_cholesky_solve, eigenmode_mpi.c:211). */
+int mcmb_cholesky_solve(mcmb_ctx* ctx, int n, const double* ortho, int nrhs,
+                        const double* alpha, double* beta);
+
+/* ---- quadrature ----------------------------------------------------------- */
+/* n-node Gauss-Legendre rule on [-1,1], nodes ascending (host only, no GPU). */
+int mcmb_gauss_legendre(int n, double* x, double* w);
+
+/* ---- timing of the last Gram launch sequence ------------------------------ */
+/* Milliseconds spent in the dominant kernel of the last mcmb_gamma_tet /
+ * mcmb_gamma_gl call (HIP events on the context stream) and its launch count. */
+int mcmb_last_kernel_ms(mcmb_ctx* ctx, double* ms, int* launches);
+
+/* ---- reference-named entry points (legacy shims over the bound context) --- */
+/* Bind the context the legacy symbols use (not owned). */
+int mcmb_legacy_bind(mcmb_ctx* ctx);
+/* Status of the last void legacy call (gamma_3d_host, get_next_ijk_bound). */
+int mcmb_legacy_status(void);
+
+/* tetrapyd_tools.h:53-65,98-111 */
+int  get_ijk_next_XXX(int lmax, int* i, int* j, int* k);
+void check_ijk_min_XXX(int lmin, int* i, int* j, int* k);
+void check_ijk_max_XXX(int lmax, int* i, int* j, int* k);
+void decompose_tetrapyd_XXX(tetrapyd_limits* thread_limits, int rank, int numranks,
+                            int nthreads, int lmin, int lmax);
+int  get_ijk_next_prim(int max, int* i, int* j, int* k);
+void check_ijk_min_prim(int min, int* i, int* j, int* k);
+void check_ijk_max_prim(int max, int* i, int* j, int* k);
+void decompose_tetrapyd_prim(tetrapyd_limits* thread_limits, int rank, int numranks,
+                             int nthreads, int min, int max);
+void divide_tasks(long int tasks, int nranks, long int* tasks_per_rank);
+/* get_next_ijk_bound with an explicit lsize = lmax + 1 (host only) */
+void mcmb_next_ijk_bound(int lsize, int i0, int j0, int k0, int* i1, int* j1, int* k1,
+                         int final_i, int final_j, int final_k, int* blocksize,
+                         int maxblocksize);
+/* calculate_gamma_3d.h:72-73 (lsize taken from the bound l-space setup) */
+void get_next_ijk_bound(int i0, int j0, int k0, int* i1, int* j1, int* k1,
+                        int final_i, int final_j, int final_k, int* blocksize,
+                        int maxblocksize);
+/* fixed_data.c:4244 */
+double permsix(int i, int j, int k);
+/* calculate_geometric.c:59 */
+double calculate_geometric(int l1, int l2, int l3);
+/* basis_functions.c:88 */
+int basis_functions_legendre(double** array, int size, int pmax, double min, double max,
+                             double* values);
+/* gamma_mpi.c:30 -- fills the caller's row-major gamma_size x gamma_size
+ * buffer (zeroed first) with the FP64 tetrapyd Gram of the bound context;
+ * rank/numranks/workshare are accepted and ignored (one GPU, whole tetrapyd). */
+void gamma_3d_host(double* gamma_flat, int gamma_size, int rank, int numranks,
+                   double workshare);
+/* orthogonalisation_late.c:338 -- result[0] = Gamma_mn (TTT) of the bound
+ * context's cached FP64 tetrapyd Gram (computed on first use). */
+int calculate_orthol_3D(int m, int n, double* result);
+/* orthogonalisation_late.c:239 -- result[0] = Gamma_mn from the cached
+ * Gauss-Legendre-factorised Gram. */
+int calculate_orthol(int m, int n, double* result);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,250 @@
+// Context, error reporting and l-space setup of libmodalcmb.
+#include "mcmb_internal.h"
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+namespace mcmb {
+
+static thread_local char g_msg[512] = "";
+
+int fail(int status, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_msg, sizeof(g_msg), fmt, ap);
+    va_end(ap);
+    return status;
+}
+
+int DevBuf::alloc(size_t b)
+{
+    if (b <= bytes && p) return MCMB_OK;
+    release();
+    if (b == 0) return MCMB_OK;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+        p = nullptr;
+        bytes = 0;
+        return fail(MCMB_ERR_NOMEM, "hipMalloc(%zu) failed: %s", b, hipGetErrorString(e));
+    }
+    bytes = b;
+    return MCMB_OK;
+}
+
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+int set_device(mcmb_ctx* ctx)
+{
+    MCMB_HIP(hipSetDevice(ctx->device));
+    return MCMB_OK;
+}
+
+// Rows of T(lmin, lmax) in the order get_ijk_next_XXX walks them
+// (tetrapyd_tools.c:104-142): i ascending from lmin, j from i up to
+// lmax - i%2, k from j + i%2 in steps of 2 up to min(i + j, lmax).
+void build_lspace_rows(int lmin, int lmax, std::vector<RowDesc>& rows, long& npoints)
+{
+    rows.clear();
+    npoints = 0;
+    for (int i = lmin; i <= lmax; ++i) {
+        int par = i & 1;
+        for (int j = i; j <= lmax - par; ++j) {
+            int kb = j + par;
+            int ke = (i + j < lmax) ? i + j : lmax;
+            if (kb > ke) continue;
+            int nk = (ke - kb) / 2 + 1;
+            rows.push_back(RowDesc{i, j, kb, nk});
+            npoints += nk;
+        }
+    }
+}
+
+}  // namespace mcmb
+
+using namespace mcmb;
+
+extern "C" {
+
+const char* mcmb_last_error(void) { return g_msg; }
+
+const char* mcmb_status_string(int s)
+{
+    switch (s) {
+    case MCMB_OK: return "ok";
+    case MCMB_ERR_ARG: return "bad argument";
+    case MCMB_ERR_HIP: return "HIP runtime error";
+    case MCMB_ERR_NOT_SPD: return "matrix not positive definite";
+    case MCMB_ERR_ORDER: return "bad mode order table";
+    case MCMB_ERR_IO: return "I/O error";
+    case MCMB_ERR_NOMEM: return "out of memory";
+    case MCMB_ERR_STATE: return "setup call missing";
+    default: return "unknown status";
+    }
+}
+
+int mcmb_create(mcmb_ctx** out, int device)
+{
+    if (!out) return fail(MCMB_ERR_ARG, "mcmb_create: null output pointer");
+    *out = nullptr;
+    int ndev = 0;
+    MCMB_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return fail(MCMB_ERR_ARG, "mcmb_create: device %d not in [0,%d)", device, ndev);
+    mcmb_ctx* c = new mcmb_ctx();
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(MCMB_ERR_HIP, "mcmb_create: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return MCMB_OK;
+}
+
+int mcmb_destroy(mcmb_ctx* c)
+{
+    if (!c) return MCMB_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->ls.d_rows.release();
+    c->ls.d_q.release();
+    c->ls.d_qf.release();
+    c->ls.d_v.release();
+    c->ls.d_ftab.release();
+    c->ls.d_modes.release();
+    c->ls.d_gamma_cache.release();
+    c->ls.d_gl_cache.release();
+    c->d_partial.release();
+    c->d_gamma_tmp.release();
+    c->d_aux.release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MCMB_OK;
+}
+
+int mcmb_sync(mcmb_ctx* c)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_sync: null context");
+    MCMB_TRY(set_device(c));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return MCMB_OK;
+}
+
+int mcmb_lspace_set(mcmb_ctx* c, int lmin, int lmax, int ldq, const double* q, const double* v,
+                    int nmodes, const int* order, int h2_mode)
+{
+    if (!c || !q || !v || !order) return fail(MCMB_ERR_ARG, "mcmb_lspace_set: null argument");
+    if (lmin < 0 || lmax < lmin + 1 || lmax > 8190)
+        return fail(MCMB_ERR_ARG, "mcmb_lspace_set: need 0 <= lmin < lmax <= 8190 (got %d, %d)", lmin, lmax);
+    if (ldq < 1 || ldq > 64) return fail(MCMB_ERR_ARG, "mcmb_lspace_set: ldq %d not in [1,64]", ldq);
+    if (nmodes < 1 || nmodes > 65536) return fail(MCMB_ERR_ARG, "mcmb_lspace_set: nmodes %d", nmodes);
+    if (h2_mode != MCMB_H2_EXACT && h2_mode != MCMB_H2_ASYMPTOTIC)
+        return fail(MCMB_ERR_ARG, "mcmb_lspace_set: h2_mode %d", h2_mode);
+    int R = 0;
+    for (int m = 0; m < nmodes; ++m)
+        for (int a = 0; a < 3; ++a) {
+            int p = order[3 * m + a];
+            if (p < 0 || p >= ldq)
+                return fail(MCMB_ERR_ORDER, "mcmb_lspace_set: order[%d][%d] = %d outside [0,%d)", m, a, p, ldq);
+            if (p + 1 > R) R = p + 1;
+        }
+    MCMB_TRY(set_device(c));
+    LSpace& L = c->ls;
+    L.ready = false;
+    L.gamma_cached = false;
+    L.gl_cached = false;
+    L.lmin = lmin;
+    L.lmax = lmax;
+    L.ldq = ldq;
+    L.n = nmodes;
+    L.R = R;
+    L.h2mode = h2_mode;
+    L.order.assign(order, order + 3 * (size_t)nmodes);
+    L.q.assign(q, q + (size_t)(lmax + 1) * ldq);
+    L.v.assign(v, v + (size_t)(lmax + 1));
+    build_lspace_rows(lmin, lmax, L.rows, L.npoints);
+    // f(a) = prod_{k<=a}(2k-1)/(2k) in long double, a <= 3 lmax / 2 + 1
+    int amax = 3 * lmax / 2 + 2;
+    std::vector<double> ftab(amax + 1);
+    long double acc = 1.0L;
+    ftab[0] = 1.0;
+    for (int a = 1; a <= amax; ++a) {
+        acc *= (long double)(2 * a - 1) / (long double)(2 * a);
+        ftab[a] = (double)acc;
+    }
+    std::vector<int> modes(nmodes);
+    for (int m = 0; m < nmodes; ++m)
+        modes[m] = order[3 * m] | (order[3 * m + 1] << 8) | (order[3 * m + 2] << 16);
+    MCMB_TRY(L.d_rows.alloc(sizeof(RowDesc) * L.rows.size()));
+    MCMB_TRY(L.d_q.alloc(sizeof(double) * L.q.size()));
+    MCMB_TRY(L.d_v.alloc(sizeof(double) * L.v.size()));
+    MCMB_TRY(L.d_ftab.alloc(sizeof(double) * ftab.size()));
+    MCMB_TRY(L.d_modes.alloc(sizeof(int) * modes.size()));
+    MCMB_HIP(hipMemcpyAsync(L.d_rows.p, L.rows.data(), sizeof(RowDesc) * L.rows.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(L.d_q.p, L.q.data(), sizeof(double) * L.q.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(L.d_v.p, L.v.data(), sizeof(double) * L.v.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(L.d_ftab.p, ftab.data(), sizeof(double) * ftab.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(L.d_modes.p, modes.data(), sizeof(int) * modes.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    L.ready = true;
+    L.generation += 1;
+    return MCMB_OK;
+}
+
+int mcmb_lspace_info(mcmb_ctx* c, int* lmin, int* lmax, int* nmodes, long* generation)
+{
+    if (!c || !c->ls.ready) return fail(MCMB_ERR_STATE, "mcmb_lspace_info: no l-space setup");
+    if (lmin) *lmin = c->ls.lmin;
+    if (lmax) *lmax = c->ls.lmax;
+    if (nmodes) *nmodes = c->ls.n;
+    if (generation) *generation = c->ls.generation;
+    return MCMB_OK;
+}
+
+long mcmb_lspace_npoints(mcmb_ctx* c)
+{
+    if (!c || !c->ls.ready) return fail(MCMB_ERR_STATE, "mcmb_lspace_npoints: no l-space setup");
+    return c->ls.npoints;
+}
+
+int mcmb_last_kernel_ms(mcmb_ctx* c, double* ms, int* launches)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_last_kernel_ms: null context");
+    if (ms) *ms = c->last_ms;
+    if (launches) *launches = c->last_launches;
+    return MCMB_OK;
+}
+
+int mcmb_gamma_tet(mcmb_ctx* c, int precision, double* gamma_host, double* gamma_dev)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_gamma_tet: null context");
+    if (!c->ls.ready) return fail(MCMB_ERR_STATE, "mcmb_gamma_tet: call mcmb_lspace_set first");
+    if (precision != MCMB_FP64 && precision != MCMB_MIXED)
+        return fail(MCMB_ERR_ARG, "mcmb_gamma_tet: precision %d", precision);
+    MCMB_TRY(set_device(c));
+    int n = c->ls.n;
+    double* dst = gamma_dev;
+    if (!dst) {
+        MCMB_TRY(c->d_gamma_tmp.alloc(sizeof(double) * (size_t)n * n));
+        dst = c->d_gamma_tmp.as<double>();
+    }
+    MCMB_TRY(gram_tet_launch(c, precision, dst));
+    if (gamma_host)
+        MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return MCMB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,363 @@
+// Tetrapyd Gram contraction  Gamma = sum_p w_p Q(p) Q(p)^T  on gfx950.
+//
+// Work decomposition.  The ordered tetrapyd is a list of rows (i, j) with
+// k = k0 + kstep*t, t < nk (walk order of get_ijk_next_XXX).  A row is cut
+// into chunks of KC consecutive points; chunks never cross rows, so within a
+// chunk the mode product factorises as
+//   Q_m(i,j,k) = (1/6) [S_rs q_t(k) + S_rt q_s(k) + S_st q_r(k)],
+//   S_ab = q_a(i) q_b(j) + q_b(i) q_a(j)      ((r,s,t) = order[m]),
+// three FMAs per (point, mode).  The output is tiled into TM x TM mode tiles
+// (I <= J, upper triangle of tiles); the point rows are cut into segments of
+// equal chunk count.  One workgroup owns one (tile, segment): it forms the
+// operand tiles A[kk][m] = sqrt(w) Q_m / ... in LDS (double buffered, chunk
+// c+1 formed while chunk c is contracted) and accumulates A^T B with MFMA in
+// registers; per-segment partial tiles are summed in a fixed order by a
+// reduction kernel, so results are deterministic.
+#include "mcmb_internal.h"
+#include <algorithm>
+#include <cmath>
+
+namespace mcmb {
+
+constexpr int TM = 128;        // mode tile (FP64 path)
+constexpr int KC = 16;         // points per chunk
+constexpr int NT = 256;        // threads per workgroup (4 waves)
+constexpr int LDA = TM + 4;    // LDS row stride (doubles), staggers banks
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ double permsix_d(int i, int j, int k)
+{
+    return (i == j && j == k) ? 1.0 : ((i == j || j == k || i == k) ? 3.0 : 6.0);
+}
+
+// Exact h^2 = (2l1+1)(2l2+1)(2l3+1)/(4 pi) * f(g-l1) f(g-l2) f(g-l3)/((2g+1) f(g)),
+// f(a) = prod_{k<=a}(2k-1)/(2k), g = (l1+l2+l3)/2 (even-parity triangle only).
+__device__ __forceinline__ double h2_exact(int i, int j, int k, const double* __restrict__ f)
+{
+    int g = (i + j + k) >> 1;
+    double pre = (2.0 * i + 1.0) * (2.0 * j + 1.0) * (2.0 * k + 1.0) * 0.079577471545947667884;  // 1/(4 pi)
+    return pre * (f[g - i] * f[g - j] * f[g - k]) / ((2.0 * g + 1.0) * f[g]);
+}
+
+// calculate_geometric.c:59-92 (the reference's asymptotic h^2)
+__device__ __forceinline__ double h2_asym(int l1, int l2, int l3)
+{
+    double L = (double)(l1 + l2 + l3);
+    double a1 = (double)(l2 + l3 - l1), a2 = (double)(l3 + l1 - l2), a3 = (double)(l1 + l2 - l3);
+    double pref = 0.31830988618379067154 * 0.31830988618379067154 / 2.0;
+    double p1 = pref * ((2.0 * l1 + 1.0) * (2.0 * l2 + 1.0) * (2.0 * l3 + 1.0)) / (L + 1.0);
+    double p2 = (L + 1.0 / 3.0) / ((a1 + 1.0 / 3.0) * (a2 + 1.0 / 3.0) * (a3 + 1.0 / 3.0));
+    double p3 = sqrt((a1 + 1.0 / 6.0) * (a2 + 1.0 / 6.0) * (a3 + 1.0 / 6.0) / (L + 1.0 / 6.0));
+    return p1 * p2 * p3;
+}
+
+__device__ __forceinline__ double lspace_weight(int i, int j, int k, int asym,
+                                                const double* __restrict__ f,
+                                                const double* __restrict__ v)
+{
+    double h2 = asym ? h2_asym(i, j, k) : h2_exact(i, j, k, f);
+    return permsix_d(i, j, k) * h2 * v[i] * v[j] * v[k];
+}
+
+// One wave per row: w_p (mode 0) or sqrt(w_p)/6 (mode 1), p in walk order.
+__global__ __launch_bounds__(256) void lspace_weights_kernel(
+    const RowDesc* __restrict__ rows, const long* __restrict__ rowptr, long nrows, int asym,
+    const double* __restrict__ f, const double* __restrict__ v, int scaled,
+    double* __restrict__ out)
+{
+    long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= nrows) return;
+    RowDesc rd = rows[row];
+    long base = rowptr[row];
+    for (int t = threadIdx.x & 63; t < rd.nk; t += 64) {
+        int k = rd.k0 + 2 * t;
+        double w = lspace_weight(rd.i, rd.j, k, asym, f, v);
+        out[base + t] = scaled ? sqrt(w) * (1.0 / 6.0) : w;
+    }
+}
+
+__global__ __launch_bounds__(256) void lspace_points_kernel(
+    const RowDesc* __restrict__ rows, const long* __restrict__ rowptr, long nrows, long cap,
+    int* __restrict__ ijk)
+{
+    long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= nrows) return;
+    RowDesc rd = rows[row];
+    long base = rowptr[row];
+    for (int t = threadIdx.x & 63; t < rd.nk; t += 64) {
+        long p = base + t;
+        if (p < cap) {
+            ijk[3 * p] = rd.i;
+            ijk[3 * p + 1] = rd.j;
+            ijk[3 * p + 2] = rd.k0 + 2 * t;
+        }
+    }
+}
+
+struct ModeInfo {
+    int r, s, t;
+    bool ok;
+};
+
+__device__ __forceinline__ ModeInfo load_mode(const int* __restrict__ modes, int m, int n)
+{
+    ModeInfo mi;
+    if (m < n) {
+        int pk = modes[m];
+        mi.r = pk & 255; mi.s = (pk >> 8) & 255; mi.t = (pk >> 16) & 255; mi.ok = true;
+    } else {
+        mi.r = mi.s = mi.t = 0; mi.ok = false;
+    }
+    return mi;
+}
+
+// Form KC/2 operand values of one mode for one chunk into LDS column mloc.
+__device__ __forceinline__ void form_f64(const RowDesc& rd, int c0, long pbase, int kstep,
+                                         const ModeInfo& mi, const double* __restrict__ q, int ldq,
+                                         const double* __restrict__ ws, double* __restrict__ buf,
+                                         int mloc, int kb)
+{
+    const double* qi = q + (long)rd.i * ldq;
+    const double* qj = q + (long)rd.j * ldq;
+    double Srs = 0.0, Srt = 0.0, Sst = 0.0;
+    if (mi.ok) {
+        double ir = qi[mi.r], is = qi[mi.s], it = qi[mi.t];
+        double jr = qj[mi.r], js = qj[mi.s], jt = qj[mi.t];
+        Srs = ir * js + is * jr;
+        Srt = ir * jt + it * jr;
+        Sst = is * jt + it * js;
+    }
+#pragma unroll
+    for (int e = 0; e < KC / 2; ++e) {
+        int kk = kb + e;
+        int t = c0 + kk;
+        double a = 0.0;
+        if (t < rd.nk) {
+            const double* qk = q + (long)(rd.k0 + kstep * t) * ldq;
+            a = ws[pbase + kk] * (Srs * qk[mi.t] + Srt * qk[mi.s] + Sst * qk[mi.r]);
+        }
+        buf[kk * LDA + mloc] = a;
+    }
+}
+
+__global__ __launch_bounds__(NT, 2) void gram_tet_f64_kernel(
+    const RowDesc* __restrict__ rows, const long* __restrict__ rowptr,
+    const int* __restrict__ seg_rows, int nseg, const int2* __restrict__ tiles, int ntiles,
+    const int* __restrict__ modes, int n, const double* __restrict__ q, int ldq, int kstep,
+    const double* __restrict__ ws, double* __restrict__ partial, int n_pad)
+{
+    __shared__ double sA[2][KC * LDA];
+    __shared__ double sB[2][KC * LDA];
+    const int tile = blockIdx.x % ntiles;
+    const int seg = blockIdx.x / ntiles;
+    const int2 IJ = tiles[tile];
+    const bool diag = IJ.x == IJ.y;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int mloc = tid & (TM - 1), kb = (tid >> 7) * (KC / 2);
+    const ModeInfo mA = load_mode(modes, IJ.x * TM + mloc, n);
+    const ModeInfo mB = load_mode(modes, IJ.y * TM + mloc, n);
+    const int wm = wave >> 1, wn = wave & 1;
+
+    d4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+
+    int row = seg_rows[seg];
+    const int row_end = seg_rows[seg + 1];
+    int c0 = 0;
+    if (row < row_end) {
+        RowDesc rd = rows[row];
+        long pb = rowptr[row];
+        form_f64(rd, 0, pb, kstep, mA, q, ldq, ws, sA[0], mloc, kb);
+        if (!diag) form_f64(rd, 0, pb, kstep, mB, q, ldq, ws, sB[0], mloc, kb);
+        __syncthreads();
+        int cur = 0;
+        while (true) {
+            // advance to the next chunk (uniform over the workgroup)
+            int nrow = row, nc0 = c0 + KC;
+            if (nc0 >= rows[row].nk) { ++nrow; nc0 = 0; }
+            const bool has_next = nrow < row_end;
+            if (has_next) {
+                RowDesc nd = rows[nrow];
+                long npb = rowptr[nrow] + nc0;
+                form_f64(nd, nc0, npb, kstep, mA, q, ldq, ws, sA[cur ^ 1], mloc, kb);
+                if (!diag) form_f64(nd, nc0, npb, kstep, mB, q, ldq, ws, sB[cur ^ 1], mloc, kb);
+            }
+            const double* A = sA[cur];
+            const double* B = diag ? sA[cur] : sB[cur];
+#pragma unroll
+            for (int ks = 0; ks < KC / 4; ++ks) {
+                const int kk = ks * 4 + (lane >> 4);
+                double av[4], bv[4];
+#pragma unroll
+                for (int br = 0; br < 4; ++br) av[br] = A[kk * LDA + wm * 64 + br * 16 + (lane & 15)];
+#pragma unroll
+                for (int bc = 0; bc < 4; ++bc) bv[bc] = B[kk * LDA + wn * 64 + bc * 16 + (lane & 15)];
+#pragma unroll
+                for (int br = 0; br < 4; ++br)
+#pragma unroll
+                    for (int bc = 0; bc < 4; ++bc)
+                        acc[br][bc] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[br], bv[bc], acc[br][bc], 0, 0, 0);
+            }
+            __syncthreads();
+            if (!has_next) break;
+            row = nrow;
+            c0 = nc0;
+            cur ^= 1;
+        }
+    }
+    // D layout of v_mfma_f64_16x16x4: lane holds rows 4v + lane/16, column lane%16.
+    double* P = partial + (size_t)seg * n_pad * n_pad;
+#pragma unroll
+    for (int br = 0; br < 4; ++br)
+#pragma unroll
+        for (int bc = 0; bc < 4; ++bc)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int m = IJ.x * TM + wm * 64 + br * 16 + 4 * v + (lane >> 4);
+                int c = IJ.y * TM + wn * 64 + bc * 16 + (lane & 15);
+                P[(size_t)m * n_pad + c] = acc[br][bc][v];
+            }
+}
+
+// Gamma[m][c] = Gamma[c][m] = sum_seg partial[seg][m][c] for m <= c < n.
+__global__ __launch_bounds__(256) void gram_reduce_kernel(const double* __restrict__ partial,
+                                                          int nseg, int n_pad, int n,
+                                                          double* __restrict__ out)
+{
+    int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    int m = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (m >= n || c >= n || m > c) return;
+    double s = 0.0;
+    size_t off = (size_t)m * n_pad + c, stride = (size_t)n_pad * n_pad;
+    for (int g = 0; g < nseg; ++g) s += partial[off + g * stride];
+    out[(size_t)m * n + c] = s;
+    out[(size_t)c * n + m] = s;
+}
+
+// Split rows into nseg segments of about equal chunk count.
+static void make_segments(const std::vector<RowDesc>& rows, int kc, int nseg, std::vector<int>& seg)
+{
+    std::vector<long> pre(rows.size() + 1, 0);
+    for (size_t r = 0; r < rows.size(); ++r) pre[r + 1] = pre[r] + (rows[r].nk + kc - 1) / kc;
+    long tot = pre.back();
+    seg.assign(nseg + 1, 0);
+    for (int s = 1; s < nseg; ++s) {
+        long target = tot * s / nseg;
+        seg[s] = (int)(std::lower_bound(pre.begin(), pre.end(), target) - pre.begin());
+        if (seg[s] < seg[s - 1]) seg[s] = seg[s - 1];
+    }
+    seg[nseg] = (int)rows.size();
+}
+
+static int ensure_lspace_aux(mcmb_ctx* c, DevBuf& rowptr, DevBuf& ws, int scaled)
+{
+    LSpace& L = c->ls;
+    std::vector<long> rp(L.rows.size() + 1, 0);
+    for (size_t r = 0; r < L.rows.size(); ++r) rp[r + 1] = rp[r] + L.rows[r].nk;
+    MCMB_TRY(rowptr.alloc(sizeof(long) * rp.size()));
+    MCMB_HIP(hipMemcpyAsync(rowptr.p, rp.data(), sizeof(long) * rp.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_TRY(ws.alloc(sizeof(double) * (size_t)L.npoints));
+    long nrows = (long)L.rows.size();
+    dim3 grid((unsigned)((nrows + 3) / 4));
+    hipLaunchKernelGGL(lspace_weights_kernel, grid, dim3(256), 0, c->stream, L.d_rows.as<RowDesc>(),
+                       rowptr.as<long>(), nrows, L.h2mode == MCMB_H2_ASYMPTOTIC ? 1 : 0,
+                       L.d_ftab.as<double>(), L.d_v.as<double>(), scaled, ws.as<double>());
+    MCMB_HIP(hipGetLastError());
+    return MCMB_OK;
+}
+
+int gram_mixed_launch(mcmb_ctx* c, const long* d_rowptr, const double* d_ws, double* d_gamma);
+
+int gram_tet_launch(mcmb_ctx* c, int precision, double* d_gamma)
+{
+    LSpace& L = c->ls;
+    DevBuf rowptr, ws;
+    MCMB_TRY(ensure_lspace_aux(c, rowptr, ws, 1));
+    if (precision == MCMB_MIXED) return gram_mixed_launch(c, rowptr.as<long>(), ws.as<double>(), d_gamma);
+
+    const int n = L.n;
+    const int T = (n + TM - 1) / TM, n_pad = T * TM;
+    std::vector<int2> tiles;
+    for (int I = 0; I < T; ++I)
+        for (int J = I; J < T; ++J) tiles.push_back(make_int2(I, J));
+    const int ntiles = (int)tiles.size();
+    // ~8 workgroups per CU in flight over the launch, bounded by rows and by
+    // 4 GiB of partial tiles
+    long target = 2048;
+    int nseg = (int)std::max<long>(1, (target + ntiles - 1) / ntiles);
+    nseg = (int)std::min<long>(nseg, (long)L.rows.size());
+    size_t per = (size_t)n_pad * n_pad * sizeof(double);
+    nseg = (int)std::max<size_t>(1, std::min<size_t>((size_t)nseg, ((size_t)4 << 30) / per));
+    std::vector<int> seg;
+    make_segments(L.rows, KC, nseg, seg);
+
+    DevBuf d_tiles, d_seg;
+    MCMB_TRY(d_tiles.alloc(sizeof(int2) * tiles.size()));
+    MCMB_TRY(d_seg.alloc(sizeof(int) * seg.size()));
+    MCMB_TRY(c->d_partial.alloc(per * nseg));
+    MCMB_HIP(hipMemcpyAsync(d_tiles.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(d_seg.p, seg.data(), sizeof(int) * seg.size(), hipMemcpyHostToDevice, c->stream));
+
+    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(gram_tet_f64_kernel, dim3((unsigned)(ntiles * nseg)), dim3(NT), 0, c->stream,
+                       L.d_rows.as<RowDesc>(), rowptr.as<long>(), d_seg.as<int>(), nseg,
+                       d_tiles.as<int2>(), ntiles, L.d_modes.as<int>(), n, L.d_q.as<double>(), L.ldq, 2,
+                       ws.as<double>(), c->d_partial.as<double>(), n_pad);
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipEventRecord(c->ev1, c->stream));
+    dim3 rgrid((unsigned)((n + 63) / 64), (unsigned)((n + 3) / 4));
+    hipLaunchKernelGGL(gram_reduce_kernel, rgrid, dim3(256), 0, c->stream, c->d_partial.as<double>(),
+                       nseg, n_pad, n, d_gamma);
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = 1;
+    return MCMB_OK;
+}
+
+}  // namespace mcmb
+
+using namespace mcmb;
+
+extern "C" long mcmb_lspace_points(mcmb_ctx* c, int* ijk, long cap)
+{
+    if (!c || !c->ls.ready) return fail(MCMB_ERR_STATE, "mcmb_lspace_points: no l-space setup");
+    if (cap < 0 || (cap > 0 && !ijk)) return fail(MCMB_ERR_ARG, "mcmb_lspace_points: bad buffer");
+    MCMB_TRY(set_device(c));
+    LSpace& L = c->ls;
+    long n = L.npoints;
+    long m = std::min(cap, n);
+    if (m == 0) return n;
+    std::vector<long> rp(L.rows.size() + 1, 0);
+    for (size_t r = 0; r < L.rows.size(); ++r) rp[r + 1] = rp[r] + L.rows[r].nk;
+    DevBuf rowptr, out;
+    MCMB_TRY(rowptr.alloc(sizeof(long) * rp.size()));
+    MCMB_TRY(out.alloc(sizeof(int) * 3 * (size_t)m));
+    MCMB_HIP(hipMemcpyAsync(rowptr.p, rp.data(), sizeof(long) * rp.size(), hipMemcpyHostToDevice, c->stream));
+    long nrows = (long)L.rows.size();
+    hipLaunchKernelGGL(lspace_points_kernel, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, c->stream,
+                       L.d_rows.as<RowDesc>(), rowptr.as<long>(), nrows, m, out.as<int>());
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipMemcpyAsync(ijk, out.p, sizeof(int) * 3 * (size_t)m, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return n;
+}
+
+extern "C" long mcmb_lspace_weights(mcmb_ctx* c, double* w, long cap)
+{
+    if (!c || !c->ls.ready) return fail(MCMB_ERR_STATE, "mcmb_lspace_weights: no l-space setup");
+    if (cap < 0 || (cap > 0 && !w)) return fail(MCMB_ERR_ARG, "mcmb_lspace_weights: bad buffer");
+    MCMB_TRY(set_device(c));
+    DevBuf rowptr, ws;
+    MCMB_TRY(ensure_lspace_aux(c, rowptr, ws, 0));
+    long m = std::min(cap, c->ls.npoints);
+    if (m > 0) MCMB_HIP(hipMemcpyAsync(w, ws.p, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return c->ls.npoints;
+}

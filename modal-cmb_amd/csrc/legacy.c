@@ -1,0 +1,299 @@
+/* Reference-named entry points of PetarSuman/Modal-CMB, implemented over the
+ * library's own host code and the bound GPU context.  Same names, argument
+ * meaning and array layouts as the reference; errors are recorded (query
+ * mcmb_legacy_status / mcmb_last_error) instead of exiting. */
+#include "modal_cmb.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static mcmb_ctx* g_ctx = NULL;
+static int g_lsize = 0;          /* lmax + 1 of the bound l-space setup */
+static int g_status = MCMB_OK;   /* last status of a void legacy call */
+static double* g_gamma3d = NULL; /* cached FP64 tetrapyd Gram, n x n */
+static double* g_gammagl = NULL; /* cached GL-factorised Gram, n x n */
+static int g_n = 0;
+static long g_gen = -1;          /* l-space generation the caches belong to */
+
+int mcmb_legacy_bind(mcmb_ctx* ctx)
+{
+    g_ctx = ctx;
+    free(g_gamma3d); g_gamma3d = NULL;
+    free(g_gammagl); g_gammagl = NULL;
+    g_n = 0;
+    g_gen = -1;
+    g_status = MCMB_OK;
+    return MCMB_OK;
+}
+
+/* Refresh sizes from the bound context; drop caches of an older setup. */
+static int refresh(void)
+{
+    int lmin, lmax, n;
+    long gen;
+    if (!g_ctx) return MCMB_ERR_STATE;
+    int st = mcmb_lspace_info(g_ctx, &lmin, &lmax, &n, &gen);
+    if (st != MCMB_OK) return st;
+    if (gen != g_gen) {
+        free(g_gamma3d); g_gamma3d = NULL;
+        free(g_gammagl); g_gammagl = NULL;
+        g_gen = gen;
+    }
+    g_lsize = lmax + 1;
+    g_n = n;
+    return MCMB_OK;
+}
+
+int mcmb_legacy_status(void) { return g_status; }
+
+/* ---- tetrapyd walks ------------------------------------------------------ */
+
+/* tetrapyd_tools.c:104-142: next point of {i<=j<=k<=lmax, k<=i+j, even sum};
+ * 0 when the step lands on the final point. */
+int get_ijk_next_XXX(int lmax, int* pi, int* pj, int* pk)
+{
+    int i = *pi, j = *pj, k = *pk;
+    int odd = i & 1;
+    if (j + odd == lmax) { ++i; j = i; k = j + (i & 1); }
+    else if (k + 2 > lmax || k >= i + j) { ++j; k = j + odd; }
+    else k += 2;
+    int more = 1;
+    if (i + j + k + 1 >= 3 * lmax) {
+        more = 0;
+        i = (lmax & 1) ? lmax - 1 : lmax;
+        j = k = lmax;
+    }
+    *pi = i; *pj = j; *pk = k;
+    return more;
+}
+
+static void order3(int* a, int* b, int* c)
+{
+    int t;
+    if (*a > *b) { t = *a; *a = *b; *b = t; }
+    if (*b > *c) { t = *b; *b = *c; *c = t; if (*a > *b) { t = *a; *a = *b; *b = t; } }
+}
+
+/* tetrapyd_tools.c:144-180 */
+void check_ijk_min_XXX(int lmin, int* pi, int* pj, int* pk)
+{
+    int i = *pi < lmin ? lmin : *pi, j = *pj < lmin ? lmin : *pj, k = *pk < lmin ? lmin : *pk;
+    order3(&i, &j, &k);
+    if (k > i + j) k = i + j;
+    else if ((i + j + k) & 1) ++k;
+    *pi = i; *pj = j; *pk = k;
+}
+
+/* tetrapyd_tools.c:182-224 */
+void check_ijk_max_XXX(int lmax, int* pi, int* pj, int* pk)
+{
+    int i = *pi > lmax ? lmax : *pi, j = *pj > lmax ? lmax : *pj, k = *pk > lmax ? lmax : *pk;
+    order3(&i, &j, &k);
+    if (k > i + j) k = i + j;
+    else if ((i + j + k) & 1) {
+        if (k > j) --k;
+        else if (j > i) --j;
+        else --i;
+    }
+    *pi = i; *pj = j; *pk = k;
+}
+
+/* tetrapyd_tools.c:595-608 */
+void divide_tasks(long int tasks, int nranks, long int* per)
+{
+    long int q = tasks / nranks, r = tasks % nranks;
+    for (int w = 0; w < nranks; ++w) per[w] = q + (w < r);
+}
+
+/* tetrapyd_tools.c:11-102 */
+void decompose_tetrapyd_XXX(tetrapyd_limits* blk, int rank, int nranks, int nthreads, int lmin, int lmax)
+{
+    int i0 = lmin, j0 = lmin, k0 = lmin;
+    check_ijk_min_XXX(lmin, &i0, &j0, &k0);
+    int i = i0, j = j0, k = k0;
+    long cap = (long)lmax * lmax * lmax + 1, count = 1;
+    while (count < cap && get_ijk_next_XXX(lmax, &i, &j, &k)) ++count;
+    ++count;
+    long* pr = (long*)malloc(sizeof(long) * (size_t)nranks);
+    long* pt = (long*)malloc(sizeof(long) * (size_t)nthreads);
+    divide_tasks(count, nranks, pr);
+    divide_tasks(pr[rank], nthreads, pt);
+    i = i0; j = j0; k = k0;
+    for (int r = 0; r < rank; ++r)
+        for (long s = 0; s < pr[r]; ++s) get_ijk_next_XXX(lmax, &i, &j, &k);
+    for (int t = 0; t < nthreads; ++t) {
+        if (t)
+            for (long s = 0; s < pt[t - 1]; ++s) get_ijk_next_XXX(lmax, &i, &j, &k);
+        blk[t].i_bgn = i; blk[t].j_bgn = j; blk[t].k_bgn = k; blk[t].loops = (int)pt[t];
+    }
+    free(pr);
+    free(pt);
+}
+
+/* tetrapyd_tools.c:485-517 */
+int get_ijk_next_prim(int max, int* pi, int* pj, int* pk)
+{
+    int i = *pi, j = *pj, k = *pk;
+    if (j == max) { ++i; j = i; k = j; }
+    else if (k == max || k >= i + j + 2) { ++j; k = j; }
+    else ++k;
+    int more = 1;
+    if (i >= max) { more = 0; i = j = k = max; }
+    *pi = i; *pj = j; *pk = k;
+    return more;
+}
+
+/* tetrapyd_tools.c:519-554 */
+void check_ijk_min_prim(int min, int* pi, int* pj, int* pk)
+{
+    int i = *pi < min ? min : *pi, j = *pj < min ? min : *pj, k = *pk < min ? min : *pk;
+    order3(&i, &j, &k);
+    if (k > i + j + 2) k = i + j + 2;
+    *pi = i; *pj = j; *pk = k;
+}
+
+/* tetrapyd_tools.c:556-592 */
+void check_ijk_max_prim(int max, int* pi, int* pj, int* pk)
+{
+    int i = *pi > max ? max : *pi, j = *pj > max ? max : *pj, k = *pk > max ? max : *pk;
+    order3(&i, &j, &k);
+    if (k > i + j + 2) { k = i + j + 2; if (k > max) k = max; }
+    *pi = i; *pj = j; *pk = k;
+}
+
+/* tetrapyd_tools.c:416-483 */
+void decompose_tetrapyd_prim(tetrapyd_limits* blk, int rank, int nranks, int nthreads, int min, int max)
+{
+    int i0 = min, j0 = min, k0 = min;
+    check_ijk_min_prim(min, &i0, &j0, &k0);
+    int i = i0, j = j0, k = k0;
+    long cap = (long)(max + 1) * (max + 1) * (max + 1), count = 1;
+    while (count < cap && get_ijk_next_prim(max, &i, &j, &k)) ++count;
+    ++count;
+    long* pr = (long*)malloc(sizeof(long) * (size_t)nranks);
+    long* pt = (long*)malloc(sizeof(long) * (size_t)nthreads);
+    divide_tasks(count, nranks, pr);
+    divide_tasks(pr[rank], nthreads, pt);
+    i = i0; j = j0; k = k0;
+    for (int r = 0; r < rank; ++r)
+        for (long s = 0; s < pr[r]; ++s) get_ijk_next_prim(max, &i, &j, &k);
+    for (int t = 0; t < nthreads; ++t) {
+        if (t)
+            for (long s = 0; s < pt[t - 1]; ++s) get_ijk_next_prim(max, &i, &j, &k);
+        blk[t].i_bgn = i; blk[t].j_bgn = j; blk[t].k_bgn = k; blk[t].loops = (int)pt[t];
+    }
+    free(pr);
+    free(pt);
+}
+
+/* calculate_gamma_3d.c:213-272: advance up to maxblocksize points through
+ * T(2, lsize-1) from (i0,j0,k0); (i1,j1,k1) is the exclusive end, capped at
+ * the final bound; *blocksize is the number of points passed. */
+void mcmb_next_ijk_bound(int ls, int i0, int j0, int k0, int* i1, int* j1, int* k1,
+                         int fi, int fj, int fk, int* blocksize, int maxblocksize)
+{
+    int oi = fi, oj = fj, ok = fk, cnt = 0;
+    for (int i = i0; i <= fi; ++i) {
+        int jlo = (i == i0) ? j0 : i;
+        int jhi = (i == fi) ? fj + 1 : ls;
+        for (int j = jlo; j < jhi; ++j) {
+            int klo = (i == i0 && j == j0) ? k0 : j + (i % 2);
+            int kmx = (i + j < ls - 1) ? i + j : ls - 1;
+            int khi = (i == fi && j == fj) ? fk : kmx + 1;
+            for (int k = klo; k < khi; k += 2) {
+                if (cnt == maxblocksize) { oi = i; oj = j; ok = k; goto out; }
+                ++cnt;
+            }
+        }
+    }
+out:
+    if (oi >= fi && oj >= fj && ok > fk) { oi = fi; oj = fj; ok = fk; }
+    *i1 = oi; *j1 = oj; *k1 = ok;
+    *blocksize = cnt;
+}
+
+void get_next_ijk_bound(int i0, int j0, int k0, int* i1, int* j1, int* k1,
+                        int fi, int fj, int fk, int* blocksize, int maxblocksize)
+{
+    if (refresh() != MCMB_OK) { *i1 = fi; *j1 = fj; *k1 = fk; *blocksize = 0; g_status = MCMB_ERR_STATE; return; }
+    mcmb_next_ijk_bound(g_lsize, i0, j0, k0, i1, j1, k1, fi, fj, fk, blocksize, maxblocksize);
+}
+
+/* ---- weights and basis -------------------------------------------------- */
+
+/* fixed_data.c:4244-4254 */
+double permsix(int i, int j, int k)
+{
+    if (i == j && j == k) return 1.0;
+    return (i == j || j == k || i == k) ? 3.0 : 6.0;
+}
+
+/* calculate_geometric.c:59-92 */
+double calculate_geometric(int l1, int l2, int l3)
+{
+    if (((l1 + l2 + l3) & 1) || l1 + l2 < l3 || l2 + l3 < l1 || l3 + l1 < l2) return 0.0;
+    double s1 = l1 + l2 + l3 + 1.0, s3 = l1 + l2 + l3 + 1.0 / 3.0, s6 = l1 + l2 + l3 + 1.0 / 6.0;
+    double t13 = l2 + l3 - l1 + 1.0 / 3.0, t23 = l3 + l1 - l2 + 1.0 / 3.0, t33 = l1 + l2 - l3 + 1.0 / 3.0;
+    double t16 = l2 + l3 - l1 + 1.0 / 6.0, t26 = l3 + l1 - l2 + 1.0 / 6.0, t36 = l1 + l2 - l3 + 1.0 / 6.0;
+    double c = M_1_PI * M_1_PI / 2.0;
+    double p1 = c * ((2.0 * l1 + 1.0) * (2.0 * l2 + 1.0) * (2.0 * l3 + 1.0)) / s1;
+    double p2 = s3 / (t13 * t23 * t33);
+    double p3 = sqrt(t16 * t26 * t36 / s6);
+    return p1 * p2 * p3;
+}
+
+/* basis_functions.c:88-107 */
+int basis_functions_legendre(double** a, int size, int pmax, double min, double max, double* values)
+{
+    for (int i = 0; i < size; ++i) {
+        double x = (values[i] - min) / (max - min);
+        a[i][0] = 1.0;
+        if (pmax > 0) a[i][1] = 2.0 * x - 1.0;
+        for (int j = 2; j <= pmax; ++j)
+            a[i][j] = ((2.0 * j - 1.0) * a[i][1] * a[i][j - 1] - (j - 1.0) * a[i][j - 2]) / (1.0 * j);
+    }
+    return 0;
+}
+
+/* ---- Gram entry points ---------------------------------------------------- */
+
+/* gamma_mpi.c:30-152 */
+void gamma_3d_host(double* gamma_flat, int gamma_size, int rank, int numranks, double workshare)
+{
+    (void)rank; (void)numranks; (void)workshare;
+    memset(gamma_flat, 0, sizeof(double) * (size_t)gamma_size * gamma_size);
+    if ((g_status = refresh()) != MCMB_OK) return;
+    if (gamma_size != g_n) { g_status = MCMB_ERR_ARG; return; }
+    g_status = mcmb_gamma_tet(g_ctx, MCMB_FP64, gamma_flat, NULL);
+}
+
+static int cached(double** slot, int gl)
+{
+    int st0 = refresh();
+    if (st0 != MCMB_OK) return st0;
+    if (*slot) return MCMB_OK;
+    double* g = (double*)malloc(sizeof(double) * (size_t)g_n * g_n);
+    if (!g) return MCMB_ERR_NOMEM;
+    int st = gl ? mcmb_gamma_gl(g_ctx, 0, g, NULL) : mcmb_gamma_tet(g_ctx, MCMB_FP64, g, NULL);
+    if (st != MCMB_OK) { free(g); return st; }
+    *slot = g;
+    return MCMB_OK;
+}
+
+/* orthogonalisation_late.c:338 (TTT only) */
+int calculate_orthol_3D(int m, int n, double* result)
+{
+    g_status = cached(&g_gamma3d, 0);
+    if (g_status != MCMB_OK || m < 0 || n < 0 || m >= g_n || n >= g_n) { result[0] = 0.0; return g_status ? g_status : MCMB_ERR_ARG; }
+    result[0] = g_gamma3d[(size_t)m * g_n + n];
+    return 0;
+}
+
+/* orthogonalisation_late.c:239 (TTT only) */
+int calculate_orthol(int m, int n, double* result)
+{
+    g_status = cached(&g_gammagl, 1);
+    if (g_status != MCMB_OK || m < 0 || n < 0 || m >= g_n || n >= g_n) { result[0] = 0.0; return g_status ? g_status : MCMB_ERR_ARG; }
+    result[0] = g_gammagl[(size_t)m * g_n + n];
+    return 0;
+}

@@ -1,0 +1,84 @@
+// Internal declarations shared by the HIP translation units of libmodalcmb.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "modal_cmb.h"
+
+namespace mcmb {
+
+// Record a formatted error message for mcmb_last_error() and return status.
+int fail(int status, const char* fmt, ...);
+
+#define MCMB_HIP(call)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return ::mcmb::fail(MCMB_ERR_HIP, "%s failed: %s (%s:%d)", #call,       \
+                                hipGetErrorString(e_), __FILE__, __LINE__);         \
+    } while (0)
+
+#define MCMB_TRY(call)                                                              \
+    do {                                                                            \
+        int s_ = (call);                                                            \
+        if (s_ != MCMB_OK) return s_;                                               \
+    } while (0)
+
+// One row of the ordered tetrapyd: fixed (i, j), k = k0 + kstep * t, t < nk.
+struct RowDesc {
+    int i, j, k0, nk;
+};
+
+// Device buffer that frees itself.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    int alloc(size_t b);
+    void release();
+    ~DevBuf() { release(); }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Weight of a tetrapyd point.
+enum WeightKind : int { W_LSPACE_EXACT = 0, W_LSPACE_ASYM = 1, W_KSPACE_GL = 2 };
+
+struct LSpace {
+    bool ready = false;
+    long generation = 0;
+    int lmin = 0, lmax = 0, ldq = 0, n = 0, h2mode = 0, R = 0;
+    long npoints = 0;
+    std::vector<RowDesc> rows;        // host copy of the row table (walk order)
+    std::vector<long> row_chunk_prefix;  // prefix of ceil(nk/KC) over rows
+    std::vector<int> order;           // [n][3]
+    std::vector<double> q;            // [(lmax+1)][ldq]
+    std::vector<double> v;            // [lmax+1]
+    DevBuf d_rows, d_q, d_qf, d_v, d_ftab, d_modes;
+    DevBuf d_gamma_cache;             // cached FP64 tetrapyd Gram (legacy shims)
+    bool gamma_cached = false;
+    DevBuf d_gl_cache;
+    bool gl_cached = false;
+};
+
+}  // namespace mcmb
+
+struct mcmb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int last_launches = 0;
+    mcmb::LSpace ls;
+    // scratch for partial Gram tiles
+    mcmb::DevBuf d_partial;
+    mcmb::DevBuf d_gamma_tmp;
+    mcmb::DevBuf d_aux;
+};
+
+namespace mcmb {
+// Build the ordered-tetrapyd row table of T(lmin, lmax) in walk order.
+void build_lspace_rows(int lmin, int lmax, std::vector<RowDesc>& rows, long& npoints);
+int gram_tet_launch(mcmb_ctx* ctx, int precision, double* d_gamma);
+int gram_gl_launch(mcmb_ctx* ctx, int npt, double* d_gamma);
+int set_device(mcmb_ctx* ctx);
+}  // namespace mcmb

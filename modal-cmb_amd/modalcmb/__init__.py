@@ -1,0 +1,211 @@
+"""Python binding of libmodalcmb (the MI355X Gamma / shape-projection library).
+
+A thin ctypes layer over the C ABI declared in ``include/modal_cmb.h``.  All
+compute runs in the HIP library; this module only moves host arrays in and out
+and turns status codes into exceptions.  Importing it does not touch the GPU;
+creating a :class:`Context` does, and raises if the library or a device is
+missing (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmodalcmb.so")
+
+FP64 = 0
+MIXED = 1
+H2_EXACT = 0
+H2_ASYMPTOTIC = 1
+
+_STATUS = {
+    0: "ok", -1: "bad argument", -2: "HIP runtime error", -3: "matrix not positive definite",
+    -4: "bad mode order table", -5: "I/O error", -6: "out of memory", -7: "setup call missing",
+}
+
+
+class ModalError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}: {_STATUS.get(status, '?')}] {message}")
+        self.status = status
+
+
+class NotSPDError(ModalError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libmodalcmb.so (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise OSError(f"libmodalcmb.so not built at {path}: run __graft_entry__.build()")
+        lib = C.CDLL(path)
+        vp, ip, dp = C.c_void_p, C.c_int, C.c_double
+        lib.mcmb_last_error.restype = C.c_char_p
+        lib.mcmb_status_string.restype = C.c_char_p
+        lib.mcmb_create.argtypes = [C.POINTER(vp), ip]
+        lib.mcmb_destroy.argtypes = [vp]
+        lib.mcmb_sync.argtypes = [vp]
+        lib.mcmb_lspace_set.argtypes = [vp, ip, ip, ip, vp, vp, ip, vp, ip]
+        lib.mcmb_gamma_tet.argtypes = [vp, ip, vp, vp]
+        lib.mcmb_gamma_gl.argtypes = [vp, ip, vp, vp]
+        lib.mcmb_lspace_npoints.argtypes = [vp]
+        lib.mcmb_lspace_npoints.restype = C.c_long
+        lib.mcmb_lspace_points.argtypes = [vp, vp, C.c_long]
+        lib.mcmb_lspace_points.restype = C.c_long
+        lib.mcmb_lspace_weights.argtypes = [vp, vp, C.c_long]
+        lib.mcmb_lspace_weights.restype = C.c_long
+        lib.mcmb_last_kernel_ms.argtypes = [vp, C.POINTER(dp), C.POINTER(ip)]
+        lib.mcmb_orthonormalise.argtypes = [vp, ip, vp, vp, vp, vp, C.POINTER(ip)]
+        lib.mcmb_cholesky_solve.argtypes = [vp, ip, vp, ip, vp, vp]
+        for name in ("get_ijk_next_XXX", "get_ijk_next_prim"):
+            getattr(lib, name).argtypes = [ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip)]
+        lib.mcmb_gauss_legendre.argtypes = [ip, vp, vp]
+        lib.permsix.restype = dp
+        lib.permsix.argtypes = [ip, ip, ip]
+        lib.calculate_geometric.restype = dp
+        lib.calculate_geometric.argtypes = [ip, ip, ip]
+        _lib = lib
+    return _lib
+
+
+EXPORTED_SYMBOLS = (
+    "mcmb_last_error", "mcmb_status_string", "mcmb_create", "mcmb_destroy", "mcmb_sync",
+    "mcmb_lspace_set", "mcmb_gamma_tet", "mcmb_gamma_gl", "mcmb_lspace_npoints",
+    "mcmb_lspace_points", "mcmb_lspace_weights", "mcmb_orthonormalise", "mcmb_cholesky_solve",
+    "mcmb_last_kernel_ms", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
+    "mcmb_gauss_legendre",
+    "get_ijk_next_XXX", "check_ijk_min_XXX", "check_ijk_max_XXX", "decompose_tetrapyd_XXX",
+    "get_ijk_next_prim", "check_ijk_min_prim", "check_ijk_max_prim", "decompose_tetrapyd_prim",
+    "divide_tasks", "get_next_ijk_bound", "permsix", "calculate_geometric",
+    "basis_functions_legendre", "gamma_3d_host", "calculate_orthol_3D", "calculate_orthol",
+)
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = _lib.mcmb_last_error().decode(errors="replace")
+        if status == -3:
+            raise NotSPDError(status, msg)
+        raise ModalError(status, msg)
+
+
+class Context:
+    """One GPU, one stream; owns the device tables of the l-space setup."""
+
+    def __init__(self, device: int = 0):
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.mcmb_create(C.byref(h), device))
+        self._h = h
+        self.n = 0
+
+    def close(self) -> None:
+        if self._h:
+            _lib.mcmb_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- l-space Gram -------------------------------------------------------
+    def lspace_set(self, lmin: int, lmax: int, q: np.ndarray, v: np.ndarray, order: np.ndarray,
+                   h2_mode: int = H2_EXACT) -> None:
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        order = np.ascontiguousarray(order, dtype=np.int32).reshape(-1, 3)
+        if q.ndim != 2 or q.shape[0] != lmax + 1 or v.shape[0] != lmax + 1:
+            raise ValueError("q must be [(lmax+1), ldq] and v [(lmax+1)]")
+        self._keep = (q, v, order)
+        _check(_lib.mcmb_lspace_set(self._h, lmin, lmax, q.shape[1], _ptr(q), _ptr(v),
+                                    order.shape[0], _ptr(order), h2_mode))
+        self.n = order.shape[0]
+
+    def gamma_tet(self, precision: int = FP64) -> np.ndarray:
+        g = np.empty((self.n, self.n), dtype=np.float64)
+        _check(_lib.mcmb_gamma_tet(self._h, precision, _ptr(g), None))
+        return g
+
+    def gamma_gl(self, npt: int = 0) -> np.ndarray:
+        g = np.empty((self.n, self.n), dtype=np.float64)
+        _check(_lib.mcmb_gamma_gl(self._h, npt, _ptr(g), None))
+        return g
+
+    def gamma_tet_device(self, precision: int, dev_ptr: int) -> None:
+        _check(_lib.mcmb_gamma_tet(self._h, precision, None, C.c_void_p(dev_ptr)))
+
+    def npoints(self) -> int:
+        r = _lib.mcmb_lspace_npoints(self._h)
+        if r < 0:
+            _check(int(r))
+        return int(r)
+
+    def points(self, cap: int | None = None) -> np.ndarray:
+        n = self.npoints()
+        cap = n if cap is None else min(cap, n)
+        out = np.empty((cap, 3), dtype=np.int32)
+        r = _lib.mcmb_lspace_points(self._h, _ptr(out), cap)
+        if r < 0:
+            _check(int(r))
+        return out
+
+    def weights(self, cap: int | None = None) -> np.ndarray:
+        n = self.npoints()
+        cap = n if cap is None else min(cap, n)
+        out = np.empty(cap, dtype=np.float64)
+        r = _lib.mcmb_lspace_weights(self._h, _ptr(out), cap)
+        if r < 0:
+            _check(int(r))
+        return out
+
+    def last_kernel_ms(self) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_int()
+        _check(_lib.mcmb_last_kernel_ms(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def sync(self) -> None:
+        _check(_lib.mcmb_sync(self._h))
+
+    # ---- orthonormalisation --------------------------------------------------
+    def orthonormalise(self, gamma: np.ndarray):
+        """Return (ortho, lambda, norm) as ortho_prim_mpi.c:198-284 leaves them."""
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        n = g.shape[0]
+        ortho = np.empty_like(g)
+        lam = np.empty_like(g)
+        norm = np.empty(n)
+        bad = C.c_int(-1)
+        st = _lib.mcmb_orthonormalise(self._h, n, _ptr(g), _ptr(ortho), _ptr(lam), _ptr(norm),
+                                      C.byref(bad))
+        if st == -3:
+            raise NotSPDError(st, f"{_lib.mcmb_last_error().decode()} (pivot {bad.value})")
+        _check(st)
+        return ortho, lam, norm
+
+    def cholesky_solve(self, ortho: np.ndarray, alpha: np.ndarray) -> np.ndarray:
+        f = np.ascontiguousarray(ortho, dtype=np.float64)
+        n = f.shape[0]
+        a = np.asfortranarray(np.asarray(alpha, dtype=np.float64).reshape(n, -1))
+        out = np.empty_like(a, order="F")
+        _check(_lib.mcmb_cholesky_solve(self._h, n, _ptr(f), a.shape[1], _ptr(a), _ptr(out)))
+        return out.reshape(np.shape(alpha))

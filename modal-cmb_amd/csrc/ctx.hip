@@ -240,7 +240,37 @@ int mcmb_gamma_tet(mcmb_ctx* c, int precision, double* gamma_host, double* gamma
         MCMB_TRY(c->d_gamma_tmp.alloc(sizeof(double) * (size_t)n * n));
         dst = c->d_gamma_tmp.as<double>();
     }
-    MCMB_TRY(gram_tet_launch(c, precision, dst));
+    MCMB_TRY(gram_tet_launch(c, precision, dst, 0, (int)c->ls.rows.size()));
+    if (gamma_host)
+        MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return MCMB_OK;
+}
+
+int mcmb_gamma_tet_slice(mcmb_ctx* c, int precision, int rank, int nranks, double* gamma_host,
+                         double* gamma_dev, double* work_fraction)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_gamma_tet_slice: null context");
+    if (!c->ls.ready) return fail(MCMB_ERR_STATE, "mcmb_gamma_tet_slice: call mcmb_lspace_set first");
+    if (precision != MCMB_FP64 && precision != MCMB_MIXED)
+        return fail(MCMB_ERR_ARG, "mcmb_gamma_tet_slice: precision %d", precision);
+    if (nranks < 1 || rank < 0 || rank >= nranks)
+        return fail(MCMB_ERR_ARG, "mcmb_gamma_tet_slice: rank %d of %d", rank, nranks);
+    MCMB_TRY(set_device(c));
+    int n = c->ls.n, lo, hi;
+    lspace_slice(c->ls, 32, rank, nranks, &lo, &hi, work_fraction);
+    double* dst = gamma_dev;
+    if (!dst) {
+        MCMB_TRY(c->d_gamma_tmp.alloc(sizeof(double) * (size_t)n * n));
+        dst = c->d_gamma_tmp.as<double>();
+    }
+    if (lo >= hi) {
+        MCMB_HIP(hipMemsetAsync(dst, 0, sizeof(double) * (size_t)n * n, c->stream));
+        c->last_ms = 0.0;
+        c->last_launches = 0;
+    } else {
+        MCMB_TRY(gram_tet_launch(c, precision, dst, lo, hi));
+    }
     if (gamma_host)
         MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));

@@ -251,7 +251,7 @@ __global__ void to_float_table(const double* __restrict__ q, int ldq, int R, int
     qf[e] = c < R ? (float)q[(long)l * ldq + c] : 0.f;
 }
 
-int gram_mixed_launch(mcmb_ctx* c, const long* d_rowptr, const double* d_ws, double* d_gamma)
+int gram_mixed_launch(mcmb_ctx* c, const long* d_rowptr, const double* d_ws, double* d_gamma, int row_lo, int row_hi)
 {
     LSpace& L = c->ls;
     const int n = L.n, R = L.R;
@@ -265,18 +265,25 @@ int gram_mixed_launch(mcmb_ctx* c, const long* d_rowptr, const double* d_ws, dou
     // Longer segments cut the FP64 flush traffic (n_pad^2/2 doubles per
     // segment) at the price of longer FP32 accumulation.
     long nchunks_all = 0;
-    for (size_t r = 0; r < L.rows.size(); ++r) nchunks_all += (L.rows[r].nk + MKC - 1) / MKC;
+    for (int r = row_lo; r < row_hi; ++r) nchunks_all += (L.rows[r].nk + MKC - 1) / MKC;
     // keep at least ~4 rounds of 256 workgroups on small problems
     long cap = std::max<long>(16, nchunks_all * ntiles / 1024);
     int segch = (int)std::min<long>(SEGCH_DEFAULT, cap);
     if (const char* e = getenv("MCMB_MIXED_SEGCH")) segch = std::max(1, atoi(e));
     std::vector<int2> seg;
     long nchunks = 0;
-    for (size_t r = 0; r < L.rows.size(); ++r)
+    for (int r = row_lo; r < row_hi; ++r)
         for (int c0 = 0; c0 < L.rows[r].nk; c0 += MKC) {
-            if (nchunks % segch == 0) seg.push_back(make_int2((int)r, c0));
+            if (nchunks % segch == 0) seg.push_back(make_int2(r, c0));
             ++nchunks;
         }
+    if (nchunks == 0) {   // empty slice: a zero Gram
+        MCMB_HIP(hipMemsetAsync(d_gamma, 0, sizeof(double) * (size_t)n * n, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        c->last_ms = 0.0;
+        c->last_launches = 0;
+        return MCMB_OK;
+    }
     const long nseg = (long)seg.size();
 
     DevBuf d_dt, d_ot, d_seg, d_corr;

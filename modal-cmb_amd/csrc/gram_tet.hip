@@ -238,19 +238,33 @@ __global__ __launch_bounds__(256) void gram_reduce_kernel(const double* __restri
     out[(size_t)c * n + m] = s;
 }
 
-// Split rows into nseg segments of about equal chunk count.
-static void make_segments(const std::vector<RowDesc>& rows, int kc, int nseg, std::vector<int>& seg)
+// Split rows [lo, hi) into nseg segments of about equal chunk count.
+static void make_segments(const std::vector<RowDesc>& rows, int lo, int hi, int kc, int nseg, std::vector<int>& seg)
 {
+    std::vector<long> pre(hi - lo + 1, 0);
+    for (int r = lo; r < hi; ++r) pre[r - lo + 1] = pre[r - lo] + (rows[r].nk + kc - 1) / kc;
+    long tot = pre.back();
+    seg.assign(nseg + 1, lo);
+    for (int s = 1; s < nseg; ++s) {
+        long target = tot * s / nseg;
+        seg[s] = lo + (int)(std::lower_bound(pre.begin(), pre.end(), target) - pre.begin());
+        if (seg[s] < seg[s - 1]) seg[s] = seg[s - 1];
+    }
+    seg[nseg] = hi;
+}
+
+void lspace_slice(const LSpace& L, int kc, int rank, int nranks, int* row_lo, int* row_hi, double* frac)
+{
+    const std::vector<RowDesc>& rows = L.rows;
     std::vector<long> pre(rows.size() + 1, 0);
     for (size_t r = 0; r < rows.size(); ++r) pre[r + 1] = pre[r] + (rows[r].nk + kc - 1) / kc;
     long tot = pre.back();
-    seg.assign(nseg + 1, 0);
-    for (int s = 1; s < nseg; ++s) {
-        long target = tot * s / nseg;
-        seg[s] = (int)(std::lower_bound(pre.begin(), pre.end(), target) - pre.begin());
-        if (seg[s] < seg[s - 1]) seg[s] = seg[s - 1];
-    }
-    seg[nseg] = (int)rows.size();
+    auto at = [&](long target) {
+        return (int)(std::lower_bound(pre.begin(), pre.end(), target) - pre.begin());
+    };
+    *row_lo = rank == 0 ? 0 : at(tot * rank / nranks);
+    *row_hi = rank == nranks - 1 ? (int)rows.size() : at(tot * (rank + 1) / nranks);
+    if (frac) *frac = tot ? (double)(pre[*row_hi] - pre[*row_lo]) / (double)tot : 0.0;
 }
 
 static int ensure_lspace_aux(mcmb_ctx* c, DevBuf& rowptr, DevBuf& ws, int scaled)
@@ -270,14 +284,15 @@ static int ensure_lspace_aux(mcmb_ctx* c, DevBuf& rowptr, DevBuf& ws, int scaled
     return MCMB_OK;
 }
 
-int gram_mixed_launch(mcmb_ctx* c, const long* d_rowptr, const double* d_ws, double* d_gamma);
+int gram_mixed_launch(mcmb_ctx* c, const long* d_rowptr, const double* d_ws, double* d_gamma, int row_lo, int row_hi);
 
-int gram_tet_launch(mcmb_ctx* c, int precision, double* d_gamma)
+int gram_tet_launch(mcmb_ctx* c, int precision, double* d_gamma, int row_lo, int row_hi)
 {
     LSpace& L = c->ls;
     DevBuf rowptr, ws;
     MCMB_TRY(ensure_lspace_aux(c, rowptr, ws, 1));
-    if (precision == MCMB_MIXED) return gram_mixed_launch(c, rowptr.as<long>(), ws.as<double>(), d_gamma);
+    if (precision == MCMB_MIXED)
+        return gram_mixed_launch(c, rowptr.as<long>(), ws.as<double>(), d_gamma, row_lo, row_hi);
 
     const int n = L.n;
     const int T = (n + TM - 1) / TM, n_pad = T * TM;
@@ -289,11 +304,11 @@ int gram_tet_launch(mcmb_ctx* c, int precision, double* d_gamma)
     // 4 GiB of partial tiles
     long target = 2048;
     int nseg = (int)std::max<long>(1, (target + ntiles - 1) / ntiles);
-    nseg = (int)std::min<long>(nseg, (long)L.rows.size());
+    nseg = (int)std::max<long>(1, std::min<long>(nseg, (long)(row_hi - row_lo)));
     size_t per = (size_t)n_pad * n_pad * sizeof(double);
     nseg = (int)std::max<size_t>(1, std::min<size_t>((size_t)nseg, ((size_t)4 << 30) / per));
     std::vector<int> seg;
-    make_segments(L.rows, KC, nseg, seg);
+    make_segments(L.rows, row_lo, row_hi, KC, nseg, seg);
 
     DevBuf d_tiles, d_seg;
     MCMB_TRY(d_tiles.alloc(sizeof(int2) * tiles.size()));

@@ -260,11 +260,12 @@ int basis_functions_legendre(double** a, int size, int pmax, double min, double 
 /* gamma_mpi.c:30-152 */
 void gamma_3d_host(double* gamma_flat, int gamma_size, int rank, int numranks, double workshare)
 {
-    (void)rank; (void)numranks; (void)workshare;
+    (void)workshare;
     memset(gamma_flat, 0, sizeof(double) * (size_t)gamma_size * gamma_size);
     if ((g_status = refresh()) != MCMB_OK) return;
     if (gamma_size != g_n) { g_status = MCMB_ERR_ARG; return; }
-    g_status = mcmb_gamma_tet(g_ctx, MCMB_FP64, gamma_flat, NULL);
+    if (numranks <= 1) g_status = mcmb_gamma_tet(g_ctx, MCMB_FP64, gamma_flat, NULL);
+    else g_status = mcmb_gamma_tet_slice(g_ctx, MCMB_FP64, rank, numranks, gamma_flat, NULL, NULL);
 }
 
 static int cached(double** slot, int gl)

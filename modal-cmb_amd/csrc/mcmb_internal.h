@@ -78,7 +78,9 @@ struct mcmb_ctx {
 namespace mcmb {
 // Build the ordered-tetrapyd row table of T(lmin, lmax) in walk order.
 void build_lspace_rows(int lmin, int lmax, std::vector<RowDesc>& rows, long& npoints);
-int gram_tet_launch(mcmb_ctx* ctx, int precision, double* d_gamma);
+int gram_tet_launch(mcmb_ctx* ctx, int precision, double* d_gamma, int row_lo, int row_hi);
+// contiguous share [row_lo, row_hi) of rank r of nr, balanced by chunk count
+void lspace_slice(const LSpace& L, int kc, int rank, int nranks, int* row_lo, int* row_hi, double* frac);
 int gram_gl_launch(mcmb_ctx* ctx, int npt, double* d_gamma);
 int set_device(mcmb_ctx* ctx);
 }  // namespace mcmb

@@ -56,6 +56,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_lspace_set.argtypes = [vp, ip, ip, ip, vp, vp, ip, vp, ip]
         lib.mcmb_gamma_tet.argtypes = [vp, ip, vp, vp]
         lib.mcmb_gamma_gl.argtypes = [vp, ip, vp, vp]
+        lib.mcmb_gamma_tet_slice.argtypes = [vp, ip, ip, ip, vp, vp, C.POINTER(dp)]
         lib.mcmb_lspace_npoints.argtypes = [vp]
         lib.mcmb_lspace_npoints.restype = C.c_long
         lib.mcmb_lspace_points.argtypes = [vp, vp, C.c_long]
@@ -81,7 +82,7 @@ EXPORTED_SYMBOLS = (
     "mcmb_lspace_set", "mcmb_gamma_tet", "mcmb_gamma_gl", "mcmb_lspace_npoints",
     "mcmb_lspace_points", "mcmb_lspace_weights", "mcmb_orthonormalise", "mcmb_cholesky_solve",
     "mcmb_last_kernel_ms", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
-    "mcmb_gauss_legendre",
+    "mcmb_gauss_legendre", "mcmb_gamma_tet_slice",
     "get_ijk_next_XXX", "check_ijk_min_XXX", "check_ijk_max_XXX", "decompose_tetrapyd_XXX",
     "get_ijk_next_prim", "check_ijk_min_prim", "check_ijk_max_prim", "decompose_tetrapyd_prim",
     "divide_tasks", "get_next_ijk_bound", "permsix", "calculate_geometric",
@@ -145,6 +146,14 @@ class Context:
         g = np.empty((self.n, self.n), dtype=np.float64)
         _check(_lib.mcmb_gamma_tet(self._h, precision, _ptr(g), None))
         return g
+
+    def gamma_tet_slice(self, precision: int, rank: int, nranks: int):
+        """Partial Gram of one contiguous share of the tetrapyd; returns
+        (gamma_part, work_fraction)."""
+        g = np.empty((self.n, self.n), dtype=np.float64)
+        frac = C.c_double()
+        _check(_lib.mcmb_gamma_tet_slice(self._h, precision, rank, nranks, _ptr(g), None, C.byref(frac)))
+        return g, frac.value
 
     def gamma_gl(self, npt: int = 0) -> np.ndarray:
         g = np.empty((self.n, self.n), dtype=np.float64)

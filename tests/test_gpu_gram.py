@@ -101,3 +101,32 @@ def test_legacy_gamma_3d_host_and_orthol(ctx):
     res = (C.c_double * 4)()
     assert lib.calculate_orthol_3D(3, 7, res) == 0 and res[0] == G[3, 7]
     lib.mcmb_legacy_bind(None)
+
+
+@pytest.mark.parametrize("prec", [modalcmb.FP64, modalcmb.MIXED])
+def test_slices_sum_to_whole(ctx, prec):
+    ws = W.lspace(150, 5, True)
+    ctx.lspace_set(2, 150, ws["q"], ws["v"], ws["order"])
+    G = ctx.gamma_tet(prec)
+    parts = [ctx.gamma_tet_slice(prec, r, 5) for r in range(5)]
+    assert sum(f for _, f in parts) == pytest.approx(1.0, abs=1e-12)
+    S = sum(p for p, _ in parts)
+    assert rel_elem(S, G) < (1e-13 if prec == modalcmb.FP64 else 2e-5)
+
+
+def test_legacy_gamma_3d_host_ranks(ctx):
+    import ctypes as C
+    lib = modalcmb.load_library()
+    ws = W.lspace(90, 4)
+    ctx.lspace_set(2, 90, ws["q"], ws["v"], ws["order"])
+    G = ctx.gamma_tet(modalcmb.FP64)
+    lib.mcmb_legacy_bind(ctx._h)
+    n = G.shape[0]
+    tot = np.zeros((n, n))
+    for r in range(3):
+        buf = np.empty((n, n))
+        lib.gamma_3d_host(buf.ctypes.data_as(C.c_void_p), n, r, 3, C.c_double(0.0))
+        assert lib.mcmb_legacy_status() == 0
+        tot += buf
+    assert rel_elem(tot, G) < 1e-13
+    lib.mcmb_legacy_bind(None)

@@ -38,6 +38,8 @@ extern "C" {
                            FP32 chunk accumulation, FP64 cross-chunk sums, FP64
                            diagonal lo*lo correction                              */
 
+#define MCMB_FP32   2   /* projection only: FP32 MFMA k-sum, FP64 mode sums    */
+
 /* geometric factor h^2 */
 #define MCMB_H2_EXACT      0  /* (2l1+1)(2l2+1)(2l3+1)/(4pi) (l1 l2 l3;0 0 0)^2   */
 #define MCMB_H2_ASYMPTOTIC 1  /* calculate_geometric.c:59-92                     */
@@ -114,6 +116,36 @@ int mcmb_orthonormalise(mcmb_ctx* ctx, int n, const double* gamma, double* ortho
  * (gsl_linalg_cholesky_solve, eigenmode_mpi.c:211). */
 int mcmb_cholesky_solve(mcmb_ctx* ctx, int n, const double* ortho, int nrhs,
                         const double* alpha, double* beta);
+
+/* ---- primordial k-space: Gram and shape projection ------------------------ */
+/* Gauss-Legendre k-tetrapyd: N ascending nodes kn[] with weights kw[], basis
+ * q[a][0..ldq) at node a, mode order[n][3].  Points are the node-index triples
+ * a <= b <= c with kn[c] <= kn[a] + kn[b] (FP64 comparison), multiplicity
+ * permsix(a,b,c), weight w_a w_b w_c / (k_a + k_b + k_c): the inner product of
+ * calculate_ortho / calculate_eigen (orthogonalisation.c:14-86,
+ * calculate_eigen.c:86-145) on the GL grid instead of the uniform grid. */
+int mcmb_kspace_set(mcmb_ctx* ctx, int N, const double* kn, const double* kw, int ldq,
+                    const double* q, int nmodes, const int* order);
+long mcmb_kspace_npoints(mcmb_ctx* ctx);
+/* Gram of the k-space modes (calculate_ortho, orthogonalisation.c:14-86, full
+ * symmetric n x n instead of the packed triangle); precision FP64 or MIXED. */
+int mcmb_kgram(mcmb_ctx* ctx, int precision, double* gamma_host, double* gamma_dev);
+/* Projection of the feature shape (calculate_shape.c:933-946, model 4) for
+ * ntheta parameter pairs theta[b] = (omega, phi), phi in radians (the
+ * reference's a2 percent-of-2pi phase is phi = 2 * 3.1415927 * a2 / 100):
+ *   alpha[b][n]      = <S(theta_b), Q_n>      (host, ntheta x n)
+ *   dalpha[b][0][n]  = d alpha / d omega, dalpha[b][1][n] = d alpha / d phi
+ *                      (tangent mode, same pass; NULL to skip)
+ * precision: MCMB_FP64 (FP64 MFMA) or MCMB_FP32 (FP32 MFMA k-sum, FP64 mode
+ * sums).  Replaces calculate_eigen's alpha (eigenmode_mpi.c:175); beta follows
+ * from mcmb_cholesky_solve with the k-space Gram. */
+int mcmb_project_fwd(mcmb_ctx* ctx, int precision, int ntheta, const double* theta,
+                     double* alpha, double* dalpha);
+/* Adjoint: dtheta[b][0..1] = sum_n gbar[b][n] d alpha_n / d (omega, phi),
+ * computed in reverse mode (gbar scattered on the mode triples and pulled back
+ * through the mode sums to the k-sum), no Jacobian formed. */
+int mcmb_project_bwd(mcmb_ctx* ctx, int precision, int ntheta, const double* theta,
+                     const double* gbar, double* dtheta);
 
 /* ---- quadrature ----------------------------------------------------------- */
 /* n-node Gauss-Legendre rule on [-1,1], nodes ascending (host only, no GPU). */

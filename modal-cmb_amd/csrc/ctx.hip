@@ -66,6 +66,21 @@ void build_lspace_rows(int lmin, int lmax, std::vector<RowDesc>& rows, long& npo
     }
 }
 
+// Copy rows, basis and mode table of a point set to the device.
+int upload_pointset(mcmb_ctx* c, PointSet& L, const std::vector<int>& order)
+{
+    std::vector<int> modes(L.n);
+    for (int m = 0; m < L.n; ++m) modes[m] = order[3 * m] | (order[3 * m + 1] << 8) | (order[3 * m + 2] << 16);
+    MCMB_TRY(L.d_rows.alloc(sizeof(RowDesc) * L.rows.size()));
+    MCMB_TRY(L.d_q.alloc(sizeof(double) * L.q.size()));
+    MCMB_TRY(L.d_modes.alloc(sizeof(int) * modes.size()));
+    MCMB_HIP(hipMemcpyAsync(L.d_rows.p, L.rows.data(), sizeof(RowDesc) * L.rows.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(L.d_q.p, L.q.data(), sizeof(double) * L.q.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(L.d_modes.p, modes.data(), sizeof(int) * modes.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return MCMB_OK;
+}
+
 }  // namespace mcmb
 
 using namespace mcmb;
@@ -116,14 +131,11 @@ int mcmb_destroy(mcmb_ctx* c)
     if (!c) return MCMB_OK;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    c->ls.d_rows.release();
-    c->ls.d_q.release();
-    c->ls.d_qf.release();
-    c->ls.d_v.release();
-    c->ls.d_ftab.release();
-    c->ls.d_modes.release();
-    c->ls.d_gamma_cache.release();
-    c->ls.d_gl_cache.release();
+    for (PointSet* P : {&c->ls, &c->ks}) {
+        P->d_rows.release(); P->d_q.release(); P->d_qf.release(); P->d_v.release(); P->d_ftab.release();
+        P->d_modes.release(); P->d_kn.release(); P->d_kw.release();
+    }
+    c->d_W.release();
     c->d_partial.release();
     c->d_gamma_tmp.release();
     c->d_aux.release();
@@ -163,8 +175,8 @@ int mcmb_lspace_set(mcmb_ctx* c, int lmin, int lmax, int ldq, const double* q, c
     MCMB_TRY(set_device(c));
     LSpace& L = c->ls;
     L.ready = false;
-    L.gamma_cached = false;
-    L.gl_cached = false;
+    L.kspace = false;
+    L.kstep = 2;
     L.lmin = lmin;
     L.lmax = lmax;
     L.ldq = ldq;
@@ -184,20 +196,11 @@ int mcmb_lspace_set(mcmb_ctx* c, int lmin, int lmax, int ldq, const double* q, c
         acc *= (long double)(2 * a - 1) / (long double)(2 * a);
         ftab[a] = (double)acc;
     }
-    std::vector<int> modes(nmodes);
-    for (int m = 0; m < nmodes; ++m)
-        modes[m] = order[3 * m] | (order[3 * m + 1] << 8) | (order[3 * m + 2] << 16);
-    MCMB_TRY(L.d_rows.alloc(sizeof(RowDesc) * L.rows.size()));
-    MCMB_TRY(L.d_q.alloc(sizeof(double) * L.q.size()));
-    MCMB_TRY(L.d_v.alloc(sizeof(double) * L.v.size()));
     MCMB_TRY(L.d_ftab.alloc(sizeof(double) * ftab.size()));
-    MCMB_TRY(L.d_modes.alloc(sizeof(int) * modes.size()));
-    MCMB_HIP(hipMemcpyAsync(L.d_rows.p, L.rows.data(), sizeof(RowDesc) * L.rows.size(), hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipMemcpyAsync(L.d_q.p, L.q.data(), sizeof(double) * L.q.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_TRY(L.d_v.alloc(sizeof(double) * L.v.size()));
     MCMB_HIP(hipMemcpyAsync(L.d_v.p, L.v.data(), sizeof(double) * L.v.size(), hipMemcpyHostToDevice, c->stream));
     MCMB_HIP(hipMemcpyAsync(L.d_ftab.p, ftab.data(), sizeof(double) * ftab.size(), hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipMemcpyAsync(L.d_modes.p, modes.data(), sizeof(int) * modes.size(), hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipStreamSynchronize(c->stream));
+    MCMB_TRY(upload_pointset(c, L, L.order));
     L.ready = true;
     L.generation += 1;
     return MCMB_OK;
@@ -240,7 +243,7 @@ int mcmb_gamma_tet(mcmb_ctx* c, int precision, double* gamma_host, double* gamma
         MCMB_TRY(c->d_gamma_tmp.alloc(sizeof(double) * (size_t)n * n));
         dst = c->d_gamma_tmp.as<double>();
     }
-    MCMB_TRY(gram_tet_launch(c, precision, dst, 0, (int)c->ls.rows.size()));
+    MCMB_TRY(gram_tet_launch(c, c->ls, precision, dst, 0, (int)c->ls.rows.size()));
     if (gamma_host)
         MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
@@ -269,7 +272,7 @@ int mcmb_gamma_tet_slice(mcmb_ctx* c, int precision, int rank, int nranks, doubl
         c->last_ms = 0.0;
         c->last_launches = 0;
     } else {
-        MCMB_TRY(gram_tet_launch(c, precision, dst, lo, hi));
+        MCMB_TRY(gram_tet_launch(c, c->ls, precision, dst, lo, hi));
     }
     if (gamma_host)
         MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));

@@ -1,0 +1,221 @@
+// Gauss-Legendre-factorised Gram (calculate_orthol, orthogonalisation_late.c:
+// 239-336; M_ij of calculate_orthoMij_T, 99-129), FP64 on the GPU.
+//
+// With int_{-1}^{1} P_l1 P_l2 P_l3 dx = 2 (l1 l2 l3; 0 0 0)^2, the cube sum of
+// h^2 f1(l1) f2(l2) f3(l3) is (1/8 pi) int dx prod_a sum_l (2l+1) P_l(x) f_a(l),
+// exact on npt >= (3 lmax + 1)/2 nodes; symmetrising the mode products leaves
+// 1/6 of the 6 pairings, so
+//   M_ij(x_s) = sum_{l=lmin}^{lmax} (2l+1) v(l) P_l(x_s) q_i(l) q_j(l)
+//   Gamma_mn  = sum_s w_s/(48 pi) perm[ M_{m_a n_b}(x_s) ]_{a,b=1..3}
+// (perm = permanent of the 3 x 3 matrix).  Three kernels: Legendre table by
+// recurrence (one thread per node), M by a tiled FP64 GEMM over l, and the
+// node sum per 64 x 64 tile of (m, n) with M staged in LDS 8 nodes at a time.
+// O(R^2 npt L + n^2 npt) instead of O(n^2 N_tet).
+#include "mcmb_internal.h"
+#include <cmath>
+#include <vector>
+
+namespace mcmb {
+
+constexpr int GLR = 32;   // max basis width
+constexpr int SC = 8;     // nodes per LDS stage
+
+// Pt[l][s] = (2l+1) v(l) P_l(x_s) for lmin <= l <= lmax, 0 below lmin.
+__global__ __launch_bounds__(256) void gl_legendre(const double* __restrict__ x, int npt, int lmin, int lmax,
+                                                   const double* __restrict__ v, double* __restrict__ Pt)
+{
+    int s = blockIdx.x * 256 + threadIdx.x;
+    if (s >= npt) return;
+    double xs = x[s], p0 = 1.0, p1 = xs;
+    for (int l = 0; l <= lmax; ++l) {
+        double pl;
+        if (l == 0) pl = 1.0;
+        else if (l == 1) pl = xs;
+        else {
+            pl = ((2.0 * l - 1.0) * xs * p1 - (l - 1.0) * p0) / l;
+            p0 = p1;
+            p1 = pl;
+        }
+        Pt[(size_t)l * npt + s] = (l >= lmin) ? (2.0 * l + 1.0) * v[l] * pl : 0.0;
+    }
+}
+
+// U[l][p] = q_i(l) q_j(l) for the pair p = (i, j), i <= j < R (p enumerates the
+// upper triangle row by row); l <= lmax.
+__global__ void gl_pairs(const double* __restrict__ q, int ldq, int R, int lmax, double* __restrict__ U, int npair)
+{
+    int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= (lmax + 1) * npair) return;
+    int l = e / npair, p = e % npair;
+    int i = 0, rem = p;
+    while (rem >= R - i) { rem -= R - i; ++i; }
+    int j = i + rem;
+    U[e] = q[(size_t)l * ldq + i] * q[(size_t)l * ldq + j];
+}
+
+// Mp[p][s] = sum_l U[l][p] Pt[l][s]  (tile 64 s x 32 p, K = l in steps of 16)
+__global__ __launch_bounds__(256) void gl_mgemm(const double* __restrict__ Pt, const double* __restrict__ U,
+                                                int npt, int npair, int nl, double* __restrict__ Mp)
+{
+    __shared__ double Ps[16][64 + 1], Us[16][32 + 1];
+    const int s0 = blockIdx.x * 64, p0 = blockIdx.y * 32;
+    const int ts = (threadIdx.x & 15) * 4, tp = (threadIdx.x >> 4) * 2;
+    double acc[4][2] = {};
+    for (int l0 = 0; l0 < nl; l0 += 16) {
+        for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+            int l = e / 64, s = e % 64;
+            Ps[l][s] = (l0 + l < nl && s0 + s < npt) ? Pt[(size_t)(l0 + l) * npt + s0 + s] : 0.0;
+        }
+        for (int e = threadIdx.x; e < 16 * 32; e += 256) {
+            int l = e / 32, p = e % 32;
+            Us[l][p] = (l0 + l < nl && p0 + p < npair) ? U[(size_t)(l0 + l) * npair + p0 + p] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int l = 0; l < 16; ++l) {
+            double a[4], b[2];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) a[u] = Ps[l][ts + u];
+            b[0] = Us[l][tp];
+            b[1] = Us[l][tp + 1];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) { acc[u][0] += a[u] * b[0]; acc[u][1] += a[u] * b[1]; }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            int s = s0 + ts + u, p = p0 + tp + v;
+            if (s < npt && p < npair) Mp[(size_t)p * npt + s] = acc[u][v];
+        }
+}
+
+// Gamma over 64 x 64 mode tiles (I <= J): each thread owns a 4 x 4 block of
+// (m, n) and sums w_s/(48 pi) perm(M_{m_a n_b}(x_s)) over all nodes; M for SC
+// nodes at a time is staged in LDS as Ms[s][i][j] (full R x R).
+__global__ __launch_bounds__(256) void gl_gamma(const double* __restrict__ Mp, const double* __restrict__ wgt,
+                                                int npt, int R, const int* __restrict__ modes, int n,
+                                                const int2* __restrict__ tiles, double* __restrict__ out)
+{
+    __shared__ double Ms[SC][GLR * GLR];
+    __shared__ double ws[SC];
+    const int2 IJ = tiles[blockIdx.x];
+    const int tm = (threadIdx.x >> 4) * 4, tn = (threadIdx.x & 15) * 4;
+    int ma[4][3], nb[4][3];
+    bool okm[4], okn[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        int m = IJ.x * 64 + tm + u, c = IJ.y * 64 + tn + u;
+        okm[u] = m < n;
+        okn[u] = c < n;
+        int pm = okm[u] ? modes[m] : 0, pn = okn[u] ? modes[c] : 0;
+        ma[u][0] = pm & 255; ma[u][1] = (pm >> 8) & 255; ma[u][2] = (pm >> 16) & 255;
+        nb[u][0] = pn & 255; nb[u][1] = (pn >> 8) & 255; nb[u][2] = (pn >> 16) & 255;
+    }
+    double acc[4][4] = {};
+    for (int s0 = 0; s0 < npt; s0 += SC) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < SC * R * R; e += 256) {
+            int s = e / (R * R), ij = e % (R * R), i = ij / R, j = ij % R;
+            int a = i < j ? i : j, b = i < j ? j : i;
+            int p = a * R - a * (a - 1) / 2 + (b - a);
+            Ms[s][i * GLR + j] = (s0 + s < npt) ? Mp[(size_t)p * npt + s0 + s] : 0.0;
+        }
+        if (threadIdx.x < SC) ws[threadIdx.x] = (s0 + threadIdx.x < npt) ? wgt[s0 + threadIdx.x] : 0.0;
+        __syncthreads();
+        for (int s = 0; s < SC; ++s) {
+            const double* M = Ms[s];
+            const double w = ws[s];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const double a11 = M[ma[u][0] * GLR + nb[v][0]], a12 = M[ma[u][0] * GLR + nb[v][1]], a13 = M[ma[u][0] * GLR + nb[v][2]];
+                    const double a21 = M[ma[u][1] * GLR + nb[v][0]], a22 = M[ma[u][1] * GLR + nb[v][1]], a23 = M[ma[u][1] * GLR + nb[v][2]];
+                    const double a31 = M[ma[u][2] * GLR + nb[v][0]], a32 = M[ma[u][2] * GLR + nb[v][1]], a33 = M[ma[u][2] * GLR + nb[v][2]];
+                    const double perm = a11 * (a22 * a33 + a23 * a32) + a12 * (a21 * a33 + a23 * a31)
+                                      + a13 * (a21 * a32 + a22 * a31);
+                    acc[u][v] += w * perm;
+                }
+        }
+    }
+    const double scale = 1.0 / (48.0 * M_PI);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            int m = IJ.x * 64 + tm + u, c = IJ.y * 64 + tn + v;
+            if (okm[u] && okn[v] && m <= c) {
+                out[(size_t)m * n + c] = scale * acc[u][v];
+                out[(size_t)c * n + m] = scale * acc[u][v];
+            }
+        }
+}
+
+int gram_gl_launch(mcmb_ctx* c, int npt, double* d_gamma)
+{
+    LSpace& L = c->ls;
+    const int n = L.n, R = L.R, lmax = L.lmax;
+    if (R > GLR) return fail(MCMB_ERR_ARG, "GL Gram supports basis width <= %d (got %d)", GLR, R);
+    if (npt <= 0) npt = 3 * lmax / 2 + 1;
+    std::vector<double> x(npt), w(npt);
+    MCMB_TRY(mcmb_gauss_legendre(npt, x.data(), w.data()));
+    const int npair = R * (R + 1) / 2, nl = lmax + 1;
+    DevBuf dx, dw, dPt, dU, dM, dT;
+    MCMB_TRY(dx.alloc(sizeof(double) * npt));
+    MCMB_TRY(dw.alloc(sizeof(double) * npt));
+    MCMB_TRY(dPt.alloc(sizeof(double) * (size_t)nl * npt));
+    MCMB_TRY(dU.alloc(sizeof(double) * (size_t)nl * npair));
+    MCMB_TRY(dM.alloc(sizeof(double) * (size_t)npair * npt));
+    std::vector<int2> tiles;
+    const int T = (n + 63) / 64;
+    for (int I = 0; I < T; ++I)
+        for (int J = I; J < T; ++J) tiles.push_back(make_int2(I, J));
+    MCMB_TRY(dT.alloc(sizeof(int2) * tiles.size()));
+    MCMB_HIP(hipMemcpyAsync(dx.p, x.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(dw.p, w.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(dT.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(gl_legendre, dim3((npt + 255) / 256), dim3(256), 0, c->stream, dx.as<double>(), npt, L.lmin,
+                       lmax, L.d_v.as<double>(), dPt.as<double>());
+    hipLaunchKernelGGL(gl_pairs, dim3((unsigned)((nl * npair + 255) / 256)), dim3(256), 0, c->stream, L.d_q.as<double>(),
+                       L.ldq, R, lmax, dU.as<double>(), npair);
+    hipLaunchKernelGGL(gl_mgemm, dim3((npt + 63) / 64, (npair + 31) / 32), dim3(256), 0, c->stream, dPt.as<double>(),
+                       dU.as<double>(), npt, npair, nl, dM.as<double>());
+    hipLaunchKernelGGL(gl_gamma, dim3((unsigned)tiles.size()), dim3(256), 0, c->stream, dM.as<double>(), dw.as<double>(),
+                       npt, R, L.d_modes.as<int>(), n, dT.as<int2>(), d_gamma);
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipEventRecord(c->ev1, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = 4;
+    return MCMB_OK;
+}
+
+}  // namespace mcmb
+
+using namespace mcmb;
+
+extern "C" int mcmb_gamma_gl(mcmb_ctx* c, int npt, double* gamma_host, double* gamma_dev)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_gamma_gl: null context");
+    if (!c->ls.ready) return fail(MCMB_ERR_STATE, "mcmb_gamma_gl: call mcmb_lspace_set first");
+    if (npt > 0 && 2 * npt - 1 < 3 * c->ls.lmax)
+        return fail(MCMB_ERR_ARG, "mcmb_gamma_gl: npt %d < (3 lmax + 1)/2 is not exact", npt);
+    MCMB_TRY(set_device(c));
+    const int n = c->ls.n;
+    double* dst = gamma_dev;
+    if (!dst) {
+        MCMB_TRY(c->d_gamma_tmp.alloc(sizeof(double) * (size_t)n * n));
+        dst = c->d_gamma_tmp.as<double>();
+    }
+    MCMB_TRY(gram_gl_launch(c, npt, dst));
+    if (gamma_host)
+        MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return MCMB_OK;
+}

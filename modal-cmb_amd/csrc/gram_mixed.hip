@@ -200,22 +200,30 @@ __global__ __launch_bounds__(MNT, 1) void gram_tet_mixed_kernel(
             const bf16x8* Bl = DIAG ? S.op[b][1] : S.op[b][3];
 #pragma unroll
             for (int ks = 0; ks < 2; ++ks) {
+                // all 8 operand fragments of this K step once (16 ds_read_b128),
+                // then 48 MFMAs
                 const int cidx = ks * 2 + (lane >> 5);
+                bf16x8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int ma = wm * 128 + a * 32 + (lane & 31);
+                    ah[a] = Ah[swz(ma, cidx)];
+                    al[a] = Al[swz(ma, cidx)];
+                }
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     const int mb = wn * 128 + c * 32 + (lane & 31);
-                    const bf16x8 bh = Bh[swz(mb, cidx)];
-                    const bf16x8 bl = Bl[swz(mb, cidx)];
+                    bh[c] = Bh[swz(mb, cidx)];
+                    bl[c] = Bl[swz(mb, cidx)];
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
 #pragma unroll
                     for (int a = 0; a < 4; ++a) {
-                        const int ma = wm * 128 + a * 32 + (lane & 31);
-                        const bf16x8 ah = Ah[swz(ma, cidx)];
-                        const bf16x8 al = Al[swz(ma, cidx)];
-                        acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[a][c], 0, 0, 0);
-                        acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[a][c], 0, 0, 0);
-                        acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[a][c], 0, 0, 0);
+                        acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
+                        acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
+                        acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
                     }
-                }
             }
             if (ch + 1 < nch) {
                 lo2A += form_mixed(S, b ^ 1, rA, sA, tA, okA, tid, S.op[b ^ 1][0], S.op[b ^ 1][1]);
@@ -251,9 +259,9 @@ __global__ void to_float_table(const double* __restrict__ q, int ldq, int R, int
     qf[e] = c < R ? (float)q[(long)l * ldq + c] : 0.f;
 }
 
-int gram_mixed_launch(mcmb_ctx* c, const long* d_rowptr, const double* d_ws, double* d_gamma, int row_lo, int row_hi)
+int gram_mixed_launch(mcmb_ctx* c, PointSet& L, const long* d_rowptr, const double* d_ws, double* d_gamma,
+                      int row_lo, int row_hi)
 {
-    LSpace& L = c->ls;
     const int n = L.n, R = L.R;
     if (R > QP) return fail(MCMB_ERR_ARG, "mixed Gram supports basis width <= %d (got %d)", QP, R);
     const int T = (n + MT - 1) / MT, n_pad = T * MT;
@@ -312,14 +320,14 @@ int gram_mixed_launch(mcmb_ctx* c, const long* d_rowptr, const double* d_ws, dou
     if (!otiles.empty()) {
         hipLaunchKernelGGL(gram_tet_mixed_kernel<false>, dim3((unsigned)(otiles.size() * nseg)), dim3(MNT), smem,
                            c->stream, L.d_rows.as<RowDesc>(), d_rowptr, d_seg.as<int2>(), nchunks, segch,
-                           d_ot.as<int2>(), (int)otiles.size(), L.d_modes.as<int>(), n, L.d_qf.as<float>(), R, 2,
+                           d_ot.as<int2>(), (int)otiles.size(), L.d_modes.as<int>(), n, L.d_qf.as<float>(), R, L.kstep,
                            d_ws, c->d_partial.as<double>(), d_corr.as<double>(), n_pad);
         MCMB_HIP(hipGetLastError());
         ++launches;
     }
     hipLaunchKernelGGL(gram_tet_mixed_kernel<true>, dim3((unsigned)(dtiles.size() * nseg)), dim3(MNT), smem,
                        c->stream, L.d_rows.as<RowDesc>(), d_rowptr, d_seg.as<int2>(), nchunks, segch,
-                       d_dt.as<int2>(), (int)dtiles.size(), L.d_modes.as<int>(), n, L.d_qf.as<float>(), R, 2,
+                       d_dt.as<int2>(), (int)dtiles.size(), L.d_modes.as<int>(), n, L.d_qf.as<float>(), R, L.kstep,
                        d_ws, c->d_partial.as<double>(), d_corr.as<double>(), n_pad);
     MCMB_HIP(hipGetLastError());
     ++launches;
@@ -336,27 +344,5 @@ int gram_mixed_launch(mcmb_ctx* c, const long* d_rowptr, const double* d_ws, dou
     return MCMB_OK;
 }
 
-int gram_gl_launch(mcmb_ctx* c, int npt, double* d_gamma)
-{
-    (void)c; (void)npt; (void)d_gamma;
-    return fail(MCMB_ERR_STATE, "GL-factorised Gram not available in this build");
-}
-
 }  // namespace mcmb
 
-extern "C" int mcmb_gamma_gl(mcmb_ctx* c, int npt, double* gh, double* gd)
-{
-    (void)npt; (void)gh; (void)gd;
-    if (!c) return mcmb::fail(MCMB_ERR_ARG, "null context");
-    return mcmb::gram_gl_launch(c, npt, gd);
-}
-extern "C" int mcmb_orthonormalise(mcmb_ctx* c, int n, const double* g, double* o, double* l, double* nm, int* bad)
-{
-    (void)c; (void)n; (void)g; (void)o; (void)l; (void)nm; (void)bad;
-    return mcmb::fail(MCMB_ERR_STATE, "orthonormalisation not available in this build");
-}
-extern "C" int mcmb_cholesky_solve(mcmb_ctx* c, int n, const double* f, int nrhs, const double* a, double* b)
-{
-    (void)c; (void)n; (void)f; (void)nrhs; (void)a; (void)b;
-    return mcmb::fail(MCMB_ERR_STATE, "cholesky solve not available in this build");
-}

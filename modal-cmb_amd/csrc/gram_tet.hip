@@ -60,6 +60,27 @@ __device__ __forceinline__ double lspace_weight(int i, int j, int k, int asym,
     return permsix_d(i, j, k) * h2 * v[i] * v[j] * v[k];
 }
 
+// k-space weight of the ordered GL point (a, b, c): permsix w_a w_b w_c / K
+__device__ __forceinline__ double kspace_weight(int a, int b, int c, const double* __restrict__ kn,
+                                                const double* __restrict__ kw)
+{
+    return permsix_d(a, b, c) * kw[a] * kw[b] * kw[c] / (kn[a] + kn[b] + kn[c]);
+}
+
+__global__ __launch_bounds__(256) void kspace_weights_kernel(
+    const RowDesc* __restrict__ rows, const long* __restrict__ rowptr, long nrows,
+    const double* __restrict__ kn, const double* __restrict__ kw, int scaled, double* __restrict__ out)
+{
+    long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= nrows) return;
+    RowDesc rd = rows[row];
+    long base = rowptr[row];
+    for (int t = threadIdx.x & 63; t < rd.nk; t += 64) {
+        double w = kspace_weight(rd.i, rd.j, rd.k0 + t, kn, kw);
+        out[base + t] = scaled ? sqrt(w) * (1.0 / 6.0) : w;
+    }
+}
+
 // One wave per row: w_p (mode 0) or sqrt(w_p)/6 (mode 1), p in walk order.
 __global__ __launch_bounds__(256) void lspace_weights_kernel(
     const RowDesc* __restrict__ rows, const long* __restrict__ rowptr, long nrows, int asym,
@@ -78,7 +99,7 @@ __global__ __launch_bounds__(256) void lspace_weights_kernel(
 }
 
 __global__ __launch_bounds__(256) void lspace_points_kernel(
-    const RowDesc* __restrict__ rows, const long* __restrict__ rowptr, long nrows, long cap,
+    const RowDesc* __restrict__ rows, const long* __restrict__ rowptr, long nrows, long cap, int kstep,
     int* __restrict__ ijk)
 {
     long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -90,7 +111,7 @@ __global__ __launch_bounds__(256) void lspace_points_kernel(
         if (p < cap) {
             ijk[3 * p] = rd.i;
             ijk[3 * p + 1] = rd.j;
-            ijk[3 * p + 2] = rd.k0 + 2 * t;
+            ijk[3 * p + 2] = rd.k0 + kstep * t;
         }
     }
 }
@@ -253,7 +274,7 @@ static void make_segments(const std::vector<RowDesc>& rows, int lo, int hi, int 
     seg[nseg] = hi;
 }
 
-void lspace_slice(const LSpace& L, int kc, int rank, int nranks, int* row_lo, int* row_hi, double* frac)
+void lspace_slice(const PointSet& L, int kc, int rank, int nranks, int* row_lo, int* row_hi, double* frac)
 {
     const std::vector<RowDesc>& rows = L.rows;
     std::vector<long> pre(rows.size() + 1, 0);
@@ -267,9 +288,8 @@ void lspace_slice(const LSpace& L, int kc, int rank, int nranks, int* row_lo, in
     if (frac) *frac = tot ? (double)(pre[*row_hi] - pre[*row_lo]) / (double)tot : 0.0;
 }
 
-static int ensure_lspace_aux(mcmb_ctx* c, DevBuf& rowptr, DevBuf& ws, int scaled)
+static int ensure_lspace_aux(mcmb_ctx* c, PointSet& L, DevBuf& rowptr, DevBuf& ws, int scaled)
 {
-    LSpace& L = c->ls;
     std::vector<long> rp(L.rows.size() + 1, 0);
     for (size_t r = 0; r < L.rows.size(); ++r) rp[r + 1] = rp[r] + L.rows[r].nk;
     MCMB_TRY(rowptr.alloc(sizeof(long) * rp.size()));
@@ -277,22 +297,27 @@ static int ensure_lspace_aux(mcmb_ctx* c, DevBuf& rowptr, DevBuf& ws, int scaled
     MCMB_TRY(ws.alloc(sizeof(double) * (size_t)L.npoints));
     long nrows = (long)L.rows.size();
     dim3 grid((unsigned)((nrows + 3) / 4));
-    hipLaunchKernelGGL(lspace_weights_kernel, grid, dim3(256), 0, c->stream, L.d_rows.as<RowDesc>(),
-                       rowptr.as<long>(), nrows, L.h2mode == MCMB_H2_ASYMPTOTIC ? 1 : 0,
-                       L.d_ftab.as<double>(), L.d_v.as<double>(), scaled, ws.as<double>());
+    if (L.kspace)
+        hipLaunchKernelGGL(kspace_weights_kernel, grid, dim3(256), 0, c->stream, L.d_rows.as<RowDesc>(),
+                           rowptr.as<long>(), nrows, L.d_kn.as<double>(), L.d_kw.as<double>(), scaled,
+                           ws.as<double>());
+    else
+        hipLaunchKernelGGL(lspace_weights_kernel, grid, dim3(256), 0, c->stream, L.d_rows.as<RowDesc>(),
+                           rowptr.as<long>(), nrows, L.h2mode == MCMB_H2_ASYMPTOTIC ? 1 : 0,
+                           L.d_ftab.as<double>(), L.d_v.as<double>(), scaled, ws.as<double>());
     MCMB_HIP(hipGetLastError());
     return MCMB_OK;
 }
 
-int gram_mixed_launch(mcmb_ctx* c, const long* d_rowptr, const double* d_ws, double* d_gamma, int row_lo, int row_hi);
+int gram_mixed_launch(mcmb_ctx* c, PointSet& L, const long* d_rowptr, const double* d_ws, double* d_gamma,
+                      int row_lo, int row_hi);
 
-int gram_tet_launch(mcmb_ctx* c, int precision, double* d_gamma, int row_lo, int row_hi)
+int gram_tet_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gamma, int row_lo, int row_hi)
 {
-    LSpace& L = c->ls;
     DevBuf rowptr, ws;
-    MCMB_TRY(ensure_lspace_aux(c, rowptr, ws, 1));
+    MCMB_TRY(ensure_lspace_aux(c, L, rowptr, ws, 1));
     if (precision == MCMB_MIXED)
-        return gram_mixed_launch(c, rowptr.as<long>(), ws.as<double>(), d_gamma, row_lo, row_hi);
+        return gram_mixed_launch(c, L, rowptr.as<long>(), ws.as<double>(), d_gamma, row_lo, row_hi);
 
     const int n = L.n;
     const int T = (n + TM - 1) / TM, n_pad = T * TM;
@@ -320,7 +345,7 @@ int gram_tet_launch(mcmb_ctx* c, int precision, double* d_gamma, int row_lo, int
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(gram_tet_f64_kernel, dim3((unsigned)(ntiles * nseg)), dim3(NT), 0, c->stream,
                        L.d_rows.as<RowDesc>(), rowptr.as<long>(), d_seg.as<int>(), nseg,
-                       d_tiles.as<int2>(), ntiles, L.d_modes.as<int>(), n, L.d_q.as<double>(), L.ldq, 2,
+                       d_tiles.as<int2>(), ntiles, L.d_modes.as<int>(), n, L.d_q.as<double>(), L.ldq, L.kstep,
                        ws.as<double>(), c->d_partial.as<double>(), n_pad);
     MCMB_HIP(hipGetLastError());
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
@@ -357,7 +382,7 @@ extern "C" long mcmb_lspace_points(mcmb_ctx* c, int* ijk, long cap)
     MCMB_HIP(hipMemcpyAsync(rowptr.p, rp.data(), sizeof(long) * rp.size(), hipMemcpyHostToDevice, c->stream));
     long nrows = (long)L.rows.size();
     hipLaunchKernelGGL(lspace_points_kernel, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, c->stream,
-                       L.d_rows.as<RowDesc>(), rowptr.as<long>(), nrows, m, out.as<int>());
+                       L.d_rows.as<RowDesc>(), rowptr.as<long>(), nrows, m, L.kstep, out.as<int>());
     MCMB_HIP(hipGetLastError());
     MCMB_HIP(hipMemcpyAsync(ijk, out.p, sizeof(int) * 3 * (size_t)m, hipMemcpyDeviceToHost, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
@@ -370,7 +395,7 @@ extern "C" long mcmb_lspace_weights(mcmb_ctx* c, double* w, long cap)
     if (cap < 0 || (cap > 0 && !w)) return fail(MCMB_ERR_ARG, "mcmb_lspace_weights: bad buffer");
     MCMB_TRY(set_device(c));
     DevBuf rowptr, ws;
-    MCMB_TRY(ensure_lspace_aux(c, rowptr, ws, 0));
+    MCMB_TRY(ensure_lspace_aux(c, c->ls, rowptr, ws, 0));
     long m = std::min(cap, c->ls.npoints);
     if (m > 0) MCMB_HIP(hipMemcpyAsync(w, ws.p, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));

@@ -43,22 +43,25 @@ struct DevBuf {
 // Weight of a tetrapyd point.
 enum WeightKind : int { W_LSPACE_EXACT = 0, W_LSPACE_ASYM = 1, W_KSPACE_GL = 2 };
 
-struct LSpace {
+// An ordered tetrapyd point set with its 1D basis and mode table:
+//  l-space: rows of T(lmin, lmax), k in steps of 2, weight permsix h^2 v v v;
+//  k-space: Gauss-Legendre node indices a <= b <= c with k_c <= k_a + k_b,
+//           k in steps of 1, weight permsix w_a w_b w_c / (k_a + k_b + k_c).
+struct PointSet {
     bool ready = false;
+    bool kspace = false;
     long generation = 0;
-    int lmin = 0, lmax = 0, ldq = 0, n = 0, h2mode = 0, R = 0;
+    int lmin = 0, lmax = 0;           // l-space multipole range; k-space: 0, N-1
+    int ldq = 0, n = 0, h2mode = 0, R = 0, kstep = 2;
     long npoints = 0;
     std::vector<RowDesc> rows;        // host copy of the row table (walk order)
-    std::vector<long> row_chunk_prefix;  // prefix of ceil(nk/KC) over rows
     std::vector<int> order;           // [n][3]
     std::vector<double> q;            // [(lmax+1)][ldq]
-    std::vector<double> v;            // [lmax+1]
-    DevBuf d_rows, d_q, d_qf, d_v, d_ftab, d_modes;
-    DevBuf d_gamma_cache;             // cached FP64 tetrapyd Gram (legacy shims)
-    bool gamma_cached = false;
-    DevBuf d_gl_cache;
-    bool gl_cached = false;
+    std::vector<double> v;            // l-space per-leg weight [lmax+1]
+    std::vector<double> kn, kw;       // k-space nodes and weights [N]
+    DevBuf d_rows, d_q, d_qf, d_v, d_ftab, d_modes, d_kn, d_kw;
 };
+typedef PointSet LSpace;
 
 }  // namespace mcmb
 
@@ -68,7 +71,10 @@ struct mcmb_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int last_launches = 0;
-    mcmb::LSpace ls;
+    mcmb::PointSet ls;   // late-time multipole tetrapyd
+    mcmb::PointSet ks;   // primordial k-space Gauss-Legendre tetrapyd
+    mcmb::DevBuf d_W;    // full-cube projection weight W[i][j][k] (k-space)
+    int w_ready = 0;
     // scratch for partial Gram tiles
     mcmb::DevBuf d_partial;
     mcmb::DevBuf d_gamma_tmp;
@@ -78,9 +84,10 @@ struct mcmb_ctx {
 namespace mcmb {
 // Build the ordered-tetrapyd row table of T(lmin, lmax) in walk order.
 void build_lspace_rows(int lmin, int lmax, std::vector<RowDesc>& rows, long& npoints);
-int gram_tet_launch(mcmb_ctx* ctx, int precision, double* d_gamma, int row_lo, int row_hi);
+int gram_tet_launch(mcmb_ctx* ctx, PointSet& P, int precision, double* d_gamma, int row_lo, int row_hi);
 // contiguous share [row_lo, row_hi) of rank r of nr, balanced by chunk count
-void lspace_slice(const LSpace& L, int kc, int rank, int nranks, int* row_lo, int* row_hi, double* frac);
+void lspace_slice(const PointSet& L, int kc, int rank, int nranks, int* row_lo, int* row_hi, double* frac);
+int upload_pointset(mcmb_ctx* ctx, PointSet& P, const std::vector<int>& order);
 int gram_gl_launch(mcmb_ctx* ctx, int npt, double* d_gamma);
 int set_device(mcmb_ctx* ctx);
 }  // namespace mcmb

@@ -1,0 +1,569 @@
+// Shape projection on the Gauss-Legendre k-tetrapyd, with tangent (forward)
+// and adjoint (reverse) derivatives in (omega, phi), for the feature shape
+// S = sin(omega (k1+k2+k3) + phi) / (k1 k2 k3)^2 (calculate_shape.c:933-946).
+//
+//   alpha_n(theta) = sum_{cube} W_ijk sin(omega K_ijk + phi) Q_n(i,j,k)
+//   W_ijk = mask_ijk w_i w_j w_k / (k_i + k_j + k_k)
+// restates calculate_eigen (calculate_eigen.c:86-145) on the GL grid: its
+// integrand is shape3/K = (k1k2k3)^2 S / K.  W and S are symmetric, so
+// alpha_n = A[r][s][t] with A_rst = sum_ijk F_ijk q_r(i) q_s(j) q_t(k), F = W S.
+//
+// With A_ij = omega (k_i + k_j) + phi and B_k = omega k_k,
+//   sin(A + B) = sin A cos B + cos A sin B,
+// the k-sum separates:  Y[(i,j)][c,t] = sum_k W_ijk Z[k][c,t] with the four
+// column sets Z = {cos B, sin B, k cos B, k sin B} q_t(k)  -- one MFMA GEMM
+// (W resident in HBM, N^2 x N) for a batch of shapes.  Then per shape
+//   G0 = sA Yc + cA Ys                         (value,  sin(A+B))
+//   G2 = cA Yc - sA Ys                         (d/dphi, cos(A+B))
+//   G1 = (k_i + k_j) G2 + cA Yc' - sA Ys'      (d/domega, K cos(A+B))
+//   H_x[i][s][t] = sum_j q_s(j) G_x[i][j][t],  alpha_x[n] = sum_i q_r(i) H_x[i][s][t].
+// The adjoint (VJP) takes gbar = dL/dalpha, forms Hbar[i][s][t] =
+// sum_r q_r(i) Abar[r][s][t] (Abar = gbar scattered onto the canonical
+// triples), Gbar[i][j][t] = sum_s q_s(j) Hbar[i][s][t], and reduces
+// dL/dphi = sum Gbar G2, dL/domega = sum Gbar G1.
+#include "mcmb_internal.h"
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace mcmb {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int PT = 128;     // GEMM tile (M and N)
+constexpr int PK = 16;      // GEMM K chunk
+constexpr int PBATCH = 32;  // shapes per batch
+
+// ---- full-cube weight ------------------------------------------------------
+// W[(i*N + j)*Kp + k]; the mask compares the sorted triple exactly as the
+// oracle does: k_c <= k_a + k_b (FP64).
+template <class T>
+__global__ void proj_build_W(const double* __restrict__ kn, const double* __restrict__ kw, int N, int Kp,
+                             T* __restrict__ W)
+{
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t tot = (size_t)N * N * Kp;
+    if (e >= tot) return;
+    int k = (int)(e % Kp);
+    size_t ij = e / Kp;
+    int j = (int)(ij % N), i = (int)(ij / N);
+    double w = 0.0;
+    if (k < N) {
+        int a = i, b = j, c = k, t;
+        if (a > b) { t = a; a = b; b = t; }
+        if (b > c) { t = b; b = c; c = t; if (a > b) { t = a; a = b; b = t; } }
+        if (kn[c] <= kn[a] + kn[b]) w = kw[a] * kw[b] * kw[c] / (kn[a] + kn[b] + kn[c]);
+    }
+    W[e] = (T)w;
+}
+
+// Z[k][col], col = (b*4 + c)*R + t for shapes b < nb of the batch; Kp x ldz.
+template <class T>
+__global__ void proj_build_Z(const double* __restrict__ kn, const double* __restrict__ q, int ldq, int N, int R,
+                             const double* __restrict__ theta, int nb, int Kp, int ldz, T* __restrict__ Z)
+{
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)Kp * ldz) return;
+    int k = (int)(e / ldz), col = (int)(e % ldz);
+    double z = 0.0;
+    if (k < N && col < nb * 4 * R) {
+        int t = col % R, c = (col / R) % 4, b = col / (4 * R);
+        double om = theta[2 * b];
+        double sb, cb;
+        sincos(om * kn[k], &sb, &cb);
+        double v = (c & 1) ? sb : cb;
+        if (c >= 2) v *= kn[k];
+        z = v * q[(size_t)k * ldq + t];
+    }
+    Z[e] = (T)z;
+}
+
+// ---- MFMA GEMM  C[M][Nc] = A[M][Kp] B[Kp][ldb]  (A row-major lda = Kp) ------
+template <class T> struct MF;
+template <> struct MF<double> {
+    typedef d4 V;
+    static __device__ __forceinline__ V mma(double a, double b, V c)
+    { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    // output row of element v for lane: 4v + lane/16
+    static __device__ __forceinline__ int row(int v, int lane) { return 4 * v + (lane >> 4); }
+};
+template <> struct MF<float> {
+    typedef f4 V;
+    static __device__ __forceinline__ V mma(float a, float b, V c)
+    { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    // output row of element v for lane: 4 (lane/16) + v
+    static __device__ __forceinline__ int row(int v, int lane) { return 4 * (lane >> 4) + v; }
+};
+
+template <class T>
+__global__ __launch_bounds__(256, 2) void proj_gemm(const T* __restrict__ A, const T* __restrict__ B, int M, int Kp,
+                                                    int Nc, int ldb, T* __restrict__ C, int ldc)
+{
+    __shared__ T As[2][PK][PT + 4];
+    __shared__ T Bs[2][PK][PT + 4];
+    const int m0 = blockIdx.y * PT, n0 = blockIdx.x * PT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    typename MF<T>::V acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[a][b][v] = 0;
+    // staging: thread loads 8 elements of A (rows r, k chunk) and 8 of B
+    T ra[8], rb[8];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int e = tid + u * 256;                  // 0..2047
+            int r = e >> 4, kk = e & 15;            // A: 128 rows x 16
+            int gm = m0 + r;
+            ra[u] = (gm < M) ? A[(size_t)gm * Kp + k0 + kk] : (T)0;
+            int kb = e >> 7, cc = e & 127;          // B: 16 x 128
+            int gn = n0 + cc;
+            rb[u] = (gn < Nc) ? B[(size_t)(k0 + kb) * ldb + gn] : (T)0;
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            int e = tid + u * 256;
+            As[buf][e & 15][e >> 4] = ra[u];
+            Bs[buf][e >> 7][e & 127] = rb[u];
+        }
+    };
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    const int nk = Kp / PK;
+    for (int kc = 0; kc < nk; ++kc) {
+        const int buf = kc & 1;
+        if (kc + 1 < nk) gload((kc + 1) * PK);
+#pragma unroll
+        for (int ks = 0; ks < PK / 4; ++ks) {
+            const int kk = ks * 4 + (lane >> 4);
+            T av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = As[buf][kk][wm * 64 + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bv[b] = Bs[buf][kk][wn * 64 + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = MF<T>::mma(av[a], bv[b], acc[a][b]);
+        }
+        if (kc + 1 < nk) sstore(buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int gm = m0 + wm * 64 + a * 16 + MF<T>::row(v, lane);
+                int gn = n0 + wn * 64 + b * 16 + (lane & 15);
+                if (gm < M && gn < Nc) C[(size_t)gm * ldc + gn] = acc[a][b][v];
+            }
+}
+
+// ---- forward stage 2: per (shape b, i) -------------------------------------
+// H[b][x][i][s][t] = sum_j q_s(j) G_x(i, j, t), x = value, d/domega, d/dphi.
+template <class T>
+__global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int ldy, const double* __restrict__ kn,
+                                                   const double* __restrict__ q, int ldq, int N, int R,
+                                                   const double* __restrict__ theta, int nb, double* __restrict__ H)
+{
+    __shared__ double G[3][32][33];     // [x][j chunk][t]
+    __shared__ double Qs[32][33];       // q_s(j) for the chunk
+    const int b = blockIdx.x % nb, i = blockIdx.x / nb;
+    const double om = theta[2 * b], ph = theta[2 * b + 1];
+    const int tid = threadIdx.x;
+    // thread owns up to 12 (x, s, t) accumulators: e = tid + 256 u over 3 R^2 (R <= 32)
+    const int nacc = 3 * R * R;
+    double acc[12];
+#pragma unroll
+    for (int u = 0; u < 12; ++u) acc[u] = 0.0;
+    for (int j0 = 0; j0 < N; j0 += 32) {
+        __syncthreads();
+        for (int e = tid; e < 32 * R; e += 256) {
+            int jj = e / R, t = e % R, j = j0 + jj;
+            double g0 = 0, g1 = 0, g2 = 0, qq = 0;
+            if (j < N) {
+                const T* y = Y + (size_t)(i * N + j) * ldy + (size_t)b * 4 * R;
+                double yc = (double)y[t], ys = (double)y[R + t], ycp = (double)y[2 * R + t], ysp = (double)y[3 * R + t];
+                double sA, cA;
+                sincos(om * (kn[i] + kn[j]) + ph, &sA, &cA);
+                g0 = sA * yc + cA * ys;
+                g2 = cA * yc - sA * ys;
+                g1 = (kn[i] + kn[j]) * g2 + cA * ycp - sA * ysp;
+                qq = q[(size_t)j * ldq + t];
+            }
+            G[0][jj][t] = g0;
+            G[1][jj][t] = g1;
+            G[2][jj][t] = g2;
+            Qs[jj][t] = qq;
+        }
+        __syncthreads();
+        const int jn = min(32, N - j0);
+#pragma unroll
+        for (int u = 0; u < 12; ++u) {
+            int e = tid + 256 * u;
+            if (e < nacc) {
+                int x = e / (R * R), st = e % (R * R), s = st / R, t = st % R;
+                double a = acc[u];
+                for (int jj = 0; jj < jn; ++jj) a += Qs[jj][s] * G[x][jj][t];
+                acc[u] = a;
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 12; ++u) {
+        int e = tid + 256 * u;
+        if (e < nacc) {
+            int x = e / (R * R), st = e % (R * R);
+            H[(((size_t)b * 3 + x) * N + i) * R * R + st] = acc[u];
+        }
+    }
+}
+
+// alpha_x[b][n] = sum_i q_{r_n}(i) H[b][x][i][s_n][t_n]
+__global__ void proj_stage3(const double* __restrict__ H, const double* __restrict__ q, int ldq, int N, int R,
+                            const int* __restrict__ modes, int n, int nb, double* __restrict__ out)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb * 3 * n) return;
+    int m = e % n, x = (e / n) % 3, b = e / (3 * n);
+    int pk = modes[m], r = pk & 255, s = (pk >> 8) & 255, t = (pk >> 16) & 255;
+    const double* h = H + (((size_t)b * 3 + x) * N) * R * R + s * R + t;
+    double a = 0.0;
+    for (int i = 0; i < N; ++i) a += q[(size_t)i * ldq + r] * h[(size_t)i * R * R];
+    out[e] = a;   // [b][x][n]
+}
+
+// ---- adjoint ---------------------------------------------------------------
+// Hbar[b][i][s][t] = sum over canonical triples (r, s, t) = order[m] of
+// gbar[b][m] q_r(i)   (one thread per (b, i, s, t))
+__global__ void proj_hbar(const double* __restrict__ gbar, const int* __restrict__ slot, const double* __restrict__ q,
+                          int ldq, int N, int R, int n, int nb, double* __restrict__ Hb)
+{
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nb * N * R * R) return;
+    int st = (int)(e % (R * R));
+    int i = (int)((e / (R * R)) % N);
+    int b = (int)(e / ((size_t)R * R * N));
+    double a = 0.0;
+    for (int r = 0; r < R; ++r) {
+        int m = slot[r * R * R + st];    // mode index of the canonical (r, s, t), or -1
+        if (m >= 0) a += gbar[(size_t)b * n + m] * q[(size_t)i * ldq + r];
+    }
+    Hb[e] = a;
+}
+
+// dtheta[b] += sum_{j,t} Gbar(i,j,t) (G1, G2)(i,j,t) for one (b, i) per workgroup
+template <class T>
+__global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int ldy, const double* __restrict__ kn,
+                                                    const double* __restrict__ q, int ldq, int N, int R,
+                                                    const double* __restrict__ theta, int nb,
+                                                    const double* __restrict__ Hb, double* __restrict__ dpart)
+{
+    __shared__ double Hs[32 * 32];
+    __shared__ double red[2][256];
+    const int b = blockIdx.x % nb, i = blockIdx.x / nb;
+    const double om = theta[2 * b], ph = theta[2 * b + 1];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < R * R; e += 256) Hs[e] = Hb[(((size_t)b * N) + i) * R * R + e];
+    __syncthreads();
+    double dom = 0.0, dph = 0.0;
+    for (int e = tid; e < N * R; e += 256) {
+        int j = e / R, t = e % R;
+        double gb = 0.0;
+        for (int s = 0; s < R; ++s) gb += q[(size_t)j * ldq + s] * Hs[s * R + t];
+        const T* y = Y + (size_t)(i * N + j) * ldy + (size_t)b * 4 * R;
+        double yc = (double)y[t], ys = (double)y[R + t], ycp = (double)y[2 * R + t], ysp = (double)y[3 * R + t];
+        double sA, cA;
+        sincos(om * (kn[i] + kn[j]) + ph, &sA, &cA);
+        double g2 = cA * yc - sA * ys;
+        double g1 = (kn[i] + kn[j]) * g2 + cA * ycp - sA * ysp;
+        dom += gb * g1;
+        dph += gb * g2;
+    }
+    red[0][tid] = dom;
+    red[1][tid] = dph;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) { red[0][tid] += red[0][tid + w]; red[1][tid] += red[1][tid + w]; }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        dpart[((size_t)b * N + i) * 2] = red[0][0];
+        dpart[((size_t)b * N + i) * 2 + 1] = red[1][0];
+    }
+}
+
+// dtheta[b][x] = sum_i dpart[b][i][x] (fixed order)
+__global__ void proj_sum_parts(const double* __restrict__ dpart, int N, int nb, double* __restrict__ out)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb * 2) return;
+    int b = e / 2, x = e % 2;
+    double s = 0.0;
+    for (int i = 0; i < N; ++i) s += dpart[((size_t)b * N + i) * 2 + x];
+    out[e] = s;
+}
+
+// ---- host side ---------------------------------------------------------------
+
+static int ensure_W(mcmb_ctx* c, int prec)
+{
+    PointSet& K = c->ks;
+    const int N = K.lmax + 1, Kp = (N + PK - 1) / PK * PK;
+    if (c->w_ready == prec + 1) return MCMB_OK;
+    size_t tot = (size_t)N * N * Kp;
+    if (prec == MCMB_FP64) {
+        MCMB_TRY(c->d_W.alloc(tot * sizeof(double)));
+        hipLaunchKernelGGL(proj_build_W<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
+                           K.d_kn.as<double>(), K.d_kw.as<double>(), N, Kp, c->d_W.as<double>());
+    } else {
+        MCMB_TRY(c->d_W.alloc(tot * sizeof(float)));
+        hipLaunchKernelGGL(proj_build_W<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
+                           K.d_kn.as<double>(), K.d_kw.as<double>(), N, Kp, c->d_W.as<float>());
+    }
+    MCMB_HIP(hipGetLastError());
+    c->w_ready = prec + 1;
+    return MCMB_OK;
+}
+
+template <class T>
+static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, DevBuf& Zb, DevBuf& Yb, int& ldy)
+{
+    PointSet& K = c->ks;
+    const int N = K.lmax + 1, R = K.R, Kp = (N + PK - 1) / PK * PK;
+    const int Nc = nb * 4 * R;
+    const int ldz = (Nc + PT - 1) / PT * PT;
+    ldy = ldz;
+    MCMB_TRY(Zb.alloc(sizeof(T) * (size_t)Kp * ldz));
+    MCMB_TRY(Yb.alloc(sizeof(T) * (size_t)N * N * ldy));
+    size_t ze = (size_t)Kp * ldz;
+    hipLaunchKernelGGL(proj_build_Z<T>, dim3((unsigned)((ze + 255) / 256)), dim3(256), 0, c->stream,
+                       K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb, Kp, ldz, Zb.as<T>());
+    const int M = N * N;
+    dim3 grid((unsigned)(ldz / PT), (unsigned)((M + PT - 1) / PT));
+    hipLaunchKernelGGL(proj_gemm<T>, grid, dim3(256), 0, c->stream, c->d_W.as<T>(), Zb.as<T>(), M, Kp, Nc, ldz,
+                       Yb.as<T>(), ldy);
+    MCMB_HIP(hipGetLastError());
+    return MCMB_OK;
+}
+
+template <class T>
+static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double* d_out /* [nb][3][n] */)
+{
+    PointSet& K = c->ks;
+    const int N = K.lmax + 1, R = K.R, n = K.n;
+    DevBuf Zb, Yb, Hb;
+    int ldy = 0;
+    MCMB_TRY(run_stage1<T>(c, d_theta, nb, Zb, Yb, ldy));
+    MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)nb * 3 * N * R * R));
+    hipLaunchKernelGGL(proj_stage2<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
+                       K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb, Hb.as<double>());
+    const int ne = nb * 3 * n;
+    hipLaunchKernelGGL(proj_stage3, dim3((ne + 255) / 256), dim3(256), 0, c->stream, Hb.as<double>(),
+                       K.d_q.as<double>(), K.ldq, N, R, K.d_modes.as<int>(), n, nb, d_out);
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return MCMB_OK;
+}
+
+template <class T>
+static int project_bwd_batch(mcmb_ctx* c, const double* d_theta, int nb, const double* d_gbar, const int* d_slot,
+                             double* d_dtheta)
+{
+    PointSet& K = c->ks;
+    const int N = K.lmax + 1, R = K.R, n = K.n;
+    DevBuf Zb, Yb, Hb, Dp;
+    int ldy = 0;
+    MCMB_TRY(run_stage1<T>(c, d_theta, nb, Zb, Yb, ldy));
+    MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)nb * N * R * R));
+    MCMB_TRY(Dp.alloc(sizeof(double) * (size_t)nb * N * 2));
+    size_t he = (size_t)nb * N * R * R;
+    hipLaunchKernelGGL(proj_hbar, dim3((unsigned)((he + 255) / 256)), dim3(256), 0, c->stream, d_gbar, d_slot,
+                       K.d_q.as<double>(), K.ldq, N, R, n, nb, Hb.as<double>());
+    hipLaunchKernelGGL(proj_adjoint<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
+                       K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb, Hb.as<double>(),
+                       Dp.as<double>());
+    hipLaunchKernelGGL(proj_sum_parts, dim3((nb * 2 + 255) / 256), dim3(256), 0, c->stream, Dp.as<double>(), N, nb,
+                       d_dtheta);
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return MCMB_OK;
+}
+
+}  // namespace mcmb
+
+using namespace mcmb;
+
+extern "C" {
+
+int mcmb_kspace_set(mcmb_ctx* c, int N, const double* kn, const double* kw, int ldq, const double* q, int nmodes,
+                    const int* order)
+{
+    if (!c || !kn || !kw || !q || !order) return fail(MCMB_ERR_ARG, "mcmb_kspace_set: null argument");
+    if (N < 2 || N > 4096) return fail(MCMB_ERR_ARG, "mcmb_kspace_set: N %d not in [2,4096]", N);
+    if (ldq < 1 || ldq > 32) return fail(MCMB_ERR_ARG, "mcmb_kspace_set: ldq %d not in [1,32]", ldq);
+    if (nmodes < 1 || nmodes > 65536) return fail(MCMB_ERR_ARG, "mcmb_kspace_set: nmodes %d", nmodes);
+    for (int a = 1; a < N; ++a)
+        if (!(kn[a] > kn[a - 1]) || !(kn[0] > 0.0))
+            return fail(MCMB_ERR_ARG, "mcmb_kspace_set: nodes must be positive and strictly ascending");
+    int R = 0;
+    for (int m = 0; m < nmodes; ++m)
+        for (int a = 0; a < 3; ++a) {
+            int p = order[3 * m + a];
+            if (p < 0 || p >= ldq)
+                return fail(MCMB_ERR_ORDER, "mcmb_kspace_set: order[%d][%d] = %d outside [0,%d)", m, a, p, ldq);
+            if (p + 1 > R) R = p + 1;
+        }
+    MCMB_TRY(set_device(c));
+    PointSet& K = c->ks;
+    K.ready = false;
+    K.kspace = true;
+    K.kstep = 1;
+    K.lmin = 0;
+    K.lmax = N - 1;
+    K.ldq = ldq;
+    K.n = nmodes;
+    K.R = R;
+    K.order.assign(order, order + 3 * (size_t)nmodes);
+    K.q.assign(q, q + (size_t)N * ldq);
+    K.kn.assign(kn, kn + N);
+    K.kw.assign(kw, kw + N);
+    // ordered masked points a <= b <= c, k_c <= k_a + k_b (nodes ascending)
+    K.rows.clear();
+    K.npoints = 0;
+    for (int a = 0; a < N; ++a)
+        for (int b = a; b < N; ++b) {
+            int cmax = b;
+            while (cmax + 1 < N && kn[cmax + 1] <= kn[a] + kn[b]) ++cmax;
+            K.rows.push_back(RowDesc{a, b, b, cmax - b + 1});
+            K.npoints += cmax - b + 1;
+        }
+    MCMB_TRY(K.d_kn.alloc(sizeof(double) * N));
+    MCMB_TRY(K.d_kw.alloc(sizeof(double) * N));
+    MCMB_HIP(hipMemcpyAsync(K.d_kn.p, kn, sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(K.d_kw.p, kw, sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+    MCMB_TRY(upload_pointset(c, K, K.order));
+    c->w_ready = 0;
+    K.ready = true;
+    K.generation += 1;
+    return MCMB_OK;
+}
+
+long mcmb_kspace_npoints(mcmb_ctx* c)
+{
+    if (!c || !c->ks.ready) return fail(MCMB_ERR_STATE, "mcmb_kspace_npoints: no k-space setup");
+    return c->ks.npoints;
+}
+
+int mcmb_kgram(mcmb_ctx* c, int precision, double* gamma_host, double* gamma_dev)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_kgram: null context");
+    if (!c->ks.ready) return fail(MCMB_ERR_STATE, "mcmb_kgram: call mcmb_kspace_set first");
+    if (precision != MCMB_FP64 && precision != MCMB_MIXED) return fail(MCMB_ERR_ARG, "mcmb_kgram: precision %d", precision);
+    MCMB_TRY(set_device(c));
+    const int n = c->ks.n;
+    double* dst = gamma_dev;
+    if (!dst) {
+        MCMB_TRY(c->d_gamma_tmp.alloc(sizeof(double) * (size_t)n * n));
+        dst = c->d_gamma_tmp.as<double>();
+    }
+    MCMB_TRY(gram_tet_launch(c, c->ks, precision, dst, 0, (int)c->ks.rows.size()));
+    if (gamma_host)
+        MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return MCMB_OK;
+}
+
+int mcmb_project_fwd(mcmb_ctx* c, int precision, int ntheta, const double* theta, double* alpha, double* dalpha)
+{
+    if (!c || !theta || !alpha || ntheta < 1) return fail(MCMB_ERR_ARG, "mcmb_project_fwd: bad argument");
+    if (!c->ks.ready) return fail(MCMB_ERR_STATE, "mcmb_project_fwd: call mcmb_kspace_set first");
+    if (precision != MCMB_FP64 && precision != MCMB_FP32) return fail(MCMB_ERR_ARG, "mcmb_project_fwd: precision %d", precision);
+    MCMB_TRY(set_device(c));
+    const int n = c->ks.n;
+    MCMB_TRY(ensure_W(c, precision));
+    DevBuf dth, dout;
+    std::vector<double> out((size_t)PBATCH * 3 * n);
+    MCMB_TRY(dth.alloc(sizeof(double) * 2 * PBATCH));
+    MCMB_TRY(dout.alloc(sizeof(double) * (size_t)PBATCH * 3 * n));
+    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int b0 = 0; b0 < ntheta; b0 += PBATCH) {
+        const int nb = std::min(PBATCH, ntheta - b0);
+        MCMB_HIP(hipMemcpyAsync(dth.p, theta + 2 * b0, sizeof(double) * 2 * nb, hipMemcpyHostToDevice, c->stream));
+        if (precision == MCMB_FP64) MCMB_TRY(project_fwd_batch<double>(c, dth.as<double>(), nb, dout.as<double>()));
+        else MCMB_TRY(project_fwd_batch<float>(c, dth.as<double>(), nb, dout.as<double>()));
+        MCMB_HIP(hipMemcpy(out.data(), dout.p, sizeof(double) * (size_t)nb * 3 * n, hipMemcpyDeviceToHost));
+        for (int b = 0; b < nb; ++b) {
+            std::copy(out.begin() + (size_t)b * 3 * n, out.begin() + (size_t)b * 3 * n + n, alpha + (size_t)(b0 + b) * n);
+            if (dalpha) {
+                // [b][0][n] = d/domega, [b][1][n] = d/dphi
+                std::copy(out.begin() + (size_t)b * 3 * n + n, out.begin() + (size_t)b * 3 * n + 2 * n,
+                          dalpha + (size_t)(b0 + b) * 2 * n);
+                std::copy(out.begin() + (size_t)b * 3 * n + 2 * n, out.begin() + (size_t)b * 3 * n + 3 * n,
+                          dalpha + (size_t)(b0 + b) * 2 * n + n);
+            }
+        }
+    }
+    MCMB_HIP(hipEventRecord(c->ev1, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = (ntheta + PBATCH - 1) / PBATCH;
+    return MCMB_OK;
+}
+
+int mcmb_project_bwd(mcmb_ctx* c, int precision, int ntheta, const double* theta, const double* gbar, double* dtheta)
+{
+    if (!c || !theta || !gbar || !dtheta || ntheta < 1) return fail(MCMB_ERR_ARG, "mcmb_project_bwd: bad argument");
+    if (!c->ks.ready) return fail(MCMB_ERR_STATE, "mcmb_project_bwd: call mcmb_kspace_set first");
+    if (precision != MCMB_FP64 && precision != MCMB_FP32) return fail(MCMB_ERR_ARG, "mcmb_project_bwd: precision %d", precision);
+    MCMB_TRY(set_device(c));
+    PointSet& K = c->ks;
+    const int n = K.n, R = K.R;
+    MCMB_TRY(ensure_W(c, precision));
+    // slot[r][s][t] = mode index of the canonical triple (r,s,t) = order[m]
+    std::vector<int> slot((size_t)R * R * R, -1);
+    for (int m = 0; m < n; ++m) {
+        int r = K.order[3 * m], s = K.order[3 * m + 1], t = K.order[3 * m + 2];
+        int& sl = slot[((size_t)r * R + s) * R + t];
+        if (sl >= 0) return fail(MCMB_ERR_ORDER, "mcmb_project_bwd: repeated mode (%d,%d,%d)", r, s, t);
+        sl = m;
+    }
+    DevBuf dslot, dth, dg, dd;
+    MCMB_TRY(dslot.alloc(sizeof(int) * slot.size()));
+    MCMB_TRY(dth.alloc(sizeof(double) * 2 * PBATCH));
+    MCMB_TRY(dg.alloc(sizeof(double) * (size_t)PBATCH * n));
+    MCMB_TRY(dd.alloc(sizeof(double) * 2 * PBATCH));
+    MCMB_HIP(hipMemcpyAsync(dslot.p, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
+    for (int b0 = 0; b0 < ntheta; b0 += PBATCH) {
+        const int nb = std::min(PBATCH, ntheta - b0);
+        MCMB_HIP(hipMemcpyAsync(dth.p, theta + 2 * b0, sizeof(double) * 2 * nb, hipMemcpyHostToDevice, c->stream));
+        MCMB_HIP(hipMemcpyAsync(dg.p, gbar + (size_t)b0 * n, sizeof(double) * (size_t)nb * n, hipMemcpyHostToDevice, c->stream));
+        if (precision == MCMB_FP64)
+            MCMB_TRY(project_bwd_batch<double>(c, dth.as<double>(), nb, dg.as<double>(), dslot.as<int>(), dd.as<double>()));
+        else
+            MCMB_TRY(project_bwd_batch<float>(c, dth.as<double>(), nb, dg.as<double>(), dslot.as<int>(), dd.as<double>()));
+        MCMB_HIP(hipMemcpy(dtheta + 2 * b0, dd.p, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost));
+    }
+    MCMB_HIP(hipEventRecord(c->ev1, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = (ntheta + PBATCH - 1) / PBATCH;
+    return MCMB_OK;
+}
+
+}  // extern "C"

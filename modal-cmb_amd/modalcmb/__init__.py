@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libmodalcmb.so")
 
 FP64 = 0
 MIXED = 1
+FP32 = 2
 H2_EXACT = 0
 H2_ASYMPTOTIC = 1
 
@@ -69,6 +70,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         for name in ("get_ijk_next_XXX", "get_ijk_next_prim"):
             getattr(lib, name).argtypes = [ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip)]
         lib.mcmb_gauss_legendre.argtypes = [ip, vp, vp]
+        lib.mcmb_kspace_set.argtypes = [vp, ip, vp, vp, ip, vp, ip, vp]
+        lib.mcmb_kspace_npoints.argtypes = [vp]
+        lib.mcmb_kspace_npoints.restype = C.c_long
+        lib.mcmb_kgram.argtypes = [vp, ip, vp, vp]
+        lib.mcmb_project_fwd.argtypes = [vp, ip, ip, vp, vp, vp]
+        lib.mcmb_project_bwd.argtypes = [vp, ip, ip, vp, vp, vp]
         lib.permsix.restype = dp
         lib.permsix.argtypes = [ip, ip, ip]
         lib.calculate_geometric.restype = dp
@@ -82,7 +89,8 @@ EXPORTED_SYMBOLS = (
     "mcmb_lspace_set", "mcmb_gamma_tet", "mcmb_gamma_gl", "mcmb_lspace_npoints",
     "mcmb_lspace_points", "mcmb_lspace_weights", "mcmb_orthonormalise", "mcmb_cholesky_solve",
     "mcmb_last_kernel_ms", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
-    "mcmb_gauss_legendre", "mcmb_gamma_tet_slice",
+    "mcmb_gauss_legendre", "mcmb_gamma_tet_slice", "mcmb_kspace_set", "mcmb_kspace_npoints",
+    "mcmb_kgram", "mcmb_project_fwd", "mcmb_project_bwd",
     "get_ijk_next_XXX", "check_ijk_min_XXX", "check_ijk_max_XXX", "decompose_tetrapyd_XXX",
     "get_ijk_next_prim", "check_ijk_min_prim", "check_ijk_max_prim", "decompose_tetrapyd_prim",
     "divide_tasks", "get_next_ijk_bound", "permsix", "calculate_geometric",
@@ -194,6 +202,44 @@ class Context:
 
     def sync(self) -> None:
         _check(_lib.mcmb_sync(self._h))
+
+    # ---- k-space Gram and projection ---------------------------------------
+    def kspace_set(self, kn: np.ndarray, kw: np.ndarray, q: np.ndarray, order: np.ndarray) -> None:
+        kn = np.ascontiguousarray(kn, dtype=np.float64)
+        kw = np.ascontiguousarray(kw, dtype=np.float64)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        order = np.ascontiguousarray(order, dtype=np.int32).reshape(-1, 3)
+        self._kkeep = (kn, kw, q, order)
+        _check(_lib.mcmb_kspace_set(self._h, kn.shape[0], _ptr(kn), _ptr(kw), q.shape[1], _ptr(q),
+                                    order.shape[0], _ptr(order)))
+        self.nk = order.shape[0]
+
+    def kspace_npoints(self) -> int:
+        r = _lib.mcmb_kspace_npoints(self._h)
+        if r < 0:
+            _check(int(r))
+        return int(r)
+
+    def kgram(self, precision: int = FP64) -> np.ndarray:
+        g = np.empty((self.nk, self.nk), dtype=np.float64)
+        _check(_lib.mcmb_kgram(self._h, precision, _ptr(g), None))
+        return g
+
+    def project_fwd(self, theta: np.ndarray, precision: int = FP64, derivatives: bool = True):
+        """alpha[b][n] and (if derivatives) dalpha[b][2][n] (d/domega, d/dphi)."""
+        th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64).reshape(-1, 2))
+        nt = th.shape[0]
+        a = np.empty((nt, self.nk))
+        d = np.empty((nt, 2, self.nk)) if derivatives else None
+        _check(_lib.mcmb_project_fwd(self._h, precision, nt, _ptr(th), _ptr(a), _ptr(d)))
+        return (a, d) if derivatives else a
+
+    def project_bwd(self, theta: np.ndarray, gbar: np.ndarray, precision: int = FP64) -> np.ndarray:
+        th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64).reshape(-1, 2))
+        g = np.ascontiguousarray(np.asarray(gbar, dtype=np.float64).reshape(th.shape[0], self.nk))
+        out = np.empty((th.shape[0], 2))
+        _check(_lib.mcmb_project_bwd(self._h, precision, th.shape[0], _ptr(th), _ptr(g), _ptr(out)))
+        return out
 
     # ---- orthonormalisation --------------------------------------------------
     def orthonormalise(self, gamma: np.ndarray):

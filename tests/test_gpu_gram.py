@@ -130,3 +130,27 @@ def test_legacy_gamma_3d_host_ranks(ctx):
         tot += buf
     assert rel_elem(tot, G) < 1e-13
     lib.mcmb_legacy_bind(None)
+
+
+@pytest.mark.parametrize("L,R,mod", [(60, 4, False), (150, 6, True), (500, 10, False)])
+def test_gamma_gl_gpu_vs_oracle(ctx, L, R, mod):
+    ws = W.lspace(L, R, mod)
+    ctx.lspace_set(2, L, ws["q"], ws["v"], ws["order"])
+    G = ctx.gamma_gl()
+    ref = O.gamma_gl(2, L, ws["q"], ws["v"], ws["order"])
+    assert rel_elem(G, ref) < 1e-12
+    assert np.array_equal(G, G.T)
+
+
+def test_gamma_gl_equals_tetrapyd_on_gpu(ctx):
+    # two independent GPU evaluations of the same sum at config 1 size
+    ws = W.lspace(500, 10, True)
+    ctx.lspace_set(2, 500, ws["q"], ws["v"], ws["order"])
+    assert rel_elem(ctx.gamma_tet(modalcmb.FP64), ctx.gamma_gl()) < 1e-11
+
+
+def test_gamma_gl_rejects_inexact_node_count(ctx):
+    ws = W.lspace(100, 3)
+    ctx.lspace_set(2, 100, ws["q"], ws["v"], ws["order"])
+    with pytest.raises(modalcmb.ModalError):
+        ctx.gamma_gl(npt=100)

@@ -64,7 +64,7 @@ int mcmb_sync(mcmb_ctx* ctx);
 /* Load the 1D basis q[l][0..ldq) for l = 0..lmax (row l is multipole l; only
  * l in [lmin,lmax] is read), per-leg weight v[l] (= 1/C_l for the config
  * weight h^2/(C_l1 C_l2 C_l3)), and the mode order table order[n][3]
- * (r,s,t < ldq).  Replaces the state create_basis_late / set_terms_late
+ * (r,s,t < ldq, ldq <= 32).  Replaces the state create_basis_late / set_terms_late
  * leave behind (fixed_data.c:272-1986).  Tables are copied to the device. */
 int mcmb_lspace_set(mcmb_ctx* ctx, int lmin, int lmax, int ldq, const double* q,
                     const double* v, int nmodes, const int* order, int h2_mode);

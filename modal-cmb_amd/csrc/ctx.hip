@@ -160,7 +160,7 @@ int mcmb_lspace_set(mcmb_ctx* c, int lmin, int lmax, int ldq, const double* q, c
     if (!c || !q || !v || !order) return fail(MCMB_ERR_ARG, "mcmb_lspace_set: null argument");
     if (lmin < 0 || lmax < lmin + 1 || lmax > 8190)
         return fail(MCMB_ERR_ARG, "mcmb_lspace_set: need 0 <= lmin < lmax <= 8190 (got %d, %d)", lmin, lmax);
-    if (ldq < 1 || ldq > 64) return fail(MCMB_ERR_ARG, "mcmb_lspace_set: ldq %d not in [1,64]", ldq);
+    if (ldq < 1 || ldq > 32) return fail(MCMB_ERR_ARG, "mcmb_lspace_set: ldq %d not in [1,32]", ldq);
     if (nmodes < 1 || nmodes > 65536) return fail(MCMB_ERR_ARG, "mcmb_lspace_set: nmodes %d", nmodes);
     if (h2_mode != MCMB_H2_EXACT && h2_mode != MCMB_H2_ASYMPTOTIC)
         return fail(MCMB_ERR_ARG, "mcmb_lspace_set: h2_mode %d", h2_mode);

@@ -13,9 +13,9 @@
 // positive bias sum_p lo_pm^2 on it, so diagonal tiles also sum lo^2 per mode
 // in FP64 (VALU) and the reduction adds it to Gamma_mm.
 //
-// Workgroup: 4 waves (one per SIMD, 512 registers each), 2 x 2 grid of
-// 128 x 128 wave sub-tiles of a 256 x 256 mode tile; each thread forms all 32
-// points of one mode of each operand.  Diagonal and off-diagonal tiles are
+// Workgroup: 8 waves (two per SIMD, so one wave's operand formation overlaps
+// the other's MFMA work), 2 x 4 grid of 128 x 64 wave sub-tiles of a 256 x 256
+// mode tile; thread t forms 16 of the 32 points of mode t % 256 of each operand.  Diagonal and off-diagonal tiles are
 // separate instantiations (the diagonal one forms a single operand).
 #include "mcmb_internal.h"
 #include <algorithm>
@@ -26,7 +26,18 @@ namespace mcmb {
 
 constexpr int MT = 256;          // mode tile
 constexpr int MKC = 32;          // points per chunk (two K=16 MFMA steps)
-constexpr int MNT = 256;         // threads (4 waves)
+#ifdef MCMB_MIXED_4W
+constexpr int MNT = 256;         // threads (4 waves, one per SIMD)
+constexpr int WN_T = 128;        // wave sub-tile columns
+#else
+constexpr int MNT = 512;         // threads (8 waves, two per SIMD)
+constexpr int WN_T = 64;         // wave sub-tile columns
+#endif
+constexpr int NPIECE = 4 * 256 / MNT;   // 16-byte point chunks formed per thread and operand
+constexpr int NB_C = WN_T / 32;         // 32-wide MFMA blocks per wave row
+// waves tile the 256 x 256 mode tile as a (MT/128) x (MT/WN_T) grid of 128 x WN_T sub-tiles
+static_assert((MNT / 64) == (MT / 128) * (MT / WN_T), "wave grid must cover the mode tile exactly");
+static_assert(NPIECE * (MNT / MT) == 4, "threads per mode must cover the 4 point pieces");
 constexpr int SEGCH_DEFAULT = 2048;  // chunks per segment (FP32 accumulation span)
 constexpr int QP = 32;           // padded basis width
 
@@ -47,7 +58,7 @@ struct MixedSmem {
 // Global-load half of the staging of chunk (rd, c0); branch-free (addresses
 // clamped into the row, invalid entries selected to zero).
 struct StageRegs {
-    float q[4], x;
+    float q[1024 / MNT], x;
 };
 
 __device__ __forceinline__ StageRegs stage_load(const RowDesc& rd, int c0, long pbase, int kstep,
@@ -57,7 +68,7 @@ __device__ __forceinline__ StageRegs stage_load(const RowDesc& rd, int c0, long 
     StageRegs g;
     const int last = rd.nk - 1;
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 1024 / MNT; ++u) {
         const int e = tid + u * MNT, c = e / MKC, kk = e % MKC, t = c0 + kk;
         const int tc = t < last ? t : last;
         const float v = qf[(long)(rd.k0 + kstep * tc) * QP + c];
@@ -76,13 +87,14 @@ __device__ __forceinline__ StageRegs stage_load(const RowDesc& rd, int c0, long 
 __device__ __forceinline__ void stage_store(MixedSmem& S, int b, const StageRegs& g, int tid)
 {
 #pragma unroll
-    for (int u = 0; u < 4; ++u) (&S.qT[b][0][0])[tid + u * MNT] = g.q[u];
+    for (int u = 0; u < 1024 / MNT; ++u) (&S.qT[b][0][0])[tid + u * MNT] = g.q[u];
     if (tid < 96) S.stg[b][tid] = g.x;
 }
 
-// Form the 32 points of mode (r,s,t) into hi/lo row m; returns sum lo^2.
+// Form points 8 c_lo .. 8 (c_lo + NPIECE) - 1 of mode (r,s,t) into hi/lo row m;
+// returns sum lo^2.
 __device__ __forceinline__ float form_mixed(const MixedSmem& S, int b, int r, int s, int t, bool ok, int m,
-                                            bf16x8* __restrict__ hi, bf16x8* __restrict__ lo)
+                                            int c_lo, bf16x8* __restrict__ hi, bf16x8* __restrict__ lo)
 {
     const float* qi = &S.stg[b][0];
     const float* qj = &S.stg[b][32];
@@ -98,7 +110,7 @@ __device__ __forceinline__ float form_mixed(const MixedSmem& S, int b, int r, in
     const f32x4* wv = reinterpret_cast<const f32x4*>(&S.stg[b][64]);
     float lo2 = 0.f;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {   // 16-byte chunk c = points 8c..8c+7
+    for (int c = c_lo; c < c_lo + NPIECE; ++c) {   // 16-byte chunk c = points 8c..8c+7
         // bound the LDS reads the scheduler hoists (each piece is 32 floats)
         __builtin_amdgcn_sched_barrier(0);
         bf16x8 h8, l8;
@@ -136,29 +148,30 @@ __global__ __launch_bounds__(MNT, 1) void gram_tet_mixed_kernel(
     const int tile = blockIdx.x % ntiles;
     const int2 IJ = tiles[tile];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / (MT / WN_T) , wn = wave % (MT / WN_T);
+    const int ml = tid & (MT - 1), c_lo = (tid >> 8) * NPIECE;
 
-    const int mAg = IJ.x * MT + tid, mBg = IJ.y * MT + tid;
+    const int mAg = IJ.x * MT + ml, mBg = IJ.y * MT + ml;
     const bool okA = mAg < n, okB = mBg < n;
     const int pkA = okA ? modes[mAg] : 0, pkB = okB ? modes[mBg] : 0;
     const int rA = pkA & 255, sA = (pkA >> 8) & 255, tA = (pkA >> 16) & 255;
     const int rB = pkB & 255, sB = (pkB >> 8) & 255, tB = (pkB >> 16) & 255;
 
-    f32x16 acc[4][4];
+    f32x16 acc[4][NB_C];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b)
+        for (int b = 0; b < NB_C; ++b)
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[a][b][v] = 0.f;
 
     // this wave's corner of the FP64 Gram accumulator
-    double* const Pw = gacc + (size_t)(IJ.x * MT + wm * 128 + 4 * (lane >> 5)) * n_pad + IJ.y * MT + wn * 128 + (lane & 31);
+    double* const Pw = gacc + (size_t)(IJ.x * MT + wm * 128 + 4 * (lane >> 5)) * n_pad + IJ.y * MT + wn * WN_T + (lane & 31);
     auto flush = [&]() {
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
+            for (int c = 0; c < NB_C; ++c) {
                 double* pb = Pw + (size_t)(a * 32) * n_pad + c * 32;
 #pragma unroll
                 for (int v = 0; v < 16; ++v)
@@ -183,8 +196,8 @@ __global__ __launch_bounds__(MNT, 1) void gram_tet_mixed_kernel(
         stage_next(g);
         stage_store(S, 0, g, tid);
         __syncthreads();
-        lo2A += form_mixed(S, 0, rA, sA, tA, okA, tid, S.op[0][0], S.op[0][1]);
-        if (!DIAG) form_mixed(S, 0, rB, sB, tB, okB, tid, S.op[0][2], S.op[0][3]);
+        lo2A += form_mixed(S, 0, rA, sA, tA, okA, ml, c_lo, S.op[0][0], S.op[0][1]);
+        if (!DIAG) form_mixed(S, 0, rB, sB, tB, okB, ml, c_lo, S.op[0][2], S.op[0][3]);
         if (nch > 1) {
             stage_next(g);
             stage_store(S, 1, g, tid);
@@ -203,7 +216,7 @@ __global__ __launch_bounds__(MNT, 1) void gram_tet_mixed_kernel(
                 // all 8 operand fragments of this K step once (16 ds_read_b128),
                 // then 48 MFMAs
                 const int cidx = ks * 2 + (lane >> 5);
-                bf16x8 ah[4], al[4], bh[4], bl[4];
+                bf16x8 ah[4], al[4], bh[NB_C], bl[NB_C];
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
                     const int ma = wm * 128 + a * 32 + (lane & 31);
@@ -211,13 +224,13 @@ __global__ __launch_bounds__(MNT, 1) void gram_tet_mixed_kernel(
                     al[a] = Al[swz(ma, cidx)];
                 }
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    const int mb = wn * 128 + c * 32 + (lane & 31);
+                for (int c = 0; c < NB_C; ++c) {
+                    const int mb = wn * WN_T + c * 32 + (lane & 31);
                     bh[c] = Bh[swz(mb, cidx)];
                     bl[c] = Bl[swz(mb, cidx)];
                 }
 #pragma unroll
-                for (int c = 0; c < 4; ++c)
+                for (int c = 0; c < NB_C; ++c)
 #pragma unroll
                     for (int a = 0; a < 4; ++a) {
                         acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
@@ -225,16 +238,20 @@ __global__ __launch_bounds__(MNT, 1) void gram_tet_mixed_kernel(
                         acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
                     }
             }
-            if (ch + 1 < nch) {
-                lo2A += form_mixed(S, b ^ 1, rA, sA, tA, okA, tid, S.op[b ^ 1][0], S.op[b ^ 1][1]);
-                if (!DIAG) form_mixed(S, b ^ 1, rB, sB, tB, okB, tid, S.op[b ^ 1][2], S.op[b ^ 1][3]);
+            // form chunk ch+1 (unconditionally, so that it shares the basic block
+            // with the MFMAs above; after the last chunk the result is unused)
+            {
+                const float l2 = form_mixed(S, b ^ 1, rA, sA, tA, okA, ml, c_lo, S.op[b ^ 1][0], S.op[b ^ 1][1]);
+                lo2A += (ch + 1 < nch) ? l2 : 0.f;
+                if (!DIAG) form_mixed(S, b ^ 1, rB, sB, tB, okB, ml, c_lo, S.op[b ^ 1][2], S.op[b ^ 1][3]);
             }
+
             if (more2) stage_store(S, b, g, tid);
             __syncthreads();
         }
     }
     flush();
-    if (DIAG && okA) unsafeAtomicAdd(corr + mAg, lo2A);
+    if (DIAG && okA) unsafeAtomicAdd(corr + mAg, lo2A);   // both point halves add
 }
 
 // Gamma[m][c] = Gamma[c][m] = acc[m][c] (m <= c), + corr[m] on the diagonal.

@@ -133,32 +133,67 @@ __device__ __forceinline__ ModeInfo load_mode(const int* __restrict__ modes, int
     return mi;
 }
 
-// Form KC/2 operand values of one mode for one chunk into LDS column mloc.
-__device__ __forceinline__ void form_f64(const RowDesc& rd, int c0, long pbase, int kstep,
-                                         const ModeInfo& mi, const double* __restrict__ q, int ldq,
-                                         const double* __restrict__ ws, double* __restrict__ buf,
+constexpr int QW = 32;   // LDS basis width (ldq <= 32)
+
+// Per-chunk basis staging (LDS): q_c(k) for the KC points [c][kk], q(i), q(j)
+// and sqrt(w)/6.  Loaded to registers one chunk ahead (before the MFMA work of
+// the current chunk) and stored to LDS after it; branch-free.
+struct F64Stage {
+    double qk[QW][KC];
+    double qi[QW], qj[QW], w[KC];
+};
+struct F64Regs {
+    double q[2];
+    double x;
+};
+
+__device__ __forceinline__ F64Regs f64_stage_load(const RowDesc& rd, int c0, long pbase, int kstep,
+                                                  const double* __restrict__ q, int ldq,
+                                                  const double* __restrict__ ws, int tid)
+{
+    F64Regs g;
+    const int last = rd.nk - 1;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {             // 512 = QW x KC entries, 2 per thread
+        const int e = tid + u * NT, c = e / KC, kk = e % KC, t = c0 + kk;
+        const int tc = t < last ? t : last, cc = c < ldq ? c : 0;
+        const double v = q[(long)(rd.k0 + kstep * tc) * ldq + cc];
+        g.q[u] = (c < ldq && t <= last) ? v : 0.0;
+    }
+    // tid < 32: q(i)[tid]; 32..63: q(j)[tid-32]; 64..79: w of point c0 + tid-64
+    const int c = tid & 31, cc = c < ldq ? c : 0;
+    const double qv = q[(long)((tid < 32) ? rd.i : rd.j) * ldq + cc];
+    const int kk = tid - 64, t = c0 + (kk < 0 ? 0 : kk);
+    const double wv = ws[pbase + ((t <= last) ? (kk < 0 ? 0 : kk) : last - c0)];
+    g.x = (tid < 64) ? (c < ldq ? qv : 0.0) : ((kk < KC && t <= last) ? wv : 0.0);
+    return g;
+}
+
+__device__ __forceinline__ void f64_stage_store(F64Stage& S, const F64Regs& g, int tid)
+{
+#pragma unroll
+    for (int u = 0; u < 2; ++u) (&S.qk[0][0])[tid + u * NT] = g.q[u];
+    if (tid < 32) S.qi[tid] = g.x;
+    else if (tid < 64) S.qj[tid - 32] = g.x;
+    else if (tid < 64 + KC) S.w[tid - 64] = g.x;
+}
+
+// Form KC/2 operand values of one mode from the staged chunk into column mloc.
+__device__ __forceinline__ void form_f64(const F64Stage& S, const ModeInfo& mi, double* __restrict__ buf,
                                          int mloc, int kb)
 {
-    const double* qi = q + (long)rd.i * ldq;
-    const double* qj = q + (long)rd.j * ldq;
     double Srs = 0.0, Srt = 0.0, Sst = 0.0;
     if (mi.ok) {
-        double ir = qi[mi.r], is = qi[mi.s], it = qi[mi.t];
-        double jr = qj[mi.r], js = qj[mi.s], jt = qj[mi.t];
+        const double ir = S.qi[mi.r], is = S.qi[mi.s], it = S.qi[mi.t];
+        const double jr = S.qj[mi.r], js = S.qj[mi.s], jt = S.qj[mi.t];
         Srs = ir * js + is * jr;
         Srt = ir * jt + it * jr;
         Sst = is * jt + it * js;
     }
 #pragma unroll
     for (int e = 0; e < KC / 2; ++e) {
-        int kk = kb + e;
-        int t = c0 + kk;
-        double a = 0.0;
-        if (t < rd.nk) {
-            const double* qk = q + (long)(rd.k0 + kstep * t) * ldq;
-            a = ws[pbase + kk] * (Srs * qk[mi.t] + Srt * qk[mi.s] + Sst * qk[mi.r]);
-        }
-        buf[kk * LDA + mloc] = a;
+        const int kk = kb + e;
+        buf[kk * LDA + mloc] = S.w[kk] * (Srs * S.qk[mi.t][kk] + Srt * S.qk[mi.s][kk] + Sst * S.qk[mi.r][kk]);
     }
 }
 
@@ -170,6 +205,7 @@ __global__ __launch_bounds__(NT, 2) void gram_tet_f64_kernel(
 {
     __shared__ double sA[2][KC * LDA];
     __shared__ double sB[2][KC * LDA];
+    __shared__ F64Stage stg[2];
     const int tile = blockIdx.x % ntiles;
     const int seg = blockIdx.x / ntiles;
     const int2 IJ = tiles[tile];
@@ -186,29 +222,34 @@ __global__ __launch_bounds__(NT, 2) void gram_tet_f64_kernel(
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = d4{0.0, 0.0, 0.0, 0.0};
 
-    int row = seg_rows[seg];
-    const int row_end = seg_rows[seg + 1];
-    int c0 = 0;
-    if (row < row_end) {
-        RowDesc rd = rows[row];
-        long pb = rowptr[row];
-        form_f64(rd, 0, pb, kstep, mA, q, ldq, ws, sA[0], mloc, kb);
-        if (!diag) form_f64(rd, 0, pb, kstep, mB, q, ldq, ws, sB[0], mloc, kb);
+    const int row_begin = seg_rows[seg], row_end = seg_rows[seg + 1];
+    long nch = 0;
+    for (int r = row_begin; r < row_end; ++r) nch += (rows[r].nk + KC - 1) / KC;
+    int prow = row_begin, pc0 = 0;   // next chunk to stage
+    auto stage_next = [&](F64Regs& g) {
+        const RowDesc rd = rows[prow];
+        g = f64_stage_load(rd, pc0, rowptr[prow] + pc0, kstep, q, ldq, ws, tid);
+        pc0 += KC;
+        if (pc0 >= rd.nk) { ++prow; pc0 = 0; }
+    };
+    if (nch > 0) {
+        F64Regs g;
+        stage_next(g);
+        f64_stage_store(stg[0], g, tid);
         __syncthreads();
-        int cur = 0;
-        while (true) {
-            // advance to the next chunk (uniform over the workgroup)
-            int nrow = row, nc0 = c0 + KC;
-            if (nc0 >= rows[row].nk) { ++nrow; nc0 = 0; }
-            const bool has_next = nrow < row_end;
-            if (has_next) {
-                RowDesc nd = rows[nrow];
-                long npb = rowptr[nrow] + nc0;
-                form_f64(nd, nc0, npb, kstep, mA, q, ldq, ws, sA[cur ^ 1], mloc, kb);
-                if (!diag) form_f64(nd, nc0, npb, kstep, mB, q, ldq, ws, sB[cur ^ 1], mloc, kb);
-            }
-            const double* A = sA[cur];
-            const double* B = diag ? sA[cur] : sB[cur];
+        form_f64(stg[0], mA, sA[0], mloc, kb);
+        if (!diag) form_f64(stg[0], mB, sB[0], mloc, kb);
+        if (nch > 1) {
+            stage_next(g);
+            f64_stage_store(stg[1], g, tid);
+        }
+        __syncthreads();
+        for (long ch = 0; ch < nch; ++ch) {
+            const int b = (int)(ch & 1);
+            const bool more2 = ch + 2 < nch;
+            if (more2) stage_next(g);
+            const double* A = sA[b];
+            const double* B = diag ? sA[b] : sB[b];
 #pragma unroll
             for (int ks = 0; ks < KC / 4; ++ks) {
                 const int kk = ks * 4 + (lane >> 4);
@@ -223,11 +264,12 @@ __global__ __launch_bounds__(NT, 2) void gram_tet_f64_kernel(
                     for (int bc = 0; bc < 4; ++bc)
                         acc[br][bc] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[br], bv[bc], acc[br][bc], 0, 0, 0);
             }
+            if (ch + 1 < nch) {
+                form_f64(stg[b ^ 1], mA, sA[b ^ 1], mloc, kb);
+                if (!diag) form_f64(stg[b ^ 1], mB, sB[b ^ 1], mloc, kb);
+            }
+            if (more2) f64_stage_store(stg[b], g, tid);
             __syncthreads();
-            if (!has_next) break;
-            row = nrow;
-            c0 = nc0;
-            cur ^= 1;
         }
     }
     // D layout of v_mfma_f64_16x16x4: lane holds rows 4v + lane/16, column lane%16.

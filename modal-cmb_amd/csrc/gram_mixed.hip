@@ -26,33 +26,50 @@ namespace mcmb {
 
 constexpr int MT = 256;          // mode tile
 constexpr int MKC = 32;          // points per chunk (two K=16 MFMA steps)
-#ifdef MCMB_MIXED_4W
-constexpr int MNT = 256;         // threads (4 waves, one per SIMD)
-constexpr int WN_T = 128;        // wave sub-tile columns
-#else
 constexpr int MNT = 512;         // threads (8 waves, two per SIMD)
 constexpr int WN_T = 64;         // wave sub-tile columns
-#endif
 constexpr int NPIECE = 4 * 256 / MNT;   // 16-byte point chunks formed per thread and operand
 constexpr int NB_C = WN_T / 32;         // 32-wide MFMA blocks per wave row
 // waves tile the 256 x 256 mode tile as a (MT/128) x (MT/WN_T) grid of 128 x WN_T sub-tiles
 static_assert((MNT / 64) == (MT / 128) * (MT / WN_T), "wave grid must cover the mode tile exactly");
 static_assert(NPIECE * (MNT / MT) == 4, "threads per mode must cover the 4 point pieces");
+static_assert(NPIECE == 2 && NB_C == 2, "the chunk schedule below interleaves 4 formation units with 4 MFMA groups");
 constexpr int SEGCH_DEFAULT = 2048;  // chunks per segment (FP32 accumulation span)
 constexpr int QP = 32;           // padded basis width
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+#define MCMB_SGB_1 __builtin_amdgcn_sched_group_barrier(0x002, 5, 0); __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+#define MCMB_SGB_VALU_MFMA MCMB_SGB_1 MCMB_SGB_1 MCMB_SGB_1 MCMB_SGB_1 MCMB_SGB_1 \
+                           MCMB_SGB_1 MCMB_SGB_1 MCMB_SGB_1 MCMB_SGB_1 MCMB_SGB_1
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// 16-byte chunk c (0..3) of operand row m, XOR-swizzled so that the 8 rows a
-// ds_read_b128 phase touches land on distinct banks.
+// 16-byte chunk c (0..3) of operand row m, XOR-swizzled so that the rows one
+// ds_read_b128 / ds_write_b128 pass touches land on distinct banks.
+#ifdef MCMB_SWZ8
 __device__ __forceinline__ int swz(int m, int c) { return (m << 2) + (c ^ ((m >> 1) & 3)); }
+#else
+// conflict-free both for 8-lane passes over 32 banks (rows m..m+7) and for
+// 16-lane passes over 64 banks (rows m..m+15): the XOR key takes four
+// distinct values over bits (1,2) of m for fixed bit 3, and over bits (2,3)
+// for fixed bit 1.
+__device__ __forceinline__ int swz(int m, int c)
+{
+    const int key = ((m >> 2) & 1) | ((((m >> 1) ^ (m >> 3)) & 1) << 1);
+    return (m << 2) + (c ^ key);
+}
+#endif
+
+// Row stride of the staged chunk basis: 36 floats puts the rows of eight
+// consecutive third indices t on eight distinct 4-bank groups, so the
+// ds_read_b128 of qT[t] by eight lanes with different t is conflict-free.
+constexpr int QTS = MKC + 4;
 
 struct MixedSmem {
     bf16x8 op[2][4][MT * 4];   // [buf][A_hi, A_lo, B_hi, B_lo][row*4 + chunk]
-    float qT[2][QP][MKC];      // chunk basis q_c(k_kk), [c][kk]
-    float stg[2][3 * 32];      // q(i)[32], q(j)[32], sqrt(w)/6 [32]
+    float qT[2][QP][QTS];      // weighted chunk basis sqrt(w_kk)/6 q_c(k_kk), [c][kk]
+    float stg[2][2 * 32];      // q(i)[32], q(j)[32]
 };
 
 // Global-load half of the staging of chunk (rd, c0); branch-free (addresses
@@ -72,66 +89,115 @@ __device__ __forceinline__ StageRegs stage_load(const RowDesc& rd, int c0, long 
         const int e = tid + u * MNT, c = e / MKC, kk = e % MKC, t = c0 + kk;
         const int tc = t < last ? t : last;
         const float v = qf[(long)(rd.k0 + kstep * tc) * QP + c];
-        g.q[u] = (c < R && t <= last) ? v : 0.f;
+        const float w = (float)ws[pbase + (tc - c0)];
+        g.q[u] = (c < R && t <= last) ? w * v : 0.f;
     }
-    // tid 0..31: q(i)[c]; 32..63: q(j)[c]; 64..95: w of point c0 + c
+    // tid 0..31: q(i)[c]; 32..63: q(j)[c]
     const int c = tid & 31;
     const int l = (tid < 32) ? rd.i : rd.j;
     const float qv = qf[(long)l * QP + c];
-    const int t = c0 + c;
-    const double wv = ws[pbase + (t <= last ? c : last - c0)];
-    g.x = (tid < 64) ? (c < R ? qv : 0.f) : ((t <= last) ? (float)wv : 0.f);
+    g.x = c < R ? qv : 0.f;
     return g;
 }
 
 __device__ __forceinline__ void stage_store(MixedSmem& S, int b, const StageRegs& g, int tid)
 {
 #pragma unroll
-    for (int u = 0; u < 1024 / MNT; ++u) (&S.qT[b][0][0])[tid + u * MNT] = g.q[u];
-    if (tid < 96) S.stg[b][tid] = g.x;
+    for (int u = 0; u < 1024 / MNT; ++u) {
+        const int e = tid + u * MNT;
+        S.qT[b][e / MKC][e % MKC] = g.q[u];
+    }
+    if (tid < 64) S.stg[b][tid] = g.x;
 }
 
-// Form points 8 c_lo .. 8 (c_lo + NPIECE) - 1 of mode (r,s,t) into hi/lo row m;
-// returns sum lo^2.
-__device__ __forceinline__ float form_mixed(const MixedSmem& S, int b, int r, int s, int t, bool ok, int m,
-                                            int c_lo, bf16x8* __restrict__ hi, bf16x8* __restrict__ lo)
+// Per-chunk coefficients of one operand row: Q_m on the points of row (i,j)
+// is Srs q_t(k) + Srt q_s(k) + Sst q_r(k) with S_ab = q_a(i) q_b(j) + q_b(i) q_a(j).
+struct ModeCoef {
+    float Srs, Srt, Sst;
+    const float *qt, *qs, *qr;   // weighted chunk basis rows in LDS
+};
+
+__device__ __forceinline__ ModeCoef mode_coef(const MixedSmem& S, int b, int r, int s, int t, bool ok)
 {
     const float* qi = &S.stg[b][0];
     const float* qj = &S.stg[b][32];
-    float ir = qi[r], is = qi[s], it = qi[t];
-    float jr = qj[r], js = qj[s], jt = qj[t];
-    float Srs = ir * js + is * jr;
-    float Srt = ir * jt + it * jr;
-    float Sst = is * jt + it * js;
-    if (!ok) Srs = Srt = Sst = 0.f;
-    const f32x4* qt = reinterpret_cast<const f32x4*>(&S.qT[b][t][0]);
-    const f32x4* qs = reinterpret_cast<const f32x4*>(&S.qT[b][s][0]);
-    const f32x4* qr = reinterpret_cast<const f32x4*>(&S.qT[b][r][0]);
-    const f32x4* wv = reinterpret_cast<const f32x4*>(&S.stg[b][64]);
+    const float ir = qi[r], is = qi[s], it = qi[t];
+    const float jr = qj[r], js = qj[s], jt = qj[t];
+    ModeCoef k;
+    k.Srs = ok ? ir * js + is * jr : 0.f;
+    k.Srt = ok ? ir * jt + it * jr : 0.f;
+    k.Sst = ok ? is * jt + it * js : 0.f;
+    k.qt = &S.qT[b][t][0];
+    k.qs = &S.qT[b][s][0];
+    k.qr = &S.qT[b][r][0];
+    return k;
+}
+
+__device__ __forceinline__ unsigned pack_bf16x2(float x0, float x1)
+{
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    const b2 h = __builtin_convertvector((f2){x0, x1}, b2);   // one v_cvt_pk_bf16_f32, RNE
+    return __builtin_bit_cast(unsigned, h);
+}
+
+// Formation of points 8c .. 8c+7 of one operand row: the LDS reads (issued
+// early, so that MFMAs cover their latency) and the arithmetic + stores.
+struct PieceIn {
+    f32x4 t[2], s[2], r[2];
+};
+
+__device__ __forceinline__ PieceIn piece_load(const ModeCoef& k, int c)
+{
+    PieceIn in;
+#pragma unroll
+    for (int q4 = 0; q4 < 2; ++q4) {
+        in.t[q4] = reinterpret_cast<const f32x4*>(k.qt)[2 * c + q4];
+        in.s[q4] = reinterpret_cast<const f32x4*>(k.qs)[2 * c + q4];
+        in.r[q4] = reinterpret_cast<const f32x4*>(k.qr)[2 * c + q4];
+    }
+    return in;
+}
+
+// Writes the hi and lo parts of the 8 points into row m; returns sum lo^2
+// over the 8 points (as rounded to BF16) when LO2.
+template <bool LO2>
+__device__ __forceinline__ float piece_store(const ModeCoef& k, const PieceIn& in, int c, int m,
+                                             bf16x8* __restrict__ hi, bf16x8* __restrict__ lo)
+{
+    float x[8];
+#pragma unroll
+    for (int q4 = 0; q4 < 2; ++q4)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            x[q4 * 4 + e] = k.Srs * in.t[q4][e] + k.Srt * in.s[q4][e] + k.Sst * in.r[q4][e];
+    unsigned hw[4], lw[4];
     float lo2 = 0.f;
 #pragma unroll
-    for (int c = c_lo; c < c_lo + NPIECE; ++c) {   // 16-byte chunk c = points 8c..8c+7
-        // bound the LDS reads the scheduler hoists (each piece is 32 floats)
-        __builtin_amdgcn_sched_barrier(0);
-        bf16x8 h8, l8;
-#pragma unroll
-        for (int q4 = 0; q4 < 2; ++q4) {
-            const f32x4 a = qt[c * 2 + q4], bq = qs[c * 2 + q4], cq = qr[c * 2 + q4], w4 = wv[c * 2 + q4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const float x = w4[e] * (Srs * a[e] + Srt * bq[e] + Sst * cq[e]);
-                const __bf16 xh = (__bf16)x;
-                const __bf16 xl = (__bf16)(x - (float)xh);
-                const float fl = (float)xl;
-                lo2 += fl * fl;
-                h8[q4 * 4 + e] = xh;
-                l8[q4 * 4 + e] = xl;
-            }
+    for (int p = 0; p < 4; ++p) {
+        const unsigned h = pack_bf16x2(x[2 * p], x[2 * p + 1]);
+        const float h0 = __builtin_bit_cast(float, h << 16);
+        const float h1 = __builtin_bit_cast(float, h & 0xffff0000u);
+        const unsigned l = pack_bf16x2(x[2 * p] - h0, x[2 * p + 1] - h1);
+        hw[p] = h;
+        lw[p] = l;
+        if (LO2) {
+            const float l0 = __builtin_bit_cast(float, l << 16);
+            const float l1 = __builtin_bit_cast(float, l & 0xffff0000u);
+            lo2 += l0 * l0 + l1 * l1;
         }
-        hi[swz(m, c)] = h8;
-        lo[swz(m, c)] = l8;
     }
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    hi[swz(m, c)] = __builtin_bit_cast(bf16x8, (u4){hw[0], hw[1], hw[2], hw[3]});
+    lo[swz(m, c)] = __builtin_bit_cast(bf16x8, (u4){lw[0], lw[1], lw[2], lw[3]});
     return lo2;
+}
+
+template <bool LO2>
+__device__ __forceinline__ float form_piece(const ModeCoef& k, int c, int m, bf16x8* __restrict__ hi,
+                                            bf16x8* __restrict__ lo)
+{
+    return piece_store<LO2>(k, piece_load(k, c), c, m, hi, lo);
 }
 
 template <bool DIAG>
@@ -196,8 +262,16 @@ __global__ __launch_bounds__(MNT, 1) void gram_tet_mixed_kernel(
         stage_next(g);
         stage_store(S, 0, g, tid);
         __syncthreads();
-        lo2A += form_mixed(S, 0, rA, sA, tA, okA, ml, c_lo, S.op[0][0], S.op[0][1]);
-        if (!DIAG) form_mixed(S, 0, rB, sB, tB, okB, ml, c_lo, S.op[0][2], S.op[0][3]);
+        {
+            const ModeCoef kA = mode_coef(S, 0, rA, sA, tA, okA);
+#pragma unroll
+            for (int u = 0; u < NPIECE; ++u) lo2A += form_piece<DIAG>(kA, c_lo + u, ml, S.op[0][0], S.op[0][1]);
+            if (!DIAG) {
+                const ModeCoef kB = mode_coef(S, 0, rB, sB, tB, okB);
+#pragma unroll
+                for (int u = 0; u < NPIECE; ++u) form_piece<false>(kB, c_lo + u, ml, S.op[0][2], S.op[0][3]);
+            }
+        }
         if (nch > 1) {
             stage_next(g);
             stage_store(S, 1, g, tid);
@@ -211,40 +285,68 @@ __global__ __launch_bounds__(MNT, 1) void gram_tet_mixed_kernel(
             const bf16x8* Al = S.op[b][1];
             const bf16x8* Bh = DIAG ? S.op[b][0] : S.op[b][2];
             const bf16x8* Bl = DIAG ? S.op[b][1] : S.op[b][3];
+            // Operand fragments of one K step: A rows wm*128 + a*32 + (lane&31),
+            // B rows wn*64 + c*32 + (lane&31), 16-byte chunk ks*2 + lane/32.
+            bf16x8 ah[4], al[4], bh[NB_C], bl[NB_C];
+            auto ldA = [&](int ks, int a) {
+                const int i = swz(wm * 128 + a * 32 + (lane & 31), ks * 2 + (lane >> 5));
+                ah[a] = Ah[i];
+                al[a] = Al[i];
+            };
+            auto ldB = [&](int ks, int c) {
+                const int i = swz(wn * WN_T + c * 32 + (lane & 31), ks * 2 + (lane >> 5));
+                bh[c] = Bh[i];
+                bl[c] = Bl[i];
+            };
+            auto mfma3 = [&](int a, int c) {
+                acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
+                acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
+                acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
+            };
 #pragma unroll
-            for (int ks = 0; ks < 2; ++ks) {
-                // all 8 operand fragments of this K step once (16 ds_read_b128),
-                // then 48 MFMAs
-                const int cidx = ks * 2 + (lane >> 5);
-                bf16x8 ah[4], al[4], bh[NB_C], bl[NB_C];
+            for (int a = 0; a < 4; ++a) ldA(0, a);
+#pragma unroll
+            for (int c = 0; c < NB_C; ++c) ldB(0, c);
+            // MFMAs on chunk ch in four groups (K step ks, column block c),
+            // each paired with one formation unit of chunk ch+1 into the other
+            // buffer: the unit's LDS reads go first, the MFMAs cover their
+            // latency, then its arithmetic and stores, which the scheduler
+            // interleaves with the MFMAs in flight.  Units: A pieces then B
+            // pieces (off-diagonal); A pieces in the even units (diagonal).
+            // The fragments of K step 1 are loaded in group (0, 1), each
+            // right after the last use of the registers it replaces.  After
+            // the last chunk the formed data is unused.
+            const ModeCoef kA = mode_coef(S, b ^ 1, rA, sA, tA, okA);
+            const ModeCoef kB = mode_coef(S, b ^ 1, rB, sB, tB, okB);
+            float l2 = 0.f;
+#pragma unroll
+            for (int unit = 0; unit < 4; ++unit) {
+                const int ks = unit >> 1, c = unit & 1;
+                const bool formA = DIAG ? (unit & 1) == 0 : unit < NPIECE;
+                const bool formB = !DIAG && unit >= NPIECE;
+                const int piece = c_lo + (DIAG ? unit / 2 : (unit & 1));
+                PieceIn in;
+                if (formA) in = piece_load(kA, piece);
+                if (formB) in = piece_load(kB, piece);
+                if (unit == 1) ldB(1, 0);
 #pragma unroll
                 for (int a = 0; a < 4; ++a) {
-                    const int ma = wm * 128 + a * 32 + (lane & 31);
-                    ah[a] = Ah[swz(ma, cidx)];
-                    al[a] = Al[swz(ma, cidx)];
+                    mfma3(a, c);
+                    if (unit == 1) ldA(1, a);
                 }
-#pragma unroll
-                for (int c = 0; c < NB_C; ++c) {
-                    const int mb = wn * WN_T + c * 32 + (lane & 31);
-                    bh[c] = Bh[swz(mb, cidx)];
-                    bl[c] = Bl[swz(mb, cidx)];
-                }
-#pragma unroll
-                for (int c = 0; c < NB_C; ++c)
-#pragma unroll
-                    for (int a = 0; a < 4; ++a) {
-                        acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bh[c], acc[a][c], 0, 0, 0);
-                        acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[a], bl[c], acc[a][c], 0, 0, 0);
-                        acc[a][c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[a], bh[c], acc[a][c], 0, 0, 0);
-                    }
+                if (unit == 1) ldB(1, 1);
+                if (formA) l2 += piece_store<DIAG>(kA, in, piece, ml, S.op[b ^ 1][0], S.op[b ^ 1][1]);
+                if (formB) piece_store<false>(kB, in, piece, ml, S.op[b ^ 1][2], S.op[b ^ 1][3]);
+#ifndef MCMB_NO_SGB
+                // order within the group: LDS reads, two MFMAs, then one MFMA
+                // per five VALU instructions
+                __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
+                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+                MCMB_SGB_VALU_MFMA __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+#endif
+                __builtin_amdgcn_sched_barrier(0);
             }
-            // form chunk ch+1 (unconditionally, so that it shares the basic block
-            // with the MFMAs above; after the last chunk the result is unused)
-            {
-                const float l2 = form_mixed(S, b ^ 1, rA, sA, tA, okA, ml, c_lo, S.op[b ^ 1][0], S.op[b ^ 1][1]);
-                lo2A += (ch + 1 < nch) ? l2 : 0.f;
-                if (!DIAG) form_mixed(S, b ^ 1, rB, sB, tB, okB, ml, c_lo, S.op[b ^ 1][2], S.op[b ^ 1][3]);
-            }
+            lo2A += (ch + 1 < nch) ? l2 : 0.f;
 
             if (more2) stage_store(S, b, g, tid);
             __syncthreads();

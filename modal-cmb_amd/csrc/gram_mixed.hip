@@ -34,7 +34,12 @@ constexpr int NB_C = WN_T / 32;         // 32-wide MFMA blocks per wave row
 static_assert((MNT / 64) == (MT / 128) * (MT / WN_T), "wave grid must cover the mode tile exactly");
 static_assert(NPIECE * (MNT / MT) == 4, "threads per mode must cover the 4 point pieces");
 static_assert(NPIECE == 2 && NB_C == 2, "the chunk schedule below interleaves 4 formation units with 4 MFMA groups");
-constexpr int SEGCH_DEFAULT = 2048;  // chunks per segment (FP32 accumulation span)
+// Chunks per segment = the FP32 accumulation span (8192 points).  The FP32
+// accumulation error grows about linearly with the span: at lmax=2000, 1540
+// modes the max elementwise Gram error is 1.1e-7 / 3.1e-7 / 2.4e-6 / 5.1e-6
+// at 32 / 256 / 1024 / 2048 chunks, while the time is flat from 256 up and
+// rises below it (more FP64 flushes).
+constexpr int SEGCH_DEFAULT = 256;
 constexpr int QP = 32;           // padded basis width
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -203,14 +208,14 @@ __device__ __forceinline__ float form_piece(const ModeCoef& k, int c, int m, bf1
 template <bool DIAG>
 __global__ __launch_bounds__(MNT, 1) void gram_tet_mixed_kernel(
     const RowDesc* __restrict__ rows, const long* __restrict__ rowptr,
-    const int2* __restrict__ seg_start, long nchunks_total, int segch,
+    const int2* __restrict__ seg_start, long seg0, long nchunks_total, int segch,
     const int2* __restrict__ tiles, int ntiles, const int* __restrict__ modes, int n,
     const float* __restrict__ qf, int R, int kstep, const double* __restrict__ ws,
     double* __restrict__ gacc, double* __restrict__ corr, int n_pad)
 {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     MixedSmem& S = *reinterpret_cast<MixedSmem*>(smem_raw);
-    const long seg = blockIdx.x / ntiles;
+    const long seg = seg0 + blockIdx.x / ntiles;
     const int tile = blockIdx.x % ntiles;
     const int2 IJ = tiles[tile];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -321,7 +326,7 @@ __global__ __launch_bounds__(MNT, 1) void gram_tet_mixed_kernel(
             float l2 = 0.f;
 #pragma unroll
             for (int unit = 0; unit < 4; ++unit) {
-                const int ks = unit >> 1, c = unit & 1;
+                const int c = unit & 1;
                 const bool formA = DIAG ? (unit & 1) == 0 : unit < NPIECE;
                 const bool formB = !DIAG && unit >= NPIECE;
                 const int piece = c_lo + (DIAG ? unit / 2 : (unit & 1));
@@ -390,7 +395,8 @@ int gram_mixed_launch(mcmb_ctx* c, PointSet& L, const long* d_rowptr, const doub
     const long ntiles = (long)T * (T + 1) / 2;
     // segments: segch consecutive chunks in walk order (start position kept).
     // Longer segments cut the FP64 flush traffic (n_pad^2/2 doubles per
-    // segment) at the price of longer FP32 accumulation.
+    // segment) at the price of longer FP32 accumulation; MCMB_MIXED_SEGCH
+    // overrides the length.
     long nchunks_all = 0;
     for (int r = row_lo; r < row_hi; ++r) nchunks_all += (L.rows[r].nk + MKC - 1) / MKC;
     // keep at least ~4 rounds of 256 workgroups on small problems
@@ -435,21 +441,35 @@ int gram_mixed_launch(mcmb_ctx* c, PointSet& L, const long* d_rowptr, const doub
     MCMB_HIP(hipFuncSetAttribute((const void*)gram_tet_mixed_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     MCMB_HIP(hipFuncSetAttribute((const void*)gram_tet_mixed_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
+    // one dispatch holds fewer than 2^32 work-items: launch the segments in
+    // batches of at most max_wg / ntiles segments (MCMB_MIXED_MAX_WG lowers
+    // the cap, so that tests reach the batching at small sizes)
+    long max_wg = ((1ll << 32) - 1) / MNT;
+    if (const char* e = getenv("MCMB_MIXED_MAX_WG")) max_wg = std::max(1l, std::min(max_wg, atol(e)));
     int launches = 0;
-    if (!otiles.empty()) {
-        hipLaunchKernelGGL(gram_tet_mixed_kernel<false>, dim3((unsigned)(otiles.size() * nseg)), dim3(MNT), smem,
-                           c->stream, L.d_rows.as<RowDesc>(), d_rowptr, d_seg.as<int2>(), nchunks, segch,
-                           d_ot.as<int2>(), (int)otiles.size(), L.d_modes.as<int>(), n, L.d_qf.as<float>(), R, L.kstep,
-                           d_ws, c->d_partial.as<double>(), d_corr.as<double>(), n_pad);
-        MCMB_HIP(hipGetLastError());
-        ++launches;
-    }
-    hipLaunchKernelGGL(gram_tet_mixed_kernel<true>, dim3((unsigned)(dtiles.size() * nseg)), dim3(MNT), smem,
-                       c->stream, L.d_rows.as<RowDesc>(), d_rowptr, d_seg.as<int2>(), nchunks, segch,
-                       d_dt.as<int2>(), (int)dtiles.size(), L.d_modes.as<int>(), n, L.d_qf.as<float>(), R, L.kstep,
-                       d_ws, c->d_partial.as<double>(), d_corr.as<double>(), n_pad);
-    MCMB_HIP(hipGetLastError());
-    ++launches;
+    auto launch_all = [&](bool diag, const std::vector<int2>& tl, const int2* d_tl) -> int {
+        const long nt = (long)tl.size();
+        const long per = std::max(1l, max_wg / nt);
+        for (long s0 = 0; s0 < nseg; s0 += per) {
+            const long ns = std::min(per, nseg - s0);
+            MCMB_TRY(launch_guard("mixed Gram", (unsigned long long)(ns * nt), MNT));
+            if (diag)
+                hipLaunchKernelGGL(gram_tet_mixed_kernel<true>, dim3((unsigned)(ns * nt)), dim3(MNT), smem, c->stream,
+                                   L.d_rows.as<RowDesc>(), d_rowptr, d_seg.as<int2>(), s0, nchunks, segch, d_tl,
+                                   (int)nt, L.d_modes.as<int>(), n, L.d_qf.as<float>(), R, L.kstep, d_ws,
+                                   c->d_partial.as<double>(), d_corr.as<double>(), n_pad);
+            else
+                hipLaunchKernelGGL(gram_tet_mixed_kernel<false>, dim3((unsigned)(ns * nt)), dim3(MNT), smem, c->stream,
+                                   L.d_rows.as<RowDesc>(), d_rowptr, d_seg.as<int2>(), s0, nchunks, segch, d_tl,
+                                   (int)nt, L.d_modes.as<int>(), n, L.d_qf.as<float>(), R, L.kstep, d_ws,
+                                   c->d_partial.as<double>(), d_corr.as<double>(), n_pad);
+            MCMB_HIP(hipGetLastError());
+            ++launches;
+        }
+        return MCMB_OK;
+    };
+    if (!otiles.empty()) MCMB_TRY(launch_all(false, otiles, d_ot.as<int2>()));
+    MCMB_TRY(launch_all(true, dtiles, d_dt.as<int2>()));
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
     dim3 rgrid((unsigned)((n + 63) / 64), (unsigned)((n + 3) / 4));
     hipLaunchKernelGGL(gram_mixed_finalize_kernel, rgrid, dim3(256), 0, c->stream, c->d_partial.as<double>(),

@@ -25,6 +25,17 @@ int fail(int status, const char* fmt, ...);
         if (s_ != MCMB_OK) return s_;                                               \
     } while (0)
 
+// The dispatch packet holds the grid size as a 32-bit count of work-items per
+// dimension: a 1D launch of blocks x threads >= 2^32 would wrap silently.
+// Returns MCMB_OK or records the error.
+inline int launch_guard(const char* what, unsigned long long blocks, unsigned threads)
+{
+    if (blocks == 0 || blocks * threads >= (1ull << 32))
+        return fail(MCMB_ERR_ARG, "%s: grid of %llu x %u work-items is outside one dispatch (limit 2^32 - 1)", what,
+                    blocks, threads);
+    return MCMB_OK;
+}
+
 // One row of the ordered tetrapyd: fixed (i, j), k = k0 + kstep * t, t < nk.
 struct RowDesc {
     int i, j, k0, nk;

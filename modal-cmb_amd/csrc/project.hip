@@ -322,6 +322,7 @@ static int ensure_W(mcmb_ctx* c, int prec)
     const int N = K.lmax + 1, Kp = (N + PK - 1) / PK * PK;
     if (c->w_ready == prec + 1) return MCMB_OK;
     size_t tot = (size_t)N * N * Kp;
+    MCMB_TRY(launch_guard("projection weight cube", (tot + 255) / 256, 256));
     if (prec == MCMB_FP64) {
         MCMB_TRY(c->d_W.alloc(tot * sizeof(double)));
         hipLaunchKernelGGL(proj_build_W<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,

@@ -15,11 +15,15 @@ Reported beside it:
                       exact reference for the errors below
   gamma_err_mixed     max_mn |G_mixed - G_ref| / sqrt(G_mm G_nn) and the
                       Frobenius relative error, full Gram
-  eig_err_mixed       eigenvalues of C = D^-1 G D^-1 (D = sqrt diag G), mixed vs
-                      reference, relative error over the well-determined ones
-  beta                FP64 Cholesky orthonormalisation of the reference Gram
-                      (status: the normalised Gram is singular to FP64 at this
-                      size, so beta is not defined by the reference algorithm)
+  eig_err_mixed       eigenvalues of C = D^-1 G D^-1 (D = sqrt diag G) from the
+                      GPU Jacobi eigen-decomposition, mixed vs reference,
+                      relative error over the well-determined ones
+  beta                orthonormalised coefficients beta^R = lambda alpha with
+                      alpha = G_ref c (seeded c), lambda from the late-side
+                      orthonormalisation (PCA fallback, as the reference takes
+                      when the normalised Gram has negative eigenvalues) of the
+                      mixed and of the reference Gram; the prim-side Cholesky
+                      status of the reference Gram
   projection          256-shape batch (omega, phi) at 300 GL nodes, 4960 modes:
                       forward with tangents (alpha, d alpha / d theta) and the
                       adjoint, FP32 MFMA with FP64 sums; gradients/s and the
@@ -60,14 +64,6 @@ def rel_elem(A, B):
     d = np.sqrt(np.outer(np.abs(np.diag(B)), np.abs(np.diag(B))))
     d[d == 0] = 1.0
     return float((np.abs(A - B) / d).max())
-
-
-def eigvals_normalised(g):
-    """Eigenvalues (descending) of D^-1 G D^-1 in FP64, LAPACK on the host
-    (measurement only)."""
-    d = 1.0 / np.sqrt(np.diag(g))
-    c = g * d[:, None] * d[None, :]
-    return np.linalg.eigvalsh(c)[::-1].copy()
 
 
 def hbm_traffic(workload):
@@ -293,9 +289,13 @@ def main():
     err = rel_elem(g_mixed, g_ref)
     frob = float(np.linalg.norm(g_mixed - g_ref) / np.linalg.norm(g_ref))
 
-    # eigenvalues of the normalised Gram
-    e_ref = eigvals_normalised(g_ref)
-    e_mix = eigvals_normalised(g_mixed)
+    # late-side orthonormalisation (eigen-decomposition on the GPU, PCA branch
+    # when the normalised Gram has a negative eigenvalue) of both Grams
+    t = time.time()
+    r_ref, lam_ref, _, _, e_ref = ctx.orthonormalise_late(g_ref, pca_flag=1)
+    t_late = time.time() - t
+    r_mix, lam_mix, _, _, e_mix = ctx.orthonormalise_late(g_mixed, pca_flag=1)
+    log(f"late orthonormalisation (Jacobi + PCA) {t_late:.2f} s per Gram")
     lmax = float(e_ref[0])
     eig = {"lambda_max": lmax, "max_abs_err_over_lambda_max": float(np.abs(e_mix - e_ref).max() / lmax),
            "negative_eigenvalues_fp64": int((e_ref < 0).sum())}
@@ -303,17 +303,27 @@ def main():
         k = e_ref > thr * lmax
         eig[f"rel_err_lambda_gt_{thr:g}_lmax"] = {"count": int(k.sum()),
                                                   "max": float((np.abs(e_mix[k] - e_ref[k]) / e_ref[k]).max())}
-    log("eigenvalues done")
-
-    # FP64 Cholesky orthonormalisation of the reference Gram
+    # beta^R = lambda alpha for alpha = Gamma_ref c (seeded c): reference vs mixed
+    cvec = np.random.default_rng(20251202).standard_normal(n)
+    alpha = g_ref @ cvec
+    b_ref, b_mix = lam_ref @ alpha, lam_mix @ alpha
+    beta = {"method": "late-side PCA (ortho_late_mpi.c:784-805): the normalised Gram has negative eigenvalues",
+            "rank_ref": int(r_ref), "rank_mixed": int(r_mix), "orthonormalise_late_s": round(t_late, 3),
+            "rel_err_l2_all": float(np.linalg.norm(b_mix - b_ref) / np.linalg.norm(b_ref))}
+    for thr in (1e-2, 1e-4, 1e-6):
+        k = e_ref > thr * lmax
+        beta[f"rel_err_l2_lambda_gt_{thr:g}_lmax"] = float(np.linalg.norm(b_mix[k] - b_ref[k]) /
+                                                          np.linalg.norm(b_ref[k]))
+    # the prim-side Cholesky (ortho_prim_mpi.c:198-284) on the same Gram
     t = time.time()
     try:
         ctx.orthonormalise(g_ref)
-        beta = {"status": "ok"}
+        beta["cholesky_prim"] = "ok"
     except modalcmb.NotSPDError as e:
-        beta = {"status": "not defined: " + str(e)}
-    beta["orthonormalise_s"] = round(time.time() - t, 4)
+        beta["cholesky_prim"] = "not positive definite in FP64: " + str(e)
+    beta["cholesky_prim_s"] = round(time.time() - t, 4)
     beta["kappa_lower_bound"] = lmax / max(float(np.abs(e_ref).min()), 1e-300)
+    log("orthonormalisation done")
     ctx.close()
 
     achieved = flops / k_mixed / 1e12

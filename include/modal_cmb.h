@@ -117,6 +117,26 @@ int mcmb_orthonormalise(mcmb_ctx* ctx, int n, const double* gamma, double* ortho
 int mcmb_cholesky_solve(mcmb_ctx* ctx, int n, const double* ortho, int nrhs,
                         const double* alpha, double* beta);
 
+/* Symmetric eigen-decomposition in FP64 on the GPU (parallel cyclic Jacobi),
+ * replacing gsl_eigen_symmv + gsl_eigen_symmv_sort(..., GSL_EIGEN_SORT_VAL_DESC)
+ * (ortho_prim_mpi.c:223-224, ortho_late_mpi.c:775-776).  a: n x n row-major
+ * symmetric.  evals[n] descending.  evecs (n x n, may be NULL): evecs[i*n + k]
+ * = component i of eigenvector k (the column layout of GSL's eigenv), with its
+ * largest-magnitude component positive.  *sweeps (may be NULL) = sweeps used. */
+int mcmb_eigen_sym(mcmb_ctx* ctx, int n, const double* a, double* evals, double* evecs, int* sweeps);
+
+/* Late-side orthonormalisation with the PCA fallback (ortho_late_mpi.c:
+ * 752-842): norm = sqrt(diag Gamma), C = D^-1 Gamma D^-1, eigenvalues evals[n]
+ * of C (descending).  PCA branch when pca_flag == 0 (the reference's
+ * gflag_pca == 0) or the smallest eigenvalue is negative: r = number of
+ * positive eigenvalues, lambda[i][j] = V[j][i] / (norm_j sqrt(e_i)) for i < r,
+ * lambdainv[i][j] = norm_i V[i][j] sqrt(e_j) for j < r, zero elsewhere (V as
+ * evecs above).  Cholesky branch otherwise: lambdainv = D L and lambda =
+ * (D L)^-1, lower triangles, zero above.  *rank = r (n in the Cholesky
+ * branch).  n x n host arrays. */
+int mcmb_orthonormalise_late(mcmb_ctx* ctx, int n, const double* gamma, int pca_flag, double* lambda,
+                             double* lambdainv, double* norm, double* evals, int* rank);
+
 /* ---- primordial k-space: Gram and shape projection ------------------------ */
 /* Gauss-Legendre k-tetrapyd: N ascending nodes kn[] with weights kw[], basis
  * q[a][0..ldq) at node a, mode order[n][3].  Points are the node-index triples

@@ -1,0 +1,370 @@
+// FP64 symmetric eigen-decomposition on the GPU (parallel cyclic Jacobi) and
+// the late-side orthonormalisation with its PCA fallback
+// (ortho_late_mpi.c:752-842; gsl_eigen_symmv + sort descending at
+// ortho_prim_mpi.c:223-224 and ortho_late_mpi.c:775-776).
+//
+// Jacobi schedule.  n_e = n rounded up to even (an odd n gets one zero row and
+// column at index n, which no rotation ever touches: its a_pq stay 0).  Each
+// step rotates n_e/2 disjoint pairs at once.  The matrix is kept in a layout
+// where the pairs of the current step are adjacent positions (2I, 2I+1); after
+// the step every position x moves to rr_next(x), the circle method of a
+// round-robin tournament (position 0 fixed, the others rotate), so that in
+// n_e - 1 steps every pair of indices meets exactly once and the layout is
+// back in the original order at the end of each sweep.  One kernel computes
+// the n_e/2 rotations of a step, one kernel applies them: each thread rewrites
+// one 2 x 2 block of A from its four inputs, A' = R_I^T A R_J, and one pair of
+// eigenvector rows, writing both at their next-step positions (double
+// buffered).  Rotations follow Golub & Van Loan (Matrix Computations, Alg.
+// 8.5.1): tau = (a_qq - a_pp)/(2 a_pq), t = sign(tau)/(|tau| + sqrt(1+tau^2)),
+// c = 1/sqrt(1+t^2), s = t c; a pair is skipped when |a_pq| <= 1e-16
+// sqrt(|a_pp a_qq|) (relative accuracy of well-separated small eigenvalues) or
+// |a_pq| <= 1e-17 ||A||_F (entries below the normwise backward error, which
+// otherwise keep rotating among eigenvalues at the FP64 noise floor of a
+// singular Gram).  Sweeps stop when a sweep applies no rotation or the
+// off-diagonal norm is below max(1e-15, 2e-17 n) ||A||_F, inside the normwise
+// bound n eps ||A|| of a Householder/QR eigensolver (eps = 1.1e-16); tighter
+// stops only rotate inside clusters of eigenvalues at the FP64 noise floor,
+// where Jacobi converges linearly.
+#include "mcmb_internal.h"
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+namespace mcmb {
+
+__device__ __forceinline__ int rr_next(int x, int m)
+{
+    if (x == 0 || m == 1) return x;
+    if (!(x & 1)) {
+        const int i = x >> 1;
+        return (i + 1 <= m - 1) ? 2 * (i + 1) : 2 * (m - 1) + 1;
+    }
+    const int i = (x - 1) >> 1;
+    return i >= 1 ? 2 * (i - 1) + 1 : 2;
+}
+
+// C = D^-1 G D^-1 (norm[i] = sqrt G_ii) into the n_e x n_e buffer A (padding
+// row/column zero), U = identity rows (U[x][i] = component i of the vector
+// at layout position x; n_e rows of n).
+__global__ void jac_init(const double* __restrict__ G, int n, int ne, int normalise, double* __restrict__ norm,
+                         double* __restrict__ A, double* __restrict__ U)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long tot = (long)ne * ne;
+    if (e < tot) {
+        const int i = (int)(e / ne), j = (int)(e % ne);
+        double v = 0.0;
+        if (i < n && j < n) {
+            v = G[(long)i * n + j];
+            if (normalise) v /= sqrt(G[(long)i * n + i]) * sqrt(G[(long)j * n + j]);
+        }
+        A[e] = v;
+    }
+    if (e < (long)ne * n) {
+        const int x = (int)(e / n), i = (int)(e % n);
+        U[e] = (x == i) ? 1.0 : 0.0;
+    }
+    if (e < n && norm) norm[e] = sqrt(G[e * n + e]);
+}
+
+// Rotation of pair I = (2I, 2I+1) of the current layout: cs[I] = (c, s, t).
+__global__ void jac_rot(const double* __restrict__ A, int ne, double abs_tol, double3* __restrict__ cs,
+                        unsigned long long* __restrict__ nrot)
+{
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = ne / 2;
+    if (I >= m) return;
+    const int p = 2 * I, q = p + 1;
+    const double app = A[(long)p * ne + p], aqq = A[(long)q * ne + q], apq = A[(long)p * ne + q];
+    double c = 1.0, s = 0.0, t = 0.0;
+    if (fabs(apq) > abs_tol && fabs(apq) > 1e-16 * sqrt(fabs(app * aqq))) {
+        const double tau = (aqq - app) / (2.0 * apq);
+        t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        c = 1.0 / sqrt(1.0 + t * t);
+        s = t * c;
+        atomicAdd(nrot, 1ull);
+    }
+    cs[I] = make_double3(c, s, t);
+}
+
+// Apply the step's rotations: threads [0, m*m) rewrite 2 x 2 blocks of A,
+// threads [m*m, m*m + m*n) rewrite pairs of U rows; outputs at the next
+// step's positions.
+__global__ __launch_bounds__(256) void jac_apply(const double* __restrict__ A, double* __restrict__ An,
+                                                 const double* __restrict__ U, double* __restrict__ Un,
+                                                 const double3* __restrict__ cs, int ne, int n)
+{
+    const int m = ne / 2;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long na = (long)m * m;
+    if (e < na) {
+        const int I = (int)(e / m), J = (int)(e % m);
+        const double3 ri = cs[I], rj = cs[J];
+        const long r0 = (long)(2 * I) * ne, r1 = r0 + ne;
+        const double2 top = *reinterpret_cast<const double2*>(A + r0 + 2 * J);
+        const double2 bot = *reinterpret_cast<const double2*>(A + r1 + 2 * J);
+        double b00, b01, b10, b11;
+        if (I == J) {
+            const double apq = top.y;
+            b00 = top.x - ri.z * apq;
+            b11 = bot.y + ri.z * apq;
+            b01 = b10 = 0.0;
+            if (ri.y == 0.0) { b00 = top.x; b11 = bot.y; b01 = top.y; b10 = bot.x; }
+        } else {
+            const double x00 = top.x * rj.x - top.y * rj.y, x01 = top.x * rj.y + top.y * rj.x;
+            const double x10 = bot.x * rj.x - bot.y * rj.y, x11 = bot.x * rj.y + bot.y * rj.x;
+            b00 = ri.x * x00 - ri.y * x10;
+            b01 = ri.x * x01 - ri.y * x11;
+            b10 = ri.y * x00 + ri.x * x10;
+            b11 = ri.y * x01 + ri.x * x11;
+        }
+        const int pi = rr_next(2 * I, m), qi = rr_next(2 * I + 1, m);
+        const int pj = rr_next(2 * J, m), qj = rr_next(2 * J + 1, m);
+        An[(long)pi * ne + pj] = b00;
+        An[(long)pi * ne + qj] = b01;
+        An[(long)qi * ne + pj] = b10;
+        An[(long)qi * ne + qj] = b11;
+        return;
+    }
+    const long ev = e - na;
+    if (ev >= (long)m * n) return;
+    const int I = (int)(ev / n), i = (int)(ev % n);
+    const double3 r = cs[I];
+    const double up = U[(long)(2 * I) * n + i], uq = U[(long)(2 * I + 1) * n + i];
+    Un[(long)rr_next(2 * I, m) * n + i] = r.x * up - r.y * uq;
+    Un[(long)rr_next(2 * I + 1, m) * n + i] = r.y * up + r.x * uq;
+}
+
+// Sum of squares of the off-diagonal and of all entries (n_e x n_e).
+__global__ __launch_bounds__(256) void jac_norms(const double* __restrict__ A, int ne, double* __restrict__ out)
+{
+    __shared__ double so[256], sa[256];
+    double o = 0.0, a = 0.0;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < (long)ne * ne; e += (long)gridDim.x * 256) {
+        const double v = A[e] * A[e];
+        a += v;
+        if (e / ne != e % ne) o += v;
+    }
+    so[threadIdx.x] = o;
+    sa[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            so[threadIdx.x] += so[threadIdx.x + w];
+            sa[threadIdx.x] += sa[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(out, so[0]);
+        atomicAdd(out + 1, sa[0]);
+    }
+}
+
+// Rows of U in descending eigenvalue order (perm), each with its largest
+// magnitude component (first on ties) made positive: V[k][i].
+__global__ __launch_bounds__(256) void jac_sort_rows(const double* __restrict__ U, int n, const int* __restrict__ perm,
+                                                     double* __restrict__ V)
+{
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    const int k = blockIdx.x;
+    const double* row = U + (long)perm[k] * n;
+    double best = -1.0;
+    int bi = 0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const double a = fabs(row[i]);
+        if (a > best) { best = a; bi = i; }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            const double b = sv[threadIdx.x + w];
+            const int j = si[threadIdx.x + w];
+            if (b > sv[threadIdx.x] || (b == sv[threadIdx.x] && j < si[threadIdx.x])) {
+                sv[threadIdx.x] = b;
+                si[threadIdx.x] = j;
+            }
+        }
+        __syncthreads();
+    }
+    const double sg = row[si[0]] < 0.0 ? -1.0 : 1.0;
+    for (int i = threadIdx.x; i < n; i += 256) V[(long)k * n + i] = sg * row[i];
+}
+
+// evecs[i][k] = V[k][i]
+__global__ void jac_transpose(const double* __restrict__ V, int n, double* __restrict__ out)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)n * n) return;
+    const int i = (int)(e / n), k = (int)(e % n);
+    out[e] = V[(long)k * n + i];
+}
+
+// PCA outputs: lambda[i][j] = V[i][j] / (norm_j sqrt(e_i)) for i < r,
+// lambdainv[i][j] = norm_i V[j][i] sqrt(e_j) for j < r (V rows = eigenvectors).
+__global__ void pca_outputs(const double* __restrict__ V, const double* __restrict__ ev,
+                            const double* __restrict__ norm, int n, int r, double* __restrict__ lam,
+                            double* __restrict__ linv)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)n * n) return;
+    const int i = (int)(e / n), j = (int)(e % n);
+    lam[e] = i < r ? V[(long)i * n + j] / (norm[j] * sqrt(ev[i])) : 0.0;
+    linv[e] = j < r ? norm[i] * V[(long)j * n + i] * sqrt(ev[j]) : 0.0;
+}
+
+// Jacobi on the device.  On return d_V (n x n) holds the sorted, sign-fixed
+// eigenvectors as rows and evals (host) the eigenvalues, descending.
+static int jacobi_device(mcmb_ctx* c, int n, const double* d_G, int normalise, double* d_norm, double* evals,
+                         DevBuf& d_V, int* sweeps_out)
+{
+    const int ne = n + (n & 1), m = ne / 2;
+    DevBuf A0, A1, U0, U1, dcs, dcnt, dnrm, dperm;
+    MCMB_TRY(A0.alloc(sizeof(double) * (size_t)ne * ne));
+    MCMB_TRY(A1.alloc(sizeof(double) * (size_t)ne * ne));
+    MCMB_TRY(U0.alloc(sizeof(double) * (size_t)ne * n));
+    MCMB_TRY(U1.alloc(sizeof(double) * (size_t)ne * n));
+    MCMB_TRY(dcs.alloc(sizeof(double3) * (size_t)m));
+    MCMB_TRY(dcnt.alloc(sizeof(unsigned long long)));
+    MCMB_TRY(dnrm.alloc(sizeof(double) * 2));
+    const long ninit = std::max<long>((long)ne * ne, (long)ne * n);
+    MCMB_TRY(launch_guard("Jacobi init", (ninit + 255) / 256, 256));
+    hipLaunchKernelGGL(jac_init, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, c->stream, d_G, n, ne,
+                       normalise, d_norm, A0.as<double>(), U0.as<double>());
+    MCMB_HIP(hipGetLastError());
+    const long napply = (long)m * m + (long)m * n;
+    MCMB_TRY(launch_guard("Jacobi step", (napply + 255) / 256, 256));
+    double* A = A0.as<double>();
+    double* An = A1.as<double>();
+    double* U = U0.as<double>();
+    double* Un = U1.as<double>();
+    int sweep = 0;
+    bool converged = false;
+    double abs_tol = 0.0;
+    for (; sweep < 50 && !converged; ++sweep) {
+        MCMB_HIP(hipMemsetAsync(dnrm.p, 0, sizeof(double) * 2, c->stream));
+        hipLaunchKernelGGL(jac_norms, dim3(1024), dim3(256), 0, c->stream, A, ne, dnrm.as<double>());
+        MCMB_HIP(hipGetLastError());
+        double nr[2];
+        MCMB_HIP(hipMemcpyAsync(nr, dnrm.p, sizeof(nr), hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        const double stop = std::max(1e-15, 2e-17 * n);
+        if (nr[0] <= stop * stop * nr[1]) { converged = true; break; }
+        if (sweep == 0) abs_tol = 1e-17 * sqrt(nr[1]);
+        MCMB_HIP(hipMemsetAsync(dcnt.p, 0, sizeof(unsigned long long), c->stream));
+        for (int step = 0; step < ne - 1; ++step) {
+            hipLaunchKernelGGL(jac_rot, dim3((m + 255) / 256), dim3(256), 0, c->stream, A, ne, abs_tol,
+                               dcs.as<double3>(), dcnt.as<unsigned long long>());
+            hipLaunchKernelGGL(jac_apply, dim3((unsigned)((napply + 255) / 256)), dim3(256), 0, c->stream, A, An,
+                               U, Un, dcs.as<double3>(), ne, n);
+            std::swap(A, An);
+            std::swap(U, Un);
+        }
+        MCMB_HIP(hipGetLastError());
+        unsigned long long cnt = 0;
+        MCMB_HIP(hipMemcpyAsync(&cnt, dcnt.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        if (cnt == 0) { converged = true; ++sweep; break; }
+    }
+    if (!converged) return fail(MCMB_ERR_ARG, "Jacobi eigen-decomposition of n=%d did not converge in 50 sweeps", n);
+    // layout is back in index order: eigenvalue of index i = A[i][i]
+    std::vector<double> diag(n);
+    MCMB_HIP(hipMemcpy2DAsync(diag.data(), sizeof(double), A, sizeof(double) * (ne + 1), sizeof(double), n,
+                              hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    std::vector<int> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return diag[a] > diag[b]; });
+    for (int k = 0; k < n; ++k) evals[k] = diag[perm[k]];
+    MCMB_TRY(dperm.alloc(sizeof(int) * (size_t)n));
+    MCMB_HIP(hipMemcpyAsync(dperm.p, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    MCMB_TRY(d_V.alloc(sizeof(double) * (size_t)n * n));
+    hipLaunchKernelGGL(jac_sort_rows, dim3(n), dim3(256), 0, c->stream, U, n, dperm.as<int>(), d_V.as<double>());
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipStreamSynchronize(c->stream));   // perm and the work buffers go out of scope
+    if (sweeps_out) *sweeps_out = sweep;
+    return MCMB_OK;
+}
+
+}  // namespace mcmb
+
+using namespace mcmb;
+
+extern "C" int mcmb_eigen_sym(mcmb_ctx* c, int n, const double* a, double* evals, double* evecs, int* sweeps)
+{
+    if (!c || !a || !evals || n < 1) return fail(MCMB_ERR_ARG, "mcmb_eigen_sym: bad arguments");
+    MCMB_TRY(set_device(c));
+    DevBuf dG, dV, dE;
+    MCMB_TRY(dG.alloc(sizeof(double) * (size_t)n * n));
+    MCMB_HIP(hipMemcpyAsync(dG.p, a, sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
+    MCMB_TRY(jacobi_device(c, n, dG.as<double>(), 0, nullptr, evals, dV, sweeps));
+    if (evecs) {
+        MCMB_TRY(dE.alloc(sizeof(double) * (size_t)n * n));
+        const long nn = (long)n * n;
+        hipLaunchKernelGGL(jac_transpose, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream,
+                           dV.as<double>(), n, dE.as<double>());
+        MCMB_HIP(hipGetLastError());
+        MCMB_HIP(hipMemcpyAsync(evecs, dE.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+    }
+    MCMB_HIP(hipEventRecord(c->ev1, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = 0;
+    return MCMB_OK;
+}
+
+extern "C" int mcmb_orthonormalise_late(mcmb_ctx* c, int n, const double* gamma, int pca_flag, double* lambda,
+                                        double* lambdainv, double* norm, double* evals, int* rank)
+{
+    if (!c || !gamma || !lambda || !lambdainv || !norm || !evals || n < 1)
+        return fail(MCMB_ERR_ARG, "mcmb_orthonormalise_late: bad arguments");
+    MCMB_TRY(set_device(c));
+    for (int i = 0; i < n; ++i)
+        if (!(gamma[(long)i * n + i] > 0.0))
+            return fail(MCMB_ERR_ARG, "mcmb_orthonormalise_late: Gamma[%d][%d] = %g is not positive", i, i,
+                        gamma[(long)i * n + i]);
+    DevBuf dG, dN, dV, dEv, dL, dLi;
+    const size_t nn = (size_t)n * n;
+    MCMB_TRY(dG.alloc(sizeof(double) * nn));
+    MCMB_TRY(dN.alloc(sizeof(double) * n));
+    MCMB_HIP(hipMemcpyAsync(dG.p, gamma, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+    MCMB_TRY(jacobi_device(c, n, dG.as<double>(), 1, dN.as<double>(), evals, dV, nullptr));
+    MCMB_HIP(hipMemcpyAsync(norm, dN.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    if (pca_flag == 0 || evals[n - 1] < 0.0) {
+        int r = 0;
+        while (r < n && evals[r] > 0.0) ++r;
+        if (r == 0) return fail(MCMB_ERR_NOT_SPD, "mcmb_orthonormalise_late: no positive eigenvalue");
+        MCMB_TRY(dEv.alloc(sizeof(double) * n));
+        MCMB_TRY(dL.alloc(sizeof(double) * nn));
+        MCMB_TRY(dLi.alloc(sizeof(double) * nn));
+        MCMB_HIP(hipMemcpyAsync(dEv.p, evals, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(pca_outputs, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream,
+                           dV.as<double>(), dEv.as<double>(), dN.as<double>(), n, r, dL.as<double>(),
+                           dLi.as<double>());
+        MCMB_HIP(hipGetLastError());
+        MCMB_HIP(hipMemcpyAsync(lambda, dL.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipMemcpyAsync(lambdainv, dLi.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        if (rank) *rank = r;
+        return MCMB_OK;
+    }
+    // Cholesky branch: lambdainv = D L, lambda = (D L)^-1, lower triangles
+    std::vector<double> ortho(nn), lam(nn);
+    int bad = -1;
+    MCMB_TRY(mcmb_orthonormalise(c, n, gamma, ortho.data(), lam.data(), norm, &bad));
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            lambdainv[(long)i * n + j] = j <= i ? ortho[(long)i * n + j] : 0.0;
+            lambda[(long)i * n + j] = j <= i ? lam[(long)i * n + j] : 0.0;
+        }
+    if (rank) *rank = n;
+    return MCMB_OK;
+}

@@ -67,6 +67,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_last_kernel_ms.argtypes = [vp, C.POINTER(dp), C.POINTER(ip)]
         lib.mcmb_orthonormalise.argtypes = [vp, ip, vp, vp, vp, vp, C.POINTER(ip)]
         lib.mcmb_cholesky_solve.argtypes = [vp, ip, vp, ip, vp, vp]
+        lib.mcmb_eigen_sym.argtypes = [vp, ip, vp, vp, vp, C.POINTER(ip)]
+        lib.mcmb_orthonormalise_late.argtypes = [vp, ip, vp, ip, vp, vp, vp, vp, C.POINTER(ip)]
         for name in ("get_ijk_next_XXX", "get_ijk_next_prim"):
             getattr(lib, name).argtypes = [ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip)]
         lib.mcmb_gauss_legendre.argtypes = [ip, vp, vp]
@@ -90,7 +92,7 @@ EXPORTED_SYMBOLS = (
     "mcmb_lspace_points", "mcmb_lspace_weights", "mcmb_orthonormalise", "mcmb_cholesky_solve",
     "mcmb_last_kernel_ms", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
     "mcmb_gauss_legendre", "mcmb_gamma_tet_slice", "mcmb_kspace_set", "mcmb_kspace_npoints",
-    "mcmb_kgram", "mcmb_project_fwd", "mcmb_project_bwd",
+    "mcmb_kgram", "mcmb_project_fwd", "mcmb_project_bwd", "mcmb_eigen_sym", "mcmb_orthonormalise_late",
     "get_ijk_next_XXX", "check_ijk_min_XXX", "check_ijk_max_XXX", "decompose_tetrapyd_XXX",
     "get_ijk_next_prim", "check_ijk_min_prim", "check_ijk_max_prim", "decompose_tetrapyd_prim",
     "divide_tasks", "get_next_ijk_bound", "permsix", "calculate_geometric",
@@ -256,6 +258,29 @@ class Context:
             raise NotSPDError(st, f"{_lib.mcmb_last_error().decode()} (pivot {bad.value})")
         _check(st)
         return ortho, lam, norm
+
+    def eigen_sym(self, a: np.ndarray, vectors: bool = True):
+        """(evals descending, evecs[:, k] or None, sweeps) on the GPU."""
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        n = a.shape[0]
+        ev = np.empty(n)
+        V = np.empty_like(a) if vectors else None
+        sw = C.c_int(0)
+        _check(_lib.mcmb_eigen_sym(self._h, n, _ptr(a), _ptr(ev), _ptr(V), C.byref(sw)))
+        return ev, V, sw.value
+
+    def orthonormalise_late(self, gamma: np.ndarray, pca_flag: int = 1):
+        """(r, lambda, lambdainv, norm, evals) as ortho_late_mpi.c:752-842 leaves them."""
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        n = g.shape[0]
+        lam, linv, norm, ev = np.empty_like(g), np.empty_like(g), np.empty(n), np.empty(n)
+        r = C.c_int(0)
+        st = _lib.mcmb_orthonormalise_late(self._h, n, _ptr(g), pca_flag, _ptr(lam), _ptr(linv), _ptr(norm),
+                                           _ptr(ev), C.byref(r))
+        if st == -3:
+            raise NotSPDError(st, _lib.mcmb_last_error().decode())
+        _check(st)
+        return r.value, lam, linv, norm, ev
 
     def cholesky_solve(self, ortho: np.ndarray, alpha: np.ndarray) -> np.ndarray:
         f = np.ascontiguousarray(ortho, dtype=np.float64)

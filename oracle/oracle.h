@@ -120,6 +120,13 @@ int or_orthonormalise(int n, const double* gamma, double* ortho, double* lambda,
 /* gsl_linalg_cholesky_solve with the symmetrised D L factor
  * (eigenmode_mpi.c:195-219): beta = Gamma^-1 alpha. */
 void or_cholesky_solve(int n, const double* ortho, const double* alpha, double* beta);
+/* Cyclic Jacobi eigen-decomposition replacing gsl_eigen_symmv + sort
+ * descending (ortho_prim_mpi.c:223-224, ortho_late_mpi.c:775-776); evecs[i*n+k]
+ * = component i of eigenvector k, largest component positive. */
+int or_eigen_sym(int n, const double* A, double* evals, double* evecs);
+/* Late-side orthonormalisation with the PCA fallback (ortho_late_mpi.c:752-842). */
+int or_ortho_late(int n, const double* G, int pca_flag, double* lambda, double* lambdainv,
+                  double* norm, double* evals);
 
 /* ---- shapes and projection ---------------------------------------------- */
 /* feature  calculate_shape.c:933-946 ; shape3 = (k1k2k3)^2 shape, 197-216 */

@@ -54,6 +54,8 @@ def lib() -> C.CDLL:
         L.or_gamma_gl.argtypes = [ip, ip, vp, ip, vp, ip, vp, ip, vp]
         L.or_orthonormalise.argtypes = [ip, vp, vp, vp, vp]
         L.or_cholesky_solve.argtypes = [ip, vp, vp, vp]
+        L.or_eigen_sym.argtypes = [ip, vp, vp, vp]
+        L.or_ortho_late.argtypes = [ip, vp, ip, vp, vp, vp, vp]
         L.or_gram_kgl.argtypes = [ip, vp, vp, vp, ip, ip, vp, vp]
         L.or_project_kgl.argtypes = [ip, vp, vp, vp, ip, ip, vp, dp, dp, vp, vp, vp]
         L.or_calculate_ortho_uniform.argtypes = [ip, dp, vp, ip, ip, vp, vp]
@@ -159,6 +161,24 @@ def orthonormalise(gamma: np.ndarray):
     ortho, lam, norm = np.empty_like(g), np.empty_like(g), np.empty(n)
     st = lib().or_orthonormalise(n, _p(g), _p(ortho), _p(lam), _p(norm))
     return st, ortho, lam, norm
+
+
+def eigen_sym(a: np.ndarray):
+    """(sweeps, evals descending, evecs[:, k]) of a symmetric matrix."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    n = a.shape[0]
+    e, v = np.empty(n), np.empty_like(a)
+    st = lib().or_eigen_sym(n, _p(a), _p(e), _p(v))
+    return st, e, v
+
+
+def ortho_late(gamma: np.ndarray, pca_flag: int = 1):
+    """(r, lambda, lambdainv, norm, evals) as ortho_late_mpi.c:752-842 leaves them."""
+    g = np.ascontiguousarray(gamma, dtype=np.float64)
+    n = g.shape[0]
+    lam, linv, norm, ev = np.empty_like(g), np.empty_like(g), np.empty(n), np.empty(n)
+    r = lib().or_ortho_late(n, _p(g), pca_flag, _p(lam), _p(linv), _p(norm), _p(ev))
+    return r, lam, linv, norm, ev
 
 
 def cholesky_solve(ortho: np.ndarray, alpha: np.ndarray) -> np.ndarray:

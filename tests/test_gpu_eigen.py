@@ -1,0 +1,108 @@
+"""GPU: FP64 Jacobi eigen-decomposition and the late-side orthonormalisation
+with its PCA fallback, against LAPACK (numpy eigh) and the oracle."""
+import numpy as np
+import pytest
+
+import modalcmb
+import pyoracle as O
+from modalcmb import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _sym(n, seed):
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((n, n))
+    return 0.5 * (a + a.T)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 33, 64, 257, 600])
+def test_eigen_sym_against_lapack(ctx, n):
+    A = _sym(n, n)
+    ev, V, sweeps = ctx.eigen_sym(A)
+    ref, Wl = np.linalg.eigh(A)
+    ref, Wl = ref[::-1], Wl[:, ::-1]
+    nrm = max(np.abs(ref).max(), 1.0)
+    assert np.all(np.diff(ev) <= 0)
+    assert np.abs(ev - ref).max() < 1e-13 * nrm * max(n, 4)
+    assert np.abs(A @ V - V * ev).max() < 1e-12 * nrm * max(1.0, np.sqrt(n))
+    assert np.abs(V.T @ V - np.eye(n)).max() < 1e-12 * max(1.0, np.sqrt(n))
+    imax = np.argmax(np.abs(V), axis=0)
+    assert np.all(V[imax, np.arange(n)] > 0)
+    gaps = np.abs(np.diff(ref))
+    for k in range(n):
+        sep = min(gaps[k - 1] if k > 0 else np.inf, gaps[k] if k < n - 1 else np.inf)
+        if sep > 1e-4 * nrm:
+            assert abs(abs(V[:, k] @ Wl[:, k]) - 1.0) < 1e-9
+    assert 0 < sweeps < 20 or n == 1
+
+
+def test_eigen_sym_matches_oracle_vectors(ctx):
+    """Same sign convention as the oracle: the eigenvectors agree entrywise."""
+    A = _sym(120, 7)
+    ev, V, _ = ctx.eigen_sym(A)
+    st, e2, V2 = O.eigen_sym(A)
+    assert st >= 0
+    assert np.abs(ev - e2).max() < 1e-12
+    assert np.abs(V - V2).max() < 1e-9
+
+
+def test_orthonormalise_late_config1_pca(ctx):
+    """lmax=500, 220 modes, PCA branch forced (pca_flag = 0, the reference's
+    gflag_pca == 0): lambda orthonormalises Gamma on the kept directions,
+    lambdainv reconstructs Gamma, and both agree with the oracle's
+    restatement.  The normalised Gram has kappa ~ 1e17, so the count r of
+    positive eigenvalues is itself at the FP64 noise floor: only directions
+    with e > 1e-10 e_max are compared."""
+    c = W.CONFIGS[1]
+    ws = W.lspace(c["L"], c["R"], c["modulated"])
+    ctx.lspace_set(2, c["L"], ws["q"], ws["v"], ws["order"])
+    G = ctx.gamma_gl()
+    r, lam, linv, norm, ev = ctx.orthonormalise_late(G, pca_flag=0)
+    r2, lam2, linv2, norm2, ev2 = O.ortho_late(G, pca_flag=0)
+    assert 0 < r <= 220 and 0 < r2 <= 220
+    assert np.array_equal(norm, norm2)
+    assert np.abs(ev - ev2).max() < 1e-13 * ev[0]
+    k = ev > 1e-10 * ev[0]
+    nk = int(k.sum())
+    assert nk <= min(r, r2)
+    P = lam[:nk] @ G @ lam[:nk].T
+    assert np.abs(P - np.eye(nk)).max() < 1e-6
+    assert np.abs(lam[:nk] - lam2[:nk]).max() < 1e-6 * np.abs(lam2[:nk]).max()
+    Gr = linv @ linv.T
+    d = np.sqrt(np.outer(np.diag(G), np.diag(G)))
+    assert (np.abs(Gr - G) / d).max() < 1e-10
+    assert np.all(lam[r:] == 0) and np.all(linv[:, r:] == 0)
+
+
+def test_orthonormalise_late_config1_branch(ctx):
+    """pca_flag = 1: PCA when the smallest eigenvalue is negative, else the
+    Cholesky branch, which on this singular-to-FP64 Gram may fail cleanly
+    with MCMB_ERR_NOT_SPD (the reference aborts in GSL there)."""
+    c = W.CONFIGS[1]
+    ws = W.lspace(c["L"], c["R"], c["modulated"])
+    ctx.lspace_set(2, c["L"], ws["q"], ws["v"], ws["order"])
+    G = ctx.gamma_gl()
+    ev, _, _ = ctx.eigen_sym(G / np.sqrt(np.outer(np.diag(G), np.diag(G))), vectors=False)
+    try:
+        r, lam, linv, norm, ev2 = ctx.orthonormalise_late(G, pca_flag=1)
+    except modalcmb.NotSPDError:
+        assert ev[-1] >= 0      # Cholesky branch taken and failed
+        return
+    if ev[-1] < 0:
+        assert r < 220
+    else:
+        assert r == 220 and np.allclose(np.triu(lam, 1), 0)
+
+
+def test_orthonormalise_late_cholesky_branch_matches_prim(ctx):
+    """SPD Gram and pca_flag = 1: the Cholesky branch, equal to the lower
+    triangles of mcmb_orthonormalise's symmetrised outputs."""
+    rng = np.random.default_rng(3)
+    X = rng.standard_normal((300, 90))
+    G = X.T @ X
+    r, lam, linv, norm, ev = ctx.orthonormalise_late(G, pca_flag=1)
+    assert r == 90
+    o, l2, n2 = ctx.orthonormalise(G)
+    assert np.array_equal(linv, np.tril(o)) and np.array_equal(lam, np.tril(l2))
+    assert np.allclose(lam @ linv, np.eye(90), atol=1e-11)

@@ -238,3 +238,86 @@ def test_feature_phase_convention():
     s = O.lib().or_shape3_feature(0.01, 0.02, 0.025, 150.0, 11.0)
     assert s == pytest.approx(np.sin(150.0 * 0.055 + 2 * 3.1415927 * 11.0 * 1e-2), rel=1e-12)
     assert O.lib().or_feature(0.0, 0.1, 0.1, 1.0, 0.0) == 0.0
+
+
+def _sym(n, seed, cond=None):
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    ev = rng.standard_normal(n) if cond is None else np.geomspace(1.0, 1.0 / cond, n) * rng.choice([-1, 1], n)
+    return (q * ev) @ q.T
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (7, 2), (50, 3), (101, 4)])
+def test_eigen_sym_against_lapack(n, seed):
+    """Jacobi eigenvalues against LAPACK (numpy eigh, an independent
+    algorithm); eigenvectors up to sign against LAPACK's, and A V = V diag(e)."""
+    A = _sym(n, seed)
+    A = 0.5 * (A + A.T)
+    st, e, V = O.eigen_sym(A)
+    assert st >= 0
+    ref, W = np.linalg.eigh(A)
+    ref, W = ref[::-1], W[:, ::-1]
+    nrm = np.abs(ref).max()
+    assert np.abs(e - ref).max() < 1e-13 * max(nrm, 1.0) * max(n, 1)
+    assert np.abs(A @ V - V * e).max() < 1e-12 * max(nrm, 1.0)
+    assert np.abs(V.T @ V - np.eye(n)).max() < 1e-12
+    gaps = np.abs(np.diff(ref))
+    for k in range(n):
+        sep = min(gaps[k - 1] if k > 0 else np.inf, gaps[k] if k < n - 1 else np.inf)
+        if sep > 1e-3 * nrm:
+            assert abs(abs(V[:, k] @ W[:, k]) - 1.0) < 1e-10
+        i = np.argmax(np.abs(V[:, k]))
+        assert V[i, k] > 0          # sign convention: largest component positive
+
+
+def test_eigen_sym_graded_spectrum():
+    """Eigenvalues spread over 12 decades with both signs: each to a relative
+    accuracy set by the largest, as a backward-stable method guarantees."""
+    A = _sym(60, 9, cond=1e12)
+    st, e, V = O.eigen_sym(A)
+    ref = np.linalg.eigvalsh(A)[::-1]
+    assert np.abs(e - ref).max() < 1e-13
+
+
+def test_ortho_late_pca_identities():
+    """PCA branch (forced, pca_flag = 0): lambda G lambda^T = I on the r
+    positive directions and lambda lambdainv = I_r, zero outside."""
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((40, 25))
+    G = X.T @ X + 1e-3 * np.eye(25)
+    r, lam, linv, norm, ev = O.ortho_late(G, pca_flag=0)
+    assert r == 25
+    assert np.allclose(lam @ G @ lam.T, np.eye(25), atol=1e-10)
+    assert np.allclose(lam @ linv, np.eye(25), atol=1e-10)
+    assert np.allclose(linv @ linv.T, G, rtol=0, atol=1e-10 * np.abs(G).max())
+
+
+def test_ortho_late_pca_on_indefinite_gram():
+    """A Gram with a negative eigenvalue takes the PCA branch even with
+    pca_flag = 1; r counts the positive eigenvalues; rows of lambda at and
+    beyond r are zero."""
+    rng = np.random.default_rng(11)
+    q, _ = np.linalg.qr(rng.standard_normal((30, 30)))
+    A = (q * np.r_[np.linspace(2.0, 0.5, 27), -0.05, -0.1, -0.2]) @ q.T
+    assert np.diag(A).min() > 0
+    d = rng.uniform(0.5, 3.0, 30)
+    G = A * np.outer(d, d)
+    w = np.linalg.eigvalsh(G / np.sqrt(np.outer(np.diag(G), np.diag(G))))
+    assert w.min() < 0 < w.max()
+    r, lam, linv, norm, ev = O.ortho_late(G, pca_flag=1)
+    assert r == int((w > 0).sum())
+    assert np.all(lam[r:] == 0) and np.all(linv[:, r:] == 0)
+    P = lam[:r] @ G @ lam[:r].T
+    assert np.allclose(P, np.eye(r), atol=1e-9)
+
+
+def test_ortho_late_cholesky_branch():
+    """SPD Gram with pca_flag = 1: lambdainv = D L (lower), lambda = its inverse."""
+    rng = np.random.default_rng(6)
+    X = rng.standard_normal((60, 20))
+    G = X.T @ X
+    r, lam, linv, norm, ev = O.ortho_late(G, pca_flag=1)
+    assert r == 20
+    assert np.allclose(np.triu(linv, 1), 0) and np.allclose(np.triu(lam, 1), 0)
+    assert np.allclose(linv @ linv.T, G, rtol=1e-12, atol=1e-12 * np.abs(G).max())
+    assert np.allclose(lam @ linv, np.eye(20), atol=1e-11)

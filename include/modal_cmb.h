@@ -167,6 +167,31 @@ int mcmb_project_fwd(mcmb_ctx* ctx, int precision, int ntheta, const double* the
 int mcmb_project_bwd(mcmb_ctx* ctx, int precision, int ntheta, const double* theta,
                      const double* gbar, double* dtheta);
 
+/* ---- primordial k-space on the uniform grid (reference parity) ----------- */
+/* Uniform k-grid tetrapyd of calculate_ortho / calculate_eigen
+ * (orthogonalisation.c:14-86, calculate_eigen.c:86-145): grid points
+ * 0..max with k = kmax * index / max (eigenmode_mpi.c:56-59), the walk of
+ * get_ijk_next_prim from (min,min,min), point weight
+ * calculate_weight(min,max,i,j,k) * dk^3 / (k1+k2+k3) (calculate_volume.c:
+ * 12-132; 0 at the origin), basis q[index][0..ldq) (create_basis_prim layout,
+ * fixed_data.c:1670-1715), modes order[n][3]. */
+int mcmb_kuniform_set(mcmb_ctx* ctx, int min, int max, double kmax, int ldq, const double* q,
+                      int nmodes, const int* order);
+/* min, max, mode count, kmax, generation of the uniform setup (outputs may be NULL). */
+int mcmb_kuniform_info(mcmb_ctx* ctx, int* min, int* max, int* nmodes, double* kmax, long* generation);
+long mcmb_kuniform_npoints(mcmb_ctx* ctx);
+/* Point weights in walk order (up to cap); returns the point count. */
+long mcmb_kuniform_weights(mcmb_ctx* ctx, double* w, long cap);
+/* Gram over walk points [p0, p1) (p1 < 0: to the end), full symmetric n x n,
+ * FP64 or MIXED (calculate_ortho's sum, orthogonalisation.c:14-86). */
+int mcmb_kuniform_gram(mcmb_ctx* ctx, int precision, long p0, long p1, double* gamma_host,
+                       double* gamma_dev);
+/* alpha[b][n] over walk points [p0, p1) for nshape feature shapes
+ * theta[b] = (omega, phi), phi in radians (calculate_eigen with shape3 of
+ * model 4, calculate_eigen.c:86-145, calculate_shape.c:933-946); FP64. */
+int mcmb_kuniform_project(mcmb_ctx* ctx, int nshape, const double* theta, long p0, long p1,
+                          double* alpha);
+
 /* ---- quadrature ----------------------------------------------------------- */
 /* n-node Gauss-Legendre rule on [-1,1], nodes ascending (host only, no GPU). */
 int mcmb_gauss_legendre(int n, double* x, double* w);
@@ -222,6 +247,17 @@ int calculate_orthol_3D(int m, int n, double* result);
 /* orthogonalisation_late.c:239 -- result[0] = Gamma_mn from the cached
  * Gauss-Legendre-factorised Gram. */
 int calculate_orthol(int m, int n, double* result);
+/* orthogonalisation.c:14 -- adds the FP64 Gram of the share of the uniform
+ * walk covered by block[0 .. nthreads) of the last decompose_tetrapyd_prim
+ * call into the packed upper triangle ortho[n(n+1)/2] (r <= s, row major),
+ * using the bound context's mcmb_kuniform_set tables (same min, max).  Every
+ * block is integrated; the reference's threaded build integrates block 0
+ * only (its OpenMP pragma is commented out, orthogonalisation.c:34,37). */
+void calculate_ortho(int min, int max, tetrapyd_limits* block, double* ortho);
+/* calculate_eigen.c:86 -- adds alpha[n] of the feature shape (model 4:
+ * params.a1 = omega, params.a2 = phase in percent of 2 pi) over the same
+ * share of the uniform walk. */
+void calculate_eigen(int min, int max, shape_params params, tetrapyd_limits* block, double* alpha);
 
 #ifdef __cplusplus
 }

@@ -330,6 +330,9 @@ void lspace_slice(const PointSet& L, int kc, int rank, int nranks, int* row_lo, 
     if (frac) *frac = tot ? (double)(pre[*row_hi] - pre[*row_lo]) / (double)tot : 0.0;
 }
 
+// uniform k-grid weights (kuniform.hip)
+int uniform_weights_launch(mcmb_ctx* c, PointSet& L, const long* rowptr, double* ws, int scaled);
+
 static int ensure_lspace_aux(mcmb_ctx* c, PointSet& L, DevBuf& rowptr, DevBuf& ws, int scaled)
 {
     std::vector<long> rp(L.rows.size() + 1, 0);
@@ -339,6 +342,8 @@ static int ensure_lspace_aux(mcmb_ctx* c, PointSet& L, DevBuf& rowptr, DevBuf& w
     MCMB_TRY(ws.alloc(sizeof(double) * (size_t)L.npoints));
     long nrows = (long)L.rows.size();
     dim3 grid((unsigned)((nrows + 3) / 4));
+    if (L.uniform)
+        return uniform_weights_launch(c, L, rowptr.as<long>(), ws.as<double>(), scaled);
     if (L.kspace)
         hipLaunchKernelGGL(kspace_weights_kernel, grid, dim3(256), 0, c->stream, L.d_rows.as<RowDesc>(),
                            rowptr.as<long>(), nrows, L.d_kn.as<double>(), L.d_kw.as<double>(), scaled,
@@ -353,6 +358,11 @@ static int ensure_lspace_aux(mcmb_ctx* c, PointSet& L, DevBuf& rowptr, DevBuf& w
 
 int gram_mixed_launch(mcmb_ctx* c, PointSet& L, const long* d_rowptr, const double* d_ws, double* d_gamma,
                       int row_lo, int row_hi);
+
+int pointset_weights(mcmb_ctx* c, PointSet& P, DevBuf& rowptr, DevBuf& ws, int scaled)
+{
+    return ensure_lspace_aux(c, P, rowptr, ws, scaled);
+}
 
 int gram_tet_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gamma, int row_lo, int row_hi)
 {

@@ -14,6 +14,7 @@ static double* g_gamma3d = NULL; /* cached FP64 tetrapyd Gram, n x n */
 static double* g_gammagl = NULL; /* cached GL-factorised Gram, n x n */
 static int g_n = 0;
 static long g_gen = -1;          /* l-space generation the caches belong to */
+static int g_prim_nblocks = 1;   /* thread count of the last decompose_tetrapyd_prim */
 
 int mcmb_legacy_bind(mcmb_ctx* ctx)
 {
@@ -164,6 +165,7 @@ void check_ijk_max_prim(int max, int* pi, int* pj, int* pk)
 /* tetrapyd_tools.c:416-483 */
 void decompose_tetrapyd_prim(tetrapyd_limits* blk, int rank, int nranks, int nthreads, int min, int max)
 {
+    g_prim_nblocks = nthreads > 0 ? nthreads : 1;
     int i0 = min, j0 = min, k0 = min;
     check_ijk_min_prim(min, &i0, &j0, &k0);
     int i = i0, j = j0, k = k0;
@@ -297,4 +299,76 @@ int calculate_orthol(int m, int n, double* result)
     if (g_status != MCMB_OK || m < 0 || n < 0 || m >= g_n || n >= g_n) { result[0] = 0.0; return g_status ? g_status : MCMB_ERR_ARG; }
     result[0] = g_gammagl[(size_t)m * g_n + n];
     return 0;
+}
+
+/* Index of (i, j, k) in the prim walk from (min, min, min): rows (i, j) in
+ * order hold k in [j, min(max, i+j+2)]. */
+static long prim_walk_index(int min, int max, int i, int j, int k)
+{
+    long p = 0;
+    for (int a = min; a <= i; ++a)
+        for (int b = a; b <= max; ++b) {
+            if (a == i && b == j) return p + (k - j);
+            int khi = a + b + 2 < max ? a + b + 2 : max;
+            p += khi - b + 1;
+        }
+    return -1;
+}
+
+/* Point range of the blocks of the last decompose_tetrapyd_prim: block[0]'s
+ * start and the loops of all its threads (one contiguous share of the walk). */
+static int prim_block_range(int min, int max, const tetrapyd_limits* block, long* p0, long* p1)
+{
+    int umin, umax, n;
+    double kmax;
+    if (!g_ctx) return MCMB_ERR_STATE;
+    int st = mcmb_kuniform_info(g_ctx, &umin, &umax, &n, &kmax, NULL);
+    if (st != MCMB_OK) return st;
+    if (umin != min || umax != max) return MCMB_ERR_ARG;
+    *p0 = prim_walk_index(min, max, block[0].i_bgn, block[0].j_bgn, block[0].k_bgn);
+    if (*p0 < 0) return MCMB_ERR_ARG;
+    long loops = 0;
+    for (int t = 0; t < g_prim_nblocks; ++t) loops += block[t].loops;
+    *p1 = *p0 + loops;
+    return MCMB_OK;
+}
+
+/* orthogonalisation.c:14 -- adds the Gram of the blocks' share of the
+ * uniform-grid walk into the packed upper triangle ortho[n(n+1)/2] (row
+ * major, r <= s), integrating every block (single-thread semantics). */
+void calculate_ortho(int min, int max, tetrapyd_limits* block, double* ortho)
+{
+    long p0, p1;
+    int n;
+    g_status = prim_block_range(min, max, block, &p0, &p1);
+    if (g_status != MCMB_OK) return;
+    mcmb_kuniform_info(g_ctx, NULL, NULL, &n, NULL, NULL);
+    double* g = (double*)malloc(sizeof(double) * (size_t)n * n);
+    if (!g) { g_status = MCMB_ERR_NOMEM; return; }
+    g_status = mcmb_kuniform_gram(g_ctx, MCMB_FP64, p0, p1, g, NULL);
+    if (g_status == MCMB_OK) {
+        long m = 0;
+        for (int r = 0; r < n; ++r)
+            for (int c = r; c < n; ++c) ortho[m++] += g[(long)r * n + c];
+    }
+    free(g);
+}
+
+/* calculate_eigen.c:86 -- adds alpha of the feature shape (model 4;
+ * params.a1 = omega, params.a2 = phase in percent of 2 pi as
+ * calculate_shape.c:933-946 reads it) over the blocks' share of the walk. */
+void calculate_eigen(int min, int max, shape_params params, tetrapyd_limits* block, double* alpha)
+{
+    long p0, p1;
+    int n;
+    g_status = prim_block_range(min, max, block, &p0, &p1);
+    if (g_status != MCMB_OK) return;
+    mcmb_kuniform_info(g_ctx, NULL, NULL, &n, NULL, NULL);
+    double* a = (double*)malloc(sizeof(double) * (size_t)n);
+    if (!a) { g_status = MCMB_ERR_NOMEM; return; }
+    double theta[2] = {params.a1, 2.0 * 3.1415927 * params.a2 * 1e-2};
+    g_status = mcmb_kuniform_project(g_ctx, 1, theta, p0, p1, a);
+    if (g_status == MCMB_OK)
+        for (int r = 0; r < n; ++r) alpha[r] += a[r];
+    free(a);
 }

@@ -70,6 +70,8 @@ struct PointSet {
     std::vector<double> q;            // [(lmax+1)][ldq]
     std::vector<double> v;            // l-space per-leg weight [lmax+1]
     std::vector<double> kn, kw;       // k-space nodes and weights [N]
+    bool uniform = false;             // uniform k-grid (calculate_weight weights)
+    double ukmax = 0.0;               // uniform grid: k = ukmax * i / lmax
     DevBuf d_rows, d_q, d_qf, d_v, d_ftab, d_modes, d_kn, d_kw;
 };
 typedef PointSet LSpace;
@@ -84,6 +86,7 @@ struct mcmb_ctx {
     int last_launches = 0;
     mcmb::PointSet ls;   // late-time multipole tetrapyd
     mcmb::PointSet ks;   // primordial k-space Gauss-Legendre tetrapyd
+    mcmb::PointSet ku;   // primordial k-space uniform-grid tetrapyd (prim walk)
     mcmb::DevBuf d_W;    // full-cube projection weight W[i][j][k] (k-space)
     int w_ready = 0;
     // scratch for partial Gram tiles
@@ -101,4 +104,7 @@ void lspace_slice(const PointSet& L, int kc, int rank, int nranks, int* row_lo, 
 int upload_pointset(mcmb_ctx* ctx, PointSet& P, const std::vector<int>& order);
 int gram_gl_launch(mcmb_ctx* ctx, int npt, double* d_gamma);
 int set_device(mcmb_ctx* ctx);
+// Per-point weights of P (walk order) into ws, scaled = sqrt(w)/6 if nonzero;
+// rowptr[r] = first point of row r.  Weights by the point set's kind.
+int pointset_weights(mcmb_ctx* ctx, PointSet& P, DevBuf& rowptr, DevBuf& ws, int scaled);
 }  // namespace mcmb

@@ -68,6 +68,15 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_orthonormalise.argtypes = [vp, ip, vp, vp, vp, vp, C.POINTER(ip)]
         lib.mcmb_cholesky_solve.argtypes = [vp, ip, vp, ip, vp, vp]
         lib.mcmb_eigen_sym.argtypes = [vp, ip, vp, vp, vp, C.POINTER(ip)]
+        lib.mcmb_kuniform_set.argtypes = [vp, ip, ip, dp, ip, vp, ip, vp]
+        lib.mcmb_kuniform_info.argtypes = [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip), C.POINTER(dp),
+                                           C.POINTER(C.c_long)]
+        lib.mcmb_kuniform_npoints.argtypes = [vp]
+        lib.mcmb_kuniform_npoints.restype = C.c_long
+        lib.mcmb_kuniform_weights.argtypes = [vp, vp, C.c_long]
+        lib.mcmb_kuniform_weights.restype = C.c_long
+        lib.mcmb_kuniform_gram.argtypes = [vp, ip, C.c_long, C.c_long, vp, vp]
+        lib.mcmb_kuniform_project.argtypes = [vp, ip, vp, C.c_long, C.c_long, vp]
         lib.mcmb_orthonormalise_late.argtypes = [vp, ip, vp, ip, vp, vp, vp, vp, C.POINTER(ip)]
         for name in ("get_ijk_next_XXX", "get_ijk_next_prim"):
             getattr(lib, name).argtypes = [ip, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip)]
@@ -93,6 +102,8 @@ EXPORTED_SYMBOLS = (
     "mcmb_last_kernel_ms", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
     "mcmb_gauss_legendre", "mcmb_gamma_tet_slice", "mcmb_kspace_set", "mcmb_kspace_npoints",
     "mcmb_kgram", "mcmb_project_fwd", "mcmb_project_bwd", "mcmb_eigen_sym", "mcmb_orthonormalise_late",
+    "mcmb_kuniform_set", "mcmb_kuniform_info", "mcmb_kuniform_npoints", "mcmb_kuniform_weights",
+    "mcmb_kuniform_gram", "mcmb_kuniform_project", "calculate_ortho", "calculate_eigen",
     "get_ijk_next_XXX", "check_ijk_min_XXX", "check_ijk_max_XXX", "decompose_tetrapyd_XXX",
     "get_ijk_next_prim", "check_ijk_min_prim", "check_ijk_max_prim", "decompose_tetrapyd_prim",
     "divide_tasks", "get_next_ijk_bound", "permsix", "calculate_geometric",
@@ -281,6 +292,39 @@ class Context:
             raise NotSPDError(st, _lib.mcmb_last_error().decode())
         _check(st)
         return r.value, lam, linv, norm, ev
+
+    # ---- uniform k-grid (calculate_ortho / calculate_eigen) -------------------
+    def kuniform_set(self, mn: int, mx: int, kmax: float, q: np.ndarray, order: np.ndarray) -> None:
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        order = np.ascontiguousarray(order, dtype=np.int32).reshape(-1, 3)
+        self._ukeep = (q, order)
+        _check(_lib.mcmb_kuniform_set(self._h, mn, mx, kmax, q.shape[1], _ptr(q), order.shape[0], _ptr(order)))
+        self.nu = order.shape[0]
+
+    def kuniform_npoints(self) -> int:
+        r = _lib.mcmb_kuniform_npoints(self._h)
+        if r < 0:
+            _check(int(r))
+        return int(r)
+
+    def kuniform_weights(self) -> np.ndarray:
+        n = self.kuniform_npoints()
+        w = np.empty(n)
+        r = _lib.mcmb_kuniform_weights(self._h, _ptr(w), n)
+        if r < 0:
+            _check(int(r))
+        return w
+
+    def kuniform_gram(self, precision: int = FP64, p0: int = 0, p1: int = -1) -> np.ndarray:
+        g = np.empty((self.nu, self.nu))
+        _check(_lib.mcmb_kuniform_gram(self._h, precision, p0, p1, _ptr(g), None))
+        return g
+
+    def kuniform_project(self, theta: np.ndarray, p0: int = 0, p1: int = -1) -> np.ndarray:
+        th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64).reshape(-1, 2))
+        a = np.empty((th.shape[0], self.nu))
+        _check(_lib.mcmb_kuniform_project(self._h, th.shape[0], _ptr(th), p0, p1, _ptr(a)))
+        return a
 
     def cholesky_solve(self, ortho: np.ndarray, alpha: np.ndarray) -> np.ndarray:
         f = np.ascontiguousarray(ortho, dtype=np.float64)

@@ -88,6 +88,7 @@ struct mcmb_ctx {
     mcmb::PointSet ks;   // primordial k-space Gauss-Legendre tetrapyd
     mcmb::PointSet ku;   // primordial k-space uniform-grid tetrapyd (prim walk)
     mcmb::DevBuf d_W;    // full-cube projection weight W[i][j][k] (k-space)
+    mcmb::DevBuf pz, py, ph, pd;   // projection batch buffers (Z, Y, H, partials), kept between calls
     int w_ready = 0;
     // scratch for partial Gram tiles
     mcmb::DevBuf d_partial;

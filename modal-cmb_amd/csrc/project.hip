@@ -38,19 +38,26 @@ constexpr int PBATCH = 32;  // shapes per batch
 // ---- full-cube weight ------------------------------------------------------
 // W[(i*N + j)*Kp + k]; the mask compares the sorted triple exactly as the
 // oracle does: k_c <= k_a + k_b (FP64).
+// Row of the pair (i, j) in the packed i <= j order of the stage-1 product.
+__device__ __forceinline__ long pair_row(int i, int j, int N)
+{
+    const int lo = min(i, j), hi = max(i, j);
+    return (long)lo * N - (long)lo * (lo - 1) / 2 + (hi - lo);
+}
+
 template <class T>
 __global__ void proj_build_W(const double* __restrict__ kn, const double* __restrict__ kw, int N, int Kp,
-                             T* __restrict__ W)
+                             const int2* __restrict__ rowij, long M, T* __restrict__ W)
 {
+    // W[(i,j) pair row][k], pair rows i <= j only (W is symmetric in i, j)
     size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t tot = (size_t)N * N * Kp;
+    size_t tot = (size_t)M * Kp;
     if (e >= tot) return;
     int k = (int)(e % Kp);
-    size_t ij = e / Kp;
-    int j = (int)(ij % N), i = (int)(ij / N);
+    const int2 ij = rowij[e / Kp];
     double w = 0.0;
     if (k < N) {
-        int a = i, b = j, c = k, t;
+        int a = ij.x, b = ij.y, c = k, t;
         if (a > b) { t = a; a = b; b = t; }
         if (b > c) { t = b; b = c; c = t; if (a > b) { t = a; a = b; b = t; } }
         if (kn[c] <= kn[a] + kn[b]) w = kw[a] * kw[b] * kw[c] / (kn[a] + kn[b] + kn[c]);
@@ -176,57 +183,73 @@ __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int 
                                                    const double* __restrict__ q, int ldq, int N, int R,
                                                    const double* __restrict__ theta, int nb, double* __restrict__ H)
 {
-    __shared__ double G[3][32][33];     // [x][j chunk][t]
-    __shared__ double Qs[32][33];       // q_s(j) for the chunk
+    // one workgroup per (shape b, node i); H_x[s][t] = sum_j q_s(j) G_x[j][t].
+    // Thread (sb, t): s = 4 sb .. 4 sb + 3, all three x: 12 FP64 accumulators
+    // from 5 LDS loads per j.  sincos of A_ij once per j.
+    __shared__ double G[3][32][32];     // [x][j in chunk][t]
+    __shared__ __align__(16) double Qs[32][32];   // [j in chunk][s]
+    __shared__ double sc[32][3];        // sin A, cos A, k_i + k_j
     const int b = blockIdx.x % nb, i = blockIdx.x / nb;
     const double om = theta[2 * b], ph = theta[2 * b + 1];
-    const int tid = threadIdx.x;
-    // thread owns up to 12 (x, s, t) accumulators: e = tid + 256 u over 3 R^2 (R <= 32)
-    const int nacc = 3 * R * R;
-    double acc[12];
+    const int tid = threadIdx.x, t = tid & 31, s0 = (tid >> 5) * 4;
+    const double ki = kn[i];
+    double acc[3][4];
 #pragma unroll
-    for (int u = 0; u < 12; ++u) acc[u] = 0.0;
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[x][u] = 0.0;
     for (int j0 = 0; j0 < N; j0 += 32) {
         __syncthreads();
-        for (int e = tid; e < 32 * R; e += 256) {
-            int jj = e / R, t = e % R, j = j0 + jj;
-            double g0 = 0, g1 = 0, g2 = 0, qq = 0;
+        if (tid < 32) {
+            const int j = j0 + tid;
+            double sA = 0.0, cA = 0.0, kij = 0.0;
             if (j < N) {
-                const T* y = Y + (size_t)(i * N + j) * ldy + (size_t)b * 4 * R;
-                double yc = (double)y[t], ys = (double)y[R + t], ycp = (double)y[2 * R + t], ysp = (double)y[3 * R + t];
-                double sA, cA;
-                sincos(om * (kn[i] + kn[j]) + ph, &sA, &cA);
+                kij = ki + kn[j];
+                sincos(om * kij + ph, &sA, &cA);
+            }
+            sc[tid][0] = sA;
+            sc[tid][1] = cA;
+            sc[tid][2] = kij;
+        }
+        __syncthreads();
+        for (int e = tid; e < 32 * 32; e += 256) {
+            const int jj = e >> 5, tt = e & 31, j = j0 + jj;
+            double g0 = 0, g1 = 0, g2 = 0, qq = 0;
+            if (j < N && tt < R) {
+                const T* y = Y + (size_t)pair_row(i, j, N) * ldy + (size_t)b * 4 * R;
+                const double yc = (double)y[tt], ys = (double)y[R + tt];
+                const double ycp = (double)y[2 * R + tt], ysp = (double)y[3 * R + tt];
+                const double sA = sc[jj][0], cA = sc[jj][1];
                 g0 = sA * yc + cA * ys;
                 g2 = cA * yc - sA * ys;
-                g1 = (kn[i] + kn[j]) * g2 + cA * ycp - sA * ysp;
-                qq = q[(size_t)j * ldq + t];
+                g1 = sc[jj][2] * g2 + cA * ycp - sA * ysp;
+                qq = q[(size_t)j * ldq + tt];
             }
-            G[0][jj][t] = g0;
-            G[1][jj][t] = g1;
-            G[2][jj][t] = g2;
-            Qs[jj][t] = qq;
+            G[0][jj][tt] = g0;
+            G[1][jj][tt] = g1;
+            G[2][jj][tt] = g2;
+            Qs[jj][tt] = qq;
         }
         __syncthreads();
         const int jn = min(32, N - j0);
+        for (int jj = 0; jj < jn; ++jj) {
+            const double2 qa = *reinterpret_cast<const double2*>(&Qs[jj][s0]);
+            const double2 qb = *reinterpret_cast<const double2*>(&Qs[jj][s0 + 2]);
+            const double qv[4] = {qa.x, qa.y, qb.x, qb.y};
 #pragma unroll
-        for (int u = 0; u < 12; ++u) {
-            int e = tid + 256 * u;
-            if (e < nacc) {
-                int x = e / (R * R), st = e % (R * R), s = st / R, t = st % R;
-                double a = acc[u];
-                for (int jj = 0; jj < jn; ++jj) a += Qs[jj][s] * G[x][jj][t];
-                acc[u] = a;
+            for (int x = 0; x < 3; ++x) {
+                const double g = G[x][jj][t];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) acc[x][u] += qv[u] * g;
             }
         }
     }
+    if (t < R)
 #pragma unroll
-    for (int u = 0; u < 12; ++u) {
-        int e = tid + 256 * u;
-        if (e < nacc) {
-            int x = e / (R * R), st = e % (R * R);
-            H[(((size_t)b * 3 + x) * N + i) * R * R + st] = acc[u];
-        }
-    }
+        for (int x = 0; x < 3; ++x)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (s0 + u < R) H[(((size_t)b * 3 + x) * N + i) * R * R + (s0 + u) * R + t] = acc[x][u];
 }
 
 // alpha_x[b][n] = sum_i q_{r_n}(i) H[b][x][i][s_n][t_n]
@@ -269,26 +292,65 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
                                                     const double* __restrict__ theta, int nb,
                                                     const double* __restrict__ Hb, double* __restrict__ dpart)
 {
-    __shared__ double Hs[32 * 32];
+    // one workgroup per (shape b, node i): Gbar[j][t] = sum_s q_s(j) Hbar[s][t],
+    // then d/domega += Gbar G1, d/dphi += Gbar G2 over (j, t).  Thread (jb, t):
+    // j = 4 jb .. 4 jb + 3 of the 32-node chunk, register tile over s.
+    __shared__ double Hs[32][32];                  // [s][t]
+    __shared__ __align__(16) double Qs[32][32];    // [s][j in chunk]
+    __shared__ double sc[32][3];
     __shared__ double red[2][256];
     const int b = blockIdx.x % nb, i = blockIdx.x / nb;
     const double om = theta[2 * b], ph = theta[2 * b + 1];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < R * R; e += 256) Hs[e] = Hb[(((size_t)b * N) + i) * R * R + e];
-    __syncthreads();
+    const int tid = threadIdx.x, t = tid & 31, j4 = (tid >> 5) * 4;
+    const double ki = kn[i];
+    for (int e = tid; e < 32 * 32; e += 256) {
+        const int ss = e >> 5, tt = e & 31;
+        Hs[ss][tt] = (ss < R && tt < R) ? Hb[(((size_t)b * N) + i) * R * R + ss * R + tt] : 0.0;
+    }
     double dom = 0.0, dph = 0.0;
-    for (int e = tid; e < N * R; e += 256) {
-        int j = e / R, t = e % R;
-        double gb = 0.0;
-        for (int s = 0; s < R; ++s) gb += q[(size_t)j * ldq + s] * Hs[s * R + t];
-        const T* y = Y + (size_t)(i * N + j) * ldy + (size_t)b * 4 * R;
-        double yc = (double)y[t], ys = (double)y[R + t], ycp = (double)y[2 * R + t], ysp = (double)y[3 * R + t];
-        double sA, cA;
-        sincos(om * (kn[i] + kn[j]) + ph, &sA, &cA);
-        double g2 = cA * yc - sA * ys;
-        double g1 = (kn[i] + kn[j]) * g2 + cA * ycp - sA * ysp;
-        dom += gb * g1;
-        dph += gb * g2;
+    for (int j0 = 0; j0 < N; j0 += 32) {
+        __syncthreads();
+        if (tid < 32) {
+            const int j = j0 + tid;
+            double sA = 0.0, cA = 0.0, kij = 0.0;
+            if (j < N) {
+                kij = ki + kn[j];
+                sincos(om * kij + ph, &sA, &cA);
+            }
+            sc[tid][0] = sA;
+            sc[tid][1] = cA;
+            sc[tid][2] = kij;
+        }
+        for (int e = tid; e < 32 * 32; e += 256) {
+            const int ss = e >> 5, jj = e & 31, j = j0 + jj;
+            Qs[ss][jj] = (ss < R && j < N) ? q[(size_t)j * ldq + ss] : 0.0;
+        }
+        __syncthreads();
+        double gb[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int ss = 0; ss < R; ++ss) {
+            const double h = Hs[ss][t];
+            const double2 qa = *reinterpret_cast<const double2*>(&Qs[ss][j4]);
+            const double2 qb = *reinterpret_cast<const double2*>(&Qs[ss][j4 + 2]);
+            gb[0] += qa.x * h;
+            gb[1] += qa.y * h;
+            gb[2] += qb.x * h;
+            gb[3] += qb.y * h;
+        }
+        if (t < R)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int jj = j4 + u, j = j0 + jj;
+                if (j < N) {
+                    const T* y = Y + (size_t)pair_row(i, j, N) * ldy + (size_t)b * 4 * R;
+                    const double yc = (double)y[t], ys = (double)y[R + t];
+                    const double ycp = (double)y[2 * R + t], ysp = (double)y[3 * R + t];
+                    const double sA = sc[jj][0], cA = sc[jj][1];
+                    const double g2 = cA * yc - sA * ys;
+                    const double g1 = sc[jj][2] * g2 + cA * ycp - sA * ysp;
+                    dom += gb[u] * g1;
+                    dph += gb[u] * g2;
+                }
+            }
     }
     red[0][tid] = dom;
     red[1][tid] = dph;
@@ -321,36 +383,49 @@ static int ensure_W(mcmb_ctx* c, int prec)
     PointSet& K = c->ks;
     const int N = K.lmax + 1, Kp = (N + PK - 1) / PK * PK;
     if (c->w_ready == prec + 1) return MCMB_OK;
-    size_t tot = (size_t)N * N * Kp;
+    const long M = (long)N * (N + 1) / 2;
+    std::vector<int2> rowij;
+    rowij.reserve(M);
+    for (int i = 0; i < N; ++i)
+        for (int j = i; j < N; ++j) rowij.push_back(make_int2(i, j));
+    DevBuf d_ij;
+    MCMB_TRY(d_ij.alloc(sizeof(int2) * (size_t)M));
+    MCMB_HIP(hipMemcpyAsync(d_ij.p, rowij.data(), sizeof(int2) * (size_t)M, hipMemcpyHostToDevice, c->stream));
+    size_t tot = (size_t)M * Kp;
     MCMB_TRY(launch_guard("projection weight cube", (tot + 255) / 256, 256));
     if (prec == MCMB_FP64) {
         MCMB_TRY(c->d_W.alloc(tot * sizeof(double)));
         hipLaunchKernelGGL(proj_build_W<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
-                           K.d_kn.as<double>(), K.d_kw.as<double>(), N, Kp, c->d_W.as<double>());
+                           K.d_kn.as<double>(), K.d_kw.as<double>(), N, Kp, d_ij.as<int2>(), M,
+                           c->d_W.as<double>());
     } else {
         MCMB_TRY(c->d_W.alloc(tot * sizeof(float)));
         hipLaunchKernelGGL(proj_build_W<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
-                           K.d_kn.as<double>(), K.d_kw.as<double>(), N, Kp, c->d_W.as<float>());
+                           K.d_kn.as<double>(), K.d_kw.as<double>(), N, Kp, d_ij.as<int2>(), M,
+                           c->d_W.as<float>());
     }
     MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipStreamSynchronize(c->stream));   // the pair table goes out of scope
     c->w_ready = prec + 1;
     return MCMB_OK;
 }
 
 template <class T>
-static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, DevBuf& Zb, DevBuf& Yb, int& ldy)
+static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, int& ldy)
 {
+    DevBuf& Zb = c->pz;
+    DevBuf& Yb = c->py;
     PointSet& K = c->ks;
     const int N = K.lmax + 1, R = K.R, Kp = (N + PK - 1) / PK * PK;
     const int Nc = nb * 4 * R;
     const int ldz = (Nc + PT - 1) / PT * PT;
     ldy = ldz;
     MCMB_TRY(Zb.alloc(sizeof(T) * (size_t)Kp * ldz));
-    MCMB_TRY(Yb.alloc(sizeof(T) * (size_t)N * N * ldy));
+    const long M = (long)N * (N + 1) / 2;   // pair rows i <= j
+    MCMB_TRY(Yb.alloc(sizeof(T) * (size_t)M * ldy));
     size_t ze = (size_t)Kp * ldz;
     hipLaunchKernelGGL(proj_build_Z<T>, dim3((unsigned)((ze + 255) / 256)), dim3(256), 0, c->stream,
                        K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb, Kp, ldz, Zb.as<T>());
-    const int M = N * N;
     dim3 grid((unsigned)(ldz / PT), (unsigned)((M + PT - 1) / PT));
     hipLaunchKernelGGL(proj_gemm<T>, grid, dim3(256), 0, c->stream, c->d_W.as<T>(), Zb.as<T>(), M, Kp, Nc, ldz,
                        Yb.as<T>(), ldy);
@@ -363,10 +438,11 @@ static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double*
 {
     PointSet& K = c->ks;
     const int N = K.lmax + 1, R = K.R, n = K.n;
-    DevBuf Zb, Yb, Hb;
+    DevBuf& Yb = c->py;
+    DevBuf& Hb = c->ph;
     int ldy = 0;
-    MCMB_TRY(run_stage1<T>(c, d_theta, nb, Zb, Yb, ldy));
-    MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)nb * 3 * N * R * R));
+    MCMB_TRY(run_stage1<T>(c, d_theta, nb, ldy));
+    MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)PBATCH * 3 * N * R * R));
     hipLaunchKernelGGL(proj_stage2<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
                        K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb, Hb.as<double>());
     const int ne = nb * 3 * n;
@@ -383,11 +459,13 @@ static int project_bwd_batch(mcmb_ctx* c, const double* d_theta, int nb, const d
 {
     PointSet& K = c->ks;
     const int N = K.lmax + 1, R = K.R, n = K.n;
-    DevBuf Zb, Yb, Hb, Dp;
+    DevBuf& Yb = c->py;
+    DevBuf& Hb = c->ph;
+    DevBuf& Dp = c->pd;
     int ldy = 0;
-    MCMB_TRY(run_stage1<T>(c, d_theta, nb, Zb, Yb, ldy));
-    MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)nb * N * R * R));
-    MCMB_TRY(Dp.alloc(sizeof(double) * (size_t)nb * N * 2));
+    MCMB_TRY(run_stage1<T>(c, d_theta, nb, ldy));
+    MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)PBATCH * 3 * N * R * R));
+    MCMB_TRY(Dp.alloc(sizeof(double) * (size_t)PBATCH * N * 2));
     size_t he = (size_t)nb * N * R * R;
     hipLaunchKernelGGL(proj_hbar, dim3((unsigned)((he + 255) / 256)), dim3(256), 0, c->stream, d_gbar, d_slot,
                        K.d_q.as<double>(), K.ldq, N, R, n, nb, Hb.as<double>());

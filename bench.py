@@ -100,28 +100,33 @@ def cpu_baseline(ws, L, target_s):
         O.gamma_tet(2, L, ws["q"], ws["v"], ws["order"], 0, i_lo, i_hi)
         return time.time() - t
 
-    def rows_for(points):
-        """Lowest i such that rows i..L hold about `points` points."""
+    def rows_from_top(points, min_rows):
+        """Lowest i such that i-rows i..L hold about `points` points (at least min_rows rows)."""
         i, acc = L + 1, 0
-        while i > 2 and acc + cnt[i - 1] <= points:
+        while i > 2 and (acc + cnt[i - 1] <= points or L + 1 - i < min_rows):
             i -= 1
             acc += int(cnt[i])
-        return min(i, L - 15)
+        return i
 
-    # probe on ~2 s of work (rate from a first small guess), then size the sample
-    i_lo = rows_for(max(1, int(2.0 * 1e9 / (2.0 * n * n))))
+    # fixed cost of one call (per-thread partial Grams, their reduction),
+    # paid once by a whole Gamma: timed on an empty row range
+    t_fixed = run(L + 1, L + 1)
+    min_rows = 4 * max(16, nthreads)
+    i_lo = rows_from_top(0, min_rows)
     pts = int(cnt[i_lo:].sum())
     t_probe = run(i_lo, L + 1)
-    i_lo2 = rows_for(int(pts / t_probe * target_s))
+    rate = pts / max(t_probe - t_fixed, 1e-3)
+    i_lo2 = rows_from_top(int(rate * target_s), min_rows)
     pts2 = int(cnt[i_lo2:].sum())
     t2 = run(i_lo2, L + 1)
     return {
-        "value": t2 * total / pts2,
+        "value": t_fixed + (t2 - t_fixed) * total / pts2,
         "unit": "s per Gamma (extrapolated)",
         "cores": int(nthreads),
         "kind": "port",
         "sample": f"oracle direct tetrapyd FP64 Gram, i-rows {i_lo2}..{L} of T(2,{L}): {pts2} of {total} points "
-                  f"({t2:.1f} s), scaled by point count; n={n}",
+                  f"({t2:.1f} s, of which {t_fixed:.1f} s fixed per-call cost), the point-dependent part scaled "
+                  f"by point count; n={n}",
     }
 
 
@@ -164,9 +169,11 @@ def bench_projection(modalcmb, W, args):
     def rel(x, y):
         return float(np.abs(x - y).max() / np.abs(y).max())
 
-    # stage-1 contraction flops per shape: 2 N^3 (4R) (cube x 4 angle/basis
-    # columns per basis index), forward; the adjoint reuses it
+    # stage-1 contraction flops per shape: 2 (N(N+1)/2) Kp (4R): the masked
+    # cube over pair rows i <= j (W is symmetric in i, j) times the four
+    # angle / basis column sets per basis index; the adjoint repeats it
     N, R = cfg["N"], cfg["R"]
+    Kp = (N + 15) // 16 * 16
     out = {
         "workload": f"{B} shapes S=sin(omega K + phi)/(k1k2k3)^2 form, GL {N}/side (k in [2e-4,0.2]), "
                     f"{n} modes (r<=s<=t<{R}), masked cube {npts} ordered nodes",
@@ -177,7 +184,8 @@ def bench_projection(modalcmb, W, args):
         "grads_per_s_fwd_plus_adjoint": round(2 * B / (t_fwd + t_bwd), 1),
         "err_fp32_vs_fp64": {"alpha": rel(a32, a64), "dalpha_dtheta": rel(d32, d64), "adjoint": rel(g32, g64),
                              "norm": "max |x32 - x64| / max |x64| over the batch"},
-        "fwd_stage1_tflops": round(2.0 * N ** 3 * 4 * R * B / (k_fwd * 1e-3) / 1e12, 2),
+        "fwd_stage1_flops": 2.0 * (N * (N + 1) // 2) * Kp * 4 * R * B,
+        "fwd_tflops_stage1_over_fwd_time": round(2.0 * (N * (N + 1) // 2) * Kp * 4 * R * B / (k_fwd * 1e-3) / 1e12, 2),
         "fp32_mfma_peak_tflops": PEAK_FP32_TFLOPS,
     }
     if not args.no_cpu:

@@ -192,6 +192,32 @@ int mcmb_kuniform_gram(mcmb_ctx* ctx, int precision, long p0, long p1, double* g
 int mcmb_kuniform_project(mcmb_ctx* ctx, int nshape, const double* theta, long p0, long p1,
                           double* alpha);
 
+/* Read-only views of the host tables of the uniform k-grid / l-space setups
+ * (valid until the next mcmb_kuniform_set / mcmb_lspace_set; outputs may be
+ * NULL).  No GPU work. */
+int mcmb_kuniform_tables(mcmb_ctx* ctx, const double** q, int* ldq, const int** order, int* nmodes,
+                         int* max);
+int mcmb_lspace_tables(mcmb_ctx* ctx, const double** q, int* ldq, const int** order, int* nmodes,
+                       int* lmin, int* lmax);
+
+/* ---- Gamma post-processing (gamma_mpi.c:260-295, 801-840) --------------- */
+/* orthoinv = lambda^T lambda (orthoinv[i][j] = sum_k lambda[k][i] lambda[k][j])
+ * and gamma_out = orthoinv Gamma, FP64 on the GPU; n x n row-major host
+ * arrays, orthoinv may be NULL. */
+int mcmb_gamma_post(mcmb_ctx* ctx, int n, const double* lambda, const double* gamma, double* orthoinv,
+                    double* gamma_out);
+
+/* ---- file layouts (host only) --------------------------------------------- */
+/* array_write / array_read (arrays.c:6-21, 54-67) as used by output_gamma,
+ * output_ortho, output_lambda, output_eigen (fixed_data.c:3077-3091,
+ * 3297-3310, 3588-3601, 2899-2928): raw native-endian FP64, no header.
+ * Errors (MCMB_ERR_IO, including a short read) are returned, not exits. */
+int mcmb_write_raw(const char* path, long count, const double* data);
+int mcmb_read_raw(const char* path, long count, double* data);
+/* Mode order table as load_three_int reads it (io.c:229-258): one triple per
+ * line, '#' lines skipped; at most cap triples into order[3*cap]. */
+int mcmb_load_order(const char* path, int* order, int cap, int* nmodes);
+
 /* ---- quadrature ----------------------------------------------------------- */
 /* n-node Gauss-Legendre rule on [-1,1], nodes ascending (host only, no GPU). */
 int mcmb_gauss_legendre(int n, double* x, double* w);
@@ -258,6 +284,25 @@ void calculate_ortho(int min, int max, tetrapyd_limits* block, double* ortho);
  * params.a1 = omega, params.a2 = phase in percent of 2 pi) over the same
  * share of the uniform walk. */
 void calculate_eigen(int min, int max, shape_params params, tetrapyd_limits* block, double* alpha);
+/* orthogonalisation.c:279 -- pijk of the bound uniform k-grid setup */
+double pijk(int r, int i, int j, int k);
+/* orthogonalisation_late.c:620 -- plijk_TTT of the bound l-space setup
+ * (basis row = multipole; 0 outside [lmin, lmax]) */
+double plijk_TTT(int r, int l1, int l2, int l3);
+/* fixed_data.c:263 */
+void find_perm_prim(int r, int* p1, int* p2, int* p3);
+/* global.h:429, 730 -- of the bound uniform k-grid setup (0 if unset) */
+int get_terms_prim(void);
+double get_kmax(void);
+/* calculate_shape.c:26, 197 -- the feature shape (model 4) only */
+double shape(double k1, double k2, double k3, shape_params params);
+double shape3(double k1, double k2, double k3, shape_params params);
+/* fixed_data.c:1670 -- Legendre orders: rebuilds the bound uniform k-grid
+ * setup on size points of [0, max] with basis P_p(2 vec[i]/max - 1), keeping
+ * its order table; returns 0 or a negative status. */
+int create_basis_prim(int size, double max, double* vec);
+/* io.c:229 -- 1 on success, 0 if the file cannot be read */
+int load_three_int(char* filename, int* values, int* size);
 
 #ifdef __cplusplus
 }

@@ -281,3 +281,17 @@ int mcmb_gamma_tet_slice(mcmb_ctx* c, int precision, int rank, int nranks, doubl
 }
 
 }  // extern "C"
+
+extern "C" int mcmb_lspace_tables(mcmb_ctx* c, const double** q, int* ldq, const int** order, int* nmodes,
+                                  int* lmin, int* lmax)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_lspace_tables: null context");
+    if (!c->ls.ready) return fail(MCMB_ERR_STATE, "mcmb_lspace_tables: call mcmb_lspace_set first");
+    if (q) *q = c->ls.q.data();
+    if (ldq) *ldq = c->ls.ldq;
+    if (order) *order = c->ls.order.data();
+    if (nmodes) *nmodes = c->ls.n;
+    if (lmin) *lmin = c->ls.lmin;
+    if (lmax) *lmax = c->ls.lmax;
+    return MCMB_OK;
+}

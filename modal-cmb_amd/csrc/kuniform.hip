@@ -445,3 +445,16 @@ extern "C" int mcmb_kuniform_project(mcmb_ctx* c, int nshape, const double* thet
     c->last_launches = (nshape + UNB - 1) / UNB;
     return MCMB_OK;
 }
+
+extern "C" int mcmb_kuniform_tables(mcmb_ctx* c, const double** q, int* ldq, const int** order, int* nmodes,
+                                    int* max)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_kuniform_tables: null context");
+    if (!c->ku.ready) return fail(MCMB_ERR_STATE, "mcmb_kuniform_tables: call mcmb_kuniform_set first");
+    if (q) *q = c->ku.q.data();
+    if (ldq) *ldq = c->ku.ldq;
+    if (order) *order = c->ku.order.data();
+    if (nmodes) *nmodes = c->ku.n;
+    if (max) *max = c->ku.lmax;
+    return MCMB_OK;
+}

@@ -372,3 +372,125 @@ void calculate_eigen(int min, int max, shape_params params, tetrapyd_limits* blo
         for (int r = 0; r < n; ++r) alpha[r] += a[r];
     free(a);
 }
+
+/* ---- reference helpers over the bound context's tables ------------------ */
+
+/* orthogonalisation.c:279-295 -- pijk of the uniform k-grid setup (basis row =
+ * grid index, order table of mcmb_kuniform_set); 0 if unset or out of range. */
+double pijk(int r, int i, int j, int k)
+{
+    const double* q;
+    const int* order;
+    int ldq, n, mx;
+    if (!g_ctx || mcmb_kuniform_tables(g_ctx, &q, &ldq, &order, &n, &mx) != MCMB_OK) return 0.0;
+    if (r < 0 || r >= n || i < 0 || j < 0 || k < 0 || i > mx || j > mx || k > mx) return 0.0;
+    const int p1 = order[3 * r], p2 = order[3 * r + 1], p3 = order[3 * r + 2];
+    const double* qi = q + (long)i * ldq;
+    const double* qj = q + (long)j * ldq;
+    const double* qk = q + (long)k * ldq;
+    double b1 = qi[p1] * qj[p2] * qk[p3];
+    double b2 = qi[p2] * qj[p3] * qk[p1];
+    double b3 = qi[p3] * qj[p1] * qk[p2];
+    double b4 = qi[p3] * qj[p2] * qk[p1];
+    double b5 = qi[p2] * qj[p1] * qk[p3];
+    double b6 = qi[p1] * qj[p3] * qk[p2];
+    return (b1 + b2 + b3 + b4 + b5 + b6) / (6.0);
+}
+
+/* orthogonalisation_late.c:620-644 -- the same product over the l-space setup
+ * (basis row = multipole l), 0 outside [lmin, lmax]. */
+double plijk_TTT(int r, int l1, int l2, int l3)
+{
+    const double* q;
+    const int* order;
+    int ldq, n, lmin, lmax;
+    if (!g_ctx || mcmb_lspace_tables(g_ctx, &q, &ldq, &order, &n, &lmin, &lmax) != MCMB_OK) return 0.0;
+    if (r < 0 || r >= n) return 0.0;
+    if (l1 < lmin || l1 > lmax || l2 < lmin || l2 > lmax || l3 < lmin || l3 > lmax) return 0.0;
+    const int p1 = order[3 * r], p2 = order[3 * r + 1], p3 = order[3 * r + 2];
+    const double* qi = q + (long)l1 * ldq;
+    const double* qj = q + (long)l2 * ldq;
+    const double* qk = q + (long)l3 * ldq;
+    double b1 = qi[p1] * qj[p2] * qk[p3];
+    double b2 = qi[p2] * qj[p3] * qk[p1];
+    double b3 = qi[p3] * qj[p1] * qk[p2];
+    double b4 = qi[p3] * qj[p2] * qk[p1];
+    double b5 = qi[p2] * qj[p1] * qk[p3];
+    double b6 = qi[p1] * qj[p3] * qk[p2];
+    return (b1 + b2 + b3 + b4 + b5 + b6) / (6e0);
+}
+
+/* fixed_data.c:263-270 */
+void find_perm_prim(int r, int* p1, int* p2, int* p3)
+{
+    const double* q;
+    const int* order;
+    int ldq, n, mx;
+    *p1 = *p2 = *p3 = -1;
+    if (!g_ctx || mcmb_kuniform_tables(g_ctx, &q, &ldq, &order, &n, &mx) != MCMB_OK || r < 0 || r >= n) return;
+    *p1 = order[3 * r];
+    *p2 = order[3 * r + 1];
+    *p3 = order[3 * r + 2];
+}
+
+/* fixed_data.c get_terms_prim / get_kmax of the uniform setup (0 if unset) */
+int get_terms_prim(void)
+{
+    int n = 0;
+    if (!g_ctx || mcmb_kuniform_info(g_ctx, NULL, NULL, &n, NULL, NULL) != MCMB_OK) return 0;
+    return n;
+}
+
+double get_kmax(void)
+{
+    double kmax = 0.0;
+    if (!g_ctx || mcmb_kuniform_info(g_ctx, NULL, NULL, NULL, &kmax, NULL) != MCMB_OK) return 0.0;
+    return kmax;
+}
+
+/* calculate_shape.c:933-946 (model 4, the feature shape): sin(a1 K + 2 pi a2 %)
+ * / (k1 k2 k3)^2, 0 when k1 k2 k3 <= 1e-15; pi as the reference writes it. */
+double shape(double k1, double k2, double k3, shape_params params)
+{
+    double p1 = k1 + k2 + k3, p111 = k1 * k2 * k3;
+    double factor = 0.0;
+    if (p111 > 1e-15) factor = 1.0 / (p111 * p111);
+    return factor * sin(params.a1 * p1 + 2.0 * 3.1415927 * params.a2 * 1e-2);
+}
+
+/* calculate_shape.c:197-216 */
+double shape3(double k1, double k2, double k3, shape_params params)
+{
+    double p222 = (k1 * k1) * (k2 * k2) * (k3 * k3);
+    return p222 * shape(k1, k2, k3, params);
+}
+
+/* fixed_data.c:1670-1715 for the Legendre orders (order_prim < 4): rebuild the
+ * uniform k-grid setup of the bound context on size points of [0, max] with
+ * basis q[i][p] = P_p(2 vec[i]/max - 1) (basis_functions_legendre with min 0)
+ * for p up to the largest index of its order table.  Returns 0 (the
+ * reference's value) or a negative status. */
+int create_basis_prim(int size, double max, double* vec)
+{
+    const double* q0;
+    const int* order;
+    int ldq, n, mx;
+    if (!g_ctx) return MCMB_ERR_STATE;
+    int st = mcmb_kuniform_tables(g_ctx, &q0, &ldq, &order, &n, &mx);
+    if (st != MCMB_OK) return st;
+    if (size < 2) return MCMB_ERR_ARG;
+    int pmax = 0;
+    for (int e = 0; e < 3 * n; ++e) if (order[e] > pmax) pmax = order[e];
+    int* ord = (int*)malloc(sizeof(int) * 3 * (size_t)n);
+    double* q = (double*)malloc(sizeof(double) * (size_t)size * (pmax + 1));
+    double** rows = (double**)malloc(sizeof(double*) * (size_t)size);
+    if (!ord || !q || !rows) { free(ord); free(q); free(rows); return MCMB_ERR_NOMEM; }
+    memcpy(ord, order, sizeof(int) * 3 * (size_t)n);
+    for (int i = 0; i < size; ++i) rows[i] = q + (long)i * (pmax + 1);
+    basis_functions_legendre(rows, size, pmax, 0.0, max, vec);
+    st = mcmb_kuniform_set(g_ctx, 0, size - 1, max, pmax + 1, q, n, ord);
+    free(rows);
+    free(q);
+    free(ord);
+    return st;
+}

@@ -646,3 +646,55 @@ int mcmb_project_bwd(mcmb_ctx* c, int precision, int ntheta, const double* theta
 }
 
 }  // extern "C"
+
+// ---- Gamma post-processing (gamma_mpi.c:260-295, 801-840) -------------------
+// orthoinv = lambda^T lambda, Gamma' = orthoinv Gamma, FP64 MFMA products
+// (proj_gemm<double>: A n x Kp, B Kp x n, both zero-padded in K).
+__global__ void la_pad(const double* __restrict__ src, int n, int trans, int rows, int cols,
+                       double* __restrict__ dst)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)rows * cols) return;
+    const int r = (int)(e / cols), c = (int)(e % cols);
+    dst[e] = (r < n && c < n) ? (trans ? src[(long)c * n + r] : src[(long)r * n + c]) : 0.0;
+}
+
+extern "C" int mcmb_gamma_post(mcmb_ctx* c, int n, const double* lambda, const double* gamma, double* orthoinv,
+                               double* gamma_out)
+{
+    if (!c || n < 1 || !lambda || !gamma || !gamma_out) return fail(MCMB_ERR_ARG, "mcmb_gamma_post: bad arguments");
+    MCMB_TRY(set_device(c));
+    const int Kp = (n + PK - 1) / PK * PK;
+    const size_t nn = (size_t)n * n, np = (size_t)n * Kp;
+    DevBuf dL, dG, dA, dB, dC;
+    MCMB_TRY(dL.alloc(sizeof(double) * nn));
+    MCMB_TRY(dG.alloc(sizeof(double) * nn));
+    MCMB_TRY(dA.alloc(sizeof(double) * np));
+    MCMB_TRY(dB.alloc(sizeof(double) * np));
+    MCMB_TRY(dC.alloc(sizeof(double) * nn));
+    MCMB_HIP(hipMemcpyAsync(dL.p, lambda, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(dG.p, gamma, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+    const unsigned gp = (unsigned)((np + 255) / 256);
+    const dim3 grid((unsigned)((n + PT - 1) / PT), (unsigned)((n + PT - 1) / PT));
+    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
+    // orthoinv[i][j] = sum_k lambda[k][i] lambda[k][j]
+    hipLaunchKernelGGL(la_pad, dim3(gp), dim3(256), 0, c->stream, dL.as<double>(), n, 1, n, Kp, dA.as<double>());
+    hipLaunchKernelGGL(la_pad, dim3(gp), dim3(256), 0, c->stream, dL.as<double>(), n, 0, Kp, n, dB.as<double>());
+    hipLaunchKernelGGL(proj_gemm<double>, grid, dim3(256), 0, c->stream, dA.as<double>(), dB.as<double>(), n, Kp, n,
+                       n, dC.as<double>(), n);
+    // Gamma'[i][j] = sum_m orthoinv[i][m] Gamma[m][j]
+    hipLaunchKernelGGL(la_pad, dim3(gp), dim3(256), 0, c->stream, dC.as<double>(), n, 0, n, Kp, dA.as<double>());
+    hipLaunchKernelGGL(la_pad, dim3(gp), dim3(256), 0, c->stream, dG.as<double>(), n, 0, Kp, n, dB.as<double>());
+    if (orthoinv) MCMB_HIP(hipMemcpyAsync(orthoinv, dC.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+    hipLaunchKernelGGL(proj_gemm<double>, grid, dim3(256), 0, c->stream, dA.as<double>(), dB.as<double>(), n, Kp, n,
+                       n, dG.as<double>(), n);
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipEventRecord(c->ev1, c->stream));
+    MCMB_HIP(hipMemcpyAsync(gamma_out, dG.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = 2;
+    return MCMB_OK;
+}

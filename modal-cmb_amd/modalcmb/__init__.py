@@ -69,6 +69,14 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_cholesky_solve.argtypes = [vp, ip, vp, ip, vp, vp]
         lib.mcmb_eigen_sym.argtypes = [vp, ip, vp, vp, vp, C.POINTER(ip)]
         lib.mcmb_kuniform_set.argtypes = [vp, ip, ip, dp, ip, vp, ip, vp]
+        lib.mcmb_gamma_post.argtypes = [vp, ip, vp, vp, vp, vp]
+        lib.mcmb_write_raw.argtypes = [C.c_char_p, C.c_long, vp]
+        lib.mcmb_read_raw.argtypes = [C.c_char_p, C.c_long, vp]
+        lib.mcmb_load_order.argtypes = [C.c_char_p, vp, ip, C.POINTER(ip)]
+        for f in ("pijk", "plijk_TTT"):
+            getattr(lib, f).restype = dp
+            getattr(lib, f).argtypes = [ip, ip, ip, ip]
+        lib.get_kmax.restype = dp
         lib.mcmb_kuniform_info.argtypes = [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip), C.POINTER(dp),
                                            C.POINTER(C.c_long)]
         lib.mcmb_kuniform_npoints.argtypes = [vp]
@@ -104,11 +112,34 @@ EXPORTED_SYMBOLS = (
     "mcmb_kgram", "mcmb_project_fwd", "mcmb_project_bwd", "mcmb_eigen_sym", "mcmb_orthonormalise_late",
     "mcmb_kuniform_set", "mcmb_kuniform_info", "mcmb_kuniform_npoints", "mcmb_kuniform_weights",
     "mcmb_kuniform_gram", "mcmb_kuniform_project", "calculate_ortho", "calculate_eigen",
+    "mcmb_kuniform_tables", "mcmb_lspace_tables", "mcmb_gamma_post", "mcmb_write_raw", "mcmb_read_raw",
+    "mcmb_load_order", "pijk", "plijk_TTT", "find_perm_prim", "get_terms_prim", "get_kmax", "shape", "shape3",
+    "create_basis_prim", "load_three_int",
     "get_ijk_next_XXX", "check_ijk_min_XXX", "check_ijk_max_XXX", "decompose_tetrapyd_XXX",
     "get_ijk_next_prim", "check_ijk_min_prim", "check_ijk_max_prim", "decompose_tetrapyd_prim",
     "divide_tasks", "get_next_ijk_bound", "permsix", "calculate_geometric",
     "basis_functions_legendre", "gamma_3d_host", "calculate_orthol_3D", "calculate_orthol",
 )
+
+
+def write_raw(path: str, a: np.ndarray) -> None:
+    """array_write layout: raw native FP64, no header."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    _check(load_library().mcmb_write_raw(path.encode(), a.size, _ptr(a)))
+
+
+def read_raw(path: str, shape) -> np.ndarray:
+    a = np.empty(shape, dtype=np.float64)
+    _check(load_library().mcmb_read_raw(path.encode(), a.size, _ptr(a)))
+    return a
+
+
+def load_order(path: str, cap: int = 1 << 20) -> np.ndarray:
+    """Mode order table as load_three_int reads it."""
+    buf = np.empty((cap, 3), dtype=np.int32)
+    n = C.c_int(0)
+    _check(load_library().mcmb_load_order(path.encode(), _ptr(buf), cap, C.byref(n)))
+    return buf[:n.value].copy()
 
 
 def _ptr(a: np.ndarray | None):
@@ -325,6 +356,14 @@ class Context:
         a = np.empty((th.shape[0], self.nu))
         _check(_lib.mcmb_kuniform_project(self._h, th.shape[0], _ptr(th), p0, p1, _ptr(a)))
         return a
+
+    def gamma_post(self, lam: np.ndarray, gamma: np.ndarray):
+        """(orthoinv = lambda^T lambda, orthoinv @ gamma) as gamma_mpi.c:260-295, 801-840."""
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        g = np.ascontiguousarray(gamma, dtype=np.float64)
+        oi, out = np.empty_like(g), np.empty_like(g)
+        _check(_lib.mcmb_gamma_post(self._h, g.shape[0], _ptr(lam), _ptr(g), _ptr(oi), _ptr(out)))
+        return oi, out
 
     def cholesky_solve(self, ortho: np.ndarray, alpha: np.ndarray) -> np.ndarray:
         f = np.ascontiguousarray(ortho, dtype=np.float64)

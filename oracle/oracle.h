@@ -1,9 +1,10 @@
 /*
  * oracle.h -- FP64 CPU restatement of the reference's Gamma / projection path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
- * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
- * never as the thing measured or shipped.  The product path (the HIP library
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle the GPU path is verified
+ * against: only tests/, the smoke test of the repository's entry module and
+ * bench.py's CPU-baseline leg may load it, and never as the thing measured or
+ * shipped.  The product path (the HIP library
  * under modal-cmb_amd/) does not link, load or call anything in oracle/.
  *
  * Every function cites the reference file:line whose behaviour it restates

@@ -1,9 +1,10 @@
 """ctypes wrapper of oracle/liboracle.so -- the FP64 CPU restatement of the
 reference path.
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  Nothing in the product
-(modal-cmb_amd/) imports this module.
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke test of the
+repository's entry module and the CPU-baseline leg of bench.py, as the oracle
+results are compared with.  Nothing in the product (modal-cmb_amd/) imports
+this module.
 """
 from __future__ import annotations
 

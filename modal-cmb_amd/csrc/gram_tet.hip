@@ -16,6 +16,7 @@
 #include "mcmb_internal.h"
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 namespace mcmb {
 
@@ -366,6 +367,12 @@ int pointset_weights(mcmb_ctx* c, PointSet& P, DevBuf& rowptr, DevBuf& ws, int s
 
 int gram_tet_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gamma, int row_lo, int row_hi)
 {
+    // the factored contraction (row moments x pair products) wherever the
+    // basis width allows it, unless MCMB_GRAM_DIRECT=1 asks for the direct
+    // point-by-point contraction below
+    const char* direct = getenv("MCMB_GRAM_DIRECT");
+    if (gram_factored_supported(L, precision) && !(direct && atoi(direct) == 1))
+        return gram_factored_launch(c, L, precision, d_gamma, row_lo, row_hi);
     DevBuf rowptr, ws;
     MCMB_TRY(ensure_lspace_aux(c, L, rowptr, ws, 1));
     if (precision == MCMB_MIXED)

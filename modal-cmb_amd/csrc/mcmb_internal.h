@@ -108,4 +108,8 @@ int set_device(mcmb_ctx* ctx);
 // Per-point weights of P (walk order) into ws, scaled = sqrt(w)/6 if nonzero;
 // rowptr[r] = first point of row r.  Weights by the point set's kind.
 int pointset_weights(mcmb_ctx* ctx, PointSet& P, DevBuf& rowptr, DevBuf& ws, int scaled);
+// Factored Gram (row moments x pair products): whether it handles this basis
+// width and precision, and the launch over rows [row_lo, row_hi).
+bool gram_factored_supported(const PointSet& P, int precision);
+int gram_factored_launch(mcmb_ctx* ctx, PointSet& P, int precision, double* d_gamma, int row_lo, int row_hi);
 }  // namespace mcmb

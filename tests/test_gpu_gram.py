@@ -154,3 +154,76 @@ def test_gamma_gl_rejects_inexact_node_count(ctx):
     ctx.lspace_set(2, 100, ws["q"], ws["v"], ws["order"])
     with pytest.raises(modalcmb.ModalError):
         ctx.gamma_gl(npt=100)
+
+
+@pytest.fixture
+def direct_env():
+    """Route the FP64 Gram through the direct point-by-point contraction."""
+    old = os.environ.get("MCMB_GRAM_DIRECT")
+    os.environ["MCMB_GRAM_DIRECT"] = "1"
+    yield
+    if old is None:
+        del os.environ["MCMB_GRAM_DIRECT"]
+    else:
+        os.environ["MCMB_GRAM_DIRECT"] = old
+
+
+def _gram_pair(ctx, prec):
+    """(factored, direct) evaluations of the same Gram."""
+    Gf = ctx.gamma_tet(prec)
+    os.environ["MCMB_GRAM_DIRECT"] = "1"
+    try:
+        Gd = ctx.gamma_tet(prec)
+    finally:
+        del os.environ["MCMB_GRAM_DIRECT"]
+    return Gf, Gd
+
+
+@pytest.mark.parametrize("lmin,L,R,mod,h2", [(2, 500, 10, True, 0), (3, 401, 12, False, 1), (0, 257, 30, True, 0)])
+def test_gamma_factored_vs_direct(ctx, lmin, L, R, mod, h2):
+    # two summation orders of the same FP64 sum: point by point, and row
+    # moments times pair products
+    ws = W.lspace(L, R, mod, lmin=lmin)
+    ctx.lspace_set(lmin, L, ws["q"], ws["v"], ws["order"], h2_mode=h2)
+    Gf, Gd = _gram_pair(ctx, modalcmb.FP64)
+    assert rel_elem(Gf, Gd) < 1e-12
+    assert np.array_equal(Gf, Gf.T)
+
+
+def test_gamma_factored_permuted_mode_order(ctx):
+    # modes listed in any order, with their indices in any arrangement
+    ws = W.lspace(300, 8, True)
+    order = ws["order"].copy()
+    rng = np.random.default_rng(7)
+    order = order[rng.permutation(len(order))]
+    for m in range(len(order)):
+        order[m] = order[m][rng.permutation(3)]
+    ctx.lspace_set(2, 300, ws["q"], ws["v"], order)
+    Gf, Gd = _gram_pair(ctx, modalcmb.FP64)
+    assert rel_elem(Gf, Gd) < 1e-12
+
+
+def test_gamma_factored_slices(ctx):
+    ws = W.lspace(300, 9)
+    ctx.lspace_set(2, 300, ws["q"], ws["v"], ws["order"])
+    G = ctx.gamma_tet(modalcmb.FP64)
+    parts = [ctx.gamma_tet_slice(modalcmb.FP64, r, 7)[0] for r in range(7)]
+    assert rel_elem(sum(parts), G) < 1e-13
+    ref = O.gamma_tet(2, 300, ws["q"], ws["v"], ws["order"])
+    assert rel_elem(G, ref) < 1e-12
+
+
+def test_gamma_wide_basis_uses_direct(ctx, direct_env):
+    # R = 34 is past the factored kernel's pair capacity: the call still works
+    # (direct contraction) and agrees with the oracle
+    ws = W.lspace(60, 34)
+    order = ws["order"][::37].copy()           # 193 modes over all 34 basis columns
+    assert order.max() == 33
+    ctx.lspace_set(2, 60, ws["q"], ws["v"], order)
+    G = ctx.gamma_tet(modalcmb.FP64)
+    del os.environ["MCMB_GRAM_DIRECT"]
+    G2 = ctx.gamma_tet(modalcmb.FP64)
+    os.environ["MCMB_GRAM_DIRECT"] = "1"
+    assert np.array_equal(G, G2)
+    ref = O.gamma_tet(2, 60, ws["q"], ws["v"], order)
+    assert rel_elem(G, ref) < 1e-12

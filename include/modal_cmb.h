@@ -226,6 +226,12 @@ int mcmb_gauss_legendre(int n, double* x, double* w);
 /* Milliseconds spent in the dominant kernel of the last mcmb_gamma_tet /
  * mcmb_gamma_gl call (HIP events on the context stream) and its launch count. */
 int mcmb_last_kernel_ms(mcmb_ctx* ctx, double* ms, int* launches);
+/* Per-phase milliseconds of the last call that records phases (the FP64
+ * separable Gram: weights, pair products, l3 / l2 / l1 contractions, assembly),
+ * each between HIP events on the context stream.  Writes up to cap entries of
+ * ms[] and names[] (static strings) and returns the phase count (0 if the last
+ * such call recorded none).  No reference counterpart. */
+int mcmb_last_phases(mcmb_ctx* ctx, int cap, double* ms, const char** names);
 
 /* ---- reference-named entry points (legacy shims over the bound context) --- */
 /* Bind the context the legacy symbols use (not owned). */

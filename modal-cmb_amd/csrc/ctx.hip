@@ -118,6 +118,7 @@ int mcmb_create(mcmb_ctx** out, int device)
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&c->ev0);
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    for (int k = 0; k <= MAX_PHASES && e == hipSuccess; ++k) e = hipEventCreate(&c->evp[k]);
     if (e != hipSuccess) {
         delete c;
         return fail(MCMB_ERR_HIP, "mcmb_create: %s", hipGetErrorString(e));
@@ -141,6 +142,8 @@ int mcmb_destroy(mcmb_ctx* c)
     c->d_aux.release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (int k = 0; k <= MAX_PHASES; ++k)
+        if (c->evp[k]) (void)hipEventDestroy(c->evp[k]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MCMB_OK;
@@ -228,6 +231,19 @@ int mcmb_last_kernel_ms(mcmb_ctx* c, double* ms, int* launches)
     if (ms) *ms = c->last_ms;
     if (launches) *launches = c->last_launches;
     return MCMB_OK;
+}
+
+int mcmb_last_phases(mcmb_ctx* c, int cap, double* ms, const char** names)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_last_phases: null context");
+    if (cap < 0 || (cap > 0 && (!ms || !names))) return fail(MCMB_ERR_ARG, "mcmb_last_phases: bad buffers");
+    for (int k = 0; k < c->nphases && k < cap; ++k) {
+        float t = 0.f;
+        MCMB_HIP(hipEventElapsedTime(&t, c->evp[k], c->evp[k + 1]));
+        ms[k] = t;
+        names[k] = c->phase_names[k];
+    }
+    return c->nphases;
 }
 
 int mcmb_gamma_tet(mcmb_ctx* c, int precision, double* gamma_host, double* gamma_dev)

@@ -34,12 +34,6 @@ constexpr int FZ_TN = 128;    // cd columns per tile
 constexpr int FZ_KC = 16;     // rows per K chunk
 constexpr int FZ_MAXU = 9;    // pairs per lane in the row-moment kernel (NP <= 576)
 
-// pair index of (x, y), x <= y < R: x-major, y from x
-__host__ __device__ __forceinline__ int pair_index(int x, int y, int R)
-{
-    return x * R - x * (x - 1) / 2 + (y - x);
-}
-
 // M[row][cd] = sum_t w_t q_c(k_t) q_d(k_t), one wave per row, lanes over pairs.
 __global__ __launch_bounds__(256) void fz_rowmom(const RowDesc* __restrict__ rows, const long* __restrict__ rowptr,
                                                  int row_lo, int nrows, const double* __restrict__ q, int ldq,

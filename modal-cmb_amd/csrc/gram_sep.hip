@@ -1,0 +1,581 @@
+// Separable tetrapyd Gram: Gamma_mn = sum_{(l1,l2,l3) in T} w(l) Q_m(l) Q_n(l)
+// through three successive contractions of the pair products of the basis.
+//
+// With Q_m(l) = (1/6) sum_pi prod_a q_{m_pi(a)}(l_a) ((m_1, m_2, m_3) = order[m]),
+//   Q_m(l) Q_n(l) = (1/36) sum_{pi, rho} prod_a U_{m_pi(a) n_rho(a)}(l_a),
+//   U_{xy}(l) = q_x(l) q_y(l)  (indexed by the unordered pair, NP = R(R+1)/2),
+// so Gamma_mn = (1/36) sum_{pi, rho} G[{m_pi1 n_rho1}][{m_pi2 n_rho2}][{m_pi3 n_rho3}]
+// with the slot-ordered tensor
+//   G[a][e][f] = sum_{l in T} w(l) U_a(l1) U_e(l2) U_f(l3),
+// formed in three stages over the rows (l1, l2) of the ordered tetrapyd:
+//   1. M[row][f]    = sum_{l3 in row} w U_f(l3)          (row moments, FP64 VALU)
+//   2. Y[i][e][f]   = sum_{rows of l1 = i} U_e(l2) M[row][f]   (FP64 MFMA, batched per i)
+//   3. G[a][e, f]   = sum_i U_a(i) Y[i][e][f]             (FP64 MFMA, one GEMM)
+// and a 36-term gather per Gamma entry.  At lmax 2500 with R = 30 (NP = 465)
+// the stages cost 3.0e11, 6.7e11 and 2.5e11 multiply-adds, against 8e15 for
+// the point-by-point contraction; the summation is FP64 throughout and in a
+// fixed order (no atomics), so repeated calls are bit-identical.
+#include "mcmb_internal.h"
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <vector>
+
+namespace mcmb {
+
+typedef double fd4 __attribute__((ext_vector_type(4)));
+typedef double fd2 __attribute__((ext_vector_type(2)));
+
+constexpr int SG_FM = 3, SG_FN = 3;   // 16 x 16 blocks per wave (rows, columns); 2 x 2 waves
+constexpr int SG_TM = 32 * SG_FM;     // GEMM tile rows (96)
+constexpr int SG_TN = 32 * SG_FN;     // GEMM tile columns (96)
+constexpr int SG_KC = 16;             // K rows per LDS stage
+constexpr int SG_LDA = SG_TM + 16;    // LDS row strides in doubles (32 banks apart)
+constexpr int SG_LDB = SG_TN + 16;
+constexpr int SR_G = 16;            // rows per row-moment group
+constexpr int SR_KC = 16;           // l3 values per weight stage (SR_G * SR_KC = 256 threads)
+constexpr int SR_FMAX = 3;          // pair columns per thread in stage 1: NP <= 768
+
+struct SepTask {
+    long boff;   // first B row of the task
+    long coff;   // first C element of the task
+    int aoff;    // first entry of the A row index list
+    int K;       // rows of the task
+};
+
+// U[l][f] = q_x(l) q_y(l), f = pair (x, y)
+// (padding columns f >= NP are zero)
+__global__ void sep_utab(const double* __restrict__ q, int ldq, int nl, const int2* __restrict__ pxy, int NP, int ld,
+                         double* __restrict__ U)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)nl * ld) return;
+    const int l = (int)(e / ld), f = (int)(e % ld);
+    if (f >= NP) {
+        U[e] = 0.0;
+        return;
+    }
+    const int2 xy = pxy[f];
+    U[e] = q[(long)l * ldq + xy.x] * q[(long)l * ldq + xy.y];
+}
+
+// Stage 1.  One workgroup per group of up to 16 rows that share l1 and the
+// parity class of l3, so that their l3 ranges interleave on one grid of step
+// kstep; thread t owns pair columns t, t + 256, ...  Per stage of 16 l3 values
+// the pair products U[l3] are loaded into registers first (all loads in
+// flight together), then the 16 x 16 weights (zero outside a row's range) are
+// staged in LDS and applied.  F = pair columns per thread, H = halves of the
+// stage (two for F = 3, to stay within the register file).
+template <int F, int H>
+__global__ __launch_bounds__(256) void sep_rowmom(const RowDesc* __restrict__ rows, const long* __restrict__ rowptr,
+                                                  const double* __restrict__ ws, const int* __restrict__ groups,
+                                                  int row_base, int kstep, const double* __restrict__ U, int ldu,
+                                                  int NP, int ld, double* __restrict__ M)
+{
+    constexpr int KH = SR_KC / H;
+    __shared__ int gr[SR_G], gk0[SR_G], gnk[SR_G];
+    __shared__ long gp[SR_G];
+    __shared__ double Wt[2][SR_KC][SR_G];
+    const int tid = threadIdx.x;
+    if (tid < SR_G) {
+        const int r = groups[(long)blockIdx.x * SR_G + tid];
+        gr[tid] = r;
+        if (r >= 0) {
+            const RowDesc d = rows[r];
+            gk0[tid] = d.k0;
+            gnk[tid] = d.nk;
+            gp[tid] = rowptr[r];
+        } else {
+            gk0[tid] = 0;
+            gnk[tid] = 0;
+            gp[tid] = 0;
+        }
+    }
+    __syncthreads();
+    int kmin = INT_MAX, kmax = INT_MIN;
+    for (int g = 0; g < SR_G; ++g)
+        if (gnk[g] > 0) {
+            kmin = min(kmin, gk0[g]);
+            kmax = max(kmax, gk0[g] + kstep * (gnk[g] - 1));
+        }
+    const int nkk = kmax >= kmin ? (kmax - kmin) / kstep + 1 : 0;
+
+    double acc[SR_G][F];
+#pragma unroll
+    for (int g = 0; g < SR_G; ++g)
+#pragma unroll
+        for (int u = 0; u < F; ++u) acc[g][u] = 0.0;
+
+    int buf = 0;
+    for (int c0 = 0; c0 < nkk; c0 += SR_KC, buf ^= 1) {
+        double uu[KH][F];
+        // l3 values past the group's range repeat the last one (weight 0)
+        auto uload = [&](int h) {
+#pragma unroll
+            for (int kk = 0; kk < KH; ++kk) {
+                const int kc = min(c0 + h * KH + kk, nkk - 1);
+                const double* Uk = U + (long)(kmin + kstep * kc) * ldu;
+#pragma unroll
+                for (int u = 0; u < F; ++u) {
+                    const int f = tid + 256 * u;
+                    uu[kk][u] = f < NP ? Uk[f] : 0.0;
+                }
+            }
+        };
+        uload(0);
+        {
+            const int kk = tid >> 4, g = tid & 15;
+            double w = 0.0;
+            if (c0 + kk < nkk) {
+                const int d = kmin + kstep * (c0 + kk) - gk0[g];
+                if (d >= 0 && d % kstep == 0 && d / kstep < gnk[g]) w = ws[gp[g] + d / kstep];
+            }
+            Wt[buf][kk][g] = w;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            if (h > 0) uload(h);
+#pragma unroll
+            for (int kk = 0; kk < KH; ++kk)
+#pragma unroll
+                for (int g = 0; g < SR_G; ++g) {
+                    const double w = Wt[buf][h * KH + kk][g];
+#pragma unroll
+                    for (int u = 0; u < F; ++u) acc[g][u] = fma(w, uu[kk][u], acc[g][u]);
+                }
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < SR_G; ++g) {
+        if (gr[g] < 0) continue;
+        double* Mr = M + (long)(gr[g] - row_base) * ld;
+#pragma unroll
+        for (int u = 0; u < F; ++u) {
+            const int f = tid + 256 * u;
+            if (f < NP) Mr[f] = acc[g][u];
+        }
+    }
+}
+
+// Stages 2 and 3:  C_task[m][n] (+)= sum_{kk < K} U[row(kk)][m] * B[boff + kk][n]
+// for m < Md, n < Nd, over a list of 96 x 96 tiles (task, m0, n0); row(kk) =
+// aoff + kk when the A rows are contiguous (CONTIG), else aidx[aoff + kk].
+// Both operands are K-major with even leading dimensions (16-byte loads; the
+// element past an odd Md or Nd is padding and only feeds outputs that are not
+// stored).  16 K rows per stage, double-buffered in LDS; FP64 MFMA 16x16x4
+// with 4 waves of 48 x 48.  The tile list holds groups of GS consecutive
+// tiles that share one B panel (the m tiles of one (task, n0)).  Blocks b,
+// b+8, ... share an XCD (and its L2): the groups are dealt to the 8 XCDs in
+// turn, each group's tiles to consecutive blocks of one XCD, so the tiles of
+// a group meet in the same L2 and the XCDs get alike shares of the work.
+template <bool CONTIG>
+__global__ __launch_bounds__(256, 2) void sep_gemm(const double* __restrict__ U, int ldu,
+                                                   const int* __restrict__ aidx, const double* __restrict__ B,
+                                                   long ldb, double* __restrict__ C, long ldc, int Md, int Nd,
+                                                   const SepTask* __restrict__ tasks, const int4* __restrict__ tiles,
+                                                   int ntiles, int GS, int accumulate)
+{
+    __shared__ double As[2][SG_KC][SG_LDA];
+    __shared__ double Bs[2][SG_KC][SG_LDB];
+    const int slot = blockIdx.x >> 3;
+    const int t = ((blockIdx.x & 7) + 8 * (slot / GS)) * GS + slot % GS;
+    if (t >= ntiles) return;
+    const int4 tl = tiles[t];
+    const SepTask tk = tasks[tl.x];
+    const int m0 = tl.y, n0 = tl.z, K = tk.K;
+    const int* ai = aidx + tk.aoff;
+    const double* Bb = B + tk.boff * ldb;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+
+    fd4 acc[SG_FM][SG_FN];
+#pragma unroll
+    for (int a = 0; a < SG_FM; ++a)
+#pragma unroll
+        for (int c = 0; c < SG_FN; ++c) acc[a][c] = (fd4){0.0, 0.0, 0.0, 0.0};
+
+    constexpr int HA = SG_TM / 2, HB = SG_TN / 2;   // double2 per tile row
+    constexpr int PA = SG_KC * HA / 256;             // double2 of A per thread
+    constexpr int PB = SG_KC * HB / 256;             // double2 of B per thread
+    static_assert(PA * 256 == SG_KC * HA && PB * 256 == SG_KC * HB, "tile loads split evenly over 256 threads");
+    const fd2 z2 = {0.0, 0.0};
+    fd2 ra[PA], rb[PB];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < PA; ++u) {
+            const int e = tid + 256 * u;
+            const int kk = e / HA, mm = 2 * (e - HA * kk);
+            const int kg = k0 + kk;
+            const bool kin = kg < K;
+            const long arow = CONTIG ? (long)(tk.aoff + kg) : (long)(kin ? ai[kg] : 0);
+            ra[u] = (kin && m0 + mm < Md) ? *reinterpret_cast<const fd2*>(U + arow * ldu + m0 + mm) : z2;
+        }
+#pragma unroll
+        for (int u = 0; u < PB; ++u) {
+            const int e = tid + 256 * u;
+            const int kk = e / HB, nn = 2 * (e - HB * kk);
+            const int kg = k0 + kk;
+            rb[u] = (kg < K && n0 + nn < Nd) ? *reinterpret_cast<const fd2*>(Bb + (long)kg * ldb + n0 + nn) : z2;
+        }
+    };
+    auto sstore = [&](int b) {
+#pragma unroll
+        for (int u = 0; u < PA; ++u) {
+            const int e = tid + 256 * u;
+            const int kk = e / HA, mm = 2 * (e - HA * kk);
+            *reinterpret_cast<fd2*>(&As[b][kk][mm]) = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < PB; ++u) {
+            const int e = tid + 256 * u;
+            const int kk = e / HB, nn = 2 * (e - HB * kk);
+            *reinterpret_cast<fd2*>(&Bs[b][kk][nn]) = rb[u];
+        }
+    };
+
+    const int nch = (K + SG_KC - 1) / SG_KC;
+    if (nch > 0) {
+        gload(0);
+        sstore(0);
+        __syncthreads();
+        for (int ch = 0; ch < nch; ++ch) {
+            const int b = ch & 1;
+            if (ch + 1 < nch) gload((ch + 1) * SG_KC);
+#pragma unroll
+            for (int ks = 0; ks < SG_KC / 4; ++ks) {
+                const int kk = ks * 4 + (lane >> 4);
+                double av[SG_FM], bv[SG_FN];
+#pragma unroll
+                for (int a = 0; a < SG_FM; ++a) av[a] = As[b][kk][wm * (SG_TM / 2) + a * 16 + (lane & 15)];
+#pragma unroll
+                for (int c = 0; c < SG_FN; ++c) bv[c] = Bs[b][kk][wn * (SG_TN / 2) + c * 16 + (lane & 15)];
+#pragma unroll
+                for (int a = 0; a < SG_FM; ++a)
+#pragma unroll
+                    for (int c = 0; c < SG_FN; ++c)
+                        acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[c], acc[a][c], 0, 0, 0);
+            }
+            if (ch + 1 < nch) sstore(b ^ 1);
+            __syncthreads();
+        }
+    }
+    // output element v of a lane: row 4v + lane/16, column lane%16 of the 16 x 16 block
+    double* const Cb = C + tk.coff;
+#pragma unroll
+    for (int a = 0; a < SG_FM; ++a)
+#pragma unroll
+        for (int c = 0; c < SG_FN; ++c) {
+            const int n = n0 + wn * (SG_TN / 2) + c * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int m = m0 + wm * (SG_TM / 2) + a * 16 + 4 * v + (lane >> 4);
+                if (m < Md && n < Nd) {
+                    double* p = Cb + (long)m * ldc + n;
+                    *p = accumulate ? *p + acc[a][c][v] : acc[a][c][v];
+                }
+            }
+        }
+}
+
+// Gamma[m][n] = Gamma[n][m] = (1/36) sum over the 6 x 6 slot assignments of G.
+__global__ void sep_assemble(const double* __restrict__ G, int NP, int ld, int R, const int* __restrict__ modes,
+                             int n, double* __restrict__ gamma)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)n * n) return;
+    const int m = (int)(e / n), nn = (int)(e % n);
+    if (m > nn) return;
+    const int pm = modes[m], pn = modes[nn];
+    const int a[3] = {pm & 255, (pm >> 8) & 255, (pm >> 16) & 255};
+    const int b[3] = {pn & 255, (pn >> 8) & 255, (pn >> 16) & 255};
+    int pid[3][3];
+#pragma unroll
+    for (int x = 0; x < 3; ++x)
+#pragma unroll
+        for (int y = 0; y < 3; ++y) pid[x][y] = pair_index(min(a[x], b[y]), max(a[x], b[y]), R);
+    constexpr int P6[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    const long NP2 = (long)NP * ld;
+    double acc = 0.0;
+#pragma unroll
+    for (int p = 0; p < 6; ++p)
+#pragma unroll
+        for (int r = 0; r < 6; ++r)
+            acc += G[pid[P6[p][0]][P6[r][0]] * NP2 + (long)pid[P6[p][1]][P6[r][1]] * ld + pid[P6[p][2]][P6[r][2]]];
+    acc *= (1.0 / 36.0);
+    gamma[(long)m * n + nn] = acc;
+    gamma[(long)nn * n + m] = acc;
+}
+
+bool gram_separable_supported(const PointSet& L, int precision)
+{
+    return precision == MCMB_FP64 && L.R >= 1 && L.R <= 255 && L.R * (L.R + 1) / 2 <= 256 * SR_FMAX;
+}
+
+namespace {
+
+struct Run {
+    int r0, r1, i;
+};
+
+// Host tables of one block of runs, concatenated into one int array.
+struct BlockTables {
+    SepBlock b;
+    std::vector<int> buf;
+};
+
+template <class T> size_t append(std::vector<int>& buf, const std::vector<T>& v)
+{
+    static_assert(sizeof(T) % sizeof(int) == 0, "table entries are whole ints");
+    while ((buf.size() * sizeof(int)) % 16) buf.push_back(0);   // 16-byte alignment
+    const size_t off = buf.size();
+    buf.resize(off + v.size() * sizeof(T) / sizeof(int));
+    if (!v.empty()) memcpy(buf.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+}
+
+// NP pairs, ld = padded leading dimension of U, M and Y rows.
+void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1, int NP, int ld, BlockTables& BT)
+{
+    SepBlock& T = BT.b;
+    const int kstep = std::max(1, L.kstep);
+    T.row0 = runs[b0].r0;
+    T.nrows = runs[b1 - 1].r1 - T.row0;
+    T.nruns = b1 - b0;
+    std::vector<int> groups, aidx2(T.nrows), aidx3(T.nruns);
+    std::vector<SepTask> task2(T.nruns), task3(1);
+    std::vector<int4> tile2, tile3;
+    std::vector<std::vector<int>> cls(kstep);
+    auto flush = [&](std::vector<int>& c) {
+        if (c.empty()) return;
+        for (int g = 0; g < SR_G; ++g) groups.push_back(g < (int)c.size() ? c[g] : -1);
+        c.clear();
+    };
+    T.contig2 = T.contig3 = true;
+    for (int ru = b0; ru < b1; ++ru) {
+        const Run& R = runs[ru];
+        const int j0 = L.rows[R.r0].j;
+        for (int r = R.r0; r < R.r1; ++r) {
+            const RowDesc& d = L.rows[r];
+            aidx2[r - T.row0] = d.j;
+            if (d.j != j0 + (r - R.r0)) T.contig2 = false;
+            std::vector<int>& c = cls[((d.k0 % kstep) + kstep) % kstep];
+            c.push_back(r);
+            if ((int)c.size() == SR_G) flush(c);
+        }
+        for (auto& c : cls) flush(c);
+        const int lt = ru - b0;
+        task2[lt].boff = R.r0 - T.row0;
+        task2[lt].coff = (long)lt * NP * ld;
+        task2[lt].aoff = R.r0 - T.row0;
+        task2[lt].K = R.r1 - R.r0;
+        aidx3[lt] = R.i;
+        if (R.i != runs[b0].i + lt) T.contig3 = false;
+        for (int n0 = 0; n0 < NP; n0 += SG_TN)
+            for (int m0 = 0; m0 < NP; m0 += SG_TM) tile2.push_back(make_int4(lt, m0, n0, 0));
+    }
+    // contiguous A rows: the task's first row index replaces the list offset
+    if (T.contig2)
+        for (int lt = 0; lt < T.nruns; ++lt) task2[lt].aoff = L.rows[runs[b0 + lt].r0].j;
+    task3[0].boff = 0;
+    task3[0].coff = 0;
+    task3[0].aoff = T.contig3 ? runs[b0].i : 0;
+    task3[0].K = T.nruns;
+    const long N3 = (long)NP * ld;
+    for (long n0 = 0; n0 < N3; n0 += SG_TN)
+        for (int m0 = 0; m0 < NP; m0 += SG_TM) tile3.push_back(make_int4(0, m0, (int)n0, 0));
+    T.ngroups = (int)(groups.size() / SR_G);
+    T.nt2 = (int)tile2.size();
+    T.nt3 = (int)tile3.size();
+    BT.buf.clear();
+    T.o_groups = append(BT.buf, groups);
+    T.o_aidx2 = append(BT.buf, aidx2);
+    T.o_aidx3 = append(BT.buf, aidx3);
+    T.o_task2 = append(BT.buf, task2);
+    T.o_task3 = append(BT.buf, task3);
+    T.o_tile2 = append(BT.buf, tile2);
+    T.o_tile3 = append(BT.buf, tile3);
+}
+
+// blocks for ntiles tiles in groups of GS dealt to 8 XCDs
+unsigned sep_blocks(long ntiles, int GS)
+{
+    const long groups = (ntiles + GS - 1) / GS;
+    return (unsigned)(8 * GS * ((groups + 7) / 8));
+}
+
+}  // namespace
+
+// Tables of the rows [row_lo, row_hi) of L, uploaded to c->sg_tab (kept for
+// the next call on the same point set, generation and rows), and the row
+// pointers in c->sg_rp.
+static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int NP, int ld, bool& rowptr_ready)
+{
+    SepPlan& P = c->sep;
+    const bool same_set = P.ps == &L && P.gen == L.generation;
+    rowptr_ready = same_set && c->sg_rp.p;
+    if (same_set && P.row_lo == row_lo && P.row_hi == row_hi) return MCMB_OK;
+    P = SepPlan();
+    // runs of rows with equal l1, and blocks of runs whose stage outputs fit
+    // the caps (M: 16 GiB, Y: 8 GiB)
+    std::vector<Run> runs;
+    for (int r = row_lo; r < row_hi; ++r) {
+        const int i = L.rows[r].i;
+        if (runs.empty() || runs.back().i != i) runs.push_back({r, r + 1, i});
+        else runs.back().r1 = r + 1;
+    }
+    const long NPL = (long)NP * ld;
+    const long cap_rows = std::max<long>(1, (16l << 30) / ((long)ld * 8));
+    const long cap_runs = std::max<long>(1, (8l << 30) / (NPL * 8));
+    std::vector<std::pair<int, int>> blocks;
+    for (int b0 = 0; b0 < (int)runs.size();) {
+        int b1 = b0 + 1;
+        long nr = runs[b0].r1 - runs[b0].r0;
+        while (b1 < (int)runs.size() && b1 - b0 < cap_runs && nr + (runs[b1].r1 - runs[b1].r0) <= cap_rows) {
+            nr += runs[b1].r1 - runs[b1].r0;
+            ++b1;
+        }
+        blocks.push_back({b0, b1});
+        P.max_rows = std::max(P.max_rows, nr);
+        P.max_runs = std::max<long>(P.max_runs, b1 - b0);
+        b0 = b1;
+    }
+    std::vector<BlockTables> tabs(blocks.size());
+    for (size_t b = 0; b < blocks.size(); ++b) {
+        build_block(L, runs, blocks[b].first, blocks[b].second, NP, ld, tabs[b]);
+        P.max_tab = std::max(P.max_tab, tabs[b].buf.size());
+    }
+    P.max_tab = (P.max_tab + 3) / 4 * 4;   // every block's tables 16-byte aligned
+    MCMB_TRY(c->sg_tab.alloc(sizeof(int) * P.max_tab * std::max<size_t>(1, blocks.size())));
+    for (size_t b = 0; b < blocks.size(); ++b) {
+        MCMB_HIP(hipMemcpy(c->sg_tab.as<int>() + b * P.max_tab, tabs[b].buf.data(), sizeof(int) * tabs[b].buf.size(),
+                           hipMemcpyHostToDevice));
+        P.blocks.push_back(tabs[b].b);
+    }
+    P.ps = &L;
+    P.gen = L.generation;
+    P.row_lo = row_lo;
+    P.row_hi = row_hi;
+    return MCMB_OK;
+}
+
+int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gamma, int row_lo, int row_hi)
+{
+    if (!gram_separable_supported(L, precision))
+        return fail(MCMB_ERR_ARG, "separable Gram: precision %d / basis width %d not supported", precision, L.R);
+    const int n = L.n, R = L.R, NP = R * (R + 1) / 2;
+    const int ld = NP + (NP & 1);          // even leading dimension of M, Y and G rows: 16-byte aligned
+    const int ldu = (NP + 63) / 64 * 64;   // pair-product rows: whole 64-column blocks, padding 0
+    const long NPL = (long)NP * ld;        // one Y slab / one G row
+    const int nl = (int)(L.q.size() / (size_t)L.ldq);
+    c->nphases = 0;
+    if (row_hi <= row_lo) {
+        MCMB_HIP(hipMemsetAsync(d_gamma, 0, sizeof(double) * (size_t)n * n, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        c->last_ms = 0.0;
+        c->last_launches = 0;
+        return MCMB_OK;
+    }
+    bool rowptr_ready = false;
+    MCMB_TRY(separable_plan(c, L, row_lo, row_hi, NP, ld, rowptr_ready));
+    const SepPlan& P = c->sep;
+    std::vector<int2> pxy(NP);
+    for (int x = 0; x < R; ++x)
+        for (int y = x; y < R; ++y) pxy[pair_index(x, y, R)] = make_int2(x, y);
+
+    MCMB_TRY(c->sg_U.alloc(sizeof(double) * (size_t)nl * ldu + sizeof(int2) * NP));
+    MCMB_TRY(c->sg_M.alloc(sizeof(double) * (size_t)P.max_rows * ld));
+    MCMB_TRY(c->sg_Y.alloc(sizeof(double) * (size_t)P.max_runs * NPL));
+    MCMB_TRY(c->sg_G.alloc(sizeof(double) * (size_t)NP * NPL));
+    double* dU = c->sg_U.as<double>();
+    int2* d_pxy = reinterpret_cast<int2*>(dU + (size_t)nl * ldu);
+    MCMB_HIP(hipMemcpy(d_pxy, pxy.data(), sizeof(int2) * NP, hipMemcpyHostToDevice));
+
+    // phases: weights, pair products, then per block the three contractions
+    // (times summed over blocks), assembly
+    int np = 0;   // phases started; phase k runs from event k to event k + 1
+    auto phase = [&](const char* name) -> int {
+        c->phase_names[np] = name;
+        MCMB_HIP(hipEventRecord(c->evp[np], c->stream));
+        ++np;
+        return MCMB_OK;
+    };
+    const bool one_block = P.blocks.size() == 1;
+    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
+    MCMB_TRY(phase("weights"));
+    MCMB_TRY(pointset_weights(c, L, c->sg_rp, c->sg_ws, 0, rowptr_ready));
+    int launches = 1;
+    MCMB_TRY(phase("pair products"));
+    {
+        const long e = (long)nl * ldu;
+        MCMB_TRY(launch_guard("pair products", (unsigned long long)((e + 255) / 256), 256));
+        hipLaunchKernelGGL(sep_utab, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, c->stream, L.d_q.as<double>(),
+                           L.ldq, nl, d_pxy, NP, ldu, dU);
+        MCMB_HIP(hipGetLastError());
+        ++launches;
+    }
+    if (!one_block) MCMB_TRY(phase("l3, l2, l1 contractions (all blocks)"));
+    for (size_t b = 0; b < P.blocks.size(); ++b) {
+        const SepBlock& T = P.blocks[b];
+        const int* tab = c->sg_tab.as<int>() + b * P.max_tab;
+        // stage 1
+        if (one_block) MCMB_TRY(phase("l3 contraction (row moments)"));
+        MCMB_TRY(launch_guard("row moments", (unsigned long long)T.ngroups, 256));
+        const int F = (NP + 255) / 256;
+#define SEP_ROWMOM(FF, HH)                                                                                   \
+    hipLaunchKernelGGL((sep_rowmom<FF, HH>), dim3((unsigned)T.ngroups), dim3(256), 0, c->stream,            \
+                       L.d_rows.as<RowDesc>(), c->sg_rp.as<long>(), c->sg_ws.as<double>(), tab + T.o_groups, \
+                       T.row0, L.kstep, (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
+        if (F == 1) SEP_ROWMOM(1, 1);
+        else if (F == 2) SEP_ROWMOM(2, 1);
+        else SEP_ROWMOM(3, 2);
+#undef SEP_ROWMOM
+        MCMB_HIP(hipGetLastError());
+        // stages 2 and 3
+        const int GS = (NP + SG_TM - 1) / SG_TM;   // m tiles per B panel, both stages
+#define SEP_GEMM(CG, NT, AIDX, BP, LDB, CP, LDC, MD, ND, TASK, TILE, ACC)                                          \
+    hipLaunchKernelGGL((sep_gemm<CG>), dim3(sep_blocks(NT, GS)), dim3(256), 0, c->stream, (const double*)dU, ldu,\
+                       AIDX, (const double*)(BP), (long)(LDB), CP, (long)(LDC), MD, ND,                          \
+                       reinterpret_cast<const SepTask*>(TASK), reinterpret_cast<const int4*>(TILE), NT, GS, ACC)
+        if (one_block) MCMB_TRY(phase("l2 contraction (FP64 MFMA)"));
+        MCMB_TRY(launch_guard("l2 contraction", sep_blocks(T.nt2, GS), 256));
+        if (T.contig2)
+            SEP_GEMM(true, T.nt2, tab + T.o_aidx2, c->sg_M.as<double>(), ld, c->sg_Y.as<double>(), ld, NP, NP,
+                     tab + T.o_task2, tab + T.o_tile2, 0);
+        else
+            SEP_GEMM(false, T.nt2, tab + T.o_aidx2, c->sg_M.as<double>(), ld, c->sg_Y.as<double>(), ld, NP, NP,
+                     tab + T.o_task2, tab + T.o_tile2, 0);
+        MCMB_HIP(hipGetLastError());
+        if (one_block) MCMB_TRY(phase("l1 contraction (FP64 MFMA)"));
+        MCMB_TRY(launch_guard("l1 contraction", sep_blocks(T.nt3, GS), 256));
+        if (T.contig3)
+            SEP_GEMM(true, T.nt3, tab + T.o_aidx3, c->sg_Y.as<double>(), NPL, c->sg_G.as<double>(), NPL, NP,
+                     (int)NPL, tab + T.o_task3, tab + T.o_tile3, b > 0 ? 1 : 0);
+        else
+            SEP_GEMM(false, T.nt3, tab + T.o_aidx3, c->sg_Y.as<double>(), NPL, c->sg_G.as<double>(), NPL, NP,
+                     (int)NPL, tab + T.o_task3, tab + T.o_tile3, b > 0 ? 1 : 0);
+#undef SEP_GEMM
+        MCMB_HIP(hipGetLastError());
+        launches += 3;
+    }
+    MCMB_TRY(phase("assembly"));
+    {
+        const long ng = (long)n * n;
+        MCMB_TRY(launch_guard("assembly", (unsigned long long)((ng + 255) / 256), 256));
+        hipLaunchKernelGGL(sep_assemble, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, c->stream,
+                           (const double*)c->sg_G.as<double>(), NP, ld, R, L.d_modes.as<int>(), n, d_gamma);
+        MCMB_HIP(hipGetLastError());
+        ++launches;
+    }
+    MCMB_HIP(hipEventRecord(c->evp[np], c->stream));
+    MCMB_HIP(hipEventRecord(c->ev1, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = launches;
+    c->nphases = np;
+    return MCMB_OK;
+}
+
+}  // namespace mcmb

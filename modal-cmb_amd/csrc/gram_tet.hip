@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 
 namespace mcmb {
 
@@ -334,12 +335,16 @@ void lspace_slice(const PointSet& L, int kc, int rank, int nranks, int* row_lo, 
 // uniform k-grid weights (kuniform.hip)
 int uniform_weights_launch(mcmb_ctx* c, PointSet& L, const long* rowptr, double* ws, int scaled);
 
-static int ensure_lspace_aux(mcmb_ctx* c, PointSet& L, DevBuf& rowptr, DevBuf& ws, int scaled)
+static int ensure_lspace_aux(mcmb_ctx* c, PointSet& L, DevBuf& rowptr, DevBuf& ws, int scaled,
+                             bool rowptr_ready = false)
 {
-    std::vector<long> rp(L.rows.size() + 1, 0);
-    for (size_t r = 0; r < L.rows.size(); ++r) rp[r + 1] = rp[r] + L.rows[r].nk;
-    MCMB_TRY(rowptr.alloc(sizeof(long) * rp.size()));
-    MCMB_HIP(hipMemcpyAsync(rowptr.p, rp.data(), sizeof(long) * rp.size(), hipMemcpyHostToDevice, c->stream));
+    if (!rowptr_ready) {
+        std::vector<long> rp(L.rows.size() + 1, 0);
+        for (size_t r = 0; r < L.rows.size(); ++r) rp[r + 1] = rp[r] + L.rows[r].nk;
+        MCMB_TRY(rowptr.alloc(sizeof(long) * rp.size()));
+        // synchronous: rp is a local
+        MCMB_HIP(hipMemcpy(rowptr.p, rp.data(), sizeof(long) * rp.size(), hipMemcpyHostToDevice));
+    }
     MCMB_TRY(ws.alloc(sizeof(double) * (size_t)L.npoints));
     long nrows = (long)L.rows.size();
     dim3 grid((unsigned)((nrows + 3) / 4));
@@ -360,18 +365,23 @@ static int ensure_lspace_aux(mcmb_ctx* c, PointSet& L, DevBuf& rowptr, DevBuf& w
 int gram_mixed_launch(mcmb_ctx* c, PointSet& L, const long* d_rowptr, const double* d_ws, double* d_gamma,
                       int row_lo, int row_hi);
 
-int pointset_weights(mcmb_ctx* c, PointSet& P, DevBuf& rowptr, DevBuf& ws, int scaled)
+int pointset_weights(mcmb_ctx* c, PointSet& P, DevBuf& rowptr, DevBuf& ws, int scaled, bool rowptr_ready)
 {
-    return ensure_lspace_aux(c, P, rowptr, ws, scaled);
+    return ensure_lspace_aux(c, P, rowptr, ws, scaled, rowptr_ready);
 }
 
 int gram_tet_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gamma, int row_lo, int row_hi)
 {
-    // the factored contraction (row moments x pair products) wherever the
-    // basis width allows it, unless MCMB_GRAM_DIRECT=1 asks for the direct
-    // point-by-point contraction below
-    const char* direct = getenv("MCMB_GRAM_DIRECT");
-    if (gram_factored_supported(L, precision) && !(direct && atoi(direct) == 1))
+    c->nphases = 0;
+    // FP64: the separable contraction wherever the basis width allows it,
+    // else the factored one, else the direct point-by-point contraction below.
+    // MCMB_GRAM_PATH = separable | factored | direct selects one explicitly.
+    const char* path = getenv("MCMB_GRAM_PATH");
+    const bool want_sep = !path || !*path || !strcmp(path, "separable");
+    const bool want_fac = !path || !*path || !strcmp(path, "factored");
+    if (want_sep && gram_separable_supported(L, precision))
+        return gram_separable_launch(c, L, precision, d_gamma, row_lo, row_hi);
+    if (want_fac && gram_factored_supported(L, precision))
         return gram_factored_launch(c, L, precision, d_gamma, row_lo, row_hi);
     DevBuf rowptr, ws;
     MCMB_TRY(ensure_lspace_aux(c, L, rowptr, ws, 1));

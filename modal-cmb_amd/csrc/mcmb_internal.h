@@ -36,6 +36,12 @@ inline int launch_guard(const char* what, unsigned long long blocks, unsigned th
     return MCMB_OK;
 }
 
+// Index of the unordered basis pair (x, y), x <= y < R: x-major, y from x.
+__host__ __device__ __forceinline__ int pair_index(int x, int y, int R)
+{
+    return x * R - x * (x - 1) / 2 + (y - x);
+}
+
 // One row of the ordered tetrapyd: fixed (i, j), k = k0 + kstep * t, t < nk.
 struct RowDesc {
     int i, j, k0, nk;
@@ -76,6 +82,24 @@ struct PointSet {
 };
 typedef PointSet LSpace;
 
+// Device-side tables of one block of the separable Gram (offsets in ints
+// into the context's table buffer) and the plan they belong to.
+struct SepBlock {
+    int row0 = 0, nrows = 0, nruns = 0, ngroups = 0, nt2 = 0, nt3 = 0;
+    bool contig2 = true, contig3 = true;
+    size_t o_groups = 0, o_aidx2 = 0, o_aidx3 = 0, o_task2 = 0, o_task3 = 0, o_tile2 = 0, o_tile3 = 0;
+};
+struct SepPlan {
+    const void* ps = nullptr;   // point set the tables were built for
+    long gen = -1;              // its generation
+    int row_lo = -1, row_hi = -1;
+    size_t max_tab = 0;         // ints per block in the table buffer
+    long max_rows = 0, max_runs = 0;
+    std::vector<SepBlock> blocks;
+};
+
+constexpr int MAX_PHASES = 8;
+
 }  // namespace mcmb
 
 struct mcmb_ctx {
@@ -94,6 +118,13 @@ struct mcmb_ctx {
     mcmb::DevBuf d_partial;
     mcmb::DevBuf d_gamma_tmp;
     mcmb::DevBuf d_aux;
+    // separable Gram: weights, row pointers, pair products, stage outputs, tables
+    mcmb::DevBuf sg_ws, sg_rp, sg_U, sg_M, sg_Y, sg_G, sg_tab;
+    mcmb::SepPlan sep;
+    // per-launch timing of the last call that records phases (event k+1 - event k)
+    hipEvent_t evp[mcmb::MAX_PHASES + 1] = {};
+    int nphases = 0;
+    const char* phase_names[mcmb::MAX_PHASES] = {};
 };
 
 namespace mcmb {
@@ -106,10 +137,15 @@ int upload_pointset(mcmb_ctx* ctx, PointSet& P, const std::vector<int>& order);
 int gram_gl_launch(mcmb_ctx* ctx, int npt, double* d_gamma);
 int set_device(mcmb_ctx* ctx);
 // Per-point weights of P (walk order) into ws, scaled = sqrt(w)/6 if nonzero;
-// rowptr[r] = first point of row r.  Weights by the point set's kind.
-int pointset_weights(mcmb_ctx* ctx, PointSet& P, DevBuf& rowptr, DevBuf& ws, int scaled);
+// rowptr[r] = first point of row r (built and uploaded unless rowptr_ready).
+// Weights by the point set's kind.
+int pointset_weights(mcmb_ctx* ctx, PointSet& P, DevBuf& rowptr, DevBuf& ws, int scaled, bool rowptr_ready = false);
 // Factored Gram (row moments x pair products): whether it handles this basis
 // width and precision, and the launch over rows [row_lo, row_hi).
 bool gram_factored_supported(const PointSet& P, int precision);
 int gram_factored_launch(mcmb_ctx* ctx, PointSet& P, int precision, double* d_gamma, int row_lo, int row_hi);
+// Separable Gram (contractions over l3, then l2, then l1 of the pair products):
+// whether it handles this basis width and precision, and the launch.
+bool gram_separable_supported(const PointSet& P, int precision);
+int gram_separable_launch(mcmb_ctx* ctx, PointSet& P, int precision, double* d_gamma, int row_lo, int row_hi);
 }  // namespace mcmb

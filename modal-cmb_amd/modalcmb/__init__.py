@@ -65,6 +65,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_lspace_weights.argtypes = [vp, vp, C.c_long]
         lib.mcmb_lspace_weights.restype = C.c_long
         lib.mcmb_last_kernel_ms.argtypes = [vp, C.POINTER(dp), C.POINTER(ip)]
+        lib.mcmb_last_phases.argtypes = [vp, ip, C.POINTER(dp), C.POINTER(C.c_char_p)]
         lib.mcmb_orthonormalise.argtypes = [vp, ip, vp, vp, vp, vp, C.POINTER(ip)]
         lib.mcmb_cholesky_solve.argtypes = [vp, ip, vp, ip, vp, vp]
         lib.mcmb_eigen_sym.argtypes = [vp, ip, vp, vp, vp, C.POINTER(ip)]
@@ -107,7 +108,7 @@ EXPORTED_SYMBOLS = (
     "mcmb_last_error", "mcmb_status_string", "mcmb_create", "mcmb_destroy", "mcmb_sync",
     "mcmb_lspace_set", "mcmb_gamma_tet", "mcmb_gamma_gl", "mcmb_lspace_npoints",
     "mcmb_lspace_points", "mcmb_lspace_weights", "mcmb_orthonormalise", "mcmb_cholesky_solve",
-    "mcmb_last_kernel_ms", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
+    "mcmb_last_kernel_ms", "mcmb_last_phases", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
     "mcmb_gauss_legendre", "mcmb_gamma_tet_slice", "mcmb_kspace_set", "mcmb_kspace_npoints",
     "mcmb_kgram", "mcmb_project_fwd", "mcmb_project_bwd", "mcmb_eigen_sym", "mcmb_orthonormalise_late",
     "mcmb_kuniform_set", "mcmb_kuniform_info", "mcmb_kuniform_npoints", "mcmb_kuniform_weights",
@@ -238,6 +239,16 @@ class Context:
         if r < 0:
             _check(int(r))
         return out
+
+    def last_phases(self) -> list[tuple[str, float]]:
+        """(name, ms) per phase of the last call that records phases."""
+        cap = 16
+        ms = (C.c_double * cap)()
+        names = (C.c_char_p * cap)()
+        k = _lib.mcmb_last_phases(self._h, cap, ms, names)
+        if k < 0:
+            _check(k)
+        return [(names[i].decode(), float(ms[i])) for i in range(min(k, cap))]
 
     def last_kernel_ms(self) -> tuple[float, int]:
         ms, n = C.c_double(), C.c_int()

@@ -156,41 +156,38 @@ def test_gamma_gl_rejects_inexact_node_count(ctx):
         ctx.gamma_gl(npt=100)
 
 
-@pytest.fixture
-def direct_env():
-    """Route the FP64 Gram through the direct point-by-point contraction."""
-    old = os.environ.get("MCMB_GRAM_DIRECT")
-    os.environ["MCMB_GRAM_DIRECT"] = "1"
-    yield
-    if old is None:
-        del os.environ["MCMB_GRAM_DIRECT"]
-    else:
-        os.environ["MCMB_GRAM_DIRECT"] = old
-
-
-def _gram_pair(ctx, prec):
-    """(factored, direct) evaluations of the same Gram."""
-    Gf = ctx.gamma_tet(prec)
-    os.environ["MCMB_GRAM_DIRECT"] = "1"
+def _gram_paths(ctx, prec, paths=("separable", "factored", "direct")):
+    """The same Gram through each contraction order named in paths."""
+    out = {}
+    old = os.environ.get("MCMB_GRAM_PATH")
     try:
-        Gd = ctx.gamma_tet(prec)
+        for p in paths:
+            os.environ["MCMB_GRAM_PATH"] = p
+            out[p] = ctx.gamma_tet(prec)
     finally:
-        del os.environ["MCMB_GRAM_DIRECT"]
-    return Gf, Gd
+        if old is None:
+            os.environ.pop("MCMB_GRAM_PATH", None)
+        else:
+            os.environ["MCMB_GRAM_PATH"] = old
+    return out
 
 
 @pytest.mark.parametrize("lmin,L,R,mod,h2", [(2, 500, 10, True, 0), (3, 401, 12, False, 1), (0, 257, 30, True, 0)])
-def test_gamma_factored_vs_direct(ctx, lmin, L, R, mod, h2):
-    # two summation orders of the same FP64 sum: point by point, and row
-    # moments times pair products
+def test_gamma_contraction_orders_agree(ctx, lmin, L, R, mod, h2):
+    # three summation orders of the same FP64 sum: point by point; row moments
+    # times pair products; successive contractions over l3, l2, l1
     ws = W.lspace(L, R, mod, lmin=lmin)
     ctx.lspace_set(lmin, L, ws["q"], ws["v"], ws["order"], h2_mode=h2)
-    Gf, Gd = _gram_pair(ctx, modalcmb.FP64)
-    assert rel_elem(Gf, Gd) < 1e-12
-    assert np.array_equal(Gf, Gf.T)
+    G = _gram_paths(ctx, modalcmb.FP64)
+    assert rel_elem(G["separable"], G["direct"]) < 1e-12
+    assert rel_elem(G["factored"], G["direct"]) < 1e-12
+    for g in G.values():
+        assert np.array_equal(g, g.T)
+    # the default is the separable order, bit-identical on repeat
+    assert np.array_equal(ctx.gamma_tet(modalcmb.FP64), G["separable"])
 
 
-def test_gamma_factored_permuted_mode_order(ctx):
+def test_gamma_separable_permuted_mode_order(ctx):
     # modes listed in any order, with their indices in any arrangement
     ws = W.lspace(300, 8, True)
     order = ws["order"].copy()
@@ -199,31 +196,46 @@ def test_gamma_factored_permuted_mode_order(ctx):
     for m in range(len(order)):
         order[m] = order[m][rng.permutation(3)]
     ctx.lspace_set(2, 300, ws["q"], ws["v"], order)
-    Gf, Gd = _gram_pair(ctx, modalcmb.FP64)
-    assert rel_elem(Gf, Gd) < 1e-12
+    G = _gram_paths(ctx, modalcmb.FP64, ("separable", "direct"))
+    assert rel_elem(G["separable"], G["direct"]) < 1e-12
 
 
-def test_gamma_factored_slices(ctx):
+@pytest.mark.parametrize("nranks", [7, 400])
+def test_gamma_separable_slices(ctx, nranks):
+    # shares that cut the rows of one l1 apart
     ws = W.lspace(300, 9)
     ctx.lspace_set(2, 300, ws["q"], ws["v"], ws["order"])
     G = ctx.gamma_tet(modalcmb.FP64)
-    parts = [ctx.gamma_tet_slice(modalcmb.FP64, r, 7)[0] for r in range(7)]
-    assert rel_elem(sum(parts), G) < 1e-13
+    S = np.zeros_like(G)
+    for r in range(nranks):
+        S += ctx.gamma_tet_slice(modalcmb.FP64, r, nranks)[0]
+    assert rel_elem(S, G) < 1e-13
     ref = O.gamma_tet(2, 300, ws["q"], ws["v"], ws["order"])
     assert rel_elem(G, ref) < 1e-12
 
 
-def test_gamma_wide_basis_uses_direct(ctx, direct_env):
-    # R = 34 is past the factored kernel's pair capacity: the call still works
-    # (direct contraction) and agrees with the oracle
-    ws = W.lspace(60, 34)
-    order = ws["order"][::37].copy()           # 193 modes over all 34 basis columns
-    assert order.max() == 33
-    ctx.lspace_set(2, 60, ws["q"], ws["v"], order)
-    G = ctx.gamma_tet(modalcmb.FP64)
-    del os.environ["MCMB_GRAM_DIRECT"]
+def test_separable_phases_and_repeat_calls(ctx):
+    # per-phase event timings cover the whole call; the cached tables are
+    # rebuilt when the setup or the rows change
+    ws = W.lspace(300, 10, True)
+    ctx.lspace_set(2, 300, ws["q"], ws["v"], ws["order"])
+    G1 = ctx.gamma_tet(modalcmb.FP64)
+    ph = ctx.last_phases()
+    names = [p for p, _ in ph]
+    assert names[0] == "weights" and names[-1] == "assembly" and len(ph) == 6
+    total, _ = ctx.last_kernel_ms()
+    assert sum(t for _, t in ph) == pytest.approx(total, rel=0.05, abs=0.05)
+    part, _ = ctx.gamma_tet_slice(modalcmb.FP64, 1, 3)
+    assert np.array_equal(ctx.gamma_tet(modalcmb.FP64), G1)
+    ws2 = W.lspace(300, 10, False)
+    ctx.lspace_set(2, 300, ws2["q"], ws2["v"], ws2["order"])
     G2 = ctx.gamma_tet(modalcmb.FP64)
-    os.environ["MCMB_GRAM_DIRECT"] = "1"
-    assert np.array_equal(G, G2)
-    ref = O.gamma_tet(2, 60, ws["q"], ws["v"], order)
-    assert rel_elem(G, ref) < 1e-12
+    ref = O.gamma_tet(2, 300, ws2["q"], ws2["v"], ws2["order"])
+    assert rel_elem(G2, ref) < 1e-12
+    # a Gram through another contraction order records no phases
+    os.environ["MCMB_GRAM_PATH"] = "factored"
+    try:
+        ctx.gamma_tet(modalcmb.FP64)
+    finally:
+        del os.environ["MCMB_GRAM_PATH"]
+    assert ctx.last_phases() == []

@@ -1,41 +1,42 @@
 """Benchmark of the Gamma and shape-projection hot path on one MI355X.
 
-Headline (value): seconds per mixed-precision tetrapyd Gamma at lmax=2500 with
-4960 modes (r <= s <= t < 30), 653,383,126 tetrapyd points.  One step = one full
-Gamma; the 1D basis, weights and mode table are resident on the device before
-the timed region and the Gram comes back to the host each step.
+Headline (value): seconds per FP64 tetrapyd Gamma at lmax=2500 with 4960 modes
+(r <= s <= t < 30), 653,383,126 tetrapyd points, by the separable contraction
+(row moments over l3, then FP64 MFMA contractions over l2 and l1 of the basis
+pair products, 36-term assembly).  One step = one full Gamma computed from the
+resident basis, per-leg weights and mode table (the exact 3j weights are
+recomputed inside the step) into a device buffer; the step is the C-ABI call
+mcmb_gamma_tet, timed on the host around the call.
 
 Reported beside it:
-  fp64_gpu_s          FP64 tetrapyd Gamma on the GPU (one contiguous slice of
-                      the chunk list timed, extrapolated by its work fraction;
-                      --fp64-full times the whole Gram)
-  speedup_vs_fp64     fp64_gpu_s / value
-  fp64_gl_gpu_s       the same Gamma by the Gauss-Legendre factorisation in
-                      FP64 (exact 3j weights make it identical); it is the
-                      exact reference for the errors below
-  gamma_err_mixed     max_mn |G_mixed - G_ref| / sqrt(G_mm G_nn) and the
-                      Frobenius relative error, full Gram
-  eig_err_mixed       eigenvalues of C = D^-1 G D^-1 (D = sqrt diag G) from the
-                      GPU Jacobi eigen-decomposition, mixed vs reference,
-                      relative error over the well-determined ones
-  beta                orthonormalised coefficients beta^R = lambda alpha with
-                      alpha = G_ref c (seeded c), lambda from the late-side
-                      orthonormalisation (PCA fallback, as the reference takes
-                      when the normalised Gram has negative eigenvalues) of the
-                      mixed and of the reference Gram; the prim-side Cholesky
-                      status of the reference Gram
+  gamma_kernel_ms     HIP-event time of the call and its phases
+  fp64_factored_s     the same FP64 Gamma by row moments x pair products
+  fp64_direct_s       the point-by-point FP64 contraction on the GPU (the
+                      reference's algorithm), timed on one slice of the rows
+                      and extrapolated by its work fraction
+  mixed_direct_s      the point-by-point contraction in mixed precision (BF16
+                      hi+lo x3, FP32 tiles, FP64 sums), same slice method
+  gamma_err           max_mn |G - G_x| / sqrt(G_mm G_nn) between the
+                      separable Gram and the factored FP64 Gram (full) and the
+                      Gauss-Legendre factorised FP64 Gram (full); between the
+                      factored FP64 slice and the direct FP64 and mixed slices
+                      (same rows)
+  eig / beta          eigenvalues of C = D^-1 G D^-1 and the orthonormalised
+                      coefficients beta^R = lambda alpha (alpha = G c, seeded
+                      c; lambda from the late-side orthonormalisation with the
+                      PCA branch) of the separable Gram against those of the
+                      factored Gram and of the GL Gram; prim-side Cholesky
   projection          256-shape batch (omega, phi) at 300 GL nodes, 4960 modes:
-                      forward with tangents (alpha, d alpha / d theta) and the
-                      adjoint, FP32 MFMA with FP64 sums; gradients/s and the
-                      error against the FP64 path
-  roofline            the mixed Gram kernel against BF16 dense MFMA peak
-                      (algorithmic flops N_tet * n * (n+1) per Gamma)
-  cpu_baseline        the FP64 C oracle (same direct tetrapyd algorithm) on
-                      all host cores over a bounded sample of i-rows,
-                      extrapolated by point count
+                      forward with tangents and the adjoint, FP32 MFMA with
+                      FP64 sums; gradients/s and the error against FP64
+  roofline            the l2 contraction (the largest phase) against the FP64
+                      dense MFMA peak, algorithmic flops 2 NP^2 x rows
+  cpu_baseline        the FP64 C oracle (direct tetrapyd algorithm) on all
+                      host cores over a bounded sample of i-rows, extrapolated
+                      by point count
 
-Usage: python bench.py [--gpus 1] [--steps 1] [--warmup 1] [--fp64-slices 16]
-                       [--fp64-full] [--cpu-seconds 20] [--no-cpu] [--config 3]
+Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--slices 32]
+                       [--cpu-seconds 20] [--no-cpu] [--no-projection] [--config 3]
 """
 from __future__ import annotations
 
@@ -52,7 +53,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "modal-cmb_amd"))
 
 METRIC = "s per Gamma (lmax 2500, 4960 modes) mixed vs FP64 GPU at 1e-10 err; grads/s"
-PEAK_BF16_TFLOPS = 2500.0
+PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 
 
@@ -66,17 +67,23 @@ def rel_elem(A, B):
     return float((np.abs(A - B) / d).max())
 
 
-def hbm_traffic(workload):
-    """HBM bytes per launch of the mixed Gram kernel from the committed PMC
+def hbm_traffic(workload, fname="gamma_sep_l2_hbm_traffic.md"):
+    """HBM bytes per launch of the dominant kernel from the committed PMC
     summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
-    p = os.path.join(ROOT, "profiles", "gamma_tet_mixed_hbm_traffic.md")
+    p = os.path.join(ROOT, "profiles", fname)
     if not os.path.exists(p):
         return None
     for line in open(p):
-        m = re.match(r"\s*traffic_bytes_per_gamma\[(\w+)\]\s*=\s*([0-9.eE+-]+)", line)
+        m = re.match(r"\s*traffic_bytes_per_launch\[(\w+)\]\s*=\s*([0-9.eE+-]+)", line)
         if m and m.group(1) == workload:
             return float(m.group(2))
     return None
+
+
+def lspace_rows(L, lmin=2):
+    """Rows (l1, l2) of T(lmin, L), as the walk holds them."""
+    i = np.arange(lmin, L + 1)
+    return int(np.maximum(0, L - (i & 1) - i + 1).sum())
 
 
 def cpu_baseline(ws, L, target_s):
@@ -196,7 +203,7 @@ def bench_projection(modalcmb, W, args):
 
 
 def traffic_program(config):
-    """Calibration copies and one mixed Gamma, nothing else on the device.
+    """Calibration copies and one FP64 Gamma, nothing else on the device.
     torch (only used for the plain copy kernel) is imported before the
     library so that the process holds one HIP runtime."""
     import torch
@@ -213,26 +220,38 @@ def traffic_program(config):
     with modalcmb.Context(0) as ctx:
         ctx.lspace_set(2, cfg["L"], ws["q"], ws["v"], ws["order"])
         t = time.time()
-        ctx.gamma_tet(modalcmb.MIXED)
-        print(f"traffic program: 3 x 1 GiB copies, mixed Gamma lmax={cfg['L']} modes={ws['order'].shape[0]} "
-              f"in {time.time() - t:.2f} s", flush=True)
+        ctx.gamma_tet(modalcmb.FP64)
+        print(f"traffic program: 3 x 1 GiB copies, FP64 Gamma lmax={cfg['L']} modes={ws['order'].shape[0]} "
+              f"in {time.time() - t:.3f} s", flush=True)
+
+
+def with_path(path, fn):
+    """Run fn() with the Gram contraction order MCMB_GRAM_PATH=path."""
+    old = os.environ.get("MCMB_GRAM_PATH")
+    os.environ["MCMB_GRAM_PATH"] = path
+    try:
+        return fn()
+    finally:
+        if old is None:
+            del os.environ["MCMB_GRAM_PATH"]
+        else:
+            os.environ["MCMB_GRAM_PATH"] = old
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=3)
-    ap.add_argument("--fp64-slices", type=int, default=16,
-                    help="time FP64 on one of this many contiguous slices and extrapolate")
-    ap.add_argument("--fp64-full", action="store_true", help="time the full FP64 Gamma")
+    ap.add_argument("--slices", type=int, default=32,
+                    help="time the direct contractions on one of this many row slices and extrapolate")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-projection", action="store_true")
     ap.add_argument("--traffic-program", action="store_true",
                     help="for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes: three 1 GiB device copies "
-                         "(calibration), then one mixed Gamma of the config; prints one line")
+                         "(calibration), then one FP64 Gamma of the config; prints one line")
     args = ap.parse_args()
     if args.traffic_program:
         traffic_program(args.config)
@@ -249,129 +268,153 @@ def main():
     L, R = cfg["L"], cfg["R"]
     ws = W.lspace(L, R, cfg["modulated"])
     n = ws["order"].shape[0]
+    NP = R * (R + 1) // 2
+    rows = lspace_rows(L)
     ctx = modalcmb.Context(0)
     ctx.lspace_set(2, L, ws["q"], ws["v"], ws["order"])
     npts = ctx.npoints()
-    flops = float(npts) * n * (n + 1)
     workload = f"gamma_l{L}_n{n}"
+    log(f"config {args.config}: lmax={L} modes={n} points={npts} rows={rows}")
 
-    log(f"config {args.config}: lmax={L} modes={n} points={npts}")
+    # headline: FP64 separable Gamma into a device buffer
+    d_gamma = ctx.device_alloc(n * n * 8)
     for _ in range(args.warmup):
-        ctx.gamma_tet(modalcmb.MIXED)
-        log(f"warmup mixed Gamma done ({ctx.last_kernel_ms()[0] / 1e3:.3f} s kernel)")
+        ctx.gamma_tet_device(modalcmb.FP64, d_gamma)
+    log(f"warmup done ({ctx.last_kernel_ms()[0]:.2f} ms kernel)")
     ctx.sync()
-    kms = []
+    kms, phases = [], {}
     t0 = time.time()
     for _ in range(args.steps):
-        g_mixed = ctx.gamma_tet(modalcmb.MIXED)
+        ctx.gamma_tet_device(modalcmb.FP64, d_gamma)
         kms.append(ctx.last_kernel_ms()[0])
-        log(f"timed mixed Gamma step done ({kms[-1] / 1e3:.3f} s kernel)")
+        for name, ms in ctx.last_phases():
+            phases.setdefault(name, []).append(ms)
     ctx.sync()
-    t_mixed = (time.time() - t0) / args.steps
-    k_mixed = float(np.mean(kms)) / 1e3
-
-    # exact FP64 reference: Gauss-Legendre factorised Gram
+    t_step = (time.time() - t0) / args.steps
+    k_step = float(np.mean(kms))
+    phases = {k: round(float(np.mean(v)), 3) for k, v in phases.items()}
+    G = ctx.device_to_host(d_gamma, np.empty((n, n)))
+    ctx.device_free(d_gamma)
+    log(f"separable FP64 Gamma: {t_step * 1e3:.2f} ms per step, {k_step:.2f} ms kernel, phases {phases}")
     t = time.time()
-    g_ref = ctx.gamma_gl()
+    G_host = ctx.gamma_tet(modalcmb.FP64)
+    t_host = time.time() - t
+    assert np.array_equal(G, G_host)
+
+    # the same Gram by the other contraction orders
+    t = time.time()
+    G_gl = ctx.gamma_gl()
     t_gl = time.time() - t
-    log(f"GL-factorised FP64 Gamma {t_gl:.3f} s")
-
-    # FP64 tetrapyd path on the GPU
-    if args.fp64_full:
-        t = time.time()
-        g64 = ctx.gamma_tet(modalcmb.FP64)
-        fp64_s = time.time() - t
-        fp64_note = "full"
-        fp64_err = rel_elem(g64, g_ref)
-    else:
-        s = args.fp64_slices
-        r = s // 2
-        t = time.time()
-        _, frac = ctx.gamma_tet_slice(modalcmb.FP64, r, s)
-        t64 = time.time() - t
-        fp64_s = t64 / frac
-        fp64_note = f"extrapolated from slice {r} of {s} ({frac:.4f} of the point chunks, {t64:.2f} s)"
-        fp64_err = None
-    log(f"fp64 tetrapyd path: {fp64_note}")
-
-    err = rel_elem(g_mixed, g_ref)
-    frob = float(np.linalg.norm(g_mixed - g_ref) / np.linalg.norm(g_ref))
+    t = time.time()
+    G_fac = with_path("factored", lambda: ctx.gamma_tet(modalcmb.FP64))
+    t_fac = time.time() - t
+    log(f"GL {t_gl:.3f} s, factored {t_fac:.2f} s")
+    s_cnt = args.slices
+    r_sl = s_cnt // 2
+    # slices by the factored order (the separable launches stay full-size, so
+    # their profiled averages are per full Gamma)
+    S_fac, frac = with_path("factored", lambda: ctx.gamma_tet_slice(modalcmb.FP64, r_sl, s_cnt))
+    t = time.time()
+    S_dir, _ = with_path("direct", lambda: ctx.gamma_tet_slice(modalcmb.FP64, r_sl, s_cnt))
+    t_dir = time.time() - t
+    t = time.time()
+    S_mix, _ = ctx.gamma_tet_slice(modalcmb.MIXED, r_sl, s_cnt)
+    t_mix = time.time() - t
+    log(f"direct slice {r_sl}/{s_cnt} ({frac:.4f}): FP64 {t_dir:.2f} s, mixed {t_mix:.2f} s")
+    gamma_err = {
+        "vs_fp64_factored_full": rel_elem(G, G_fac),
+        "fp64_factored_vs_direct_slice": rel_elem(S_fac, S_dir),
+        "mixed_direct_vs_fp64_factored_slice": rel_elem(S_mix, S_fac),
+        "vs_fp64_gauss_legendre_full": rel_elem(G, G_gl),
+        "gl_note": "the GL factorised Gram carries ~1e-10 rounding of its own at lmax 2500 (it moves by that "
+                   "much between 3L/2+1, 2L+1 and 3L nodes); the two tetrapyd orders agree to ~1e-13",
+        "norm": "max_mn |G - G_x| / sqrt(|G_x,mm G_x,nn|)",
+    }
 
     # late-side orthonormalisation (eigen-decomposition on the GPU, PCA branch
-    # when the normalised Gram has a negative eigenvalue) of both Grams
+    # when the normalised Gram has a negative eigenvalue)
     t = time.time()
-    r_ref, lam_ref, _, _, e_ref = ctx.orthonormalise_late(g_ref, pca_flag=1)
+    r_sep, lam_sep, _, _, e_sep = ctx.orthonormalise_late(G, pca_flag=1)
     t_late = time.time() - t
-    r_mix, lam_mix, _, _, e_mix = ctx.orthonormalise_late(g_mixed, pca_flag=1)
+    r_fac, lam_fac, _, _, e_fac = ctx.orthonormalise_late(G_fac, pca_flag=1)
+    r_gl, lam_gl, _, _, e_gl = ctx.orthonormalise_late(G_gl, pca_flag=1)
     log(f"late orthonormalisation (Jacobi + PCA) {t_late:.2f} s per Gram")
-    lmax = float(e_ref[0])
-    eig = {"lambda_max": lmax, "max_abs_err_over_lambda_max": float(np.abs(e_mix - e_ref).max() / lmax),
-           "negative_eigenvalues_fp64": int((e_ref < 0).sum())}
-    for thr in (1e-2, 1e-4):
-        k = e_ref > thr * lmax
-        eig[f"rel_err_lambda_gt_{thr:g}_lmax"] = {"count": int(k.sum()),
-                                                  "max": float((np.abs(e_mix[k] - e_ref[k]) / e_ref[k]).max())}
-    # beta^R = lambda alpha for alpha = Gamma_ref c (seeded c): reference vs mixed
+    lmax = float(e_fac[0])
     cvec = np.random.default_rng(20251202).standard_normal(n)
-    alpha = g_ref @ cvec
-    b_ref, b_mix = lam_ref @ alpha, lam_mix @ alpha
+    alpha = G_fac @ cvec
+    b_sep, b_fac, b_gl = lam_sep @ alpha, lam_fac @ alpha, lam_gl @ alpha
+    eig = {"lambda_max": lmax, "negative_eigenvalues_fp64": int((e_fac < 0).sum())}
     beta = {"method": "late-side PCA (ortho_late_mpi.c:784-805): the normalised Gram has negative eigenvalues",
-            "rank_ref": int(r_ref), "rank_mixed": int(r_mix), "orthonormalise_late_s": round(t_late, 3),
-            "rel_err_l2_all": float(np.linalg.norm(b_mix - b_ref) / np.linalg.norm(b_ref))}
-    for thr in (1e-2, 1e-4, 1e-6):
-        k = e_ref > thr * lmax
-        beta[f"rel_err_l2_lambda_gt_{thr:g}_lmax"] = float(np.linalg.norm(b_mix[k] - b_ref[k]) /
-                                                          np.linalg.norm(b_ref[k]))
+            "rank_separable": int(r_sep), "rank_factored": int(r_fac), "orthonormalise_late_s": round(t_late, 3)}
+    for name, e_x, b_x in (("vs_factored", e_fac, b_fac), ("vs_gauss_legendre", e_gl, b_gl)):
+        eig[name] = {"max_abs_err_over_lambda_max": float(np.abs(e_sep - e_x).max() / lmax)}
+        beta[name] = {"rel_err_l2_all": float(np.linalg.norm(b_sep - b_x) / np.linalg.norm(b_x))}
+        for thr in (1e-2, 1e-4, 1e-6):
+            k = e_x > thr * lmax
+            eig[name][f"rel_err_lambda_gt_{thr:g}_lmax"] = float((np.abs(e_sep[k] - e_x[k]) / e_x[k]).max())
+            beta[name][f"rel_err_l2_lambda_gt_{thr:g}_lmax"] = float(np.linalg.norm(b_sep[k] - b_x[k]) /
+                                                                   np.linalg.norm(b_x[k]))
     # the prim-side Cholesky (ortho_prim_mpi.c:198-284) on the same Gram
     t = time.time()
     try:
-        ctx.orthonormalise(g_ref)
+        ctx.orthonormalise(G)
         beta["cholesky_prim"] = "ok"
     except modalcmb.NotSPDError as e:
         beta["cholesky_prim"] = "not positive definite in FP64: " + str(e)
     beta["cholesky_prim_s"] = round(time.time() - t, 4)
-    beta["kappa_lower_bound"] = lmax / max(float(np.abs(e_ref).min()), 1e-300)
+    beta["kappa_lower_bound"] = lmax / max(float(np.abs(e_fac).min()), 1e-300)
     log("orthonormalisation done")
     ctx.close()
 
-    achieved = flops / k_mixed / 1e12
+    l2_name = "l2 contraction (FP64 MFMA)"
+    l2_ms = phases.get(l2_name)
+    l2_flops = 2.0 * NP * NP * rows
+    achieved = l2_flops / (l2_ms * 1e-3) / 1e12 if l2_ms else None
+    fp64_direct_s = t_dir / frac
+    mixed_direct_s = t_mix / frac
     out = {
         "metric": METRIC,
-        "value": round(t_mixed, 4),
+        "value": round(t_step, 5),
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t_mixed * 1e3, 2),
+        "ms_per_step": round(t_step * 1e3, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16",
+        "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"tetrapyd Gamma, mixed precision (bf16 hi+lo x3, fp32 accumulate, fp64 sums), "
-                               f"lmax={L}, modes={n} (r<=s<=t<{R}), exact 3j weights",
-                   "lmax": L, "modes": n, "tetrapyd_points": int(npts)},
-        "fp64_gpu_s": round(fp64_s, 3),
-        "fp64_gpu_note": fp64_note,
-        "speedup_vs_fp64": round(fp64_s / t_mixed, 3),
-        "fp64_gl_gpu_s": round(t_gl, 4),
-        "gamma_err_mixed": {"max_elem_rel": err, "frob_rel": frob,
-                            "vs": "FP64 Gauss-Legendre factorised Gram (full)"},
-        "eig_err_mixed": eig,
+        "config": {"workload": f"tetrapyd Gamma, FP64 separable contraction, lmax={L}, modes={n} "
+                               f"(r<=s<=t<{R}), exact 3j weights, output in HBM",
+                   "lmax": L, "modes": n, "tetrapyd_points": int(npts), "tetrapyd_rows": rows, "pairs": NP},
+        "gamma_kernel_ms": round(k_step, 3),
+        "phases_ms": phases,
+        "gamma_with_host_copy_s": round(t_host, 4),
+        "fp64_factored_s": round(t_fac, 3),
+        "fp64_direct_s": round(fp64_direct_s, 2),
+        "mixed_direct_s": round(mixed_direct_s, 2),
+        "direct_note": f"slice {r_sl} of {s_cnt} ({frac:.4f} of the point chunks) timed and extrapolated",
+        "speedup_vs_fp64_direct": round(fp64_direct_s / t_step, 1),
+        "speedup_vs_mixed_direct": round(mixed_direct_s / t_step, 1),
+        "fp64_gl_s": round(t_gl, 4),
+        "mixed_note": "the separable FP64 contraction is faster than the mixed-precision direct contraction and "
+                      "meets the 1e-10 bar, so the headline is FP64",
+        "gamma_err": gamma_err,
+        "eig": eig,
         "beta": beta,
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": PEAK_BF16_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16_TFLOPS, 4),
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2) if achieved else None, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4) if achieved else None,
                      "traffic": hbm_traffic(workload),
-                     "kernel": "gram_tet_mixed_kernel (diagonal + off-diagonal tile launches)",
-                     "algorithmic_flops_per_gamma": flops, "kernel_s": round(k_mixed, 4)},
+                     "kernel": "sep_gemm, l2 contraction (FP64 MFMA 16x16x4, 96 x 96 tiles)",
+                     "algorithmic_flops_per_launch": l2_flops, "kernel_ms": l2_ms},
     }
-    if fp64_err is not None:
-        out["gamma_err_fp64_tet"] = fp64_err
     if not args.no_projection:
         out["projection"] = bench_projection(modalcmb, W, args)
         log("projection done")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ws, L, args.cpu_seconds)
+        out["speedup_vs_cpu"] = round(out["cpu_baseline"]["value"] / t_step, 1)
         log("cpu baseline done")
     print(json.dumps(out), flush=True)
 

@@ -18,6 +18,8 @@
 #ifndef MODAL_CMB_H
 #define MODAL_CMB_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -226,6 +228,15 @@ int mcmb_gauss_legendre(int n, double* x, double* w);
 /* Milliseconds spent in the dominant kernel of the last mcmb_gamma_tet /
  * mcmb_gamma_gl call (HIP events on the context stream) and its launch count. */
 int mcmb_last_kernel_ms(mcmb_ctx* ctx, double* ms, int* launches);
+/* ---- device buffers for callers without their own allocator ------------- */
+/* Allocate / free device memory on the context's GPU, and copy a device buffer
+ * to the host (ordered after the context stream's work).  No reference
+ * counterpart: they let a C caller keep the Gram in HBM (mcmb_gamma_tet with a
+ * device output) without linking HIP itself. */
+int mcmb_device_alloc(mcmb_ctx* ctx, size_t bytes, void** dev);
+int mcmb_device_free(mcmb_ctx* ctx, void* dev);
+int mcmb_device_to_host(mcmb_ctx* ctx, void* host, const void* dev, size_t bytes);
+
 /* Per-phase milliseconds of the last call that records phases (the FP64
  * separable Gram: weights, pair products, l3 / l2 / l1 contractions, assembly),
  * each between HIP events on the context stream.  Writes up to cap entries of

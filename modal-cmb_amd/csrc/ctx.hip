@@ -1,5 +1,6 @@
 // Context, error reporting and l-space setup of libmodalcmb.
 #include "mcmb_internal.h"
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -16,6 +17,12 @@ int fail(int status, const char* fmt, ...)
     vsnprintf(g_msg, sizeof(g_msg), fmt, ap);
     va_end(ap);
     return status;
+}
+
+long next_generation()
+{
+    static std::atomic<long> counter{0};
+    return ++counter;
 }
 
 int DevBuf::alloc(size_t b)
@@ -205,7 +212,7 @@ int mcmb_lspace_set(mcmb_ctx* c, int lmin, int lmax, int ldq, const double* q, c
     MCMB_HIP(hipMemcpyAsync(L.d_ftab.p, ftab.data(), sizeof(double) * ftab.size(), hipMemcpyHostToDevice, c->stream));
     MCMB_TRY(upload_pointset(c, L, L.order));
     L.ready = true;
-    L.generation += 1;
+    L.generation = next_generation();
     return MCMB_OK;
 }
 
@@ -230,6 +237,39 @@ int mcmb_last_kernel_ms(mcmb_ctx* c, double* ms, int* launches)
     if (!c) return fail(MCMB_ERR_ARG, "mcmb_last_kernel_ms: null context");
     if (ms) *ms = c->last_ms;
     if (launches) *launches = c->last_launches;
+    return MCMB_OK;
+}
+
+int mcmb_device_alloc(mcmb_ctx* c, size_t bytes, void** dev)
+{
+    if (!c || !dev) return fail(MCMB_ERR_ARG, "mcmb_device_alloc: null argument");
+    *dev = nullptr;
+    MCMB_TRY(set_device(c));
+    if (bytes == 0) return MCMB_OK;
+    hipError_t e = hipMalloc(dev, bytes);
+    if (e != hipSuccess) {
+        *dev = nullptr;
+        return fail(MCMB_ERR_NOMEM, "mcmb_device_alloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    return MCMB_OK;
+}
+
+int mcmb_device_free(mcmb_ctx* c, void* dev)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_device_free: null context");
+    if (!dev) return MCMB_OK;
+    MCMB_TRY(set_device(c));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    MCMB_HIP(hipFree(dev));
+    return MCMB_OK;
+}
+
+int mcmb_device_to_host(mcmb_ctx* c, void* host, const void* dev, size_t bytes)
+{
+    if (!c || (bytes && (!host || !dev))) return fail(MCMB_ERR_ARG, "mcmb_device_to_host: null argument");
+    MCMB_TRY(set_device(c));
+    if (bytes) MCMB_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
     return MCMB_OK;
 }
 

@@ -169,7 +169,8 @@ __global__ __launch_bounds__(256) void sep_rowmom(const RowDesc* __restrict__ ro
 // b+8, ... share an XCD (and its L2): the groups are dealt to the 8 XCDs in
 // turn, each group's tiles to consecutive blocks of one XCD, so the tiles of
 // a group meet in the same L2 and the XCDs get alike shares of the work.
-template <bool CONTIG>
+// STAGE (2: l2, 3: l1) only names the two launches apart in profiles.
+template <bool CONTIG, int STAGE>
 __global__ __launch_bounds__(256, 2) void sep_gemm(const double* __restrict__ U, int ldu,
                                                    const int* __restrict__ aidx, const double* __restrict__ B,
                                                    long ldb, double* __restrict__ C, long ldc, int Md, int Nd,
@@ -533,26 +534,27 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         MCMB_HIP(hipGetLastError());
         // stages 2 and 3
         const int GS = (NP + SG_TM - 1) / SG_TM;   // m tiles per B panel, both stages
-#define SEP_GEMM(CG, NT, AIDX, BP, LDB, CP, LDC, MD, ND, TASK, TILE, ACC)                                          \
-    hipLaunchKernelGGL((sep_gemm<CG>), dim3(sep_blocks(NT, GS)), dim3(256), 0, c->stream, (const double*)dU, ldu,\
+#define SEP_GEMM(CG, ST, NT, AIDX, BP, LDB, CP, LDC, MD, ND, TASK, TILE, ACC)                                 \
+    hipLaunchKernelGGL((sep_gemm<CG, ST>), dim3(sep_blocks(NT, GS)), dim3(256), 0, c->stream, (const double*)dU, \
+                       ldu,                                                                                     \
                        AIDX, (const double*)(BP), (long)(LDB), CP, (long)(LDC), MD, ND,                          \
                        reinterpret_cast<const SepTask*>(TASK), reinterpret_cast<const int4*>(TILE), NT, GS, ACC)
         if (one_block) MCMB_TRY(phase("l2 contraction (FP64 MFMA)"));
         MCMB_TRY(launch_guard("l2 contraction", sep_blocks(T.nt2, GS), 256));
         if (T.contig2)
-            SEP_GEMM(true, T.nt2, tab + T.o_aidx2, c->sg_M.as<double>(), ld, c->sg_Y.as<double>(), ld, NP, NP,
+            SEP_GEMM(true, 2, T.nt2, tab + T.o_aidx2, c->sg_M.as<double>(), ld, c->sg_Y.as<double>(), ld, NP, NP,
                      tab + T.o_task2, tab + T.o_tile2, 0);
         else
-            SEP_GEMM(false, T.nt2, tab + T.o_aidx2, c->sg_M.as<double>(), ld, c->sg_Y.as<double>(), ld, NP, NP,
+            SEP_GEMM(false, 2, T.nt2, tab + T.o_aidx2, c->sg_M.as<double>(), ld, c->sg_Y.as<double>(), ld, NP, NP,
                      tab + T.o_task2, tab + T.o_tile2, 0);
         MCMB_HIP(hipGetLastError());
         if (one_block) MCMB_TRY(phase("l1 contraction (FP64 MFMA)"));
         MCMB_TRY(launch_guard("l1 contraction", sep_blocks(T.nt3, GS), 256));
         if (T.contig3)
-            SEP_GEMM(true, T.nt3, tab + T.o_aidx3, c->sg_Y.as<double>(), NPL, c->sg_G.as<double>(), NPL, NP,
+            SEP_GEMM(true, 3, T.nt3, tab + T.o_aidx3, c->sg_Y.as<double>(), NPL, c->sg_G.as<double>(), NPL, NP,
                      (int)NPL, tab + T.o_task3, tab + T.o_tile3, b > 0 ? 1 : 0);
         else
-            SEP_GEMM(false, T.nt3, tab + T.o_aidx3, c->sg_Y.as<double>(), NPL, c->sg_G.as<double>(), NPL, NP,
+            SEP_GEMM(false, 3, T.nt3, tab + T.o_aidx3, c->sg_Y.as<double>(), NPL, c->sg_G.as<double>(), NPL, NP,
                      (int)NPL, tab + T.o_task3, tab + T.o_tile3, b > 0 ? 1 : 0);
 #undef SEP_GEMM
         MCMB_HIP(hipGetLastError());

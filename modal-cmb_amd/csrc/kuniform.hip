@@ -282,7 +282,7 @@ extern "C" int mcmb_kuniform_set(mcmb_ctx* c, int min, int max, double kmax, int
     build_prim_rows(min, max, U.rows, U.npoints);
     MCMB_TRY(upload_pointset(c, U, U.order));
     U.ready = true;
-    U.generation += 1;
+    U.generation = next_generation();
     return MCMB_OK;
 }
 

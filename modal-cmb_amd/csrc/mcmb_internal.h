@@ -60,6 +60,10 @@ struct DevBuf {
 // Weight of a tetrapyd point.
 enum WeightKind : int { W_LSPACE_EXACT = 0, W_LSPACE_ASYM = 1, W_KSPACE_GL = 2 };
 
+// A process-wide increasing counter: every new or changed point set takes a
+// fresh value, so (address, generation) never repeats, also for temporaries.
+long next_generation();
+
 // An ordered tetrapyd point set with its 1D basis and mode table:
 //  l-space: rows of T(lmin, lmax), k in steps of 2, weight permsix h^2 v v v;
 //  k-space: Gauss-Legendre node indices a <= b <= c with k_c <= k_a + k_b,
@@ -67,7 +71,7 @@ enum WeightKind : int { W_LSPACE_EXACT = 0, W_LSPACE_ASYM = 1, W_KSPACE_GL = 2 }
 struct PointSet {
     bool ready = false;
     bool kspace = false;
-    long generation = 0;
+    long generation = next_generation();
     int lmin = 0, lmax = 0;           // l-space multipole range; k-space: 0, N-1
     int ldq = 0, n = 0, h2mode = 0, R = 0, kstep = 2;
     long npoints = 0;

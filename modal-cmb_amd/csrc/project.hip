@@ -534,7 +534,7 @@ int mcmb_kspace_set(mcmb_ctx* c, int N, const double* kn, const double* kw, int 
     MCMB_TRY(upload_pointset(c, K, K.order));
     c->w_ready = 0;
     K.ready = true;
-    K.generation += 1;
+    K.generation = next_generation();
     return MCMB_OK;
 }
 

@@ -66,6 +66,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_lspace_weights.restype = C.c_long
         lib.mcmb_last_kernel_ms.argtypes = [vp, C.POINTER(dp), C.POINTER(ip)]
         lib.mcmb_last_phases.argtypes = [vp, ip, C.POINTER(dp), C.POINTER(C.c_char_p)]
+        lib.mcmb_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
+        lib.mcmb_device_free.argtypes = [vp, vp]
+        lib.mcmb_device_to_host.argtypes = [vp, vp, vp, C.c_size_t]
         lib.mcmb_orthonormalise.argtypes = [vp, ip, vp, vp, vp, vp, C.POINTER(ip)]
         lib.mcmb_cholesky_solve.argtypes = [vp, ip, vp, ip, vp, vp]
         lib.mcmb_eigen_sym.argtypes = [vp, ip, vp, vp, vp, C.POINTER(ip)]
@@ -108,7 +111,7 @@ EXPORTED_SYMBOLS = (
     "mcmb_last_error", "mcmb_status_string", "mcmb_create", "mcmb_destroy", "mcmb_sync",
     "mcmb_lspace_set", "mcmb_gamma_tet", "mcmb_gamma_gl", "mcmb_lspace_npoints",
     "mcmb_lspace_points", "mcmb_lspace_weights", "mcmb_orthonormalise", "mcmb_cholesky_solve",
-    "mcmb_last_kernel_ms", "mcmb_last_phases", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
+    "mcmb_last_kernel_ms", "mcmb_last_phases", "mcmb_device_alloc", "mcmb_device_free", "mcmb_device_to_host", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
     "mcmb_gauss_legendre", "mcmb_gamma_tet_slice", "mcmb_kspace_set", "mcmb_kspace_npoints",
     "mcmb_kgram", "mcmb_project_fwd", "mcmb_project_bwd", "mcmb_eigen_sym", "mcmb_orthonormalise_late",
     "mcmb_kuniform_set", "mcmb_kuniform_info", "mcmb_kuniform_npoints", "mcmb_kuniform_weights",
@@ -215,6 +218,19 @@ class Context:
 
     def gamma_tet_device(self, precision: int, dev_ptr: int) -> None:
         _check(_lib.mcmb_gamma_tet(self._h, precision, None, C.c_void_p(dev_ptr)))
+
+    def device_alloc(self, nbytes: int) -> int:
+        """Device buffer on the context's GPU (an integer address)."""
+        p = C.c_void_p()
+        _check(_lib.mcmb_device_alloc(self._h, nbytes, C.byref(p)))
+        return int(p.value or 0)
+
+    def device_free(self, dev_ptr: int) -> None:
+        _check(_lib.mcmb_device_free(self._h, C.c_void_p(dev_ptr)))
+
+    def device_to_host(self, dev_ptr: int, out: np.ndarray) -> np.ndarray:
+        _check(_lib.mcmb_device_to_host(self._h, _ptr(out), C.c_void_p(dev_ptr), out.nbytes))
+        return out
 
     def npoints(self) -> int:
         r = _lib.mcmb_lspace_npoints(self._h)

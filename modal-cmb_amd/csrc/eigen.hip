@@ -28,6 +28,8 @@
 #include "mcmb_internal.h"
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <numeric>
 #include <vector>
 
@@ -217,6 +219,233 @@ __global__ void pca_outputs(const double* __restrict__ V, const double* __restri
     linv[e] = j < r ? norm[i] * V[(long)j * n + i] * sqrt(ev[j]) : 0.0;
 }
 
+// ---------------------------------------------------------------------------
+// Two-sided block Jacobi.  The n_e x n_e matrix (n_e a multiple of 2 BJ_B,
+// padding rows and columns zero) is cut into nb = n_e / BJ_B blocks of BJ_B
+// indices.  A step pairs the blocks disjointly (the circle method of a
+// round-robin tournament over nb blocks: block nb-1 meets block s, block
+// (s+k) mod (nb-1) meets (s-k) mod (nb-1)), so nb-1 steps meet every pair of
+// blocks once (a sweep).  For pair k the 2 BJ_B x 2 BJ_B principal submatrix
+// S_k = A[X_k, X_k] (X_k = the indices of its two blocks) is diagonalised in
+// LDS by cyclic Jacobi, S_k = Q_k D_k Q_k^T, then
+//   A <- P^T A P,  W <- P^T W,   P = the block-diagonal of the Q_k,
+// block by block: A[X_a, X_b] <- Q_a^T A[X_a, X_b] Q_b (one workgroup per
+// a <= b, writing both symmetric blocks in place) and W[X_k, cols] <-
+// Q_k^T W[X_k, cols].  W's rows are the eigenvectors being accumulated.
+// Padding indices meet only zero entries, so no rotation touches them.  The
+// rotation and skip rules inside S_k are those of the scalar Jacobi above.
+constexpr int BJ_B = 16;            // block size
+constexpr int BJ_N = 2 * BJ_B;      // subproblem size
+constexpr int BJ_INNER = 10;        // inner sweeps at most per subproblem
+
+__host__ __device__ __forceinline__ void bj_pair(int s, int k, int nb, int& I, int& J)
+{
+    const int m = nb - 1;
+    if (k == 0) {
+        I = m;
+        J = s;
+    } else {
+        I = (s + k) % m;
+        J = (s - k + m) % m;
+    }
+}
+
+// global index of local index a of pair (I, J)
+__device__ __forceinline__ int bj_index(int a, int I, int J)
+{
+    return a < BJ_B ? I * BJ_B + a : J * BJ_B + (a - BJ_B);
+}
+
+// Diagonalise S_k for every pair k of step s: Qg[k] (BJ_N x BJ_N, row-major)
+// with S_k Q = Q D; rotations applied counted into *nrot.
+__global__ __launch_bounds__(128) void bj_solve(const double* __restrict__ A, int ne, int nb, int s,
+                                                double abs_tol, double* __restrict__ Qg,
+                                                unsigned long long* __restrict__ nrot)
+{
+    __shared__ double S[BJ_N][BJ_N + 1];
+    __shared__ double Q[BJ_N][BJ_N + 1];
+    __shared__ double rc[BJ_N / 2], rs[BJ_N / 2];
+    __shared__ int rp[BJ_N / 2], rq[BJ_N / 2];
+    __shared__ int changed;
+    const int k = blockIdx.x, tid = threadIdx.x;
+    int I, J;
+    bj_pair(s, k, nb, I, J);
+    for (int e = tid; e < BJ_N * BJ_N; e += 128) {
+        const int a = e / BJ_N, b = e % BJ_N;
+        S[a][b] = A[(long)bj_index(a, I, J) * ne + bj_index(b, I, J)];
+        Q[a][b] = a == b ? 1.0 : 0.0;
+    }
+    unsigned long long applied = 0;
+    if (tid == 0) changed = 0;
+    __syncthreads();
+    for (int sweep = 0; sweep < BJ_INNER; ++sweep) {
+        for (int r = 0; r < BJ_N - 1; ++r) {
+            if (tid < BJ_N / 2) {
+                // pair tid of round r over BJ_N local indices (circle method)
+                int p, q;
+                if (tid == 0) {
+                    p = BJ_N - 1;
+                    q = r;
+                } else {
+                    p = (r + tid) % (BJ_N - 1);
+                    q = (r - tid + BJ_N - 1) % (BJ_N - 1);
+                }
+                const double apq = S[p][q], app = S[p][p], aqq = S[q][q];
+                double c = 1.0, sn = 0.0;
+                if (fabs(apq) > fmax(1e-16 * sqrt(fabs(app * aqq)), abs_tol)) {
+                    const double tau = (aqq - app) / (2.0 * apq);
+                    const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    c = 1.0 / sqrt(1.0 + t * t);
+                    sn = t * c;
+                    changed = 1;
+                    ++applied;
+                }
+                rp[tid] = p;
+                rq[tid] = q;
+                rc[tid] = c;
+                rs[tid] = sn;
+            }
+            __syncthreads();
+            // S <- J^T S J in 2 x 2 blocks (pair u rows, pair v columns); Q <- Q J
+            for (int e = tid; e < (BJ_N / 2) * (BJ_N / 2); e += 128) {
+                const int u = e / (BJ_N / 2), v = e % (BJ_N / 2);
+                const int p = rp[u], q = rq[u], x = rp[v], y = rq[v];
+                const double cu = rc[u], su = rs[u], cv = rc[v], sv = rs[v];
+                const double s_px = S[p][x], s_py = S[p][y], s_qx = S[q][x], s_qy = S[q][y];
+                // rows: p' = c p - s q, q' = s p + c q
+                const double t_px = cu * s_px - su * s_qx, t_py = cu * s_py - su * s_qy;
+                const double t_qx = su * s_px + cu * s_qx, t_qy = su * s_py + cu * s_qy;
+                // columns: x' = c x - s y, y' = s x + c y
+                S[p][x] = cv * t_px - sv * t_py;
+                S[p][y] = sv * t_px + cv * t_py;
+                S[q][x] = cv * t_qx - sv * t_qy;
+                S[q][y] = sv * t_qx + cv * t_qy;
+            }
+            for (int e = tid; e < (BJ_N / 2) * BJ_N; e += 128) {
+                const int v = e / BJ_N, row = e % BJ_N;
+                const int x = rp[v], y = rq[v];
+                const double cv = rc[v], sv = rs[v];
+                const double qx = Q[row][x], qy = Q[row][y];
+                Q[row][x] = cv * qx - sv * qy;
+                Q[row][y] = sv * qx + cv * qy;
+            }
+            __syncthreads();
+        }
+        const int ch = changed;   // read by every thread before the reset below
+        __syncthreads();
+        if (!ch) break;
+        if (tid == 0) changed = 0;
+        __syncthreads();
+    }
+    // One Newton-Schulz step, Q <- Q (3 I - Q^T Q) / 2, takes the product of
+    // up to BJ_INNER (BJ_N - 1) rotations back to orthogonality (the error
+    // goes from k eps to (k eps)^2), so that repeated block updates do not
+    // scale A or W.  S is free after the sweeps.
+    __syncthreads();
+    for (int e = tid; e < BJ_N * BJ_N; e += 128) {
+        const int i = e / BJ_N, j = e % BJ_N;
+        double m = 0.0;
+        for (int l = 0; l < BJ_N; ++l) m = fma(Q[l][i], Q[l][j], m);
+        S[i][j] = (i == j ? 1.5 : 0.0) - 0.5 * m;
+    }
+    __syncthreads();
+    for (int e = tid; e < BJ_N * BJ_N; e += 128) {
+        const int a = e / BJ_N, b = e % BJ_N;
+        double q = 0.0;
+        for (int l = 0; l < BJ_N; ++l) q = fma(Q[a][l], S[l][b], q);
+        Qg[(long)k * BJ_N * BJ_N + e] = q;
+    }
+    if (applied) atomicAdd(nrot, applied);
+}
+
+// A[X_a, X_b] <- Q_a^T A[X_a, X_b] Q_b for pairs a <= b of step s (both
+// symmetric blocks written; in place, each block belongs to one workgroup).
+__global__ __launch_bounds__(256) void bj_update_A(double* __restrict__ A, int ne, int nb, int s,
+                                                   const double* __restrict__ Qg)
+{
+    __shared__ double Qa[BJ_N][BJ_N + 1], Qb[BJ_N][BJ_N + 1], Bm[BJ_N][BJ_N + 1], T[BJ_N][BJ_N + 1];
+    const int P = nb / 2;
+    // linear index -> (a, b), a <= b
+    int idx = blockIdx.x, a = 0;
+    while (idx >= P - a) {
+        idx -= P - a;
+        ++a;
+    }
+    const int b = a + idx;
+    int Ia, Ja, Ib, Jb;
+    bj_pair(s, a, nb, Ia, Ja);
+    bj_pair(s, b, nb, Ib, Jb);
+    const int tid = threadIdx.x;
+    for (int e = tid; e < BJ_N * BJ_N; e += 256) {
+        const int i = e / BJ_N, j = e % BJ_N;
+        Qa[i][j] = Qg[(long)a * BJ_N * BJ_N + e];
+        Qb[i][j] = Qg[(long)b * BJ_N * BJ_N + e];
+        Bm[i][j] = A[(long)bj_index(i, Ia, Ja) * ne + bj_index(j, Ib, Jb)];
+    }
+    __syncthreads();
+    // T = Qa^T Bm, then R = T Qb; each thread owns runs of 4 outputs of a row
+    for (int e = 4 * tid; e < BJ_N * BJ_N; e += 1024) {
+        const int i = e / BJ_N, j0 = e % BJ_N;
+        double t[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int l = 0; l < BJ_N; ++l) {
+            const double q = Qa[l][i];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) t[u] = fma(q, Bm[l][j0 + u], t[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) T[i][j0 + u] = t[u];
+    }
+    __syncthreads();
+    for (int e = 4 * tid; e < BJ_N * BJ_N; e += 1024) {
+        const int i = e / BJ_N, j0 = e % BJ_N;
+        double r[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int l = 0; l < BJ_N; ++l) {
+            const double t = T[i][l];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) r[u] = fma(t, Qb[l][j0 + u], r[u]);
+        }
+        const int gi = bj_index(i, Ia, Ja);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int gj = bj_index(j0 + u, Ib, Jb);
+            A[(long)gi * ne + gj] = r[u];
+            if (a != b) A[(long)gj * ne + gi] = r[u];
+        }
+    }
+}
+
+// W[X_k, c0 .. c0 + BJ_N) <- Q_k^T W[X_k, ...] for every pair k of step s
+// (W: n_e rows of n, row-major).
+__global__ __launch_bounds__(256) void bj_update_W(double* __restrict__ W, int n, int nb, int s,
+                                                   const double* __restrict__ Qg)
+{
+    __shared__ double Qk[BJ_N][BJ_N + 1], Bm[BJ_N][BJ_N + 1];
+    const int ncb = (n + BJ_N - 1) / BJ_N;
+    const int k = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * BJ_N;
+    int I, J;
+    bj_pair(s, k, nb, I, J);
+    const int tid = threadIdx.x;
+    for (int e = tid; e < BJ_N * BJ_N; e += 256) {
+        const int i = e / BJ_N, j = e % BJ_N;
+        Qk[i][j] = Qg[(long)k * BJ_N * BJ_N + e];
+        Bm[i][j] = c0 + j < n ? W[(long)bj_index(i, I, J) * n + c0 + j] : 0.0;
+    }
+    __syncthreads();
+    for (int e = 4 * tid; e < BJ_N * BJ_N; e += 1024) {
+        const int i = e / BJ_N, j0 = e % BJ_N;
+        double r[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int l = 0; l < BJ_N; ++l) {
+            const double q = Qk[l][i];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) r[u] = fma(q, Bm[l][j0 + u], r[u]);
+        }
+        const int gi = bj_index(i, I, J);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (c0 + j0 + u < n) W[(long)gi * n + c0 + j0 + u] = r[u];
+    }
+}
+
 // Jacobi on the device.  On return d_V (n x n) holds the sorted, sign-fixed
 // eigenvectors as rows and evals (host) the eigenvalues, descending.
 static int jacobi_device(mcmb_ctx* c, int n, const double* d_G, int normalise, double* d_norm, double* evals,
@@ -290,6 +519,83 @@ static int jacobi_device(mcmb_ctx* c, int n, const double* d_G, int normalise, d
     return MCMB_OK;
 }
 
+// Block Jacobi on the device; same outputs as jacobi_device.
+static int block_jacobi_device(mcmb_ctx* c, int n, const double* d_G, int normalise, double* d_norm, double* evals,
+                               DevBuf& d_V, int* sweeps_out)
+{
+    const int ne = (n + BJ_N - 1) / BJ_N * BJ_N, nb = ne / BJ_B, P = nb / 2;
+    const int ncb = (n + BJ_N - 1) / BJ_N;
+    DevBuf A, W, Qg, dcnt, dnrm, dperm;
+    MCMB_TRY(A.alloc(sizeof(double) * (size_t)ne * ne));
+    MCMB_TRY(W.alloc(sizeof(double) * (size_t)ne * n));
+    MCMB_TRY(Qg.alloc(sizeof(double) * (size_t)P * BJ_N * BJ_N));
+    MCMB_TRY(dcnt.alloc(sizeof(unsigned long long)));
+    MCMB_TRY(dnrm.alloc(sizeof(double) * 2));
+    const long ninit = std::max<long>((long)ne * ne, (long)ne * n);
+    MCMB_TRY(launch_guard("block Jacobi init", (ninit + 255) / 256, 256));
+    hipLaunchKernelGGL(jac_init, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, c->stream, d_G, n, ne,
+                       normalise, d_norm, A.as<double>(), W.as<double>());
+    MCMB_HIP(hipGetLastError());
+    MCMB_TRY(launch_guard("block Jacobi A update", (unsigned long long)P * (P + 1) / 2, 256));
+    MCMB_TRY(launch_guard("block Jacobi W update", (unsigned long long)P * ncb, 256));
+    int sweep = 0;
+    bool converged = false;
+    double abs_tol = 0.0;
+    for (; sweep < 60 && !converged; ++sweep) {
+        MCMB_HIP(hipMemsetAsync(dnrm.p, 0, sizeof(double) * 2, c->stream));
+        hipLaunchKernelGGL(jac_norms, dim3(1024), dim3(256), 0, c->stream, A.as<double>(), ne, dnrm.as<double>());
+        MCMB_HIP(hipGetLastError());
+        double nr[2];
+        MCMB_HIP(hipMemcpyAsync(nr, dnrm.p, sizeof(nr), hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        const double stop = std::max(1e-15, 2e-17 * n);
+        if (nr[0] <= stop * stop * nr[1]) { converged = true; break; }
+        if (sweep == 0) abs_tol = 1e-17 * sqrt(nr[1]);
+        MCMB_HIP(hipMemsetAsync(dcnt.p, 0, sizeof(unsigned long long), c->stream));
+        for (int step = 0; step < nb - 1; ++step) {
+            hipLaunchKernelGGL(bj_solve, dim3(P), dim3(128), 0, c->stream, (const double*)A.as<double>(), ne, nb, step,
+                               abs_tol, Qg.as<double>(), dcnt.as<unsigned long long>());
+            hipLaunchKernelGGL(bj_update_A, dim3((unsigned)((long)P * (P + 1) / 2)), dim3(256), 0, c->stream,
+                               A.as<double>(), ne, nb, step, (const double*)Qg.as<double>());
+            hipLaunchKernelGGL(bj_update_W, dim3((unsigned)((long)P * ncb)), dim3(256), 0, c->stream,
+                               W.as<double>(), n, nb, step, (const double*)Qg.as<double>());
+        }
+        MCMB_HIP(hipGetLastError());
+        unsigned long long cnt = 0;
+        MCMB_HIP(hipMemcpyAsync(&cnt, dcnt.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        if (cnt == 0) { converged = true; ++sweep; break; }
+    }
+    if (!converged)
+        return fail(MCMB_ERR_ARG, "block Jacobi eigen-decomposition of n=%d did not converge in 60 sweeps", n);
+    std::vector<double> diag(n);
+    MCMB_HIP(hipMemcpy2DAsync(diag.data(), sizeof(double), A.as<double>(), sizeof(double) * (ne + 1), sizeof(double),
+                              n, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    std::vector<int> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return diag[a] > diag[b]; });
+    for (int k = 0; k < n; ++k) evals[k] = diag[perm[k]];
+    MCMB_TRY(dperm.alloc(sizeof(int) * (size_t)n));
+    MCMB_HIP(hipMemcpyAsync(dperm.p, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    MCMB_TRY(d_V.alloc(sizeof(double) * (size_t)n * n));
+    hipLaunchKernelGGL(jac_sort_rows, dim3(n), dim3(256), 0, c->stream, (const double*)W.as<double>(), n,
+                       dperm.as<int>(), d_V.as<double>());
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipStreamSynchronize(c->stream));   // perm and the work buffers go out of scope
+    if (sweeps_out) *sweeps_out = sweep;
+    return MCMB_OK;
+}
+
+// Eigen-decomposition: block Jacobi, or the scalar Jacobi when MCMB_EIGEN=scalar.
+static int eigen_device(mcmb_ctx* c, int n, const double* d_G, int normalise, double* d_norm, double* evals,
+                        DevBuf& d_V, int* sweeps_out)
+{
+    const char* e = getenv("MCMB_EIGEN");
+    if (e && !strcmp(e, "scalar")) return jacobi_device(c, n, d_G, normalise, d_norm, evals, d_V, sweeps_out);
+    return block_jacobi_device(c, n, d_G, normalise, d_norm, evals, d_V, sweeps_out);
+}
+
 }  // namespace mcmb
 
 using namespace mcmb;
@@ -302,7 +608,7 @@ extern "C" int mcmb_eigen_sym(mcmb_ctx* c, int n, const double* a, double* evals
     MCMB_TRY(dG.alloc(sizeof(double) * (size_t)n * n));
     MCMB_HIP(hipMemcpyAsync(dG.p, a, sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, c->stream));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
-    MCMB_TRY(jacobi_device(c, n, dG.as<double>(), 0, nullptr, evals, dV, sweeps));
+    MCMB_TRY(eigen_device(c, n, dG.as<double>(), 0, nullptr, evals, dV, sweeps));
     if (evecs) {
         MCMB_TRY(dE.alloc(sizeof(double) * (size_t)n * n));
         const long nn = (long)n * n;
@@ -335,10 +641,10 @@ extern "C" int mcmb_orthonormalise_late(mcmb_ctx* c, int n, const double* gamma,
     MCMB_TRY(dG.alloc(sizeof(double) * nn));
     MCMB_TRY(dN.alloc(sizeof(double) * n));
     MCMB_HIP(hipMemcpyAsync(dG.p, gamma, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
-    MCMB_TRY(jacobi_device(c, n, dG.as<double>(), 1, dN.as<double>(), evals, dV, nullptr));
+    MCMB_TRY(eigen_device(c, n, dG.as<double>(), 1, dN.as<double>(), evals, dV, nullptr));
     MCMB_HIP(hipMemcpyAsync(norm, dN.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
-    if (pca_flag == 0 || evals[n - 1] < 0.0) {
+    auto pca = [&]() -> int {
         int r = 0;
         while (r < n && evals[r] > 0.0) ++r;
         if (r == 0) return fail(MCMB_ERR_NOT_SPD, "mcmb_orthonormalise_late: no positive eigenvalue");
@@ -355,11 +661,21 @@ extern "C" int mcmb_orthonormalise_late(mcmb_ctx* c, int n, const double* gamma,
         MCMB_HIP(hipStreamSynchronize(c->stream));
         if (rank) *rank = r;
         return MCMB_OK;
-    }
-    // Cholesky branch: lambdainv = D L, lambda = (D L)^-1, lower triangles
+    };
+    if (pca_flag == 0 || evals[n - 1] < 0.0) return pca();
+    // Cholesky branch: lambdainv = D L, lambda = (D L)^-1, lower triangles.  A
+    // Gram whose eigenvalues all come out >= 0 can still fail the FP64
+    // Cholesky (its smallest ones are rounding noise); the reference would
+    // abort there, this takes the PCA branch instead.
     std::vector<double> ortho(nn), lam(nn);
     int bad = -1;
-    MCMB_TRY(mcmb_orthonormalise(c, n, gamma, ortho.data(), lam.data(), norm, &bad));
+    const int st = mcmb_orthonormalise(c, n, gamma, ortho.data(), lam.data(), norm, &bad);
+    if (st == MCMB_ERR_NOT_SPD) {
+        MCMB_HIP(hipMemcpyAsync(norm, dN.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        return pca();
+    }
+    MCMB_TRY(st);
     for (int i = 0; i < n; ++i)
         for (int j = 0; j < n; ++j) {
             lambdainv[(long)i * n + j] = j <= i ? ortho[(long)i * n + j] : 0.0;

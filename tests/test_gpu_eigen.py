@@ -1,5 +1,7 @@
 """GPU: FP64 Jacobi eigen-decomposition and the late-side orthonormalisation
 with its PCA fallback, against LAPACK (numpy eigh) and the oracle."""
+import os
+
 import numpy as np
 import pytest
 
@@ -16,8 +18,23 @@ def _sym(n, seed):
     return 0.5 * (a + a.T)
 
 
+@pytest.fixture(params=["block", "scalar"])
+def eig_mode(request):
+    """Block Jacobi (the default) or the scalar Jacobi (MCMB_EIGEN=scalar)."""
+    old = os.environ.get("MCMB_EIGEN")
+    if request.param == "scalar":
+        os.environ["MCMB_EIGEN"] = "scalar"
+    else:
+        os.environ.pop("MCMB_EIGEN", None)
+    yield request.param
+    if old is None:
+        os.environ.pop("MCMB_EIGEN", None)
+    else:
+        os.environ["MCMB_EIGEN"] = old
+
+
 @pytest.mark.parametrize("n", [1, 2, 3, 8, 33, 64, 257, 600])
-def test_eigen_sym_against_lapack(ctx, n):
+def test_eigen_sym_against_lapack(ctx, n, eig_mode):
     A = _sym(n, n)
     ev, V, sweeps = ctx.eigen_sym(A)
     ref, Wl = np.linalg.eigh(A)
@@ -35,6 +52,25 @@ def test_eigen_sym_against_lapack(ctx, n):
         if sep > 1e-4 * nrm:
             assert abs(abs(V[:, k] @ Wl[:, k]) - 1.0) < 1e-9
     assert 0 < sweeps < 20 or n == 1
+
+
+@pytest.mark.parametrize("n", [97, 480])
+def test_eigen_sym_graded_singular_spectrum(ctx, n, eig_mode):
+    # a Gram-like spectrum: 12 decades, then entries at the FP64 noise floor
+    rng = np.random.default_rng(n)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.r_[np.logspace(0, -12, n - 20), 1e-17 * rng.standard_normal(20)]
+    A = (Q * lam) @ Q.T
+    A = 0.5 * (A + A.T)
+    ev, V, sweeps = ctx.eigen_sym(A)
+    ref = np.linalg.eigvalsh(A)[::-1]
+    assert np.abs(ev - ref).max() < 1e-13 * n
+    assert np.abs(A @ V - V * ev).max() < 1e-13 * np.sqrt(n)
+    assert np.abs(V.T @ V - np.eye(n)).max() < 1e-12 * np.sqrt(n)
+    big = int((lam > 1e-3).sum())
+    # the large eigenvalues to normwise accuracy, relative to themselves
+    assert np.abs(ev[:big] / ref[:big] - 1).max() < 1e-10
+    assert sweeps < 60
 
 
 def test_eigen_sym_matches_oracle_vectors(ctx):
@@ -77,20 +113,23 @@ def test_orthonormalise_late_config1_pca(ctx):
 
 def test_orthonormalise_late_config1_branch(ctx):
     """pca_flag = 1: PCA when the smallest eigenvalue is negative, else the
-    Cholesky branch, which on this singular-to-FP64 Gram may fail cleanly
-    with MCMB_ERR_NOT_SPD (the reference aborts in GSL there)."""
+    Cholesky branch; on this singular-to-FP64 Gram the Cholesky may fail (the
+    reference aborts in GSL there), and then the PCA branch is taken."""
     c = W.CONFIGS[1]
     ws = W.lspace(c["L"], c["R"], c["modulated"])
     ctx.lspace_set(2, c["L"], ws["q"], ws["v"], ws["order"])
     G = ctx.gamma_gl()
     ev, _, _ = ctx.eigen_sym(G / np.sqrt(np.outer(np.diag(G), np.diag(G))), vectors=False)
+    r, lam, linv, norm, ev2 = ctx.orthonormalise_late(G, pca_flag=1)
+    spd = True
     try:
-        r, lam, linv, norm, ev2 = ctx.orthonormalise_late(G, pca_flag=1)
+        ctx.orthonormalise(G)
     except modalcmb.NotSPDError:
-        assert ev[-1] >= 0      # Cholesky branch taken and failed
-        return
-    if ev[-1] < 0:
-        assert r < 220
+        spd = False
+    if ev[-1] < 0 or not spd:
+        # PCA: the same outputs as with the branch forced
+        r0, lam0, linv0, _, _ = ctx.orthonormalise_late(G, pca_flag=0)
+        assert r == r0 and np.array_equal(lam, lam0) and np.array_equal(linv, linv0)
     else:
         assert r == 220 and np.allclose(np.triu(lam, 1), 0)
 

@@ -35,6 +35,8 @@
 
 namespace mcmb {
 
+typedef double fd4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ int rr_next(int x, int m)
 {
     if (x == 0 || m == 1) return x;
@@ -236,7 +238,7 @@ __global__ void pca_outputs(const double* __restrict__ V, const double* __restri
 // rotation and skip rules inside S_k are those of the scalar Jacobi above.
 constexpr int BJ_B = 16;            // block size
 constexpr int BJ_N = 2 * BJ_B;      // subproblem size
-constexpr int BJ_INNER = 10;        // inner sweeps at most per subproblem
+constexpr int BJ_INNER = 1;         // inner sweeps per subproblem (more do not cut the outer sweeps)
 
 __host__ __device__ __forceinline__ void bj_pair(int s, int k, int nb, int& I, int& J)
 {
@@ -358,14 +360,17 @@ __global__ __launch_bounds__(128) void bj_solve(const double* __restrict__ A, in
     if (applied) atomicAdd(nrot, applied);
 }
 
-// A[X_a, X_b] <- Q_a^T A[X_a, X_b] Q_b for pairs a <= b of step s (both
-// symmetric blocks written; in place, each block belongs to one workgroup).
-__global__ __launch_bounds__(256) void bj_update_A(double* __restrict__ A, int ne, int nb, int s,
-                                                   const double* __restrict__ Qg)
+// A[X_a, X_b] <- Q_a^T A[X_a, X_b] Q_b for pairs a <= b of step s, one wave
+// per block (both symmetric blocks written; in place, each block belongs to
+// one workgroup).  FP64 MFMA 16x16x4: T = Q_a^T A_ab, then R = T Q_b, each a
+// 2 x 2 grid of 16 x 16 blocks over K = 32; R goes through LDS so that both
+// stores run along rows.
+__global__ __launch_bounds__(64) void bj_update_A(double* __restrict__ A, int ne, int nb, int s,
+                                                  const double* __restrict__ Qg)
 {
-    __shared__ double Qa[BJ_N][BJ_N + 1], Qb[BJ_N][BJ_N + 1], Bm[BJ_N][BJ_N + 1], T[BJ_N][BJ_N + 1];
+    static_assert(BJ_N == 32, "one wave per 32 x 32 block");
+    __shared__ double Qa[BJ_N][BJ_N + 1], Qb[BJ_N][BJ_N + 1], T[BJ_N][BJ_N + 1];
     const int P = nb / 2;
-    // linear index -> (a, b), a <= b
     int idx = blockIdx.x, a = 0;
     while (idx >= P - a) {
         idx -= P - a;
@@ -375,75 +380,114 @@ __global__ __launch_bounds__(256) void bj_update_A(double* __restrict__ A, int n
     int Ia, Ja, Ib, Jb;
     bj_pair(s, a, nb, Ia, Ja);
     bj_pair(s, b, nb, Ib, Jb);
-    const int tid = threadIdx.x;
-    for (int e = tid; e < BJ_N * BJ_N; e += 256) {
+    const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
+    for (int e = lane; e < BJ_N * BJ_N; e += 64) {
+        Qa[e / BJ_N][e % BJ_N] = Qg[(long)a * BJ_N * BJ_N + e];
+        Qb[e / BJ_N][e % BJ_N] = Qg[(long)b * BJ_N * BJ_N + e];
+    }
+    // B operand of T: A_ab[l][j], l = 4 ks + lane/16, j = 16 c + lane%16
+    double bv[BJ_N / 4][2];
+#pragma unroll
+    for (int ks = 0; ks < BJ_N / 4; ++ks) {
+        const long row = (long)bj_index(4 * ks + lk, Ia, Ja) * ne;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) bv[ks][c] = A[row + bj_index(16 * c + lr, Ib, Jb)];
+    }
+    __syncthreads();
+    fd4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[x][c] = (fd4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < BJ_N / 4; ++ks)
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            const double av = Qa[4 * ks + lk][16 * x + lr];   // (Q_a^T)[i][l] = Q_a[l][i]
+#pragma unroll
+            for (int c = 0; c < 2; ++c) acc[x][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[ks][c], acc[x][c], 0, 0, 0);
+        }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) T[16 * x + 4 * v + lk][16 * c + lr] = acc[x][c][v];
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[x][c] = (fd4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < BJ_N / 4; ++ks)
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            const double av = T[16 * x + lr][4 * ks + lk];
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+                acc[x][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Qb[4 * ks + lk][16 * c + lr], acc[x][c], 0, 0, 0);
+        }
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) T[16 * x + 4 * v + lk][16 * c + lr] = acc[x][c][v];
+    __syncthreads();
+    // R[i][j] to A[X_a[i]][X_b[j]] and A[X_b[j]][X_a[i]], rows contiguous
+    for (int e = lane; e < BJ_N * BJ_N; e += 64) {
         const int i = e / BJ_N, j = e % BJ_N;
-        Qa[i][j] = Qg[(long)a * BJ_N * BJ_N + e];
-        Qb[i][j] = Qg[(long)b * BJ_N * BJ_N + e];
-        Bm[i][j] = A[(long)bj_index(i, Ia, Ja) * ne + bj_index(j, Ib, Jb)];
-    }
-    __syncthreads();
-    // T = Qa^T Bm, then R = T Qb; each thread owns runs of 4 outputs of a row
-    for (int e = 4 * tid; e < BJ_N * BJ_N; e += 1024) {
-        const int i = e / BJ_N, j0 = e % BJ_N;
-        double t[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int l = 0; l < BJ_N; ++l) {
-            const double q = Qa[l][i];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) t[u] = fma(q, Bm[l][j0 + u], t[u]);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) T[i][j0 + u] = t[u];
-    }
-    __syncthreads();
-    for (int e = 4 * tid; e < BJ_N * BJ_N; e += 1024) {
-        const int i = e / BJ_N, j0 = e % BJ_N;
-        double r[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int l = 0; l < BJ_N; ++l) {
-            const double t = T[i][l];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) r[u] = fma(t, Qb[l][j0 + u], r[u]);
-        }
-        const int gi = bj_index(i, Ia, Ja);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int gj = bj_index(j0 + u, Ib, Jb);
-            A[(long)gi * ne + gj] = r[u];
-            if (a != b) A[(long)gj * ne + gi] = r[u];
-        }
+        A[(long)bj_index(i, Ia, Ja) * ne + bj_index(j, Ib, Jb)] = T[i][j];
+        if (a != b) A[(long)bj_index(i, Ib, Jb) * ne + bj_index(j, Ia, Ja)] = T[j][i];
     }
 }
 
-// W[X_k, c0 .. c0 + BJ_N) <- Q_k^T W[X_k, ...] for every pair k of step s
-// (W: n_e rows of n, row-major).
+// W[X_k, c0 .. c0 + 256) <- Q_k^T W[X_k, c0 .. c0 + 256) for every pair k of
+// step s (W: n_e rows of n, row-major), FP64 MFMA: wave w takes 64 columns,
+// its B operands read from W straight into registers before any store.
 __global__ __launch_bounds__(256) void bj_update_W(double* __restrict__ W, int n, int nb, int s,
                                                    const double* __restrict__ Qg)
 {
-    __shared__ double Qk[BJ_N][BJ_N + 1], Bm[BJ_N][BJ_N + 1];
-    const int ncb = (n + BJ_N - 1) / BJ_N;
-    const int k = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * BJ_N;
+    static_assert(BJ_N == 32, "2 x 4 blocks of 16 x 16 per wave");
+    __shared__ double Qk[BJ_N][BJ_N + 1];
+    const int ncb = (n + 255) / 256;
+    const int k = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * 256;
     int I, J;
     bj_pair(s, k, nb, I, J);
-    const int tid = threadIdx.x;
-    for (int e = tid; e < BJ_N * BJ_N; e += 256) {
-        const int i = e / BJ_N, j = e % BJ_N;
-        Qk[i][j] = Qg[(long)k * BJ_N * BJ_N + e];
-        Bm[i][j] = c0 + j < n ? W[(long)bj_index(i, I, J) * n + c0 + j] : 0.0;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
+    for (int e = tid; e < BJ_N * BJ_N; e += 256) Qk[e / BJ_N][e % BJ_N] = Qg[(long)k * BJ_N * BJ_N + e];
+    const int cb = c0 + 64 * wave + lr;
+    double bv[BJ_N / 4][4];
+#pragma unroll
+    for (int ks = 0; ks < BJ_N / 4; ++ks) {
+        const long row = (long)bj_index(4 * ks + lk, I, J) * n;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bv[ks][c] = cb + 16 * c < n ? W[row + cb + 16 * c] : 0.0;
     }
     __syncthreads();
-    for (int e = 4 * tid; e < BJ_N * BJ_N; e += 1024) {
-        const int i = e / BJ_N, j0 = e % BJ_N;
-        double r[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int l = 0; l < BJ_N; ++l) {
-            const double q = Qk[l][i];
+    fd4 acc[2][4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) r[u] = fma(q, Bm[l][j0 + u], r[u]);
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[x][c] = (fd4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < BJ_N / 4; ++ks)
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            const double av = Qk[4 * ks + lk][16 * x + lr];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[x][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[ks][c], acc[x][c], 0, 0, 0);
         }
-        const int gi = bj_index(i, I, J);
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (c0 + j0 + u < n) W[(long)gi * n + c0 + j0 + u] = r[u];
-    }
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const long row = (long)bj_index(16 * x + 4 * v + lk, I, J) * n;
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (cb + 16 * c < n) W[row + cb + 16 * c] = acc[x][c][v];
+        }
 }
 
 // Jacobi on the device.  On return d_V (n x n) holds the sorted, sign-fixed
@@ -524,7 +568,7 @@ static int block_jacobi_device(mcmb_ctx* c, int n, const double* d_G, int normal
                                DevBuf& d_V, int* sweeps_out)
 {
     const int ne = (n + BJ_N - 1) / BJ_N * BJ_N, nb = ne / BJ_B, P = nb / 2;
-    const int ncb = (n + BJ_N - 1) / BJ_N;
+    const int ncb = (n + 255) / 256;
     DevBuf A, W, Qg, dcnt, dnrm, dperm;
     MCMB_TRY(A.alloc(sizeof(double) * (size_t)ne * ne));
     MCMB_TRY(W.alloc(sizeof(double) * (size_t)ne * n));
@@ -536,7 +580,7 @@ static int block_jacobi_device(mcmb_ctx* c, int n, const double* d_G, int normal
     hipLaunchKernelGGL(jac_init, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, c->stream, d_G, n, ne,
                        normalise, d_norm, A.as<double>(), W.as<double>());
     MCMB_HIP(hipGetLastError());
-    MCMB_TRY(launch_guard("block Jacobi A update", (unsigned long long)P * (P + 1) / 2, 256));
+    MCMB_TRY(launch_guard("block Jacobi A update", (unsigned long long)P * (P + 1) / 2, 64));
     MCMB_TRY(launch_guard("block Jacobi W update", (unsigned long long)P * ncb, 256));
     int sweep = 0;
     bool converged = false;
@@ -555,7 +599,7 @@ static int block_jacobi_device(mcmb_ctx* c, int n, const double* d_G, int normal
         for (int step = 0; step < nb - 1; ++step) {
             hipLaunchKernelGGL(bj_solve, dim3(P), dim3(128), 0, c->stream, (const double*)A.as<double>(), ne, nb, step,
                                abs_tol, Qg.as<double>(), dcnt.as<unsigned long long>());
-            hipLaunchKernelGGL(bj_update_A, dim3((unsigned)((long)P * (P + 1) / 2)), dim3(256), 0, c->stream,
+            hipLaunchKernelGGL(bj_update_A, dim3((unsigned)((long)P * (P + 1) / 2)), dim3(64), 0, c->stream,
                                A.as<double>(), ne, nb, step, (const double*)Qg.as<double>());
             hipLaunchKernelGGL(bj_update_W, dim3((unsigned)((long)P * ncb)), dim3(256), 0, c->stream,
                                W.as<double>(), n, nb, step, (const double*)Qg.as<double>());

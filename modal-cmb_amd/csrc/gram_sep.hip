@@ -16,6 +16,7 @@
 // the point-by-point contraction; the summation is FP64 throughout and in a
 // fixed order (no atomics), so repeated calls are bit-identical.
 #include "mcmb_internal.h"
+#include "weights.h"
 #include <algorithm>
 #include <climits>
 #include <cstring>
@@ -34,7 +35,7 @@ constexpr int SG_LDA = SG_TM + 16;    // LDS row strides in doubles (32 banks ap
 constexpr int SG_LDB = SG_TN + 16;
 constexpr int SR_G = 16;            // rows per row-moment group
 constexpr int SR_KC = 16;           // l3 values per weight stage (SR_G * SR_KC = 256 threads)
-constexpr int SR_FMAX = 3;          // pair columns per thread in stage 1: NP <= 768
+constexpr int SR_NPMAX = 768;       // pair columns of stage 1 (3 per thread of 256)
 
 struct SepTask {
     long boff;   // first B row of the task
@@ -59,100 +60,124 @@ __global__ void sep_utab(const double* __restrict__ q, int ldq, int nl, const in
     U[e] = q[(long)l * ldq + xy.x] * q[(long)l * ldq + xy.y];
 }
 
-// Stage 1.  One workgroup per group of up to 16 rows that share l1 and the
-// parity class of l3, so that their l3 ranges interleave on one grid of step
-// kstep; thread t owns pair columns t, t + 256, ...  Per stage of 16 l3 values
-// the pair products U[l3] are loaded into registers first (all loads in
-// flight together), then the 16 x 16 weights (zero outside a row's range) are
-// staged in LDS and applied.  F = pair columns per thread, H = halves of the
-// stage (two for F = 3, to stay within the register file).
-template <int F, int H>
-__global__ __launch_bounds__(256) void sep_rowmom(const RowDesc* __restrict__ rows, const long* __restrict__ rowptr,
+// Weights of the row groups, staged densely for stage 1: for group gb,
+// chunk c of 16 l3 values (l3 = kmin + kstep (16 c + kk)) and member row g,
+// Wd[wofs[gb] + 256 c + 16 kk + g] = w (0 outside the row's range).  KIND 0 /
+// 1: the l-space weight permsix h^2 v v v with the exact / asymptotic h^2,
+// formed here; KIND 2: copied from the walk-order weights ws.
+template <int KIND>
+__global__ __launch_bounds__(256) void sep_wstage(const RowDesc* __restrict__ rows, const long* __restrict__ rowptr,
                                                   const double* __restrict__ ws, const int* __restrict__ groups,
-                                                  int row_base, int kstep, const double* __restrict__ U, int ldu,
-                                                  int NP, int ld, double* __restrict__ M)
+                                                  const int4* __restrict__ ginfo, const long* __restrict__ wofs,
+                                                  int kstep, const double* __restrict__ f,
+                                                  const double* __restrict__ v, double* __restrict__ Wd)
 {
-    constexpr int KH = SR_KC / H;
-    __shared__ int gr[SR_G], gk0[SR_G], gnk[SR_G];
+    __shared__ int gi[SR_G], gj[SR_G], gk0[SR_G], gnk[SR_G];
     __shared__ long gp[SR_G];
-    __shared__ double Wt[2][SR_KC][SR_G];
     const int tid = threadIdx.x;
     if (tid < SR_G) {
         const int r = groups[(long)blockIdx.x * SR_G + tid];
-        gr[tid] = r;
+        RowDesc d = {0, 0, 0, 0};
+        long p = 0;
         if (r >= 0) {
-            const RowDesc d = rows[r];
-            gk0[tid] = d.k0;
-            gnk[tid] = d.nk;
-            gp[tid] = rowptr[r];
-        } else {
-            gk0[tid] = 0;
-            gnk[tid] = 0;
-            gp[tid] = 0;
+            d = rows[r];
+            if (KIND == 2) p = rowptr[r];
         }
+        gi[tid] = d.i;
+        gj[tid] = d.j;
+        gk0[tid] = d.k0;
+        gnk[tid] = d.nk;
+        gp[tid] = p;
     }
     __syncthreads();
-    int kmin = INT_MAX, kmax = INT_MIN;
-    for (int g = 0; g < SR_G; ++g)
-        if (gnk[g] > 0) {
-            kmin = min(kmin, gk0[g]);
-            kmax = max(kmax, gk0[g] + kstep * (gnk[g] - 1));
+    const int4 info = ginfo[blockIdx.x];
+    const int kmin = info.x, nkk = info.y;
+    double* const W = Wd + wofs[blockIdx.x];
+    const int kk = tid >> 4, g = tid & 15;
+    for (int c0 = 0; c0 < nkk; c0 += SR_KC) {
+        double w = 0.0;
+        if (c0 + kk < nkk) {
+            const int k = kmin + kstep * (c0 + kk);
+            const int d = k - gk0[g];
+            if (d >= 0 && d % kstep == 0 && d / kstep < gnk[g])
+                w = KIND == 2 ? ws[gp[g] + d / kstep] : lspace_weight(gi[g], gj[g], k, KIND, f, v);
         }
-    const int nkk = kmax >= kmin ? (kmax - kmin) / kstep + 1 : 0;
+        W[16 * c0 + tid] = w;
+    }
+}
 
-    double acc[SR_G][F];
+// Stage 1.  One workgroup per group of up to 16 rows that share l1 and the
+// parity class of l3, so that their l3 ranges interleave on one grid of step
+// kstep.  The NT threads split into GS row sets of 16/GS rows; within a set
+// thread t owns pair columns t, t + NT/GS, ... (F of them).  Per stage of 16
+// l3 values the pair products U[l3] go to registers first (all loads in
+// flight together) and the 16 x 16 staged weights to LDS (one coalesced load,
+// double-buffered); every weight read from LDS (a broadcast) feeds F FMAs.
+// H = parts of the stage, to bound the registers held for U.
+template <int NT, int GS, int F, int H>
+__global__ __launch_bounds__(NT) void sep_rowmom(const int* __restrict__ groups, const int4* __restrict__ ginfo,
+                                                 const long* __restrict__ wofs, const double* __restrict__ Wd,
+                                                 int row_base, int kstep, const double* __restrict__ U, int ldu,
+                                                 int NP, int ld, double* __restrict__ M)
+{
+    constexpr int KH = SR_KC / H;
+    constexpr int WPT = SR_G * SR_KC / NT;   // staged weights loaded per thread per stage
+    constexpr int RG = SR_G / GS;            // rows per thread
+    constexpr int CT = NT / GS;              // threads per row set
+    __shared__ double Wt[2][SR_KC][SR_G];
+    const int tid = threadIdx.x, ct = tid % CT, g0 = RG * (tid / CT);
+    const int4 info = ginfo[blockIdx.x];
+    const int kmin = info.x, nkk = info.y;
+    const double* const Wg = Wd + wofs[blockIdx.x];
+
+    double acc[RG][F];
 #pragma unroll
-    for (int g = 0; g < SR_G; ++g)
+    for (int g = 0; g < RG; ++g)
 #pragma unroll
         for (int u = 0; u < F; ++u) acc[g][u] = 0.0;
 
     int buf = 0;
     for (int c0 = 0; c0 < nkk; c0 += SR_KC, buf ^= 1) {
-        double uu[KH][F];
-        // l3 values past the group's range repeat the last one (weight 0)
-        auto uload = [&](int h) {
+        double wr[WPT];
+#pragma unroll
+        for (int x = 0; x < WPT; ++x) wr[x] = Wg[16 * c0 + WPT * tid + x];
+#pragma unroll
+        for (int h = 0; h < H; ++h) {
+            double uu[KH][F];
+            // l3 values past the group's range repeat the last one (weight 0)
 #pragma unroll
             for (int kk = 0; kk < KH; ++kk) {
                 const int kc = min(c0 + h * KH + kk, nkk - 1);
                 const double* Uk = U + (long)(kmin + kstep * kc) * ldu;
 #pragma unroll
                 for (int u = 0; u < F; ++u) {
-                    const int f = tid + 256 * u;
+                    const int f = ct + CT * u;
                     uu[kk][u] = f < NP ? Uk[f] : 0.0;
                 }
             }
-        };
-        uload(0);
-        {
-            const int kk = tid >> 4, g = tid & 15;
-            double w = 0.0;
-            if (c0 + kk < nkk) {
-                const int d = kmin + kstep * (c0 + kk) - gk0[g];
-                if (d >= 0 && d % kstep == 0 && d / kstep < gnk[g]) w = ws[gp[g] + d / kstep];
-            }
-            Wt[buf][kk][g] = w;
-        }
-        __syncthreads();
+            if (h == 0) {
 #pragma unroll
-        for (int h = 0; h < H; ++h) {
-            if (h > 0) uload(h);
+                for (int x = 0; x < WPT; ++x) (&Wt[buf][0][0])[WPT * tid + x] = wr[x];
+                __syncthreads();
+            }
 #pragma unroll
             for (int kk = 0; kk < KH; ++kk)
 #pragma unroll
-                for (int g = 0; g < SR_G; ++g) {
-                    const double w = Wt[buf][h * KH + kk][g];
+                for (int g = 0; g < RG; ++g) {
+                    const double w = Wt[buf][h * KH + kk][g0 + g];
 #pragma unroll
                     for (int u = 0; u < F; ++u) acc[g][u] = fma(w, uu[kk][u], acc[g][u]);
                 }
         }
     }
 #pragma unroll
-    for (int g = 0; g < SR_G; ++g) {
-        if (gr[g] < 0) continue;
-        double* Mr = M + (long)(gr[g] - row_base) * ld;
+    for (int g = 0; g < RG; ++g) {
+        const int r = groups[(long)blockIdx.x * SR_G + g0 + g];
+        if (r < 0) continue;
+        double* Mr = M + (long)(r - row_base) * ld;
 #pragma unroll
         for (int u = 0; u < F; ++u) {
-            const int f = tid + 256 * u;
+            const int f = ct + CT * u;
             if (f < NP) Mr[f] = acc[g][u];
         }
     }
@@ -310,7 +335,7 @@ __global__ void sep_assemble(const double* __restrict__ G, int NP, int ld, int R
 
 bool gram_separable_supported(const PointSet& L, int precision)
 {
-    return precision == MCMB_FP64 && L.R >= 1 && L.R <= 255 && L.R * (L.R + 1) / 2 <= 256 * SR_FMAX;
+    return precision == MCMB_FP64 && L.R >= 1 && L.R <= 255 && L.R * (L.R + 1) / 2 <= SR_NPMAX;
 }
 
 namespace {
@@ -344,12 +369,27 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
     T.nrows = runs[b1 - 1].r1 - T.row0;
     T.nruns = b1 - b0;
     std::vector<int> groups, aidx2(T.nrows), aidx3(T.nruns);
+    std::vector<int4> ginfo;     // per group: l3 start, l3 count on the group's grid
+    std::vector<long> wofs;      // per group: first staged weight
     std::vector<SepTask> task2(T.nruns), task3(1);
     std::vector<int4> tile2, tile3;
     std::vector<std::vector<int>> cls(kstep);
+    long wsize = 0;
     auto flush = [&](std::vector<int>& c) {
         if (c.empty()) return;
-        for (int g = 0; g < SR_G; ++g) groups.push_back(g < (int)c.size() ? c[g] : -1);
+        int kmin = INT_MAX, kmax = INT_MIN;
+        for (int g = 0; g < SR_G; ++g) {
+            groups.push_back(g < (int)c.size() ? c[g] : -1);
+            if (g < (int)c.size() && L.rows[c[g]].nk > 0) {
+                const RowDesc& d = L.rows[c[g]];
+                kmin = std::min(kmin, d.k0);
+                kmax = std::max(kmax, d.k0 + kstep * (d.nk - 1));
+            }
+        }
+        const int nkk = kmax >= kmin ? (kmax - kmin) / kstep + 1 : 0;
+        ginfo.push_back(make_int4(nkk ? kmin : 0, nkk, 0, 0));
+        wofs.push_back(wsize);
+        wsize += (long)SR_G * SR_KC * ((nkk + SR_KC - 1) / SR_KC);
         c.clear();
     };
     T.contig2 = T.contig3 = true;
@@ -388,8 +428,11 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
     T.ngroups = (int)(groups.size() / SR_G);
     T.nt2 = (int)tile2.size();
     T.nt3 = (int)tile3.size();
+    T.wsize = wsize;
     BT.buf.clear();
     T.o_groups = append(BT.buf, groups);
+    T.o_ginfo = append(BT.buf, ginfo);
+    T.o_wofs = append(BT.buf, wofs);
     T.o_aidx2 = append(BT.buf, aidx2);
     T.o_aidx3 = append(BT.buf, aidx3);
     T.o_task2 = append(BT.buf, task2);
@@ -445,6 +488,7 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
     for (size_t b = 0; b < blocks.size(); ++b) {
         build_block(L, runs, blocks[b].first, blocks[b].second, NP, ld, tabs[b]);
         P.max_tab = std::max(P.max_tab, tabs[b].buf.size());
+        P.max_wsize = std::max(P.max_wsize, tabs[b].b.wsize);
     }
     P.max_tab = (P.max_tab + 3) / 4 * 4;   // every block's tables 16-byte aligned
     MCMB_TRY(c->sg_tab.alloc(sizeof(int) * P.max_tab * std::max<size_t>(1, blocks.size())));
@@ -501,11 +545,13 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         ++np;
         return MCMB_OK;
     };
+    MCMB_TRY(c->sg_wd.alloc(sizeof(double) * (size_t)std::max<long>(1, P.max_wsize)));
     const bool one_block = P.blocks.size() == 1;
+    // l-space weights are formed where they are staged; other point sets stage
+    // their walk-order weights
+    const int wkind = (L.kspace || L.uniform) ? 2 : (L.h2mode == MCMB_H2_ASYMPTOTIC ? 1 : 0);
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
-    MCMB_TRY(phase("weights"));
-    MCMB_TRY(pointset_weights(c, L, c->sg_rp, c->sg_ws, 0, rowptr_ready));
-    int launches = 1;
+    int launches = 0;
     MCMB_TRY(phase("pair products"));
     {
         const long e = (long)nl * ldu;
@@ -515,21 +561,49 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         MCMB_HIP(hipGetLastError());
         ++launches;
     }
+    // staged weights of one block's row groups
+    auto wstage = [&](size_t b) -> int {
+        const SepBlock& T = P.blocks[b];
+        const int* tab = c->sg_tab.as<int>() + b * P.max_tab;
+        MCMB_TRY(launch_guard("weight staging", (unsigned long long)T.ngroups, 256));
+#define SEP_WSTAGE(KK)                                                                                          \
+    hipLaunchKernelGGL((sep_wstage<KK>), dim3((unsigned)T.ngroups), dim3(256), 0, c->stream,                    \
+                       L.d_rows.as<RowDesc>(), (const long*)c->sg_rp.as<long>(), (const double*)c->sg_ws.as<double>(), \
+                       tab + T.o_groups, reinterpret_cast<const int4*>(tab + T.o_ginfo),                        \
+                       reinterpret_cast<const long*>(tab + T.o_wofs), L.kstep,                                   \
+                       (const double*)L.d_ftab.as<double>(), (const double*)L.d_v.as<double>(),                  \
+                       c->sg_wd.as<double>())
+        if (wkind == 0) SEP_WSTAGE(0);
+        else if (wkind == 1) SEP_WSTAGE(1);
+        else SEP_WSTAGE(2);
+#undef SEP_WSTAGE
+        MCMB_HIP(hipGetLastError());
+        ++launches;
+        return MCMB_OK;
+    };
+    MCMB_TRY(phase("weights"));
+    if (wkind == 2) {
+        MCMB_TRY(pointset_weights(c, L, c->sg_rp, c->sg_ws, 0, rowptr_ready));
+        ++launches;
+    }
+    if (one_block) MCMB_TRY(wstage(0));
     if (!one_block) MCMB_TRY(phase("l3, l2, l1 contractions (all blocks)"));
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const SepBlock& T = P.blocks[b];
         const int* tab = c->sg_tab.as<int>() + b * P.max_tab;
+        const int4* ginfo = reinterpret_cast<const int4*>(tab + T.o_ginfo);
+        const long* wofs = reinterpret_cast<const long*>(tab + T.o_wofs);
+        if (!one_block) MCMB_TRY(wstage(b));
         // stage 1
         if (one_block) MCMB_TRY(phase("l3 contraction (row moments)"));
         MCMB_TRY(launch_guard("row moments", (unsigned long long)T.ngroups, 256));
-        const int F = (NP + 255) / 256;
-#define SEP_ROWMOM(FF, HH)                                                                                   \
-    hipLaunchKernelGGL((sep_rowmom<FF, HH>), dim3((unsigned)T.ngroups), dim3(256), 0, c->stream,            \
-                       L.d_rows.as<RowDesc>(), c->sg_rp.as<long>(), c->sg_ws.as<double>(), tab + T.o_groups, \
-                       T.row0, L.kstep, (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
-        if (F == 1) SEP_ROWMOM(1, 1);
-        else if (F == 2) SEP_ROWMOM(2, 1);
-        else SEP_ROWMOM(3, 2);
+#define SEP_ROWMOM(NN, GG, FF, HH)                                                                            \
+    hipLaunchKernelGGL((sep_rowmom<NN, GG, FF, HH>), dim3((unsigned)T.ngroups), dim3(NN), 0, c->stream,      \
+                       tab + T.o_groups, ginfo, wofs, (const double*)c->sg_wd.as<double>(), T.row0, L.kstep,  \
+                       (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
+        if (NP <= 256) SEP_ROWMOM(256, 1, 1, 1);
+        else if (NP <= 512) SEP_ROWMOM(256, 1, 2, 1);
+        else SEP_ROWMOM(256, 1, 3, 2);
 #undef SEP_ROWMOM
         MCMB_HIP(hipGetLastError());
         // stages 2 and 3

@@ -14,6 +14,7 @@
 // registers; per-segment partial tiles are summed in a fixed order by a
 // reduction kernel, so results are deterministic.
 #include "mcmb_internal.h"
+#include "weights.h"
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -27,47 +28,6 @@ constexpr int NT = 256;        // threads per workgroup (4 waves)
 constexpr int LDA = TM + 4;    // LDS row stride (doubles), staggers banks
 
 typedef double d4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ double permsix_d(int i, int j, int k)
-{
-    return (i == j && j == k) ? 1.0 : ((i == j || j == k || i == k) ? 3.0 : 6.0);
-}
-
-// Exact h^2 = (2l1+1)(2l2+1)(2l3+1)/(4 pi) * f(g-l1) f(g-l2) f(g-l3)/((2g+1) f(g)),
-// f(a) = prod_{k<=a}(2k-1)/(2k), g = (l1+l2+l3)/2 (even-parity triangle only).
-__device__ __forceinline__ double h2_exact(int i, int j, int k, const double* __restrict__ f)
-{
-    int g = (i + j + k) >> 1;
-    double pre = (2.0 * i + 1.0) * (2.0 * j + 1.0) * (2.0 * k + 1.0) * 0.079577471545947667884;  // 1/(4 pi)
-    return pre * (f[g - i] * f[g - j] * f[g - k]) / ((2.0 * g + 1.0) * f[g]);
-}
-
-// calculate_geometric.c:59-92 (the reference's asymptotic h^2)
-__device__ __forceinline__ double h2_asym(int l1, int l2, int l3)
-{
-    double L = (double)(l1 + l2 + l3);
-    double a1 = (double)(l2 + l3 - l1), a2 = (double)(l3 + l1 - l2), a3 = (double)(l1 + l2 - l3);
-    double pref = 0.31830988618379067154 * 0.31830988618379067154 / 2.0;
-    double p1 = pref * ((2.0 * l1 + 1.0) * (2.0 * l2 + 1.0) * (2.0 * l3 + 1.0)) / (L + 1.0);
-    double p2 = (L + 1.0 / 3.0) / ((a1 + 1.0 / 3.0) * (a2 + 1.0 / 3.0) * (a3 + 1.0 / 3.0));
-    double p3 = sqrt((a1 + 1.0 / 6.0) * (a2 + 1.0 / 6.0) * (a3 + 1.0 / 6.0) / (L + 1.0 / 6.0));
-    return p1 * p2 * p3;
-}
-
-__device__ __forceinline__ double lspace_weight(int i, int j, int k, int asym,
-                                                const double* __restrict__ f,
-                                                const double* __restrict__ v)
-{
-    double h2 = asym ? h2_asym(i, j, k) : h2_exact(i, j, k, f);
-    return permsix_d(i, j, k) * h2 * v[i] * v[j] * v[k];
-}
-
-// k-space weight of the ordered GL point (a, b, c): permsix w_a w_b w_c / K
-__device__ __forceinline__ double kspace_weight(int a, int b, int c, const double* __restrict__ kn,
-                                                const double* __restrict__ kw)
-{
-    return permsix_d(a, b, c) * kw[a] * kw[b] * kw[c] / (kn[a] + kn[b] + kn[c]);
-}
 
 __global__ __launch_bounds__(256) void kspace_weights_kernel(
     const RowDesc* __restrict__ rows, const long* __restrict__ rowptr, long nrows,

@@ -91,14 +91,16 @@ typedef PointSet LSpace;
 struct SepBlock {
     int row0 = 0, nrows = 0, nruns = 0, ngroups = 0, nt2 = 0, nt3 = 0;
     bool contig2 = true, contig3 = true;
-    size_t o_groups = 0, o_aidx2 = 0, o_aidx3 = 0, o_task2 = 0, o_task3 = 0, o_tile2 = 0, o_tile3 = 0;
+    size_t o_groups = 0, o_ginfo = 0, o_wofs = 0, o_aidx2 = 0, o_aidx3 = 0, o_task2 = 0, o_task3 = 0, o_tile2 = 0,
+           o_tile3 = 0;
+    long wsize = 0;   // doubles of the staged weights of the block's row groups
 };
 struct SepPlan {
     const void* ps = nullptr;   // point set the tables were built for
     long gen = -1;              // its generation
     int row_lo = -1, row_hi = -1;
     size_t max_tab = 0;         // ints per block in the table buffer
-    long max_rows = 0, max_runs = 0;
+    long max_rows = 0, max_runs = 0, max_wsize = 0;
     std::vector<SepBlock> blocks;
 };
 
@@ -123,7 +125,7 @@ struct mcmb_ctx {
     mcmb::DevBuf d_gamma_tmp;
     mcmb::DevBuf d_aux;
     // separable Gram: weights, row pointers, pair products, stage outputs, tables
-    mcmb::DevBuf sg_ws, sg_rp, sg_U, sg_M, sg_Y, sg_G, sg_tab;
+    mcmb::DevBuf sg_ws, sg_rp, sg_wd, sg_U, sg_M, sg_Y, sg_G, sg_tab;
     mcmb::SepPlan sep;
     // per-launch timing of the last call that records phases (event k+1 - event k)
     hipEvent_t evp[mcmb::MAX_PHASES + 1] = {};

@@ -406,7 +406,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2) if achieved else None, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4) if achieved else None,
                      "traffic": hbm_traffic(workload),
-                     "kernel": "sep_gemm, l2 contraction (FP64 MFMA 16x16x4, 96 x 96 tiles)",
+                     "kernel": "sep_gemm, l2 contraction (FP64 MFMA 16x16x4, 160 x 96 tiles)",
                      "algorithmic_flops_per_launch": l2_flops, "kernel_ms": l2_ms},
     }
     if not args.no_projection:

@@ -27,8 +27,8 @@ namespace mcmb {
 typedef double fd4 __attribute__((ext_vector_type(4)));
 typedef double fd2 __attribute__((ext_vector_type(2)));
 
-constexpr int SG_FM = 3, SG_FN = 3;   // 16 x 16 blocks per wave (rows, columns); 2 x 2 waves
-constexpr int SG_TM = 32 * SG_FM;     // GEMM tile rows (96)
+constexpr int SG_FM = 5, SG_FN = 3;   // 16 x 16 blocks per wave (rows, columns); 2 x 2 waves
+constexpr int SG_TM = 32 * SG_FM;     // GEMM tile rows (160)
 constexpr int SG_TN = 32 * SG_FN;     // GEMM tile columns (96)
 constexpr int SG_KC = 16;             // K rows per LDS stage
 constexpr int SG_LDA = SG_TM + 16;    // LDS row strides in doubles (32 banks apart)
@@ -184,12 +184,12 @@ __global__ __launch_bounds__(NT) void sep_rowmom(const int* __restrict__ groups,
 }
 
 // Stages 2 and 3:  C_task[m][n] (+)= sum_{kk < K} U[row(kk)][m] * B[boff + kk][n]
-// for m < Md, n < Nd, over a list of 96 x 96 tiles (task, m0, n0); row(kk) =
+// for m < Md, n < Nd, over a list of 160 x 96 tiles (task, m0, n0); row(kk) =
 // aoff + kk when the A rows are contiguous (CONTIG), else aidx[aoff + kk].
 // Both operands are K-major with even leading dimensions (16-byte loads; the
 // element past an odd Md or Nd is padding and only feeds outputs that are not
 // stored).  16 K rows per stage, double-buffered in LDS; FP64 MFMA 16x16x4
-// with 4 waves of 48 x 48.  The tile list holds groups of GS consecutive
+// with 4 waves of 80 x 48.  The tile list holds groups of GS consecutive
 // tiles that share one B panel (the m tiles of one (task, n0)).  Blocks b,
 // b+8, ... share an XCD (and its L2): the groups are dealt to the 8 XCDs in
 // turn, each group's tiles to consecutive blocks of one XCD, so the tiles of

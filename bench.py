@@ -165,9 +165,14 @@ def bench_projection(modalcmb, W, args):
     t_fwd = time.time() - t
     k_fwd = ctx.last_kernel_ms()[0]
     t = time.time()
-    g32 = ctx.project_bwd(theta, gbar, modalcmb.FP32)
+    g32 = ctx.project_bwd(theta, gbar, modalcmb.FP32)   # reuses the forward's stage-1 products
     t_bwd = time.time() - t
     k_bwd = ctx.last_kernel_ms()[0]
+    ctx.project_fwd(theta[::-1].copy(), modalcmb.FP32)  # other shapes: the next adjoint recomputes stage 1
+    t = time.time()
+    g32b = ctx.project_bwd(theta, gbar, modalcmb.FP32)
+    t_bwd_alone = time.time() - t
+    assert np.array_equal(g32, g32b)
     a64, d64 = ctx.project_fwd(theta, modalcmb.FP64)
     g64 = ctx.project_bwd(theta, gbar, modalcmb.FP64)
     npts = ctx.kspace_npoints()
@@ -186,6 +191,9 @@ def bench_projection(modalcmb, W, args):
                     f"{n} modes (r<=s<=t<{R}), masked cube {npts} ordered nodes",
         "fwd_tangent_s": round(t_fwd, 4), "fwd_kernel_ms": round(k_fwd, 2),
         "adjoint_s": round(t_bwd, 4), "adjoint_kernel_ms": round(k_bwd, 2),
+        "adjoint_note": "right after the forward on the same shapes, reusing its stage-1 products; "
+                        "adjoint_alone_s recomputes them",
+        "adjoint_alone_s": round(t_bwd_alone, 4),
         "shapes_per_s_fwd_tangent": round(B / t_fwd, 1),
         "grads_per_s_fwd_tangent": round(2 * B / t_fwd, 1),
         "grads_per_s_fwd_plus_adjoint": round(2 * B / (t_fwd + t_bwd), 1),

@@ -47,10 +47,30 @@ struct RowDesc {
     int i, j, k0, nk;
 };
 
-// Device buffer that frees itself.
+// Device buffer that frees itself.  Movable, not copyable (a copy would free
+// the allocation twice).
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes)
+    {
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            bytes = o.bytes;
+            o.p = nullptr;
+            o.bytes = 0;
+        }
+        return *this;
+    }
     int alloc(size_t b);
     void release();
     ~DevBuf() { release(); }
@@ -119,6 +139,12 @@ struct mcmb_ctx {
     mcmb::PointSet ku;   // primordial k-space uniform-grid tetrapyd (prim walk)
     mcmb::DevBuf d_W;    // full-cube projection weight W[i][j][k] (k-space)
     mcmb::DevBuf pz, py, ph, pd;   // projection batch buffers (Z, Y, H, partials), kept between calls
+    // stage-1 products Y = W Z of the last forward projection, per batch of
+    // shapes, reused by a backward call on the same shapes
+    std::vector<mcmb::DevBuf> ycache;
+    std::vector<double> ycache_theta;
+    int ycache_prec = -1;
+    long ycache_gen = -1;
     int w_ready = 0;
     // scratch for partial Gram tiles
     mcmb::DevBuf d_partial;

@@ -22,6 +22,7 @@
 // triples), Gbar[i][j][t] = sum_s q_s(j) Hbar[i][s][t], and reduces
 // dL/dphi = sum Gbar G2, dL/domega = sum Gbar G1.
 #include "mcmb_internal.h"
+#include <cstring>
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -410,15 +411,21 @@ static int ensure_W(mcmb_ctx* c, int prec)
     return MCMB_OK;
 }
 
+// Leading dimension of stage 1's Z and Y for a batch of nb shapes (4R
+// columns per shape, rounded up to whole GEMM tiles).
+static int stage1_ld(int nb, int R)
+{
+    return (nb * 4 * R + PT - 1) / PT * PT;
+}
+
 template <class T>
-static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, int& ldy)
+static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, int& ldy, DevBuf& Yb)
 {
     DevBuf& Zb = c->pz;
-    DevBuf& Yb = c->py;
     PointSet& K = c->ks;
     const int N = K.lmax + 1, R = K.R, Kp = (N + PK - 1) / PK * PK;
     const int Nc = nb * 4 * R;
-    const int ldz = (Nc + PT - 1) / PT * PT;
+    const int ldz = stage1_ld(nb, R);
     ldy = ldz;
     MCMB_TRY(Zb.alloc(sizeof(T) * (size_t)Kp * ldz));
     const long M = (long)N * (N + 1) / 2;   // pair rows i <= j
@@ -433,15 +440,17 @@ static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, int& ldy)
     return MCMB_OK;
 }
 
+// Y is stage 1's product buffer (the context's batch buffer, or a slot of
+// the forward cache).
 template <class T>
-static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double* d_out /* [nb][3][n] */)
+static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double* d_out /* [nb][3][n] */,
+                             DevBuf& Yb)
 {
     PointSet& K = c->ks;
     const int N = K.lmax + 1, R = K.R, n = K.n;
-    DevBuf& Yb = c->py;
     DevBuf& Hb = c->ph;
     int ldy = 0;
-    MCMB_TRY(run_stage1<T>(c, d_theta, nb, ldy));
+    MCMB_TRY(run_stage1<T>(c, d_theta, nb, ldy, Yb));
     MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)PBATCH * 3 * N * R * R));
     hipLaunchKernelGGL(proj_stage2<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
                        K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb, Hb.as<double>());
@@ -449,21 +458,26 @@ static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double*
     hipLaunchKernelGGL(proj_stage3, dim3((ne + 255) / 256), dim3(256), 0, c->stream, Hb.as<double>(),
                        K.d_q.as<double>(), K.ldq, N, R, K.d_modes.as<int>(), n, nb, d_out);
     MCMB_HIP(hipGetLastError());
-    MCMB_HIP(hipStreamSynchronize(c->stream));
     return MCMB_OK;
 }
 
+// Ycached: stage 1's product of this batch from the forward cache, or null to
+// compute it.
 template <class T>
 static int project_bwd_batch(mcmb_ctx* c, const double* d_theta, int nb, const double* d_gbar, const int* d_slot,
-                             double* d_dtheta)
+                             double* d_dtheta, DevBuf* Ycached)
 {
     PointSet& K = c->ks;
     const int N = K.lmax + 1, R = K.R, n = K.n;
-    DevBuf& Yb = c->py;
     DevBuf& Hb = c->ph;
     DevBuf& Dp = c->pd;
-    int ldy = 0;
-    MCMB_TRY(run_stage1<T>(c, d_theta, nb, ldy));
+    // a cached product is this batch's own: same shape count, so the same
+    // leading dimension; anything smaller than that is recomputed
+    const long M = (long)N * (N + 1) / 2;
+    int ldy = stage1_ld(nb, R);
+    if (Ycached && Ycached->bytes < sizeof(T) * (size_t)M * ldy) Ycached = nullptr;
+    DevBuf& Yb = Ycached ? *Ycached : c->py;
+    if (!Ycached) MCMB_TRY(run_stage1<T>(c, d_theta, nb, ldy, Yb));
     MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)PBATCH * 3 * N * R * R));
     MCMB_TRY(Dp.alloc(sizeof(double) * (size_t)PBATCH * N * 2));
     size_t he = (size_t)nb * N * R * R;
@@ -475,8 +489,19 @@ static int project_bwd_batch(mcmb_ctx* c, const double* d_theta, int nb, const d
     hipLaunchKernelGGL(proj_sum_parts, dim3((nb * 2 + 255) / 256), dim3(256), 0, c->stream, Dp.as<double>(), N, nb,
                        d_dtheta);
     MCMB_HIP(hipGetLastError());
-    MCMB_HIP(hipStreamSynchronize(c->stream));
     return MCMB_OK;
+}
+
+// The forward cache holds every batch's stage-1 product when they fit in
+// PCACHE_BYTES together.
+constexpr size_t PCACHE_BYTES = 24ull << 30;
+
+static size_t ybytes(const mcmb_ctx* c, int prec)
+{
+    const PointSet& K = c->ks;
+    const int N = K.lmax + 1, R = K.R;
+    const long M = (long)N * (N + 1) / 2;
+    return (prec == MCMB_FP64 ? sizeof(double) : sizeof(float)) * (size_t)M * stage1_ld(PBATCH, R);
 }
 
 }  // namespace mcmb
@@ -571,34 +596,46 @@ int mcmb_project_fwd(mcmb_ctx* c, int precision, int ntheta, const double* theta
     MCMB_TRY(set_device(c));
     const int n = c->ks.n;
     MCMB_TRY(ensure_W(c, precision));
+    const int nbat = (ntheta + PBATCH - 1) / PBATCH;
+    // keep each batch's stage-1 product for a backward call on the same shapes
+    const bool cache = (size_t)nbat * ybytes(c, precision) <= PCACHE_BYTES;
+    c->ycache_gen = -1;
+    if (cache && (int)c->ycache.size() < nbat) c->ycache.resize(nbat);
     DevBuf dth, dout;
-    std::vector<double> out((size_t)PBATCH * 3 * n);
-    MCMB_TRY(dth.alloc(sizeof(double) * 2 * PBATCH));
-    MCMB_TRY(dout.alloc(sizeof(double) * (size_t)PBATCH * 3 * n));
+    std::vector<double> out((size_t)ntheta * 3 * n);
+    MCMB_TRY(dth.alloc(sizeof(double) * 2 * (size_t)ntheta));
+    MCMB_TRY(dout.alloc(sizeof(double) * (size_t)ntheta * 3 * n));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
-    for (int b0 = 0; b0 < ntheta; b0 += PBATCH) {
+    MCMB_HIP(hipMemcpyAsync(dth.p, theta, sizeof(double) * 2 * (size_t)ntheta, hipMemcpyHostToDevice, c->stream));
+    for (int b0 = 0, bi = 0; b0 < ntheta; b0 += PBATCH, ++bi) {
         const int nb = std::min(PBATCH, ntheta - b0);
-        MCMB_HIP(hipMemcpyAsync(dth.p, theta + 2 * b0, sizeof(double) * 2 * nb, hipMemcpyHostToDevice, c->stream));
-        if (precision == MCMB_FP64) MCMB_TRY(project_fwd_batch<double>(c, dth.as<double>(), nb, dout.as<double>()));
-        else MCMB_TRY(project_fwd_batch<float>(c, dth.as<double>(), nb, dout.as<double>()));
-        MCMB_HIP(hipMemcpy(out.data(), dout.p, sizeof(double) * (size_t)nb * 3 * n, hipMemcpyDeviceToHost));
-        for (int b = 0; b < nb; ++b) {
-            std::copy(out.begin() + (size_t)b * 3 * n, out.begin() + (size_t)b * 3 * n + n, alpha + (size_t)(b0 + b) * n);
-            if (dalpha) {
-                // [b][0][n] = d/domega, [b][1][n] = d/dphi
-                std::copy(out.begin() + (size_t)b * 3 * n + n, out.begin() + (size_t)b * 3 * n + 2 * n,
-                          dalpha + (size_t)(b0 + b) * 2 * n);
-                std::copy(out.begin() + (size_t)b * 3 * n + 2 * n, out.begin() + (size_t)b * 3 * n + 3 * n,
-                          dalpha + (size_t)(b0 + b) * 2 * n + n);
-            }
-        }
+        DevBuf& Yb = cache ? c->ycache[bi] : c->py;
+        double* d_out = dout.as<double>() + (size_t)b0 * 3 * n;
+        if (precision == MCMB_FP64) MCMB_TRY(project_fwd_batch<double>(c, dth.as<double>() + 2 * b0, nb, d_out, Yb));
+        else MCMB_TRY(project_fwd_batch<float>(c, dth.as<double>() + 2 * b0, nb, d_out, Yb));
     }
+    MCMB_HIP(hipMemcpyAsync(out.data(), dout.p, sizeof(double) * out.size(), hipMemcpyDeviceToHost, c->stream));
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
+    for (int b = 0; b < ntheta; ++b) {
+        std::copy(out.begin() + (size_t)b * 3 * n, out.begin() + (size_t)b * 3 * n + n, alpha + (size_t)b * n);
+        if (dalpha) {
+            // [b][0][n] = d/domega, [b][1][n] = d/dphi
+            std::copy(out.begin() + (size_t)b * 3 * n + n, out.begin() + (size_t)b * 3 * n + 2 * n,
+                      dalpha + (size_t)b * 2 * n);
+            std::copy(out.begin() + (size_t)b * 3 * n + 2 * n, out.begin() + (size_t)b * 3 * n + 3 * n,
+                      dalpha + (size_t)b * 2 * n + n);
+        }
+    }
+    if (cache) {
+        c->ycache_theta.assign(theta, theta + 2 * (size_t)ntheta);
+        c->ycache_prec = precision;
+        c->ycache_gen = c->ks.generation;
+    }
     float ms = 0.f;
     MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
-    c->last_launches = (ntheta + PBATCH - 1) / PBATCH;
+    c->last_launches = nbat;
     return MCMB_OK;
 }
 
@@ -621,21 +658,29 @@ int mcmb_project_bwd(mcmb_ctx* c, int precision, int ntheta, const double* theta
     }
     DevBuf dslot, dth, dg, dd;
     MCMB_TRY(dslot.alloc(sizeof(int) * slot.size()));
-    MCMB_TRY(dth.alloc(sizeof(double) * 2 * PBATCH));
-    MCMB_TRY(dg.alloc(sizeof(double) * (size_t)PBATCH * n));
-    MCMB_TRY(dd.alloc(sizeof(double) * 2 * PBATCH));
+    MCMB_TRY(dth.alloc(sizeof(double) * 2 * (size_t)ntheta));
+    MCMB_TRY(dg.alloc(sizeof(double) * (size_t)ntheta * n));
+    MCMB_TRY(dd.alloc(sizeof(double) * 2 * (size_t)ntheta));
+    // the last forward's stage-1 products, when it ran on exactly these shapes
+    const bool cached = c->ycache_gen == K.generation && c->ycache_prec == precision &&
+                        c->ycache_theta.size() == 2 * (size_t)ntheta &&
+                        !memcmp(c->ycache_theta.data(), theta, sizeof(double) * 2 * (size_t)ntheta);
     MCMB_HIP(hipMemcpyAsync(dslot.p, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice, c->stream));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
-    for (int b0 = 0; b0 < ntheta; b0 += PBATCH) {
+    MCMB_HIP(hipMemcpyAsync(dth.p, theta, sizeof(double) * 2 * (size_t)ntheta, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(dg.p, gbar, sizeof(double) * (size_t)ntheta * n, hipMemcpyHostToDevice, c->stream));
+    for (int b0 = 0, bi = 0; b0 < ntheta; b0 += PBATCH, ++bi) {
         const int nb = std::min(PBATCH, ntheta - b0);
-        MCMB_HIP(hipMemcpyAsync(dth.p, theta + 2 * b0, sizeof(double) * 2 * nb, hipMemcpyHostToDevice, c->stream));
-        MCMB_HIP(hipMemcpyAsync(dg.p, gbar + (size_t)b0 * n, sizeof(double) * (size_t)nb * n, hipMemcpyHostToDevice, c->stream));
+        DevBuf* Yc = cached ? &c->ycache[bi] : nullptr;
+        const double* th = dth.as<double>() + 2 * b0;
+        const double* gb = dg.as<double>() + (size_t)b0 * n;
+        double* dt = dd.as<double>() + 2 * b0;
         if (precision == MCMB_FP64)
-            MCMB_TRY(project_bwd_batch<double>(c, dth.as<double>(), nb, dg.as<double>(), dslot.as<int>(), dd.as<double>()));
+            MCMB_TRY(project_bwd_batch<double>(c, th, nb, gb, dslot.as<int>(), dt, Yc));
         else
-            MCMB_TRY(project_bwd_batch<float>(c, dth.as<double>(), nb, dg.as<double>(), dslot.as<int>(), dd.as<double>()));
-        MCMB_HIP(hipMemcpy(dtheta + 2 * b0, dd.p, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost));
+            MCMB_TRY(project_bwd_batch<float>(c, th, nb, gb, dslot.as<int>(), dt, Yc));
     }
+    MCMB_HIP(hipMemcpyAsync(dtheta, dd.p, sizeof(double) * 2 * (size_t)ntheta, hipMemcpyDeviceToHost, c->stream));
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     float ms = 0.f;

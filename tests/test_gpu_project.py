@@ -94,3 +94,31 @@ def test_projection_beta_solve_roundtrip(ctx):
     assert st == 0
     assert relmax(beta, ref) < 1e-8
     assert np.allclose(G @ beta, a, rtol=1e-8, atol=1e-10 * np.abs(a).max())
+
+
+@pytest.mark.parametrize("prec", [modalcmb.FP64, modalcmb.FP32])
+def test_projection_adjoint_reuses_forward_products(ctx, prec):
+    # a backward call right after a forward call on the same shapes reuses the
+    # forward's stage-1 products; the result is bit-identical to recomputing
+    # them (forward on other shapes first), and several batches of 32 shapes
+    # are covered
+    ks = W.kspace(40, 5)
+    ctx.kspace_set(ks["kn"], ks["kw"], ks["q"], ks["order"])
+    th = W.shape_batch(70, seed=11)
+    other = W.shape_batch(70, seed=12)
+    gbar = np.random.default_rng(1).standard_normal((70, ks["order"].shape[0]))
+    ctx.project_fwd(th, prec)
+    reused = ctx.project_bwd(th, gbar, prec)
+    ctx.project_fwd(other, prec)
+    recomputed = ctx.project_bwd(th, gbar, prec)
+    assert np.array_equal(reused, recomputed)
+    # a new setup drops the cached products
+    ctx.project_fwd(th, prec)
+    ks2 = W.kspace(40, 4)
+    ctx.kspace_set(ks2["kn"], ks2["kw"], ks2["q"], ks2["order"])
+    g2 = gbar[:, :ks2["order"].shape[0]]
+    _, d2 = ctx.project_fwd(other, modalcmb.FP64)
+    ref = np.einsum("bn,bxn->bx", g2, ctx.project_fwd(th, modalcmb.FP64)[1])
+    ctx.project_fwd(other, prec)
+    dt = ctx.project_bwd(th, g2, prec)
+    assert np.abs(dt - ref).max() < (1e-11 if prec == modalcmb.FP64 else 1e-5) * np.abs(ref).max()

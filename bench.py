@@ -157,17 +157,21 @@ def bench_projection(modalcmb, W, args):
     ctx.kspace_set(ks["kn"], ks["kw"], ks["q"], ks["order"])
     B, n = theta.shape[0], ctx.nk
     gbar = np.random.default_rng(7).standard_normal((B, n))
-    ctx.project_fwd(theta, modalcmb.FP32)     # warm-up (builds the weight cube)
-    ctx.project_bwd(theta, gbar, modalcmb.FP32)
+    for _ in range(2):                        # warm-up (weight cube, batch and product buffers)
+        ctx.project_fwd(theta, modalcmb.FP32)
+        ctx.project_bwd(theta, gbar, modalcmb.FP32)
     ctx.sync()
-    t = time.time()
-    a32, d32 = ctx.project_fwd(theta, modalcmb.FP32)
-    t_fwd = time.time() - t
-    k_fwd = ctx.last_kernel_ms()[0]
-    t = time.time()
-    g32 = ctx.project_bwd(theta, gbar, modalcmb.FP32)   # reuses the forward's stage-1 products
-    t_bwd = time.time() - t
-    k_bwd = ctx.last_kernel_ms()[0]
+    tf, kf, tb, kb = [], [], [], []
+    for _ in range(3):
+        t = time.time()
+        a32, d32 = ctx.project_fwd(theta, modalcmb.FP32)
+        tf.append(time.time() - t)
+        kf.append(ctx.last_kernel_ms()[0])
+        t = time.time()
+        g32 = ctx.project_bwd(theta, gbar, modalcmb.FP32)   # reuses the forward's stage-1 products
+        tb.append(time.time() - t)
+        kb.append(ctx.last_kernel_ms()[0])
+    t_fwd, k_fwd, t_bwd, k_bwd = (float(np.median(x)) for x in (tf, kf, tb, kb))
     ctx.project_fwd(theta[::-1].copy(), modalcmb.FP32)  # other shapes: the next adjoint recomputes stage 1
     t = time.time()
     g32b = ctx.project_bwd(theta, gbar, modalcmb.FP32)
@@ -189,6 +193,7 @@ def bench_projection(modalcmb, W, args):
     out = {
         "workload": f"{B} shapes S=sin(omega K + phi)/(k1k2k3)^2 form, GL {N}/side (k in [2e-4,0.2]), "
                     f"{n} modes (r<=s<=t<{R}), masked cube {npts} ordered nodes",
+        "timing": "median of 3 forward + adjoint rounds after 2 warm-up rounds",
         "fwd_tangent_s": round(t_fwd, 4), "fwd_kernel_ms": round(k_fwd, 2),
         "adjoint_s": round(t_bwd, 4), "adjoint_kernel_ms": round(k_bwd, 2),
         "adjoint_note": "right after the forward on the same shapes, reusing its stage-1 products; "

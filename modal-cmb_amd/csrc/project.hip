@@ -602,7 +602,6 @@ int mcmb_project_fwd(mcmb_ctx* c, int precision, int ntheta, const double* theta
     c->ycache_gen = -1;
     if (cache && (int)c->ycache.size() < nbat) c->ycache.resize(nbat);
     DevBuf dth, dout;
-    std::vector<double> out((size_t)ntheta * 3 * n);
     MCMB_TRY(dth.alloc(sizeof(double) * 2 * (size_t)ntheta));
     MCMB_TRY(dout.alloc(sizeof(double) * (size_t)ntheta * 3 * n));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
@@ -614,19 +613,15 @@ int mcmb_project_fwd(mcmb_ctx* c, int precision, int ntheta, const double* theta
         if (precision == MCMB_FP64) MCMB_TRY(project_fwd_batch<double>(c, dth.as<double>() + 2 * b0, nb, d_out, Yb));
         else MCMB_TRY(project_fwd_batch<float>(c, dth.as<double>() + 2 * b0, nb, d_out, Yb));
     }
-    MCMB_HIP(hipMemcpyAsync(out.data(), dout.p, sizeof(double) * out.size(), hipMemcpyDeviceToHost, c->stream));
+    // device rows [b][alpha, d/domega, d/dphi] straight into the caller's
+    // alpha[b][n] and dalpha[b][2][n]
+    MCMB_HIP(hipMemcpy2DAsync(alpha, sizeof(double) * n, dout.p, sizeof(double) * 3 * n, sizeof(double) * n, ntheta,
+                              hipMemcpyDeviceToHost, c->stream));
+    if (dalpha)
+        MCMB_HIP(hipMemcpy2DAsync(dalpha, sizeof(double) * 2 * n, dout.as<double>() + n, sizeof(double) * 3 * n,
+                                  sizeof(double) * 2 * n, ntheta, hipMemcpyDeviceToHost, c->stream));
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
-    for (int b = 0; b < ntheta; ++b) {
-        std::copy(out.begin() + (size_t)b * 3 * n, out.begin() + (size_t)b * 3 * n + n, alpha + (size_t)b * n);
-        if (dalpha) {
-            // [b][0][n] = d/domega, [b][1][n] = d/dphi
-            std::copy(out.begin() + (size_t)b * 3 * n + n, out.begin() + (size_t)b * 3 * n + 2 * n,
-                      dalpha + (size_t)b * 2 * n);
-            std::copy(out.begin() + (size_t)b * 3 * n + 2 * n, out.begin() + (size_t)b * 3 * n + 3 * n,
-                      dalpha + (size_t)b * 2 * n + n);
-        }
-    }
     if (cache) {
         c->ycache_theta.assign(theta, theta + 2 * (size_t)ntheta);
         c->ycache_prec = precision;

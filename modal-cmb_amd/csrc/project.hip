@@ -253,9 +253,11 @@ __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int 
                 if (s0 + u < R) H[(((size_t)b * 3 + x) * N + i) * R * R + (s0 + u) * R + t] = acc[x][u];
 }
 
-// alpha_x[b][n] = sum_i q_{r_n}(i) H[b][x][i][s_n][t_n]
+// alpha_x[b][n] = sum_i q_{r_n}(i) H[b][x][i][s_n][t_n], written in the
+// caller's layout: alpha[b][n] (x = 0) and dalpha[b][x-1][n] (x = 1, 2)
 __global__ void proj_stage3(const double* __restrict__ H, const double* __restrict__ q, int ldq, int N, int R,
-                            const int* __restrict__ modes, int n, int nb, double* __restrict__ out)
+                            const int* __restrict__ modes, int n, int nb, double* __restrict__ alpha,
+                            double* __restrict__ dalpha)
 {
     int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nb * 3 * n) return;
@@ -264,7 +266,8 @@ __global__ void proj_stage3(const double* __restrict__ H, const double* __restri
     const double* h = H + (((size_t)b * 3 + x) * N) * R * R + s * R + t;
     double a = 0.0;
     for (int i = 0; i < N; ++i) a += q[(size_t)i * ldq + r] * h[(size_t)i * R * R];
-    out[e] = a;   // [b][x][n]
+    if (x == 0) alpha[(size_t)b * n + m] = a;
+    else dalpha[((size_t)b * 2 + x - 1) * n + m] = a;
 }
 
 // ---- adjoint ---------------------------------------------------------------
@@ -443,8 +446,8 @@ static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, int& ldy, DevB
 // Y is stage 1's product buffer (the context's batch buffer, or a slot of
 // the forward cache).
 template <class T>
-static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double* d_out /* [nb][3][n] */,
-                             DevBuf& Yb)
+static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double* d_alpha /* [nb][n] */,
+                             double* d_dalpha /* [nb][2][n] */, DevBuf& Yb)
 {
     PointSet& K = c->ks;
     const int N = K.lmax + 1, R = K.R, n = K.n;
@@ -456,7 +459,7 @@ static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double*
                        K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb, Hb.as<double>());
     const int ne = nb * 3 * n;
     hipLaunchKernelGGL(proj_stage3, dim3((ne + 255) / 256), dim3(256), 0, c->stream, Hb.as<double>(),
-                       K.d_q.as<double>(), K.ldq, N, R, K.d_modes.as<int>(), n, nb, d_out);
+                       K.d_q.as<double>(), K.ldq, N, R, K.d_modes.as<int>(), n, nb, d_alpha, d_dalpha);
     MCMB_HIP(hipGetLastError());
     return MCMB_OK;
 }
@@ -601,25 +604,27 @@ int mcmb_project_fwd(mcmb_ctx* c, int precision, int ntheta, const double* theta
     const bool cache = (size_t)nbat * ybytes(c, precision) <= PCACHE_BYTES;
     c->ycache_gen = -1;
     if (cache && (int)c->ycache.size() < nbat) c->ycache.resize(nbat);
-    DevBuf dth, dout;
+    DevBuf dth, dal, ddal;
     MCMB_TRY(dth.alloc(sizeof(double) * 2 * (size_t)ntheta));
-    MCMB_TRY(dout.alloc(sizeof(double) * (size_t)ntheta * 3 * n));
+    MCMB_TRY(dal.alloc(sizeof(double) * (size_t)ntheta * n));
+    MCMB_TRY(ddal.alloc(sizeof(double) * (size_t)ntheta * 2 * n));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
     MCMB_HIP(hipMemcpyAsync(dth.p, theta, sizeof(double) * 2 * (size_t)ntheta, hipMemcpyHostToDevice, c->stream));
     for (int b0 = 0, bi = 0; b0 < ntheta; b0 += PBATCH, ++bi) {
         const int nb = std::min(PBATCH, ntheta - b0);
         DevBuf& Yb = cache ? c->ycache[bi] : c->py;
-        double* d_out = dout.as<double>() + (size_t)b0 * 3 * n;
-        if (precision == MCMB_FP64) MCMB_TRY(project_fwd_batch<double>(c, dth.as<double>() + 2 * b0, nb, d_out, Yb));
-        else MCMB_TRY(project_fwd_batch<float>(c, dth.as<double>() + 2 * b0, nb, d_out, Yb));
+        double* d_al = dal.as<double>() + (size_t)b0 * n;
+        double* d_dal = ddal.as<double>() + (size_t)b0 * 2 * n;
+        if (precision == MCMB_FP64)
+            MCMB_TRY(project_fwd_batch<double>(c, dth.as<double>() + 2 * b0, nb, d_al, d_dal, Yb));
+        else
+            MCMB_TRY(project_fwd_batch<float>(c, dth.as<double>() + 2 * b0, nb, d_al, d_dal, Yb));
     }
-    // device rows [b][alpha, d/domega, d/dphi] straight into the caller's
-    // alpha[b][n] and dalpha[b][2][n]
-    MCMB_HIP(hipMemcpy2DAsync(alpha, sizeof(double) * n, dout.p, sizeof(double) * 3 * n, sizeof(double) * n, ntheta,
-                              hipMemcpyDeviceToHost, c->stream));
+    // the device arrays already have the caller's layout: one contiguous copy each
+    MCMB_HIP(hipMemcpyAsync(alpha, dal.p, sizeof(double) * (size_t)ntheta * n, hipMemcpyDeviceToHost, c->stream));
     if (dalpha)
-        MCMB_HIP(hipMemcpy2DAsync(dalpha, sizeof(double) * 2 * n, dout.as<double>() + n, sizeof(double) * 3 * n,
-                                  sizeof(double) * 2 * n, ntheta, hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipMemcpyAsync(dalpha, ddal.p, sizeof(double) * (size_t)ntheta * 2 * n, hipMemcpyDeviceToHost,
+                                c->stream));
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     if (cache) {

@@ -19,6 +19,7 @@
 #include "weights.h"
 #include <algorithm>
 #include <climits>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -60,23 +61,26 @@ __global__ void sep_utab(const double* __restrict__ q, int ldq, int nl, const in
     U[e] = q[(long)l * ldq + xy.x] * q[(long)l * ldq + xy.y];
 }
 
-// Weights of the row groups, staged densely for stage 1: for group gb,
-// chunk c of 16 l3 values (l3 = kmin + kstep (16 c + kk)) and member row g,
-// Wd[wofs[gb] + 256 c + 16 kk + g] = w (0 outside the row's range).  KIND 0 /
-// 1: the l-space weight permsix h^2 v v v with the exact / asymptotic h^2,
-// formed here; KIND 2: copied from the walk-order weights ws.
-template <int KIND>
+// Weights of the row groups, staged densely for stage 1: for group gb of G
+// rows, l3 index kg of the group's grid (l3 = kmin + kstep kg) and member row
+// g, Wd[wofs[gb] + G kg + g] = w (0 outside the row's range), kg padded to a
+// multiple of 16.  KIND 0 / 1: the l-space weight permsix h^2 v v v with the
+// exact / asymptotic h^2, formed here; KIND 2: copied from the walk-order
+// weights ws.
+template <int KIND, int G>
 __global__ __launch_bounds__(256) void sep_wstage(const RowDesc* __restrict__ rows, const long* __restrict__ rowptr,
                                                   const double* __restrict__ ws, const int* __restrict__ groups,
                                                   const int4* __restrict__ ginfo, const long* __restrict__ wofs,
                                                   int kstep, const double* __restrict__ f,
                                                   const double* __restrict__ v, double* __restrict__ Wd)
 {
-    __shared__ int gi[SR_G], gj[SR_G], gk0[SR_G], gnk[SR_G];
-    __shared__ long gp[SR_G];
+    static_assert(256 % G == 0 && (256 / G) <= 16, "whole l3 values per pass, passes within a 16-value chunk");
+    constexpr int KP = 256 / G;   // l3 values per pass
+    __shared__ int gi[G], gj[G], gk0[G], gnk[G];
+    __shared__ long gp[G];
     const int tid = threadIdx.x;
-    if (tid < SR_G) {
-        const int r = groups[(long)blockIdx.x * SR_G + tid];
+    if (tid < G) {
+        const int r = groups[(long)blockIdx.x * G + tid];
         RowDesc d = {0, 0, 0, 0};
         long p = 0;
         if (r >= 0) {
@@ -91,18 +95,19 @@ __global__ __launch_bounds__(256) void sep_wstage(const RowDesc* __restrict__ ro
     }
     __syncthreads();
     const int4 info = ginfo[blockIdx.x];
-    const int kmin = info.x, nkk = info.y;
+    const int kmin = info.x, nkk = info.y, nkp = (nkk + 15) / 16 * 16;
     double* const W = Wd + wofs[blockIdx.x];
-    const int kk = tid >> 4, g = tid & 15;
-    for (int c0 = 0; c0 < nkk; c0 += SR_KC) {
+    const int kk = tid / G, g = tid % G;
+    for (int k0 = 0; k0 < nkp; k0 += KP) {
+        const int kg = k0 + kk;
         double w = 0.0;
-        if (c0 + kk < nkk) {
-            const int k = kmin + kstep * (c0 + kk);
+        if (kg < nkk) {
+            const int k = kmin + kstep * kg;
             const int d = k - gk0[g];
             if (d >= 0 && d % kstep == 0 && d / kstep < gnk[g])
                 w = KIND == 2 ? ws[gp[g] + d / kstep] : lspace_weight(gi[g], gj[g], k, KIND, f, v);
         }
-        W[16 * c0 + tid] = w;
+        W[G * k0 + tid] = w;   // = W[G kg + g]
     }
 }
 
@@ -181,6 +186,91 @@ __global__ __launch_bounds__(NT) void sep_rowmom(const int* __restrict__ groups,
             if (f < NP) Mr[f] = acc[g][u];
         }
     }
+}
+
+// Stage 1 on FP64 MFMA: one workgroup (blockDim.x / 64 waves) per group of
+// 16 RT rows that share l1 and the parity class of l3.  M[row][f] =
+// sum_kg W[row][kg] U[l3(kg)][f] as 16 x 16 x 4 products with A = the staged
+// weights (rows x l3) and B = the pair products (l3 x columns).  Wave w owns
+// the 16 RT rows x the 16 CPW columns from 16 CPW w (ldu >= 16 CPW x waves,
+// padding columns of U are zero); every U element is loaded once per workgroup,
+// straight into B registers, and feeds the RT row tiles.  The operands of
+// K-steps s + 1 and s + 2 are in flight while step s computes.
+constexpr int SR_GM = 32;   // rows per group of the MFMA kernel (RT = 2)
+template <int CPW, int RT>
+__global__ __launch_bounds__(512, 2) void sep_rowmom_mfma(const int* __restrict__ groups,
+                                                          const int4* __restrict__ ginfo,
+                                                          const long* __restrict__ wofs,
+                                                          const double* __restrict__ Wd, int row_base, int kstep,
+                                                          const double* __restrict__ U, int ldu, int NP, int ld,
+                                                          double* __restrict__ M)
+{
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int4 info = ginfo[blockIdx.x];
+    const int kmin = info.x, nkk = info.y;
+    const int nks = (nkk + 3) >> 2;   // K-steps of 4 l3 values (the staged weights are zero-padded past nkk)
+    constexpr int G = 16 * RT;
+    const double* const Wg = Wd + wofs[blockIdx.x] + G * (lane >> 4) + (lane & 15);
+    const int col0 = 16 * CPW * w + (lane & 15);
+    const double* const Ub = U + (long)kmin * ldu + col0;
+    const long ustep = (long)kstep * ldu;
+
+    fd4 acc[RT][CPW];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CPW; ++ct) acc[rt][ct] = (fd4){0.0, 0.0, 0.0, 0.0};
+
+    // K-step s: A = W[16 rt + lane % 16][4 s + lane / 16],
+    //           B = U[l3(4 s + lane / 16)][col0 + 16 ct] (l3 index clamped to
+    //           the last one, where the weights are zero)
+    auto load = [&](int s, double (&a)[RT], double (&b)[CPW]) {
+        const int kg = 4 * s + (lane >> 4);
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) a[rt] = Wg[G * 4 * s + 16 * rt];
+        const double* Ur = Ub + ustep * min(kg, nkk - 1);
+#pragma unroll
+        for (int ct = 0; ct < CPW; ++ct) b[ct] = Ur[16 * ct];
+    };
+    auto step = [&](const double (&a)[RT], const double (&b)[CPW]) {
+#pragma unroll
+        for (int ct = 0; ct < CPW; ++ct)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+                acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+    };
+    // three operand sets in rotation; reloads past the last K-step are clamped
+    // to it (unused), so the loop is one straight-line body
+    double a0[RT], b0[CPW], a1[RT], b1[CPW], a2[RT], b2[CPW];
+    int s = 0;
+    if (nks > 0) {
+        load(0, a0, b0);
+        load(min(1, nks - 1), a1, b1);
+    }
+    for (; s + 2 < nks; s += 3) {
+        load(s + 2, a2, b2);
+        step(a0, b0);
+        load(min(s + 3, nks - 1), a0, b0);
+        step(a1, b1);
+        load(min(s + 4, nks - 1), a1, b1);
+        step(a2, b2);
+    }
+    if (s < nks) step(a0, b0);
+    if (s + 1 < nks) step(a1, b1);
+    // output element v of a lane: row 4 v + lane / 16, column lane % 16 of the 16 x 16 block
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = groups[(long)blockIdx.x * G + 16 * rt + 4 * v + (lane >> 4)];
+            if (r < 0) continue;
+            double* const Mr = M + (long)(r - row_base) * ld;
+#pragma unroll
+            for (int ct = 0; ct < CPW; ++ct) {
+                const int f = col0 + 16 * ct;
+                if (f < NP) Mr[f] = acc[rt][ct][v];
+            }
+        }
 }
 
 // Stages 2 and 3:  C_task[m][n] (+)= sum_{kk < K} U[row(kk)][m] * B[boff + kk][n]
@@ -361,7 +451,8 @@ template <class T> size_t append(std::vector<int>& buf, const std::vector<T>& v)
 }
 
 // NP pairs, ld = padded leading dimension of U, M and Y rows.
-void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1, int NP, int ld, BlockTables& BT)
+void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1, int NP, int ld, int gsz,
+                 BlockTables& BT)
 {
     SepBlock& T = BT.b;
     const int kstep = std::max(1, L.kstep);
@@ -378,7 +469,7 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
     auto flush = [&](std::vector<int>& c) {
         if (c.empty()) return;
         int kmin = INT_MAX, kmax = INT_MIN;
-        for (int g = 0; g < SR_G; ++g) {
+        for (int g = 0; g < gsz; ++g) {
             groups.push_back(g < (int)c.size() ? c[g] : -1);
             if (g < (int)c.size() && L.rows[c[g]].nk > 0) {
                 const RowDesc& d = L.rows[c[g]];
@@ -389,7 +480,7 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
         const int nkk = kmax >= kmin ? (kmax - kmin) / kstep + 1 : 0;
         ginfo.push_back(make_int4(nkk ? kmin : 0, nkk, 0, 0));
         wofs.push_back(wsize);
-        wsize += (long)SR_G * SR_KC * ((nkk + SR_KC - 1) / SR_KC);
+        wsize += (long)gsz * 16 * ((nkk + 15) / 16);
         c.clear();
     };
     T.contig2 = T.contig3 = true;
@@ -402,7 +493,7 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
             if (d.j != j0 + (r - R.r0)) T.contig2 = false;
             std::vector<int>& c = cls[((d.k0 % kstep) + kstep) % kstep];
             c.push_back(r);
-            if ((int)c.size() == SR_G) flush(c);
+            if ((int)c.size() == gsz) flush(c);
         }
         for (auto& c : cls) flush(c);
         const int lt = ru - b0;
@@ -425,7 +516,7 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
     const long N3 = (long)NP * ld;
     for (long n0 = 0; n0 < N3; n0 += SG_TN)
         for (int m0 = 0; m0 < NP; m0 += SG_TM) tile3.push_back(make_int4(0, m0, (int)n0, 0));
-    T.ngroups = (int)(groups.size() / SR_G);
+    T.ngroups = (int)(groups.size() / gsz);
     T.nt2 = (int)tile2.size();
     T.nt3 = (int)tile3.size();
     T.wsize = wsize;
@@ -453,12 +544,13 @@ unsigned sep_blocks(long ntiles, int GS)
 // Tables of the rows [row_lo, row_hi) of L, uploaded to c->sg_tab (kept for
 // the next call on the same point set, generation and rows), and the row
 // pointers in c->sg_rp.
-static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int NP, int ld, bool& rowptr_ready)
+static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int NP, int ld, int gsz,
+                          bool& rowptr_ready)
 {
     SepPlan& P = c->sep;
     const bool same_set = P.ps == &L && P.gen == L.generation;
     rowptr_ready = same_set && c->sg_rp.p;
-    if (same_set && P.row_lo == row_lo && P.row_hi == row_hi) return MCMB_OK;
+    if (same_set && P.row_lo == row_lo && P.row_hi == row_hi && P.gsz == gsz) return MCMB_OK;
     P = SepPlan();
     // runs of rows with equal l1, and blocks of runs whose stage outputs fit
     // the caps (M: 16 GiB, Y: 8 GiB)
@@ -486,7 +578,7 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
     }
     std::vector<BlockTables> tabs(blocks.size());
     for (size_t b = 0; b < blocks.size(); ++b) {
-        build_block(L, runs, blocks[b].first, blocks[b].second, NP, ld, tabs[b]);
+        build_block(L, runs, blocks[b].first, blocks[b].second, NP, ld, gsz, tabs[b]);
         P.max_tab = std::max(P.max_tab, tabs[b].buf.size());
         P.max_wsize = std::max(P.max_wsize, tabs[b].b.wsize);
     }
@@ -501,6 +593,7 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
     P.gen = L.generation;
     P.row_lo = row_lo;
     P.row_hi = row_hi;
+    P.gsz = gsz;
     return MCMB_OK;
 }
 
@@ -510,7 +603,23 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         return fail(MCMB_ERR_ARG, "separable Gram: precision %d / basis width %d not supported", precision, L.R);
     const int n = L.n, R = L.R, NP = R * (R + 1) / 2;
     const int ld = NP + (NP & 1);          // even leading dimension of M, Y and G rows: 16-byte aligned
-    const int ldu = (NP + 63) / 64 * 64;   // pair-product rows: whole 64-column blocks, padding 0
+    // stage 1 on FP64 MFMA over 32-row groups (1, 2, 4 or 8 waves of 4, 6 or
+    // 8 column tiles: NP <= 1024), else on the VALU over 16-row groups;
+    // MCMB_SEP_L3=valu selects the VALU kernel.  The fewest padding tiles,
+    // then the fewest waves (a workgroup of 4 or 8 waves loads the 4 SIMDs
+    // of a CU evenly; 5 waves of 6 tiles at 465 pairs ran 35% slower than
+    // 4 waves of 8)
+    const char* l3env = getenv("MCMB_SEP_L3");
+    const int nct = (NP + 15) / 16;   // 16-column tiles
+    int cpw = 0, nwv = 0;             // MFMA column tiles per wave, waves
+    for (int w : {1, 2, 4, 8})
+        for (int cc : {8, 6, 4})
+            if (cc * w >= nct && (!cpw || cc * w < cpw * nwv)) cpw = cc, nwv = w;
+    const bool l3mfma = cpw > 0 && !(l3env && !strcmp(l3env, "valu"));
+    const int gsz = l3mfma ? SR_GM : SR_G;
+    // pair-product rows: the MFMA kernel's columns, or whole 64-column blocks
+    // for the VALU kernel; padding 0
+    const int ldu = l3mfma ? 16 * cpw * nwv : (NP + 63) / 64 * 64;
     const long NPL = (long)NP * ld;        // one Y slab / one G row
     const int nl = (int)(L.q.size() / (size_t)L.ldq);
     c->nphases = 0;
@@ -522,7 +631,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         return MCMB_OK;
     }
     bool rowptr_ready = false;
-    MCMB_TRY(separable_plan(c, L, row_lo, row_hi, NP, ld, rowptr_ready));
+    MCMB_TRY(separable_plan(c, L, row_lo, row_hi, NP, ld, gsz, rowptr_ready));
     const SepPlan& P = c->sep;
     std::vector<int2> pxy(NP);
     for (int x = 0; x < R; ++x)
@@ -566,16 +675,22 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         const SepBlock& T = P.blocks[b];
         const int* tab = c->sg_tab.as<int>() + b * P.max_tab;
         MCMB_TRY(launch_guard("weight staging", (unsigned long long)T.ngroups, 256));
-#define SEP_WSTAGE(KK)                                                                                          \
-    hipLaunchKernelGGL((sep_wstage<KK>), dim3((unsigned)T.ngroups), dim3(256), 0, c->stream,                    \
+#define SEP_WSTAGE(KK, GG)                                                                                      \
+    hipLaunchKernelGGL((sep_wstage<KK, GG>), dim3((unsigned)T.ngroups), dim3(256), 0, c->stream,                \
                        L.d_rows.as<RowDesc>(), (const long*)c->sg_rp.as<long>(), (const double*)c->sg_ws.as<double>(), \
                        tab + T.o_groups, reinterpret_cast<const int4*>(tab + T.o_ginfo),                        \
                        reinterpret_cast<const long*>(tab + T.o_wofs), L.kstep,                                   \
                        (const double*)L.d_ftab.as<double>(), (const double*)L.d_v.as<double>(),                  \
                        c->sg_wd.as<double>())
-        if (wkind == 0) SEP_WSTAGE(0);
-        else if (wkind == 1) SEP_WSTAGE(1);
-        else SEP_WSTAGE(2);
+        if (gsz == SR_GM) {
+            if (wkind == 0) SEP_WSTAGE(0, SR_GM);
+            else if (wkind == 1) SEP_WSTAGE(1, SR_GM);
+            else SEP_WSTAGE(2, SR_GM);
+        } else {
+            if (wkind == 0) SEP_WSTAGE(0, SR_G);
+            else if (wkind == 1) SEP_WSTAGE(1, SR_G);
+            else SEP_WSTAGE(2, SR_G);
+        }
 #undef SEP_WSTAGE
         MCMB_HIP(hipGetLastError());
         ++launches;
@@ -596,14 +711,23 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         if (!one_block) MCMB_TRY(wstage(b));
         // stage 1
         if (one_block) MCMB_TRY(phase("l3 contraction (row moments)"));
-        MCMB_TRY(launch_guard("row moments", (unsigned long long)T.ngroups, 256));
+        MCMB_TRY(launch_guard("row moments", (unsigned long long)T.ngroups, l3mfma ? 64 * nwv : 256));
 #define SEP_ROWMOM(NN, GG, FF, HH)                                                                            \
     hipLaunchKernelGGL((sep_rowmom<NN, GG, FF, HH>), dim3((unsigned)T.ngroups), dim3(NN), 0, c->stream,      \
                        tab + T.o_groups, ginfo, wofs, (const double*)c->sg_wd.as<double>(), T.row0, L.kstep,  \
                        (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
-        if (NP <= 256) SEP_ROWMOM(256, 1, 1, 1);
+#define SEP_ROWMOM_MFMA(CC)                                                                                   \
+    hipLaunchKernelGGL((sep_rowmom_mfma<CC, SR_GM / 16>), dim3((unsigned)T.ngroups), dim3(64 * nwv), 0, c->stream,       \
+                       tab + T.o_groups, ginfo, wofs, (const double*)c->sg_wd.as<double>(), T.row0, L.kstep,  \
+                       (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
+        if (l3mfma) {
+            if (cpw == 4) SEP_ROWMOM_MFMA(4);
+            else if (cpw == 6) SEP_ROWMOM_MFMA(6);
+            else SEP_ROWMOM_MFMA(8);
+        } else if (NP <= 256) SEP_ROWMOM(256, 1, 1, 1);
         else if (NP <= 512) SEP_ROWMOM(256, 1, 2, 1);
         else SEP_ROWMOM(256, 1, 3, 2);
+#undef SEP_ROWMOM_MFMA
 #undef SEP_ROWMOM
         MCMB_HIP(hipGetLastError());
         // stages 2 and 3

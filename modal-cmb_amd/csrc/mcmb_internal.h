@@ -119,6 +119,7 @@ struct SepPlan {
     const void* ps = nullptr;   // point set the tables were built for
     long gen = -1;              // its generation
     int row_lo = -1, row_hi = -1;
+    int gsz = 0;                // rows per stage-1 group
     size_t max_tab = 0;         // ints per block in the table buffer
     long max_rows = 0, max_runs = 0, max_wsize = 0;
     std::vector<SepBlock> blocks;

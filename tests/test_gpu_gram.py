@@ -157,30 +157,42 @@ def test_gamma_gl_rejects_inexact_node_count(ctx):
 
 
 def _gram_paths(ctx, prec, paths=("separable", "factored", "direct")):
-    """The same Gram through each contraction order named in paths."""
+    """The same Gram through each contraction order named in paths
+    ("separable-valu": the separable order with its l3 contraction on the
+    VALU instead of FP64 MFMA)."""
     out = {}
-    old = os.environ.get("MCMB_GRAM_PATH")
+    old = {k: os.environ.get(k) for k in ("MCMB_GRAM_PATH", "MCMB_SEP_L3")}
     try:
         for p in paths:
-            os.environ["MCMB_GRAM_PATH"] = p
+            os.environ["MCMB_GRAM_PATH"] = p.split("-")[0]
+            if p == "separable-valu":
+                os.environ["MCMB_SEP_L3"] = "valu"
+            else:
+                os.environ.pop("MCMB_SEP_L3", None)
             out[p] = ctx.gamma_tet(prec)
     finally:
-        if old is None:
-            os.environ.pop("MCMB_GRAM_PATH", None)
-        else:
-            os.environ["MCMB_GRAM_PATH"] = old
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     return out
 
 
-@pytest.mark.parametrize("lmin,L,R,mod,h2", [(2, 500, 10, True, 0), (3, 401, 12, False, 1), (0, 257, 30, True, 0)])
+@pytest.mark.parametrize("lmin,L,R,mod,h2", [(2, 500, 10, True, 0), (3, 401, 12, False, 1), (0, 257, 30, True, 0),
+                                             (2, 161, 32, True, 0)])
 def test_gamma_contraction_orders_agree(ctx, lmin, L, R, mod, h2):
     # three summation orders of the same FP64 sum: point by point; row moments
-    # times pair products; successive contractions over l3, l2, l1
+    # times pair products; successive contractions over l3, l2, l1 (with the
+    # l3 contraction on FP64 MFMA or on the VALU; R = 32 has 528 pairs, past
+    # the MFMA kernel's 512 columns, so both run the VALU kernel there)
     ws = W.lspace(L, R, mod, lmin=lmin)
     ctx.lspace_set(lmin, L, ws["q"], ws["v"], ws["order"], h2_mode=h2)
-    G = _gram_paths(ctx, modalcmb.FP64)
+    G = _gram_paths(ctx, modalcmb.FP64, ("separable", "separable-valu", "factored", "direct"))
     assert rel_elem(G["separable"], G["direct"]) < 1e-12
+    assert rel_elem(G["separable-valu"], G["direct"]) < 1e-12
     assert rel_elem(G["factored"], G["direct"]) < 1e-12
+    G.pop("separable-valu")
     for g in G.values():
         assert np.array_equal(g, g.T)
     # the default is the separable order, bit-identical on repeat

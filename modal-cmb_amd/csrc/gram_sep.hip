@@ -273,6 +273,140 @@ __global__ __launch_bounds__(512, 2) void sep_rowmom_mfma(const int* __restrict_
         }
 }
 
+// Stage 1 on FP64 MFMA with the l-space weights formed in place (no staged
+// weights in HBM): as sep_rowmom_mfma with RT = 2 and 4 waves, over chunks of
+// 16 l3 values.  Between a chunk's K-steps the workgroup forms the next
+// chunk's 16 x 32 weights (2 per thread), permsix h^2 v v v (exact h^2 from
+// the f table or the asymptotic form), from copies of the f and v tables in
+// LDS, into the other half of a double-buffered LDS tile; one barrier per
+// chunk.  That VALU and LDS work issues between MFMAs already queued.  The
+// arithmetic is lspace_weight's, so the weights are bit-identical to the
+// staged ones.  B operands are loaded 3 K-steps ahead into 4 register sets.
+// l-space only (l3 step 2).
+template <int CPW, int ASYM>
+__global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ groups,
+                                                        const int4* __restrict__ ginfo,
+                                                        const RowDesc* __restrict__ rows,
+                                                        const double* __restrict__ f, int nf,
+                                                        const double* __restrict__ v, int nv, int row_base,
+                                                        const double* __restrict__ U, int ldu, int NP, int ld,
+                                                        double* __restrict__ M)
+{
+    constexpr int G = SR_GM, RT = SR_GM / 16;
+    static_assert(16 * G == 2 * 256, "two weights of a chunk per thread");
+    extern __shared__ double tabs[];   // f[nf], then v[nv]
+    __shared__ double Wt[2][16][G];
+    __shared__ int gj[G], gk0[G], gnk[G];
+    double* const fs = tabs;
+    double* const vs = tabs + nf;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    for (int e = tid; e < nf; e += 256) fs[e] = f[e];
+    for (int e = tid; e < nv; e += 256) vs[e] = v[e];
+    if (tid < G) {
+        const int r = groups[(long)blockIdx.x * G + tid];
+        const RowDesc d = r >= 0 ? rows[r] : RowDesc{0, 0, 0, 0};
+        gj[tid] = d.j;
+        gk0[tid] = d.k0;
+        gnk[tid] = d.nk;
+    }
+    // rows of a group share l1; the first is never padding
+    const int gi = rows[groups[(long)blockIdx.x * G]].i;
+    const int4 info = ginfo[blockIdx.x];
+    const int kmin = info.x, nkk = info.y;
+    const int nks = (nkk + 3) >> 2;           // K-steps
+    const int nch = (nkk + 15) >> 4;          // chunks of 4 K-steps
+    const int col0 = 16 * CPW * w + (lane & 15);
+    const double* const Ub = U + (long)kmin * ldu + col0;
+    const long ustep = 2l * ldu;
+
+    // weights of chunk c into Wt[buf]: thread tid forms elements tid and
+    // tid + 256, element e = (kk = e / 32, g = e % 32), l3 = kmin + 2 (16 c + kk)
+    auto form = [&](int c, int buf) {
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            const int e = tid + 256 * x;
+            const int kk = e / G, g = e % G;
+            const int kg = 16 * c + kk, l3 = kmin + 2 * kg;
+            const int d = l3 - gk0[g];
+            const bool in = kg < nkk && d >= 0 && !(d & 1) && (d >> 1) < gnk[g];
+            const double wv = lspace_weight(in ? gi : 0, in ? gj[g] : 0, in ? l3 : 0, ASYM, fs, vs);
+            Wt[buf][kk][g] = in ? wv : 0.0;
+        }
+    };
+    auto loadb = [&](int s, double (&b)[CPW]) {
+        const double* Ur = Ub + ustep * min(4 * s + (lane >> 4), nkk - 1);
+#pragma unroll
+        for (int ct = 0; ct < CPW; ++ct) b[ct] = Ur[16 * ct];
+    };
+
+    fd4 acc[RT][CPW];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < CPW; ++ct) acc[rt][ct] = (fd4){0.0, 0.0, 0.0, 0.0};
+    auto mfmas = [&](const double (&a)[RT], const double (&b)[CPW]) {
+#pragma unroll
+        for (int ct = 0; ct < CPW; ++ct)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt)
+                acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+    };
+    double bs[4][CPW];
+    __syncthreads();   // tables and row info
+    if (nks > 0) {
+        form(0, 0);
+#pragma unroll
+        for (int x = 0; x < 3; ++x) loadb(min(x, nks - 1), bs[x]);
+    }
+    __syncthreads();
+    // all chunks but the last: 4 K-steps each, the next chunk's weights formed
+    // after the first step's MFMAs
+    int c = 0;
+    for (; c + 1 < nch; ++c) {
+        const int buf = c & 1;
+        double a[4][RT];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) a[ks][rt] = Wt[buf][4 * ks + (lane >> 4)][16 * rt + (lane & 15)];
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            loadb(min(4 * c + ks + 3, nks - 1), bs[(ks + 3) & 3]);
+            mfmas(a[ks], bs[ks]);
+            if (ks == 0) form(c + 1, buf ^ 1);
+        }
+        __syncthreads();
+    }
+    // the last chunk: its K-steps below nks (steps 4c .. 4c + 2 are loaded;
+    // step 4c + 3 is loaded here)
+    if (c < nch) {
+        const int buf = c & 1;
+        loadb(min(4 * c + 3, nks - 1), bs[3]);
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+            if (4 * c + ks < nks) {
+                double a[RT];
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) a[rt] = Wt[buf][4 * ks + (lane >> 4)][16 * rt + (lane & 15)];
+                mfmas(a, bs[ks]);
+            }
+        }
+    }
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int vv = 0; vv < 4; ++vv) {
+            const int r = groups[(long)blockIdx.x * G + 16 * rt + 4 * vv + (lane >> 4)];
+            if (r < 0) continue;
+            double* const Mr = M + (long)(r - row_base) * ld;
+#pragma unroll
+            for (int ct = 0; ct < CPW; ++ct) {
+                const int fcol = col0 + 16 * ct;
+                if (fcol < NP) Mr[fcol] = acc[rt][ct][vv];
+            }
+        }
+}
+
 // Stages 2 and 3:  C_task[m][n] (+)= sum_{kk < K} U[row(kk)][m] * B[boff + kk][n]
 // for m < Md, n < Nd, over a list of 160 x 96 tiles (task, m0, n0); row(kk) =
 // aoff + kk when the A rows are contiguous (CONTIG), else aidx[aoff + kk].
@@ -654,11 +788,21 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         ++np;
         return MCMB_OK;
     };
-    MCMB_TRY(c->sg_wd.alloc(sizeof(double) * (size_t)std::max<long>(1, P.max_wsize)));
     const bool one_block = P.blocks.size() == 1;
     // l-space weights are formed where they are staged; other point sets stage
     // their walk-order weights
     const int wkind = (L.kspace || L.uniform) ? 2 : (L.h2mode == MCMB_H2_ASYMPTOTIC ? 1 : 0);
+    // the MFMA l3 contraction forms l-space weights itself (with 4 waves, 171
+    // to 512 pairs) when the f and v tables and its weight tiles fit 64 KiB of
+    // LDS (lmax <= 2800);
+    // MCMB_SEP_WEIGHTS=staged stages them instead
+    const int nf = (int)(L.d_ftab.bytes / sizeof(double)), nv = (int)L.v.size();
+    const char* wenv = getenv("MCMB_SEP_WEIGHTS");
+    constexpr size_t lw_static = sizeof(double) * 2 * 16 * SR_GM + sizeof(int) * 3 * SR_GM;
+    const bool wfused = l3mfma && SR_GM == 32 && nwv == 4 && wkind != 2 && L.kstep == 2 &&
+                        (size_t)(nf + nv) * sizeof(double) + lw_static <= 65536 &&
+                        !(wenv && !strcmp(wenv, "staged"));
+    if (!wfused) MCMB_TRY(c->sg_wd.alloc(sizeof(double) * (size_t)std::max<long>(1, P.max_wsize)));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
     int launches = 0;
     MCMB_TRY(phase("pair products"));
@@ -696,19 +840,21 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         ++launches;
         return MCMB_OK;
     };
-    MCMB_TRY(phase("weights"));
-    if (wkind == 2) {
-        MCMB_TRY(pointset_weights(c, L, c->sg_rp, c->sg_ws, 0, rowptr_ready));
-        ++launches;
+    if (!wfused) {
+        MCMB_TRY(phase("weights"));
+        if (wkind == 2) {
+            MCMB_TRY(pointset_weights(c, L, c->sg_rp, c->sg_ws, 0, rowptr_ready));
+            ++launches;
+        }
+        if (one_block) MCMB_TRY(wstage(0));
     }
-    if (one_block) MCMB_TRY(wstage(0));
     if (!one_block) MCMB_TRY(phase("l3, l2, l1 contractions (all blocks)"));
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const SepBlock& T = P.blocks[b];
         const int* tab = c->sg_tab.as<int>() + b * P.max_tab;
         const int4* ginfo = reinterpret_cast<const int4*>(tab + T.o_ginfo);
         const long* wofs = reinterpret_cast<const long*>(tab + T.o_wofs);
-        if (!one_block) MCMB_TRY(wstage(b));
+        if (!one_block && !wfused) MCMB_TRY(wstage(b));
         // stage 1
         if (one_block) MCMB_TRY(phase("l3 contraction (row moments)"));
         MCMB_TRY(launch_guard("row moments", (unsigned long long)T.ngroups, l3mfma ? 64 * nwv : 256));
@@ -720,13 +866,30 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     hipLaunchKernelGGL((sep_rowmom_mfma<CC, SR_GM / 16>), dim3((unsigned)T.ngroups), dim3(64 * nwv), 0, c->stream,       \
                        tab + T.o_groups, ginfo, wofs, (const double*)c->sg_wd.as<double>(), T.row0, L.kstep,  \
                        (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
-        if (l3mfma) {
+#define SEP_ROWMOM_LW(CC, AA)                                                                                 \
+    hipLaunchKernelGGL((sep_rowmom_lw<CC, AA>), dim3((unsigned)T.ngroups), dim3(64 * nwv),                    \
+                       sizeof(double) * (size_t)(nf + nv), c->stream, tab + T.o_groups, ginfo,                 \
+                       (const RowDesc*)L.d_rows.as<RowDesc>(), (const double*)L.d_ftab.as<double>(), nf,       \
+                       (const double*)L.d_v.as<double>(), nv, T.row0, (const double*)dU, ldu, NP, ld,          \
+                       c->sg_M.as<double>())
+        if (wfused) {
+            if (wkind == 0) {
+                if (cpw == 4) SEP_ROWMOM_LW(4, 0);
+                else if (cpw == 6) SEP_ROWMOM_LW(6, 0);
+                else SEP_ROWMOM_LW(8, 0);
+            } else {
+                if (cpw == 4) SEP_ROWMOM_LW(4, 1);
+                else if (cpw == 6) SEP_ROWMOM_LW(6, 1);
+                else SEP_ROWMOM_LW(8, 1);
+            }
+        } else if (l3mfma) {
             if (cpw == 4) SEP_ROWMOM_MFMA(4);
             else if (cpw == 6) SEP_ROWMOM_MFMA(6);
             else SEP_ROWMOM_MFMA(8);
         } else if (NP <= 256) SEP_ROWMOM(256, 1, 1, 1);
         else if (NP <= 512) SEP_ROWMOM(256, 1, 2, 1);
         else SEP_ROWMOM(256, 1, 3, 2);
+#undef SEP_ROWMOM_LW
 #undef SEP_ROWMOM_MFMA
 #undef SEP_ROWMOM
         MCMB_HIP(hipGetLastError());

@@ -156,19 +156,26 @@ def test_gamma_gl_rejects_inexact_node_count(ctx):
         ctx.gamma_gl(npt=100)
 
 
+_SEP_VARIANTS = {"separable-valu": ("MCMB_SEP_L3", "valu"), "separable-staged": ("MCMB_SEP_WEIGHTS", "staged")}
+
+
 def _gram_paths(ctx, prec, paths=("separable", "factored", "direct")):
     """The same Gram through each contraction order named in paths
     ("separable-valu": the separable order with its l3 contraction on the
-    VALU instead of FP64 MFMA)."""
+    VALU instead of FP64 MFMA; "separable-staged": with the l-space weights
+    staged in HBM by their own kernel instead of formed by the l3
+    contraction)."""
     out = {}
-    old = {k: os.environ.get(k) for k in ("MCMB_GRAM_PATH", "MCMB_SEP_L3")}
+    keys = ("MCMB_GRAM_PATH", "MCMB_SEP_L3", "MCMB_SEP_WEIGHTS")
+    old = {k: os.environ.get(k) for k in keys}
     try:
         for p in paths:
             os.environ["MCMB_GRAM_PATH"] = p.split("-")[0]
-            if p == "separable-valu":
-                os.environ["MCMB_SEP_L3"] = "valu"
-            else:
-                os.environ.pop("MCMB_SEP_L3", None)
+            for k in keys[1:]:
+                os.environ.pop(k, None)
+            if p in _SEP_VARIANTS:
+                k, val = _SEP_VARIANTS[p]
+                os.environ[k] = val
             out[p] = ctx.gamma_tet(prec)
     finally:
         for k, v in old.items():
@@ -184,15 +191,17 @@ def _gram_paths(ctx, prec, paths=("separable", "factored", "direct")):
 def test_gamma_contraction_orders_agree(ctx, lmin, L, R, mod, h2):
     # three summation orders of the same FP64 sum: point by point; row moments
     # times pair products; successive contractions over l3, l2, l1 (with the
-    # l3 contraction on FP64 MFMA or on the VALU; R = 32 has 528 pairs, past
-    # the MFMA kernel's 512 columns, so both run the VALU kernel there)
+    # l3 contraction on FP64 MFMA or on the VALU, the weights formed in it or
+    # staged).  The l3 contraction sums each row in l3 order with fused
+    # multiply-adds on every variant, so the separable variants agree bit for
+    # bit.
     ws = W.lspace(L, R, mod, lmin=lmin)
     ctx.lspace_set(lmin, L, ws["q"], ws["v"], ws["order"], h2_mode=h2)
-    G = _gram_paths(ctx, modalcmb.FP64, ("separable", "separable-valu", "factored", "direct"))
+    G = _gram_paths(ctx, modalcmb.FP64, ("separable", "separable-valu", "separable-staged", "factored", "direct"))
     assert rel_elem(G["separable"], G["direct"]) < 1e-12
-    assert rel_elem(G["separable-valu"], G["direct"]) < 1e-12
     assert rel_elem(G["factored"], G["direct"]) < 1e-12
-    G.pop("separable-valu")
+    assert np.array_equal(G.pop("separable-valu"), G["separable"])
+    assert np.array_equal(G.pop("separable-staged"), G["separable"])
     for g in G.values():
         assert np.array_equal(g, g.T)
     # the default is the separable order, bit-identical on repeat
@@ -234,7 +243,9 @@ def test_separable_phases_and_repeat_calls(ctx):
     G1 = ctx.gamma_tet(modalcmb.FP64)
     ph = ctx.last_phases()
     names = [p for p, _ in ph]
-    assert names[:2] == ["pair products", "weights"] and names[-1] == "assembly" and len(ph) == 6
+    # 55 pairs: the l3 contraction runs as one wave with staged weights
+    assert names == ["pair products", "weights", "l3 contraction (row moments)", "l2 contraction (FP64 MFMA)",
+                     "l1 contraction (FP64 MFMA)", "assembly"]
     total, _ = ctx.last_kernel_ms()
     assert sum(t for _, t in ph) == pytest.approx(total, rel=0.05, abs=0.05)
     part, _ = ctx.gamma_tet_slice(modalcmb.FP64, 1, 3)
@@ -251,3 +262,17 @@ def test_separable_phases_and_repeat_calls(ctx):
     finally:
         del os.environ["MCMB_GRAM_PATH"]
     assert ctx.last_phases() == []
+
+
+def test_separable_phases_fused_weights(ctx):
+    # 300 pairs: the l3 contraction (4 waves) forms the l-space weights itself,
+    # bit-identical to the staged weights
+    ws = W.lspace(160, 24, True)
+    ctx.lspace_set(2, 160, ws["q"], ws["v"], ws["order"])
+    G = _gram_paths(ctx, modalcmb.FP64, ("separable", "separable-staged", "factored"))
+    ctx.gamma_tet(modalcmb.FP64)
+    names = [p for p, _ in ctx.last_phases()]
+    assert names == ["pair products", "l3 contraction (row moments)", "l2 contraction (FP64 MFMA)",
+                     "l1 contraction (FP64 MFMA)", "assembly"]
+    assert np.array_equal(G["separable"], G["separable-staged"])
+    assert rel_elem(G["separable"], G["factored"]) < 1e-12

@@ -8,12 +8,19 @@
 // with the slot-ordered tensor
 //   G[a][e][f] = sum_{l in T} w(l) U_a(l1) U_e(l2) U_f(l3),
 // formed in three stages over the rows (l1, l2) of the ordered tetrapyd:
-//   1. M[row][f]    = sum_{l3 in row} w U_f(l3)          (row moments, FP64 VALU)
+//   1. M[row][f]    = sum_{l3 in row} w U_f(l3)          (row moments, FP64 MFMA)
 //   2. Y[i][e][f]   = sum_{rows of l1 = i} U_e(l2) M[row][f]   (FP64 MFMA, batched per i)
-//   3. G[a][e, f]   = sum_i U_a(i) Y[i][e][f]             (FP64 MFMA, one GEMM)
-// and a 36-term gather per Gamma entry.  At lmax 2500 with R = 30 (NP = 465)
-// the stages cost 3.0e11, 6.7e11 and 2.5e11 multiply-adds, against 8e15 for
-// the point-by-point contraction; the summation is FP64 throughout and in a
+//   3. G2[a][{e,f}] = sum_i U_a(i) (Y[i][e][f] + Y[i][f][e])   (FP64 MFMA, one GEMM
+//                     over the NP(NP+1)/2 unordered pairs {e, f})
+// Only the fully symmetrised tensor enters Gamma: summing the 36 slot
+// assignments over the 6 orders of the three matched pairs first,
+//   Gamma_mn = (1/36) sum_{sigma in S3} Gs(p_{1 s1}, p_{2 s2}, p_{3 s3}),
+//   p_ab = {m_a, n_b},  Gs(x, y, z) = G2[x][{y,z}] + G2[y][{x,z}] + G2[z][{x,y}]
+// (Gs = the sum of G over the 6 orders of its slots, kept for x <= y <= z),
+// so stage 3 runs over half the columns and the assembly gathers 6 entries of
+// a 0.14 GB table.  At lmax 2500 with R = 30 (NP = 465) the stages cost
+// 3.0e11, 6.7e11 and 1.3e11 multiply-adds, against 8e15 for the
+// point-by-point contraction; the summation is FP64 throughout and in a
 // fixed order (no atomics), so repeated calls are bit-identical.
 #include "mcmb_internal.h"
 #include "weights.h"
@@ -528,9 +535,67 @@ __global__ __launch_bounds__(256, 2) void sep_gemm(const double* __restrict__ U,
         }
 }
 
-// Gamma[m][n] = Gamma[n][m] = (1/36) sum over the 6 x 6 slot assignments of G.
-__global__ void sep_assemble(const double* __restrict__ G, int NP, int ld, int R, const int* __restrict__ modes,
-                             int n, double* __restrict__ gamma)
+// Ys[i][{e,f}] = Y[i][e][f] + Y[i][f][e] for e <= f ({e,f} = pair_index(e, f,
+// NP)): one workgroup per (32 x 32 block pair E <= F, run i); both blocks are
+// read row by row (coalesced), the second transposed through LDS.  The
+// padding element of each Ys row (index NP(NP+1)/2 when that is odd) is
+// zeroed.
+__global__ __launch_bounds__(256) void sep_ysym(const double* __restrict__ Y, long ldyi, int ld, int NP, int nb,
+                                                double* __restrict__ Ys, long ldys)
+{
+    __shared__ double A[32][33], Bt[32][33];
+    int bE = 0, rem = blockIdx.x;   // block pair (bE <= bF) in row-major upper-triangle order
+    while (rem >= nb - bE) {
+        rem -= nb - bE;
+        ++bE;
+    }
+    const int bF = bE + rem;
+    const int e0 = 32 * bE, f0 = 32 * bF;
+    const double* const Yi = Y + (long)blockIdx.y * ldyi;
+    double* const Yo = Ys + (long)blockIdx.y * ldys;
+    const int tid = threadIdx.x, c = tid & 31;
+#pragma unroll
+    for (int r4 = 0; r4 < 4; ++r4) {
+        const int r = (tid >> 5) + 8 * r4;
+        A[r][c] = (e0 + r < NP && f0 + c < NP) ? Yi[(long)(e0 + r) * ld + f0 + c] : 0.0;
+        Bt[r][c] = (f0 + r < NP && e0 + c < NP) ? Yi[(long)(f0 + r) * ld + e0 + c] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r4 = 0; r4 < 4; ++r4) {
+        const int r = (tid >> 5) + 8 * r4;
+        const int e = e0 + r, f = f0 + c;
+        if (e < NP && f < NP && e <= f) Yo[pair_index(e, f, NP)] = A[r][c] + Bt[c][r];
+    }
+    const long npp = (long)NP * (NP + 1) / 2;
+    if (blockIdx.x == 0 && tid == 0 && ldys > npp) Yo[npp] = 0.0;
+}
+
+// Gs[off[{x,y}] + z - y] = G2[x][{y,z}] + G2[y][{x,z}] + G2[z][{x,y}] for
+// x <= y <= z: one workgroup per pair {x, y}, threads over z.
+__global__ __launch_bounds__(256) void sep_gsym(const double* __restrict__ G2, long ldg, int NP,
+                                                const long* __restrict__ off, double* __restrict__ Gs)
+{
+    const int p = blockIdx.x;
+    int lo = 0, hi = NP - 1;   // x = the largest row whose first pair index is <= p
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pair_index(mid, mid, NP) <= p) lo = mid;
+        else hi = mid - 1;
+    }
+    const int x = lo, y = x + (p - pair_index(x, x, NP));
+    const double* const Gx = G2 + (long)x * ldg + pair_index(y, y, NP);   // [{y, z}] at z - y
+    const double* const Gy = G2 + (long)y * ldg + pair_index(x, x, NP) - x;   // [{x, z}] at z
+    const long pxy = pair_index(x, y, NP);
+    double* const out = Gs + off[p];
+    for (int z = y + threadIdx.x; z < NP; z += blockDim.x)
+        out[z - y] = Gx[z - y] + Gy[z] + G2[(long)z * ldg + pxy];
+}
+
+// Gamma[m][n] = Gamma[n][m] = (1/36) sum over the 6 matchings sigma of the
+// slots of m and n of Gs at the sorted pair triple.
+__global__ void sep_assemble(const double* __restrict__ Gs, const long* __restrict__ off, int NP, int R,
+                             const int* __restrict__ modes, int n, double* __restrict__ gamma)
 {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (long)n * n) return;
@@ -545,13 +610,13 @@ __global__ void sep_assemble(const double* __restrict__ G, int NP, int ld, int R
 #pragma unroll
         for (int y = 0; y < 3; ++y) pid[x][y] = pair_index(min(a[x], b[y]), max(a[x], b[y]), R);
     constexpr int P6[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
-    const long NP2 = (long)NP * ld;
     double acc = 0.0;
 #pragma unroll
-    for (int p = 0; p < 6; ++p)
-#pragma unroll
-        for (int r = 0; r < 6; ++r)
-            acc += G[pid[P6[p][0]][P6[r][0]] * NP2 + (long)pid[P6[p][1]][P6[r][1]] * ld + pid[P6[p][2]][P6[r][2]]];
+    for (int s = 0; s < 6; ++s) {
+        int u = pid[0][P6[s][0]], v = pid[1][P6[s][1]], w = pid[2][P6[s][2]];
+        const int x = min(u, min(v, w)), z = max(u, max(v, w)), y = u + v + w - x - z;
+        acc += Gs[off[pair_index(x, y, NP)] + (z - y)];
+    }
     acc *= (1.0 / 36.0);
     gamma[(long)m * n + nn] = acc;
     gamma[(long)nn * n + m] = acc;
@@ -647,7 +712,7 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
     task3[0].coff = 0;
     task3[0].aoff = T.contig3 ? runs[b0].i : 0;
     task3[0].K = T.nruns;
-    const long N3 = (long)NP * ld;
+    const long N3 = (long)NP * (NP + 1) / 2;   // stage 3 runs over the unordered pairs {e, f}
     for (long n0 = 0; n0 < N3; n0 += SG_TN)
         for (int m0 = 0; m0 < NP; m0 += SG_TM) tile3.push_back(make_int4(0, m0, (int)n0, 0));
     T.ngroups = (int)(groups.size() / gsz);
@@ -773,8 +838,26 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
 
     MCMB_TRY(c->sg_U.alloc(sizeof(double) * (size_t)nl * ldu + sizeof(int2) * NP));
     MCMB_TRY(c->sg_M.alloc(sizeof(double) * (size_t)P.max_rows * ld));
+    const long NPP = (long)NP * (NP + 1) / 2;   // unordered pairs {e, f}
+    const long ldp = NPP + (NPP & 1);          // even leading dimension of Ys and G2 rows
+    const long NGS = NPP * (NP + 2) / 3;       // x <= y <= z triples: NP (NP + 1) (NP + 2) / 6
     MCMB_TRY(c->sg_Y.alloc(sizeof(double) * (size_t)P.max_runs * NPL));
-    MCMB_TRY(c->sg_G.alloc(sizeof(double) * (size_t)NP * NPL));
+    MCMB_TRY(c->sg_Ys.alloc(sizeof(double) * (size_t)P.max_runs * ldp));
+    MCMB_TRY(c->sg_G.alloc(sizeof(double) * (size_t)NP * ldp));
+    MCMB_TRY(c->sg_Gs.alloc(sizeof(double) * (size_t)NGS));
+    if (c->sg_off_np != NP) {
+        // off[{x,y}] = first Gs element of the triples (x, y, z >= y)
+        std::vector<long> off(NPP);
+        long o = 0;
+        for (int x = 0; x < NP; ++x)
+            for (int y = x; y < NP; ++y) {
+                off[pair_index(x, y, NP)] = o;
+                o += NP - y;
+            }
+        MCMB_TRY(c->sg_off.alloc(sizeof(long) * (size_t)NPP));
+        MCMB_HIP(hipMemcpy(c->sg_off.p, off.data(), sizeof(long) * (size_t)NPP, hipMemcpyHostToDevice));
+        c->sg_off_np = NP;
+    }
     double* dU = c->sg_U.as<double>();
     int2* d_pxy = reinterpret_cast<int2*>(dU + (size_t)nl * ldu);
     MCMB_HIP(hipMemcpy(d_pxy, pxy.data(), sizeof(int2) * NP, hipMemcpyHostToDevice));
@@ -909,26 +992,43 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
             SEP_GEMM(false, 2, T.nt2, tab + T.o_aidx2, c->sg_M.as<double>(), ld, c->sg_Y.as<double>(), ld, NP, NP,
                      tab + T.o_task2, tab + T.o_tile2, 0);
         MCMB_HIP(hipGetLastError());
+        // Ys[i][{e,f}] = Y[i][e][f] + Y[i][f][e]
+        if (one_block) MCMB_TRY(phase("symmetrisation"));
+        {
+            const int nb32 = (NP + 31) / 32;
+            const unsigned nbp = (unsigned)(nb32 * (nb32 + 1) / 2);
+            if (T.nruns > 65535) return fail(MCMB_ERR_ARG, "separable Gram: %d l1 runs in one block", T.nruns);
+            MCMB_TRY(launch_guard("symmetrisation", (unsigned long long)nbp * T.nruns, 256));
+            hipLaunchKernelGGL(sep_ysym, dim3(nbp, (unsigned)T.nruns), dim3(256), 0, c->stream,
+                               (const double*)c->sg_Y.as<double>(), NPL, ld, NP, nb32, c->sg_Ys.as<double>(), ldp);
+            MCMB_HIP(hipGetLastError());
+        }
         if (one_block) MCMB_TRY(phase("l1 contraction (FP64 MFMA)"));
         MCMB_TRY(launch_guard("l1 contraction", sep_blocks(T.nt3, GS), 256));
         if (T.contig3)
-            SEP_GEMM(true, 3, T.nt3, tab + T.o_aidx3, c->sg_Y.as<double>(), NPL, c->sg_G.as<double>(), NPL, NP,
-                     (int)NPL, tab + T.o_task3, tab + T.o_tile3, b > 0 ? 1 : 0);
+            SEP_GEMM(true, 3, T.nt3, tab + T.o_aidx3, c->sg_Ys.as<double>(), ldp, c->sg_G.as<double>(), ldp, NP,
+                     (int)NPP, tab + T.o_task3, tab + T.o_tile3, b > 0 ? 1 : 0);
         else
-            SEP_GEMM(false, 3, T.nt3, tab + T.o_aidx3, c->sg_Y.as<double>(), NPL, c->sg_G.as<double>(), NPL, NP,
-                     (int)NPL, tab + T.o_task3, tab + T.o_tile3, b > 0 ? 1 : 0);
+            SEP_GEMM(false, 3, T.nt3, tab + T.o_aidx3, c->sg_Ys.as<double>(), ldp, c->sg_G.as<double>(), ldp, NP,
+                     (int)NPP, tab + T.o_task3, tab + T.o_tile3, b > 0 ? 1 : 0);
 #undef SEP_GEMM
         MCMB_HIP(hipGetLastError());
-        launches += 3;
+        launches += 4;
     }
     MCMB_TRY(phase("assembly"));
     {
+        // Gs from G2, then the 6-term sums
+        MCMB_TRY(launch_guard("symmetrised G", (unsigned long long)NPP, 256));
+        hipLaunchKernelGGL(sep_gsym, dim3((unsigned)NPP), dim3(256), 0, c->stream, (const double*)c->sg_G.as<double>(),
+                           ldp, NP, (const long*)c->sg_off.as<long>(), c->sg_Gs.as<double>());
+        MCMB_HIP(hipGetLastError());
         const long ng = (long)n * n;
         MCMB_TRY(launch_guard("assembly", (unsigned long long)((ng + 255) / 256), 256));
         hipLaunchKernelGGL(sep_assemble, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, c->stream,
-                           (const double*)c->sg_G.as<double>(), NP, ld, R, L.d_modes.as<int>(), n, d_gamma);
+                           (const double*)c->sg_Gs.as<double>(), (const long*)c->sg_off.as<long>(), NP, R,
+                           L.d_modes.as<int>(), n, d_gamma);
         MCMB_HIP(hipGetLastError());
-        ++launches;
+        launches += 2;
     }
     MCMB_HIP(hipEventRecord(c->evp[np], c->stream));
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));

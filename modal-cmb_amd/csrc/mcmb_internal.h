@@ -153,6 +153,8 @@ struct mcmb_ctx {
     mcmb::DevBuf d_aux;
     // separable Gram: weights, row pointers, pair products, stage outputs, tables
     mcmb::DevBuf sg_ws, sg_rp, sg_wd, sg_U, sg_M, sg_Y, sg_G, sg_tab;
+    mcmb::DevBuf sg_Ys, sg_Gs, sg_off;   // symmetrised Y, symmetrised G, its row offsets
+    int sg_off_np = 0;                   // pair count the offsets were built for
     mcmb::SepPlan sep;
     // per-launch timing of the last call that records phases (event k+1 - event k)
     hipEvent_t evp[mcmb::MAX_PHASES + 1] = {};

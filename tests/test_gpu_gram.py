@@ -245,7 +245,7 @@ def test_separable_phases_and_repeat_calls(ctx):
     names = [p for p, _ in ph]
     # 55 pairs: the l3 contraction runs as one wave with staged weights
     assert names == ["pair products", "weights", "l3 contraction (row moments)", "l2 contraction (FP64 MFMA)",
-                     "l1 contraction (FP64 MFMA)", "assembly"]
+                     "symmetrisation", "l1 contraction (FP64 MFMA)", "assembly"]
     total, _ = ctx.last_kernel_ms()
     assert sum(t for _, t in ph) == pytest.approx(total, rel=0.05, abs=0.05)
     part, _ = ctx.gamma_tet_slice(modalcmb.FP64, 1, 3)
@@ -273,6 +273,6 @@ def test_separable_phases_fused_weights(ctx):
     ctx.gamma_tet(modalcmb.FP64)
     names = [p for p, _ in ctx.last_phases()]
     assert names == ["pair products", "l3 contraction (row moments)", "l2 contraction (FP64 MFMA)",
-                     "l1 contraction (FP64 MFMA)", "assembly"]
+                     "symmetrisation", "l1 contraction (FP64 MFMA)", "assembly"]
     assert np.array_equal(G["separable"], G["separable-staged"])
     assert rel_elem(G["separable"], G["factored"]) < 1e-12

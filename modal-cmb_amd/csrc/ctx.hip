@@ -197,14 +197,15 @@ int mcmb_lspace_set(mcmb_ctx* c, int lmin, int lmax, int ldq, const double* q, c
     L.q.assign(q, q + (size_t)(lmax + 1) * ldq);
     L.v.assign(v, v + (size_t)(lmax + 1));
     build_lspace_rows(lmin, lmax, L.rows, L.npoints);
-    // f(a) = prod_{k<=a}(2k-1)/(2k) in long double, a <= 3 lmax / 2 + 1
+    // f(a) = prod_{k<=a}(2k-1)/(2k) in long double, a <= 3 lmax / 2 + 1, and
+    // 1/((2a+1) f(a)), interleaved: ftab[2a], ftab[2a+1]
     int amax = 3 * lmax / 2 + 2;
-    std::vector<double> ftab(amax + 1);
+    std::vector<double> ftab(2 * (size_t)(amax + 1));
     long double acc = 1.0L;
-    ftab[0] = 1.0;
-    for (int a = 1; a <= amax; ++a) {
-        acc *= (long double)(2 * a - 1) / (long double)(2 * a);
-        ftab[a] = (double)acc;
+    for (int a = 0; a <= amax; ++a) {
+        if (a > 0) acc *= (long double)(2 * a - 1) / (long double)(2 * a);
+        ftab[2 * a] = (double)acc;
+        ftab[2 * a + 1] = (double)(1.0L / ((long double)(2 * a + 1) * acc));
     }
     MCMB_TRY(L.d_ftab.alloc(sizeof(double) * ftab.size()));
     MCMB_TRY(L.d_v.alloc(sizeof(double) * L.v.size()));

@@ -284,9 +284,8 @@ __global__ __launch_bounds__(512, 2) void sep_rowmom_mfma(const int* __restrict_
 // weights in HBM): as sep_rowmom_mfma with RT = 2 and 4 waves, over chunks of
 // 16 l3 values.  Between a chunk's K-steps the workgroup forms the next
 // chunk's 16 x 32 weights (2 per thread), permsix h^2 v v v (exact h^2 from
-// the f table or the asymptotic form), from copies of the f and v tables in
-// LDS, into the other half of a double-buffered LDS tile; one barrier per
-// chunk.  That VALU and LDS work issues between MFMAs already queued.  The
+// the f table or the asymptotic form; the tables are cache-resident), into
+// the other half of a double-buffered LDS tile; one barrier per chunk.  The
 // arithmetic is lspace_weight's, so the weights are bit-identical to the
 // staged ones.  B operands are loaded 3 K-steps ahead into 4 register sets.
 // l-space only (l3 step 2).
@@ -294,21 +293,16 @@ template <int CPW, int ASYM>
 __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ groups,
                                                         const int4* __restrict__ ginfo,
                                                         const RowDesc* __restrict__ rows,
-                                                        const double* __restrict__ f, int nf,
-                                                        const double* __restrict__ v, int nv, int row_base,
+                                                        const double* __restrict__ f,
+                                                        const double* __restrict__ v, int row_base,
                                                         const double* __restrict__ U, int ldu, int NP, int ld,
                                                         double* __restrict__ M)
 {
     constexpr int G = SR_GM, RT = SR_GM / 16;
     static_assert(16 * G == 2 * 256, "two weights of a chunk per thread");
-    extern __shared__ double tabs[];   // f[nf], then v[nv]
     __shared__ double Wt[2][16][G];
     __shared__ int gj[G], gk0[G], gnk[G];
-    double* const fs = tabs;
-    double* const vs = tabs + nf;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    for (int e = tid; e < nf; e += 256) fs[e] = f[e];
-    for (int e = tid; e < nv; e += 256) vs[e] = v[e];
     if (tid < G) {
         const int r = groups[(long)blockIdx.x * G + tid];
         const RowDesc d = r >= 0 ? rows[r] : RowDesc{0, 0, 0, 0};
@@ -326,20 +320,6 @@ __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ 
     const double* const Ub = U + (long)kmin * ldu + col0;
     const long ustep = 2l * ldu;
 
-    // weights of chunk c into Wt[buf]: thread tid forms elements tid and
-    // tid + 256, element e = (kk = e / 32, g = e % 32), l3 = kmin + 2 (16 c + kk)
-    auto form = [&](int c, int buf) {
-#pragma unroll
-        for (int x = 0; x < 2; ++x) {
-            const int e = tid + 256 * x;
-            const int kk = e / G, g = e % G;
-            const int kg = 16 * c + kk, l3 = kmin + 2 * kg;
-            const int d = l3 - gk0[g];
-            const bool in = kg < nkk && d >= 0 && !(d & 1) && (d >> 1) < gnk[g];
-            const double wv = lspace_weight(in ? gi : 0, in ? gj[g] : 0, in ? l3 : 0, ASYM, fs, vs);
-            Wt[buf][kk][g] = in ? wv : 0.0;
-        }
-    };
     auto loadb = [&](int s, double (&b)[CPW]) {
         const double* Ur = Ub + ustep * min(4 * s + (lane >> 4), nkk - 1);
 #pragma unroll
@@ -359,15 +339,67 @@ __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ 
                 acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
     };
     double bs[4][CPW];
-    __syncthreads();   // tables and row info
+    __syncthreads();   // row info
+    // weights of chunk c into Wt[buf]: thread tid forms elements tid and
+    // tid + 256, element e = (kk = e / 32, g = e % 32), l3 = kmin + 2 (16 c + kk);
+    // both are of row g = tid % 32, whose constants stay in registers.  The
+    // table reads of a chunk (gather) are issued a chunk's MFMAs before their
+    // use (weigh); the arithmetic is lspace_weight's, in its order.
+    const int g = tid & (G - 1), kk0 = tid / G;
+    const int rj = gj[g], rk0 = gk0[g], rnk = gnk[g];
+    const double vi = v[gi], vj = v[rj];
+    const long oddij = (long)(2 * gi + 1) * (2 * rj + 1);
+    struct Tab {
+        double f1, f2, f3, rg, vk;
+        int k;
+        bool in;
+    };
+    auto gather = [&](int c, Tab (&t)[2]) {
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            const int kg = 16 * c + kk0 + 8 * x, l3 = kmin + 2 * kg;
+            const int d = l3 - rk0;
+            t[x].in = kg < nkk && d >= 0 && !(d & 1) && (d >> 1) < rnk;
+            if (ASYM) {
+                t[x].k = l3;
+            } else {
+                // out of range: the indices of (0, 0, 0), in the tables
+                const int ii = t[x].in ? gi : 0, jj = t[x].in ? rj : 0, k3 = t[x].in ? l3 : 0;
+                const int gg = (ii + jj + k3) >> 1;
+                t[x].k = k3;
+                t[x].f1 = f[2 * (gg - ii)];
+                t[x].f2 = f[2 * (gg - jj)];
+                t[x].f3 = f[2 * (gg - k3)];
+                t[x].rg = f[2 * gg + 1];
+                t[x].vk = v[k3];
+            }
+        }
+    };
+    auto weigh = [&](const Tab (&t)[2], int c, int buf) {
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            double wv;
+            if (ASYM) {
+                wv = lspace_weight(t[x].in ? gi : 0, t[x].in ? rj : 0, t[x].in ? t[x].k : 0, 1, f, v);
+            } else {
+                const double pre = (double)(oddij * (2 * t[x].k + 1)) * 0.079577471545947667884;
+                const double h2 = pre * (t[x].f1 * t[x].f2 * t[x].f3) * t[x].rg;
+                wv = permsix_d(gi, rj, t[x].k) * h2 * vi * vj * t[x].vk;
+            }
+            Wt[buf][kk0 + 8 * x][g] = t[x].in ? wv : 0.0;
+        }
+        (void)c;
+    };
     if (nks > 0) {
-        form(0, 0);
+        Tab t0[2];
+        gather(0, t0);
+        weigh(t0, 0, 0);
 #pragma unroll
         for (int x = 0; x < 3; ++x) loadb(min(x, nks - 1), bs[x]);
     }
     __syncthreads();
-    // all chunks but the last: 4 K-steps each, the next chunk's weights formed
-    // after the first step's MFMAs
+    // all chunks but the last: 4 K-steps each; the next chunk's table reads
+    // are issued first and its weights formed after the third step's MFMAs
     int c = 0;
     for (; c + 1 < nch; ++c) {
         const int buf = c & 1;
@@ -376,11 +408,17 @@ __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ 
         for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) a[ks][rt] = Wt[buf][4 * ks + (lane >> 4)][16 * rt + (lane & 15)];
+        Tab t[2];
+        gather(c + 1, t);
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
             loadb(min(4 * c + ks + 3, nks - 1), bs[(ks + 3) & 3]);
             mfmas(a[ks], bs[ks]);
-            if (ks == 0) form(c + 1, buf ^ 1);
+            if (ks == 2) {
+                __builtin_amdgcn_sched_barrier(0);
+                weigh(t, c + 1, buf ^ 1);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
         __syncthreads();
     }
@@ -875,15 +913,10 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     // l-space weights are formed where they are staged; other point sets stage
     // their walk-order weights
     const int wkind = (L.kspace || L.uniform) ? 2 : (L.h2mode == MCMB_H2_ASYMPTOTIC ? 1 : 0);
-    // the MFMA l3 contraction forms l-space weights itself (with 4 waves, 171
-    // to 512 pairs) when the f and v tables and its weight tiles fit 64 KiB of
-    // LDS (lmax <= 2800);
-    // MCMB_SEP_WEIGHTS=staged stages them instead
-    const int nf = (int)(L.d_ftab.bytes / sizeof(double)), nv = (int)L.v.size();
+    // the MFMA l3 contraction forms l-space weights itself when it runs 4
+    // waves (171 to 512 pairs); MCMB_SEP_WEIGHTS=staged stages them instead
     const char* wenv = getenv("MCMB_SEP_WEIGHTS");
-    constexpr size_t lw_static = sizeof(double) * 2 * 16 * SR_GM + sizeof(int) * 3 * SR_GM;
     const bool wfused = l3mfma && SR_GM == 32 && nwv == 4 && wkind != 2 && L.kstep == 2 &&
-                        (size_t)(nf + nv) * sizeof(double) + lw_static <= 65536 &&
                         !(wenv && !strcmp(wenv, "staged"));
     if (!wfused) MCMB_TRY(c->sg_wd.alloc(sizeof(double) * (size_t)std::max<long>(1, P.max_wsize)));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
@@ -950,11 +983,10 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
                        tab + T.o_groups, ginfo, wofs, (const double*)c->sg_wd.as<double>(), T.row0, L.kstep,  \
                        (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
 #define SEP_ROWMOM_LW(CC, AA)                                                                                 \
-    hipLaunchKernelGGL((sep_rowmom_lw<CC, AA>), dim3((unsigned)T.ngroups), dim3(64 * nwv),                    \
-                       sizeof(double) * (size_t)(nf + nv), c->stream, tab + T.o_groups, ginfo,                 \
-                       (const RowDesc*)L.d_rows.as<RowDesc>(), (const double*)L.d_ftab.as<double>(), nf,       \
-                       (const double*)L.d_v.as<double>(), nv, T.row0, (const double*)dU, ldu, NP, ld,          \
-                       c->sg_M.as<double>())
+    hipLaunchKernelGGL((sep_rowmom_lw<CC, AA>), dim3((unsigned)T.ngroups), dim3(64 * nwv), 0, c->stream,     \
+                       tab + T.o_groups, ginfo, (const RowDesc*)L.d_rows.as<RowDesc>(),                        \
+                       (const double*)L.d_ftab.as<double>(), (const double*)L.d_v.as<double>(), T.row0,        \
+                       (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
         if (wfused) {
             if (wkind == 0) {
                 if (cpw == 4) SEP_ROWMOM_LW(4, 0);

@@ -12,11 +12,15 @@ __device__ __forceinline__ double permsix_d(int i, int j, int k)
 
 // Exact h^2 = (2l1+1)(2l2+1)(2l3+1)/(4 pi) * f(g-l1) f(g-l2) f(g-l3)/((2g+1) f(g)),
 // f(a) = prod_{k<=a}(2k-1)/(2k), g = (l1+l2+l3)/2 (even-parity triangle only).
-__device__ __forceinline__ double h2_exact(int i, int j, int k, const double* __restrict__ f)
+// ft interleaves ft[2a] = f(a) and ft[2a+1] = 1/((2a+1) f(a)) (rounded once
+// from long double), so no division is made per point.  The product of the
+// three odd factors is exact in a double for l < 1e5.
+__device__ __forceinline__ double h2_exact(int i, int j, int k, const double* __restrict__ ft)
 {
     int g = (i + j + k) >> 1;
-    double pre = (2.0 * i + 1.0) * (2.0 * j + 1.0) * (2.0 * k + 1.0) * 0.079577471545947667884;  // 1/(4 pi)
-    return pre * (f[g - i] * f[g - j] * f[g - k]) / ((2.0 * g + 1.0) * f[g]);
+    const long odd = (long)(2 * i + 1) * (2 * j + 1) * (2 * k + 1);
+    const double pre = (double)odd * 0.079577471545947667884;  // 1/(4 pi)
+    return pre * (ft[2 * (g - i)] * ft[2 * (g - j)] * ft[2 * (g - k)]) * ft[2 * g + 1];
 }
 
 // calculate_geometric.c:59-92 (the reference's asymptotic h^2)

@@ -22,6 +22,7 @@
 // triples), Gbar[i][j][t] = sum_s q_s(j) Hbar[i][s][t], and reduces
 // dL/dphi = sum Gbar G2, dL/domega = sum Gbar G1.
 #include "mcmb_internal.h"
+#include <type_traits>
 #include <cstring>
 #include <algorithm>
 #include <cmath>
@@ -421,6 +422,98 @@ static int stage1_ld(int nb, int R)
     return (nb * 4 * R + PT - 1) / PT * PT;
 }
 
+// FP32 stage-1 GEMM C[M][ldc] = A[M][Kp] B[Kp][ldb] (B zero-padded to ldb
+// columns and Kp rows, A to Kp columns): 128 x 128 tiles, 4 waves of 64 x 64
+// on v_mfma_f32_16x16x4f32, K chunks of 16 in double-buffered LDS.  Operands
+// come in as 16-byte loads two chunks ahead of their use (two register
+// sets), A stored [m][k] and B [k][n] with row strides (20 and 144 floats)
+// that keep both the 16-byte stores and the MFMA operand reads free of bank
+// conflicts.  The K order is that of proj_gemm<float>, so the results are
+// the same bit for bit.
+constexpr int PF_LDA = PK + 4, PF_LDB = PT + 16;
+__global__ __launch_bounds__(256, 2) void proj_gemm_f32(const float* __restrict__ A, const float* __restrict__ B,
+                                                        int M, int Kp, int ldb, float* __restrict__ C, int ldc)
+{
+    __shared__ __align__(16) float As[2][PT][PF_LDA];
+    __shared__ __align__(16) float Bs[2][PK][PF_LDB];
+    const int m0 = blockIdx.y * PT, n0 = blockIdx.x * PT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    f4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = (f4){0.f, 0.f, 0.f, 0.f};
+    // a chunk is 512 float4 of A (row r = e / 4, k = 4 (e % 4)) and 512 of B
+    // (k = e / 32, n = 4 (e % 32)), e = tid + 256 u
+    const f4 z4 = {0.f, 0.f, 0.f, 0.f};
+    auto gload = [&](int k0, f4 (&ra)[2], f4 (&rb)[2]) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = tid + 256 * u;
+            const int r = e >> 2, kq = e & 3, gm = m0 + r;
+            ra[u] = gm < M ? *reinterpret_cast<const f4*>(A + (size_t)gm * Kp + k0 + 4 * kq) : z4;
+            const int kb = e >> 5, c4 = e & 31;
+            rb[u] = *reinterpret_cast<const f4*>(B + (size_t)(k0 + kb) * ldb + n0 + 4 * c4);
+        }
+    };
+    auto sstore = [&](int buf, const f4 (&ra)[2], const f4 (&rb)[2]) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = tid + 256 * u;
+            *reinterpret_cast<f4*>(&As[buf][e >> 2][4 * (e & 3)]) = ra[u];
+            *reinterpret_cast<f4*>(&Bs[buf][e >> 5][4 * (e & 31)]) = rb[u];
+        }
+    };
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int ks = 0; ks < PK / 4; ++ks) {
+            const int kk = ks * 4 + (lane >> 4);
+            float av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = As[buf][wm * 64 + a * 16 + (lane & 15)][kk];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bv[b] = Bs[buf][kk][wn * 64 + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+    };
+    // chunk kc is in LDS buffer kc & 1, chunk kc + 1 in registers and chunk
+    // kc + 2 loading; loads past the last chunk repeat it (unused), so every
+    // load is unconditional
+    const int nk = Kp / PK;
+    f4 ra0[2], rb0[2], ra1[2], rb1[2];
+    gload(0, ra0, rb0);
+    gload(min(1, nk - 1) * PK, ra1, rb1);
+    sstore(0, ra0, rb0);
+    __syncthreads();
+    for (int kc = 0;; kc += 2) {
+        gload(min(kc + 2, nk - 1) * PK, ra0, rb0);
+        compute(0);
+        sstore(1, ra1, rb1);
+        __syncthreads();
+        if (kc + 1 >= nk) break;
+        gload(min(kc + 3, nk - 1) * PK, ra1, rb1);
+        compute(1);
+        sstore(0, ra0, rb0);
+        __syncthreads();
+        if (kc + 2 >= nk) break;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gm = m0 + wm * 64 + a * 16 + MF<float>::row(v, lane);
+                const int gn = n0 + wn * 64 + b * 16 + (lane & 15);
+                if (gm < M) C[(size_t)gm * ldc + gn] = acc[a][b][v];
+            }
+}
+
 template <class T>
 static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, int& ldy, DevBuf& Yb)
 {
@@ -437,8 +530,12 @@ static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, int& ldy, DevB
     hipLaunchKernelGGL(proj_build_Z<T>, dim3((unsigned)((ze + 255) / 256)), dim3(256), 0, c->stream,
                        K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb, Kp, ldz, Zb.as<T>());
     dim3 grid((unsigned)(ldz / PT), (unsigned)((M + PT - 1) / PT));
-    hipLaunchKernelGGL(proj_gemm<T>, grid, dim3(256), 0, c->stream, c->d_W.as<T>(), Zb.as<T>(), M, Kp, Nc, ldz,
-                       Yb.as<T>(), ldy);
+    if (std::is_same<T, float>::value)
+        hipLaunchKernelGGL(proj_gemm_f32, grid, dim3(256), 0, c->stream, (const float*)c->d_W.as<T>(),
+                           (const float*)Zb.as<T>(), (int)M, Kp, ldz, (float*)Yb.as<T>(), ldy);
+    else
+        hipLaunchKernelGGL(proj_gemm<T>, grid, dim3(256), 0, c->stream, c->d_W.as<T>(), Zb.as<T>(), M, Kp, Nc, ldz,
+                           Yb.as<T>(), ldy);
     MCMB_HIP(hipGetLastError());
     return MCMB_OK;
 }

@@ -178,80 +178,108 @@ __global__ __launch_bounds__(256, 2) void proj_gemm(const T* __restrict__ A, con
             }
 }
 
+// sin and cos of omega_b k_j for the shapes of a batch: scb[b][j] = (sin, cos).
+// Stage 2 and the adjoint form sin / cos of A_ij = (omega k_i + phi) +
+// omega k_j by the angle-addition formulas from these and one sincos per
+// workgroup.
+__global__ void proj_sincos_tab(const double* __restrict__ kn, const double* __restrict__ theta, int N, int nb,
+                                double* __restrict__ scb)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nb * N) return;
+    const int b = e / N, j = e % N;
+    double sb, cb;
+    sincos(theta[2 * b] * kn[j], &sb, &cb);
+    scb[2 * e] = sb;
+    scb[2 * e + 1] = cb;
+}
+
 // ---- forward stage 2: per (shape b, i) -------------------------------------
 // H[b][x][i][s][t] = sum_j q_s(j) G_x(i, j, t), x = value, d/domega, d/dphi.
 template <class T>
 __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int ldy, const double* __restrict__ kn,
                                                    const double* __restrict__ q, int ldq, int N, int R,
-                                                   const double* __restrict__ theta, int nb, double* __restrict__ H)
+                                                   const double* __restrict__ theta, int nb,
+                                                   const double* __restrict__ scb, double* __restrict__ H)
 {
-    // one workgroup per (shape b, node i); H_x[s][t] = sum_j q_s(j) G_x[j][t].
-    // Thread (sb, t): s = 4 sb .. 4 sb + 3, all three x: 12 FP64 accumulators
-    // from 5 LDS loads per j.  sincos of A_ij once per j.
+    // one workgroup per (shape b, node i); H_x[s][t] = sum_j q_s(j) G_x[j][t]
+    // over chunks of 32 nodes j, on FP64 MFMA 16x16x4: A = Q^T (s x j), B =
+    // G_x (j x t).  Wave w owns the s tile w / 2 and the t tile w % 2 of all
+    // three x (3 accumulators, the A operand shared).  The thread's 4
+    // elements (j = j0 + tid / 32 + 8 u, t = tid % 32) of the next chunk are
+    // loaded (Y, q, k_j, sin / cos omega k_j) while the current chunk's
+    // products run.
     __shared__ double G[3][32][32];     // [x][j in chunk][t]
-    __shared__ __align__(16) double Qs[32][32];   // [j in chunk][s]
-    __shared__ double sc[32][3];        // sin A, cos A, k_i + k_j
+    __shared__ double Qs[32][32];       // [j in chunk][s]
     const int b = blockIdx.x % nb, i = blockIdx.x / nb;
-    const double om = theta[2 * b], ph = theta[2 * b + 1];
-    const int tid = threadIdx.x, t = tid & 31, s0 = (tid >> 5) * 4;
+    const int tid = threadIdx.x, t = tid & 31, jr = tid >> 5;
+    const int lane = tid & 63, w = tid >> 6, sm = 16 * (w >> 1), tn = 16 * (w & 1);
     const double ki = kn[i];
-    double acc[3][4];
+    double sa, ca;   // sin, cos of omega k_i + phi
+    sincos(theta[2 * b] * ki + theta[2 * b + 1], &sa, &ca);
+    const double* const sc = scb + 2 * (size_t)b * N;
+    d4 acc[3];
 #pragma unroll
-    for (int x = 0; x < 3; ++x)
+    for (int x = 0; x < 3; ++x) acc[x] = (d4){0.0, 0.0, 0.0, 0.0};
+    T yv[4][4];
+    double qv[4], kv[4], sv[4], cv[4];
+    // loads of chunk j0 (nodes past N repeat the last one and are zeroed in stage())
+    auto fetch = [&](int j0) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc[x][u] = 0.0;
+        for (int u = 0; u < 4; ++u) {
+            const int j = min(j0 + jr + 8 * u, N - 1), tt = min(t, R - 1);
+            const T* y = Y + (size_t)pair_row(i, j, N) * ldy + (size_t)b * 4 * R + tt;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) yv[u][c] = y[c * R];
+            qv[u] = q[(size_t)j * ldq + tt];
+            kv[u] = kn[j];
+            sv[u] = sc[2 * j];
+            cv[u] = sc[2 * j + 1];
+        }
+    };
+    auto stage = [&](int j0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int jj = jr + 8 * u;
+            const bool in = j0 + jj < N && t < R;
+            const double sA = sa * cv[u] + ca * sv[u], cA = ca * cv[u] - sa * sv[u];
+            const double yc = (double)yv[u][0], ys = (double)yv[u][1];
+            const double ycp = (double)yv[u][2], ysp = (double)yv[u][3];
+            const double g0 = sA * yc + cA * ys;
+            const double g2 = cA * yc - sA * ys;
+            const double g1 = (ki + kv[u]) * g2 + cA * ycp - sA * ysp;
+            G[0][jj][t] = in ? g0 : 0.0;
+            G[1][jj][t] = in ? g1 : 0.0;
+            G[2][jj][t] = in ? g2 : 0.0;
+            Qs[jj][t] = in ? qv[u] : 0.0;
+        }
+    };
+    fetch(0);
     for (int j0 = 0; j0 < N; j0 += 32) {
+        __syncthreads();   // the previous chunk's products are done with G and Qs
+        stage(j0);
         __syncthreads();
-        if (tid < 32) {
-            const int j = j0 + tid;
-            double sA = 0.0, cA = 0.0, kij = 0.0;
-            if (j < N) {
-                kij = ki + kn[j];
-                sincos(om * kij + ph, &sA, &cA);
-            }
-            sc[tid][0] = sA;
-            sc[tid][1] = cA;
-            sc[tid][2] = kij;
-        }
-        __syncthreads();
-        for (int e = tid; e < 32 * 32; e += 256) {
-            const int jj = e >> 5, tt = e & 31, j = j0 + jj;
-            double g0 = 0, g1 = 0, g2 = 0, qq = 0;
-            if (j < N && tt < R) {
-                const T* y = Y + (size_t)pair_row(i, j, N) * ldy + (size_t)b * 4 * R;
-                const double yc = (double)y[tt], ys = (double)y[R + tt];
-                const double ycp = (double)y[2 * R + tt], ysp = (double)y[3 * R + tt];
-                const double sA = sc[jj][0], cA = sc[jj][1];
-                g0 = sA * yc + cA * ys;
-                g2 = cA * yc - sA * ys;
-                g1 = sc[jj][2] * g2 + cA * ycp - sA * ysp;
-                qq = q[(size_t)j * ldq + tt];
-            }
-            G[0][jj][tt] = g0;
-            G[1][jj][tt] = g1;
-            G[2][jj][tt] = g2;
-            Qs[jj][tt] = qq;
-        }
-        __syncthreads();
-        const int jn = min(32, N - j0);
-        for (int jj = 0; jj < jn; ++jj) {
-            const double2 qa = *reinterpret_cast<const double2*>(&Qs[jj][s0]);
-            const double2 qb = *reinterpret_cast<const double2*>(&Qs[jj][s0 + 2]);
-            const double qv[4] = {qa.x, qa.y, qb.x, qb.y};
+        fetch(min(j0 + 32, N - 1));
+        // 8 K-steps of 4 nodes (those past N have G = Q = 0)
 #pragma unroll
-            for (int x = 0; x < 3; ++x) {
-                const double g = G[x][jj][t];
+        for (int ks = 0; ks < 8; ++ks) {
+            const int jj = 4 * ks + (lane >> 4);
+            const double a = Qs[jj][sm + (lane & 15)];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) acc[x][u] += qv[u] * g;
-            }
+            for (int x = 0; x < 3; ++x)
+                acc[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, G[x][jj][tn + (lane & 15)], acc[x], 0, 0, 0);
         }
     }
-    if (t < R)
+    // element v of a lane: s = sm + 4 v + lane / 16, t = tn + lane % 16
+    const int tt = tn + (lane & 15);
+    if (tt < R)
 #pragma unroll
         for (int x = 0; x < 3; ++x)
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (s0 + u < R) H[(((size_t)b * 3 + x) * N + i) * R * R + (s0 + u) * R + t] = acc[x][u];
+            for (int v = 0; v < 4; ++v) {
+                const int ss = sm + 4 * v + (lane >> 4);
+                if (ss < R) H[(((size_t)b * 3 + x) * N + i) * R * R + ss * R + tt] = acc[x][v];
+            }
 }
 
 // alpha_x[b][n] = sum_i q_{r_n}(i) H[b][x][i][s_n][t_n], written in the
@@ -295,42 +323,63 @@ template <class T>
 __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int ldy, const double* __restrict__ kn,
                                                     const double* __restrict__ q, int ldq, int N, int R,
                                                     const double* __restrict__ theta, int nb,
+                                                    const double* __restrict__ scb,
                                                     const double* __restrict__ Hb, double* __restrict__ dpart)
 {
     // one workgroup per (shape b, node i): Gbar[j][t] = sum_s q_s(j) Hbar[s][t],
     // then d/domega += Gbar G1, d/dphi += Gbar G2 over (j, t).  Thread (jb, t):
-    // j = 4 jb .. 4 jb + 3 of the 32-node chunk, register tile over s.
+    // j = 4 jb .. 4 jb + 3 of the 32-node chunk, register tile over s.  The
+    // thread's Y, k_j and sin / cos omega k_j of the next chunk are loaded
+    // while the current chunk's sums run.
     __shared__ double Hs[32][32];                  // [s][t]
     __shared__ __align__(16) double Qs[32][32];    // [s][j in chunk]
-    __shared__ double sc[32][3];
     __shared__ double red[2][256];
     const int b = blockIdx.x % nb, i = blockIdx.x / nb;
-    const double om = theta[2 * b], ph = theta[2 * b + 1];
     const int tid = threadIdx.x, t = tid & 31, j4 = (tid >> 5) * 4;
     const double ki = kn[i];
+    double sa, ca;   // sin, cos of omega k_i + phi
+    sincos(theta[2 * b] * ki + theta[2 * b + 1], &sa, &ca);
+    const double* const sc = scb + 2 * (size_t)b * N;
     for (int e = tid; e < 32 * 32; e += 256) {
         const int ss = e >> 5, tt = e & 31;
         Hs[ss][tt] = (ss < R && tt < R) ? Hb[(((size_t)b * N) + i) * R * R + ss * R + tt] : 0.0;
     }
-    double dom = 0.0, dph = 0.0;
-    for (int j0 = 0; j0 < N; j0 += 32) {
-        __syncthreads();
-        if (tid < 32) {
-            const int j = j0 + tid;
-            double sA = 0.0, cA = 0.0, kij = 0.0;
-            if (j < N) {
-                kij = ki + kn[j];
-                sincos(om * kij + ph, &sA, &cA);
-            }
-            sc[tid][0] = sA;
-            sc[tid][1] = cA;
-            sc[tid][2] = kij;
+    T yv[4][4];
+    double kv[4], sv[4], cv[4];
+    // the thread's nodes j0 + j4 + u of a chunk (past N: the last node, unused)
+    auto fetch = [&](int j0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int j = min(j0 + j4 + u, N - 1), tt = min(t, R - 1);
+            const T* y = Y + (size_t)pair_row(i, j, N) * ldy + (size_t)b * 4 * R + tt;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) yv[u][c] = y[c * R];
+            kv[u] = kn[j];
+            sv[u] = sc[2 * j];
+            cv[u] = sc[2 * j + 1];
         }
+    };
+    double dom = 0.0, dph = 0.0;
+    fetch(0);
+    for (int j0 = 0; j0 < N; j0 += 32) {
+        __syncthreads();   // Hs written / the previous chunk is done with Qs
         for (int e = tid; e < 32 * 32; e += 256) {
             const int ss = e >> 5, jj = e & 31, j = j0 + jj;
             Qs[ss][jj] = (ss < R && j < N) ? q[(size_t)j * ldq + ss] : 0.0;
         }
         __syncthreads();
+        // this chunk's node data, then the next chunk's loads
+        T yc[4][4];
+        double kc[4], scs[4], ccs[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) yc[u][c] = yv[u][c];
+            kc[u] = kv[u];
+            scs[u] = sv[u];
+            ccs[u] = cv[u];
+        }
+        fetch(min(j0 + 32, N - 1));
         double gb[4] = {0.0, 0.0, 0.0, 0.0};
         for (int ss = 0; ss < R; ++ss) {
             const double h = Hs[ss][t];
@@ -344,14 +393,12 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
         if (t < R)
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                const int jj = j4 + u, j = j0 + jj;
-                if (j < N) {
-                    const T* y = Y + (size_t)pair_row(i, j, N) * ldy + (size_t)b * 4 * R;
-                    const double yc = (double)y[t], ys = (double)y[R + t];
-                    const double ycp = (double)y[2 * R + t], ysp = (double)y[3 * R + t];
-                    const double sA = sc[jj][0], cA = sc[jj][1];
-                    const double g2 = cA * yc - sA * ys;
-                    const double g1 = sc[jj][2] * g2 + cA * ycp - sA * ysp;
+                if (j0 + j4 + u < N) {
+                    const double sA = sa * ccs[u] + ca * scs[u], cA = ca * ccs[u] - sa * scs[u];
+                    const double yr = (double)yc[u][0], ys = (double)yc[u][1];
+                    const double ycp = (double)yc[u][2], ysp = (double)yc[u][3];
+                    const double g2 = cA * yr - sA * ys;
+                    const double g1 = (ki + kc[u]) * g2 + cA * ycp - sA * ysp;
                     dom += gb[u] * g1;
                     dph += gb[u] * g2;
                 }
@@ -540,6 +587,17 @@ static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, int& ldy, DevB
     return MCMB_OK;
 }
 
+// sin / cos of omega_b k_j for the nb shapes of a batch into c->psc
+static int sincos_table(mcmb_ctx* c, const double* d_theta, int nb)
+{
+    const int N = c->ks.lmax + 1;
+    MCMB_TRY(c->psc.alloc(sizeof(double) * 2 * (size_t)PBATCH * N));
+    hipLaunchKernelGGL(proj_sincos_tab, dim3((unsigned)((nb * N + 255) / 256)), dim3(256), 0, c->stream,
+                       c->ks.d_kn.as<double>(), d_theta, N, nb, c->psc.as<double>());
+    MCMB_HIP(hipGetLastError());
+    return MCMB_OK;
+}
+
 // Y is stage 1's product buffer (the context's batch buffer, or a slot of
 // the forward cache).
 template <class T>
@@ -552,8 +610,10 @@ static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double*
     int ldy = 0;
     MCMB_TRY(run_stage1<T>(c, d_theta, nb, ldy, Yb));
     MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)PBATCH * 3 * N * R * R));
+    MCMB_TRY(sincos_table(c, d_theta, nb));
     hipLaunchKernelGGL(proj_stage2<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
-                       K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb, Hb.as<double>());
+                       K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb,
+                       (const double*)c->psc.as<double>(), Hb.as<double>());
     const int ne = nb * 3 * n;
     hipLaunchKernelGGL(proj_stage3, dim3((ne + 255) / 256), dim3(256), 0, c->stream, Hb.as<double>(),
                        K.d_q.as<double>(), K.ldq, N, R, K.d_modes.as<int>(), n, nb, d_alpha, d_dalpha);
@@ -583,9 +643,10 @@ static int project_bwd_batch(mcmb_ctx* c, const double* d_theta, int nb, const d
     size_t he = (size_t)nb * N * R * R;
     hipLaunchKernelGGL(proj_hbar, dim3((unsigned)((he + 255) / 256)), dim3(256), 0, c->stream, d_gbar, d_slot,
                        K.d_q.as<double>(), K.ldq, N, R, n, nb, Hb.as<double>());
+    MCMB_TRY(sincos_table(c, d_theta, nb));
     hipLaunchKernelGGL(proj_adjoint<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
-                       K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb, Hb.as<double>(),
-                       Dp.as<double>());
+                       K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb,
+                       (const double*)c->psc.as<double>(), Hb.as<double>(), Dp.as<double>());
     hipLaunchKernelGGL(proj_sum_parts, dim3((nb * 2 + 255) / 256), dim3(256), 0, c->stream, Dp.as<double>(), N, nb,
                        d_dtheta);
     MCMB_HIP(hipGetLastError());

@@ -326,16 +326,19 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
                                                     const double* __restrict__ scb,
                                                     const double* __restrict__ Hb, double* __restrict__ dpart)
 {
-    // one workgroup per (shape b, node i): Gbar[j][t] = sum_s q_s(j) Hbar[s][t],
-    // then d/domega += Gbar G1, d/dphi += Gbar G2 over (j, t).  Thread (jb, t):
-    // j = 4 jb .. 4 jb + 3 of the 32-node chunk, register tile over s.  The
-    // thread's Y, k_j and sin / cos omega k_j of the next chunk are loaded
-    // while the current chunk's sums run.
-    __shared__ double Hs[32][32];                  // [s][t]
-    __shared__ __align__(16) double Qs[32][32];    // [s][j in chunk]
+    // one workgroup per (shape b, node i), over chunks of 32 nodes j:
+    // Gbar[j][t] = sum_s q_s(j) Hbar[s][t] on FP64 MFMA 16x16x4 (A = Q, j x s;
+    // B = Hbar, s x t; wave w owns the j tile w / 2 and the t tile w % 2),
+    // then each lane adds Gbar G1 and Gbar G2 at its 4 output elements.  The
+    // Y, k_j and sin / cos omega k_j of those elements in the next chunk are
+    // loaded while the current chunk's products run.
+    __shared__ double Hs[32][32];   // [s][t]
+    __shared__ double Qs[32][32];   // [s][j in chunk]
     __shared__ double red[2][256];
     const int b = blockIdx.x % nb, i = blockIdx.x / nb;
-    const int tid = threadIdx.x, t = tid & 31, j4 = (tid >> 5) * 4;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int jm = 16 * (w >> 1), tn = 16 * (w & 1);
+    const int t = tn + (lane & 15);   // the lane's output column; rows jm + 4 v + lane / 16
     const double ki = kn[i];
     double sa, ca;   // sin, cos of omega k_i + phi
     sincos(theta[2 * b] * ki + theta[2 * b + 1], &sa, &ca);
@@ -346,17 +349,18 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
     }
     T yv[4][4];
     double kv[4], sv[4], cv[4];
-    // the thread's nodes j0 + j4 + u of a chunk (past N: the last node, unused)
+    // the lane's elements (j0 + jm + 4 v + lane / 16, t) of a chunk (nodes
+    // past N: the last node; columns past R: the last column; both unused)
     auto fetch = [&](int j0) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int j = min(j0 + j4 + u, N - 1), tt = min(t, R - 1);
+        for (int v = 0; v < 4; ++v) {
+            const int j = min(j0 + jm + 4 * v + (lane >> 4), N - 1), tt = min(t, R - 1);
             const T* y = Y + (size_t)pair_row(i, j, N) * ldy + (size_t)b * 4 * R + tt;
 #pragma unroll
-            for (int c = 0; c < 4; ++c) yv[u][c] = y[c * R];
-            kv[u] = kn[j];
-            sv[u] = sc[2 * j];
-            cv[u] = sc[2 * j + 1];
+            for (int c = 0; c < 4; ++c) yv[v][c] = y[c * R];
+            kv[v] = kn[j];
+            sv[v] = sc[2 * j];
+            cv[v] = sc[2 * j + 1];
         }
     };
     double dom = 0.0, dph = 0.0;
@@ -368,47 +372,43 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
             Qs[ss][jj] = (ss < R && j < N) ? q[(size_t)j * ldq + ss] : 0.0;
         }
         __syncthreads();
-        // this chunk's node data, then the next chunk's loads
+        // this chunk's element data, then the next chunk's loads
         T yc[4][4];
         double kc[4], scs[4], ccs[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int v = 0; v < 4; ++v) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c) yc[u][c] = yv[u][c];
-            kc[u] = kv[u];
-            scs[u] = sv[u];
-            ccs[u] = cv[u];
+            for (int c = 0; c < 4; ++c) yc[v][c] = yv[v][c];
+            kc[v] = kv[v];
+            scs[v] = sv[v];
+            ccs[v] = cv[v];
         }
         fetch(min(j0 + 32, N - 1));
-        double gb[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int ss = 0; ss < R; ++ss) {
-            const double h = Hs[ss][t];
-            const double2 qa = *reinterpret_cast<const double2*>(&Qs[ss][j4]);
-            const double2 qb = *reinterpret_cast<const double2*>(&Qs[ss][j4 + 2]);
-            gb[0] += qa.x * h;
-            gb[1] += qa.y * h;
-            gb[2] += qb.x * h;
-            gb[3] += qb.y * h;
+        d4 gb = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+            const int ss = 4 * ks + (lane >> 4);
+            gb = __builtin_amdgcn_mfma_f64_16x16x4f64(Qs[ss][jm + (lane & 15)], Hs[ss][t], gb, 0, 0, 0);
         }
         if (t < R)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                if (j0 + j4 + u < N) {
-                    const double sA = sa * ccs[u] + ca * scs[u], cA = ca * ccs[u] - sa * scs[u];
-                    const double yr = (double)yc[u][0], ys = (double)yc[u][1];
-                    const double ycp = (double)yc[u][2], ysp = (double)yc[u][3];
+            for (int v = 0; v < 4; ++v) {
+                if (j0 + jm + 4 * v + (lane >> 4) < N) {
+                    const double sA = sa * ccs[v] + ca * scs[v], cA = ca * ccs[v] - sa * scs[v];
+                    const double yr = (double)yc[v][0], ys = (double)yc[v][1];
+                    const double ycp = (double)yc[v][2], ysp = (double)yc[v][3];
                     const double g2 = cA * yr - sA * ys;
-                    const double g1 = (ki + kc[u]) * g2 + cA * ycp - sA * ysp;
-                    dom += gb[u] * g1;
-                    dph += gb[u] * g2;
+                    const double g1 = (ki + kc[v]) * g2 + cA * ycp - sA * ysp;
+                    dom += gb[v] * g1;
+                    dph += gb[v] * g2;
                 }
             }
     }
     red[0][tid] = dom;
     red[1][tid] = dph;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w) { red[0][tid] += red[0][tid + w]; red[1][tid] += red[1][tid + w]; }
+    for (int wd = 128; wd > 0; wd >>= 1) {
+        if (tid < wd) { red[0][tid] += red[0][tid + wd]; red[1][tid] += red[1][tid + wd]; }
         __syncthreads();
     }
     if (tid == 0) {

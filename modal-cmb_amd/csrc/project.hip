@@ -301,21 +301,42 @@ __global__ void proj_stage3(const double* __restrict__ H, const double* __restri
 
 // ---- adjoint ---------------------------------------------------------------
 // Hbar[b][i][s][t] = sum over canonical triples (r, s, t) = order[m] of
-// gbar[b][m] q_r(i)   (one thread per (b, i, s, t))
-__global__ void proj_hbar(const double* __restrict__ gbar, const int* __restrict__ slot, const double* __restrict__ q,
-                          int ldq, int N, int R, int n, int nb, double* __restrict__ Hb)
+// gbar[b][m] q_r(i): per shape a product Q (N x R) . Gb (R x R^2), Gb[r][st]
+// = gbar[b][slot[r][st]] (0 where slot is -1), on FP64 MFMA 16x16x4.  One
+// wave per (shape b, block of 16 columns st): its B operand (Gb, R padded to
+// 32) is gathered once into registers, then the N / 16 row tiles of Q run
+// through 8 K-steps each.
+__global__ __launch_bounds__(64) void proj_hbar(const double* __restrict__ gbar, const int* __restrict__ slot,
+                                                const double* __restrict__ q, int ldq, int N, int R, int n, int nb,
+                                                double* __restrict__ Hb)
 {
-    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)nb * N * R * R) return;
-    int st = (int)(e % (R * R));
-    int i = (int)((e / (R * R)) % N);
-    int b = (int)(e / ((size_t)R * R * N));
-    double a = 0.0;
-    for (int r = 0; r < R; ++r) {
-        int m = slot[r * R * R + st];    // mode index of the canonical (r, s, t), or -1
-        if (m >= 0) a += gbar[(size_t)b * n + m] * q[(size_t)i * ldq + r];
+    const int lane = threadIdx.x;
+    const int RR = R * R, nblk = (RR + 15) / 16;
+    const int b = blockIdx.x / nblk, st = 16 * (blockIdx.x % nblk) + (lane & 15);
+    double bv[8];
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+        const int r = 4 * ks + (lane >> 4);
+        const int m = (r < R && st < RR) ? slot[r * RR + st] : -1;
+        bv[ks] = m >= 0 ? gbar[(size_t)b * n + m] : 0.0;
     }
-    Hb[e] = a;
+    for (int i0 = 0; i0 < N; i0 += 16) {
+        const int ia = min(i0 + (lane & 15), N - 1);   // A row (rows past N: the last, not stored)
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+            const int r = 4 * ks + (lane >> 4);
+            const double a = r < R ? q[(size_t)ia * ldq + r] : 0.0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[ks], acc, 0, 0, 0);
+        }
+        // element v: row i0 + 4 v + lane / 16, column st
+        if (st < RR)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + 4 * v + (lane >> 4);
+                if (i < N) Hb[((size_t)b * N + i) * RR + st] = acc[v];
+            }
+    }
 }
 
 // dtheta[b] += sum_{j,t} Gbar(i,j,t) (G1, G2)(i,j,t) for one (b, i) per workgroup
@@ -640,9 +661,9 @@ static int project_bwd_batch(mcmb_ctx* c, const double* d_theta, int nb, const d
     if (!Ycached) MCMB_TRY(run_stage1<T>(c, d_theta, nb, ldy, Yb));
     MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)PBATCH * 3 * N * R * R));
     MCMB_TRY(Dp.alloc(sizeof(double) * (size_t)PBATCH * N * 2));
-    size_t he = (size_t)nb * N * R * R;
-    hipLaunchKernelGGL(proj_hbar, dim3((unsigned)((he + 255) / 256)), dim3(256), 0, c->stream, d_gbar, d_slot,
-                       K.d_q.as<double>(), K.ldq, N, R, n, nb, Hb.as<double>());
+    const unsigned hblocks = (unsigned)(nb * ((R * R + 15) / 16));
+    hipLaunchKernelGGL(proj_hbar, dim3(hblocks), dim3(64), 0, c->stream, d_gbar, d_slot, K.d_q.as<double>(), K.ldq,
+                       N, R, n, nb, Hb.as<double>());
     MCMB_TRY(sincos_table(c, d_theta, nb));
     hipLaunchKernelGGL(proj_adjoint<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
                        K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb,

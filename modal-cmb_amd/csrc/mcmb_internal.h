@@ -139,7 +139,7 @@ struct mcmb_ctx {
     mcmb::PointSet ks;   // primordial k-space Gauss-Legendre tetrapyd
     mcmb::PointSet ku;   // primordial k-space uniform-grid tetrapyd (prim walk)
     mcmb::DevBuf d_W;    // full-cube projection weight W[i][j][k] (k-space)
-    mcmb::DevBuf pz, py, ph, pd, psc;   // projection batch buffers (Z, Y, H, partials, sin / cos table)
+    mcmb::DevBuf pz, py, ph, pd, psc, pdm;   // projection batch buffers (Z, Y, H, partials, sin / cos, D)
     // stage-1 products Y = W Z of the last forward projection, per batch of
     // shapes, reused by a backward call on the same shapes
     std::vector<mcmb::DevBuf> ycache;

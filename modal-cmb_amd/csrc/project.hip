@@ -282,19 +282,51 @@ __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int 
             }
 }
 
-// alpha_x[b][n] = sum_i q_{r_n}(i) H[b][x][i][s_n][t_n], written in the
-// caller's layout: alpha[b][n] (x = 0) and dalpha[b][x-1][n] (x = 1, 2)
-__global__ void proj_stage3(const double* __restrict__ H, const double* __restrict__ q, int ldq, int N, int R,
-                            const int* __restrict__ modes, int n, int nb, double* __restrict__ alpha,
-                            double* __restrict__ dalpha)
+// D[b][x][r][st] = sum_i q_r(i) H[b][x][i][st] (st = s R + t) on FP64 MFMA
+// 16x16x4: one wave per (shape b, x, block of 16 columns st), the two r
+// tiles sharing each B load (H, read once); K runs over the nodes i.
+__global__ __launch_bounds__(64) void proj_stage3(const double* __restrict__ H, const double* __restrict__ q, int ldq,
+                                                  int N, int R, double* __restrict__ D)
 {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x;
+    const int RR = R * R, nblk = (RR + 15) / 16;
+    const int bx = blockIdx.x / nblk, st = 16 * (blockIdx.x % nblk) + (lane & 15);
+    const double* const h = H + (size_t)bx * N * RR + min(st, RR - 1);
+    d4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    const int nks = (N + 3) / 4;
+#pragma unroll 5
+    for (int ks = 0; ks < nks; ++ks) {
+        const int i = 4 * ks + (lane >> 4);
+        const bool in = i < N;
+        const int ic = min(i, N - 1);
+        const double bv = in ? h[(size_t)ic * RR] : 0.0;
+        const int r0 = lane & 15, r1 = 16 + (lane & 15);
+        const double a0 = r0 < R ? q[(size_t)ic * ldq + r0] : 0.0;
+        const double a1 = r1 < R ? q[(size_t)ic * ldq + r1] : 0.0;
+        acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[1], 0, 0, 0);
+    }
+    // element v of tile rt: r = 16 rt + 4 v + lane / 16, column st
+    if (st < RR)
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 16 * rt + 4 * v + (lane >> 4);
+                if (r < R) D[((size_t)bx * R + r) * RR + st] = acc[rt][v];
+            }
+}
+
+// alpha_x[b][m] = D[b][x][r_m][s_m R + t_m], written in the caller's layout:
+// alpha[b][m] (x = 0) and dalpha[b][x-1][m] (x = 1, 2)
+__global__ void proj_stage3_modes(const double* __restrict__ D, int R, const int* __restrict__ modes, int n, int nb,
+                                  double* __restrict__ alpha, double* __restrict__ dalpha)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nb * 3 * n) return;
-    int m = e % n, x = (e / n) % 3, b = e / (3 * n);
-    int pk = modes[m], r = pk & 255, s = (pk >> 8) & 255, t = (pk >> 16) & 255;
-    const double* h = H + (((size_t)b * 3 + x) * N) * R * R + s * R + t;
-    double a = 0.0;
-    for (int i = 0; i < N; ++i) a += q[(size_t)i * ldq + r] * h[(size_t)i * R * R];
+    const int m = e % n, x = (e / n) % 3, b = e / (3 * n);
+    const int pk = modes[m], r = pk & 255, s = (pk >> 8) & 255, t = (pk >> 16) & 255;
+    const double a = D[(((size_t)b * 3 + x) * R + r) * R * R + s * R + t];
     if (x == 0) alpha[(size_t)b * n + m] = a;
     else dalpha[((size_t)b * 2 + x - 1) * n + m] = a;
 }
@@ -635,9 +667,12 @@ static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double*
     hipLaunchKernelGGL(proj_stage2<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
                        K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb,
                        (const double*)c->psc.as<double>(), Hb.as<double>());
+    MCMB_TRY(c->pdm.alloc(sizeof(double) * (size_t)PBATCH * 3 * R * R * R));
+    hipLaunchKernelGGL(proj_stage3, dim3((unsigned)(nb * 3 * ((R * R + 15) / 16))), dim3(64), 0, c->stream,
+                       (const double*)Hb.as<double>(), K.d_q.as<double>(), K.ldq, N, R, c->pdm.as<double>());
     const int ne = nb * 3 * n;
-    hipLaunchKernelGGL(proj_stage3, dim3((ne + 255) / 256), dim3(256), 0, c->stream, Hb.as<double>(),
-                       K.d_q.as<double>(), K.ldq, N, R, K.d_modes.as<int>(), n, nb, d_alpha, d_dalpha);
+    hipLaunchKernelGGL(proj_stage3_modes, dim3((ne + 255) / 256), dim3(256), 0, c->stream,
+                       (const double*)c->pdm.as<double>(), R, K.d_modes.as<int>(), n, nb, d_alpha, d_dalpha);
     MCMB_HIP(hipGetLastError());
     return MCMB_OK;
 }

@@ -39,7 +39,9 @@ def test_kspace_points_and_gram(ctx, N, R):
 THETAS = np.array([[150.0, 0.7], [10.0, 0.0], [499.0, 6.1], [37.5, 3.3]])
 
 
-@pytest.mark.parametrize("N,R", [(24, 4), (60, 7)])
+# R > 16 fills both 16-row tiles of the basis index in the MFMA contractions;
+# N = 33 and 20 leave partial node chunks (of 32) and an odd count of K chunks
+@pytest.mark.parametrize("N,R", [(24, 4), (60, 7), (33, 20), (20, 32)])
 def test_projection_fwd_vs_oracle(ctx, N, R):
     ks = W.kspace(N, R)
     ctx.kspace_set(ks["kn"], ks["kw"], ks["q"], ks["order"])
@@ -67,8 +69,9 @@ def test_projection_tangent_vs_central_differences(ctx):
     assert relmax(d[0, 1], fd_ph) < 1e-6
 
 
-def test_projection_adjoint_equals_jacobian_contraction(ctx):
-    ks = W.kspace(50, 6)
+@pytest.mark.parametrize("N,R", [(50, 6), (33, 20)])
+def test_projection_adjoint_equals_jacobian_contraction(ctx, N, R):
+    ks = W.kspace(N, R)
     ctx.kspace_set(ks["kn"], ks["kw"], ks["q"], ks["order"])
     th = W.shape_batch(40, seed=5)
     rng = np.random.default_rng(0)

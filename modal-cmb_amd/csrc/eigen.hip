@@ -572,7 +572,32 @@ static int block_jacobi_device(mcmb_ctx* c, int n, const double* d_G, int normal
     DevBuf A, W, Qg, dcnt, dnrm, dperm;
     MCMB_TRY(A.alloc(sizeof(double) * (size_t)ne * ne));
     MCMB_TRY(W.alloc(sizeof(double) * (size_t)ne * n));
-    MCMB_TRY(Qg.alloc(sizeof(double) * (size_t)P * BJ_N * BJ_N));
+    // two sets of rotations: step s writes set s % 2 (see the W stream below)
+    const size_t qset = (size_t)P * BJ_N * BJ_N;
+    MCMB_TRY(Qg.alloc(sizeof(double) * 2 * qset));
+    // The eigenvector rows W are rotated on a second stream (default
+    // priority): W(s) needs only step s's rotations, and the next steps'
+    // solves and A updates need only A, so W(s) overlaps solve(s + 1) and
+    // A(s + 1).  Step s + 2 reuses the rotation set of step s after W(s).
+    struct Side {
+        hipStream_t st = nullptr;
+        hipEvent_t solved[2] = {nullptr, nullptr}, rotated[2] = {nullptr, nullptr};
+        ~Side()
+        {
+            if (st) hipStreamSynchronize(st);
+            for (int k = 0; k < 2; ++k) {
+                if (solved[k]) hipEventDestroy(solved[k]);
+                if (rotated[k]) hipEventDestroy(rotated[k]);
+            }
+            if (st) hipStreamDestroy(st);
+        }
+    } side;
+    MCMB_HIP(hipStreamCreate(&side.st));
+    for (int k = 0; k < 2; ++k) {
+        MCMB_HIP(hipEventCreateWithFlags(&side.solved[k], hipEventDisableTiming));
+        MCMB_HIP(hipEventCreateWithFlags(&side.rotated[k], hipEventDisableTiming));
+    }
+    long gstep = 0;   // steps issued so far, over all sweeps
     MCMB_TRY(dcnt.alloc(sizeof(unsigned long long)));
     MCMB_TRY(dnrm.alloc(sizeof(double) * 2));
     const long ninit = std::max<long>((long)ne * ne, (long)ne * n);
@@ -596,13 +621,19 @@ static int block_jacobi_device(mcmb_ctx* c, int n, const double* d_G, int normal
         if (nr[0] <= stop * stop * nr[1]) { converged = true; break; }
         if (sweep == 0) abs_tol = 1e-17 * sqrt(nr[1]);
         MCMB_HIP(hipMemsetAsync(dcnt.p, 0, sizeof(unsigned long long), c->stream));
-        for (int step = 0; step < nb - 1; ++step) {
+        for (int step = 0; step < nb - 1; ++step, ++gstep) {
+            const int q = (int)(gstep & 1);
+            double* const Qs = Qg.as<double>() + q * qset;
+            if (gstep >= 2) MCMB_HIP(hipStreamWaitEvent(c->stream, side.rotated[q], 0));
             hipLaunchKernelGGL(bj_solve, dim3(P), dim3(128), 0, c->stream, (const double*)A.as<double>(), ne, nb, step,
-                               abs_tol, Qg.as<double>(), dcnt.as<unsigned long long>());
+                               abs_tol, Qs, dcnt.as<unsigned long long>());
+            MCMB_HIP(hipEventRecord(side.solved[q], c->stream));
             hipLaunchKernelGGL(bj_update_A, dim3((unsigned)((long)P * (P + 1) / 2)), dim3(64), 0, c->stream,
-                               A.as<double>(), ne, nb, step, (const double*)Qg.as<double>());
-            hipLaunchKernelGGL(bj_update_W, dim3((unsigned)((long)P * ncb)), dim3(256), 0, c->stream,
-                               W.as<double>(), n, nb, step, (const double*)Qg.as<double>());
+                               A.as<double>(), ne, nb, step, (const double*)Qs);
+            MCMB_HIP(hipStreamWaitEvent(side.st, side.solved[q], 0));
+            hipLaunchKernelGGL(bj_update_W, dim3((unsigned)((long)P * ncb)), dim3(256), 0, side.st, W.as<double>(), n,
+                               nb, step, (const double*)Qs);
+            MCMB_HIP(hipEventRecord(side.rotated[q], side.st));
         }
         MCMB_HIP(hipGetLastError());
         unsigned long long cnt = 0;
@@ -610,6 +641,8 @@ static int block_jacobi_device(mcmb_ctx* c, int n, const double* d_G, int normal
         MCMB_HIP(hipStreamSynchronize(c->stream));
         if (cnt == 0) { converged = true; ++sweep; break; }
     }
+    // W complete before it is read
+    if (gstep > 0) MCMB_HIP(hipStreamWaitEvent(c->stream, side.rotated[(gstep - 1) & 1], 0));
     if (!converged)
         return fail(MCMB_ERR_ARG, "block Jacobi eigen-decomposition of n=%d did not converge in 60 sweeps", n);
     std::vector<double> diag(n);

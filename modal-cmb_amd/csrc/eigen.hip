@@ -369,7 +369,7 @@ __global__ __launch_bounds__(64) void bj_update_A(double* __restrict__ A, int ne
                                                   const double* __restrict__ Qg)
 {
     static_assert(BJ_N == 32, "one wave per 32 x 32 block");
-    __shared__ double Qa[BJ_N][BJ_N + 1], Qb[BJ_N][BJ_N + 1], T[BJ_N][BJ_N + 1];
+    __shared__ double T[BJ_N][BJ_N + 1];
     const int P = nb / 2;
     int idx = blockIdx.x, a = 0;
     while (idx >= P - a) {
@@ -381,10 +381,9 @@ __global__ __launch_bounds__(64) void bj_update_A(double* __restrict__ A, int ne
     bj_pair(s, a, nb, Ia, Ja);
     bj_pair(s, b, nb, Ib, Jb);
     const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
-    for (int e = lane; e < BJ_N * BJ_N; e += 64) {
-        Qa[e / BJ_N][e % BJ_N] = Qg[(long)a * BJ_N * BJ_N + e];
-        Qb[e / BJ_N][e % BJ_N] = Qg[(long)b * BJ_N * BJ_N + e];
-    }
+    // the rotations are read as MFMA operands straight from Qg (cache-resident)
+    const double* const Qa = Qg + (long)a * BJ_N * BJ_N;
+    const double* const Qb = Qg + (long)b * BJ_N * BJ_N;
     // B operand of T: A_ab[l][j], l = 4 ks + lane/16, j = 16 c + lane%16
     double bv[BJ_N / 4][2];
 #pragma unroll
@@ -403,7 +402,7 @@ __global__ __launch_bounds__(64) void bj_update_A(double* __restrict__ A, int ne
     for (int ks = 0; ks < BJ_N / 4; ++ks)
 #pragma unroll
         for (int x = 0; x < 2; ++x) {
-            const double av = Qa[4 * ks + lk][16 * x + lr];   // (Q_a^T)[i][l] = Q_a[l][i]
+            const double av = Qa[(4 * ks + lk) * BJ_N + 16 * x + lr];   // (Q_a^T)[i][l] = Q_a[l][i]
 #pragma unroll
             for (int c = 0; c < 2; ++c) acc[x][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[ks][c], acc[x][c], 0, 0, 0);
         }
@@ -425,7 +424,8 @@ __global__ __launch_bounds__(64) void bj_update_A(double* __restrict__ A, int ne
             const double av = T[16 * x + lr][4 * ks + lk];
 #pragma unroll
             for (int c = 0; c < 2; ++c)
-                acc[x][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Qb[4 * ks + lk][16 * c + lr], acc[x][c], 0, 0, 0);
+                acc[x][c] =
+                    __builtin_amdgcn_mfma_f64_16x16x4f64(av, Qb[(4 * ks + lk) * BJ_N + 16 * c + lr], acc[x][c], 0, 0, 0);
         }
     __syncthreads();
 #pragma unroll

@@ -450,13 +450,12 @@ __global__ __launch_bounds__(256) void bj_update_W(double* __restrict__ W, int n
                                                    const double* __restrict__ Qg)
 {
     static_assert(BJ_N == 32, "2 x 4 blocks of 16 x 16 per wave");
-    __shared__ double Qk[BJ_N][BJ_N + 1];
     const int ncb = (n + 255) / 256;
     const int k = blockIdx.x / ncb, c0 = (blockIdx.x % ncb) * 256;
     int I, J;
     bj_pair(s, k, nb, I, J);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, lr = lane & 15, lk = lane >> 4;
-    for (int e = tid; e < BJ_N * BJ_N; e += 256) Qk[e / BJ_N][e % BJ_N] = Qg[(long)k * BJ_N * BJ_N + e];
+    const double* const Qk = Qg + (long)k * BJ_N * BJ_N;   // read as MFMA operands from cache
     const int cb = c0 + 64 * wave + lr;
     double bv[BJ_N / 4][4];
 #pragma unroll
@@ -465,7 +464,6 @@ __global__ __launch_bounds__(256) void bj_update_W(double* __restrict__ W, int n
 #pragma unroll
         for (int c = 0; c < 4; ++c) bv[ks][c] = cb + 16 * c < n ? W[row + cb + 16 * c] : 0.0;
     }
-    __syncthreads();
     fd4 acc[2][4];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
@@ -475,7 +473,7 @@ __global__ __launch_bounds__(256) void bj_update_W(double* __restrict__ W, int n
     for (int ks = 0; ks < BJ_N / 4; ++ks)
 #pragma unroll
         for (int x = 0; x < 2; ++x) {
-            const double av = Qk[4 * ks + lk][16 * x + lr];
+            const double av = Qk[(4 * ks + lk) * BJ_N + 16 * x + lr];
 #pragma unroll
             for (int c = 0; c < 4; ++c) acc[x][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[ks][c], acc[x][c], 0, 0, 0);
         }

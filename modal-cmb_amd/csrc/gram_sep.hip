@@ -785,9 +785,12 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
                           bool& rowptr_ready)
 {
     SepPlan& P = c->sep;
+    const char* brenv = getenv("MCMB_SEP_BLOCK_ROWS");
+    const long block_rows = brenv ? atol(brenv) : 0;
     const bool same_set = P.ps == &L && P.gen == L.generation;
     rowptr_ready = same_set && c->sg_rp.p;
-    if (same_set && P.row_lo == row_lo && P.row_hi == row_hi && P.gsz == gsz) return MCMB_OK;
+    if (same_set && P.row_lo == row_lo && P.row_hi == row_hi && P.gsz == gsz && P.block_rows == block_rows)
+        return MCMB_OK;
     P = SepPlan();
     // runs of rows with equal l1, and blocks of runs whose stage outputs fit
     // the caps (M: 16 GiB, Y: 8 GiB)
@@ -798,8 +801,11 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
         else runs.back().r1 = r + 1;
     }
     const long NPL = (long)NP * ld;
-    const long cap_rows = std::max<long>(1, (16l << 30) / ((long)ld * 8));
+    long cap_rows = std::max<long>(1, (16l << 30) / ((long)ld * 8));
     const long cap_runs = std::max<long>(1, (8l << 30) / (NPL * 8));
+    // MCMB_SEP_BLOCK_ROWS lowers the row cap (a test of the multi-block path
+    // on small problems)
+    if (const char* e = getenv("MCMB_SEP_BLOCK_ROWS")) cap_rows = std::max(1l, std::min(cap_rows, atol(e)));
     std::vector<std::pair<int, int>> blocks;
     for (int b0 = 0; b0 < (int)runs.size();) {
         int b1 = b0 + 1;
@@ -831,6 +837,7 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
     P.row_lo = row_lo;
     P.row_hi = row_hi;
     P.gsz = gsz;
+    P.block_rows = block_rows;
     return MCMB_OK;
 }
 

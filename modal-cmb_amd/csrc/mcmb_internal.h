@@ -120,6 +120,7 @@ struct SepPlan {
     long gen = -1;              // its generation
     int row_lo = -1, row_hi = -1;
     int gsz = 0;                // rows per stage-1 group
+    long block_rows = 0;        // row cap override the blocks were cut with (0: none)
     size_t max_tab = 0;         // ints per block in the table buffer
     long max_rows = 0, max_runs = 0, max_wsize = 0;
     std::vector<SepBlock> blocks;

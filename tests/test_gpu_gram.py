@@ -276,3 +276,25 @@ def test_separable_phases_fused_weights(ctx):
                      "symmetrisation", "l1 contraction (FP64 MFMA)", "assembly"]
     assert np.array_equal(G["separable"], G["separable-staged"])
     assert rel_elem(G["separable"], G["factored"]) < 1e-12
+
+
+@pytest.mark.parametrize("R", [10, 24])
+def test_separable_multi_block(ctx, R):
+    # rows cut into blocks of at most 3000 (of 21,841) by MCMB_SEP_BLOCK_ROWS:
+    # per-block weights, contractions and symmetrisation, stage 3 accumulating
+    # over blocks; against the one-block Gram and the oracle's direct sum
+    ws = W.lspace(210, R, True)
+    ctx.lspace_set(2, 210, ws["q"], ws["v"], ws["order"])
+    G1 = ctx.gamma_tet(modalcmb.FP64)
+    os.environ["MCMB_SEP_BLOCK_ROWS"] = "3000"
+    try:
+        Gb = ctx.gamma_tet(modalcmb.FP64)
+        names = [p for p, _ in ctx.last_phases()]
+    finally:
+        del os.environ["MCMB_SEP_BLOCK_ROWS"]
+    assert "l3, l2, l1 contractions (all blocks)" in names
+    assert rel_elem(Gb, G1) < 1e-13
+    assert np.array_equal(Gb, Gb.T)
+    if R == 10:
+        assert rel_elem(Gb, O.gamma_tet(2, 210, ws["q"], ws["v"], ws["order"])) < 1e-12
+

@@ -289,7 +289,7 @@ __global__ __launch_bounds__(512, 2) void sep_rowmom_mfma(const int* __restrict_
 // arithmetic is lspace_weight's, so the weights are bit-identical to the
 // staged ones.  B operands are loaded 3 K-steps ahead into 4 register sets.
 // l-space only (l3 step 2).
-template <int CPW, int ASYM>
+template <int CPW, int ASYM, int NW>
 __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ groups,
                                                         const int4* __restrict__ ginfo,
                                                         const RowDesc* __restrict__ rows,
@@ -299,7 +299,8 @@ __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ 
                                                         double* __restrict__ M)
 {
     constexpr int G = SR_GM, RT = SR_GM / 16;
-    static_assert(16 * G == 2 * 256, "two weights of a chunk per thread");
+    constexpr int WPT = 8 / NW;   // weights of a chunk per thread (64 NW threads)
+    static_assert(WPT * 64 * NW == 16 * G, "a chunk's weights split evenly over the threads");
     __shared__ double Wt[2][16][G];
     __shared__ int gj[G], gk0[G], gnk[G];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -340,12 +341,14 @@ __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ 
     };
     double bs[4][CPW];
     __syncthreads();   // row info
-    // weights of chunk c into Wt[buf]: thread tid forms elements tid and
-    // tid + 256, element e = (kk = e / 32, g = e % 32), l3 = kmin + 2 (16 c + kk);
-    // both are of row g = tid % 32, whose constants stay in registers.  The
-    // table reads of a chunk (gather) are issued a chunk's MFMAs before their
-    // use (weigh); the arithmetic is lspace_weight's, in its order.
-    const int g = tid & (G - 1), kk0 = tid / G;
+    // weights of chunk c into Wt[buf]: thread tid forms the elements
+    // tid + 64 NW x (x < 8 / NW), element e = (kk = e / 32, g = e % 32),
+    // l3 = kmin + 2 (16 c + kk);
+    // all are of row g = tid % 32, whose constants stay in registers.  The
+    // table reads of a group of two weights (gather) are issued one or more
+    // K-steps of MFMAs before their use (weigh); the arithmetic is
+    // lspace_weight's, in its order.
+    const int g = tid & (G - 1), kk0 = tid / G;   // element x: l3 index kk0 + 2 NW x of the chunk
     const int rj = gj[g], rk0 = gk0[g], rnk = gnk[g];
     const double vi = v[gi], vj = v[rj];
     const long oddij = (long)(2 * gi + 1) * (2 * rj + 1);
@@ -354,46 +357,55 @@ __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ 
         int k;
         bool in;
     };
-    auto gather = [&](int c, Tab (&t)[2]) {
+    // a thread's weights of a chunk are formed in groups of two (elements 2 gr
+    // and 2 gr + 1), one group's table reads in flight at a time
+    constexpr int NG = WPT / 2;
+    auto gather = [&](int c, int gr, Tab (&t)[2]) {
 #pragma unroll
-        for (int x = 0; x < 2; ++x) {
-            const int kg = 16 * c + kk0 + 8 * x, l3 = kmin + 2 * kg;
+        for (int y = 0; y < 2; ++y) {
+            const int x = 2 * gr + y;
+            Tab& tx = t[y];
+            const int kg = 16 * c + kk0 + 2 * NW * x, l3 = kmin + 2 * kg;
             const int d = l3 - rk0;
-            t[x].in = kg < nkk && d >= 0 && !(d & 1) && (d >> 1) < rnk;
+            tx.in = kg < nkk && d >= 0 && !(d & 1) && (d >> 1) < rnk;
             if (ASYM) {
-                t[x].k = l3;
+                tx.k = l3;
             } else {
                 // out of range: the indices of (0, 0, 0), in the tables
-                const int ii = t[x].in ? gi : 0, jj = t[x].in ? rj : 0, k3 = t[x].in ? l3 : 0;
+                const int ii = tx.in ? gi : 0, jj = tx.in ? rj : 0, k3 = tx.in ? l3 : 0;
                 const int gg = (ii + jj + k3) >> 1;
-                t[x].k = k3;
-                t[x].f1 = f[2 * (gg - ii)];
-                t[x].f2 = f[2 * (gg - jj)];
-                t[x].f3 = f[2 * (gg - k3)];
-                t[x].rg = f[2 * gg + 1];
-                t[x].vk = v[k3];
+                tx.k = k3;
+                tx.f1 = f[2 * (gg - ii)];
+                tx.f2 = f[2 * (gg - jj)];
+                tx.f3 = f[2 * (gg - k3)];
+                tx.rg = f[2 * gg + 1];
+                tx.vk = v[k3];
             }
         }
     };
-    auto weigh = [&](const Tab (&t)[2], int c, int buf) {
+    auto weigh = [&](const Tab (&t)[2], int gr, int buf) {
 #pragma unroll
-        for (int x = 0; x < 2; ++x) {
+        for (int y = 0; y < 2; ++y) {
+            const int x = 2 * gr + y;
+            const Tab& tx = t[y];
             double wv;
             if (ASYM) {
-                wv = lspace_weight(t[x].in ? gi : 0, t[x].in ? rj : 0, t[x].in ? t[x].k : 0, 1, f, v);
+                wv = lspace_weight(tx.in ? gi : 0, tx.in ? rj : 0, tx.in ? tx.k : 0, 1, f, v);
             } else {
-                const double pre = (double)(oddij * (2 * t[x].k + 1)) * 0.079577471545947667884;
-                const double h2 = pre * (t[x].f1 * t[x].f2 * t[x].f3) * t[x].rg;
-                wv = permsix_d(gi, rj, t[x].k) * h2 * vi * vj * t[x].vk;
+                const double pre = (double)(oddij * (2 * tx.k + 1)) * 0.079577471545947667884;
+                const double h2 = pre * (tx.f1 * tx.f2 * tx.f3) * tx.rg;
+                wv = permsix_d(gi, rj, tx.k) * h2 * vi * vj * tx.vk;
             }
-            Wt[buf][kk0 + 8 * x][g] = t[x].in ? wv : 0.0;
+            Wt[buf][kk0 + 2 * NW * x][g] = tx.in ? wv : 0.0;
         }
-        (void)c;
     };
     if (nks > 0) {
         Tab t0[2];
-        gather(0, t0);
-        weigh(t0, 0, 0);
+#pragma unroll
+        for (int gr = 0; gr < NG; ++gr) {
+            gather(0, gr, t0);
+            weigh(t0, gr, 0);
+        }
 #pragma unroll
         for (int x = 0; x < 3; ++x) loadb(min(x, nks - 1), bs[x]);
     }
@@ -409,14 +421,19 @@ __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ 
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) a[ks][rt] = Wt[buf][4 * ks + (lane >> 4)][16 * rt + (lane & 15)];
         Tab t[2];
-        gather(c + 1, t);
+        gather(c + 1, 0, t);
 #pragma unroll
         for (int ks = 0; ks < 4; ++ks) {
             loadb(min(4 * c + ks + 3, nks - 1), bs[(ks + 3) & 3]);
             mfmas(a[ks], bs[ks]);
-            if (ks == 2) {
+            // group gr's arithmetic after the MFMAs of step 2 (one group),
+            // steps 0 and 2 (two) or steps 0..3 (four), the next group's
+            // reads right behind it
+            const int gr = NG == 1 ? (ks == 2 ? 0 : -1) : NG == 2 ? (ks == 0 ? 0 : ks == 2 ? 1 : -1) : ks;
+            if (gr >= 0) {
                 __builtin_amdgcn_sched_barrier(0);
-                weigh(t, c + 1, buf ^ 1);
+                weigh(t, gr, buf ^ 1);
+                if (gr + 1 < NG) gather(c + 1, gr + 1, t);
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
@@ -920,10 +937,10 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     // l-space weights are formed where they are staged; other point sets stage
     // their walk-order weights
     const int wkind = (L.kspace || L.uniform) ? 2 : (L.h2mode == MCMB_H2_ASYMPTOTIC ? 1 : 0);
-    // the MFMA l3 contraction forms l-space weights itself when it runs 4
-    // waves (171 to 512 pairs); MCMB_SEP_WEIGHTS=staged stages them instead
+    // the MFMA l3 contraction forms l-space weights itself (1, 2 or 4 waves:
+    // up to 512 pairs); MCMB_SEP_WEIGHTS=staged stages them instead
     const char* wenv = getenv("MCMB_SEP_WEIGHTS");
-    const bool wfused = l3mfma && SR_GM == 32 && nwv == 4 && wkind != 2 && L.kstep == 2 &&
+    const bool wfused = l3mfma && SR_GM == 32 && nwv <= 4 && (nwv == 1 || cpw != 4) && wkind != 2 && L.kstep == 2 &&
                         !(wenv && !strcmp(wenv, "staged"));
     if (!wfused) MCMB_TRY(c->sg_wd.alloc(sizeof(double) * (size_t)std::max<long>(1, P.max_wsize)));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
@@ -989,20 +1006,28 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     hipLaunchKernelGGL((sep_rowmom_mfma<CC, SR_GM / 16>), dim3((unsigned)T.ngroups), dim3(64 * nwv), 0, c->stream,       \
                        tab + T.o_groups, ginfo, wofs, (const double*)c->sg_wd.as<double>(), T.row0, L.kstep,  \
                        (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
-#define SEP_ROWMOM_LW(CC, AA)                                                                                 \
-    hipLaunchKernelGGL((sep_rowmom_lw<CC, AA>), dim3((unsigned)T.ngroups), dim3(64 * nwv), 0, c->stream,     \
+#define SEP_ROWMOM_LW(CC, AA, NN)                                                                             \
+    hipLaunchKernelGGL((sep_rowmom_lw<CC, AA, NN>), dim3((unsigned)T.ngroups), dim3(64 * NN), 0, c->stream,   \
                        tab + T.o_groups, ginfo, (const RowDesc*)L.d_rows.as<RowDesc>(),                        \
                        (const double*)L.d_ftab.as<double>(), (const double*)L.d_v.as<double>(), T.row0,        \
                        (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
+#define SEP_LW(CC, NN)                                                                                        \
+    do {                                                                                                      \
+        if (wkind == 0) SEP_ROWMOM_LW(CC, 0, NN);                                                             \
+        else SEP_ROWMOM_LW(CC, 1, NN);                                                                        \
+    } while (0)
         if (wfused) {
-            if (wkind == 0) {
-                if (cpw == 4) SEP_ROWMOM_LW(4, 0);
-                else if (cpw == 6) SEP_ROWMOM_LW(6, 0);
-                else SEP_ROWMOM_LW(8, 0);
+            // the (column tiles, waves) pairs the width selection above produces
+            if (nwv == 1) {
+                if (cpw == 4) SEP_LW(4, 1);
+                else if (cpw == 6) SEP_LW(6, 1);
+                else SEP_LW(8, 1);
+            } else if (nwv == 2) {
+                if (cpw == 6) SEP_LW(6, 2);
+                else SEP_LW(8, 2);
             } else {
-                if (cpw == 4) SEP_ROWMOM_LW(4, 1);
-                else if (cpw == 6) SEP_ROWMOM_LW(6, 1);
-                else SEP_ROWMOM_LW(8, 1);
+                if (cpw == 6) SEP_LW(6, 4);
+                else SEP_LW(8, 4);
             }
         } else if (l3mfma) {
             if (cpw == 4) SEP_ROWMOM_MFMA(4);
@@ -1011,6 +1036,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         } else if (NP <= 256) SEP_ROWMOM(256, 1, 1, 1);
         else if (NP <= 512) SEP_ROWMOM(256, 1, 2, 1);
         else SEP_ROWMOM(256, 1, 3, 2);
+#undef SEP_LW
 #undef SEP_ROWMOM_LW
 #undef SEP_ROWMOM_MFMA
 #undef SEP_ROWMOM

@@ -243,8 +243,8 @@ def test_separable_phases_and_repeat_calls(ctx):
     G1 = ctx.gamma_tet(modalcmb.FP64)
     ph = ctx.last_phases()
     names = [p for p, _ in ph]
-    # 55 pairs: the l3 contraction runs as one wave with staged weights
-    assert names == ["pair products", "weights", "l3 contraction (row moments)", "l2 contraction (FP64 MFMA)",
+    # l-space weights are formed inside the l3 contraction (55 pairs: one wave)
+    assert names == ["pair products", "l3 contraction (row moments)", "l2 contraction (FP64 MFMA)",
                      "symmetrisation", "l1 contraction (FP64 MFMA)", "assembly"]
     total, _ = ctx.last_kernel_ms()
     assert sum(t for _, t in ph) == pytest.approx(total, rel=0.05, abs=0.05)
@@ -264,16 +264,24 @@ def test_separable_phases_and_repeat_calls(ctx):
     assert ctx.last_phases() == []
 
 
-def test_separable_phases_fused_weights(ctx):
-    # 300 pairs: the l3 contraction (4 waves) forms the l-space weights itself,
-    # bit-identical to the staged weights
-    ws = W.lspace(160, 24, True)
+@pytest.mark.parametrize("R", [20, 24])
+def test_separable_phases_fused_weights(ctx, R):
+    # 210 and 300 pairs: the l3 contraction (2 and 4 waves) forms the l-space
+    # weights itself, bit-identical to the staged weights; staged, they have
+    # a phase of their own
+    ws = W.lspace(160, R, True)
     ctx.lspace_set(2, 160, ws["q"], ws["v"], ws["order"])
     G = _gram_paths(ctx, modalcmb.FP64, ("separable", "separable-staged", "factored"))
     ctx.gamma_tet(modalcmb.FP64)
     names = [p for p, _ in ctx.last_phases()]
     assert names == ["pair products", "l3 contraction (row moments)", "l2 contraction (FP64 MFMA)",
                      "symmetrisation", "l1 contraction (FP64 MFMA)", "assembly"]
+    os.environ["MCMB_SEP_WEIGHTS"] = "staged"
+    try:
+        ctx.gamma_tet(modalcmb.FP64)
+        assert [p for p, _ in ctx.last_phases()][:2] == ["pair products", "weights"]
+    finally:
+        del os.environ["MCMB_SEP_WEIGHTS"]
     assert np.array_equal(G["separable"], G["separable-staged"])
     assert rel_elem(G["separable"], G["factored"]) < 1e-12
 

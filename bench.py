@@ -2,8 +2,8 @@
 
 Headline (value): seconds per FP64 tetrapyd Gamma at lmax=2500 with 4960 modes
 (r <= s <= t < 30), 653,383,126 tetrapyd points, by the separable contraction
-(row moments over l3, then FP64 MFMA contractions over l2 and l1 of the basis
-pair products, 36-term assembly).  One step = one full Gamma computed from the
+(row moments over l3, then FP64 MFMA contractions over l2 and the symmetrised
+l1 of the basis pair products, 6-term assembly).  One step = one full Gamma computed from the
 resident basis, per-leg weights and mode table (the exact 3j weights are
 recomputed inside the step) into a device buffer; the step is the C-ABI call
 mcmb_gamma_tet, timed on the host around the call.
@@ -26,6 +26,9 @@ Reported beside it:
                       c; lambda from the late-side orthonormalisation with the
                       PCA branch) of the separable Gram against those of the
                       factored Gram and of the GL Gram; prim-side Cholesky
+  fp64_gamma_other_configs
+                      the same FP64 Gamma at lmax 500 / 220 modes and lmax
+                      2000 / 1540 modes (median of 3 calls)
   projection          256-shape batch (omega, phi) at 300 GL nodes, 4960 modes:
                       forward with tangents and the adjoint, FP32 MFMA with
                       FP64 sums; gradients/s and the error against FP64
@@ -147,6 +150,31 @@ def cpu_projection(ks, theta):
     dt = time.time() - t
     return {"value": 1.0 / dt, "unit": "shapes/s (alpha + d alpha/d theta)", "cores": int(O.lib().or_max_threads()),
             "kind": "port", "sample": f"one shape (omega={theta[0]:.3f}, phi={theta[1]:.3f}) of the batch, {dt:.2f} s"}
+
+
+def bench_other_configs(modalcmb, W, skip):
+    """FP64 separable Gamma of the other l-space configurations, into a
+    device buffer: median of 3 calls after one warm-up."""
+    out = {}
+    for k in sorted(W.CONFIGS):
+        cfg = W.CONFIGS[k]
+        if k == skip or "L" not in cfg:
+            continue
+        ws = W.lspace(cfg["L"], cfg["R"], cfg["modulated"])
+        n = ws["order"].shape[0]
+        with modalcmb.Context(0) as ctx:
+            ctx.lspace_set(2, cfg["L"], ws["q"], ws["v"], ws["order"])
+            d = ctx.device_alloc(n * n * 8)
+            ctx.gamma_tet_device(modalcmb.FP64, d)
+            ctx.sync()
+            ts = []
+            for _ in range(3):
+                t = time.time()
+                ctx.gamma_tet_device(modalcmb.FP64, d)
+                ts.append(time.time() - t)
+            ctx.device_free(d)
+        out[f"lmax{cfg['L']}_modes{n}_s"] = round(float(np.median(ts)), 5)
+    return out
 
 
 def bench_projection(modalcmb, W, args):
@@ -422,6 +450,8 @@ def main():
                      "kernel": "sep_gemm, l2 contraction (FP64 MFMA 16x16x4, 160 x 96 tiles)",
                      "algorithmic_flops_per_launch": l2_flops, "kernel_ms": l2_ms},
     }
+    out["fp64_gamma_other_configs"] = bench_other_configs(modalcmb, W, args.config)
+    log("other configurations done")
     if not args.no_projection:
         out["projection"] = bench_projection(modalcmb, W, args)
         log("projection done")

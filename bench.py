@@ -26,7 +26,7 @@ Reported beside it:
                       c; lambda from the late-side orthonormalisation with the
                       PCA branch) of the separable Gram against those of the
                       factored Gram and of the GL Gram; prim-side Cholesky
-  fp64_gamma_other_configs
+  fp64_gamma_other_workloads
                       the same FP64 Gamma at lmax 500 / 220 modes and lmax
                       2000 / 1540 modes (median of 3 calls)
   projection          256-shape batch (omega, phi) at 300 GL nodes, 4960 modes:
@@ -39,7 +39,7 @@ Reported beside it:
                       by point count
 
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--slices 32]
-                       [--cpu-seconds 20] [--no-cpu] [--no-projection] [--config 3]
+                       [--cpu-seconds 20] [--no-cpu] [--no-projection] [--workload lmax2500_r30]
 """
 from __future__ import annotations
 
@@ -152,12 +152,12 @@ def cpu_projection(ks, theta):
             "kind": "port", "sample": f"one shape (omega={theta[0]:.3f}, phi={theta[1]:.3f}) of the batch, {dt:.2f} s"}
 
 
-def bench_other_configs(modalcmb, W, skip):
-    """FP64 separable Gamma of the other l-space configurations, into a
+def bench_other_workloads(modalcmb, W, skip):
+    """FP64 separable Gamma of the other l-space workloads, into a
     device buffer: median of 3 calls after one warm-up."""
     out = {}
-    for k in sorted(W.CONFIGS):
-        cfg = W.CONFIGS[k]
+    for k in sorted(W.WORKLOADS):
+        cfg = W.WORKLOADS[k]
         if k == skip or "L" not in cfg:
             continue
         ws = W.lspace(cfg["L"], cfg["R"], cfg["modulated"])
@@ -178,7 +178,7 @@ def bench_other_configs(modalcmb, W, skip):
 
 
 def bench_projection(modalcmb, W, args):
-    cfg = W.CONFIGS[5]
+    cfg = W.WORKLOADS["gl300_r30_batch256"]
     ks = W.kspace(cfg["N"], cfg["R"])
     theta = W.shape_batch(cfg["batch"], cfg["seed"])
     ctx = modalcmb.Context(0)
@@ -256,7 +256,7 @@ def traffic_program(config):
     del x, y
     import modalcmb
     from modalcmb import workloads as W
-    cfg = W.CONFIGS[config]
+    cfg = W.WORKLOADS[config]
     ws = W.lspace(cfg["L"], cfg["R"], cfg["modulated"])
     with modalcmb.Context(0) as ctx:
         ctx.lspace_set(2, cfg["L"], ws["q"], ws["v"], ws["order"])
@@ -284,7 +284,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=3)
+    ap.add_argument("--workload", default="lmax2500_r30",
+                    help="l-space workload of the headline Gamma (a key of modalcmb.workloads.WORKLOADS)")
     ap.add_argument("--slices", type=int, default=32,
                     help="time the direct contractions on one of this many row slices and extrapolate")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -292,10 +293,10 @@ def main():
     ap.add_argument("--no-projection", action="store_true")
     ap.add_argument("--traffic-program", action="store_true",
                     help="for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes: three 1 GiB device copies "
-                         "(calibration), then one FP64 Gamma of the config; prints one line")
+                         "(calibration), then one FP64 Gamma of the workload; prints one line")
     args = ap.parse_args()
     if args.traffic_program:
-        traffic_program(args.config)
+        traffic_program(args.workload)
         return
     if args.gpus != 1:
         print(f"bench.py: this path runs on one GPU; --gpus {args.gpus} is not supported "
@@ -305,7 +306,7 @@ def main():
     import modalcmb
     from modalcmb import workloads as W
 
-    cfg = W.CONFIGS[args.config]
+    cfg = W.WORKLOADS[args.workload]
     L, R = cfg["L"], cfg["R"]
     ws = W.lspace(L, R, cfg["modulated"])
     n = ws["order"].shape[0]
@@ -315,7 +316,7 @@ def main():
     ctx.lspace_set(2, L, ws["q"], ws["v"], ws["order"])
     npts = ctx.npoints()
     workload = f"gamma_l{L}_n{n}"
-    log(f"config {args.config}: lmax={L} modes={n} points={npts} rows={rows}")
+    log(f"workload {args.workload}: lmax={L} modes={n} points={npts} rows={rows}")
 
     # headline: FP64 separable Gamma into a device buffer
     d_gamma = ctx.device_alloc(n * n * 8)
@@ -450,8 +451,8 @@ def main():
                      "kernel": "sep_gemm, l2 contraction (FP64 MFMA 16x16x4, 160 x 96 tiles)",
                      "algorithmic_flops_per_launch": l2_flops, "kernel_ms": l2_ms},
     }
-    out["fp64_gamma_other_configs"] = bench_other_configs(modalcmb, W, args.config)
-    log("other configurations done")
+    out["fp64_gamma_other_workloads"] = bench_other_workloads(modalcmb, W, args.workload)
+    log("other workloads done")
     if not args.no_projection:
         out["projection"] = bench_projection(modalcmb, W, args)
         log("projection done")

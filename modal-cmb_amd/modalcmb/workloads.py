@@ -1,4 +1,4 @@
-"""Synthetic inputs of the five benchmark / parity configurations.
+"""Synthetic inputs of the benchmark and parity workloads.
 
 All inputs are generated deterministically (no data files):
 
@@ -19,12 +19,14 @@ import numpy as np
 
 KMIN, KMAX = 2e-4, 0.2
 
-CONFIGS = {
-    1: dict(L=500, R=10, modulated=False),
-    2: dict(L=2000, R=20, modulated=True),
-    3: dict(L=2500, R=30, modulated=True),
-    4: dict(N=200, R=20, omega=150.0, phi=0.7),
-    5: dict(N=300, R=30, batch=256, seed=20251202),
+# l-space Grams (lmax L, basis width R) and k-space projections (N GL nodes per
+# side, basis width R), by name
+WORKLOADS = {
+    "lmax500_r10": dict(L=500, R=10, modulated=False),
+    "lmax2000_r20": dict(L=2000, R=20, modulated=True),
+    "lmax2500_r30": dict(L=2500, R=30, modulated=True),
+    "gl200_r20": dict(N=200, R=20, omega=150.0, phi=0.7),
+    "gl300_r30_batch256": dict(N=300, R=30, batch=256, seed=20251202),
 }
 
 

@@ -83,14 +83,14 @@ def test_eigen_sym_matches_oracle_vectors(ctx):
     assert np.abs(V - V2).max() < 1e-9
 
 
-def test_orthonormalise_late_config1_pca(ctx):
+def test_orthonormalise_late_lmax500_pca(ctx):
     """lmax=500, 220 modes, PCA branch forced (pca_flag = 0, the reference's
     gflag_pca == 0): lambda orthonormalises Gamma on the kept directions,
     lambdainv reconstructs Gamma, and both agree with the oracle's
     restatement.  The normalised Gram has kappa ~ 1e17, so the count r of
     positive eigenvalues is itself at the FP64 noise floor: only directions
     with e > 1e-10 e_max are compared."""
-    c = W.CONFIGS[1]
+    c = W.WORKLOADS["lmax500_r10"]
     ws = W.lspace(c["L"], c["R"], c["modulated"])
     ctx.lspace_set(2, c["L"], ws["q"], ws["v"], ws["order"])
     G = ctx.gamma_gl()
@@ -111,11 +111,11 @@ def test_orthonormalise_late_config1_pca(ctx):
     assert np.all(lam[r:] == 0) and np.all(linv[:, r:] == 0)
 
 
-def test_orthonormalise_late_config1_branch(ctx):
+def test_orthonormalise_late_lmax500_branch(ctx):
     """pca_flag = 1: PCA when the smallest eigenvalue is negative, else the
     Cholesky branch; on this singular-to-FP64 Gram the Cholesky may fail (the
     reference aborts in GSL there), and then the PCA branch is taken."""
-    c = W.CONFIGS[1]
+    c = W.WORKLOADS["lmax500_r10"]
     ws = W.lspace(c["L"], c["R"], c["modulated"])
     ctx.lspace_set(2, c["L"], ws["q"], ws["v"], ws["order"])
     G = ctx.gamma_gl()

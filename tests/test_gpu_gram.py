@@ -73,8 +73,8 @@ def test_gamma_fp64_vs_oracle_direct(ctx, L, R, mod, h2):
     assert rel_elem(G, ref) < 1e-12
 
 
-def test_gamma_fp64_config1_vs_oracle(ctx):
-    c = W.CONFIGS[1]
+def test_gamma_fp64_lmax500_vs_oracle(ctx):
+    c = W.WORKLOADS["lmax500_r10"]
     ws = W.lspace(c["L"], c["R"], c["modulated"])
     ctx.lspace_set(2, c["L"], ws["q"], ws["v"], ws["order"])
     G = ctx.gamma_tet(modalcmb.FP64)
@@ -143,7 +143,7 @@ def test_gamma_gl_gpu_vs_oracle(ctx, L, R, mod):
 
 
 def test_gamma_gl_equals_tetrapyd_on_gpu(ctx):
-    # two independent GPU evaluations of the same sum at config 1 size
+    # two independent GPU evaluations of the same sum at lmax 500
     ws = W.lspace(500, 10, True)
     ctx.lspace_set(2, 500, ws["q"], ws["v"], ws["order"])
     assert rel_elem(ctx.gamma_tet(modalcmb.FP64), ctx.gamma_gl()) < 1e-11

@@ -54,8 +54,8 @@ def test_cholesky_solve_many_rhs(ctx):
         assert np.allclose(B[:, j], O.cholesky_solve(ortho, A[:, j]), rtol=1e-10, atol=1e-10)
 
 
-def test_config1_gram_orthonormalisation_status(ctx):
-    # The config-1 Gram (r<=s<=t<10 Legendre modes, weight h^2/C^3) has
+def test_lmax500_gram_orthonormalisation_status(ctx):
+    # The lmax-500 Gram (r<=s<=t<10 Legendre modes, weight h^2/C^3) has
     # kappa(D^-1 G D^-1) ~ 1e17 > 1/eps: whether FP64 Cholesky meets a
     # non-positive pivot depends on summation order (the oracle passes, the GPU
     # order stops near column 174).  Either outcome must be clean: a NOT_SPD

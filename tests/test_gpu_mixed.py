@@ -32,8 +32,8 @@ def test_mixed_vs_oracle_small(ctx, L, R, mod):
     assert e < 1e-5, e   # few points per element here: less averaging of the split error
 
 
-def test_mixed_config1_vs_fp64(ctx):
-    c = W.CONFIGS[1]
+def test_mixed_lmax500_vs_fp64(ctx):
+    c = W.WORKLOADS["lmax500_r10"]
     ws = W.lspace(c["L"], c["R"], c["modulated"])
     ctx.lspace_set(2, c["L"], ws["q"], ws["v"], ws["order"])
     G64 = ctx.gamma_tet(modalcmb.FP64)
@@ -45,10 +45,10 @@ def test_mixed_config1_vs_fp64(ctx):
     assert abs(d.mean()) < 2e-7
 
 
-def test_mixed_config2_vs_gl(ctx):
+def test_mixed_lmax2000_vs_gl(ctx):
     """lmax=2000, 1540 modes (334,831,501 points): the full mixed Gram against
     the FP64 Gauss-Legendre factorisation of the same Gram."""
-    c = W.CONFIGS[2]
+    c = W.WORKLOADS["lmax2000_r20"]
     ws = W.lspace(c["L"], c["R"], c["modulated"])
     ctx.lspace_set(2, c["L"], ws["q"], ws["v"], ws["order"])
     ref = ctx.gamma_gl()
@@ -56,7 +56,7 @@ def test_mixed_config2_vs_gl(ctx):
     assert np.array_equal(Gm, Gm.T)
     e = rel_elem(Gm, ref)
     frob = float(np.linalg.norm(Gm - ref) / np.linalg.norm(ref))
-    print(f"config 2 mixed vs GL: max elem {e:.3e} frob {frob:.3e}")
+    print(f"lmax 2000 mixed vs GL: max elem {e:.3e} frob {frob:.3e}")
     assert e < 1e-6, e
     assert frob < 1e-6, frob
 
@@ -67,7 +67,7 @@ def test_mixed_batched_dispatches(ctx):
     by capping the workgroups per dispatch, with short segments so that many
     dispatches run.  The result must match the single-dispatch Gram to FP64
     rounding of the (reordered) segment sums."""
-    c = W.CONFIGS[1]
+    c = W.WORKLOADS["lmax500_r10"]
     ws = W.lspace(c["L"], c["R"], c["modulated"])
     ctx.lspace_set(2, c["L"], ws["q"], ws["v"], ws["order"])
     os.environ["MCMB_MIXED_SEGCH"] = "16"

@@ -57,7 +57,7 @@ def test_projection_fwd_vs_oracle(ctx, N, R):
 
 
 def test_projection_tangent_vs_central_differences(ctx):
-    # config-4 protocol (omega=150, phi=0.7, h=1e-4) at a reduced grid
+    # the single-shape protocol (omega=150, phi=0.7, h=1e-4) at a reduced grid
     ks = W.kspace(40, 6)
     ctx.kspace_set(ks["kn"], ks["kw"], ks["q"], ks["order"])
     om, ph, h = 150.0, 0.7, 1e-4

@@ -204,6 +204,7 @@ __global__ __launch_bounds__(NT) void sep_rowmom(const int* __restrict__ groups,
 // straight into B registers, and feeds the RT row tiles.  The operands of
 // K-steps s + 1 and s + 2 are in flight while step s computes.
 constexpr int SR_GM = 32;   // rows per group of the MFMA kernel (RT = 2)
+constexpr int SR_XC = 64;   // consecutive groups per XCD turn (fused l3 kernel)
 template <int CPW, int RT>
 __global__ __launch_bounds__(512, 2) void sep_rowmom_mfma(const int* __restrict__ groups,
                                                           const int4* __restrict__ ginfo,
@@ -294,7 +295,7 @@ __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ 
                                                         const int4* __restrict__ ginfo,
                                                         const RowDesc* __restrict__ rows,
                                                         const double* __restrict__ f,
-                                                        const double* __restrict__ v, int row_base,
+                                                        const double* __restrict__ v, int row_base, int ngroups,
                                                         const double* __restrict__ U, int ldu, int NP, int ld,
                                                         double* __restrict__ M)
 {
@@ -303,17 +304,23 @@ __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ 
     static_assert(WPT * 64 * NW == 16 * G, "a chunk's weights split evenly over the threads");
     __shared__ double Wt[2][16][G];
     __shared__ int gj[G], gk0[G], gnk[G];
+    // runs of SR_XC consecutive groups (overlapping l3 windows, so shared pair
+    // products in L2) are dealt to the 8 XCDs in turn: blocks b, b + 8, ...
+    // share an XCD
+    const int slot = blockIdx.x >> 3;
+    const int gb = ((blockIdx.x & 7) + 8 * (slot / SR_XC)) * SR_XC + slot % SR_XC;
+    if (gb >= ngroups) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid < G) {
-        const int r = groups[(long)blockIdx.x * G + tid];
+        const int r = groups[(long)gb * G + tid];
         const RowDesc d = r >= 0 ? rows[r] : RowDesc{0, 0, 0, 0};
         gj[tid] = d.j;
         gk0[tid] = d.k0;
         gnk[tid] = d.nk;
     }
     // rows of a group share l1; the first is never padding
-    const int gi = rows[groups[(long)blockIdx.x * G]].i;
-    const int4 info = ginfo[blockIdx.x];
+    const int gi = rows[groups[(long)gb * G]].i;
+    const int4 info = ginfo[gb];
     const int kmin = info.x, nkk = info.y;
     const int nks = (nkk + 3) >> 2;           // K-steps
     const int nch = (nkk + 15) >> 4;          // chunks of 4 K-steps
@@ -458,7 +465,7 @@ __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ 
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) {
-            const int r = groups[(long)blockIdx.x * G + 16 * rt + 4 * vv + (lane >> 4)];
+            const int r = groups[(long)gb * G + 16 * rt + 4 * vv + (lane >> 4)];
             if (r < 0) continue;
             double* const Mr = M + (long)(r - row_base) * ld;
 #pragma unroll
@@ -1007,10 +1014,11 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
                        tab + T.o_groups, ginfo, wofs, (const double*)c->sg_wd.as<double>(), T.row0, L.kstep,  \
                        (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
 #define SEP_ROWMOM_LW(CC, AA, NN)                                                                             \
-    hipLaunchKernelGGL((sep_rowmom_lw<CC, AA, NN>), dim3((unsigned)T.ngroups), dim3(64 * NN), 0, c->stream,   \
-                       tab + T.o_groups, ginfo, (const RowDesc*)L.d_rows.as<RowDesc>(),                        \
+    hipLaunchKernelGGL((sep_rowmom_lw<CC, AA, NN>),                                                         \
+                       dim3((unsigned)(8 * SR_XC * ((T.ngroups + 8 * SR_XC - 1) / (8 * SR_XC)))), dim3(64 * NN), \
+                       0, c->stream, tab + T.o_groups, ginfo, (const RowDesc*)L.d_rows.as<RowDesc>(),          \
                        (const double*)L.d_ftab.as<double>(), (const double*)L.d_v.as<double>(), T.row0,        \
-                       (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
+                       T.ngroups, (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
 #define SEP_LW(CC, NN)                                                                                        \
     do {                                                                                                      \
         if (wkind == 0) SEP_ROWMOM_LW(CC, 0, NN);                                                             \

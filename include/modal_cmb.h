@@ -243,6 +243,12 @@ int mcmb_device_to_host(mcmb_ctx* ctx, void* host, const void* dev, size_t bytes
  * ms[] and names[] (static strings) and returns the phase count (0 if the last
  * such call recorded none).  No reference counterpart. */
 int mcmb_last_phases(mcmb_ctx* ctx, int cap, double* ms, const char** names);
+/* *rank = the basis columns the last FP64 separable Gram contracted over: r
+ * when the pair products q_x q_y had numerical rank r <= NP / 2 (NP =
+ * R(R+1)/2) and were replaced by an orthonormal basis of their span, 0 when
+ * the NP pair products themselves were contracted, -1 when the last Gram took
+ * another contraction order.  No reference counterpart. */
+int mcmb_last_basis_rank(mcmb_ctx* ctx, int* rank);
 
 /* ---- reference-named entry points (legacy shims over the bound context) --- */
 /* Bind the context the legacy symbols use (not owned). */

@@ -287,6 +287,13 @@ int mcmb_last_phases(mcmb_ctx* c, int cap, double* ms, const char** names)
     return c->nphases;
 }
 
+int mcmb_last_basis_rank(mcmb_ctx* c, int* rank)
+{
+    if (!c || !rank) return fail(MCMB_ERR_ARG, "mcmb_last_basis_rank: null argument");
+    *rank = c->last_rank;
+    return MCMB_OK;
+}
+
 int mcmb_gamma_tet(mcmb_ctx* c, int precision, double* gamma_host, double* gamma_dev)
 {
     if (!c) return fail(MCMB_ERR_ARG, "mcmb_gamma_tet: null context");

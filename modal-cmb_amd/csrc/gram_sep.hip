@@ -597,13 +597,13 @@ __global__ __launch_bounds__(256, 2) void sep_gemm(const double* __restrict__ U,
         }
 }
 
-// Ys[i][{e,f}] = Y[i][e][f] + Y[i][f][e] for e <= f ({e,f} = pair_index(e, f,
-// NP)): one workgroup per (32 x 32 block pair E <= F, run i); both blocks are
+// Ys[i][{e,f}] = scale (Y[i][e][f] + Y[i][f][e]) for e <= f ({e,f} =
+// pair_index(e, f, NP); scale 1, or 1/2 for the mean of a symmetric Y): one workgroup per (32 x 32 block pair E <= F, run i); both blocks are
 // read row by row (coalesced), the second transposed through LDS.  The
 // padding element of each Ys row (index NP(NP+1)/2 when that is odd) is
 // zeroed.
 __global__ __launch_bounds__(256) void sep_ysym(const double* __restrict__ Y, long ldyi, int ld, int NP, int nb,
-                                                double* __restrict__ Ys, long ldys)
+                                                double* __restrict__ Ys, long ldys, double scale)
 {
     __shared__ double A[32][33], Bt[32][33];
     int bE = 0, rem = blockIdx.x;   // block pair (bE <= bF) in row-major upper-triangle order
@@ -627,16 +627,17 @@ __global__ __launch_bounds__(256) void sep_ysym(const double* __restrict__ Y, lo
     for (int r4 = 0; r4 < 4; ++r4) {
         const int r = (tid >> 5) + 8 * r4;
         const int e = e0 + r, f = f0 + c;
-        if (e < NP && f < NP && e <= f) Yo[pair_index(e, f, NP)] = A[r][c] + Bt[c][r];
+        if (e < NP && f < NP && e <= f) Yo[pair_index(e, f, NP)] = scale * (A[r][c] + Bt[c][r]);
     }
     const long npp = (long)NP * (NP + 1) / 2;
     if (blockIdx.x == 0 && tid == 0 && ldys > npp) Yo[npp] = 0.0;
 }
 
 // Gs[off[{x,y}] + z - y] = G2[x][{y,z}] + G2[y][{x,z}] + G2[z][{x,y}] for
-// x <= y <= z: one workgroup per pair {x, y}, threads over z.
+// x <= y <= z (terms 3), or G2[x][{y,z}] alone when G2 is already the fully
+// symmetrised tensor (terms 1): one workgroup per pair {x, y}, threads over z.
 __global__ __launch_bounds__(256) void sep_gsym(const double* __restrict__ G2, long ldg, int NP,
-                                                const long* __restrict__ off, double* __restrict__ Gs)
+                                                const long* __restrict__ off, double* __restrict__ Gs, int terms)
 {
     const int p = blockIdx.x;
     int lo = 0, hi = NP - 1;   // x = the largest row whose first pair index is <= p
@@ -650,8 +651,174 @@ __global__ __launch_bounds__(256) void sep_gsym(const double* __restrict__ G2, l
     const double* const Gy = G2 + (long)y * ldg + pair_index(x, x, NP) - x;   // [{x, z}] at z
     const long pxy = pair_index(x, y, NP);
     double* const out = Gs + off[p];
-    for (int z = y + threadIdx.x; z < NP; z += blockDim.x)
-        out[z - y] = Gx[z - y] + Gy[z] + G2[(long)z * ldg + pxy];
+    if (terms == 1)
+        for (int z = y + threadIdx.x; z < NP; z += blockDim.x) out[z - y] = Gx[z - y];
+    else
+        for (int z = y + threadIdx.x; z < NP; z += blockDim.x)
+            out[z - y] = Gx[z - y] + Gy[z] + G2[(long)z * ldg + pxy];
+}
+
+// ---- compressed pair basis ----
+//
+// The pair products U_f(l) = q_x(l) q_y(l) of a basis of R functions often
+// span far fewer than R(R+1)/2 directions in l (products of polynomials of
+// degree < R are polynomials of degree < 2R - 1).  Pivoted modified
+// Gram-Schmidt over the columns finds U = P Ct + E with P orthonormal (r
+// columns) and |E_f| below a relative tolerance; the three contractions then
+// run over the r columns of P, giving the symmetrised tensor H over P, and the
+// expansion Gs(x, y, z) = sum_{a,b,g} Ct[a][x] Ct[b][y] Ct[g][z] H[a][b][g]
+// returns to pairs.
+
+// sum over the 64 lanes of a wave (butterfly, fixed order)
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+// Uc[f][l] = q_x(l) q_y(l), f = pair (x, y): the pair products as columns
+// (leading dimension nlp; rows l >= nl zero).
+__global__ void sep_ucols(const double* __restrict__ q, int ldq, int nl, int nlp, const int2* __restrict__ pxy,
+                          int NP, double* __restrict__ Uc)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)NP * nlp) return;
+    const int f = (int)(e / nlp), l = (int)(e % nlp);
+    const int2 xy = pxy[f];
+    Uc[e] = l < nl ? q[(long)l * ldq + xy.x] * q[(long)l * ldq + xy.y] : 0.0;
+}
+
+// nrm[f] = |Uc[f]|^2, one wave per column
+__global__ __launch_bounds__(256) void sep_mgs_norms(const double* __restrict__ Uc, int nlp, int NP,
+                                                     double* __restrict__ nrm)
+{
+    const int lane = threadIdx.x & 63;
+    const int f = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (f >= NP) return;
+    const double* const u = Uc + (long)f * nlp;
+    double s = 0.0;
+    for (int l = lane; l < nlp; l += 64) s += u[l] * u[l];
+    s = wave_sum(s);
+    if (lane == 0) nrm[f] = s;
+}
+
+// Step a of pivoted modified Gram-Schmidt on the residual columns Uc (updated
+// in place).  nin[f] is the squared residual norm of column f after step a - 1
+// (-1 once f has been a pivot); the pivot p is the largest (the smallest f on
+// ties).  The factorisation stops (rank[0] = a) when the pivot's squared norm
+// is at most tol2 times the first pivot's (kept in n0[0]), and gives up
+// (rank[0] = -2) at a = amax.  Otherwise, with v = Uc[p] / |Uc[p]|: Pc[a] = v;
+// for every column that has not been a pivot Ct[a][f] = v . Uc[f], Uc[f] -=
+// Ct[a][f] v and nout[f] = |Uc[f]|^2; Ct[a][p] = |Uc[p]| and Ct[a][f] = 0 for
+// earlier pivots.  Every workgroup takes the same decisions from the same nin
+// and the pivot column is only read, so the in-place update is race-free.
+// Launched for a = 0, 1, ... in stream order; after rank[0] is set the later
+// launches return at once.  One wave per column.
+__global__ __launch_bounds__(256) void sep_mgs_step(double* __restrict__ Uc, int nlp, int NP,
+                                                    const double* __restrict__ nin, double* __restrict__ nout,
+                                                    int a, int amax, double tol2, double* __restrict__ n0,
+                                                    int* __restrict__ rank, double* __restrict__ Pc,
+                                                    double* __restrict__ Ct, int ldct)
+{
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    if (*rank != -1) return;
+    const int tid = threadIdx.x;
+    double best = -2.0;
+    int bi = NP;
+    for (int f = tid; f < NP; f += 256) {
+        const double x = nin[f];
+        if (x > best) {
+            best = x;
+            bi = f;
+        }
+    }
+    sv[tid] = best;
+    si[tid] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s) {
+            const double o = sv[tid + s];
+            const int oi = si[tid + s];
+            if (o > sv[tid] || (o == sv[tid] && oi < si[tid])) {
+                sv[tid] = o;
+                si[tid] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const double n2 = sv[0];
+    const int p = si[0];
+    const double ref = a == 0 ? n2 : n0[0];
+    if (!(n2 > tol2 * ref)) {
+        if (blockIdx.x == 0 && tid == 0) *rank = a;
+        return;
+    }
+    if (a >= amax) {
+        if (blockIdx.x == 0 && tid == 0) *rank = -2;
+        return;
+    }
+    if (a == 0 && blockIdx.x == 0 && tid == 0) n0[0] = n2;
+    const double nrm = sqrt(n2), inv = 1.0 / nrm;
+    const double* const pv = Uc + (long)p * nlp;
+    if (blockIdx.x == 0)
+        for (int l = tid; l < nlp; l += 256) Pc[(long)a * nlp + l] = pv[l] * inv;
+    const int lane = tid & 63;
+    for (int f = blockIdx.x * 4 + (tid >> 6); f < NP; f += gridDim.x * 4) {
+        if (f == p || nin[f] < 0.0) {
+            if (lane == 0) {
+                Ct[(long)a * ldct + f] = f == p ? nrm : 0.0;
+                nout[f] = -1.0;
+            }
+            continue;
+        }
+        double* const u = Uc + (long)f * nlp;
+        double d = 0.0;
+        for (int l = lane; l < nlp; l += 64) d += (pv[l] * inv) * u[l];
+        d = wave_sum(d);
+        double s = 0.0;
+        for (int l = lane; l < nlp; l += 64) {
+            const double x = u[l] - d * (pv[l] * inv);
+            u[l] = x;
+            s += x * x;
+        }
+        s = wave_sum(s);
+        if (lane == 0) {
+            Ct[(long)a * ldct + f] = d;
+            nout[f] = s;
+        }
+    }
+}
+
+// P[l][a] = Pc[a][l] for a < r, 0 for r <= a < ldu: the compressed basis in
+// the row layout of the pair products
+__global__ void sep_ptab(const double* __restrict__ Pc, int nlp, int nl, int r, int ldu, double* __restrict__ P)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)nl * ldu) return;
+    const int l = (int)(e / ldu), a = (int)(e % ldu);
+    P[e] = a < r ? Pc[(long)a * nlp + l] : 0.0;
+}
+
+// H[g][a r + b] = Hs(sorted (g, a, b)) with Hs(x, y, z) = G2[x][{y,z}] +
+// G2[y][{x,z}] + G2[z][{x,y}] (x <= y <= z, pairs over r): every entry of the
+// fully symmetrised tensor over the compressed basis, equal for all orders of
+// (g, a, b); the padding entry of each row (r^2 odd) is zeroed.
+__global__ void sep_hfull(const double* __restrict__ G2, long ldg, int r, double* __restrict__ H, long ldh)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)r * ldh) return;
+    const int g = (int)(e / ldh);
+    const long ab = e % ldh;
+    if (ab >= (long)r * r) {
+        H[e] = 0.0;
+        return;
+    }
+    const int a = (int)(ab / r), b = (int)(ab % r);
+    const int x = min(g, min(a, b)), z = max(g, max(a, b)), y = g + a + b - x - z;
+    H[e] = G2[(long)x * ldg + pair_index(y, z, r)] + G2[(long)y * ldg + pair_index(x, z, r)] +
+           G2[(long)z * ldg + pair_index(x, y, r)];
 }
 
 // Gamma[m][n] = Gamma[n][m] = (1/36) sum over the 6 matchings sigma of the
@@ -813,7 +980,8 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
     const long block_rows = brenv ? atol(brenv) : 0;
     const bool same_set = P.ps == &L && P.gen == L.generation;
     rowptr_ready = same_set && c->sg_rp.p;
-    if (same_set && P.row_lo == row_lo && P.row_hi == row_hi && P.gsz == gsz && P.block_rows == block_rows)
+    if (same_set && P.row_lo == row_lo && P.row_hi == row_hi && P.gsz == gsz && P.nb == NP &&
+        P.block_rows == block_rows)
         return MCMB_OK;
     P = SepPlan();
     // runs of rows with equal l1, and blocks of runs whose stage outputs fit
@@ -861,7 +1029,146 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
     P.row_lo = row_lo;
     P.row_hi = row_hi;
     P.gsz = gsz;
+    P.nb = NP;
     P.block_rows = block_rows;
+    return MCMB_OK;
+}
+
+// Stage-1 geometry for NB basis columns: on FP64 MFMA over 32-row groups (1,
+// 2, 4 or 8 waves of 4, 6 or 8 column tiles: NB <= 1024), else on the VALU
+// over 16-row groups; MCMB_SEP_L3=valu selects the VALU kernel.  The fewest
+// padding tiles, then the fewest waves (a workgroup of 4 or 8 waves loads the
+// 4 SIMDs of a CU evenly; 5 waves of 6 tiles at 465 pairs ran 35% slower than
+// 4 waves of 8).  ldu: the row width of the basis table the kernel reads (its
+// columns, or whole 64-column blocks for the VALU kernel; padding 0).
+struct L3Shape {
+    int cpw = 0, nwv = 0;   // MFMA column tiles per wave, waves
+    bool mfma = false;
+    int gsz = SR_G, ldu = 0;
+};
+static L3Shape sep_l3_shape(int NB)
+{
+    L3Shape s;
+    const char* l3env = getenv("MCMB_SEP_L3");
+    const int nct = (NB + 15) / 16;   // 16-column tiles
+    for (int w : {1, 2, 4, 8})
+        for (int cc : {8, 6, 4})
+            if (cc * w >= nct && (!s.cpw || cc * w < s.cpw * s.nwv)) s.cpw = cc, s.nwv = w;
+    s.mfma = s.cpw > 0 && !(l3env && !strcmp(l3env, "valu"));
+    s.gsz = s.mfma ? SR_GM : SR_G;
+    s.ldu = s.mfma ? 16 * s.cpw * s.nwv : (NB + 63) / 64 * 64;
+    return s;
+}
+
+// Relative column norm below which a pivoted Gram-Schmidt residual counts as
+// zero.  The Legendre bases leave a gap of ~13 decades there: at lmax 2500 and
+// R = 30 the 59th pivot is 1.3e-2 of the first and the 60th 4.9e-16.
+constexpr double SEP_RANK_TOL = 1e-12;
+
+// Rank-revealing factorisation U = P Ct of the pair products of L into
+// c->sepc, by pivoted modified Gram-Schmidt over the NP columns (one launch
+// per step; the rank is read back every 8 steps).  r = 0 when it needs more
+// than NP / 2 steps (too little saving to pay for the expansion) or the
+// products vanish.  Kept per point set; the expansion tables are built with
+// it.
+static int separable_compress(mcmb_ctx* c, PointSet& L, int nl, int NP, int ld, const std::vector<int2>& pxy)
+{
+    SepComp& S = c->sepc;
+    if (S.ps == &L && S.gen == L.generation && S.NP == NP) return MCMB_OK;
+    S.ps = nullptr;
+    S.r = -1;
+    S.ldu = 0;
+    const int nlp = nl + (nl & 1);
+    const int amax = NP / 2;
+    DevBuf uc, nrm, dpxy;
+    MCMB_TRY(uc.alloc(sizeof(double) * (size_t)NP * nlp));
+    MCMB_TRY(nrm.alloc(sizeof(double) * (2 * (size_t)NP + 2)));
+    MCMB_TRY(dpxy.alloc(sizeof(int2) * (size_t)NP));
+    MCMB_TRY(S.pc.alloc(sizeof(double) * (size_t)std::max(1, amax) * nlp));
+    MCMB_TRY(S.ct.alloc(sizeof(double) * (size_t)std::max(1, amax) * ld));
+    double* const dn = nrm.as<double>();
+    double* const n0 = dn + 2 * NP;
+    int* const rank = reinterpret_cast<int*>(n0 + 1);
+    MCMB_HIP(hipMemcpyAsync(dpxy.p, pxy.data(), sizeof(int2) * (size_t)NP, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemsetAsync(S.ct.p, 0, S.ct.bytes, c->stream));
+    MCMB_HIP(hipMemsetAsync(rank, 0xff, sizeof(int), c->stream));   // -1: running
+    const long e = (long)NP * nlp;
+    MCMB_TRY(launch_guard("pair columns", (unsigned long long)((e + 255) / 256), 256));
+    hipLaunchKernelGGL(sep_ucols, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, c->stream, L.d_q.as<double>(),
+                       L.ldq, nl, nlp, dpxy.as<int2>(), NP, uc.as<double>());
+    const unsigned ncb = (unsigned)((NP + 3) / 4);
+    hipLaunchKernelGGL(sep_mgs_norms, dim3(ncb), dim3(256), 0, c->stream, (const double*)uc.as<double>(), nlp, NP,
+                       dn);
+    MCMB_HIP(hipGetLastError());
+    int hr = -1;
+    for (int a0 = 0; a0 <= amax && hr == -1; a0 += 8) {
+        for (int a = a0; a < a0 + 8 && a <= amax; ++a)
+            hipLaunchKernelGGL(sep_mgs_step, dim3(ncb), dim3(256), 0, c->stream, uc.as<double>(), nlp, NP,
+                               (const double*)(dn + (a & 1) * NP), dn + ((a + 1) & 1) * NP, a, amax,
+                               SEP_RANK_TOL * SEP_RANK_TOL, n0, rank, S.pc.as<double>(), S.ct.as<double>(), ld);
+        MCMB_HIP(hipGetLastError());
+        MCMB_HIP(hipMemcpyAsync(&hr, rank, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+    }
+    const int r = hr > 0 ? hr : 0;
+    if (r > 0) {
+        // expansion: T1[(a,b)][z] = sum_g H[g][(a,b)] Ct[g][z]; per a,
+        // Y2[a][y][z] = sum_b Ct[b][y] T1[(a,b)][z]; G[x][{y,z}] = sum_a
+        // Ct[a][x] Y2s[a][{y,z}] (tile groups share a B panel, as in the
+        // contractions)
+        const long ldh = (long)r * r + ((r * r) & 1);
+        const long NPP = (long)NP * (NP + 1) / 2;
+        std::vector<SepTask> t1(1), t2(r), t3(1);
+        std::vector<int4> g1, g2, g3;
+        t1[0] = SepTask{0, 0, 0, r};
+        for (int n0c = 0; n0c < NP; n0c += SG_TN)
+            for (int m0 = 0; m0 < r * r; m0 += SG_TM) g1.push_back(make_int4(0, m0, n0c, 0));
+        for (int a = 0; a < r; ++a) {
+            t2[a] = SepTask{(long)a * r, (long)a * NP * ld, 0, r};
+            for (int n0c = 0; n0c < NP; n0c += SG_TN)
+                for (int m0 = 0; m0 < NP; m0 += SG_TM) g2.push_back(make_int4(a, m0, n0c, 0));
+        }
+        t3[0] = SepTask{0, 0, 0, r};
+        for (long n0c = 0; n0c < NPP; n0c += SG_TN)
+            for (int m0 = 0; m0 < NP; m0 += SG_TM) g3.push_back(make_int4(0, m0, (int)n0c, 0));
+        std::vector<int> buf;
+        S.o_task[0] = append(buf, t1);
+        S.o_tile[0] = append(buf, g1);
+        S.o_task[1] = append(buf, t2);
+        S.o_tile[1] = append(buf, g2);
+        S.o_task[2] = append(buf, t3);
+        S.o_tile[2] = append(buf, g3);
+        S.nt[0] = (int)g1.size();
+        S.nt[1] = (int)g2.size();
+        S.nt[2] = (int)g3.size();
+        S.gs[0] = (r * r + SG_TM - 1) / SG_TM;
+        S.gs[1] = S.gs[2] = (NP + SG_TM - 1) / SG_TM;
+        MCMB_TRY(S.tab.alloc(sizeof(int) * buf.size()));
+        MCMB_HIP(hipMemcpy(S.tab.p, buf.data(), sizeof(int) * buf.size(), hipMemcpyHostToDevice));
+        MCMB_TRY(S.h.alloc(sizeof(double) * (size_t)r * ldh));
+    }
+    S.ps = &L;
+    S.gen = L.generation;
+    S.NP = NP;
+    S.nl = nl;
+    S.nlp = nlp;
+    S.r = r;
+    return MCMB_OK;
+}
+
+// The P table [l][a] of the compressed basis with row width ldu (the stage-1
+// kernel's columns), rebuilt when the width changes.
+static int separable_ptab(mcmb_ctx* c, int ldu)
+{
+    SepComp& S = c->sepc;
+    if (S.ldu == ldu) return MCMB_OK;
+    MCMB_TRY(S.pt.alloc(sizeof(double) * (size_t)S.nl * ldu));
+    const long e = (long)S.nl * ldu;
+    MCMB_TRY(launch_guard("compressed basis table", (unsigned long long)((e + 255) / 256), 256));
+    hipLaunchKernelGGL(sep_ptab, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, c->stream,
+                       (const double*)S.pc.as<double>(), S.nlp, S.nl, S.r, ldu, S.pt.as<double>());
+    MCMB_HIP(hipGetLastError());
+    S.ldu = ldu;
     return MCMB_OK;
 }
 
@@ -870,25 +1177,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     if (!gram_separable_supported(L, precision))
         return fail(MCMB_ERR_ARG, "separable Gram: precision %d / basis width %d not supported", precision, L.R);
     const int n = L.n, R = L.R, NP = R * (R + 1) / 2;
-    const int ld = NP + (NP & 1);          // even leading dimension of M, Y and G rows: 16-byte aligned
-    // stage 1 on FP64 MFMA over 32-row groups (1, 2, 4 or 8 waves of 4, 6 or
-    // 8 column tiles: NP <= 1024), else on the VALU over 16-row groups;
-    // MCMB_SEP_L3=valu selects the VALU kernel.  The fewest padding tiles,
-    // then the fewest waves (a workgroup of 4 or 8 waves loads the 4 SIMDs
-    // of a CU evenly; 5 waves of 6 tiles at 465 pairs ran 35% slower than
-    // 4 waves of 8)
-    const char* l3env = getenv("MCMB_SEP_L3");
-    const int nct = (NP + 15) / 16;   // 16-column tiles
-    int cpw = 0, nwv = 0;             // MFMA column tiles per wave, waves
-    for (int w : {1, 2, 4, 8})
-        for (int cc : {8, 6, 4})
-            if (cc * w >= nct && (!cpw || cc * w < cpw * nwv)) cpw = cc, nwv = w;
-    const bool l3mfma = cpw > 0 && !(l3env && !strcmp(l3env, "valu"));
-    const int gsz = l3mfma ? SR_GM : SR_G;
-    // pair-product rows: the MFMA kernel's columns, or whole 64-column blocks
-    // for the VALU kernel; padding 0
-    const int ldu = l3mfma ? 16 * cpw * nwv : (NP + 63) / 64 * 64;
-    const long NPL = (long)NP * ld;        // one Y slab / one G row
+    const int ld = NP + (NP & 1);          // even leading dimension of pair rows: 16-byte aligned
     const int nl = (int)(L.q.size() / (size_t)L.ldq);
     c->nphases = 0;
     if (row_hi <= row_lo) {
@@ -898,20 +1187,55 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         c->last_launches = 0;
         return MCMB_OK;
     }
-    bool rowptr_ready = false;
-    MCMB_TRY(separable_plan(c, L, row_lo, row_hi, NP, ld, gsz, rowptr_ready));
-    const SepPlan& P = c->sep;
     std::vector<int2> pxy(NP);
     for (int x = 0; x < R; ++x)
         for (int y = x; y < R; ++y) pxy[pair_index(x, y, R)] = make_int2(x, y);
 
-    MCMB_TRY(c->sg_U.alloc(sizeof(double) * (size_t)nl * ldu + sizeof(int2) * NP));
-    MCMB_TRY(c->sg_M.alloc(sizeof(double) * (size_t)P.max_rows * ld));
+    // phases: [basis compression], pair products, [weights], per block the
+    // three contractions (times summed over blocks), [expansion], assembly
+    int np = 0;   // phases started; phase k runs from event k to event k + 1
+    auto phase = [&](const char* name) -> int {
+        c->phase_names[np] = name;
+        MCMB_HIP(hipEventRecord(c->evp[np], c->stream));
+        ++np;
+        return MCMB_OK;
+    };
+    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
+    int launches = 0;
+    // The contractions run over NB basis columns: the pairs, or the r columns
+    // of the compressed basis when the pair products have rank r <= NP / 2
+    // (MCMB_SEP_COMPRESS=0: always the pairs).
+    const char* cenv = getenv("MCMB_SEP_COMPRESS");
+    SepComp& S = c->sepc;
+    bool comp = false;
+    if (!(cenv && !strcmp(cenv, "0"))) {
+        if (!(S.ps == &L && S.gen == L.generation && S.NP == NP)) {
+            MCMB_TRY(phase("basis compression"));
+            launches += 2 + NP / 2;
+        }
+        MCMB_TRY(separable_compress(c, L, nl, NP, ld, pxy));
+        comp = S.r > 0;
+    }
+    const int NB = comp ? S.r : NP;
+    const int ldB = NB + (NB & 1);         // even leading dimension of M, Y rows
+    const L3Shape sh = sep_l3_shape(NB);
+    const int cpw = sh.cpw, nwv = sh.nwv, gsz = sh.gsz, ldu = sh.ldu;
+    const bool l3mfma = sh.mfma;
+    const long NPL = (long)NB * ldB;       // one Y slab
+
+    bool rowptr_ready = false;
+    MCMB_TRY(separable_plan(c, L, row_lo, row_hi, NB, ldB, gsz, rowptr_ready));
+    const SepPlan& P = c->sep;
+
     const long NPP = (long)NP * (NP + 1) / 2;   // unordered pairs {e, f}
     const long ldp = NPP + (NPP & 1);          // even leading dimension of Ys and G2 rows
     const long NGS = NPP * (NP + 2) / 3;       // x <= y <= z triples: NP (NP + 1) (NP + 2) / 6
-    MCMB_TRY(c->sg_Y.alloc(sizeof(double) * (size_t)P.max_runs * NPL));
-    MCMB_TRY(c->sg_Ys.alloc(sizeof(double) * (size_t)P.max_runs * ldp));
+    const long NBP = (long)NB * (NB + 1) / 2;  // unordered basis pairs
+    const long ldq2 = NBP + (NBP & 1);
+    const long ldh = (long)NB * NB + ((NB * NB) & 1);
+    MCMB_TRY(c->sg_M.alloc(sizeof(double) * (size_t)std::max(P.max_rows * ldB, comp ? ldh * ld : 0l)));
+    MCMB_TRY(c->sg_Y.alloc(sizeof(double) * (size_t)std::max(P.max_runs * NPL, comp ? (long)NB * NP * ld : 0l)));
+    MCMB_TRY(c->sg_Ys.alloc(sizeof(double) * (size_t)std::max(P.max_runs * ldq2, comp ? (long)NB * ldp : 0l)));
     MCMB_TRY(c->sg_G.alloc(sizeof(double) * (size_t)NP * ldp));
     MCMB_TRY(c->sg_Gs.alloc(sizeof(double) * (size_t)NGS));
     if (c->sg_off_np != NP) {
@@ -927,40 +1251,36 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         MCMB_HIP(hipMemcpy(c->sg_off.p, off.data(), sizeof(long) * (size_t)NPP, hipMemcpyHostToDevice));
         c->sg_off_np = NP;
     }
-    double* dU = c->sg_U.as<double>();
-    int2* d_pxy = reinterpret_cast<int2*>(dU + (size_t)nl * ldu);
-    MCMB_HIP(hipMemcpy(d_pxy, pxy.data(), sizeof(int2) * NP, hipMemcpyHostToDevice));
 
-    // phases: weights, pair products, then per block the three contractions
-    // (times summed over blocks), assembly
-    int np = 0;   // phases started; phase k runs from event k to event k + 1
-    auto phase = [&](const char* name) -> int {
-        c->phase_names[np] = name;
-        MCMB_HIP(hipEventRecord(c->evp[np], c->stream));
-        ++np;
-        return MCMB_OK;
-    };
+    const double* dU;   // basis table [l][ldu] the contractions read
+    if (comp) {
+        MCMB_TRY(separable_ptab(c, ldu));
+        dU = S.pt.as<double>();
+    } else {
+        MCMB_TRY(c->sg_U.alloc(sizeof(double) * (size_t)nl * ldu + sizeof(int2) * NP));
+        double* const dUw = c->sg_U.as<double>();
+        int2* d_pxy = reinterpret_cast<int2*>(dUw + (size_t)nl * ldu);
+        MCMB_HIP(hipMemcpy(d_pxy, pxy.data(), sizeof(int2) * NP, hipMemcpyHostToDevice));
+        MCMB_TRY(phase("pair products"));
+        const long e = (long)nl * ldu;
+        MCMB_TRY(launch_guard("pair products", (unsigned long long)((e + 255) / 256), 256));
+        hipLaunchKernelGGL(sep_utab, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, c->stream, L.d_q.as<double>(),
+                           L.ldq, nl, d_pxy, NP, ldu, dUw);
+        MCMB_HIP(hipGetLastError());
+        ++launches;
+        dU = dUw;
+    }
+
     const bool one_block = P.blocks.size() == 1;
     // l-space weights are formed where they are staged; other point sets stage
     // their walk-order weights
     const int wkind = (L.kspace || L.uniform) ? 2 : (L.h2mode == MCMB_H2_ASYMPTOTIC ? 1 : 0);
     // the MFMA l3 contraction forms l-space weights itself (1, 2 or 4 waves:
-    // up to 512 pairs); MCMB_SEP_WEIGHTS=staged stages them instead
+    // up to 512 columns); MCMB_SEP_WEIGHTS=staged stages them instead
     const char* wenv = getenv("MCMB_SEP_WEIGHTS");
     const bool wfused = l3mfma && SR_GM == 32 && nwv <= 4 && (nwv == 1 || cpw != 4) && wkind != 2 && L.kstep == 2 &&
                         !(wenv && !strcmp(wenv, "staged"));
     if (!wfused) MCMB_TRY(c->sg_wd.alloc(sizeof(double) * (size_t)std::max<long>(1, P.max_wsize)));
-    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
-    int launches = 0;
-    MCMB_TRY(phase("pair products"));
-    {
-        const long e = (long)nl * ldu;
-        MCMB_TRY(launch_guard("pair products", (unsigned long long)((e + 255) / 256), 256));
-        hipLaunchKernelGGL(sep_utab, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, c->stream, L.d_q.as<double>(),
-                           L.ldq, nl, d_pxy, NP, ldu, dU);
-        MCMB_HIP(hipGetLastError());
-        ++launches;
-    }
     // staged weights of one block's row groups
     auto wstage = [&](size_t b) -> int {
         const SepBlock& T = P.blocks[b];
@@ -996,6 +1316,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         if (one_block) MCMB_TRY(wstage(0));
     }
     if (!one_block) MCMB_TRY(phase("l3, l2, l1 contractions (all blocks)"));
+    const int GS = (NB + SG_TM - 1) / SG_TM;   // m tiles per B panel, both contractions
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const SepBlock& T = P.blocks[b];
         const int* tab = c->sg_tab.as<int>() + b * P.max_tab;
@@ -1008,17 +1329,17 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
 #define SEP_ROWMOM(NN, GG, FF, HH)                                                                            \
     hipLaunchKernelGGL((sep_rowmom<NN, GG, FF, HH>), dim3((unsigned)T.ngroups), dim3(NN), 0, c->stream,      \
                        tab + T.o_groups, ginfo, wofs, (const double*)c->sg_wd.as<double>(), T.row0, L.kstep,  \
-                       (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
+                       dU, ldu, NB, ldB, c->sg_M.as<double>())
 #define SEP_ROWMOM_MFMA(CC)                                                                                   \
     hipLaunchKernelGGL((sep_rowmom_mfma<CC, SR_GM / 16>), dim3((unsigned)T.ngroups), dim3(64 * nwv), 0, c->stream,       \
                        tab + T.o_groups, ginfo, wofs, (const double*)c->sg_wd.as<double>(), T.row0, L.kstep,  \
-                       (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
+                       dU, ldu, NB, ldB, c->sg_M.as<double>())
 #define SEP_ROWMOM_LW(CC, AA, NN)                                                                             \
     hipLaunchKernelGGL((sep_rowmom_lw<CC, AA, NN>),                                                         \
                        dim3((unsigned)(8 * SR_XC * ((T.ngroups + 8 * SR_XC - 1) / (8 * SR_XC)))), dim3(64 * NN), \
                        0, c->stream, tab + T.o_groups, ginfo, (const RowDesc*)L.d_rows.as<RowDesc>(),          \
                        (const double*)L.d_ftab.as<double>(), (const double*)L.d_v.as<double>(), T.row0,        \
-                       T.ngroups, (const double*)dU, ldu, NP, ld, c->sg_M.as<double>())
+                       T.ngroups, dU, ldu, NB, ldB, c->sg_M.as<double>())
 #define SEP_LW(CC, NN)                                                                                        \
     do {                                                                                                      \
         if (wkind == 0) SEP_ROWMOM_LW(CC, 0, NN);                                                             \
@@ -1041,8 +1362,8 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
             if (cpw == 4) SEP_ROWMOM_MFMA(4);
             else if (cpw == 6) SEP_ROWMOM_MFMA(6);
             else SEP_ROWMOM_MFMA(8);
-        } else if (NP <= 256) SEP_ROWMOM(256, 1, 1, 1);
-        else if (NP <= 512) SEP_ROWMOM(256, 1, 2, 1);
+        } else if (NB <= 256) SEP_ROWMOM(256, 1, 1, 1);
+        else if (NB <= 512) SEP_ROWMOM(256, 1, 2, 1);
         else SEP_ROWMOM(256, 1, 3, 2);
 #undef SEP_LW
 #undef SEP_ROWMOM_LW
@@ -1050,50 +1371,80 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
 #undef SEP_ROWMOM
         MCMB_HIP(hipGetLastError());
         // stages 2 and 3
-        const int GS = (NP + SG_TM - 1) / SG_TM;   // m tiles per B panel, both stages
-#define SEP_GEMM(CG, ST, NT, AIDX, BP, LDB, CP, LDC, MD, ND, TASK, TILE, ACC)                                 \
-    hipLaunchKernelGGL((sep_gemm<CG, ST>), dim3(sep_blocks(NT, GS)), dim3(256), 0, c->stream, (const double*)dU, \
-                       ldu,                                                                                     \
-                       AIDX, (const double*)(BP), (long)(LDB), CP, (long)(LDC), MD, ND,                          \
-                       reinterpret_cast<const SepTask*>(TASK), reinterpret_cast<const int4*>(TILE), NT, GS, ACC)
+#define SEP_GEMM(CG, ST, NT, GSZ, AP, LDA, AIDX, BP, LDB, CP, LDC, MD, ND, TASK, TILE, ACC)                  \
+    hipLaunchKernelGGL((sep_gemm<CG, ST>), dim3(sep_blocks(NT, GSZ)), dim3(256), 0, c->stream, (const double*)(AP), \
+                       (int)(LDA), AIDX, (const double*)(BP), (long)(LDB), CP, (long)(LDC), MD, ND,              \
+                       reinterpret_cast<const SepTask*>(TASK), reinterpret_cast<const int4*>(TILE), NT, GSZ, ACC)
         if (one_block) MCMB_TRY(phase("l2 contraction (FP64 MFMA)"));
         MCMB_TRY(launch_guard("l2 contraction", sep_blocks(T.nt2, GS), 256));
         if (T.contig2)
-            SEP_GEMM(true, 2, T.nt2, tab + T.o_aidx2, c->sg_M.as<double>(), ld, c->sg_Y.as<double>(), ld, NP, NP,
-                     tab + T.o_task2, tab + T.o_tile2, 0);
+            SEP_GEMM(true, 2, T.nt2, GS, dU, ldu, tab + T.o_aidx2, c->sg_M.as<double>(), ldB, c->sg_Y.as<double>(), ldB,
+                     NB, NB, tab + T.o_task2, tab + T.o_tile2, 0);
         else
-            SEP_GEMM(false, 2, T.nt2, tab + T.o_aidx2, c->sg_M.as<double>(), ld, c->sg_Y.as<double>(), ld, NP, NP,
-                     tab + T.o_task2, tab + T.o_tile2, 0);
+            SEP_GEMM(false, 2, T.nt2, GS, dU, ldu, tab + T.o_aidx2, c->sg_M.as<double>(), ldB, c->sg_Y.as<double>(),
+                     ldB, NB, NB, tab + T.o_task2, tab + T.o_tile2, 0);
         MCMB_HIP(hipGetLastError());
         // Ys[i][{e,f}] = Y[i][e][f] + Y[i][f][e]
         if (one_block) MCMB_TRY(phase("symmetrisation"));
         {
-            const int nb32 = (NP + 31) / 32;
+            const int nb32 = (NB + 31) / 32;
             const unsigned nbp = (unsigned)(nb32 * (nb32 + 1) / 2);
             if (T.nruns > 65535) return fail(MCMB_ERR_ARG, "separable Gram: %d l1 runs in one block", T.nruns);
             MCMB_TRY(launch_guard("symmetrisation", (unsigned long long)nbp * T.nruns, 256));
             hipLaunchKernelGGL(sep_ysym, dim3(nbp, (unsigned)T.nruns), dim3(256), 0, c->stream,
-                               (const double*)c->sg_Y.as<double>(), NPL, ld, NP, nb32, c->sg_Ys.as<double>(), ldp);
+                               (const double*)c->sg_Y.as<double>(), NPL, ldB, NB, nb32, c->sg_Ys.as<double>(), ldq2,
+                               1.0);
             MCMB_HIP(hipGetLastError());
         }
         if (one_block) MCMB_TRY(phase("l1 contraction (FP64 MFMA)"));
         MCMB_TRY(launch_guard("l1 contraction", sep_blocks(T.nt3, GS), 256));
         if (T.contig3)
-            SEP_GEMM(true, 3, T.nt3, tab + T.o_aidx3, c->sg_Ys.as<double>(), ldp, c->sg_G.as<double>(), ldp, NP,
-                     (int)NPP, tab + T.o_task3, tab + T.o_tile3, b > 0 ? 1 : 0);
+            SEP_GEMM(true, 3, T.nt3, GS, dU, ldu, tab + T.o_aidx3, c->sg_Ys.as<double>(), ldq2, c->sg_G.as<double>(),
+                     ldq2, NB, (int)NBP, tab + T.o_task3, tab + T.o_tile3, b > 0 ? 1 : 0);
         else
-            SEP_GEMM(false, 3, T.nt3, tab + T.o_aidx3, c->sg_Ys.as<double>(), ldp, c->sg_G.as<double>(), ldp, NP,
-                     (int)NPP, tab + T.o_task3, tab + T.o_tile3, b > 0 ? 1 : 0);
-#undef SEP_GEMM
+            SEP_GEMM(false, 3, T.nt3, GS, dU, ldu, tab + T.o_aidx3, c->sg_Ys.as<double>(), ldq2, c->sg_G.as<double>(),
+                     ldq2, NB, (int)NBP, tab + T.o_task3, tab + T.o_tile3, b > 0 ? 1 : 0);
         MCMB_HIP(hipGetLastError());
         launches += 4;
     }
+    if (comp) {
+        // H over the compressed basis (r^3 entries), then back to pairs:
+        // T1[(a,b)][z] = sum_g H[g][(a,b)] Ct[g][z] (into M), Y2[a][y][z] =
+        // sum_b Ct[b][y] T1[(a,b)][z] (into Y), the means of Y2[a] and its
+        // transpose over pairs {y,z} (into Ys), G[x][{y,z}] = sum_a Ct[a][x]
+        // Y2s[a][{y,z}] (into G): the fully symmetrised tensor over pairs
+        MCMB_TRY(phase("expansion to pairs"));
+        const int r = NB;
+        const int* tb = S.tab.as<int>();
+        const long eh = (long)r * ldh;
+        MCMB_TRY(launch_guard("expansion", (unsigned long long)((eh + 255) / 256), 256));
+        hipLaunchKernelGGL(sep_hfull, dim3((unsigned)((eh + 255) / 256)), dim3(256), 0, c->stream,
+                           (const double*)c->sg_G.as<double>(), ldq2, r, S.h.as<double>(), ldh);
+        MCMB_TRY(launch_guard("expansion", sep_blocks(S.nt[0], S.gs[0]), 256));
+        SEP_GEMM(true, 4, S.nt[0], S.gs[0], S.h.as<double>(), ldh, tb, S.ct.as<double>(), ld, c->sg_M.as<double>(),
+                 ld, r * r, NP, tb + S.o_task[0], tb + S.o_tile[0], 0);
+        MCMB_TRY(launch_guard("expansion", sep_blocks(S.nt[1], S.gs[1]), 256));
+        SEP_GEMM(true, 4, S.nt[1], S.gs[1], S.ct.as<double>(), ld, tb, c->sg_M.as<double>(), ld, c->sg_Y.as<double>(),
+                 ld, NP, NP, tb + S.o_task[1], tb + S.o_tile[1], 0);
+        const int nb32 = (NP + 31) / 32;
+        const unsigned nbp = (unsigned)(nb32 * (nb32 + 1) / 2);
+        MCMB_TRY(launch_guard("expansion", (unsigned long long)nbp * r, 256));
+        hipLaunchKernelGGL(sep_ysym, dim3(nbp, (unsigned)r), dim3(256), 0, c->stream,
+                           (const double*)c->sg_Y.as<double>(), (long)NP * ld, ld, NP, nb32, c->sg_Ys.as<double>(),
+                           ldp, 0.5);
+        MCMB_TRY(launch_guard("expansion", sep_blocks(S.nt[2], S.gs[2]), 256));
+        SEP_GEMM(true, 4, S.nt[2], S.gs[2], S.ct.as<double>(), ld, tb, c->sg_Ys.as<double>(), ldp,
+                 c->sg_G.as<double>(), ldp, NP, (int)NPP, tb + S.o_task[2], tb + S.o_tile[2], 0);
+        MCMB_HIP(hipGetLastError());
+        launches += 5;
+    }
+#undef SEP_GEMM
     MCMB_TRY(phase("assembly"));
     {
-        // Gs from G2, then the 6-term sums
+        // Gs from G2 (or from the expanded tensor), then the 6-term sums
         MCMB_TRY(launch_guard("symmetrised G", (unsigned long long)NPP, 256));
         hipLaunchKernelGGL(sep_gsym, dim3((unsigned)NPP), dim3(256), 0, c->stream, (const double*)c->sg_G.as<double>(),
-                           ldp, NP, (const long*)c->sg_off.as<long>(), c->sg_Gs.as<double>());
+                           ldp, NP, (const long*)c->sg_off.as<long>(), c->sg_Gs.as<double>(), comp ? 1 : 3);
         MCMB_HIP(hipGetLastError());
         const long ng = (long)n * n;
         MCMB_TRY(launch_guard("assembly", (unsigned long long)((ng + 255) / 256), 256));
@@ -1111,6 +1462,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     c->last_ms = ms;
     c->last_launches = launches;
     c->nphases = np;
+    c->last_rank = comp ? NB : 0;
     return MCMB_OK;
 }
 

@@ -333,6 +333,7 @@ int pointset_weights(mcmb_ctx* c, PointSet& P, DevBuf& rowptr, DevBuf& ws, int s
 int gram_tet_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gamma, int row_lo, int row_hi)
 {
     c->nphases = 0;
+    c->last_rank = -1;
     // FP64: the separable contraction wherever the basis width allows it,
     // else the factored one, else the direct point-by-point contraction below.
     // MCMB_GRAM_PATH = separable | factored | direct selects one explicitly.

@@ -120,13 +120,28 @@ struct SepPlan {
     long gen = -1;              // its generation
     int row_lo = -1, row_hi = -1;
     int gsz = 0;                // rows per stage-1 group
+    int nb = 0;                 // basis columns of the contractions
     long block_rows = 0;        // row cap override the blocks were cut with (0: none)
     size_t max_tab = 0;         // ints per block in the table buffer
     long max_rows = 0, max_runs = 0, max_wsize = 0;
     std::vector<SepBlock> blocks;
 };
 
-constexpr int MAX_PHASES = 8;
+// Rank-revealing factorisation of the pair products of one point set, U =
+// P Ct (r orthonormal columns P over l), and the tables of the expansion of
+// the contracted tensor back to pairs.  Kept per point set.
+struct SepComp {
+    const void* ps = nullptr;   // point set the factorisation belongs to
+    long gen = -1;              // its generation
+    int NP = 0, nl = 0, nlp = 0;
+    int r = -1;                 // rank; 0: no saving, the pairs are contracted as they are
+    int ldu = 0;                // leading dimension the P table was laid out with
+    DevBuf pc, pt, ct, h, tab;  // P by columns, P table [l][a], Ct [a][pair], H, expansion tables
+    size_t o_task[3] = {}, o_tile[3] = {};
+    int nt[3] = {}, gs[3] = {};
+};
+
+constexpr int MAX_PHASES = 10;
 
 }  // namespace mcmb
 
@@ -157,9 +172,11 @@ struct mcmb_ctx {
     mcmb::DevBuf sg_Ys, sg_Gs, sg_off;   // symmetrised Y, symmetrised G, its row offsets
     int sg_off_np = 0;                   // pair count the offsets were built for
     mcmb::SepPlan sep;
+    mcmb::SepComp sepc;
     // per-launch timing of the last call that records phases (event k+1 - event k)
     hipEvent_t evp[mcmb::MAX_PHASES + 1] = {};
     int nphases = 0;
+    int last_rank = -1;   // basis columns of the last separable Gram's contractions (0: the pairs)
     const char* phase_names[mcmb::MAX_PHASES] = {};
 };
 

@@ -156,7 +156,13 @@ def test_gamma_gl_rejects_inexact_node_count(ctx):
         ctx.gamma_gl(npt=100)
 
 
-_SEP_VARIANTS = {"separable-valu": ("MCMB_SEP_L3", "valu"), "separable-staged": ("MCMB_SEP_WEIGHTS", "staged")}
+_SEP_VARIANTS = {
+    "separable-valu": [("MCMB_SEP_L3", "valu")],
+    "separable-staged": [("MCMB_SEP_WEIGHTS", "staged")],
+    "separable-pairs": [("MCMB_SEP_COMPRESS", "0")],
+    "separable-pairs-valu": [("MCMB_SEP_COMPRESS", "0"), ("MCMB_SEP_L3", "valu")],
+    "separable-pairs-staged": [("MCMB_SEP_COMPRESS", "0"), ("MCMB_SEP_WEIGHTS", "staged")],
+}
 
 
 def _gram_paths(ctx, prec, paths=("separable", "factored", "direct")):
@@ -164,17 +170,17 @@ def _gram_paths(ctx, prec, paths=("separable", "factored", "direct")):
     ("separable-valu": the separable order with its l3 contraction on the
     VALU instead of FP64 MFMA; "separable-staged": with the l-space weights
     staged in HBM by their own kernel instead of formed by the l3
-    contraction)."""
+    contraction; "separable-pairs...": the same over the pair products
+    themselves instead of the compressed basis of their span)."""
     out = {}
-    keys = ("MCMB_GRAM_PATH", "MCMB_SEP_L3", "MCMB_SEP_WEIGHTS")
+    keys = ("MCMB_GRAM_PATH", "MCMB_SEP_L3", "MCMB_SEP_WEIGHTS", "MCMB_SEP_COMPRESS")
     old = {k: os.environ.get(k) for k in keys}
     try:
         for p in paths:
             os.environ["MCMB_GRAM_PATH"] = p.split("-")[0]
             for k in keys[1:]:
                 os.environ.pop(k, None)
-            if p in _SEP_VARIANTS:
-                k, val = _SEP_VARIANTS[p]
+            for k, val in _SEP_VARIANTS.get(p, []):
                 os.environ[k] = val
             out[p] = ctx.gamma_tet(prec)
     finally:
@@ -192,20 +198,29 @@ def test_gamma_contraction_orders_agree(ctx, lmin, L, R, mod, h2):
     # three summation orders of the same FP64 sum: point by point; row moments
     # times pair products; successive contractions over l3, l2, l1 (with the
     # l3 contraction on FP64 MFMA or on the VALU, the weights formed in it or
-    # staged).  The l3 contraction sums each row in l3 order with fused
-    # multiply-adds on every variant, so the separable variants agree bit for
-    # bit.
+    # staged), over the compressed basis (the default: Legendre pair products
+    # have rank 2R - 1) or over the pairs.  The l3 contraction sums each row
+    # in l3 order with fused multiply-adds on every variant, so the separable
+    # variants over one basis agree bit for bit.
     ws = W.lspace(L, R, mod, lmin=lmin)
     ctx.lspace_set(lmin, L, ws["q"], ws["v"], ws["order"], h2_mode=h2)
-    G = _gram_paths(ctx, modalcmb.FP64, ("separable", "separable-valu", "separable-staged", "factored", "direct"))
+    G = _gram_paths(ctx, modalcmb.FP64, ("separable", "separable-valu", "separable-staged", "separable-pairs",
+                                         "separable-pairs-valu", "separable-pairs-staged", "factored", "direct"))
+    assert ctx.last_basis_rank() == -1   # the direct order ran last
     assert rel_elem(G["separable"], G["direct"]) < 1e-12
+    assert rel_elem(G["separable-pairs"], G["direct"]) < 1e-12
+    assert rel_elem(G["separable"], G["separable-pairs"]) < 1e-13
     assert rel_elem(G["factored"], G["direct"]) < 1e-12
     assert np.array_equal(G.pop("separable-valu"), G["separable"])
     assert np.array_equal(G.pop("separable-staged"), G["separable"])
+    assert np.array_equal(G.pop("separable-pairs-valu"), G["separable-pairs"])
+    assert np.array_equal(G.pop("separable-pairs-staged"), G["separable-pairs"])
     for g in G.values():
         assert np.array_equal(g, g.T)
-    # the default is the separable order, bit-identical on repeat
+    # the default is the separable order over the compressed basis,
+    # bit-identical on repeat
     assert np.array_equal(ctx.gamma_tet(modalcmb.FP64), G["separable"])
+    assert ctx.last_basis_rank() == 2 * R - 1
 
 
 def test_gamma_separable_permuted_mode_order(ctx):
@@ -243,13 +258,25 @@ def test_separable_phases_and_repeat_calls(ctx):
     G1 = ctx.gamma_tet(modalcmb.FP64)
     ph = ctx.last_phases()
     names = [p for p, _ in ph]
-    # l-space weights are formed inside the l3 contraction (55 pairs: one wave)
-    assert names == ["pair products", "l3 contraction (row moments)", "l2 contraction (FP64 MFMA)",
-                     "symmetrisation", "l1 contraction (FP64 MFMA)", "assembly"]
+    # the first call on the setup factorises the 55 pair products (rank 19);
+    # l-space weights are formed inside the l3 contraction (one wave)
+    body = ["l3 contraction (row moments)", "l2 contraction (FP64 MFMA)", "symmetrisation",
+            "l1 contraction (FP64 MFMA)"]
+    assert names == ["basis compression"] + body + ["expansion to pairs", "assembly"]
+    assert ctx.last_basis_rank() == 19
     total, _ = ctx.last_kernel_ms()
     assert sum(t for _, t in ph) == pytest.approx(total, rel=0.05, abs=0.05)
     part, _ = ctx.gamma_tet_slice(modalcmb.FP64, 1, 3)
     assert np.array_equal(ctx.gamma_tet(modalcmb.FP64), G1)
+    assert [p for p, _ in ctx.last_phases()] == body + ["expansion to pairs", "assembly"]
+    os.environ["MCMB_SEP_COMPRESS"] = "0"
+    try:
+        Gp = ctx.gamma_tet(modalcmb.FP64)
+        assert [p for p, _ in ctx.last_phases()] == ["pair products"] + body + ["assembly"]
+        assert ctx.last_basis_rank() == 0
+    finally:
+        del os.environ["MCMB_SEP_COMPRESS"]
+    assert rel_elem(Gp, G1) < 1e-13
     ws2 = W.lspace(300, 10, False)
     ctx.lspace_set(2, 300, ws2["q"], ws2["v"], ws2["order"])
     G2 = ctx.gamma_tet(modalcmb.FP64)
@@ -269,40 +296,81 @@ def test_separable_phases_fused_weights(ctx, R):
     # 210 and 300 pairs: the l3 contraction (2 and 4 waves) forms the l-space
     # weights itself, bit-identical to the staged weights; staged, they have
     # a phase of their own
+    # (over the pairs themselves: the compressed basis has 2R - 1 columns)
     ws = W.lspace(160, R, True)
     ctx.lspace_set(2, 160, ws["q"], ws["v"], ws["order"])
-    G = _gram_paths(ctx, modalcmb.FP64, ("separable", "separable-staged", "factored"))
-    ctx.gamma_tet(modalcmb.FP64)
-    names = [p for p, _ in ctx.last_phases()]
-    assert names == ["pair products", "l3 contraction (row moments)", "l2 contraction (FP64 MFMA)",
-                     "symmetrisation", "l1 contraction (FP64 MFMA)", "assembly"]
-    os.environ["MCMB_SEP_WEIGHTS"] = "staged"
+    G = _gram_paths(ctx, modalcmb.FP64, ("separable-pairs", "separable-pairs-staged", "factored"))
+    os.environ["MCMB_SEP_COMPRESS"] = "0"
     try:
+        ctx.gamma_tet(modalcmb.FP64)
+        names = [p for p, _ in ctx.last_phases()]
+        assert names == ["pair products", "l3 contraction (row moments)", "l2 contraction (FP64 MFMA)",
+                         "symmetrisation", "l1 contraction (FP64 MFMA)", "assembly"]
+        os.environ["MCMB_SEP_WEIGHTS"] = "staged"
         ctx.gamma_tet(modalcmb.FP64)
         assert [p for p, _ in ctx.last_phases()][:2] == ["pair products", "weights"]
     finally:
-        del os.environ["MCMB_SEP_WEIGHTS"]
-    assert np.array_equal(G["separable"], G["separable-staged"])
-    assert rel_elem(G["separable"], G["factored"]) < 1e-12
+        os.environ.pop("MCMB_SEP_WEIGHTS", None)
+        del os.environ["MCMB_SEP_COMPRESS"]
+    assert np.array_equal(G["separable-pairs"], G["separable-pairs-staged"])
+    assert rel_elem(G["separable-pairs"], G["factored"]) < 1e-12
 
 
-@pytest.mark.parametrize("R", [10, 24])
-def test_separable_multi_block(ctx, R):
+@pytest.mark.parametrize("R,compress", [(10, "1"), (24, "1"), (10, "0"), (24, "0")])
+def test_separable_multi_block(ctx, R, compress):
     # rows cut into blocks of at most 3000 (of 21,841) by MCMB_SEP_BLOCK_ROWS:
     # per-block weights, contractions and symmetrisation, stage 3 accumulating
-    # over blocks; against the one-block Gram and the oracle's direct sum
+    # over blocks; against the one-block Gram and the oracle's direct sum;
+    # over the compressed basis and over the pairs
     ws = W.lspace(210, R, True)
     ctx.lspace_set(2, 210, ws["q"], ws["v"], ws["order"])
-    G1 = ctx.gamma_tet(modalcmb.FP64)
-    os.environ["MCMB_SEP_BLOCK_ROWS"] = "3000"
+    os.environ["MCMB_SEP_COMPRESS"] = compress
     try:
+        G1 = ctx.gamma_tet(modalcmb.FP64)
+        os.environ["MCMB_SEP_BLOCK_ROWS"] = "3000"
         Gb = ctx.gamma_tet(modalcmb.FP64)
         names = [p for p, _ in ctx.last_phases()]
+        assert ctx.last_basis_rank() == (2 * R - 1 if compress == "1" else 0)
     finally:
-        del os.environ["MCMB_SEP_BLOCK_ROWS"]
+        os.environ.pop("MCMB_SEP_BLOCK_ROWS", None)
+        del os.environ["MCMB_SEP_COMPRESS"]
     assert "l3, l2, l1 contractions (all blocks)" in names
     assert rel_elem(Gb, G1) < 1e-13
     assert np.array_equal(Gb, Gb.T)
     if R == 10:
         assert rel_elem(Gb, O.gamma_tet(2, 210, ws["q"], ws["v"], ws["order"])) < 1e-12
 
+
+
+@pytest.mark.parametrize("R", [6, 12])
+def test_separable_full_rank_basis_contracts_pairs(ctx, R):
+    # a basis of random values per l: its pair products have full rank, so no
+    # compression pays and the pairs are contracted as they are
+    L = 120
+    ws = W.lspace(L, R, True)
+    q = np.random.default_rng(R).standard_normal((L + 1, R))
+    ctx.lspace_set(2, L, q, ws["v"], ws["order"])
+    G = _gram_paths(ctx, modalcmb.FP64, ("separable", "direct"))
+    ctx.gamma_tet(modalcmb.FP64)
+    assert ctx.last_basis_rank() == 0
+    assert "expansion to pairs" not in [p for p, _ in ctx.last_phases()]
+    assert rel_elem(G["separable"], G["direct"]) < 1e-12
+    assert rel_elem(G["separable"], O.gamma_tet(2, L, q, ws["v"], ws["order"])) < 1e-12
+
+
+def test_separable_low_rank_non_polynomial_basis(ctx):
+    # 8 basis functions spanning only 4 directions (q_4..q_7 are fixed
+    # combinations of q_0..q_3, none of them polynomials): the pair products
+    # have rank 10 of 36; the compressed Gram matches the pairs' and the
+    # oracle's direct sum
+    L = 150
+    ws = W.lspace(L, 8, True)
+    rng = np.random.default_rng(11)
+    base = rng.standard_normal((L + 1, 4))
+    q = np.concatenate([base, base @ rng.standard_normal((4, 4))], axis=1)
+    ctx.lspace_set(2, L, q, ws["v"], ws["order"])
+    G = _gram_paths(ctx, modalcmb.FP64, ("separable", "separable-pairs"))
+    ctx.gamma_tet(modalcmb.FP64)
+    assert ctx.last_basis_rank() == 10
+    assert rel_elem(G["separable"], G["separable-pairs"]) < 1e-12
+    assert rel_elem(G["separable"], O.gamma_tet(2, L, q, ws["v"], ws["order"])) < 1e-12

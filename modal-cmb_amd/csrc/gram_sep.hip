@@ -38,9 +38,9 @@ typedef double fd2 __attribute__((ext_vector_type(2)));
 constexpr int SG_FM = 5, SG_FN = 3;   // 16 x 16 blocks per wave (rows, columns); 2 x 2 waves
 constexpr int SG_TM = 32 * SG_FM;     // GEMM tile rows (160)
 constexpr int SG_TN = 32 * SG_FN;     // GEMM tile columns (96)
+constexpr int SG_SMALL = 64;          // basis widths up to this use 64 x 64 tiles (2 x 2 blocks per wave)
 constexpr int SG_KC = 16;             // K rows per LDS stage
-constexpr int SG_LDA = SG_TM + 16;    // LDS row strides in doubles (32 banks apart)
-constexpr int SG_LDB = SG_TN + 16;
+constexpr int SG_KSPLIT = 16;         // K splits of the l1 contraction over a compressed basis
 constexpr int SR_G = 16;            // rows per row-moment group
 constexpr int SR_KC = 16;           // l3 values per weight stage (SR_G * SR_KC = 256 threads)
 constexpr int SR_NPMAX = 768;       // pair columns of stage 1 (3 per thread of 256)
@@ -487,16 +487,20 @@ __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ 
 // b+8, ... share an XCD (and its L2): the groups are dealt to the 8 XCDs in
 // turn, each group's tiles to consecutive blocks of one XCD, so the tiles of
 // a group meet in the same L2 and the XCDs get alike shares of the work.
-// STAGE (2: l2, 3: l1) only names the two launches apart in profiles.
-template <bool CONTIG, int STAGE>
+// STAGE (2: l2, 3: l1, 4: expansion) only names the launches apart in
+// profiles.  FM x FN: 16 x 16 blocks per wave (5 x 3: 160 x 96 tiles; 2 x 2:
+// 64 x 64 tiles for narrow bases).
+template <bool CONTIG, int STAGE, int FM = SG_FM, int FN = SG_FN>
 __global__ __launch_bounds__(256, 2) void sep_gemm(const double* __restrict__ U, int ldu,
                                                    const int* __restrict__ aidx, const double* __restrict__ B,
                                                    long ldb, double* __restrict__ C, long ldc, int Md, int Nd,
                                                    const SepTask* __restrict__ tasks, const int4* __restrict__ tiles,
                                                    int ntiles, int GS, int accumulate)
 {
-    __shared__ double As[2][SG_KC][SG_LDA];
-    __shared__ double Bs[2][SG_KC][SG_LDB];
+    constexpr int TM = 32 * FM, TN = 32 * FN;
+    constexpr int LDA = TM + 16, LDB = TN + 16;   // LDS row strides in doubles (32 banks apart)
+    __shared__ double As[2][SG_KC][LDA];
+    __shared__ double Bs[2][SG_KC][LDB];
     const int slot = blockIdx.x >> 3;
     const int t = ((blockIdx.x & 7) + 8 * (slot / GS)) * GS + slot % GS;
     if (t >= ntiles) return;
@@ -508,13 +512,13 @@ __global__ __launch_bounds__(256, 2) void sep_gemm(const double* __restrict__ U,
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
 
-    fd4 acc[SG_FM][SG_FN];
+    fd4 acc[FM][FN];
 #pragma unroll
-    for (int a = 0; a < SG_FM; ++a)
+    for (int a = 0; a < FM; ++a)
 #pragma unroll
-        for (int c = 0; c < SG_FN; ++c) acc[a][c] = (fd4){0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < FN; ++c) acc[a][c] = (fd4){0.0, 0.0, 0.0, 0.0};
 
-    constexpr int HA = SG_TM / 2, HB = SG_TN / 2;   // double2 per tile row
+    constexpr int HA = TM / 2, HB = TN / 2;   // double2 per tile row
     constexpr int PA = SG_KC * HA / 256;             // double2 of A per thread
     constexpr int PB = SG_KC * HB / 256;             // double2 of B per thread
     static_assert(PA * 256 == SG_KC * HA && PB * 256 == SG_KC * HB, "tile loads split evenly over 256 threads");
@@ -564,15 +568,15 @@ __global__ __launch_bounds__(256, 2) void sep_gemm(const double* __restrict__ U,
 #pragma unroll
             for (int ks = 0; ks < SG_KC / 4; ++ks) {
                 const int kk = ks * 4 + (lane >> 4);
-                double av[SG_FM], bv[SG_FN];
+                double av[FM], bv[FN];
 #pragma unroll
-                for (int a = 0; a < SG_FM; ++a) av[a] = As[b][kk][wm * (SG_TM / 2) + a * 16 + (lane & 15)];
+                for (int a = 0; a < FM; ++a) av[a] = As[b][kk][wm * (TM / 2) + a * 16 + (lane & 15)];
 #pragma unroll
-                for (int c = 0; c < SG_FN; ++c) bv[c] = Bs[b][kk][wn * (SG_TN / 2) + c * 16 + (lane & 15)];
+                for (int c = 0; c < FN; ++c) bv[c] = Bs[b][kk][wn * (TN / 2) + c * 16 + (lane & 15)];
 #pragma unroll
-                for (int a = 0; a < SG_FM; ++a)
+                for (int a = 0; a < FM; ++a)
 #pragma unroll
-                    for (int c = 0; c < SG_FN; ++c)
+                    for (int c = 0; c < FN; ++c)
                         acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[c], acc[a][c], 0, 0, 0);
             }
             if (ch + 1 < nch) sstore(b ^ 1);
@@ -582,13 +586,13 @@ __global__ __launch_bounds__(256, 2) void sep_gemm(const double* __restrict__ U,
     // output element v of a lane: row 4v + lane/16, column lane%16 of the 16 x 16 block
     double* const Cb = C + tk.coff;
 #pragma unroll
-    for (int a = 0; a < SG_FM; ++a)
+    for (int a = 0; a < FM; ++a)
 #pragma unroll
-        for (int c = 0; c < SG_FN; ++c) {
-            const int n = n0 + wn * (SG_TN / 2) + c * 16 + (lane & 15);
+        for (int c = 0; c < FN; ++c) {
+            const int n = n0 + wn * (TN / 2) + c * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int m = m0 + wm * (SG_TM / 2) + a * 16 + 4 * v + (lane >> 4);
+                const int m = m0 + wm * (TM / 2) + a * 16 + 4 * v + (lane >> 4);
                 if (m < Md && n < Nd) {
                     double* p = Cb + (long)m * ldc + n;
                     *p = accumulate ? *p + acc[a][c][v] : acc[a][c][v];
@@ -802,10 +806,12 @@ __global__ void sep_ptab(const double* __restrict__ Pc, int nlp, int nl, int r, 
 }
 
 // H[g][a r + b] = Hs(sorted (g, a, b)) with Hs(x, y, z) = G2[x][{y,z}] +
-// G2[y][{x,z}] + G2[z][{x,y}] (x <= y <= z, pairs over r): every entry of the
+// G2[y][{x,z}] + G2[z][{x,y}] (x <= y <= z, pairs over r), each G2 entry the
+// sum of the ns partial sums G2 + s lds in order of s: every entry of the
 // fully symmetrised tensor over the compressed basis, equal for all orders of
 // (g, a, b); the padding entry of each row (r^2 odd) is zeroed.
-__global__ void sep_hfull(const double* __restrict__ G2, long ldg, int r, double* __restrict__ H, long ldh)
+__global__ void sep_hfull(const double* __restrict__ G2, long ldg, long lds, int ns, int r, double* __restrict__ H,
+                          long ldh)
 {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (long)r * ldh) return;
@@ -817,8 +823,12 @@ __global__ void sep_hfull(const double* __restrict__ G2, long ldg, int r, double
     }
     const int a = (int)(ab / r), b = (int)(ab % r);
     const int x = min(g, min(a, b)), z = max(g, max(a, b)), y = g + a + b - x - z;
-    H[e] = G2[(long)x * ldg + pair_index(y, z, r)] + G2[(long)y * ldg + pair_index(x, z, r)] +
-           G2[(long)z * ldg + pair_index(x, y, r)];
+    auto g2 = [&](int u, int p) {
+        double t = 0.0;
+        for (int sp = 0; sp < ns; ++sp) t += G2[sp * lds + (long)u * ldg + p];
+        return t;
+    };
+    H[e] = g2(x, pair_index(y, z, r)) + g2(y, pair_index(x, z, r)) + g2(z, pair_index(x, y, r));
 }
 
 // Gamma[m][n] = Gamma[n][m] = (1/36) sum over the 6 matchings sigma of the
@@ -878,9 +888,12 @@ template <class T> size_t append(std::vector<int>& buf, const std::vector<T>& v)
     return off;
 }
 
-// NP pairs, ld = padded leading dimension of U, M and Y rows.
-void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1, int NP, int ld, int gsz,
-                 BlockTables& BT)
+// NP basis columns, ld = padded leading dimension of U, M and Y rows; tm x tn
+// GEMM tiles; the l1 contraction's K (the block's runs) cut into up to ksplit
+// tasks, task s writing its own partial sum at s NP ldq2 (ldq2: the leading
+// dimension of its rows).
+void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1, int NP, int ld, int gsz, int tm,
+                 int tn, int ksplit, BlockTables& BT)
 {
     SepBlock& T = BT.b;
     const int kstep = std::max(1, L.kstep);
@@ -890,7 +903,7 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
     std::vector<int> groups, aidx2(T.nrows), aidx3(T.nruns);
     std::vector<int4> ginfo;     // per group: l3 start, l3 count on the group's grid
     std::vector<long> wofs;      // per group: first staged weight
-    std::vector<SepTask> task2(T.nruns), task3(1);
+    std::vector<SepTask> task2(T.nruns), task3;
     std::vector<int4> tile2, tile3;
     std::vector<std::vector<int>> cls(kstep);
     long wsize = 0;
@@ -931,19 +944,26 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
         task2[lt].K = R.r1 - R.r0;
         aidx3[lt] = R.i;
         if (R.i != runs[b0].i + lt) T.contig3 = false;
-        for (int n0 = 0; n0 < NP; n0 += SG_TN)
-            for (int m0 = 0; m0 < NP; m0 += SG_TM) tile2.push_back(make_int4(lt, m0, n0, 0));
+        for (int n0 = 0; n0 < NP; n0 += tn)
+            for (int m0 = 0; m0 < NP; m0 += tm) tile2.push_back(make_int4(lt, m0, n0, 0));
     }
     // contiguous A rows: the task's first row index replaces the list offset
     if (T.contig2)
         for (int lt = 0; lt < T.nruns; ++lt) task2[lt].aoff = L.rows[runs[b0 + lt].r0].j;
-    task3[0].boff = 0;
-    task3[0].coff = 0;
-    task3[0].aoff = T.contig3 ? runs[b0].i : 0;
-    task3[0].K = T.nruns;
     const long N3 = (long)NP * (NP + 1) / 2;   // stage 3 runs over the unordered pairs {e, f}
-    for (long n0 = 0; n0 < N3; n0 += SG_TN)
-        for (int m0 = 0; m0 < NP; m0 += SG_TM) tile3.push_back(make_int4(0, m0, (int)n0, 0));
+    const long ldq2 = N3 + (N3 & 1);
+    const int ns = std::max(1, ksplit);   // every block writes all partial sums (tasks of K = 0 write zeros)
+    for (int sp = 0; sp < ns; ++sp) {
+        const int k0 = (int)((long)T.nruns * sp / ns), k1 = (int)((long)T.nruns * (sp + 1) / ns);
+        SepTask t3;
+        t3.boff = k0;
+        t3.coff = (long)sp * NP * ldq2;
+        t3.aoff = (T.contig3 ? runs[b0].i : 0) + k0;
+        t3.K = k1 - k0;
+        task3.push_back(t3);
+        for (long n0 = 0; n0 < N3; n0 += tn)
+            for (int m0 = 0; m0 < NP; m0 += tm) tile3.push_back(make_int4(sp, m0, (int)n0, 0));
+    }
     T.ngroups = (int)(groups.size() / gsz);
     T.nt2 = (int)tile2.size();
     T.nt3 = (int)tile3.size();
@@ -972,16 +992,16 @@ unsigned sep_blocks(long ntiles, int GS)
 // Tables of the rows [row_lo, row_hi) of L, uploaded to c->sg_tab (kept for
 // the next call on the same point set, generation and rows), and the row
 // pointers in c->sg_rp.
-static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int NP, int ld, int gsz,
-                          bool& rowptr_ready)
+static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int NP, int ld, int gsz, int tm,
+                          int tn, int ksplit, bool& rowptr_ready)
 {
     SepPlan& P = c->sep;
     const char* brenv = getenv("MCMB_SEP_BLOCK_ROWS");
     const long block_rows = brenv ? atol(brenv) : 0;
     const bool same_set = P.ps == &L && P.gen == L.generation;
     rowptr_ready = same_set && c->sg_rp.p;
-    if (same_set && P.row_lo == row_lo && P.row_hi == row_hi && P.gsz == gsz && P.nb == NP &&
-        P.block_rows == block_rows)
+    if (same_set && P.row_lo == row_lo && P.row_hi == row_hi && P.gsz == gsz && P.nb == NP && P.tm == tm &&
+        P.ksplit == ksplit && P.block_rows == block_rows)
         return MCMB_OK;
     P = SepPlan();
     // runs of rows with equal l1, and blocks of runs whose stage outputs fit
@@ -1013,7 +1033,7 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
     }
     std::vector<BlockTables> tabs(blocks.size());
     for (size_t b = 0; b < blocks.size(); ++b) {
-        build_block(L, runs, blocks[b].first, blocks[b].second, NP, ld, gsz, tabs[b]);
+        build_block(L, runs, blocks[b].first, blocks[b].second, NP, ld, gsz, tm, tn, ksplit, tabs[b]);
         P.max_tab = std::max(P.max_tab, tabs[b].buf.size());
         P.max_wsize = std::max(P.max_wsize, tabs[b].b.wsize);
     }
@@ -1030,6 +1050,8 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
     P.row_hi = row_hi;
     P.gsz = gsz;
     P.nb = NP;
+    P.tm = tm;
+    P.ksplit = ksplit;
     P.block_rows = block_rows;
     return MCMB_OK;
 }
@@ -1223,8 +1245,13 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     const bool l3mfma = sh.mfma;
     const long NPL = (long)NB * ldB;       // one Y slab
 
+    // narrow bases: 64 x 64 GEMM tiles; over a compressed basis the l1
+    // contraction's K is split (its partial sums are added by the expansion)
+    const bool small = NB <= SG_SMALL;
+    const int tm = small ? 64 : SG_TM, tn = small ? 64 : SG_TN;
+    const int ksplit = comp ? SG_KSPLIT : 1;
     bool rowptr_ready = false;
-    MCMB_TRY(separable_plan(c, L, row_lo, row_hi, NB, ldB, gsz, rowptr_ready));
+    MCMB_TRY(separable_plan(c, L, row_lo, row_hi, NB, ldB, gsz, tm, tn, ksplit, rowptr_ready));
     const SepPlan& P = c->sep;
 
     const long NPP = (long)NP * (NP + 1) / 2;   // unordered pairs {e, f}
@@ -1236,7 +1263,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     MCMB_TRY(c->sg_M.alloc(sizeof(double) * (size_t)std::max(P.max_rows * ldB, comp ? ldh * ld : 0l)));
     MCMB_TRY(c->sg_Y.alloc(sizeof(double) * (size_t)std::max(P.max_runs * NPL, comp ? (long)NB * NP * ld : 0l)));
     MCMB_TRY(c->sg_Ys.alloc(sizeof(double) * (size_t)std::max(P.max_runs * ldq2, comp ? (long)NB * ldp : 0l)));
-    MCMB_TRY(c->sg_G.alloc(sizeof(double) * (size_t)NP * ldp));
+    MCMB_TRY(c->sg_G.alloc(sizeof(double) * (size_t)std::max(NP * ldp, ksplit * NB * ldq2)));
     MCMB_TRY(c->sg_Gs.alloc(sizeof(double) * (size_t)NGS));
     if (c->sg_off_np != NP) {
         // off[{x,y}] = first Gs element of the triples (x, y, z >= y)
@@ -1316,7 +1343,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         if (one_block) MCMB_TRY(wstage(0));
     }
     if (!one_block) MCMB_TRY(phase("l3, l2, l1 contractions (all blocks)"));
-    const int GS = (NB + SG_TM - 1) / SG_TM;   // m tiles per B panel, both contractions
+    const int GS = (NB + tm - 1) / tm;   // m tiles per B panel, both contractions
     for (size_t b = 0; b < P.blocks.size(); ++b) {
         const SepBlock& T = P.blocks[b];
         const int* tab = c->sg_tab.as<int>() + b * P.max_tab;
@@ -1371,10 +1398,15 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
 #undef SEP_ROWMOM
         MCMB_HIP(hipGetLastError());
         // stages 2 and 3
-#define SEP_GEMM(CG, ST, NT, GSZ, AP, LDA, AIDX, BP, LDB, CP, LDC, MD, ND, TASK, TILE, ACC)                  \
-    hipLaunchKernelGGL((sep_gemm<CG, ST>), dim3(sep_blocks(NT, GSZ)), dim3(256), 0, c->stream, (const double*)(AP), \
-                       (int)(LDA), AIDX, (const double*)(BP), (long)(LDB), CP, (long)(LDC), MD, ND,              \
-                       reinterpret_cast<const SepTask*>(TASK), reinterpret_cast<const int4*>(TILE), NT, GSZ, ACC)
+#define SEP_GEMMF(CG, ST, FM, FN, NT, GSZ, AP, LDA, AIDX, BP, LDB, CP, LDC, MD, ND, TASK, TILE, ACC)           \
+    hipLaunchKernelGGL((sep_gemm<CG, ST, FM, FN>), dim3(sep_blocks(NT, GSZ)), dim3(256), 0, c->stream,           \
+                       (const double*)(AP), (int)(LDA), AIDX, (const double*)(BP), (long)(LDB), CP, (long)(LDC), MD, \
+                       ND, reinterpret_cast<const SepTask*>(TASK), reinterpret_cast<const int4*>(TILE), NT, GSZ, ACC)
+#define SEP_GEMM(CG, ST, ...)                                                                                 \
+    do {                                                                                                      \
+        if (small) SEP_GEMMF(CG, ST, 2, 2, __VA_ARGS__);                                                      \
+        else SEP_GEMMF(CG, ST, SG_FM, SG_FN, __VA_ARGS__);                                                    \
+    } while (0)
         if (one_block) MCMB_TRY(phase("l2 contraction (FP64 MFMA)"));
         MCMB_TRY(launch_guard("l2 contraction", sep_blocks(T.nt2, GS), 256));
         if (T.contig2)
@@ -1419,12 +1451,13 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         const long eh = (long)r * ldh;
         MCMB_TRY(launch_guard("expansion", (unsigned long long)((eh + 255) / 256), 256));
         hipLaunchKernelGGL(sep_hfull, dim3((unsigned)((eh + 255) / 256)), dim3(256), 0, c->stream,
-                           (const double*)c->sg_G.as<double>(), ldq2, r, S.h.as<double>(), ldh);
+                           (const double*)c->sg_G.as<double>(), ldq2, (long)NB * ldq2,
+                           ksplit, r, S.h.as<double>(), ldh);
         MCMB_TRY(launch_guard("expansion", sep_blocks(S.nt[0], S.gs[0]), 256));
-        SEP_GEMM(true, 4, S.nt[0], S.gs[0], S.h.as<double>(), ldh, tb, S.ct.as<double>(), ld, c->sg_M.as<double>(),
+        SEP_GEMMF(true, 4, SG_FM, SG_FN, S.nt[0], S.gs[0], S.h.as<double>(), ldh, tb, S.ct.as<double>(), ld, c->sg_M.as<double>(),
                  ld, r * r, NP, tb + S.o_task[0], tb + S.o_tile[0], 0);
         MCMB_TRY(launch_guard("expansion", sep_blocks(S.nt[1], S.gs[1]), 256));
-        SEP_GEMM(true, 4, S.nt[1], S.gs[1], S.ct.as<double>(), ld, tb, c->sg_M.as<double>(), ld, c->sg_Y.as<double>(),
+        SEP_GEMMF(true, 4, SG_FM, SG_FN, S.nt[1], S.gs[1], S.ct.as<double>(), ld, tb, c->sg_M.as<double>(), ld, c->sg_Y.as<double>(),
                  ld, NP, NP, tb + S.o_task[1], tb + S.o_tile[1], 0);
         const int nb32 = (NP + 31) / 32;
         const unsigned nbp = (unsigned)(nb32 * (nb32 + 1) / 2);
@@ -1433,12 +1466,13 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
                            (const double*)c->sg_Y.as<double>(), (long)NP * ld, ld, NP, nb32, c->sg_Ys.as<double>(),
                            ldp, 0.5);
         MCMB_TRY(launch_guard("expansion", sep_blocks(S.nt[2], S.gs[2]), 256));
-        SEP_GEMM(true, 4, S.nt[2], S.gs[2], S.ct.as<double>(), ld, tb, c->sg_Ys.as<double>(), ldp,
+        SEP_GEMMF(true, 4, SG_FM, SG_FN, S.nt[2], S.gs[2], S.ct.as<double>(), ld, tb, c->sg_Ys.as<double>(), ldp,
                  c->sg_G.as<double>(), ldp, NP, (int)NPP, tb + S.o_task[2], tb + S.o_tile[2], 0);
         MCMB_HIP(hipGetLastError());
         launches += 5;
     }
 #undef SEP_GEMM
+#undef SEP_GEMMF
     MCMB_TRY(phase("assembly"));
     {
         // Gs from G2 (or from the expanded tensor), then the 6-term sums

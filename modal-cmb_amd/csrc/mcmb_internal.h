@@ -121,6 +121,8 @@ struct SepPlan {
     int row_lo = -1, row_hi = -1;
     int gsz = 0;                // rows per stage-1 group
     int nb = 0;                 // basis columns of the contractions
+    int tm = 0;                 // GEMM tile rows (the tile shape follows)
+    int ksplit = 0;             // K splits of the l1 contraction
     long block_rows = 0;        // row cap override the blocks were cut with (0: none)
     size_t max_tab = 0;         // ints per block in the table buffer
     long max_rows = 0, max_runs = 0, max_wsize = 0;

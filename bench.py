@@ -2,14 +2,21 @@
 
 Headline (value): seconds per FP64 tetrapyd Gamma at lmax=2500 with 4960 modes
 (r <= s <= t < 30), 653,383,126 tetrapyd points, by the separable contraction
-(row moments over l3, then FP64 MFMA contractions over l2 and the symmetrised
-l1 of the basis pair products, 6-term assembly).  One step = one full Gamma computed from the
-resident basis, per-leg weights and mode table (the exact 3j weights are
-recomputed inside the step) into a device buffer; the step is the C-ABI call
-mcmb_gamma_tet, timed on the host around the call.
+over the compressed pair basis: the 465 pair products q_x q_y span 59
+directions (rank found by pivoted Gram-Schmidt on the first call for the
+setup, kept with it); row moments over l3 with the exact 3j weights formed in
+the kernel, FP64 MFMA contractions over l2 and the symmetrised l1 on the 59
+columns, the symmetrised tensor expanded back to pairs, 6-term assembly.  One
+step = one full Gamma computed from the resident basis, per-leg weights and
+mode table into a device buffer; the step is the C-ABI call mcmb_gamma_tet,
+timed on the host around the call.
 
 Reported beside it:
   gamma_kernel_ms     HIP-event time of the call and its phases
+  basis_rank          columns of the compressed pair basis (0: pairs)
+  fp64_separable_pairs_s
+                      the same separable contraction over the 465 pair
+                      products themselves (MCMB_SEP_COMPRESS=0), per call
   fp64_factored_s     the same FP64 Gamma by row moments x pair products
   fp64_direct_s       the point-by-point FP64 contraction on the GPU (the
                       reference's algorithm), timed on one slice of the rows
@@ -32,8 +39,8 @@ Reported beside it:
   projection          256-shape batch (omega, phi) at 300 GL nodes, 4960 modes:
                       forward with tangents and the adjoint, FP32 MFMA with
                       FP64 sums; gradients/s and the error against FP64
-  roofline            the l2 contraction (the largest phase) against the FP64
-                      dense MFMA peak, algorithmic flops 2 NP^2 x rows
+  roofline            the l3 contraction (the largest phase) against the FP64
+                      dense MFMA peak, algorithmic flops 2 x points x rank
   cpu_baseline        the FP64 C oracle (direct tetrapyd algorithm) on all
                       host cores over a bounded sample of i-rows, extrapolated
                       by point count
@@ -70,7 +77,7 @@ def rel_elem(A, B):
     return float((np.abs(A - B) / d).max())
 
 
-def hbm_traffic(workload, fname="gamma_sep_l2_hbm_traffic.md"):
+def hbm_traffic(workload, fname="gamma_sep_hbm_traffic.md"):
     """HBM bytes per launch of the dominant kernel from the committed PMC
     summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), or None."""
     p = os.path.join(ROOT, "profiles", fname)
@@ -338,10 +345,27 @@ def main():
     G = ctx.device_to_host(d_gamma, np.empty((n, n)))
     ctx.device_free(d_gamma)
     log(f"separable FP64 Gamma: {t_step * 1e3:.2f} ms per step, {k_step:.2f} ms kernel, phases {phases}")
+    rank = ctx.last_basis_rank()
     t = time.time()
     G_host = ctx.gamma_tet(modalcmb.FP64)
     t_host = time.time() - t
     assert np.array_equal(G, G_host)
+    # the same separable contraction over the pair products themselves
+    os.environ["MCMB_SEP_COMPRESS"] = "0"
+    try:
+        d_gp = ctx.device_alloc(n * n * 8)
+        ctx.gamma_tet_device(modalcmb.FP64, d_gp)
+        ctx.sync()
+        t = time.time()
+        for _ in range(2):
+            ctx.gamma_tet_device(modalcmb.FP64, d_gp)
+        ctx.sync()
+        t_pairs = (time.time() - t) / 2
+        G_pairs = ctx.device_to_host(d_gp, np.empty((n, n)))
+        ctx.device_free(d_gp)
+    finally:
+        del os.environ["MCMB_SEP_COMPRESS"]
+    log(f"separable over the pairs: {t_pairs * 1e3:.2f} ms per call")
 
     # the same Gram by the other contraction orders
     t = time.time()
@@ -364,6 +388,7 @@ def main():
     t_mix = time.time() - t
     log(f"direct slice {r_sl}/{s_cnt} ({frac:.4f}): FP64 {t_dir:.2f} s, mixed {t_mix:.2f} s")
     gamma_err = {
+        "vs_fp64_separable_pairs_full": rel_elem(G, G_pairs),
         "vs_fp64_factored_full": rel_elem(G, G_fac),
         "fp64_factored_vs_direct_slice": rel_elem(S_fac, S_dir),
         "mixed_direct_vs_fp64_factored_slice": rel_elem(S_mix, S_fac),
@@ -408,10 +433,10 @@ def main():
     log("orthonormalisation done")
     ctx.close()
 
-    l2_name = "l2 contraction (FP64 MFMA)"
-    l2_ms = phases.get(l2_name)
-    l2_flops = 2.0 * NP * NP * rows
-    achieved = l2_flops / (l2_ms * 1e-3) / 1e12 if l2_ms else None
+    l3_ms = phases.get("l3 contraction (row moments)")
+    nb = rank if rank > 0 else NP
+    l3_flops = 2.0 * npts * nb
+    achieved = l3_flops / (l3_ms * 1e-3) / 1e12 if l3_ms else None
     fp64_direct_s = t_dir / frac
     mixed_direct_s = t_mix / frac
     out = {
@@ -427,17 +452,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"tetrapyd Gamma, FP64 separable contraction, lmax={L}, modes={n} "
-                               f"(r<=s<=t<{R}), exact 3j weights, output in HBM",
+        "config": {"workload": f"tetrapyd Gamma, FP64 separable contraction over the compressed pair basis, "
+                               f"lmax={L}, modes={n} (r<=s<=t<{R}), exact 3j weights, output in HBM",
                    "lmax": L, "modes": n, "tetrapyd_points": int(npts), "tetrapyd_rows": rows, "pairs": NP},
         "gamma_kernel_ms": round(k_step, 3),
         "phases_ms": phases,
+        "basis_rank": rank,
+        "fp64_separable_pairs_s": round(t_pairs, 5),
         "gamma_with_host_copy_s": round(t_host, 4),
         "fp64_factored_s": round(t_fac, 3),
         "fp64_direct_s": round(fp64_direct_s, 2),
         "mixed_direct_s": round(mixed_direct_s, 2),
         "direct_note": f"slice {r_sl} of {s_cnt} ({frac:.4f} of the point chunks) timed and extrapolated",
         "speedup_vs_fp64_direct": round(fp64_direct_s / t_step, 1),
+        "speedup_vs_fp64_separable_pairs": round(t_pairs / t_step, 2),
         "speedup_vs_mixed_direct": round(mixed_direct_s / t_step, 1),
         "fp64_gl_s": round(t_gl, 4),
         "mixed_note": "the separable FP64 contraction is faster than the mixed-precision direct contraction and "
@@ -448,8 +476,9 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2) if achieved else None, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4) if achieved else None,
                      "traffic": hbm_traffic(workload),
-                     "kernel": "sep_gemm, l2 contraction (FP64 MFMA 16x16x4, 160 x 96 tiles)",
-                     "algorithmic_flops_per_launch": l2_flops, "kernel_ms": l2_ms},
+                     "kernel": f"sep_rowmom_lw, l3 contraction over {nb} basis columns (FP64 MFMA 16x16x4, "
+                               f"3j weights formed in the kernel)",
+                     "algorithmic_flops_per_launch": l3_flops, "kernel_ms": l3_ms},
     }
     out["fp64_gamma_other_workloads"] = bench_other_workloads(modalcmb, W, args.workload)
     log("other workloads done")

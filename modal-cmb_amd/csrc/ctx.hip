@@ -1,5 +1,6 @@
 // Context, error reporting and l-space setup of libmodalcmb.
 #include "mcmb_internal.h"
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
@@ -141,6 +142,7 @@ int mcmb_destroy(mcmb_ctx* c)
     (void)hipStreamSynchronize(c->stream);
     for (PointSet* P : {&c->ls, &c->ks}) {
         P->d_rows.release(); P->d_q.release(); P->d_qf.release(); P->d_v.release(); P->d_ftab.release();
+        P->d_ftp.release(); P->d_vp.release();
         P->d_modes.release(); P->d_kn.release(); P->d_kw.release();
     }
     c->d_W.release();
@@ -211,6 +213,16 @@ int mcmb_lspace_set(mcmb_ctx* c, int lmin, int lmax, int ldq, const double* q, c
     MCMB_TRY(L.d_v.alloc(sizeof(double) * L.v.size()));
     MCMB_HIP(hipMemcpyAsync(L.d_v.p, L.v.data(), sizeof(double) * L.v.size(), hipMemcpyHostToDevice, c->stream));
     MCMB_HIP(hipMemcpyAsync(L.d_ftab.p, ftab.data(), sizeof(double) * ftab.size(), hipMemcpyHostToDevice, c->stream));
+    {
+        const long fg = ftab_guard(lmax);
+        std::vector<double> fp(fg + ftab.size() + TAB_TAIL, 0.0), vp(L.v.size() + TAB_TAIL, 0.0);
+        std::copy(ftab.begin(), ftab.end(), fp.begin() + fg);
+        std::copy(L.v.begin(), L.v.end(), vp.begin());
+        MCMB_TRY(L.d_ftp.alloc(sizeof(double) * fp.size()));
+        MCMB_TRY(L.d_vp.alloc(sizeof(double) * vp.size()));
+        MCMB_HIP(hipMemcpy(L.d_ftp.p, fp.data(), sizeof(double) * fp.size(), hipMemcpyHostToDevice));
+        MCMB_HIP(hipMemcpy(L.d_vp.p, vp.data(), sizeof(double) * vp.size(), hipMemcpyHostToDevice));
+    }
     MCMB_TRY(upload_pointset(c, L, L.order));
     L.ready = true;
     L.generation = next_generation();

@@ -291,7 +291,7 @@ __global__ __launch_bounds__(512, 2) void sep_rowmom_mfma(const int* __restrict_
 // staged ones.  B operands are loaded 3 K-steps ahead into 4 register sets.
 // l-space only (l3 step 2).
 template <int CPW, int ASYM, int NW>
-__global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ groups,
+__global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restrict__ groups,
                                                         const int4* __restrict__ ginfo,
                                                         const RowDesc* __restrict__ rows,
                                                         const double* __restrict__ f,
@@ -359,34 +359,55 @@ __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ 
     const int rj = gj[g], rk0 = gk0[g], rnk = gnk[g];
     const double vi = v[gi], vj = v[rj];
     const long oddij = (long)(2 * gi + 1) * (2 * rj + 1);
+    // permsix at l3 = j and at l3 > j (l3 >= j >= i on the row)
+    const double pe = permsix_d(gi, rj, rj), pn = permsix_d(gi, rj, rj + 1);
+    // (2 l3 + 1) (2i + 1)(2j + 1) grows by dod per element (exact integers in
+    // a double)
+    const double dod = (double)(8 * NW * oddij);
+    constexpr int XS = 4 * NW;   // l3 (and table index) step between a thread's elements
     struct Tab {
-        double f1, f2, f3, rg, vk;
+        double f1, f2, f3, rg, vk, od;
         int k;
         bool in;
     };
     // a thread's weights of a chunk are formed in groups of two (elements 2 gr
-    // and 2 gr + 1), one group's table reads in flight at a time
+    // and 2 gr + 1), one group's table reads in flight at a time.  Along the
+    // thread's row every table index is affine in the element (g = (i + j +
+    // l3) / 2 moves by 2 NW): the reads go through per-chunk base pointers
+    // with constant offsets, into tables with zero guards (f: ftab_guard
+    // doubles before index 0; f and v: TAB_TAIL after the end), so elements
+    // outside the row read in bounds and are dropped by their flag.
     constexpr int NG = WPT / 2;
     auto gather = [&](int c, int gr, Tab (&t)[2]) {
+        const int l30 = kmin + 32 * c + 2 * kk0;   // the thread's l3 at element 0
+        const int u0 = ((kmin - rk0) >> 1) + 16 * c + kk0;   // its index on the row
+        if (ASYM) {
 #pragma unroll
-        for (int y = 0; y < 2; ++y) {
-            const int x = 2 * gr + y;
-            Tab& tx = t[y];
-            const int kg = 16 * c + kk0 + 2 * NW * x, l3 = kmin + 2 * kg;
-            const int d = l3 - rk0;
-            tx.in = kg < nkk && d >= 0 && !(d & 1) && (d >> 1) < rnk;
-            if (ASYM) {
-                tx.k = l3;
-            } else {
-                // out of range: the indices of (0, 0, 0), in the tables
-                const int ii = tx.in ? gi : 0, jj = tx.in ? rj : 0, k3 = tx.in ? l3 : 0;
-                const int gg = (ii + jj + k3) >> 1;
-                tx.k = k3;
-                tx.f1 = f[2 * (gg - ii)];
-                tx.f2 = f[2 * (gg - jj)];
-                tx.f3 = f[2 * (gg - k3)];
-                tx.rg = f[2 * gg + 1];
-                tx.vk = v[k3];
+            for (int y = 0; y < 2; ++y) {
+                const int x = 2 * gr + y;
+                t[y].k = l30 + XS * x;
+                t[y].in = (unsigned)(u0 + 2 * NW * x) < (unsigned)rnk;
+            }
+        } else {
+            const int g0 = (gi + rj + l30) >> 1;
+            const double* const q1 = f + 2 * (g0 - gi);
+            const double* const q2 = f + 2 * (g0 - rj);
+            const double* const q3 = f + 2 * (g0 - l30);
+            const double* const q4 = f + 2 * g0 + 1;
+            const double* const q5 = v + l30;
+            const double od0 = (double)(oddij * (2 * l30 + 1));
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const int x = 2 * gr + y;
+                Tab& tx = t[y];
+                tx.f1 = q1[XS * x];
+                tx.f2 = q2[XS * x];
+                tx.f3 = q3[-XS * x];
+                tx.rg = q4[XS * x];
+                tx.vk = q5[XS * x];
+                tx.od = od0 + dod * (double)x;
+                tx.k = l30 + XS * x;
+                tx.in = (unsigned)(u0 + 2 * NW * x) < (unsigned)rnk;
             }
         }
     };
@@ -399,9 +420,10 @@ __global__ __launch_bounds__(256, 2) void sep_rowmom_lw(const int* __restrict__ 
             if (ASYM) {
                 wv = lspace_weight(tx.in ? gi : 0, tx.in ? rj : 0, tx.in ? tx.k : 0, 1, f, v);
             } else {
-                const double pre = (double)(oddij * (2 * tx.k + 1)) * 0.079577471545947667884;
+                // lspace_weight's arithmetic, in its order
+                const double pre = tx.od * 0.079577471545947667884;
                 const double h2 = pre * (tx.f1 * tx.f2 * tx.f3) * tx.rg;
-                wv = permsix_d(gi, rj, tx.k) * h2 * vi * vj * tx.vk;
+                wv = (tx.k == rj ? pe : pn) * h2 * vi * vj * tx.vk;
             }
             Wt[buf][kk0 + 2 * NW * x][g] = tx.in ? wv : 0.0;
         }
@@ -1365,7 +1387,8 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     hipLaunchKernelGGL((sep_rowmom_lw<CC, AA, NN>),                                                         \
                        dim3((unsigned)(8 * SR_XC * ((T.ngroups + 8 * SR_XC - 1) / (8 * SR_XC)))), dim3(64 * NN), \
                        0, c->stream, tab + T.o_groups, ginfo, (const RowDesc*)L.d_rows.as<RowDesc>(),          \
-                       (const double*)L.d_ftab.as<double>(), (const double*)L.d_v.as<double>(), T.row0,        \
+                       (const double*)L.d_ftp.as<double>() + ftab_guard(L.lmax), (const double*)L.d_vp.as<double>(), \
+                       T.row0,                                                                                  \
                        T.ngroups, dU, ldu, NB, ldB, c->sg_M.as<double>())
 #define SEP_LW(CC, NN)                                                                                        \
     do {                                                                                                      \

@@ -103,7 +103,13 @@ struct PointSet {
     bool uniform = false;             // uniform k-grid (calculate_weight weights)
     double ukmax = 0.0;               // uniform grid: k = ukmax * i / lmax
     DevBuf d_rows, d_q, d_qf, d_v, d_ftab, d_modes, d_kn, d_kw;
+    // l-space: the f table and v with zero guards (ftab_guard(lmax) doubles
+    // before index 0 of the f table, TAB_TAIL after either table's end), so
+    // that indices up to lmax + 64 outside the tables read zeros
+    DevBuf d_ftp, d_vp;
 };
+constexpr int TAB_TAIL = 128;
+inline long ftab_guard(int lmax) { return 2l * (lmax + 64); }
 typedef PointSet LSpace;
 
 // Device-side tables of one block of the separable Gram (offsets in ints

@@ -46,7 +46,8 @@ Reported beside it:
                       by point count
 
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--slices 32]
-                       [--cpu-seconds 20] [--no-cpu] [--no-projection] [--workload lmax2500_r30]
+                       [--cpu-seconds 20] [--no-cpu] [--no-projection] [--no-other-workloads]
+                       [--workload lmax2500_r30]
 """
 from __future__ import annotations
 
@@ -298,6 +299,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-projection", action="store_true")
+    ap.add_argument("--no-other-workloads", action="store_true",
+                    help="skip the lmax 500 and lmax 2000 Gammas (a kernel trace then averages one workload)")
     ap.add_argument("--traffic-program", action="store_true",
                     help="for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes: three 1 GiB device copies "
                          "(calibration), then one FP64 Gamma of the workload; prints one line")
@@ -480,8 +483,9 @@ def main():
                                f"3j weights formed in the kernel)",
                      "algorithmic_flops_per_launch": l3_flops, "kernel_ms": l3_ms},
     }
-    out["fp64_gamma_other_workloads"] = bench_other_workloads(modalcmb, W, args.workload)
-    log("other workloads done")
+    if not args.no_other_workloads:
+        out["fp64_gamma_other_workloads"] = bench_other_workloads(modalcmb, W, args.workload)
+        log("other workloads done")
     if not args.no_projection:
         out["projection"] = bench_projection(modalcmb, W, args)
         log("projection done")

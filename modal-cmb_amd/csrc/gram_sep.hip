@@ -660,10 +660,9 @@ __global__ __launch_bounds__(256) void sep_ysym(const double* __restrict__ Y, lo
 }
 
 // Gs[off[{x,y}] + z - y] = G2[x][{y,z}] + G2[y][{x,z}] + G2[z][{x,y}] for
-// x <= y <= z (terms 3), or G2[x][{y,z}] alone when G2 is already the fully
-// symmetrised tensor (terms 1): one workgroup per pair {x, y}, threads over z.
+// x <= y <= z: one workgroup per pair {x, y}, threads over z.
 __global__ __launch_bounds__(256) void sep_gsym(const double* __restrict__ G2, long ldg, int NP,
-                                                const long* __restrict__ off, double* __restrict__ Gs, int terms)
+                                                const long* __restrict__ off, double* __restrict__ Gs)
 {
     const int p = blockIdx.x;
     int lo = 0, hi = NP - 1;   // x = the largest row whose first pair index is <= p
@@ -677,11 +676,8 @@ __global__ __launch_bounds__(256) void sep_gsym(const double* __restrict__ G2, l
     const double* const Gy = G2 + (long)y * ldg + pair_index(x, x, NP) - x;   // [{x, z}] at z
     const long pxy = pair_index(x, y, NP);
     double* const out = Gs + off[p];
-    if (terms == 1)
-        for (int z = y + threadIdx.x; z < NP; z += blockDim.x) out[z - y] = Gx[z - y];
-    else
-        for (int z = y + threadIdx.x; z < NP; z += blockDim.x)
-            out[z - y] = Gx[z - y] + Gy[z] + G2[(long)z * ldg + pxy];
+    for (int z = y + threadIdx.x; z < NP; z += blockDim.x)
+        out[z - y] = Gx[z - y] + Gy[z] + G2[(long)z * ldg + pxy];
 }
 
 // ---- compressed pair basis ----
@@ -881,6 +877,65 @@ __global__ void sep_assemble(const double* __restrict__ Gs, const long* __restri
     acc *= (1.0 / 36.0);
     gamma[(long)m * n + nn] = acc;
     gamma[(long)nn * n + m] = acc;
+}
+
+// Gamma from the fully symmetrised tensor G[u][{v,w}] (the same value for
+// every order of the pairs u, v, w: the expansion's output), over 32 x 32
+// tiles (bm <= bn) of Gamma.  For each matching sigma the pair that holds
+// n's last index goes into the inner position {v, w}, so the modes of a
+// wave, consecutive in their last index, read neighbouring entries of one G
+// row.  Entries m <= nn are computed; the tile and (off the diagonal) its
+// mirror are written from LDS along rows, so Gamma is exactly symmetric.
+__global__ __launch_bounds__(256) void sep_assemble_sym(const double* __restrict__ G, long ldg, int R,
+                                                        const int* __restrict__ modes, int n, int nt,
+                                                        double* __restrict__ gamma)
+{
+    __shared__ double T[32][33];
+    int bm = 0, rem = blockIdx.x;   // tile (bm <= bn) in row-major upper-triangle order
+    while (rem >= nt - bm) {
+        rem -= nt - bm;
+        ++bm;
+    }
+    const int bn = bm + rem;
+    const int c = threadIdx.x & 31, r0 = threadIdx.x >> 5;
+    const int nn = 32 * bn + c;
+    const int pn = nn < n ? modes[nn] : 0;
+    const int b[3] = {pn & 255, (pn >> 8) & 255, (pn >> 16) & 255};
+    constexpr int P6[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = r0 + 8 * i, m = 32 * bm + r;
+        double acc = 0.0;
+        if (m < n && nn < n && m <= nn) {
+            const int pm = modes[m];
+            const int a[3] = {pm & 255, (pm >> 8) & 255, (pm >> 16) & 255};
+#pragma unroll
+            for (int s = 0; s < 6; ++s) {
+                // slot x of m meets slot P6[s][x] of n; xl: the slot meeting n's last index
+                const int xl = P6[s][0] == 2 ? 0 : P6[s][1] == 2 ? 1 : 2;
+                const int x0 = xl == 0 ? 1 : 0, x1 = xl == 2 ? 1 : 2;
+                const int u = pair_index(min(a[x0], b[P6[s][x0]]), max(a[x0], b[P6[s][x0]]), R);
+                const int v = pair_index(min(a[x1], b[P6[s][x1]]), max(a[x1], b[P6[s][x1]]), R);
+                const int w = pair_index(min(a[xl], b[2]), max(a[xl], b[2]), R);
+                acc += G[(long)u * ldg + pair_index(min(v, w), max(v, w), R * (R + 1) / 2)];
+            }
+            acc *= (1.0 / 36.0);
+        }
+        T[r][c] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = r0 + 8 * i, m = 32 * bm + r;
+        if (m < n && nn < n) gamma[(long)m * n + nn] = m <= nn ? T[r][c] : T[c][r];
+    }
+    if (bm != bn) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = r0 + 8 * i, row = 32 * bn + r, col = 32 * bm + c;
+            if (row < n && col < n) gamma[(long)row * n + col] = T[c][r];
+        }
+    }
 }
 
 bool gram_separable_supported(const PointSet& L, int precision)
@@ -1286,8 +1341,8 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     MCMB_TRY(c->sg_Y.alloc(sizeof(double) * (size_t)std::max(P.max_runs * NPL, comp ? (long)NB * NP * ld : 0l)));
     MCMB_TRY(c->sg_Ys.alloc(sizeof(double) * (size_t)std::max(P.max_runs * ldq2, comp ? (long)NB * ldp : 0l)));
     MCMB_TRY(c->sg_G.alloc(sizeof(double) * (size_t)std::max(NP * ldp, ksplit * NB * ldq2)));
-    MCMB_TRY(c->sg_Gs.alloc(sizeof(double) * (size_t)NGS));
-    if (c->sg_off_np != NP) {
+    if (!comp) MCMB_TRY(c->sg_Gs.alloc(sizeof(double) * (size_t)NGS));
+    if (!comp && c->sg_off_np != NP) {
         // off[{x,y}] = first Gs element of the triples (x, y, z >= y)
         std::vector<long> off(NPP);
         long o = 0;
@@ -1497,11 +1552,20 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
 #undef SEP_GEMM
 #undef SEP_GEMMF
     MCMB_TRY(phase("assembly"));
-    {
-        // Gs from G2 (or from the expanded tensor), then the 6-term sums
+    if (comp) {
+        // the 6-term sums straight from the expanded tensor, by 32 x 32 tiles
+        const int nt = (n + 31) / 32;
+        const unsigned long long ntile = (unsigned long long)nt * (nt + 1) / 2;
+        MCMB_TRY(launch_guard("assembly", ntile, 256));
+        hipLaunchKernelGGL(sep_assemble_sym, dim3((unsigned)ntile), dim3(256), 0, c->stream,
+                           (const double*)c->sg_G.as<double>(), ldp, R, L.d_modes.as<int>(), n, nt, d_gamma);
+        MCMB_HIP(hipGetLastError());
+        ++launches;
+    } else {
+        // Gs from G2, then the 6-term sums
         MCMB_TRY(launch_guard("symmetrised G", (unsigned long long)NPP, 256));
         hipLaunchKernelGGL(sep_gsym, dim3((unsigned)NPP), dim3(256), 0, c->stream, (const double*)c->sg_G.as<double>(),
-                           ldp, NP, (const long*)c->sg_off.as<long>(), c->sg_Gs.as<double>(), comp ? 1 : 3);
+                           ldp, NP, (const long*)c->sg_off.as<long>(), c->sg_Gs.as<double>());
         MCMB_HIP(hipGetLastError());
         const long ng = (long)n * n;
         MCMB_TRY(launch_guard("assembly", (unsigned long long)((ng + 255) / 256), 256));

@@ -304,6 +304,11 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
     static_assert(WPT * 64 * NW == 16 * G, "a chunk's weights split evenly over the threads");
     __shared__ double Wt[2][16][G];
     __shared__ int gj[G], gk0[G], gnk[G];
+    // one wave, exact h^2: a chunk's table reads come from five 64-entry
+    // windows in LDS (f at A1.., A2.., A3.., 1/((2a+1) f) at A4.., v at every
+    // other l3), loaded by one coalesced read each, double-buffered by chunk
+    constexpr bool WIN = !ASYM && NW == 1;
+    __shared__ double Wf[WIN ? 2 : 1][5][64];
     // runs of SR_XC consecutive groups (overlapping l3 windows, so shared pair
     // products in L2) are dealt to the 8 XCDs in turn: blocks b, b + 8, ...
     // share an XCD
@@ -311,27 +316,44 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
     const int gb = ((blockIdx.x & 7) + 8 * (slot / SR_XC)) * SR_XC + slot % SR_XC;
     if (gb >= ngroups) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    // rows of a group share l1; the first is never padding
+    const RowDesc d0 = rows[groups[(long)gb * G]];
+    const int gi = d0.i;
     if (tid < G) {
         const int r = groups[(long)gb * G + tid];
-        const RowDesc d = r >= 0 ? rows[r] : RowDesc{0, 0, 0, 0};
+        // padding rows: no points, the first row's j (keeps window indices in range)
+        const RowDesc d = r >= 0 ? rows[r] : RowDesc{0, d0.j, 0, 0};
         gj[tid] = d.j;
         gk0[tid] = d.k0;
         gnk[tid] = d.nk;
     }
-    // rows of a group share l1; the first is never padding
-    const int gi = rows[groups[(long)gb * G]].i;
     const int4 info = ginfo[gb];
     const int kmin = info.x, nkk = info.y;
     const int nks = (nkk + 3) >> 2;           // K-steps
     const int nch = (nkk + 15) >> 4;          // chunks of 4 K-steps
-    const int col0 = 16 * CPW * w + (lane & 15);
+    // 4 column tiles: lane l of wave w holds columns 16 CPW w + CPW (l % 16)
+    // + ct of tile ct (consecutive doubles: 16-byte loads, half the load
+    // instructions; 2.85 -> 2.68 ms over 59 columns).  Wider waves keep column
+    // 16 CPW w + 16 ct + l % 16 (the consecutive layout took 15.2 -> 16.3 ms
+    // at 8 tiles).  Each output column sums in l3 order either way.
+    constexpr bool CONS = CPW == 4;
+    const int col0 = 16 * CPW * w + (CONS ? CPW * (lane & 15) : (lane & 15));
     const double* const Ub = U + (long)kmin * ldu + col0;
     const long ustep = 2l * ldu;
 
     auto loadb = [&](int s, double (&b)[CPW]) {
         const double* Ur = Ub + ustep * min(4 * s + (lane >> 4), nkk - 1);
+        if constexpr (CONS) {
 #pragma unroll
-        for (int ct = 0; ct < CPW; ++ct) b[ct] = Ur[16 * ct];
+            for (int ct = 0; ct < CPW; ct += 2) {
+                const fd2 u2 = *reinterpret_cast<const fd2*>(Ur + ct);
+                b[ct] = u2.x;
+                b[ct + 1] = u2.y;
+            }
+        } else {
+#pragma unroll
+            for (int ct = 0; ct < CPW; ++ct) b[ct] = Ur[16 * ct];
+        }
     };
 
     fd4 acc[RT][CPW];
@@ -358,6 +380,28 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
     const int g = tid & (G - 1), kk0 = tid / G;   // element x: l3 index kk0 + 2 NW x of the chunk
     const int rj = gj[g], rk0 = gk0[g], rnk = gnk[g];
     const double vi = v[gi], vj = v[rj];
+    // window geometry: the rows' j lie in [jlo, jhi] (one parity, span <= 96:
+    // checked on the host), so for l3 = L0 + 2 t (t < 16) the f indices are
+    // A + dj + t, A + dh + t, A + dj + 15 - t and the 1/((2a+1) f) index A +
+    // dj + t, all below 64
+    int jhi = d0.j;
+    if constexpr (WIN)
+        for (int q = 1; q < G; ++q) jhi = max(jhi, gj[q]);
+    const int jlo = d0.j, dj = (rj - jlo) >> 1, dh = (jhi - rj) >> 1;
+    auto wload = [&](int c, double (&wr)[5]) {
+        const int L0 = kmin + 32 * c;
+        const int A1 = (jlo + L0 - gi) >> 1, A2 = (gi + L0 - jhi) >> 1, A3 = (gi + jlo - L0 - 30) >> 1,
+                  A4 = (gi + jlo + L0) >> 1;
+        wr[0] = f[2 * (A1 + lane)];
+        wr[1] = f[2 * (A2 + lane)];
+        wr[2] = f[2 * (A3 + lane)];
+        wr[3] = f[2 * (A4 + lane) + 1];
+        wr[4] = v[L0 + 2 * min(lane, 15)];
+    };
+    auto wstore = [&](int b, const double (&wr)[5]) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) Wf[b][q][lane] = wr[q];
+    };
     const long oddij = (long)(2 * gi + 1) * (2 * rj + 1);
     // permsix at l3 = j and at l3 > j (l3 >= j >= i on the row)
     const double pe = permsix_d(gi, rj, rj), pn = permsix_d(gi, rj, rj + 1);
@@ -387,6 +431,23 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
                 const int x = 2 * gr + y;
                 t[y].k = l30 + XS * x;
                 t[y].in = (unsigned)(u0 + 2 * NW * x) < (unsigned)rnk;
+            }
+        } else if constexpr (WIN) {
+            const double* const W = &Wf[c & 1][0][0];
+            const double od0 = (double)(oddij * (2 * l30 + 1));
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const int x = 2 * gr + y;
+                const int tt = kk0 + 2 * x;   // l3 = kmin + 32 c + 2 tt
+                Tab& tx = t[y];
+                tx.f1 = W[dj + tt];
+                tx.f2 = W[64 + dh + tt];
+                tx.f3 = W[128 + dj + 15 - tt];
+                tx.rg = W[192 + dj + tt];
+                tx.vk = W[256 + tt];
+                tx.od = od0 + dod * (double)x;
+                tx.k = l30 + XS * x;
+                tx.in = (unsigned)(u0 + 2 * NW * x) < (unsigned)rnk;
             }
         } else {
             const int g0 = (gi + rj + l30) >> 1;
@@ -428,6 +489,16 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
             Wt[buf][kk0 + 2 * NW * x][g] = tx.in ? wv : 0.0;
         }
     };
+    if constexpr (WIN) {
+        if (nks > 0) {
+            double wr[5];
+            wload(0, wr);
+            wstore(0, wr);
+            wload(min(1, nch - 1), wr);
+            wstore(1, wr);
+        }
+        __syncthreads();
+    }
     if (nks > 0) {
         Tab t0[2];
 #pragma unroll
@@ -449,6 +520,9 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
         for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) a[ks][rt] = Wt[buf][4 * ks + (lane >> 4)][16 * rt + (lane & 15)];
+        // windows of chunk c + 2, stored into this chunk's buffer at its end
+        double wr[WIN ? 5 : 1];
+        if constexpr (WIN) wload(min(c + 2, nch - 1), wr);
         Tab t[2];
         gather(c + 1, 0, t);
 #pragma unroll
@@ -466,6 +540,7 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
+        if constexpr (WIN) wstore(buf, wr);
         __syncthreads();
     }
     // the last chunk: its K-steps below nks (steps 4c .. 4c + 2 are loaded;
@@ -492,7 +567,7 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
             double* const Mr = M + (long)(r - row_base) * ld;
 #pragma unroll
             for (int ct = 0; ct < CPW; ++ct) {
-                const int fcol = col0 + 16 * ct;
+                const int fcol = col0 + (CONS ? ct : 16 * ct);
                 if (fcol < NP) Mr[fcol] = acc[rt][ct][vv];
             }
         }
@@ -994,6 +1069,7 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
                 kmin = std::min(kmin, d.k0);
                 kmax = std::max(kmax, d.k0 + kstep * (d.nk - 1));
             }
+            if (g < (int)c.size()) T.jspan = std::max(T.jspan, L.rows[c[g]].j - L.rows[c[0]].j);
         }
         const int nkk = kmax >= kmin ? (kmax - kmin) / kstep + 1 : 0;
         ginfo.push_back(make_int4(nkk ? kmin : 0, nkk, 0, 0));
@@ -1382,8 +1458,13 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     // the MFMA l3 contraction forms l-space weights itself (1, 2 or 4 waves:
     // up to 512 columns); MCMB_SEP_WEIGHTS=staged stages them instead
     const char* wenv = getenv("MCMB_SEP_WEIGHTS");
+    // (one wave reads its tables through 64-entry windows: the rows of a group
+    // must span at most 96 in j, which the walk's 32-row groups of one parity
+    // class do)
+    int jspan = 0;
+    for (const SepBlock& T : P.blocks) jspan = std::max(jspan, T.jspan);
     const bool wfused = l3mfma && SR_GM == 32 && nwv <= 4 && (nwv == 1 || cpw != 4) && wkind != 2 && L.kstep == 2 &&
-                        !(wenv && !strcmp(wenv, "staged"));
+                        (nwv > 1 || jspan <= 96) && !(wenv && !strcmp(wenv, "staged"));
     if (!wfused) MCMB_TRY(c->sg_wd.alloc(sizeof(double) * (size_t)std::max<long>(1, P.max_wsize)));
     // staged weights of one block's row groups
     auto wstage = [&](size_t b) -> int {

@@ -104,11 +104,12 @@ struct PointSet {
     double ukmax = 0.0;               // uniform grid: k = ukmax * i / lmax
     DevBuf d_rows, d_q, d_qf, d_v, d_ftab, d_modes, d_kn, d_kw;
     // l-space: the f table and v with zero guards (ftab_guard(lmax) doubles
-    // before index 0 of the f table, TAB_TAIL after either table's end), so
-    // that indices up to lmax + 64 outside the tables read zeros
+    // before index 0 of the f table, TAB_TAIL after either table's end): the
+    // fused l3 contraction reads f indices in [-lmax - 64, 3 lmax + 158) and
+    // v indices below lmax + 64
     DevBuf d_ftp, d_vp;
 };
-constexpr int TAB_TAIL = 128;
+constexpr int TAB_TAIL = 256;
 inline long ftab_guard(int lmax) { return 2l * (lmax + 64); }
 typedef PointSet LSpace;
 
@@ -120,6 +121,7 @@ struct SepBlock {
     size_t o_groups = 0, o_ginfo = 0, o_wofs = 0, o_aidx2 = 0, o_aidx3 = 0, o_task2 = 0, o_task3 = 0, o_tile2 = 0,
            o_tile3 = 0;
     long wsize = 0;   // doubles of the staged weights of the block's row groups
+    int jspan = 0;    // largest j span of the rows of one group
 };
 struct SepPlan {
     const void* ps = nullptr;   // point set the tables were built for

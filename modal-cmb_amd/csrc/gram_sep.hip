@@ -111,8 +111,17 @@ __global__ __launch_bounds__(256) void sep_wstage(const RowDesc* __restrict__ ro
         if (kg < nkk) {
             const int k = kmin + kstep * kg;
             const int d = k - gk0[g];
-            if (d >= 0 && d % kstep == 0 && d / kstep < gnk[g])
-                w = KIND == 2 ? ws[gp[g] + d / kstep] : lspace_weight(gi[g], gj[g], k, KIND, f, v);
+            if (d >= 0 && d % kstep == 0 && d / kstep < gnk[g]) {
+                if (KIND == 2) {
+                    w = ws[gp[g] + d / kstep];
+                } else if (KIND == 1) {
+                    w = lspace_weight(gi[g], gj[g], k, 1, f, v);
+                } else {
+                    const int i = gi[g], j = gj[g], gg = (i + j + k) >> 1;
+                    w = sep_weight(sep_row_factor(i, j, permsix_d(i, j, k), v), sep_l3_factor(k, v),
+                                   f[2 * (gg - i)], f[2 * (gg - j)], f[2 * (gg - k)], f[2 * gg + 1]);
+                }
+            }
         }
         W[G * k0 + tid] = w;   // = W[G kg + g]
     }
@@ -284,12 +293,12 @@ __global__ __launch_bounds__(512, 2) void sep_rowmom_mfma(const int* __restrict_
 // Stage 1 on FP64 MFMA with the l-space weights formed in place (no staged
 // weights in HBM): as sep_rowmom_mfma with RT = 2 and 4 waves, over chunks of
 // 16 l3 values.  Between a chunk's K-steps the workgroup forms the next
-// chunk's 16 x 32 weights (2 per thread), permsix h^2 v v v (exact h^2 from
-// the f table or the asymptotic form; the tables are cache-resident), into
-// the other half of a double-buffered LDS tile; one barrier per chunk.  The
-// arithmetic is lspace_weight's, so the weights are bit-identical to the
-// staged ones.  B operands are loaded 3 K-steps ahead into 4 register sets.
-// l-space only (l3 step 2).
+// chunk's 16 x 32 weights (8 / NW per thread), permsix h^2 v v v (exact h^2
+// by sep_weight from the f table, or the asymptotic form by lspace_weight),
+// into the other half of a double-buffered LDS tile; one barrier per chunk.
+// The arithmetic is the staged kernel's, so the weights are bit-identical to
+// the staged ones.  B operands are loaded 3 K-steps ahead into 4 register
+// sets.  l-space only (l3 step 2).
 template <int CPW, int ASYM, int NW>
 __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restrict__ groups,
                                                         const int4* __restrict__ ginfo,
@@ -375,11 +384,10 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
     // l3 = kmin + 2 (16 c + kk);
     // all are of row g = tid % 32, whose constants stay in registers.  The
     // table reads of a group of two weights (gather) are issued one or more
-    // K-steps of MFMAs before their use (weigh); the arithmetic is
-    // lspace_weight's, in its order.
+    // K-steps of MFMAs before their use (weigh); exact weights by
+    // sep_weight, as the staged ones.
     const int g = tid & (G - 1), kk0 = tid / G;   // element x: l3 index kk0 + 2 NW x of the chunk
     const int rj = gj[g], rk0 = gk0[g], rnk = gnk[g];
-    const double vi = v[gi], vj = v[rj];
     // window geometry: the rows' j lie in [jlo, jhi] (one parity, span <= 96:
     // checked on the host), so for l3 = L0 + 2 t (t < 16) the f indices are
     // A + dj + t, A + dh + t, A + dj + 15 - t and the 1/((2a+1) f) index A +
@@ -396,21 +404,18 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
         wr[1] = f[2 * (A2 + lane)];
         wr[2] = f[2 * (A3 + lane)];
         wr[3] = f[2 * (A4 + lane) + 1];
-        wr[4] = v[L0 + 2 * min(lane, 15)];
+        wr[4] = sep_l3_factor(L0 + 2 * min(lane, 15), v);
     };
     auto wstore = [&](int b, const double (&wr)[5]) {
 #pragma unroll
         for (int q = 0; q < 5; ++q) Wf[b][q][lane] = wr[q];
     };
-    const long oddij = (long)(2 * gi + 1) * (2 * rj + 1);
-    // permsix at l3 = j and at l3 > j (l3 >= j >= i on the row)
-    const double pe = permsix_d(gi, rj, rj), pn = permsix_d(gi, rj, rj + 1);
-    // (2 l3 + 1) (2i + 1)(2j + 1) grows by dod per element (exact integers in
-    // a double)
-    const double dod = (double)(8 * NW * oddij);
+    // row factors of the weight at l3 = j and at l3 > j (l3 >= j >= i on the row)
+    const double re = sep_row_factor(gi, rj, permsix_d(gi, rj, rj), v),
+                 rn = sep_row_factor(gi, rj, permsix_d(gi, rj, rj + 1), v);
     constexpr int XS = 4 * NW;   // l3 (and table index) step between a thread's elements
     struct Tab {
-        double f1, f2, f3, rg, vk, od;
+        double f1, f2, f3, rg, tk;   // tk: the l3 factor (2 l3 + 1) v[l3]
         int k;
         bool in;
     };
@@ -434,7 +439,6 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
             }
         } else if constexpr (WIN) {
             const double* const W = &Wf[c & 1][0][0];
-            const double od0 = (double)(oddij * (2 * l30 + 1));
 #pragma unroll
             for (int y = 0; y < 2; ++y) {
                 const int x = 2 * gr + y;
@@ -444,8 +448,7 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
                 tx.f2 = W[64 + dh + tt];
                 tx.f3 = W[128 + dj + 15 - tt];
                 tx.rg = W[192 + dj + tt];
-                tx.vk = W[256 + tt];
-                tx.od = od0 + dod * (double)x;
+                tx.tk = W[256 + tt];
                 tx.k = l30 + XS * x;
                 tx.in = (unsigned)(u0 + 2 * NW * x) < (unsigned)rnk;
             }
@@ -456,7 +459,6 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
             const double* const q3 = f + 2 * (g0 - l30);
             const double* const q4 = f + 2 * g0 + 1;
             const double* const q5 = v + l30;
-            const double od0 = (double)(oddij * (2 * l30 + 1));
 #pragma unroll
             for (int y = 0; y < 2; ++y) {
                 const int x = 2 * gr + y;
@@ -465,9 +467,8 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
                 tx.f2 = q2[XS * x];
                 tx.f3 = q3[-XS * x];
                 tx.rg = q4[XS * x];
-                tx.vk = q5[XS * x];
-                tx.od = od0 + dod * (double)x;
                 tx.k = l30 + XS * x;
+                tx.tk = (double)(2 * tx.k + 1) * q5[XS * x];
                 tx.in = (unsigned)(u0 + 2 * NW * x) < (unsigned)rnk;
             }
         }
@@ -481,10 +482,7 @@ __global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restric
             if (ASYM) {
                 wv = lspace_weight(tx.in ? gi : 0, tx.in ? rj : 0, tx.in ? tx.k : 0, 1, f, v);
             } else {
-                // lspace_weight's arithmetic, in its order
-                const double pre = tx.od * 0.079577471545947667884;
-                const double h2 = pre * (tx.f1 * tx.f2 * tx.f3) * tx.rg;
-                wv = (tx.k == rj ? pe : pn) * h2 * vi * vj * tx.vk;
+                wv = sep_weight(tx.k == rj ? re : rn, tx.tk, tx.f1, tx.f2, tx.f3, tx.rg);
             }
             Wt[buf][kk0 + 2 * NW * x][g] = tx.in ? wv : 0.0;
         }

@@ -300,7 +300,7 @@ __global__ __launch_bounds__(512, 2) void sep_rowmom_mfma(const int* __restrict_
 // the staged ones.  B operands are loaded 3 K-steps ahead into 4 register
 // sets.  l-space only (l3 step 2).
 template <int CPW, int ASYM, int NW>
-__global__ __launch_bounds__(64 * NW, 2) void sep_rowmom_lw(const int* __restrict__ groups,
+__global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_rowmom_lw(const int* __restrict__ groups,
                                                         const int4* __restrict__ ginfo,
                                                         const RowDesc* __restrict__ rows,
                                                         const double* __restrict__ f,

@@ -584,8 +584,9 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
 // a group meet in the same L2 and the XCDs get alike shares of the work.
 // STAGE (2: l2, 3: l1, 4: expansion) only names the launches apart in
 // profiles.  FM x FN: 16 x 16 blocks per wave (5 x 3: 160 x 96 tiles; 2 x 2:
-// 64 x 64 tiles for narrow bases).
-template <bool CONTIG, int STAGE, int FM = SG_FM, int FN = SG_FN>
+// 64 x 64 tiles for narrow bases).  PAIRS: C is symmetric (Md = Nd) and only
+// its upper triangle is stored, at C + coff + pair_index(m, n, Md).
+template <bool CONTIG, int STAGE, int FM = SG_FM, int FN = SG_FN, bool PAIRS = false>
 __global__ __launch_bounds__(256, 2) void sep_gemm(const double* __restrict__ U, int ldu,
                                                    const int* __restrict__ aidx, const double* __restrict__ B,
                                                    long ldb, double* __restrict__ C, long ldc, int Md, int Nd,
@@ -688,21 +689,21 @@ __global__ __launch_bounds__(256, 2) void sep_gemm(const double* __restrict__ U,
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int m = m0 + wm * (TM / 2) + a * 16 + 4 * v + (lane >> 4);
-                if (m < Md && n < Nd) {
-                    double* p = Cb + (long)m * ldc + n;
+                if (m < Md && n < Nd && (!PAIRS || m <= n)) {
+                    double* p = Cb + (PAIRS ? (long)pair_index(m, n, Md) : (long)m * ldc + n);
                     *p = accumulate ? *p + acc[a][c][v] : acc[a][c][v];
                 }
             }
         }
 }
 
-// Ys[i][{e,f}] = scale (Y[i][e][f] + Y[i][f][e]) for e <= f ({e,f} =
-// pair_index(e, f, NP); scale 1, or 1/2 for the mean of a symmetric Y): one workgroup per (32 x 32 block pair E <= F, run i); both blocks are
+// Ys[i][{e,f}] = Y[i][e][f] + Y[i][f][e] for e <= f ({e,f} = pair_index(e, f,
+// NP)): one workgroup per (32 x 32 block pair E <= F, run i); both blocks are
 // read row by row (coalesced), the second transposed through LDS.  The
 // padding element of each Ys row (index NP(NP+1)/2 when that is odd) is
 // zeroed.
 __global__ __launch_bounds__(256) void sep_ysym(const double* __restrict__ Y, long ldyi, int ld, int NP, int nb,
-                                                double* __restrict__ Ys, long ldys, double scale)
+                                                double* __restrict__ Ys, long ldys)
 {
     __shared__ double A[32][33], Bt[32][33];
     int bE = 0, rem = blockIdx.x;   // block pair (bE <= bF) in row-major upper-triangle order
@@ -726,7 +727,7 @@ __global__ __launch_bounds__(256) void sep_ysym(const double* __restrict__ Y, lo
     for (int r4 = 0; r4 < 4; ++r4) {
         const int r = (tid >> 5) + 8 * r4;
         const int e = e0 + r, f = f0 + c;
-        if (e < NP && f < NP && e <= f) Yo[pair_index(e, f, NP)] = scale * (A[r][c] + Bt[c][r]);
+        if (e < NP && f < NP && e <= f) Yo[pair_index(e, f, NP)] = A[r][c] + Bt[c][r];
     }
     const long npp = (long)NP * (NP + 1) / 2;
     if (blockIdx.x == 0 && tid == 0 && ldys > npp) Yo[npp] = 0.0;
@@ -1286,8 +1287,8 @@ static int separable_compress(mcmb_ctx* c, PointSet& L, int nl, int NP, int ld, 
     const int r = hr > 0 ? hr : 0;
     if (r > 0) {
         // expansion: T1[(a,b)][z] = sum_g H[g][(a,b)] Ct[g][z]; per a,
-        // Y2[a][y][z] = sum_b Ct[b][y] T1[(a,b)][z]; G[x][{y,z}] = sum_a
-        // Ct[a][x] Y2s[a][{y,z}] (tile groups share a B panel, as in the
+        // Y2s[a][{y,z}] = sum_b Ct[b][y] T1[(a,b)][z] (y <= z); G[x][{y,z}] =
+        // sum_a Ct[a][x] Y2s[a][{y,z}] (tile groups share a B panel, as in the
         // contractions)
         const long ldh = (long)r * r + ((r * r) & 1);
         const long NPP = (long)NP * (NP + 1) / 2;
@@ -1296,10 +1297,13 @@ static int separable_compress(mcmb_ctx* c, PointSet& L, int nl, int NP, int ld, 
         t1[0] = SepTask{0, 0, 0, r};
         for (int n0c = 0; n0c < NP; n0c += SG_TN)
             for (int m0 = 0; m0 < r * r; m0 += SG_TM) g1.push_back(make_int4(0, m0, n0c, 0));
+        // Y2[a] is symmetric (H is): its upper triangle goes straight to pair
+        // order, and tiles wholly below the diagonal are skipped
+        const long ldp = NPP + (NPP & 1);
         for (int a = 0; a < r; ++a) {
-            t2[a] = SepTask{(long)a * r, (long)a * NP * ld, 0, r};
+            t2[a] = SepTask{(long)a * r, (long)a * ldp, 0, r};
             for (int n0c = 0; n0c < NP; n0c += SG_TN)
-                for (int m0 = 0; m0 < NP; m0 += SG_TM) g2.push_back(make_int4(a, m0, n0c, 0));
+                for (int m0 = 0; m0 < NP && m0 < n0c + SG_TN; m0 += SG_TM) g2.push_back(make_int4(a, m0, n0c, 0));
         }
         t3[0] = SepTask{0, 0, 0, r};
         for (long n0c = 0; n0c < NPP; n0c += SG_TN)
@@ -1412,7 +1416,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     const long ldq2 = NBP + (NBP & 1);
     const long ldh = (long)NB * NB + ((NB * NB) & 1);
     MCMB_TRY(c->sg_M.alloc(sizeof(double) * (size_t)std::max(P.max_rows * ldB, comp ? ldh * ld : 0l)));
-    MCMB_TRY(c->sg_Y.alloc(sizeof(double) * (size_t)std::max(P.max_runs * NPL, comp ? (long)NB * NP * ld : 0l)));
+    MCMB_TRY(c->sg_Y.alloc(sizeof(double) * (size_t)(P.max_runs * NPL)));
     MCMB_TRY(c->sg_Ys.alloc(sizeof(double) * (size_t)std::max(P.max_runs * ldq2, comp ? (long)NB * ldp : 0l)));
     MCMB_TRY(c->sg_G.alloc(sizeof(double) * (size_t)std::max(NP * ldp, ksplit * NB * ldq2)));
     if (!comp) MCMB_TRY(c->sg_Gs.alloc(sizeof(double) * (size_t)NGS));
@@ -1581,8 +1585,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
             if (T.nruns > 65535) return fail(MCMB_ERR_ARG, "separable Gram: %d l1 runs in one block", T.nruns);
             MCMB_TRY(launch_guard("symmetrisation", (unsigned long long)nbp * T.nruns, 256));
             hipLaunchKernelGGL(sep_ysym, dim3(nbp, (unsigned)T.nruns), dim3(256), 0, c->stream,
-                               (const double*)c->sg_Y.as<double>(), NPL, ldB, NB, nb32, c->sg_Ys.as<double>(), ldq2,
-                               1.0);
+                               (const double*)c->sg_Y.as<double>(), NPL, ldB, NB, nb32, c->sg_Ys.as<double>(), ldq2);
             MCMB_HIP(hipGetLastError());
         }
         if (one_block) MCMB_TRY(phase("l1 contraction (FP64 MFMA)"));
@@ -1598,10 +1601,10 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     }
     if (comp) {
         // H over the compressed basis (r^3 entries), then back to pairs:
-        // T1[(a,b)][z] = sum_g H[g][(a,b)] Ct[g][z] (into M), Y2[a][y][z] =
-        // sum_b Ct[b][y] T1[(a,b)][z] (into Y), the means of Y2[a] and its
-        // transpose over pairs {y,z} (into Ys), G[x][{y,z}] = sum_a Ct[a][x]
-        // Y2s[a][{y,z}] (into G): the fully symmetrised tensor over pairs
+        // T1[(a,b)][z] = sum_g H[g][(a,b)] Ct[g][z] (into M), Y2s[a][{y,z}] =
+        // sum_b Ct[b][y] T1[(a,b)][z] for y <= z (into Ys, pair order),
+        // G[x][{y,z}] = sum_a Ct[a][x] Y2s[a][{y,z}] (into G): the fully
+        // symmetrised tensor over pairs
         MCMB_TRY(phase("expansion to pairs"));
         const int r = NB;
         const int* tb = S.tab.as<int>();
@@ -1614,19 +1617,16 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         SEP_GEMMF(true, 4, SG_FM, SG_FN, S.nt[0], S.gs[0], S.h.as<double>(), ldh, tb, S.ct.as<double>(), ld, c->sg_M.as<double>(),
                  ld, r * r, NP, tb + S.o_task[0], tb + S.o_tile[0], 0);
         MCMB_TRY(launch_guard("expansion", sep_blocks(S.nt[1], S.gs[1]), 256));
-        SEP_GEMMF(true, 4, SG_FM, SG_FN, S.nt[1], S.gs[1], S.ct.as<double>(), ld, tb, c->sg_M.as<double>(), ld, c->sg_Y.as<double>(),
-                 ld, NP, NP, tb + S.o_task[1], tb + S.o_tile[1], 0);
-        const int nb32 = (NP + 31) / 32;
-        const unsigned nbp = (unsigned)(nb32 * (nb32 + 1) / 2);
-        MCMB_TRY(launch_guard("expansion", (unsigned long long)nbp * r, 256));
-        hipLaunchKernelGGL(sep_ysym, dim3(nbp, (unsigned)r), dim3(256), 0, c->stream,
-                           (const double*)c->sg_Y.as<double>(), (long)NP * ld, ld, NP, nb32, c->sg_Ys.as<double>(),
-                           ldp, 0.5);
+        hipLaunchKernelGGL((sep_gemm<true, 4, SG_FM, SG_FN, true>), dim3(sep_blocks(S.nt[1], S.gs[1])), dim3(256), 0,
+                           c->stream, (const double*)S.ct.as<double>(), ld, tb, (const double*)c->sg_M.as<double>(),
+                           (long)ld, c->sg_Ys.as<double>(), ldp, NP, NP,
+                           reinterpret_cast<const SepTask*>(tb + S.o_task[1]),
+                           reinterpret_cast<const int4*>(tb + S.o_tile[1]), S.nt[1], S.gs[1], 0);
         MCMB_TRY(launch_guard("expansion", sep_blocks(S.nt[2], S.gs[2]), 256));
         SEP_GEMMF(true, 4, SG_FM, SG_FN, S.nt[2], S.gs[2], S.ct.as<double>(), ld, tb, c->sg_Ys.as<double>(), ldp,
                  c->sg_G.as<double>(), ldp, NP, (int)NPP, tb + S.o_task[2], tb + S.o_tile[2], 0);
         MCMB_HIP(hipGetLastError());
-        launches += 5;
+        launches += 4;
     }
 #undef SEP_GEMM
 #undef SEP_GEMMF

@@ -374,3 +374,21 @@ def test_separable_low_rank_non_polynomial_basis(ctx):
     assert ctx.last_basis_rank() == 10
     assert rel_elem(G["separable"], G["separable-pairs"]) < 1e-12
     assert rel_elem(G["separable"], O.gamma_tet(2, L, q, ws["v"], ws["order"])) < 1e-12
+
+
+def test_separable_compressed_wide_basis(ctx):
+    # 32 basis functions spanning 12 directions: their 528 pair products have
+    # rank 78 > 64, so the compressed contractions take the 160 x 96 tiles
+    # and the l3 kernel of 6 column tiles; 400 of the modes
+    L, R = 100, 32
+    ws = W.lspace(L, R, True)
+    rng = np.random.default_rng(34)
+    q = rng.standard_normal((L + 1, 12)) @ rng.standard_normal((12, R))
+    order = ws["order"][rng.choice(len(ws["order"]), 400, replace=False)]
+    ctx.lspace_set(2, L, q, ws["v"], order)
+    G = _gram_paths(ctx, modalcmb.FP64, ("separable", "separable-pairs"))
+    ctx.gamma_tet(modalcmb.FP64)
+    assert ctx.last_basis_rank() == 78
+    assert rel_elem(G["separable"], G["separable-pairs"]) < 1e-12
+    assert rel_elem(G["separable"], O.gamma_tet(2, L, q, ws["v"], order)) < 1e-12
+    assert np.array_equal(G["separable"], G["separable"].T)

@@ -953,13 +953,11 @@ __global__ void sep_assemble(const double* __restrict__ Gs, const long* __restri
     gamma[(long)nn * n + m] = acc;
 }
 
-// Gamma from the fully symmetrised tensor G[u][{v,w}] (the same value for
-// every order of the pairs u, v, w: the expansion's output), over 32 x 32
-// tiles (bm <= bn) of Gamma.  For each matching sigma the pair that holds
-// n's last index goes into the inner position {v, w}, so the modes of a
-// wave, consecutive in their last index, read neighbouring entries of one G
-// row.  Entries m <= nn are computed; the tile and (off the diagonal) its
-// mirror are written from LDS along rows, so Gamma is exactly symmetric.
+// Gamma from the fully symmetrised tensor G[x][{y,z}] (the expansion's
+// output, computed where x <= y), over 32 x 32 tiles (bm <= bn) of Gamma:
+// for each matching sigma the sorted triple x <= y <= z of its pairs is read.
+// Entries m <= nn are computed; the tile and (off the diagonal) its mirror
+// are written from LDS along rows, so Gamma is exactly symmetric.
 __global__ __launch_bounds__(256) void sep_assemble_sym(const double* __restrict__ G, long ldg, int R,
                                                         const int* __restrict__ modes, int n, int nt,
                                                         double* __restrict__ gamma)
@@ -985,13 +983,12 @@ __global__ __launch_bounds__(256) void sep_assemble_sym(const double* __restrict
             const int a[3] = {pm & 255, (pm >> 8) & 255, (pm >> 16) & 255};
 #pragma unroll
             for (int s = 0; s < 6; ++s) {
-                // slot x of m meets slot P6[s][x] of n; xl: the slot meeting n's last index
-                const int xl = P6[s][0] == 2 ? 0 : P6[s][1] == 2 ? 1 : 2;
-                const int x0 = xl == 0 ? 1 : 0, x1 = xl == 2 ? 1 : 2;
-                const int u = pair_index(min(a[x0], b[P6[s][x0]]), max(a[x0], b[P6[s][x0]]), R);
-                const int v = pair_index(min(a[x1], b[P6[s][x1]]), max(a[x1], b[P6[s][x1]]), R);
-                const int w = pair_index(min(a[xl], b[2]), max(a[xl], b[2]), R);
-                acc += G[(long)u * ldg + pair_index(min(v, w), max(v, w), R * (R + 1) / 2)];
+                // slot x of m meets slot P6[s][x] of n
+                const int u = pair_index(min(a[0], b[P6[s][0]]), max(a[0], b[P6[s][0]]), R);
+                const int v = pair_index(min(a[1], b[P6[s][1]]), max(a[1], b[P6[s][1]]), R);
+                const int w = pair_index(min(a[2], b[P6[s][2]]), max(a[2], b[P6[s][2]]), R);
+                const int x = min(u, min(v, w)), z = max(u, max(v, w)), y = u + v + w - x - z;
+                acc += G[(long)x * ldg + pair_index(y, z, R * (R + 1) / 2)];
             }
             acc *= (1.0 / 36.0);
         }
@@ -1305,9 +1302,18 @@ static int separable_compress(mcmb_ctx* c, PointSet& L, int nl, int NP, int ld, 
             for (int n0c = 0; n0c < NP; n0c += SG_TN)
                 for (int m0 = 0; m0 < NP && m0 < n0c + SG_TN; m0 += SG_TM) g2.push_back(make_int4(a, m0, n0c, 0));
         }
+        // G[x][{y,z}] only where x <= y (the assembly reads sorted triples):
+        // tiles whose rows all lie above the pair columns' last y are skipped
         t3[0] = SepTask{0, 0, 0, r};
+        std::vector<int> ylast((NPP + SG_TN - 1) / SG_TN);
+        for (int y = 0; y < NP; ++y)
+            for (int z = y; z < NP; ++z) {
+                const long pidx = pair_index(y, z, NP);
+                ylast[pidx / SG_TN] = y;
+            }
         for (long n0c = 0; n0c < NPP; n0c += SG_TN)
-            for (int m0 = 0; m0 < NP; m0 += SG_TM) g3.push_back(make_int4(0, m0, (int)n0c, 0));
+            for (int m0 = 0; m0 < NP && m0 <= ylast[n0c / SG_TN]; m0 += SG_TM)
+                g3.push_back(make_int4(0, m0, (int)n0c, 0));
         std::vector<int> buf;
         S.o_task[0] = append(buf, t1);
         S.o_tile[0] = append(buf, g1);

@@ -238,7 +238,8 @@ int mcmb_device_free(mcmb_ctx* ctx, void* dev);
 int mcmb_device_to_host(mcmb_ctx* ctx, void* host, const void* dev, size_t bytes);
 
 /* Per-phase milliseconds of the last call that records phases (the FP64
- * separable Gram: weights, pair products, l3 / l2 / l1 contractions, assembly),
+ * separable Gram: basis compression on the first call of a setup, pair
+ * products or weights, l3 / l2 / l1 contractions, expansion, assembly),
  * each between HIP events on the context stream.  Writes up to cap entries of
  * ms[] and names[] (static strings) and returns the phase count (0 if the last
  * such call recorded none).  No reference counterpart. */

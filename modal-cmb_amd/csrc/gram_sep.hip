@@ -420,12 +420,14 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
         bool in;
     };
     // a thread's weights of a chunk are formed in groups of two (elements 2 gr
-    // and 2 gr + 1), one group's table reads in flight at a time.  Along the
-    // thread's row every table index is affine in the element (g = (i + j +
-    // l3) / 2 moves by 2 NW): the reads go through per-chunk base pointers
-    // with constant offsets, into tables with zero guards (f: ftab_guard
-    // doubles before index 0; f and v: TAB_TAIL after the end), so elements
-    // outside the row read in bounds and are dropped by their flag.
+    // and 2 gr + 1), one group's table reads in flight at a time.  With one
+    // wave (WIN) the reads come from the chunk's LDS windows.  With more
+    // waves, every table index is affine in the element along the thread's
+    // row (g = (i + j + l3) / 2 moves by 2 NW): the reads go through
+    // per-chunk base pointers with constant offsets.  Either way the tables
+    // carry zero guards (f: ftab_guard doubles before index 0; f and v:
+    // TAB_TAIL after the end), so elements outside the row read in bounds and
+    // are dropped by their flag.
     constexpr int NG = WPT / 2;
     auto gather = [&](int c, int gr, Tab (&t)[2]) {
         const int l30 = kmin + 32 * c + 2 * kk0;   // the thread's l3 at element 0

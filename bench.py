@@ -14,6 +14,8 @@ timed on the host around the call.
 Reported beside it:
   gamma_kernel_ms     HIP-event time of the call and its phases
   basis_rank          columns of the compressed pair basis (0: pairs)
+  gamma_first_call_s  the first call on the setup (the warm-up), which also
+                      factorises the basis and builds the host row tables
   fp64_separable_pairs_s
                       the same separable contraction over the 465 pair
                       products themselves (MCMB_SEP_COMPRESS=0), per call
@@ -330,7 +332,16 @@ def main():
 
     # headline: FP64 separable Gamma into a device buffer
     d_gamma = ctx.device_alloc(n * n * 8)
-    for _ in range(args.warmup):
+    # the first warm-up call on the setup also factorises the basis and
+    # builds the host row tables (kept for later calls)
+    t_first = None
+    if args.warmup > 0:
+        ctx.sync()
+        t = time.time()
+        ctx.gamma_tet_device(modalcmb.FP64, d_gamma)
+        ctx.sync()
+        t_first = round(time.time() - t, 4)
+    for _ in range(max(0, args.warmup - 1)):
         ctx.gamma_tet_device(modalcmb.FP64, d_gamma)
     log(f"warmup done ({ctx.last_kernel_ms()[0]:.2f} ms kernel)")
     ctx.sync()
@@ -461,6 +472,7 @@ def main():
         "gamma_kernel_ms": round(k_step, 3),
         "phases_ms": phases,
         "basis_rank": rank,
+        "gamma_first_call_s": t_first,
         "fp64_separable_pairs_s": round(t_pairs, 5),
         "gamma_with_host_copy_s": round(t_host, 4),
         "fp64_factored_s": round(t_fac, 3),

@@ -254,21 +254,20 @@ def bench_projection(modalcmb, W, args):
 
 
 def traffic_program(config):
-    """Calibration copies and one FP64 Gamma, nothing else on the device.
-    torch (only used for the plain copy kernel) is imported before the
-    library so that the process holds one HIP runtime."""
-    import torch
-    x = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
-    y = torch.empty_like(x)
-    for _ in range(3):
-        y.copy_(x)
-    torch.cuda.synchronize()
-    del x, y
+    """Calibration copies (the library's plain copy kernel, 1 GiB read and 1
+    GiB written, three times) and one FP64 Gamma, nothing else on the device;
+    one HIP runtime (the library's) in the process."""
     import modalcmb
     from modalcmb import workloads as W
     cfg = W.WORKLOADS[config]
     ws = W.lspace(cfg["L"], cfg["R"], cfg["modulated"])
     with modalcmb.Context(0) as ctx:
+        gib = 1 << 30
+        x, y = ctx.device_alloc(gib), ctx.device_alloc(gib)
+        for _ in range(3):
+            ctx.device_copy(y, x, gib)
+        ctx.device_free(x)
+        ctx.device_free(y)
         ctx.lspace_set(2, cfg["L"], ws["q"], ws["v"], ws["order"])
         t = time.time()
         ctx.gamma_tet(modalcmb.FP64)

@@ -42,6 +42,11 @@ extern "C" {
 
 #define MCMB_FP32   2   /* projection only: FP32 MFMA k-sum, FP64 mode sums    */
 
+/* x integration of the cross-Gamma */
+#define MCMB_SPLINE_FMM        0  /* integrate_cubic_spline (calculate_gamma_2d)    */
+#define MCMB_SPLINE_NATURAL    1  /* gsl_interp_cspline (calculate_gamma)           */
+#define MCMB_SPLINE_TRAPEZIUM  2  /* integrate() with TRAPEZIUM (gamma_3d_host)     */
+
 /* geometric factor h^2 */
 #define MCMB_H2_EXACT      0  /* (2l1+1)(2l2+1)(2l3+1)/(4pi) (l1 l2 l3;0 0 0)^2   */
 #define MCMB_H2_ASYMPTOTIC 1  /* calculate_geometric.c:59-92                     */
@@ -50,6 +55,16 @@ typedef struct mcmb_ctx mcmb_ctx;
 
 /* tetrapyd_tools.h:31-37 */
 typedef struct { int i_bgn; int j_bgn; int k_bgn; int loops; } tetrapyd_limits;
+/* calculate_gamma_3d.h:12-29: per-thread data of the 3D Gamma; the library
+ * reads and writes only the walk bounds i0..k1 (bar is the reference's
+ * barrier_t pointer, kept for the layout). */
+typedef struct {
+    int i0; int j0; int k0; int i1; int j1; int k1;
+    int xsize;
+    double* mvec; double* xvec; double* yvec; double* xdiff; double* ixdiff;
+    double* intgrlvec; double* plijkz;
+    void* bar;
+} thread_g3d_data_t;
 /* global.h:32-36 */
 typedef struct { double a1; double a2; double a3; } shape_params;
 
@@ -86,6 +101,13 @@ int mcmb_gamma_tet(mcmb_ctx* ctx, int precision, double* gamma_host, double* gam
 int mcmb_gamma_tet_slice(mcmb_ctx* ctx, int precision, int rank, int nranks, double* gamma_host,
                          double* gamma_dev, double* work_fraction);
 
+/* Partial Gram over the points with l1 in [l1_lo, l1_hi) (a contiguous range of
+ * the walk: the i-rows the loop nest of calculate_orthol_3D visits,
+ * orthogonalisation_late.c:548-615); ranges that tile [lmin, lmax] sum to
+ * mcmb_gamma_tet.  No reference counterpart: checked i-row samples. */
+int mcmb_gamma_tet_lrange(mcmb_ctx* ctx, int precision, int l1_lo, int l1_hi, double* gamma_host,
+                          double* gamma_dev);
+
 /* The same Gram through the Gauss-Legendre factorisation (calculate_orthol,
  * orthogonalisation_late.c:239-336) with npt nodes (npt <= 0: 3 lmax/2 + 1). */
 int mcmb_gamma_gl(mcmb_ctx* ctx, int npt, double* gamma_host, double* gamma_dev);
@@ -101,9 +123,50 @@ long mcmb_lspace_npoints(mcmb_ctx* ctx);
  * (i,j,k) triples; writes min(cap, N) triples, returns N or a negative status.
  * For bit-exact parity with get_ijk_next_XXX (tetrapyd_tools.c:104-142). */
 long mcmb_lspace_points(mcmb_ctx* ctx, int* ijk, long cap);
+/* The same from walk index p0: writes min(cap, N - p0) triples, returns N. */
+long mcmb_lspace_points_at(mcmb_ctx* ctx, long p0, int* ijk, long cap);
 
 /* Per-point weights w_p (walk order) computed on the device, up to cap. */
 long mcmb_lspace_weights(mcmb_ctx* ctx, double* w, long cap);
+
+/* ---- cross-Gamma against the projected primordial basis q-tilde --------- */
+/* Load q-tilde as read_qtilde leaves it (fixed_data.c:2052-2061, 2246-2290):
+ * qt[p][l][x] for p < np (primordial basis index), multipole l = 0..lmax_q
+ * (the table's l index is the multipole), x < nx on the increasing grid
+ * xvec[nx] (get_qtilde_xvec); the primordial mode table order_prim[nmodes][3]
+ * (find_perm_prim); the TT spectra cl[l], nl[l], bl[l], l = 0..lmax_q
+ * (get_cl_TT, get_noise_TT, get_beam_TT); deltaphi (model:delta_phi).  The
+ * late side is the l-space setup (mcmb_lspace_set: late basis by multipole,
+ * late mode table = find_perm_late_TTT, range [eflag_T_lmin, eflag_T_lmax]). */
+int mcmb_qtilde_set(mcmb_ctx* ctx, int np, int lmax_q, int nx, const double* qt, const double* xvec,
+                    int nmodes, const int* order_prim, const double* cl, const double* nl,
+                    const double* bl, double deltaphi);
+/* np, lmax_q, nx, nmodes and generation of the q-tilde setup (outputs may be NULL). */
+int mcmb_qtilde_info(mcmb_ctx* ctx, int* np, int* lmax_q, int* nx, int* nmodes, long* generation);
+/* Copy the x grid (nx doubles). */
+int mcmb_qtilde_xvec(mcmb_ctx* ctx, double* xvec);
+/* Gamma_3D of gamma_3d_host (gamma_mpi.c:30-152, calculate_gamma_TTT_3D_nested
+ * calculate_gamma_3d.c:274-440, calculate_sijk_TTT 813-860): the rank's share
+ * of the walk of T(2, lmax_q) split as decompose_gamma_TTT_3D_mpi splits its
+ * point count, z = permsix h^2 lw lw lw with lw(l) = (2l+1)^(-1/3) (h^2 by the
+ * l-space setup's h2_mode; the reference's is MCMB_H2_ASYMPTOTIC), late mode
+ * products (0 unless all legs are in the late range), and the x integral by
+ * the trapezium rule of q3 = (2l+1)^(1/6)/sqrt(C_l + N_l/b_l^2) q-tilde with
+ * samples x < 10 dropped where |y| > 1e-18; scaled by 6 dphi^2.  Gamma is
+ * n_late x n_prim (rows late modes), written row-major to gamma_host and/or
+ * the device buffer gamma_dev; the shares of all ranks sum to rank 0 of 1. */
+int mcmb_gamma_cross_3d(mcmb_ctx* ctx, int rank, int nranks, double* gamma_host, double* gamma_dev);
+/* Gamma_2D of calculate_gamma_2d (spline MCMB_SPLINE_FMM) or calculate_gamma
+ * (MCMB_SPLINE_NATURAL) for every (m, n) at once (calculate_gamma.c:48-303,
+ * 511-555, 772-1016): M[p][q][x][s] = sum_l P_l(mu_s) (2l+1)^(5/6) b_l /
+ * sqrt(b_l^2 C_l + N_l) qt[p][l][x] b_q(l) over the late range, npt
+ * Gauss-Legendre nodes (npt <= 0: 3 lmax/2 + 1), per x the node sum of the
+ * 3 x 3 permanents, the same x < 10 rule, the spline integral in x, scaled by
+ * dphi^2/(8 pi).  Needs lmax_q >= the late lmax. */
+int mcmb_gamma_cross_gl(mcmb_ctx* ctx, int spline, int npt, double* gamma_host, double* gamma_dev);
+/* Per-sample weights c[x] of the x rules above on the grid x[n] (host only):
+ * sum_x c[x] y[x] is the rule's integral of y over [x_0, x_{n-1}]. */
+int mcmb_spline_weights(int kind, int n, const double* x, double* c);
 
 /* ---- FP64 orthonormalisation on the GPU (ortho_prim_mpi.c:198-284) ------- */
 /* norm = sqrt(diag); C = D^-1 Gamma D^-1 = L L^T; ortho = symmetrised D L,
@@ -236,6 +299,10 @@ int mcmb_last_kernel_ms(mcmb_ctx* ctx, double* ms, int* launches);
 int mcmb_device_alloc(mcmb_ctx* ctx, size_t bytes, void** dev);
 int mcmb_device_free(mcmb_ctx* ctx, void* dev);
 int mcmb_device_to_host(mcmb_ctx* ctx, void* host, const void* dev, size_t bytes);
+/* Copy bytes (a multiple of 16, 16-byte aligned buffers) from src to dst on
+ * the device with a plain copy kernel on the context stream; no reference
+ * counterpart (a known-size read + write to calibrate HBM byte counters). */
+int mcmb_device_copy(mcmb_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* Per-phase milliseconds of the last call that records phases (the FP64
  * separable Gram: basis compression on the first call of a setup, pair
@@ -273,7 +340,16 @@ void divide_tasks(long int tasks, int nranks, long int* tasks_per_rank);
 void mcmb_next_ijk_bound(int lsize, int i0, int j0, int k0, int* i1, int* j1, int* k1,
                          int final_i, int final_j, int final_k, int* blocksize,
                          int maxblocksize);
-/* calculate_gamma_3d.h:72-73 (lsize taken from the bound l-space setup) */
+/* calculate_gamma_3d.c:92-211 with explicit lsize and thread count (host
+ * only): fills td[0 .. nthreads) with each thread's walk bounds. */
+void mcmb_decompose_gamma_TTT_3D(int lsize, int rank, int numranks, double workshare, int nthreads,
+                                 thread_g3d_data_t* td);
+/* calculate_gamma_3d.h:46 -- the same over the bound context (lsize =
+ * get_qtilde_T_lsize() of its q-tilde setup, else lmax + 1 of its l-space
+ * setup), nthreads = omp_get_num_threads() when the program links OpenMP,
+ * else 1. */
+void decompose_gamma_TTT_3D_mpi(thread_g3d_data_t* thread_data, int rank, int numranks, double workshare);
+/* calculate_gamma_3d.h:72-73 (lsize as decompose_gamma_TTT_3D_mpi takes it) */
 void get_next_ijk_bound(int i0, int j0, int k0, int* i1, int* j1, int* k1,
                         int final_i, int final_j, int final_k, int* blocksize,
                         int maxblocksize);
@@ -285,12 +361,41 @@ double calculate_geometric(int l1, int l2, int l3);
 int basis_functions_legendre(double** array, int size, int pmax, double min, double max,
                              double* values);
 /* gamma_mpi.c:30 -- fills the caller's row-major gamma_size x gamma_size
- * buffer (zeroed first) with this rank's share of the FP64 tetrapyd Gram of
- * the bound context (numranks <= 1: the whole tetrapyd); summing the buffers
- * of all ranks gives the Gram, as the MPI_Reduce at gamma_mpi.c:764 does.
- * workshare (the Xeon Phi offload split) is accepted and ignored. */
+ * buffer (zeroed first) with this rank's share of the 3D cross-Gamma of the
+ * bound context (mcmb_gamma_cross_3d: late basis of the l-space setup against
+ * the q-tilde setup, 6 dphi^2 included; numranks <= 1: the whole walk);
+ * summing the buffers of all ranks gives the whole, as the MPI_Reduce at
+ * gamma_mpi.c:764 does.  workshare (the Xeon Phi offload split) is accepted
+ * and ignored.  Without a q-tilde setup the buffer stays zero and
+ * mcmb_legacy_status() is MCMB_ERR_STATE. */
 void gamma_3d_host(double* gamma_flat, int gamma_size, int rank, int numranks,
                    double workshare);
+/* calculate_gamma.c:773 / 936 -- result[0] = Gamma_TTT(m, n) of the GL
+ * cross-Gamma (FMM spline in x / natural spline in x) of the bound context,
+ * computed whole on the first call and cached until a setup changes; safe to
+ * call from several threads at once (the first call fills the cache under a
+ * lock).  Only result[0] is written. */
+void calculate_gamma_2d(int m, int n, double* result);
+void calculate_gamma(int m, int n, double* result);
+/* calculate_gamma.c:48, 255, 354, 425 -- the M_ij tables are formed on the
+ * GPU inside the first calculate_gamma(_2d); these check the setups
+ * (precompute_gammaMij_T also forms the FMM-spline Gamma), sync_gammaMij has
+ * nothing to sum on one GPU. */
+void init_gamma_glint(void);
+void precompute_gammaMij_T(void);
+void calculate_gammaMij_T(int i, int j);
+void sync_gammaMij(void);
+/* fixed_data.c get_qtilde_xsize / get_qtilde_xvec / get_qtilde_T_lsize of the
+ * bound q-tilde setup (lsize = lmax_q + 1) */
+int get_qtilde_xsize(void);
+void get_qtilde_xvec(double* xvec);
+int get_qtilde_T_lsize(void);
+/* orthogonalisation_late.c:23, 99, 159 -- as above for the auto-Gram:
+ * init_orthol_glint returns 0, calculate_orthoMij_T checks state (every M_ij
+ * is formed by the first calculate_orthol), sync_orthoMij has nothing to sum. */
+double init_orthol_glint(void);
+void calculate_orthoMij_T(int i, int j);
+void sync_orthoMij(void);
 /* orthogonalisation_late.c:338 -- result[0] = Gamma_mn (TTT) of the bound
  * context's cached FP64 tetrapyd Gram (computed on first use). */
 int calculate_orthol_3D(int m, int n, double* result);

@@ -128,7 +128,7 @@ int mcmb_create(mcmb_ctx** out, int device)
     if (e == hipSuccess) e = hipEventCreate(&c->ev1);
     for (int k = 0; k <= MAX_PHASES && e == hipSuccess; ++k) e = hipEventCreate(&c->evp[k]);
     if (e != hipSuccess) {
-        delete c;
+        mcmb_destroy(c);
         return fail(MCMB_ERR_HIP, "mcmb_create: %s", hipGetErrorString(e));
     }
     *out = c;
@@ -277,6 +277,33 @@ int mcmb_device_free(mcmb_ctx* c, void* dev)
     return MCMB_OK;
 }
 
+}  // extern "C"
+
+namespace mcmb {
+// dst[i] = src[i] over n 16-byte words, a grid-stride loop
+__global__ void copy16_kernel(const int4* __restrict__ src, int4* __restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+}  // namespace mcmb
+
+extern "C" {
+
+int mcmb_device_copy(mcmb_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    if (!c || !dst || !src || (bytes & 15) || (((uintptr_t)dst | (uintptr_t)src) & 15))
+        return fail(MCMB_ERR_ARG, "mcmb_device_copy: null or unaligned buffer, or size not a multiple of 16");
+    MCMB_TRY(set_device(c));
+    const size_t n = bytes / 16;
+    if (n) {
+        const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+        hipLaunchKernelGGL(copy16_kernel, dim3(blocks), dim3(256), 0, c->stream, (const int4*)src, (int4*)dst, n);
+        MCMB_HIP(hipGetLastError());
+    }
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return MCMB_OK;
+}
+
 int mcmb_device_to_host(mcmb_ctx* c, void* host, const void* dev, size_t bytes)
 {
     if (!c || (bytes && (!host || !dev))) return fail(MCMB_ERR_ARG, "mcmb_device_to_host: null argument");
@@ -338,6 +365,41 @@ int mcmb_gamma_tet_slice(mcmb_ctx* c, int precision, int rank, int nranks, doubl
     MCMB_TRY(set_device(c));
     int n = c->ls.n, lo, hi;
     lspace_slice(c->ls, 32, rank, nranks, &lo, &hi, work_fraction);
+    double* dst = gamma_dev;
+    if (!dst) {
+        MCMB_TRY(c->d_gamma_tmp.alloc(sizeof(double) * (size_t)n * n));
+        dst = c->d_gamma_tmp.as<double>();
+    }
+    if (lo >= hi) {
+        MCMB_HIP(hipMemsetAsync(dst, 0, sizeof(double) * (size_t)n * n, c->stream));
+        c->last_ms = 0.0;
+        c->last_launches = 0;
+    } else {
+        MCMB_TRY(gram_tet_launch(c, c->ls, precision, dst, lo, hi));
+    }
+    if (gamma_host)
+        MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return MCMB_OK;
+}
+
+int mcmb_gamma_tet_lrange(mcmb_ctx* c, int precision, int l1_lo, int l1_hi, double* gamma_host,
+                          double* gamma_dev)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_gamma_tet_lrange: null context");
+    if (!c->ls.ready) return fail(MCMB_ERR_STATE, "mcmb_gamma_tet_lrange: call mcmb_lspace_set first");
+    if (precision != MCMB_FP64 && precision != MCMB_MIXED)
+        return fail(MCMB_ERR_ARG, "mcmb_gamma_tet_lrange: precision %d", precision);
+    if (l1_hi < l1_lo) return fail(MCMB_ERR_ARG, "mcmb_gamma_tet_lrange: l1 range [%d, %d)", l1_lo, l1_hi);
+    MCMB_TRY(set_device(c));
+    // rows are in walk order, l1 ascending: [l1_lo, l1_hi) is one contiguous row range
+    const std::vector<RowDesc>& rows = c->ls.rows;
+    int lo = 0, hi = (int)rows.size();
+    while (lo < hi && rows[lo].i < l1_lo) ++lo;
+    int e = lo;
+    while (e < hi && rows[e].i < l1_hi) ++e;
+    hi = e;
+    int n = c->ls.n;
     double* dst = gamma_dev;
     if (!dst) {
         MCMB_TRY(c->d_gamma_tmp.alloc(sizeof(double) * (size_t)n * n));

@@ -154,46 +154,100 @@ __global__ __launch_bounds__(256) void gl_gamma(const double* __restrict__ Mp, c
         }
 }
 
-int gram_gl_launch(mcmb_ctx* c, int npt, double* d_gamma)
+// Tile list of the node sum: the T (T + 1) / 2 pairs (I <= J) of 64-mode
+// tiles, I-major, one thread per tile.
+__global__ void gl_tiles(int T, int2* __restrict__ tiles)
+{
+    int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= T * (T + 1) / 2) return;
+    int I = 0;
+    while (t >= T - I) { t -= T - I; ++I; }
+    tiles[blockIdx.x * 256 + threadIdx.x] = make_int2(I, I + t);
+}
+
+// Device buffers of one GL Gram: node table, pair products, M and tiles.
+struct GlBufs {
+    double *x, *w, *Pt, *U, *M;
+    int2* tiles;
+};
+
+static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Bytes of the buffers of one GL Gram, each region aligned to 256 bytes.
+static size_t gl_bytes(int npt, int nl, int npair, int ntiles)
+{
+    return 2 * round256(sizeof(double) * (size_t)npt) + round256(sizeof(double) * (size_t)nl * npt) +
+           round256(sizeof(double) * (size_t)nl * npair) + round256(sizeof(double) * (size_t)npair * npt) +
+           round256(sizeof(int2) * (size_t)ntiles);
+}
+
+static GlBufs gl_carve(char* base, int npt, int nl, int npair)
+{
+    GlBufs b;
+    b.x = reinterpret_cast<double*>(base);
+    base += round256(sizeof(double) * (size_t)npt);
+    b.w = reinterpret_cast<double*>(base);
+    base += round256(sizeof(double) * (size_t)npt);
+    b.Pt = reinterpret_cast<double*>(base);
+    base += round256(sizeof(double) * (size_t)nl * npt);
+    b.U = reinterpret_cast<double*>(base);
+    base += round256(sizeof(double) * (size_t)nl * npair);
+    b.M = reinterpret_cast<double*>(base);
+    base += round256(sizeof(double) * (size_t)npair * npt);
+    b.tiles = reinterpret_cast<int2*>(base);
+    return b;
+}
+
+// The node table x, w (host, npt entries) is copied in, the four kernels run
+// on the context stream, and the call waits for them (x and w may live on the
+// caller's stack).
+static int gl_run(mcmb_ctx* c, int npt, const double* x, const double* w, const GlBufs& b, double* d_gamma)
 {
     LSpace& L = c->ls;
     const int n = L.n, R = L.R, lmax = L.lmax;
-    if (R > GLR) return fail(MCMB_ERR_ARG, "GL Gram supports basis width <= %d (got %d)", GLR, R);
-    if (npt <= 0) npt = 3 * lmax / 2 + 1;
-    std::vector<double> x(npt), w(npt);
-    MCMB_TRY(mcmb_gauss_legendre(npt, x.data(), w.data()));
     const int npair = R * (R + 1) / 2, nl = lmax + 1;
-    DevBuf dx, dw, dPt, dU, dM, dT;
-    MCMB_TRY(dx.alloc(sizeof(double) * npt));
-    MCMB_TRY(dw.alloc(sizeof(double) * npt));
-    MCMB_TRY(dPt.alloc(sizeof(double) * (size_t)nl * npt));
-    MCMB_TRY(dU.alloc(sizeof(double) * (size_t)nl * npair));
-    MCMB_TRY(dM.alloc(sizeof(double) * (size_t)npair * npt));
-    std::vector<int2> tiles;
-    const int T = (n + 63) / 64;
-    for (int I = 0; I < T; ++I)
-        for (int J = I; J < T; ++J) tiles.push_back(make_int2(I, J));
-    MCMB_TRY(dT.alloc(sizeof(int2) * tiles.size()));
-    MCMB_HIP(hipMemcpyAsync(dx.p, x.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipMemcpyAsync(dw.p, w.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipMemcpyAsync(dT.p, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, c->stream));
+    const int T = (n + 63) / 64, ntiles = T * (T + 1) / 2;
+    MCMB_HIP(hipMemcpyAsync(b.x, x, sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(b.w, w, sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(gl_legendre, dim3((npt + 255) / 256), dim3(256), 0, c->stream, dx.as<double>(), npt, L.lmin,
-                       lmax, L.d_v.as<double>(), dPt.as<double>());
-    hipLaunchKernelGGL(gl_pairs, dim3((unsigned)((nl * npair + 255) / 256)), dim3(256), 0, c->stream, L.d_q.as<double>(),
-                       L.ldq, R, lmax, dU.as<double>(), npair);
-    hipLaunchKernelGGL(gl_mgemm, dim3((npt + 63) / 64, (npair + 31) / 32), dim3(256), 0, c->stream, dPt.as<double>(),
-                       dU.as<double>(), npt, npair, nl, dM.as<double>());
-    hipLaunchKernelGGL(gl_gamma, dim3((unsigned)tiles.size()), dim3(256), 0, c->stream, dM.as<double>(), dw.as<double>(),
-                       npt, R, L.d_modes.as<int>(), n, dT.as<int2>(), d_gamma);
+    hipLaunchKernelGGL(gl_tiles, dim3((ntiles + 255) / 256), dim3(256), 0, c->stream, T, b.tiles);
+    hipLaunchKernelGGL(gl_legendre, dim3((npt + 255) / 256), dim3(256), 0, c->stream, (const double*)b.x, npt,
+                       L.lmin, lmax, L.d_v.as<double>(), b.Pt);
+    hipLaunchKernelGGL(gl_pairs, dim3((unsigned)((nl * npair + 255) / 256)), dim3(256), 0, c->stream,
+                       L.d_q.as<double>(), L.ldq, R, lmax, b.U, npair);
+    hipLaunchKernelGGL(gl_mgemm, dim3((npt + 63) / 64, (npair + 31) / 32), dim3(256), 0, c->stream,
+                       (const double*)b.Pt, (const double*)b.U, npt, npair, nl, b.M);
+    hipLaunchKernelGGL(gl_gamma, dim3((unsigned)ntiles), dim3(256), 0, c->stream, (const double*)b.M,
+                       (const double*)b.w, npt, R, L.d_modes.as<int>(), n, (const int2*)b.tiles, d_gamma);
     MCMB_HIP(hipGetLastError());
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
-    c->last_launches = 4;
+    c->last_launches = 5;
     return MCMB_OK;
+}
+
+static int gl_check(mcmb_ctx* c, int* npt)
+{
+    LSpace& L = c->ls;
+    if (L.R > GLR) return fail(MCMB_ERR_ARG, "GL Gram supports basis width <= %d (got %d)", GLR, L.R);
+    if (*npt <= 0) *npt = 3 * L.lmax / 2 + 1;
+    return MCMB_OK;
+}
+
+int gram_gl_launch(mcmb_ctx* c, int npt, double* d_gamma)
+{
+    LSpace& L = c->ls;
+    MCMB_TRY(gl_check(c, &npt));
+    const int n = L.n, R = L.R, nl = L.lmax + 1, npair = R * (R + 1) / 2;
+    const int T = (n + 63) / 64;
+    std::vector<double> x(npt), w(npt);
+    MCMB_TRY(mcmb_gauss_legendre(npt, x.data(), w.data()));
+    DevBuf buf;
+    MCMB_TRY(buf.alloc(gl_bytes(npt, nl, npair, T * (T + 1) / 2)));
+    return gl_run(c, npt, x.data(), w.data(), gl_carve(buf.as<char>(), npt, nl, npair), d_gamma);
 }
 
 }  // namespace mcmb

@@ -52,10 +52,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // 16-byte chunk c (0..3) of operand row m, XOR-swizzled so that the rows one
 // ds_read_b128 / ds_write_b128 pass touches land on distinct banks.
-#ifdef MCMB_SWZ8
-__device__ __forceinline__ int swz(int m, int c) { return (m << 2) + (c ^ ((m >> 1) & 3)); }
-#else
-// conflict-free both for 8-lane passes over 32 banks (rows m..m+7) and for
+// Conflict-free both for 8-lane passes over 32 banks (rows m..m+7) and for
 // 16-lane passes over 64 banks (rows m..m+15): the XOR key takes four
 // distinct values over bits (1,2) of m for fixed bit 3, and over bits (2,3)
 // for fixed bit 1.
@@ -64,7 +61,6 @@ __device__ __forceinline__ int swz(int m, int c)
     const int key = ((m >> 2) & 1) | ((((m >> 1) ^ (m >> 3)) & 1) << 1);
     return (m << 2) + (c ^ key);
 }
-#endif
 
 // Row stride of the staged chunk basis: 36 floats puts the rows of eight
 // consecutive third indices t on eight distinct 4-bank groups, so the

@@ -804,9 +804,9 @@ __global__ __launch_bounds__(256) void sep_mgs_norms(const double* __restrict__ 
 // Step a of pivoted modified Gram-Schmidt on the residual columns Uc (updated
 // in place).  nin[f] is the squared residual norm of column f after step a - 1
 // (-1 once f has been a pivot); the pivot p is the largest (the smallest f on
-// ties).  The factorisation stops (rank[0] = a) when the pivot's squared norm
-// is at most tol2 times the first pivot's (kept in n0[0]), and gives up
-// (rank[0] = -2) at a = amax.  Otherwise, with v = Uc[p] / |Uc[p]|: Pc[a] = v;
+// ties), and its squared norm is recorded in piv[a].  The factorisation stops
+// (rank[0] = a) when the pivot's squared norm is at most tol2 times the first
+// pivot's (kept in n0[0]), and gives up (rank[0] = -2) at a = amax.  Otherwise, with v = Uc[p] / |Uc[p]|: Pc[a] = v;
 // for every column that has not been a pivot Ct[a][f] = v . Uc[f], Uc[f] -=
 // Ct[a][f] v and nout[f] = |Uc[f]|^2; Ct[a][p] = |Uc[p]| and Ct[a][f] = 0 for
 // earlier pivots.  Every workgroup takes the same decisions from the same nin
@@ -816,8 +816,8 @@ __global__ __launch_bounds__(256) void sep_mgs_norms(const double* __restrict__ 
 __global__ __launch_bounds__(256) void sep_mgs_step(double* __restrict__ Uc, int nlp, int NP,
                                                     const double* __restrict__ nin, double* __restrict__ nout,
                                                     int a, int amax, double tol2, double* __restrict__ n0,
-                                                    int* __restrict__ rank, double* __restrict__ Pc,
-                                                    double* __restrict__ Ct, int ldct)
+                                                    int* __restrict__ rank, double* __restrict__ piv,
+                                                    double* __restrict__ Pc, double* __restrict__ Ct, int ldct)
 {
     __shared__ double sv[256];
     __shared__ int si[256];
@@ -849,6 +849,7 @@ __global__ __launch_bounds__(256) void sep_mgs_step(double* __restrict__ Uc, int
     const double n2 = sv[0];
     const int p = si[0];
     const double ref = a == 0 ? n2 : n0[0];
+    if (blockIdx.x == 0 && tid == 0) piv[a] = n2;
     if (!(n2 > tol2 * ref)) {
         if (blockIdx.x == 0 && tid == 0) *rank = a;
         return;
@@ -1237,6 +1238,11 @@ static L3Shape sep_l3_shape(int NB)
 // zero.  The Legendre bases leave a gap of ~13 decades there: at lmax 2500 and
 // R = 30 the 59th pivot is 1.3e-2 of the first and the 60th 4.9e-16.
 constexpr double SEP_RANK_TOL = 1e-12;
+// The cut is taken only across a gap: the last kept pivot must be at least
+// SEP_RANK_GAP times the first dropped one (column norms).  A basis whose
+// residual norms decay gradually has no numerical rank worth cutting at, and
+// its pairs are contracted as they are.
+constexpr double SEP_RANK_GAP = 1e6;
 
 // Rank-revealing factorisation U = P Ct of the pair products of L into
 // c->sepc, by pivoted modified Gram-Schmidt over the NP columns (one launch
@@ -1247,14 +1253,15 @@ constexpr double SEP_RANK_TOL = 1e-12;
 static int separable_compress(mcmb_ctx* c, PointSet& L, int nl, int NP, int ld, const std::vector<int2>& pxy)
 {
     SepComp& S = c->sepc;
-    if (S.ps == &L && S.gen == L.generation && S.NP == NP) return MCMB_OK;
+    if (S.ps == L.basis_key() && S.gen == L.basis_generation() && S.NP == NP && S.nl == nl) return MCMB_OK;
     S.ps = nullptr;
     S.r = -1;
     S.ldu = 0;
     const int nlp = nl + (nl & 1);
     const int amax = NP / 2;
-    DevBuf uc, nrm, dpxy;
+    DevBuf uc, nrm, dpxy, dpiv;
     MCMB_TRY(uc.alloc(sizeof(double) * (size_t)NP * nlp));
+    MCMB_TRY(dpiv.alloc(sizeof(double) * ((size_t)amax + 2)));
     MCMB_TRY(nrm.alloc(sizeof(double) * (2 * (size_t)NP + 2)));
     MCMB_TRY(dpxy.alloc(sizeof(int2) * (size_t)NP));
     MCMB_TRY(S.pc.alloc(sizeof(double) * (size_t)std::max(1, amax) * nlp));
@@ -1278,12 +1285,22 @@ static int separable_compress(mcmb_ctx* c, PointSet& L, int nl, int NP, int ld, 
         for (int a = a0; a < a0 + 8 && a <= amax; ++a)
             hipLaunchKernelGGL(sep_mgs_step, dim3(ncb), dim3(256), 0, c->stream, uc.as<double>(), nlp, NP,
                                (const double*)(dn + (a & 1) * NP), dn + ((a + 1) & 1) * NP, a, amax,
-                               SEP_RANK_TOL * SEP_RANK_TOL, n0, rank, S.pc.as<double>(), S.ct.as<double>(), ld);
+                               SEP_RANK_TOL * SEP_RANK_TOL, n0, rank, dpiv.as<double>(), S.pc.as<double>(),
+                               S.ct.as<double>(), ld);
         MCMB_HIP(hipGetLastError());
         MCMB_HIP(hipMemcpyAsync(&hr, rank, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         MCMB_HIP(hipStreamSynchronize(c->stream));
     }
-    const int r = hr > 0 ? hr : 0;
+    int r = hr > 0 ? hr : 0;
+    if (r > 0) {
+        // pivots r - 1 (last kept) and r (first dropped), squared norms
+        double pv[2] = {0.0, 0.0};
+        MCMB_HIP(hipMemcpyAsync(pv, dpiv.as<double>() + (r - 1), 2 * sizeof(double), hipMemcpyDeviceToHost,
+                                c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        S.gap = pv[1] > 0.0 ? sqrt(pv[0] / pv[1]) : INFINITY;
+        if (!(pv[0] >= SEP_RANK_GAP * SEP_RANK_GAP * pv[1])) r = 0;
+    }
     if (r > 0) {
         // expansion: T1[(a,b)][z] = sum_g H[g][(a,b)] Ct[g][z]; per a,
         // Y2s[a][{y,z}] = sum_b Ct[b][y] T1[(a,b)][z] (y <= z); G[x][{y,z}] =
@@ -1332,8 +1349,8 @@ static int separable_compress(mcmb_ctx* c, PointSet& L, int nl, int NP, int ld, 
         MCMB_HIP(hipMemcpy(S.tab.p, buf.data(), sizeof(int) * buf.size(), hipMemcpyHostToDevice));
         MCMB_TRY(S.h.alloc(sizeof(double) * (size_t)r * ldh));
     }
-    S.ps = &L;
-    S.gen = L.generation;
+    S.ps = L.basis_key();
+    S.gen = L.basis_generation();
     S.NP = NP;
     S.nl = nl;
     S.nlp = nlp;
@@ -1394,7 +1411,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     SepComp& S = c->sepc;
     bool comp = false;
     if (!(cenv && !strcmp(cenv, "0"))) {
-        if (!(S.ps == &L && S.gen == L.generation && S.NP == NP)) {
+        if (!(S.ps == L.basis_key() && S.gen == L.basis_generation() && S.NP == NP && S.nl == nl)) {
             MCMB_TRY(phase("basis compression"));
             launches += 2 + NP / 2;
         }

@@ -60,17 +60,19 @@ __global__ __launch_bounds__(256) void lspace_weights_kernel(
     }
 }
 
+// Points [p0, p0 + cap) of the walk: one wave per row, out[p - p0].
 __global__ __launch_bounds__(256) void lspace_points_kernel(
-    const RowDesc* __restrict__ rows, const long* __restrict__ rowptr, long nrows, long cap, int kstep,
+    const RowDesc* __restrict__ rows, const long* __restrict__ rowptr, long nrows, long p0, long cap, int kstep,
     int* __restrict__ ijk)
 {
     long row = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= nrows) return;
     RowDesc rd = rows[row];
     long base = rowptr[row];
+    if (base + rd.nk <= p0 || base >= p0 + cap) return;
     for (int t = threadIdx.x & 63; t < rd.nk; t += 64) {
-        long p = base + t;
-        if (p < cap) {
+        long p = base + t - p0;
+        if (p >= 0 && p < cap) {
             ijk[3 * p] = rd.i;
             ijk[3 * p + 1] = rd.j;
             ijk[3 * p + 2] = rd.k0 + kstep * t;
@@ -397,12 +399,17 @@ using namespace mcmb;
 
 extern "C" long mcmb_lspace_points(mcmb_ctx* c, int* ijk, long cap)
 {
+    return mcmb_lspace_points_at(c, 0, ijk, cap);
+}
+
+extern "C" long mcmb_lspace_points_at(mcmb_ctx* c, long p0, int* ijk, long cap)
+{
     if (!c || !c->ls.ready) return fail(MCMB_ERR_STATE, "mcmb_lspace_points: no l-space setup");
-    if (cap < 0 || (cap > 0 && !ijk)) return fail(MCMB_ERR_ARG, "mcmb_lspace_points: bad buffer");
+    if (cap < 0 || (cap > 0 && !ijk) || p0 < 0) return fail(MCMB_ERR_ARG, "mcmb_lspace_points: bad buffer or start");
     MCMB_TRY(set_device(c));
     LSpace& L = c->ls;
     long n = L.npoints;
-    long m = std::min(cap, n);
+    long m = std::min(cap, std::max(0l, n - p0));
     if (m == 0) return n;
     std::vector<long> rp(L.rows.size() + 1, 0);
     for (size_t r = 0; r < L.rows.size(); ++r) rp[r + 1] = rp[r] + L.rows[r].nk;
@@ -412,7 +419,7 @@ extern "C" long mcmb_lspace_points(mcmb_ctx* c, int* ijk, long cap)
     MCMB_HIP(hipMemcpyAsync(rowptr.p, rp.data(), sizeof(long) * rp.size(), hipMemcpyHostToDevice, c->stream));
     long nrows = (long)L.rows.size();
     hipLaunchKernelGGL(lspace_points_kernel, dim3((unsigned)((nrows + 3) / 4)), dim3(256), 0, c->stream,
-                       L.d_rows.as<RowDesc>(), rowptr.as<long>(), nrows, m, L.kstep, out.as<int>());
+                       L.d_rows.as<RowDesc>(), rowptr.as<long>(), nrows, p0, m, L.kstep, out.as<int>());
     MCMB_HIP(hipGetLastError());
     MCMB_HIP(hipMemcpyAsync(ijk, out.p, sizeof(int) * 3 * (size_t)m, hipMemcpyDeviceToHost, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));

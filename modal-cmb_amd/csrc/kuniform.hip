@@ -349,6 +349,8 @@ extern "C" int mcmb_kuniform_gram(mcmb_ctx* c, int precision, long p0, long p1, 
         R.R = c->ku.R;
         R.order = c->ku.order;
         R.q = c->ku.q;
+        R.basis_of = &c->ku;   // same basis: its factorisation is reused across range calls
+        R.basis_gen = c->ku.generation;
         clip_rows(c->ku.rows, p0, p1, R.rows);
         R.npoints = p1 - p0;
         MCMB_TRY(upload_pointset(c, R, R.order));

@@ -4,30 +4,46 @@
  * mcmb_legacy_status / mcmb_last_error) instead of exiting. */
 #include "modal_cmb.h"
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
+/* The caches of the per-(m, n) entry points are filled on first use under
+ * g_mu, so those entry points may be called from several threads at once
+ * (calculate_gamma_2d runs inside an omp for, gamma_mpi.c:672-688). */
+static pthread_mutex_t g_mu = PTHREAD_MUTEX_INITIALIZER;
 static mcmb_ctx* g_ctx = NULL;
-static int g_lsize = 0;          /* lmax + 1 of the bound l-space setup */
-static int g_status = MCMB_OK;   /* last status of a void legacy call */
-static double* g_gamma3d = NULL; /* cached FP64 tetrapyd Gram, n x n */
-static double* g_gammagl = NULL; /* cached GL-factorised Gram, n x n */
+static int g_lsize = 0;                 /* lmax + 1 of the bound l-space setup */
+static _Thread_local int g_status = MCMB_OK;   /* last status of this thread's void legacy call */
+static double* g_gamma3d = NULL;        /* cached FP64 tetrapyd Gram, n x n */
+static double* g_gammagl = NULL;        /* cached GL-factorised Gram, n x n */
+static double* g_cross[2] = {NULL, NULL};   /* cached GL cross-Gamma, FMM / natural spline */
+static long g_cross_gen[2] = {-1, -1};  /* q-tilde generation of each cross cache */
 static int g_n = 0;
-static long g_gen = -1;          /* l-space generation the caches belong to */
-static int g_prim_nblocks = 1;   /* thread count of the last decompose_tetrapyd_prim */
+static long g_gen = -1;                 /* l-space generation the caches belong to */
+static int g_prim_nblocks = 1;          /* thread count of the last decompose_tetrapyd_prim */
+
+static void drop_caches(void)
+{
+    free(g_gamma3d); g_gamma3d = NULL;
+    free(g_gammagl); g_gammagl = NULL;
+    for (int k = 0; k < 2; ++k) { free(g_cross[k]); g_cross[k] = NULL; g_cross_gen[k] = -1; }
+}
 
 int mcmb_legacy_bind(mcmb_ctx* ctx)
 {
+    pthread_mutex_lock(&g_mu);
     g_ctx = ctx;
-    free(g_gamma3d); g_gamma3d = NULL;
-    free(g_gammagl); g_gammagl = NULL;
+    drop_caches();
     g_n = 0;
     g_gen = -1;
     g_status = MCMB_OK;
+    pthread_mutex_unlock(&g_mu);
     return MCMB_OK;
 }
 
-/* Refresh sizes from the bound context; drop caches of an older setup. */
+/* Refresh sizes from the bound context; drop caches of an older setup.  Called
+ * with g_mu held. */
 static int refresh(void)
 {
     int lmin, lmax, n;
@@ -36,13 +52,21 @@ static int refresh(void)
     int st = mcmb_lspace_info(g_ctx, &lmin, &lmax, &n, &gen);
     if (st != MCMB_OK) return st;
     if (gen != g_gen) {
-        free(g_gamma3d); g_gamma3d = NULL;
-        free(g_gammagl); g_gammagl = NULL;
+        drop_caches();
         g_gen = gen;
     }
     g_lsize = lmax + 1;
     g_n = n;
     return MCMB_OK;
+}
+
+/* lsize of the walk of the 3D Gamma: get_qtilde_T_lsize() of the q-tilde
+ * setup when there is one, else lmax + 1 of the l-space setup. */
+static int walk_lsize(void)
+{
+    int lq;
+    if (g_ctx && mcmb_qtilde_info(g_ctx, NULL, &lq, NULL, NULL, NULL) == MCMB_OK) return lq + 1;
+    return g_lsize;
 }
 
 int mcmb_legacy_status(void) { return g_status; }
@@ -217,8 +241,65 @@ out:
 void get_next_ijk_bound(int i0, int j0, int k0, int* i1, int* j1, int* k1,
                         int fi, int fj, int fk, int* blocksize, int maxblocksize)
 {
-    if (refresh() != MCMB_OK) { *i1 = fi; *j1 = fj; *k1 = fk; *blocksize = 0; g_status = MCMB_ERR_STATE; return; }
-    mcmb_next_ijk_bound(g_lsize, i0, j0, k0, i1, j1, k1, fi, fj, fk, blocksize, maxblocksize);
+    pthread_mutex_lock(&g_mu);
+    int st = refresh();
+    int ls = walk_lsize();
+    pthread_mutex_unlock(&g_mu);
+    if (st != MCMB_OK) { *i1 = fi; *j1 = fj; *k1 = fk; *blocksize = 0; g_status = MCMB_ERR_STATE; return; }
+    mcmb_next_ijk_bound(ls, i0, j0, k0, i1, j1, k1, fi, fj, fk, blocksize, maxblocksize);
+}
+
+/* calculate_gamma_3d.c:92-211, the host build: the walk's point count split
+ * evenly over the ranks (the first count % numranks ranks take one more), the
+ * rank's host share (1 - workshare) of it, then split over the threads the
+ * same way; thread t gets [i0,j0,k0] .. [i1,j1,k1) in td[t].  The thread
+ * count is omp_get_num_threads() of the caller's team when the program links
+ * OpenMP (a weak reference), else 1. */
+extern int omp_get_num_threads(void) __attribute__((weak));
+
+void mcmb_decompose_gamma_TTT_3D(int lsize, int rank, int numranks, double workshare, int nthreads,
+                                 thread_g3d_data_t* td)
+{
+    long cnt = 0;
+    for (int i = 2; i < lsize; ++i)
+        for (int j = i; j < lsize; ++j) {
+            int ke = i + j < lsize - 1 ? i + j : lsize - 1;
+            int kb = j + (i % 2);
+            if (kb <= ke) cnt += (ke - kb) / 2 + 1;
+        }
+    int count = (int)cnt;   /* the reference counts in int */
+    int blk = count / numranks, left = count - blk * numranks, dummy;
+    int i0 = 2, j0 = 2, k0 = 2, i1, j1, k1;
+    for (int r = 0; r < rank; ++r) {
+        mcmb_next_ijk_bound(lsize, i0, j0, k0, &i1, &j1, &k1, lsize, lsize, lsize, &dummy, blk + (r < left));
+        i0 = i1; j0 = j1; k0 = k1;
+    }
+    int work = blk + (rank < left);
+    int host = (int)((1.0 - workshare) * work);
+    mcmb_next_ijk_bound(lsize, i0, j0, k0, &i1, &j1, &k1, lsize, lsize, lsize, &dummy, host);
+    int fi = i1, fj = j1, fk = k1;
+    int tb = host / nthreads, tl = host - tb * nthreads;
+    for (int t = 0; t < nthreads; ++t) {
+        mcmb_next_ijk_bound(lsize, i0, j0, k0, &i1, &j1, &k1, fi, fj, fk, &dummy, tb + (t < tl));
+        td[t].i0 = i0; td[t].j0 = j0; td[t].k0 = k0;
+        td[t].i1 = i1; td[t].j1 = j1; td[t].k1 = k1;
+        i0 = i1; j0 = j1; k0 = k1;
+    }
+}
+
+void decompose_gamma_TTT_3D_mpi(thread_g3d_data_t* td, int rank, int numranks, double workshare)
+{
+    pthread_mutex_lock(&g_mu);
+    int st = refresh();
+    int ls = walk_lsize();
+    pthread_mutex_unlock(&g_mu);
+    if (st != MCMB_OK || numranks < 1 || rank < 0 || rank >= numranks) {
+        g_status = st != MCMB_OK ? st : MCMB_ERR_ARG;
+        return;
+    }
+    int nt = omp_get_num_threads ? omp_get_num_threads() : 1;
+    mcmb_decompose_gamma_TTT_3D(ls, rank, numranks, workshare, nt > 0 ? nt : 1, td);
+    g_status = MCMB_OK;
 }
 
 /* ---- weights and basis -------------------------------------------------- */
@@ -259,17 +340,29 @@ int basis_functions_legendre(double** a, int size, int pmax, double min, double 
 
 /* ---- Gram entry points ---------------------------------------------------- */
 
-/* gamma_mpi.c:30-152 */
+/* gamma_mpi.c:30-152: zeroes the caller's buffer and fills it with this
+ * rank's share of the 3D cross-Gamma of the bound context (late basis of the
+ * l-space setup against q-tilde), 6 dphi^2 included; summing the buffers of
+ * all ranks gives the whole, as the MPI_Reduce at gamma_mpi.c:764 does.
+ * workshare (the Xeon Phi offload split) is accepted and ignored.  Needs
+ * mcmb_qtilde_set and gamma_size = both mode counts (MCMB_ERR_STATE /
+ * MCMB_ERR_ARG in mcmb_legacy_status otherwise). */
 void gamma_3d_host(double* gamma_flat, int gamma_size, int rank, int numranks, double workshare)
 {
     (void)workshare;
     memset(gamma_flat, 0, sizeof(double) * (size_t)gamma_size * gamma_size);
-    if ((g_status = refresh()) != MCMB_OK) return;
-    if (gamma_size != g_n) { g_status = MCMB_ERR_ARG; return; }
-    if (numranks <= 1) g_status = mcmb_gamma_tet(g_ctx, MCMB_FP64, gamma_flat, NULL);
-    else g_status = mcmb_gamma_tet_slice(g_ctx, MCMB_FP64, rank, numranks, gamma_flat, NULL, NULL);
+    pthread_mutex_lock(&g_mu);
+    int st = refresh();
+    pthread_mutex_unlock(&g_mu);
+    int nq = 0;
+    if (st == MCMB_OK) st = mcmb_qtilde_info(g_ctx, NULL, NULL, NULL, &nq, NULL) == MCMB_OK ? MCMB_OK : MCMB_ERR_STATE;
+    if (st != MCMB_OK) { g_status = st; return; }
+    if (gamma_size != g_n || gamma_size != nq) { g_status = MCMB_ERR_ARG; return; }
+    g_status = mcmb_gamma_cross_3d(g_ctx, rank, numranks > 0 ? numranks : 1, gamma_flat, NULL);
 }
 
+/* The whole Gram of the bound context in *slot (kind 0: FP64 tetrapyd, 1: GL),
+ * computed on first use; g_mu held. */
 static int cached(double** slot, int gl)
 {
     int st0 = refresh();
@@ -283,22 +376,147 @@ static int cached(double** slot, int gl)
     return MCMB_OK;
 }
 
+/* Entry (m, n) of a cached Gram; status */
+static int cached_entry(double** slot, int gl, int m, int n, double* out)
+{
+    pthread_mutex_lock(&g_mu);
+    int st = cached(slot, gl);
+    int nn = g_n;
+    double v = 0.0;
+    if (st == MCMB_OK && (m < 0 || n < 0 || m >= nn || n >= nn)) st = MCMB_ERR_ARG;
+    if (st == MCMB_OK) v = (*slot)[(size_t)m * nn + n];
+    pthread_mutex_unlock(&g_mu);
+    *out = v;
+    return st;
+}
+
 /* orthogonalisation_late.c:338 (TTT only) */
 int calculate_orthol_3D(int m, int n, double* result)
 {
-    g_status = cached(&g_gamma3d, 0);
-    if (g_status != MCMB_OK || m < 0 || n < 0 || m >= g_n || n >= g_n) { result[0] = 0.0; return g_status ? g_status : MCMB_ERR_ARG; }
-    result[0] = g_gamma3d[(size_t)m * g_n + n];
-    return 0;
+    g_status = cached_entry(&g_gamma3d, 0, m, n, &result[0]);
+    return g_status;
 }
 
 /* orthogonalisation_late.c:239 (TTT only) */
 int calculate_orthol(int m, int n, double* result)
 {
-    g_status = cached(&g_gammagl, 1);
-    if (g_status != MCMB_OK || m < 0 || n < 0 || m >= g_n || n >= g_n) { result[0] = 0.0; return g_status ? g_status : MCMB_ERR_ARG; }
-    result[0] = g_gammagl[(size_t)m * g_n + n];
-    return 0;
+    g_status = cached_entry(&g_gammagl, 1, m, n, &result[0]);
+    return g_status;
+}
+
+/* orthogonalisation_late.c:23-97: the GL tables are built by the first
+ * calculate_orthol on the bound context; returns 0 as the reference does. */
+double init_orthol_glint(void)
+{
+    pthread_mutex_lock(&g_mu);
+    g_status = refresh();
+    pthread_mutex_unlock(&g_mu);
+    return 0.0;
+}
+
+/* orthogonalisation_late.c:99-129: M_ij of every (i, j) is formed on the GPU
+ * in one pass when the Gram is first needed; this call only checks state. */
+void calculate_orthoMij_T(int i, int j)
+{
+    (void)i;
+    (void)j;
+    pthread_mutex_lock(&g_mu);
+    g_status = refresh();
+    pthread_mutex_unlock(&g_mu);
+}
+
+/* orthogonalisation_late.c:159: one GPU holds every M_ij, nothing to sum */
+void sync_orthoMij(void) { g_status = MCMB_OK; }
+
+/* ---- GL cross-Gamma (calculate_gamma.c) ------------------------------------ */
+
+/* Entry (m, n) of the cached GL cross-Gamma with x rule spline (0 FMM, 1
+ * natural), computed whole on first use under g_mu. */
+static int cross_entry(int spline, int m, int n, double* out)
+{
+    pthread_mutex_lock(&g_mu);
+    int nq = 0;
+    long qgen = -1;
+    int st = refresh();
+    if (st == MCMB_OK && mcmb_qtilde_info(g_ctx, NULL, NULL, NULL, &nq, &qgen) != MCMB_OK) st = MCMB_ERR_STATE;
+    if (st == MCMB_OK && g_cross[spline] && g_cross_gen[spline] != qgen) {
+        free(g_cross[spline]);
+        g_cross[spline] = NULL;
+    }
+    if (st == MCMB_OK && !g_cross[spline]) {
+        double* g = (double*)malloc(sizeof(double) * (size_t)g_n * nq);
+        if (!g) st = MCMB_ERR_NOMEM;
+        else if ((st = mcmb_gamma_cross_gl(g_ctx, spline, 0, g, NULL)) != MCMB_OK) free(g);
+        else { g_cross[spline] = g; g_cross_gen[spline] = qgen; }
+    }
+    double v = 0.0;
+    if (st == MCMB_OK && (m < 0 || n < 0 || m >= g_n || n >= nq)) st = MCMB_ERR_ARG;
+    if (st == MCMB_OK) v = g_cross[spline][(size_t)m * nq + n];
+    pthread_mutex_unlock(&g_mu);
+    *out = v;
+    return st;
+}
+
+/* calculate_gamma.c:772-843: result[0] = Gamma_TTT(m, n), FMM spline in x */
+void calculate_gamma_2d(int m, int n, double* result)
+{
+    g_status = cross_entry(0, m, n, &result[0]);
+}
+
+/* calculate_gamma.c:936-1016: result[0] = Gamma_TTT(m, n), natural spline in x */
+void calculate_gamma(int m, int n, double* result)
+{
+    g_status = cross_entry(1, m, n, &result[0]);
+}
+
+/* calculate_gamma.c:48-240: the tables are built on the GPU by the first
+ * calculate_gamma / calculate_gamma_2d; this call checks the setups. */
+void init_gamma_glint(void)
+{
+    pthread_mutex_lock(&g_mu);
+    int st = refresh();
+    if (st == MCMB_OK && mcmb_qtilde_info(g_ctx, NULL, NULL, NULL, NULL, NULL) != MCMB_OK) st = MCMB_ERR_STATE;
+    pthread_mutex_unlock(&g_mu);
+    g_status = st;
+}
+
+/* calculate_gamma.c:255-303: forms every M_ij and the FMM-spline Gamma whole */
+void precompute_gammaMij_T(void)
+{
+    double v;
+    g_status = cross_entry(0, 0, 0, &v);
+}
+
+/* calculate_gamma.c:354-386: per-(i, j) M of one rank; the GPU forms them all */
+void calculate_gammaMij_T(int i, int j)
+{
+    (void)i;
+    (void)j;
+    init_gamma_glint();
+}
+
+/* calculate_gamma.c:425-509: one GPU holds every M_ij, nothing to sum */
+void sync_gammaMij(void) { g_status = MCMB_OK; }
+
+/* fixed_data.c get_qtilde_xsize / get_qtilde_xvec / get_qtilde_T_lsize of the
+ * bound q-tilde setup (0 / untouched without one) */
+int get_qtilde_xsize(void)
+{
+    int nx = 0;
+    if (!g_ctx || mcmb_qtilde_info(g_ctx, NULL, NULL, &nx, NULL, NULL) != MCMB_OK) return 0;
+    return nx;
+}
+
+void get_qtilde_xvec(double* xvec)
+{
+    if (g_ctx) g_status = mcmb_qtilde_xvec(g_ctx, xvec);
+}
+
+int get_qtilde_T_lsize(void)
+{
+    int lq = 0;
+    if (!g_ctx || mcmb_qtilde_info(g_ctx, NULL, &lq, NULL, NULL, NULL) != MCMB_OK) return 0;
+    return lq + 1;
 }
 
 /* Index of (i, j, k) in the prim walk from (min, min, min): rows (i, j) in

@@ -108,6 +108,13 @@ struct PointSet {
     // fused l3 contraction reads f indices in [-lmax - 64, 3 lmax + 158) and
     // v indices below lmax + 64
     DevBuf d_ftp, d_vp;
+    // the point set whose 1D basis this one shares (nullptr: its own) and that
+    // set's generation: a factorisation of the basis is keyed on them, so point
+    // sets cut from one setup (ranges of its walk) reuse it
+    const PointSet* basis_of = nullptr;
+    long basis_gen = -1;
+    const void* basis_key() const { return basis_of ? (const void*)basis_of : (const void*)this; }
+    long basis_generation() const { return basis_of ? basis_gen : generation; }
 };
 constexpr int TAB_TAIL = 256;
 inline long ftab_guard(int lmax) { return 2l * (lmax + 64); }
@@ -141,15 +148,33 @@ struct SepPlan {
 // P Ct (r orthonormal columns P over l), and the tables of the expansion of
 // the contracted tensor back to pairs.  Kept per point set.
 struct SepComp {
-    const void* ps = nullptr;   // point set the factorisation belongs to
-    long gen = -1;              // its generation
+    const void* ps = nullptr;   // point set whose basis was factorised (PointSet::basis_key)
+    long gen = -1;              // its generation (PointSet::basis_generation)
     int NP = 0, nl = 0, nlp = 0;
     int r = -1;                 // rank; 0: no saving, the pairs are contracted as they are
+    double gap = 0.0;           // last kept / first dropped pivot norm at the cut (r > 0 found)
     int ldu = 0;                // leading dimension the P table was laid out with
     DevBuf pc, pt, ct, h, tab;  // P by columns, P table [l][a], Ct [a][pair], H, expansion tables
     size_t o_task[3] = {}, o_tile[3] = {};
     int nt[3] = {}, gs[3] = {};
 };
+
+// Projected primordial basis q-tilde and the late-side spectra of the
+// cross-Gamma: qt[p][l][x] by multipole l = 0..lq, sample grid xvec, the
+// primordial mode table, C_l, N_l, b_l (l = 0..lq) and delta_phi.
+struct CrossSet {
+    bool ready = false;
+    long generation = -1;
+    int np = 0, lq = 0, nx = 0, n = 0;
+    double dphi = 0.0;
+    std::vector<double> qt, xvec, cl, nl, bl;
+    std::vector<int> order;
+    DevBuf d_qt, d_x, d_modes;
+};
+
+// Per-sample weights c_x of an x rule (MCMB_SPLINE_FMM, MCMB_SPLINE_NATURAL,
+// MCMB_SPLINE_TRAPEZIUM) on the increasing grid x.
+int spline_weights(const std::vector<double>& x, int kind, std::vector<double>& cw);
 
 constexpr int MAX_PHASES = 10;
 
@@ -164,6 +189,7 @@ struct mcmb_ctx {
     mcmb::PointSet ls;   // late-time multipole tetrapyd
     mcmb::PointSet ks;   // primordial k-space Gauss-Legendre tetrapyd
     mcmb::PointSet ku;   // primordial k-space uniform-grid tetrapyd (prim walk)
+    mcmb::CrossSet xs;   // q-tilde of the cross-Gamma
     mcmb::DevBuf d_W;    // full-cube projection weight W[i][j][k] (k-space)
     mcmb::DevBuf pz, py, ph, pd, psc, pdm;   // projection batch buffers (Z, Y, H, partials, sin / cos, D)
     // stage-1 products Y = W Z of the last forward projection, per batch of

@@ -62,6 +62,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_lspace_npoints.restype = C.c_long
         lib.mcmb_lspace_points.argtypes = [vp, vp, C.c_long]
         lib.mcmb_lspace_points.restype = C.c_long
+        lib.mcmb_lspace_points_at.argtypes = [vp, C.c_long, vp, C.c_long]
+        lib.mcmb_lspace_points_at.restype = C.c_long
+        lib.mcmb_gamma_tet_lrange.argtypes = [vp, ip, ip, ip, vp, vp]
         lib.mcmb_lspace_weights.argtypes = [vp, vp, C.c_long]
         lib.mcmb_lspace_weights.restype = C.c_long
         lib.mcmb_last_kernel_ms.argtypes = [vp, C.POINTER(dp), C.POINTER(ip)]
@@ -70,6 +73,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
         lib.mcmb_device_free.argtypes = [vp, vp]
         lib.mcmb_device_to_host.argtypes = [vp, vp, vp, C.c_size_t]
+        lib.mcmb_device_copy.argtypes = [vp, vp, vp, C.c_size_t]
         lib.mcmb_orthonormalise.argtypes = [vp, ip, vp, vp, vp, vp, C.POINTER(ip)]
         lib.mcmb_cholesky_solve.argtypes = [vp, ip, vp, ip, vp, vp]
         lib.mcmb_eigen_sym.argtypes = [vp, ip, vp, vp, vp, C.POINTER(ip)]
@@ -100,6 +104,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_kgram.argtypes = [vp, ip, vp, vp]
         lib.mcmb_project_fwd.argtypes = [vp, ip, ip, vp, vp, vp]
         lib.mcmb_project_bwd.argtypes = [vp, ip, ip, vp, vp, vp]
+        lib.mcmb_qtilde_set.argtypes = [vp, ip, ip, ip, vp, vp, ip, vp, vp, vp, vp, dp]
+        lib.mcmb_qtilde_info.argtypes = [vp] + [C.POINTER(ip)] * 4 + [C.POINTER(C.c_long)]
+        lib.mcmb_gamma_cross_3d.argtypes = [vp, ip, ip, vp, vp]
+        lib.mcmb_gamma_cross_gl.argtypes = [vp, ip, ip, vp, vp]
+        lib.mcmb_spline_weights.argtypes = [ip, ip, vp, vp]
+        lib.init_orthol_glint.restype = dp
         lib.permsix.restype = dp
         lib.permsix.argtypes = [ip, ip, ip]
         lib.calculate_geometric.restype = dp
@@ -112,8 +122,8 @@ EXPORTED_SYMBOLS = (
     "mcmb_last_error", "mcmb_status_string", "mcmb_create", "mcmb_destroy", "mcmb_sync",
     "mcmb_lspace_set", "mcmb_gamma_tet", "mcmb_gamma_gl", "mcmb_lspace_npoints",
     "mcmb_lspace_points", "mcmb_lspace_weights", "mcmb_orthonormalise", "mcmb_cholesky_solve",
-    "mcmb_last_kernel_ms", "mcmb_last_phases", "mcmb_last_basis_rank", "mcmb_device_alloc", "mcmb_device_free", "mcmb_device_to_host", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
-    "mcmb_gauss_legendre", "mcmb_gamma_tet_slice", "mcmb_kspace_set", "mcmb_kspace_npoints",
+    "mcmb_last_kernel_ms", "mcmb_last_phases", "mcmb_last_basis_rank", "mcmb_device_alloc", "mcmb_device_free", "mcmb_device_to_host", "mcmb_device_copy", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
+    "mcmb_gauss_legendre", "mcmb_gamma_tet_slice", "mcmb_gamma_tet_lrange", "mcmb_lspace_points_at", "mcmb_kspace_set", "mcmb_kspace_npoints",
     "mcmb_kgram", "mcmb_project_fwd", "mcmb_project_bwd", "mcmb_eigen_sym", "mcmb_orthonormalise_late",
     "mcmb_kuniform_set", "mcmb_kuniform_info", "mcmb_kuniform_npoints", "mcmb_kuniform_weights",
     "mcmb_kuniform_gram", "mcmb_kuniform_project", "calculate_ortho", "calculate_eigen",
@@ -124,7 +134,23 @@ EXPORTED_SYMBOLS = (
     "get_ijk_next_prim", "check_ijk_min_prim", "check_ijk_max_prim", "decompose_tetrapyd_prim",
     "divide_tasks", "get_next_ijk_bound", "permsix", "calculate_geometric",
     "basis_functions_legendre", "gamma_3d_host", "calculate_orthol_3D", "calculate_orthol",
+    "mcmb_qtilde_set", "mcmb_qtilde_info", "mcmb_qtilde_xvec", "mcmb_gamma_cross_3d", "mcmb_gamma_cross_gl",
+    "mcmb_spline_weights", "mcmb_decompose_gamma_TTT_3D", "decompose_gamma_TTT_3D_mpi", "calculate_gamma_2d",
+    "calculate_gamma", "init_gamma_glint", "precompute_gammaMij_T", "calculate_gammaMij_T", "sync_gammaMij",
+    "get_qtilde_xsize", "get_qtilde_xvec", "get_qtilde_T_lsize", "init_orthol_glint", "calculate_orthoMij_T",
+    "sync_orthoMij",
 )
+
+
+SPLINE_FMM, SPLINE_NATURAL, SPLINE_TRAPEZIUM = 0, 1, 2
+
+
+def spline_weights(kind: int, x: np.ndarray) -> np.ndarray:
+    """Per-sample weights of an x rule (host only)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    c = np.empty_like(x)
+    _check(load_library().mcmb_spline_weights(kind, x.shape[0], _ptr(x), _ptr(c)))
+    return c
 
 
 def write_raw(path: str, a: np.ndarray) -> None:
@@ -212,6 +238,35 @@ class Context:
         _check(_lib.mcmb_gamma_tet_slice(self._h, precision, rank, nranks, _ptr(g), None, C.byref(frac)))
         return g, frac.value
 
+    def gamma_tet_lrange(self, precision: int, l1_lo: int, l1_hi: int) -> np.ndarray:
+        """Partial Gram over the points with l1 in [l1_lo, l1_hi)."""
+        g = np.empty((self.n, self.n), dtype=np.float64)
+        _check(_lib.mcmb_gamma_tet_lrange(self._h, precision, l1_lo, l1_hi, _ptr(g), None))
+        return g
+
+    # ---- cross-Gamma against q-tilde --------------------------------------
+    def qtilde_set(self, qt: np.ndarray, xvec: np.ndarray, order_prim: np.ndarray, cl: np.ndarray,
+                   nl: np.ndarray, bl: np.ndarray, deltaphi: float) -> None:
+        """qt[p][l][x] (read_qtilde layout, l = multipole 0..lmax_q)."""
+        qt = np.ascontiguousarray(qt, dtype=np.float64)
+        np_, nlq, nx = qt.shape
+        xvec = np.ascontiguousarray(xvec, dtype=np.float64)
+        op = np.ascontiguousarray(order_prim, dtype=np.int32).reshape(-1, 3)
+        sp = [np.ascontiguousarray(a, dtype=np.float64) for a in (cl, nl, bl)]
+        _check(_lib.mcmb_qtilde_set(self._h, np_, nlq - 1, nx, _ptr(qt), _ptr(xvec), op.shape[0], _ptr(op),
+                                    _ptr(sp[0]), _ptr(sp[1]), _ptr(sp[2]), float(deltaphi)))
+        self.nprim = op.shape[0]
+
+    def gamma_cross_3d(self, rank: int = 0, nranks: int = 1) -> np.ndarray:
+        g = np.empty((self.n, self.nprim))
+        _check(_lib.mcmb_gamma_cross_3d(self._h, rank, nranks, _ptr(g), None))
+        return g
+
+    def gamma_cross_gl(self, spline: int = 0, npt: int = 0) -> np.ndarray:
+        g = np.empty((self.n, self.nprim))
+        _check(_lib.mcmb_gamma_cross_gl(self._h, spline, npt, _ptr(g), None))
+        return g
+
     def gamma_gl(self, npt: int = 0) -> np.ndarray:
         g = np.empty((self.n, self.n), dtype=np.float64)
         _check(_lib.mcmb_gamma_gl(self._h, npt, _ptr(g), None))
@@ -233,6 +288,10 @@ class Context:
         _check(_lib.mcmb_device_to_host(self._h, _ptr(out), C.c_void_p(dev_ptr), out.nbytes))
         return out
 
+    def device_copy(self, dst: int, src: int, nbytes: int) -> None:
+        """Plain copy kernel of nbytes (multiple of 16) on the device."""
+        _check(_lib.mcmb_device_copy(self._h, C.c_void_p(dst), C.c_void_p(src), nbytes))
+
     def npoints(self) -> int:
         r = _lib.mcmb_lspace_npoints(self._h)
         if r < 0:
@@ -244,6 +303,16 @@ class Context:
         cap = n if cap is None else min(cap, n)
         out = np.empty((cap, 3), dtype=np.int32)
         r = _lib.mcmb_lspace_points(self._h, _ptr(out), cap)
+        if r < 0:
+            _check(int(r))
+        return out
+
+    def points_at(self, p0: int, cap: int) -> np.ndarray:
+        """Points p0 .. p0 + cap - 1 of the device enumeration (fewer at its end)."""
+        n = self.npoints()
+        m = max(0, min(cap, n - p0))
+        out = np.empty((m, 3), dtype=np.int32)
+        r = _lib.mcmb_lspace_points_at(self._h, p0, _ptr(out), m)
         if r < 0:
             _check(int(r))
         return out

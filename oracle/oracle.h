@@ -43,6 +43,8 @@ void or_decompose_tetrapyd_XXX(or_tetrapyd_limits* block, int rank, int nranks,
 long or_count_tetrapyd_XXX(int lmin, int lmax);
 /* Dump the walk: writes up to cap (i,j,k) triples, returns count walked. */
 long or_walk_XXX(int lmin, int lmax, int* ijk, long cap);
+/* Points p0 .. p0+cap-1 of the walk (fewer at its end); returns the count written. */
+long or_walk_XXX_window(int lmin, int lmax, long p0, int* ijk, long cap);
 
 /* prim variants, tetrapyd_tools.c:416-592 */
 int  or_get_ijk_next_prim(int max, int* i, int* j, int* k);
@@ -156,9 +158,42 @@ void or_gram_kgl(int N, const double* kn, const double* kw, const double* q, int
 void or_project_kgl(int N, const double* kn, const double* kw, const double* q, int ldq,
                     int n, const int* order, double omega, double phi,
                     double* alpha, double* dalpha_domega, double* dalpha_dphi);
+/* or_project_kgl for nshape pairs theta[b] = (omega, phi), one pass over the
+ * points: alpha / d_om / d_ph are [nshape][n] (any may be NULL). */
+void or_project_kgl_batch(int N, const double* kn, const double* kw, const double* q, int ldq,
+                          int n, const int* order, int nshape, const double* theta,
+                          double* alpha, double* d_om, double* d_ph);
 /* last index b (of sorted nodes) with kn[b] <= kn[i] + kn[j], per (i,j)
  * (N x N, row-major), -1 if none. */
 void or_kgl_limits(int N, const double* kn, int* lim);
+
+/* ---- cross-Gamma against the projected primordial basis ---------------- */
+/* init_gamma_3Dint (calculate_gamma_3d.c:23-61, TT part): lw[l] = (2l+1)^-1/3
+ * and q3[l][p][x] = (2l+1)^(1/6) / sqrt(C_l + N_l/b_l^2) qt[p][l][x] (l > 1,
+ * else 0); qt in the read_qtilde layout [p][l][x] (fixed_data.c:2246-2290). */
+void or_qtilde3d(int np, int lsize, int nx, const double* qt, const double* cl, const double* nl,
+                 const double* bl, double* q3, double* lw);
+/* gamma_3d_host / calculate_gamma_TTT_3D_nested with calculate_sijk_TTT and
+ * the trapezium integrate() (gamma_mpi.c:30-152, calculate_gamma_3d.c:
+ * 274-440, 783-860): n x n, rows late modes, columns primordial modes, scaled
+ * by 6 dphi^2 (nlate x n: the reference's are square); late basis b[l][ldb]
+ * by multipole, 0 outside [tlmin, tlmax];
+ * points [p0, p1) of the walk of T(2, lsize-1) when p1 > 0. */
+void or_gamma_cross_3d(int lsize, int tlmin, int tlmax, const double* b, int ldb, const int* order_late, int nlate,
+                       int np, const double* q3, int nx, const double* xvec, const int* order_prim, int n,
+                       const double* lw, int geom, double dphi, long p0, long p1, double* gamma);
+/* integrate_cubic_spline (calculate_gamma.c:845-934, FMM end conditions; n >= 4:
+ * with 2 points its elimination divides 0 by 0, with 3 it reads a 4th) and the natural cubic spline integral of gsl_interp_cspline
+ * (calculate_gamma.c:985-998), over [x_0, x_{n-1}]. */
+double or_spline_integral_fmm(const double* x, const double* y, int n);
+double or_spline_integral_natural(const double* x, const double* y, int n);
+/* calculate_gamma_2d (spline 0) / calculate_gamma (spline 1) for all (m, n)
+ * after init_gamma_glint + precompute_gammaMij_T (calculate_gamma.c:48-303,
+ * 511-555, 772-1016), TT part; scaled by dphi^2 / (8 pi). */
+void or_gamma_cross_gl(int tlmin, int tlmax, const double* b, int ldb, const int* order_late, int nlate, int np,
+                       int lsize_q, const double* qt, int nx, const double* xvec, const int* order_prim, int n,
+                       const double* cl, const double* nl, const double* bl, double dphi, int spline, int npt,
+                       double* gamma);
 
 /* wall clock (CLOCK_MONOTONIC) */
 double or_wtime(void);

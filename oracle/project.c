@@ -170,3 +170,59 @@ void or_project_kgl(int N, const double* kn, const double* kw, const double* q, 
     }
     free(part);
 }
+
+/* or_project_kgl for nshape parameter pairs theta[b] = (omega, phi) in one
+ * pass over the points (the mode products Q_n are formed once per point and
+ * shared by the shapes): alpha[b][n], d_om[b][n], d_ph[b][n]. */
+void or_project_kgl_batch(int N, const double* kn, const double* kw, const double* q, int ldq,
+                          int n, const int* order, int nshape, const double* theta,
+                          double* alpha, double* d_om, double* d_ph)
+{
+    int nt = or_max_threads();
+    long per = (long)nshape * 3 * n;
+    double* part = (double*)calloc((size_t)nt * per, sizeof(double));
+#pragma omp parallel num_threads(nt)
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        double* pa = part + (long)t * per;
+        double* Q = (double*)malloc(sizeof(double) * n);
+#pragma omp for schedule(dynamic, 1)
+        for (int a = 0; a < N; ++a)
+            for (int b = a; b < N; ++b)
+                for (int c = b; c < N; ++c) {
+                    if (!(kn[c] <= kn[a] + kn[b])) break;
+                    double K = kn[a] + kn[b] + kn[c];
+                    double W = or_permsix(a, b, c) * kw[a] * kw[b] * kw[c] / K;
+                    for (int m = 0; m < n; ++m) Q[m] = or_pijk(q, ldq, order, m, a, b, c);
+                    for (int s = 0; s < nshape; ++s) {
+                        double om = theta[2 * s], ph = theta[2 * s + 1];
+                        double sv = sin(om * K + ph), cv = cos(om * K + ph);
+                        double f0 = W * sv, f1 = W * K * cv, f2 = W * cv;
+                        double* ps = pa + (long)s * 3 * n;
+                        for (int m = 0; m < n; ++m) {
+                            ps[m] += f0 * Q[m];
+                            ps[n + m] += f1 * Q[m];
+                            ps[2 * n + m] += f2 * Q[m];
+                        }
+                    }
+                }
+        free(Q);
+    }
+    for (int s = 0; s < nshape; ++s)
+        for (int m = 0; m < n; ++m) {
+            double s0 = 0, s1 = 0, s2 = 0;
+            for (int t = 0; t < nt; ++t) {
+                const double* ps = part + (long)t * per + (long)s * 3 * n;
+                s0 += ps[m];
+                s1 += ps[n + m];
+                s2 += ps[2 * n + m];
+            }
+            if (alpha) alpha[(long)s * n + m] = s0;
+            if (d_om) d_om[(long)s * n + m] = s1;
+            if (d_ph) d_ph[(long)s * n + m] = s2;
+        }
+    free(part);
+}

@@ -38,6 +38,18 @@ def lib() -> C.CDLL:
         L.or_count_tetrapyd_XXX.restype = C.c_long
         L.or_walk_XXX.restype = C.c_long
         L.or_walk_XXX.argtypes = [ip, ip, vp, C.c_long]
+        L.or_walk_XXX_window.restype = C.c_long
+        L.or_walk_XXX_window.argtypes = [ip, ip, C.c_long, vp, C.c_long]
+        L.or_project_kgl_batch.argtypes = [ip, vp, vp, vp, ip, ip, vp, ip, vp, vp, vp, vp]
+        L.or_qtilde3d.argtypes = [ip, ip, ip, vp, vp, vp, vp, vp, vp]
+        L.or_gamma_cross_3d.argtypes = [ip, ip, ip, vp, ip, vp, ip, ip, vp, ip, vp, vp, ip, vp, ip, dp, C.c_long,
+                                        C.c_long, vp]
+        L.or_spline_integral_fmm.restype = dp
+        L.or_spline_integral_fmm.argtypes = [vp, vp, ip]
+        L.or_spline_integral_natural.restype = dp
+        L.or_spline_integral_natural.argtypes = [vp, vp, ip]
+        L.or_gamma_cross_gl.argtypes = [ip, ip, vp, ip, vp, ip, ip, ip, vp, ip, vp, vp, ip, vp, vp, vp, dp, ip, ip,
+                                        vp]
         L.or_walk_prim.restype = C.c_long
         L.or_walk_prim.argtypes = [ip, ip, vp, C.c_long]
         for f in ("or_permsix", "or_geometric_asym", "or_threej000_sq", "or_h2_exact"):
@@ -82,6 +94,12 @@ def walk_XXX(lmin: int, lmax: int) -> np.ndarray:
     out = np.empty((n, 3), dtype=np.int32)
     lib().or_walk_XXX(lmin, lmax, _p(out), n)
     return out
+
+
+def walk_XXX_window(lmin: int, lmax: int, p0: int, cap: int) -> np.ndarray:
+    out = np.empty((cap, 3), dtype=np.int32)
+    w = lib().or_walk_XXX_window(lmin, lmax, p0, _p(out), cap)
+    return out[:w].copy()
 
 
 def walk_prim(mn: int, mx: int) -> np.ndarray:
@@ -213,6 +231,20 @@ def project_kgl(kn, kw, q, order, omega, phi):
     return a, d1, d2
 
 
+def project_kgl_batch(kn, kw, q, order, theta):
+    """alpha[b][n], d alpha / d omega [b][n], d alpha / d phi [b][n] for theta[b] = (omega, phi)."""
+    kn = np.ascontiguousarray(kn, dtype=np.float64)
+    kw = np.ascontiguousarray(kw, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    th = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1, 2)
+    n, B = order.shape[0], th.shape[0]
+    a, d1, d2 = np.empty((B, n)), np.empty((B, n)), np.empty((B, n))
+    lib().or_project_kgl_batch(kn.shape[0], _p(kn), _p(kw), _p(q), q.shape[1], n, _p(order), B, _p(th),
+                               _p(a), _p(d1), _p(d2))
+    return a, d1, d2
+
+
 def kgl_limits(kn) -> np.ndarray:
     kn = np.ascontiguousarray(kn, dtype=np.float64)
     N = kn.shape[0]
@@ -238,3 +270,45 @@ def calculate_eigen_uniform(mx, kmax, q, order, a1, a2) -> np.ndarray:
     lib().or_calculate_eigen_uniform(mx, float(kmax), _p(q), q.shape[1], n, _p(order),
                                      float(a1), float(a2), _p(out))
     return out
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def qtilde3d(qt, cl, nl, bl):
+    """(q3[l][p][x], lw[l]) of init_gamma_3Dint from qt[p][l][x] and the spectra."""
+    qt = _c(qt)
+    np_, lsize, nx = qt.shape
+    q3, lw = np.empty((lsize, np_, nx)), np.empty(lsize)
+    lib().or_qtilde3d(np_, lsize, nx, _p(qt), _p(_c(cl)), _p(_c(nl)), _p(_c(bl)), _p(q3), _p(lw))
+    return q3, lw
+
+
+def gamma_cross_3d(tlmin, tlmax, b, order_late, q3, xvec, order_prim, lw, dphi, geom=1, p0=0, p1=0):
+    b, q3, xvec, lw = _c(b), _c(q3), _c(xvec), _c(lw)
+    ol, op = _c(order_late, np.int32), _c(order_prim, np.int32)
+    lsize, np_, nx = q3.shape
+    nl_, n = ol.shape[0], op.shape[0]
+    g = np.empty((nl_, n))
+    lib().or_gamma_cross_3d(lsize, tlmin, tlmax, _p(b), b.shape[1], _p(ol), nl_, np_, _p(q3), nx, _p(xvec), _p(op),
+                            n, _p(lw), geom, float(dphi), int(p0), int(p1), _p(g))
+    return g
+
+
+def spline_integral(x, y, kind="fmm"):
+    x, y = _c(x), _c(y)
+    f = lib().or_spline_integral_fmm if kind == "fmm" else lib().or_spline_integral_natural
+    return f(_p(x), _p(y), x.shape[0])
+
+
+def gamma_cross_gl(tlmin, tlmax, b, order_late, qt, xvec, order_prim, cl, nl, bl, dphi, spline=0, npt=None):
+    b, qt, xvec = _c(b), _c(qt), _c(xvec)
+    ol, op = _c(order_late, np.int32), _c(order_prim, np.int32)
+    np_, lsize_q, nx = qt.shape
+    nl_, n = ol.shape[0], op.shape[0]
+    npt = 3 * tlmax // 2 + 1 if npt is None else npt
+    g = np.empty((nl_, n))
+    lib().or_gamma_cross_gl(tlmin, tlmax, _p(b), b.shape[1], _p(ol), nl_, np_, lsize_q, _p(qt), nx, _p(xvec),
+                            _p(op), n, _p(_c(cl)), _p(_c(nl)), _p(_c(bl)), float(dphi), spline, npt, _p(g))
+    return g

@@ -277,3 +277,17 @@ void or_decompose_gamma_TTT_3D(int lsize, int rank, int numranks, double worksha
         i0 = i1; j0 = j1; k0 = k1;
     }
 }
+
+/* Points p0 .. p0 + cap - 1 of the same walk (fewer at its end); returns the
+ * number written.  Walks from the start, as get_ijk_next_XXX must. */
+long or_walk_XXX_window(int lmin, int lmax, long p0, int* ijk, long cap)
+{
+    int i = lmin, j = lmin, k = lmin;
+    or_check_ijk_min_XXX(lmin, &i, &j, &k);
+    long n = or_count_tetrapyd_XXX(lmin, lmax), w = 0;
+    for (long p = 0; p < n && w < cap; ++p) {
+        if (p >= p0) { ijk[3 * w] = i; ijk[3 * w + 1] = j; ijk[3 * w + 2] = k; ++w; }
+        if (p + 1 < n) or_get_ijk_next_XXX(lmax, &i, &j, &k);
+    }
+    return w;
+}

@@ -33,8 +33,8 @@ def test_device_count_known_answer(ctx):
     ws = W.lspace(2000, 1)
     ctx.lspace_set(2, 2000, ws["q"], ws["v"], ws["order"])
     assert ctx.npoints() == 334831501          # process.c:65
-    tail = ctx.points(cap=1000)
-    assert np.array_equal(tail, O.walk_XXX(2, 2000)[:1000]) if False else tail.shape == (1000, 3)
+    head = ctx.points(cap=100_000)
+    assert np.array_equal(head, O.walk_XXX_window(2, 2000, 0, 100_000))
 
 
 @pytest.mark.parametrize("h2", [modalcmb.H2_EXACT, modalcmb.H2_ASYMPTOTIC])
@@ -342,13 +342,17 @@ def test_separable_multi_block(ctx, R, compress):
 
 
 
-@pytest.mark.parametrize("R", [6, 12])
-def test_separable_full_rank_basis_contracts_pairs(ctx, R):
+# R = 15 (120 pairs) and 18 (171 pairs) take the one-wave l3 kernel of 8
+# column tiles and the two-wave kernel of 6 tiles (300 of their modes; more
+# multipoles than pairs, so the random pair products have full rank)
+@pytest.mark.parametrize("R,L", [(6, 120), (12, 120), (15, 130), (18, 180)])
+def test_separable_full_rank_basis_contracts_pairs(ctx, R, L):
     # a basis of random values per l: its pair products have full rank, so no
     # compression pays and the pairs are contracted as they are
-    L = 120
     ws = W.lspace(L, R, True)
     q = np.random.default_rng(R).standard_normal((L + 1, R))
+    if len(ws["order"]) > 300:
+        ws["order"] = ws["order"][np.random.default_rng(1).choice(len(ws["order"]), 300, replace=False)]
     ctx.lspace_set(2, L, q, ws["v"], ws["order"])
     G = _gram_paths(ctx, modalcmb.FP64, ("separable", "direct"))
     ctx.gamma_tet(modalcmb.FP64)
@@ -392,3 +396,47 @@ def test_separable_compressed_wide_basis(ctx):
     assert rel_elem(G["separable"], G["separable-pairs"]) < 1e-12
     assert rel_elem(G["separable"], O.gamma_tet(2, L, q, ws["v"], order)) < 1e-12
     assert np.array_equal(G["separable"], G["separable"].T)
+
+
+def _order9_basis(L, p=10, Tmin=2):
+    """The reference's late basis of order 9 (create_basis_late, fixed_data.c:
+    1923-1950) on multipoles Tmin..L: Legendre P_0..P_{p-2} of (l - Tmin)/(L -
+    Tmin) and the columns w/z, w z with x = l(l+1)/(L(L+1)), y = (2l+1)/(2L+1),
+    z = sqrt(x), w = y^(1/6); rows indexed by multipole (zero below Tmin)."""
+    ls = np.arange(L + 1.0)
+    q = np.zeros((L + 1, p + 1))
+    tv = ls[Tmin:]
+    q[Tmin:, :p - 1] = W.legendre_table((tv - Tmin) / (L - Tmin), p - 2)
+    x = tv * (tv + 1) / (L * (L + 1.0))
+    y = (2 * tv + 1) / (2 * L + 1.0)
+    q[Tmin:, p - 1] = y ** (1 / 6) / np.sqrt(x)
+    q[Tmin:, p] = y ** (1 / 6) * np.sqrt(x)
+    return q
+
+
+@pytest.mark.parametrize("kind,rank", [("order9", 0), ("exp", 0), ("cos", 15)])
+def test_separable_rank_needs_a_gap(ctx, kind, rank):
+    # the pair products are cut to a compressed basis only across a gap of
+    # 1e6 in the pivoted Gram-Schmidt residual norms: the reference's order-9
+    # late basis and decaying exponentials have residuals that fall through
+    # 1e-12 gradually (pivot ratio ~10 at that level), so their pairs are
+    # contracted as they are; products of cosines span exactly 15 of 36
+    # directions and are compressed.  Every result matches the pairs and the
+    # oracle's direct sum.
+    L = 200
+    ls = np.arange(L + 1.0)
+    if kind == "order9":
+        q = _order9_basis(L)
+    elif kind == "exp":
+        q = np.stack([np.exp(-r * ls / L) for r in range(8)], 1)
+    else:
+        q = np.stack([np.cos((r + 0.5) * np.pi * ls / L * 0.7) for r in range(8)], 1)
+    R = q.shape[1]
+    ws = W.lspace(L, R, True)
+    ctx.lspace_set(2, L, q, ws["v"], ws["order"])
+    G = _gram_paths(ctx, modalcmb.FP64, ("separable", "separable-pairs"))
+    ctx.gamma_tet(modalcmb.FP64)
+    assert ctx.last_basis_rank() == rank
+    ref = O.gamma_tet(2, L, q, ws["v"], ws["order"])
+    assert rel_elem(G["separable"], ref) < 1e-12
+    assert rel_elem(G["separable-pairs"], ref) < 1e-12

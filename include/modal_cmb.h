@@ -283,6 +283,43 @@ int mcmb_read_raw(const char* path, long count, double* data);
  * line, '#' lines skipped; at most cap triples into order[3*cap]. */
 int mcmb_load_order(const char* path, int* order, int cap, int* nmodes);
 
+/* ---- file names and q-tilde files (host only) ---------------------------- */
+#define MCMB_FILE_EIGEN      0   /* basis:eigen_file   */
+#define MCMB_FILE_ORTHO      1   /* basis:ortho_file   */
+#define MCMB_FILE_LAMBDA     2   /* basis:lambda_file  */
+#define MCMB_FILE_ORTHOL     3   /* basis:orthol_file  (pol: 0 TTT, 1 TTE, 2 TEE, 3 EEE) */
+#define MCMB_FILE_LAMBDAL    4   /* basis:lambdal_file (pol as above) */
+#define MCMB_FILE_GAMMA      5   /* basis:gamma_file   (pol as above) */
+#define MCMB_FILE_PROJ_DATA  6   /* basis:proj_file, data (pol: 0 T, 1 E) */
+#define MCMB_FILE_PROJ_SIZE  7   /* basis:proj_file, size file (pol: 0 T, 1 E) */
+/* The file name the reference's drivers build from the .ini (parameter.c:
+ * 244-257, 317-760): base as given if it starts with '/' (gamma: + "_<pol>",
+ * proj: + "_T" / "_E", size files + "_T_size" / "_E_size"; the late-side
+ * orthol / lambdal have a name for TTT only), else directory + base + the
+ * suffix chain, e.g. gamma TTT: "_TTT_<T_lmin>-<T_lmax>_<order_prim>_
+ * <order_late>_<alpha_max>".  MCMB_ERR_ARG if it does not fit in cap. */
+int mcmb_file_name(char* out, size_t cap, int kind, int pol, const char* directory, const char* base,
+                   int T_lmin, int T_lmax, int E_lmin, int E_lmax, int order_prim, int order_late,
+                   int alpha_max);
+/* q-tilde files as output_qtilde writes and read_qtilde reads them
+ * (fixed_data.c:2076-2330): size_file holds {lsize + 1, xsize + 1} as native
+ * ints; per primordial index p < np the file data_file + "_<p>" (order4: the
+ * last two are "_l1", "_l2") holds one (lsize+1) x (xsize+1) FP64 array with
+ * the x grid in row 0, the multipoles lvec in column 0 and qt[p][l][x].
+ * mcmb_read_qtilde returns the sizes alone when lvec / xvec / qt are NULL,
+ * and MCMB_ERR_ARG when they exceed cap_l / cap_x; qt is [np][lsize][xsize]. */
+int mcmb_write_qtilde(const char* size_file, const char* data_file, int np, int order4, int lsize, int xsize,
+                      const int* lvec, const double* xvec, const double* qt);
+int mcmb_read_qtilde(const char* size_file, const char* data_file, int np, int order4, int cap_l, int cap_x,
+                     int* lsize, int* xsize, int* lvec, double* xvec, double* qt);
+/* create_basis_late's T basis of eflag_order_late 9 (fixed_data.c:1923-1950):
+ * basis[i][0..pmax-2] = P_j((Tvec[i] - Tmin)/(Tmax - Tmin)) (shifted Legendre,
+ * basis_functions_legendre), basis[i][pmax-1] = w/z, basis[i][pmax] = w z
+ * with x = l(l+1)/(Tmax(Tmax+1)), y = (2l+1)/(2Tmax+1), z = sqrt(x), w =
+ * y^(1/6), l = Tvec[i]; basis is [Tsize][pmax+1] (row i is Tvec[i], the
+ * reference's row l - Tmin). */
+int mcmb_basis_late_order9(int Tsize, int pmax, double Tmin, double Tmax, const double* Tvec, double* basis);
+
 /* ---- quadrature ----------------------------------------------------------- */
 /* n-node Gauss-Legendre rule on [-1,1], nodes ascending (host only, no GPU). */
 int mcmb_gauss_legendre(int n, double* x, double* w);
@@ -388,7 +425,7 @@ void sync_gammaMij(void);
 /* fixed_data.c get_qtilde_xsize / get_qtilde_xvec / get_qtilde_T_lsize of the
  * bound q-tilde setup (lsize = lmax_q + 1) */
 int get_qtilde_xsize(void);
-void get_qtilde_xvec(double* xvec);
+int get_qtilde_xvec(double* xvec);   /* returns 0, or the status without a setup */
 int get_qtilde_T_lsize(void);
 /* orthogonalisation_late.c:23, 99, 159 -- as above for the auto-Gram:
  * init_orthol_glint returns 0, calculate_orthoMij_T checks state (every M_ij
@@ -420,6 +457,9 @@ double pijk(int r, int i, int j, int k);
 double plijk_TTT(int r, int l1, int l2, int l3);
 /* fixed_data.c:263 */
 void find_perm_prim(int r, int* p1, int* p2, int* p3);
+/* fixed_data.c:983-990 -- mode r of the bound l-space setup's order table
+ * (-1s outside it) */
+void find_perm_late_TTT(int r, int* p1, int* p2, int* p3);
 /* global.h:429, 730 -- of the bound uniform k-grid setup (0 if unset) */
 int get_terms_prim(void);
 double get_kmax(void);

@@ -507,9 +507,10 @@ int get_qtilde_xsize(void)
     return nx;
 }
 
-void get_qtilde_xvec(double* xvec)
+int get_qtilde_xvec(double* xvec)
 {
-    if (g_ctx) g_status = mcmb_qtilde_xvec(g_ctx, xvec);
+    g_status = g_ctx ? mcmb_qtilde_xvec(g_ctx, xvec) : MCMB_ERR_STATE;
+    return g_status;
 }
 
 int get_qtilde_T_lsize(void)
@@ -646,6 +647,19 @@ void find_perm_prim(int r, int* p1, int* p2, int* p3)
     int ldq, n, mx;
     *p1 = *p2 = *p3 = -1;
     if (!g_ctx || mcmb_kuniform_tables(g_ctx, &q, &ldq, &order, &n, &mx) != MCMB_OK || r < 0 || r >= n) return;
+    *p1 = order[3 * r];
+    *p2 = order[3 * r + 1];
+    *p3 = order[3 * r + 2];
+}
+
+/* fixed_data.c:983-990 -- the late order table of the bound l-space setup */
+void find_perm_late_TTT(int r, int* p1, int* p2, int* p3)
+{
+    const double* q;
+    const int* order;
+    int ldq, n, lmin, lmax;
+    *p1 = *p2 = *p3 = -1;
+    if (!g_ctx || mcmb_lspace_tables(g_ctx, &q, &ldq, &order, &n, &lmin, &lmax) != MCMB_OK || r < 0 || r >= n) return;
     *p1 = order[3 * r];
     *p2 = order[3 * r + 1];
     *p3 = order[3 * r + 2];

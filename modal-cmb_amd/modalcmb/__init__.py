@@ -110,6 +110,11 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_gamma_cross_gl.argtypes = [vp, ip, ip, vp, vp]
         lib.mcmb_spline_weights.argtypes = [ip, ip, vp, vp]
         lib.init_orthol_glint.restype = dp
+        lib.mcmb_file_name.argtypes = [C.c_char_p, C.c_size_t, ip, ip, C.c_char_p, C.c_char_p] + [ip] * 7
+        lib.mcmb_write_qtilde.argtypes = [C.c_char_p, C.c_char_p, ip, ip, ip, ip, vp, vp, vp]
+        lib.mcmb_read_qtilde.argtypes = [C.c_char_p, C.c_char_p, ip, ip, ip, ip, C.POINTER(ip), C.POINTER(ip), vp,
+                                         vp, vp]
+        lib.mcmb_basis_late_order9.argtypes = [ip, ip, dp, dp, vp, vp]
         lib.permsix.restype = dp
         lib.permsix.argtypes = [ip, ip, ip]
         lib.calculate_geometric.restype = dp
@@ -138,7 +143,8 @@ EXPORTED_SYMBOLS = (
     "mcmb_spline_weights", "mcmb_decompose_gamma_TTT_3D", "decompose_gamma_TTT_3D_mpi", "calculate_gamma_2d",
     "calculate_gamma", "init_gamma_glint", "precompute_gammaMij_T", "calculate_gammaMij_T", "sync_gammaMij",
     "get_qtilde_xsize", "get_qtilde_xvec", "get_qtilde_T_lsize", "init_orthol_glint", "calculate_orthoMij_T",
-    "sync_orthoMij",
+    "sync_orthoMij", "mcmb_file_name", "mcmb_write_qtilde", "mcmb_read_qtilde", "mcmb_basis_late_order9",
+    "find_perm_late_TTT",
 )
 
 

@@ -86,21 +86,21 @@ def test_gamma_fp64_lmax500_vs_oracle(ctx):
     assert np.array_equal(G, ctx.gamma_tet(modalcmb.FP64))
 
 
-def test_legacy_gamma_3d_host_and_orthol(ctx):
+def test_legacy_orthol_3d(ctx):
+    # calculate_orthol_3D reads the cached FP64 tetrapyd Gram of the bound
+    # context (gamma_3d_host is the cross-Gamma: tests/test_gpu_cross.py)
     import ctypes as C
     lib = modalcmb.load_library()
     ws = W.lspace(80, 4)
     ctx.lspace_set(2, 80, ws["q"], ws["v"], ws["order"])
     G = ctx.gamma_tet(modalcmb.FP64)
     lib.mcmb_legacy_bind(ctx._h)
-    n = G.shape[0]
-    buf = np.full((n, n), 7.0)
-    lib.gamma_3d_host(buf.ctypes.data_as(C.c_void_p), n, 0, 1, C.c_double(0.0))
-    assert lib.mcmb_legacy_status() == 0
-    assert np.array_equal(buf, G)
-    res = (C.c_double * 4)()
-    assert lib.calculate_orthol_3D(3, 7, res) == 0 and res[0] == G[3, 7]
-    lib.mcmb_legacy_bind(None)
+    try:
+        res = (C.c_double * 4)()
+        assert lib.calculate_orthol_3D(3, 7, res) == 0 and res[0] == G[3, 7]
+        assert lib.calculate_orthol_3D(3, 700, res) == -1
+    finally:
+        lib.mcmb_legacy_bind(None)
 
 
 @pytest.mark.parametrize("prec", [modalcmb.FP64, modalcmb.MIXED])
@@ -112,24 +112,6 @@ def test_slices_sum_to_whole(ctx, prec):
     assert sum(f for _, f in parts) == pytest.approx(1.0, abs=1e-12)
     S = sum(p for p, _ in parts)
     assert rel_elem(S, G) < (1e-13 if prec == modalcmb.FP64 else 2e-5)
-
-
-def test_legacy_gamma_3d_host_ranks(ctx):
-    import ctypes as C
-    lib = modalcmb.load_library()
-    ws = W.lspace(90, 4)
-    ctx.lspace_set(2, 90, ws["q"], ws["v"], ws["order"])
-    G = ctx.gamma_tet(modalcmb.FP64)
-    lib.mcmb_legacy_bind(ctx._h)
-    n = G.shape[0]
-    tot = np.zeros((n, n))
-    for r in range(3):
-        buf = np.empty((n, n))
-        lib.gamma_3d_host(buf.ctypes.data_as(C.c_void_p), n, r, 3, C.c_double(0.0))
-        assert lib.mcmb_legacy_status() == 0
-        tot += buf
-    assert rel_elem(tot, G) < 1e-13
-    lib.mcmb_legacy_bind(None)
 
 
 @pytest.mark.parametrize("L,R,mod", [(60, 4, False), (150, 6, True), (500, 10, False)])

@@ -112,6 +112,19 @@ int mcmb_gamma_tet_lrange(mcmb_ctx* ctx, int precision, int l1_lo, int l1_hi, do
  * orthogonalisation_late.c:239-336) with npt nodes (npt <= 0: 3 lmax/2 + 1). */
 int mcmb_gamma_gl(mcmb_ctx* ctx, int npt, double* gamma_host, double* gamma_dev);
 
+/* The same GL Gram into the caller's device buffer gamma_dev (n x n doubles)
+ * without allocating: no hipMalloc, hipFree or allocating library call.  The
+ * node table, Legendre table, pair products and M live in one scratch array
+ * per GPU (128 MiB, held until the process ends), enough for lmax 2500 with
+ * basis width 32 and 3751 nodes; larger setups (more than 4096 nodes or 128
+ * MiB of tables) are rejected with MCMB_ERR_ARG.  Every call on a GPU shares
+ * that GPU's array, so calls that overlap on one GPU (other threads, other
+ * contexts on the same device) race; the call returns when its kernels have
+ * finished.  No reference counterpart: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). */
+int mcmb_gamma_gl_noalloc(mcmb_ctx* ctx, int npt, double* gamma_dev);
+
 /* lmin, lmax, mode count and a generation number that changes on every
  * successful mcmb_lspace_set (any output pointer may be NULL). */
 int mcmb_lspace_info(mcmb_ctx* ctx, int* lmin, int* lmax, int* nmodes, long* generation);

@@ -131,6 +131,12 @@ int mcmb_create(mcmb_ctx** out, int device)
         mcmb_destroy(c);
         return fail(MCMB_ERR_HIP, "mcmb_create: %s", hipGetErrorString(e));
     }
+    // the allocation-free GL Gram's scratch address on this device (device current)
+    int st = gl_scratch_setup(device);
+    if (st != MCMB_OK) {
+        mcmb_destroy(c);
+        return st;
+    }
     *out = c;
     return MCMB_OK;
 }

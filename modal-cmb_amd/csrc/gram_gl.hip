@@ -13,6 +13,7 @@
 // O(R^2 npt L + n^2 npt) instead of O(n^2 N_tet).
 #include "mcmb_internal.h"
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace mcmb {
@@ -250,6 +251,36 @@ int gram_gl_launch(mcmb_ctx* c, int npt, double* d_gamma)
     return gl_run(c, npt, x.data(), w.data(), gl_carve(buf.as<char>(), npt, nl, npair), d_gamma);
 }
 
+// ---- the allocation-free variant ----------------------------------------
+// Scratch of mcmb_gamma_gl_noalloc: one array per GPU, held until the process
+// ends, shared by every call on that GPU (calls that overlap on one GPU race).
+// 128 MiB covers the GL Gram up to lmax 2500 with basis width 32 (3751 nodes).
+constexpr size_t GL_SCRATCH_BYTES = (size_t)128 << 20;
+// Node count whose node table fits the host stack of the call.
+constexpr int GL_NPT_MAX = 4096;
+static __device__ __align__(256) double scratch[GL_SCRATCH_BYTES / sizeof(double)];
+
+constexpr int GL_MAX_DEVICES = 64;
+static void* g_scratch[GL_MAX_DEVICES] = {};
+
+// Device address of this GPU's scratch array, looked up once per device with
+// that device current (the array's name is a host-side stand-in, not a
+// device address).  Called from mcmb_create.
+int gl_scratch_setup(int device)
+{
+    if (device < 0 || device >= GL_MAX_DEVICES)
+        return fail(MCMB_ERR_ARG, "device %d beyond the %d the scratch table holds", device, GL_MAX_DEVICES);
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    if (g_scratch[device]) return MCMB_OK;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(MCMB_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch[device] = ptr;
+    return MCMB_OK;
+}
+
 }  // namespace mcmb
 
 using namespace mcmb;
@@ -272,4 +303,29 @@ extern "C" int mcmb_gamma_gl(mcmb_ctx* c, int npt, double* gamma_host, double* g
         MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     return MCMB_OK;
+}
+
+extern "C" int mcmb_gamma_gl_noalloc(mcmb_ctx* c, int npt, double* gamma_dev)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_gamma_gl_noalloc: null context");
+    if (!c->ls.ready) return fail(MCMB_ERR_STATE, "mcmb_gamma_gl_noalloc: call mcmb_lspace_set first");
+    if (!gamma_dev) return fail(MCMB_ERR_ARG, "mcmb_gamma_gl_noalloc: gamma_dev (n x n device doubles) is required");
+    if (npt > 0 && 2 * npt - 1 < 3 * c->ls.lmax)
+        return fail(MCMB_ERR_ARG, "mcmb_gamma_gl_noalloc: npt %d < (3 lmax + 1)/2 is not exact", npt);
+    MCMB_TRY(set_device(c));
+    MCMB_TRY(gl_check(c, &npt));
+    if (npt > GL_NPT_MAX)
+        return fail(MCMB_ERR_ARG, "mcmb_gamma_gl_noalloc: %d nodes exceed the limit of %d", npt, GL_NPT_MAX);
+    const LSpace& L = c->ls;
+    const int n = L.n, R = L.R, nl = L.lmax + 1, npair = R * (R + 1) / 2;
+    const int T = (n + 63) / 64;
+    const size_t need = gl_bytes(npt, nl, npair, T * (T + 1) / 2);
+    if (need > GL_SCRATCH_BYTES)
+        return fail(MCMB_ERR_ARG, "mcmb_gamma_gl_noalloc: lmax %d, basis width %d, %d nodes, %d modes need %zu bytes "
+                    "of scratch, above the limit of %zu", L.lmax, R, npt, n, need, GL_SCRATCH_BYTES);
+    char* base = static_cast<char*>(c->device >= 0 && c->device < GL_MAX_DEVICES ? g_scratch[c->device] : nullptr);
+    if (!base) return fail(MCMB_ERR_STATE, "mcmb_gamma_gl_noalloc: no scratch on device %d", c->device);
+    double x[GL_NPT_MAX], w[GL_NPT_MAX];
+    MCMB_TRY(mcmb_gauss_legendre(npt, x, w));
+    return gl_run(c, npt, x, w, gl_carve(base, npt, nl, npair), gamma_dev);
 }

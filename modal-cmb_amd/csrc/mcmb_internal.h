@@ -224,6 +224,9 @@ int gram_tet_launch(mcmb_ctx* ctx, PointSet& P, int precision, double* d_gamma, 
 void lspace_slice(const PointSet& L, int kc, int rank, int nranks, int* row_lo, int* row_hi, double* frac);
 int upload_pointset(mcmb_ctx* ctx, PointSet& P, const std::vector<int>& order);
 int gram_gl_launch(mcmb_ctx* ctx, int npt, double* d_gamma);
+// Look up this device's scratch array of the allocation-free GL Gram (once per
+// device, with the device current).
+int gl_scratch_setup(int device);
 int set_device(mcmb_ctx* ctx);
 // Per-point weights of P (walk order) into ws, scaled = sqrt(w)/6 if nonzero;
 // rowptr[r] = first point of row r (built and uploaded unless rowptr_ready).

@@ -57,6 +57,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_lspace_set.argtypes = [vp, ip, ip, ip, vp, vp, ip, vp, ip]
         lib.mcmb_gamma_tet.argtypes = [vp, ip, vp, vp]
         lib.mcmb_gamma_gl.argtypes = [vp, ip, vp, vp]
+        lib.mcmb_gamma_gl_noalloc.argtypes = [vp, ip, vp]
         lib.mcmb_gamma_tet_slice.argtypes = [vp, ip, ip, ip, vp, vp, C.POINTER(dp)]
         lib.mcmb_lspace_npoints.argtypes = [vp]
         lib.mcmb_lspace_npoints.restype = C.c_long
@@ -125,7 +126,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 
 EXPORTED_SYMBOLS = (
     "mcmb_last_error", "mcmb_status_string", "mcmb_create", "mcmb_destroy", "mcmb_sync",
-    "mcmb_lspace_set", "mcmb_gamma_tet", "mcmb_gamma_gl", "mcmb_lspace_npoints",
+    "mcmb_lspace_set", "mcmb_gamma_tet", "mcmb_gamma_gl", "mcmb_gamma_gl_noalloc", "mcmb_lspace_npoints",
     "mcmb_lspace_points", "mcmb_lspace_weights", "mcmb_orthonormalise", "mcmb_cholesky_solve",
     "mcmb_last_kernel_ms", "mcmb_last_phases", "mcmb_last_basis_rank", "mcmb_device_alloc", "mcmb_device_free", "mcmb_device_to_host", "mcmb_device_copy", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
     "mcmb_gauss_legendre", "mcmb_gamma_tet_slice", "mcmb_gamma_tet_lrange", "mcmb_lspace_points_at", "mcmb_kspace_set", "mcmb_kspace_npoints",

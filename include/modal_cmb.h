@@ -51,6 +51,12 @@ extern "C" {
 #define MCMB_H2_EXACT      0  /* (2l1+1)(2l2+1)(2l3+1)/(4pi) (l1 l2 l3;0 0 0)^2   */
 #define MCMB_H2_ASYMPTOTIC 1  /* calculate_geometric.c:59-92                     */
 
+/* polarisation components (calculate_orthol / calculate_gamma result[0..3]) */
+#define MCMB_POL_TTT 0
+#define MCMB_POL_TTE 1
+#define MCMB_POL_TEE 2
+#define MCMB_POL_EEE 3
+
 typedef struct mcmb_ctx mcmb_ctx;
 
 /* tetrapyd_tools.h:31-37 */
@@ -111,6 +117,31 @@ int mcmb_gamma_tet_lrange(mcmb_ctx* ctx, int precision, int l1_lo, int l1_hi, do
 /* The same Gram through the Gauss-Legendre factorisation (calculate_orthol,
  * orthogonalisation_late.c:239-336) with npt nodes (npt <= 0: 3 lmax/2 + 1). */
 int mcmb_gamma_gl(mcmb_ctx* ctx, int npt, double* gamma_host, double* gamma_dev);
+
+/* E-mode field of the late basis for the polarised Gram (create_basis_late's
+ * E columns, fixed_data.c:1923-1950; the TTE / TEE / EEE tables of
+ * set_terms_late, fixed_data.c:272-417): basis qE[l][ldq] for l = 0..E_lmax
+ * (rows below E_lmin unused), per-leg weight vE[l], and nmodes rows per
+ * component table in find_perm_late_XXX's slot order: order_TTE (T, T, E),
+ * order_TEE (T, E, E) -- the reference's TEE file lists (E, E, T) and
+ * set_terms_late reorders it --, order_EEE (E, E, E).  T indices refer to
+ * the basis of mcmb_lspace_set. */
+int mcmb_lspace_set_E(mcmb_ctx* ctx, int E_lmin, int E_lmax, int ldq, const double* qE, const double* vE,
+                      int nmodes, const int* order_TTE, const int* order_TEE, const int* order_EEE);
+/* Drop the E setup (the reference's do_polarisation = 0). */
+int mcmb_lspace_clear_E(mcmb_ctx* ctx);
+/* E_lmin, E_lmax, mode count and generation of the E setup (outputs may be NULL). */
+int mcmb_lspace_E_info(mcmb_ctx* ctx, int* E_lmin, int* E_lmax, int* nmodes, long* generation);
+/* Component pol (MCMB_POL_*) of the Gauss-Legendre Gram, calculate_orthol
+ * (orthogonalisation_late.c:239-336, M^E of calculate_orthoMij_E 131-157):
+ * with M^X_ij(x_s) = sum_{l in X's range} (2l+1) v_X(l) P_l(x_s) q^X_i q^X_j,
+ *   TTT, EEE: sum_s w_s/(48 pi) perm(M_{m_a n_b}), one field;
+ *   TTE: sum_s w_s/(16 pi) (M^T_{m0 n0} M^T_{m1 n1} + M^T_{m1 n0} M^T_{m0 n1}) M^E_{m2 n2};
+ *   TEE: sum_s w_s/(16 pi) M^T_{m0 n0} (M^E_{m1 n1} M^E_{m2 n2} + M^E_{m2 n1} M^E_{m1 n2}).
+ * npt <= 0: 3 max(T lmax, E lmax)/2 + 1 (init_orthol_glint with
+ * polarisation); npt must be exact for the component's degree.  n x n with n
+ * = the l-space mode count (TTT) or the E setup's (the others). */
+int mcmb_gamma_gl_pol(mcmb_ctx* ctx, int pol, int npt, double* gamma_host, double* gamma_dev);
 
 /* The same GL Gram into the caller's device buffer gamma_dev (n x n doubles)
  * without allocating: no hipMalloc, hipFree or allocating library call.  The
@@ -277,6 +308,8 @@ int mcmb_kuniform_tables(mcmb_ctx* ctx, const double** q, int* ldq, const int** 
                          int* max);
 int mcmb_lspace_tables(mcmb_ctx* ctx, const double** q, int* ldq, const int** order, int* nmodes,
                        int* lmin, int* lmax);
+int mcmb_lspace_E_tables(mcmb_ctx* ctx, const double** q, int* ldq, const int** order_TTE,
+                         const int** order_TEE, const int** order_EEE, int* nmodes, int* E_lmin, int* E_lmax);
 
 /* ---- Gamma post-processing (gamma_mpi.c:260-295, 801-840) --------------- */
 /* orthoinv = lambda^T lambda (orthoinv[i][j] = sum_k lambda[k][i] lambda[k][j])
@@ -386,6 +419,17 @@ void check_ijk_max_prim(int max, int* i, int* j, int* k);
 void decompose_tetrapyd_prim(tetrapyd_limits* thread_limits, int rank, int numranks,
                              int nthreads, int min, int max);
 void divide_tasks(long int tasks, int nranks, long int* tasks_per_rank);
+/* tetrapyd_tools.h:81-84 (tetrapyd_tools.c:226-414): the mixed walk, i <= j
+ * over [lminX, lmaxX], k over [lminY, lmaxY], bit-exact with the reference
+ * including its quirks (a new i restarts at k = lminY without a parity bump;
+ * a j with an empty k range still yields one point; an odd lmaxY ends the
+ * walk at (lmaxX - 1, lmaxX, lmaxY)), so it is the mixed tetrapyd exactly
+ * when lminX = lminY and lmaxX = lmaxY, both even. */
+void decompose_tetrapyd_XXY(tetrapyd_limits* thread_limits, int rank, int numranks, int nthreads, int lminX,
+                            int lminY, int lmaxX, int lmaxY);
+int  get_ijk_next_XXY(int lminX, int lminY, int lmaxX, int lmaxY, int* i, int* j, int* k);
+void check_ijk_max_XXY(int lmaxX, int lmaxY, int* i, int* j, int* k);
+void check_ijk_min_XXY(int lminX, int lminY, int* i, int* j, int* k);
 /* get_next_ijk_bound with an explicit lsize = lmax + 1 (host only) */
 void mcmb_next_ijk_bound(int lsize, int i0, int j0, int k0, int* i1, int* j1, int* k1,
                          int final_i, int final_j, int final_k, int* blocksize,
@@ -445,12 +489,17 @@ int get_qtilde_T_lsize(void);
  * is formed by the first calculate_orthol), sync_orthoMij has nothing to sum. */
 double init_orthol_glint(void);
 void calculate_orthoMij_T(int i, int j);
+/* orthogonalisation_late.c:131 -- the same for the E field (MCMB_ERR_STATE in
+ * mcmb_legacy_status without mcmb_lspace_set_E) */
+void calculate_orthoMij_E(int i, int j);
 void sync_orthoMij(void);
 /* orthogonalisation_late.c:338 -- result[0] = Gamma_mn (TTT) of the bound
  * context's cached FP64 tetrapyd Gram (computed on first use). */
 int calculate_orthol_3D(int m, int n, double* result);
 /* orthogonalisation_late.c:239 -- result[0] = Gamma_mn from the cached
- * Gauss-Legendre-factorised Gram. */
+ * Gauss-Legendre-factorised Gram; with an E setup bound (the reference's
+ * do_polarisation = 1), result[0..3] = TTT, TTE, TEE, EEE of
+ * mcmb_gamma_gl_pol, all on 3 max(T lmax, E lmax)/2 + 1 nodes. */
 int calculate_orthol(int m, int n, double* result);
 /* orthogonalisation.c:14 -- adds the FP64 Gram of the share of the uniform
  * walk covered by block[0 .. nthreads) of the last decompose_tetrapyd_prim
@@ -468,11 +517,22 @@ double pijk(int r, int i, int j, int k);
 /* orthogonalisation_late.c:620 -- plijk_TTT of the bound l-space setup
  * (basis row = multipole; 0 outside [lmin, lmax]) */
 double plijk_TTT(int r, int l1, int l2, int l3);
+/* orthogonalisation_late.c:646-715 -- the mixed mode products of the bound
+ * T and E setups (legs T, T, E / T, E, E / E, E, E; 0 when a leg is outside
+ * its field's range) */
+double plijk_TTE(int r, int l1, int l2, int l3);
+double plijk_TEE(int r, int l1, int l2, int l3);
+double plijk_EEE(int r, int l1, int l2, int l3);
 /* fixed_data.c:263 */
 void find_perm_prim(int r, int* p1, int* p2, int* p3);
 /* fixed_data.c:983-990 -- mode r of the bound l-space setup's order table
  * (-1s outside it) */
 void find_perm_late_TTT(int r, int* p1, int* p2, int* p3);
+/* fixed_data.c:992-1019 -- mode r of the bound E setup's tables, in the slot
+ * order given to mcmb_lspace_set_E (-1s outside them) */
+void find_perm_late_TTE(int r, int* p1, int* p2, int* p3);
+void find_perm_late_TEE(int r, int* p1, int* p2, int* p3);
+void find_perm_late_EEE(int r, int* p1, int* p2, int* p3);
 /* global.h:429, 730 -- of the bound uniform k-grid setup (0 if unset) */
 int get_terms_prim(void);
 double get_kmax(void);

@@ -426,6 +426,99 @@ int mcmb_gamma_tet_lrange(mcmb_ctx* c, int precision, int l1_lo, int l1_hi, doub
 
 }  // extern "C"
 
+extern "C" int mcmb_lspace_set_E(mcmb_ctx* c, int E_lmin, int E_lmax, int ldq, const double* qE, const double* vE,
+                                 int nmodes, const int* order_TTE, const int* order_TEE, const int* order_EEE)
+{
+    if (!c || !qE || !vE || !order_TTE || !order_TEE || !order_EEE)
+        return fail(MCMB_ERR_ARG, "mcmb_lspace_set_E: null argument");
+    if (E_lmin < 0 || E_lmax < E_lmin + 1 || E_lmax > 8190)
+        return fail(MCMB_ERR_ARG, "mcmb_lspace_set_E: need 0 <= E_lmin < E_lmax <= 8190 (got %d, %d)", E_lmin, E_lmax);
+    if (ldq < 1 || ldq > 32) return fail(MCMB_ERR_ARG, "mcmb_lspace_set_E: ldq %d not in [1,32]", ldq);
+    if (nmodes < 1 || nmodes > 65536) return fail(MCMB_ERR_ARG, "mcmb_lspace_set_E: nmodes %d", nmodes);
+    const int* tabs[3] = {order_TTE, order_TEE, order_EEE};
+    // slot field per table: 0 = T (index into the l-space basis, < 32), 1 = E (< ldq)
+    static const int field[3][3] = {{0, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    for (int t = 0; t < 3; ++t)
+        for (int m = 0; m < nmodes; ++m)
+            for (int a = 0; a < 3; ++a) {
+                const int p = tabs[t][3 * m + a], lim = field[t][a] ? ldq : 32;
+                if (p < 0 || p >= lim)
+                    return fail(MCMB_ERR_ORDER, "mcmb_lspace_set_E: table %d, mode %d, slot %d: index %d outside [0,%d)",
+                                t, m, a, p, lim);
+            }
+    MCMB_TRY(set_device(c));
+    PolSet& E = c->es;
+    E.ready = false;
+    E.lmin = E_lmin;
+    E.lmax = E_lmax;
+    E.ldq = ldq;
+    E.n = nmodes;
+    E.q.assign(qE, qE + (size_t)(E_lmax + 1) * ldq);
+    E.v.assign(vE, vE + (size_t)(E_lmax + 1));
+    MCMB_TRY(E.d_q.alloc(sizeof(double) * E.q.size()));
+    MCMB_TRY(E.d_v.alloc(sizeof(double) * E.v.size()));
+    MCMB_HIP(hipMemcpyAsync(E.d_q.p, E.q.data(), sizeof(double) * E.q.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(E.d_v.p, E.v.data(), sizeof(double) * E.v.size(), hipMemcpyHostToDevice, c->stream));
+    std::vector<int> packed[3];
+    for (int t = 0; t < 3; ++t) {
+        E.order[t].assign(tabs[t], tabs[t] + 3 * (size_t)nmodes);
+        packed[t].resize(nmodes);
+        for (int m = 0; m < nmodes; ++m)
+            packed[t][m] = tabs[t][3 * m] | (tabs[t][3 * m + 1] << 8) | (tabs[t][3 * m + 2] << 16);
+        MCMB_TRY(E.d_modes[t].alloc(sizeof(int) * nmodes));
+        MCMB_HIP(hipMemcpyAsync(E.d_modes[t].p, packed[t].data(), sizeof(int) * nmodes, hipMemcpyHostToDevice,
+                                c->stream));
+    }
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    E.ready = true;
+    E.generation = next_generation();
+    return MCMB_OK;
+}
+
+extern "C" int mcmb_lspace_clear_E(mcmb_ctx* c)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_lspace_clear_E: null context");
+    PolSet& E = c->es;
+    E.ready = false;
+    E.generation = -1;
+    E.q.clear();
+    E.v.clear();
+    E.d_q.release();
+    E.d_v.release();
+    for (int t = 0; t < 3; ++t) {
+        E.order[t].clear();
+        E.d_modes[t].release();
+    }
+    return MCMB_OK;
+}
+
+extern "C" int mcmb_lspace_E_info(mcmb_ctx* c, int* E_lmin, int* E_lmax, int* nmodes, long* generation)
+{
+    if (!c || !c->es.ready) return fail(MCMB_ERR_STATE, "mcmb_lspace_E_info: no E-mode setup");
+    if (E_lmin) *E_lmin = c->es.lmin;
+    if (E_lmax) *E_lmax = c->es.lmax;
+    if (nmodes) *nmodes = c->es.n;
+    if (generation) *generation = c->es.generation;
+    return MCMB_OK;
+}
+
+extern "C" int mcmb_lspace_E_tables(mcmb_ctx* c, const double** q, int* ldq, const int** order_TTE,
+                                    const int** order_TEE, const int** order_EEE, int* nmodes, int* E_lmin,
+                                    int* E_lmax)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_lspace_E_tables: null context");
+    if (!c->es.ready) return fail(MCMB_ERR_STATE, "mcmb_lspace_E_tables: call mcmb_lspace_set_E first");
+    if (q) *q = c->es.q.data();
+    if (ldq) *ldq = c->es.ldq;
+    if (order_TTE) *order_TTE = c->es.order[0].data();
+    if (order_TEE) *order_TEE = c->es.order[1].data();
+    if (order_EEE) *order_EEE = c->es.order[2].data();
+    if (nmodes) *nmodes = c->es.n;
+    if (E_lmin) *E_lmin = c->es.lmin;
+    if (E_lmax) *E_lmax = c->es.lmax;
+    return MCMB_OK;
+}
+
 extern "C" int mcmb_lspace_tables(mcmb_ctx* c, const double** q, int* ldq, const int** order, int* nmodes,
                                   int* lmin, int* lmax)
 {

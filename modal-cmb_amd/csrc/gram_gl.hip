@@ -251,6 +251,184 @@ int gram_gl_launch(mcmb_ctx* c, int npt, double* d_gamma)
     return gl_run(c, npt, x.data(), w.data(), gl_carve(buf.as<char>(), npt, nl, npair), d_gamma);
 }
 
+// ---- polarised components (calculate_orthol result[1..3]) -----------------
+
+// Mixed components over 64 x 64 mode tiles (I <= J), per node s:
+//   KIND 1, TTE, modes (T, T, E): (MT_{a0 b0} MT_{a1 b1} + MT_{a1 b0} MT_{a0 b1}) ME_{a2 b2}
+//   KIND 2, TEE, modes (T, E, E): MT_{a0 b0} (ME_{a1 b1} ME_{a2 b2} + ME_{a2 b1} ME_{a1 b2})
+// summed with w_s / (16 pi) (orthogonalisation_late.c:302-316).  As
+// gl_gamma, with both fields' M staged in LDS GS nodes at a time.
+constexpr int GS = 3;
+template <int KIND>
+__global__ __launch_bounds__(256) void gl_gamma_mixed(const double* __restrict__ MT, int RT,
+                                                      const double* __restrict__ ME, int RE,
+                                                      const double* __restrict__ wgt, int npt,
+                                                      const int* __restrict__ modes, int n,
+                                                      const int2* __restrict__ tiles, double* __restrict__ out)
+{
+    __shared__ double Ts[GS][GLR * GLR], Es[GS][GLR * GLR];
+    __shared__ double ws[GS];
+    const int2 IJ = tiles[blockIdx.x];
+    const int tm = (threadIdx.x >> 4) * 4, tn = (threadIdx.x & 15) * 4;
+    int ma[4][3], nb[4][3];
+    bool okm[4], okn[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int m = IJ.x * 64 + tm + u, c = IJ.y * 64 + tn + u;
+        okm[u] = m < n;
+        okn[u] = c < n;
+        const int pm = okm[u] ? modes[m] : 0, pn = okn[u] ? modes[c] : 0;
+        ma[u][0] = pm & 255; ma[u][1] = (pm >> 8) & 255; ma[u][2] = (pm >> 16) & 255;
+        nb[u][0] = pn & 255; nb[u][1] = (pn >> 8) & 255; nb[u][2] = (pn >> 16) & 255;
+    }
+    // stage R x R of the pair-packed Mp (pairs of one field enumerated as in gl_pairs)
+    auto stage = [&](double (*S)[GLR * GLR], const double* Mp, int R, int s0) {
+        for (int e = threadIdx.x; e < GS * R * R; e += 256) {
+            const int s = e / (R * R), ij = e % (R * R), i = ij / R, j = ij % R;
+            const int a = i < j ? i : j, b = i < j ? j : i;
+            const int p = a * R - a * (a - 1) / 2 + (b - a);
+            S[s][i * GLR + j] = (s0 + s < npt) ? Mp[(size_t)p * npt + s0 + s] : 0.0;
+        }
+    };
+    double acc[4][4] = {};
+    for (int s0 = 0; s0 < npt; s0 += GS) {
+        __syncthreads();
+        stage(Ts, MT, RT, s0);
+        stage(Es, ME, RE, s0);
+        if (threadIdx.x < GS) ws[threadIdx.x] = (s0 + threadIdx.x < npt) ? wgt[s0 + threadIdx.x] : 0.0;
+        __syncthreads();
+        for (int s = 0; s < GS; ++s) {
+            const double* A = Ts[s];
+            const double* B = Es[s];
+            const double w = ws[s];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    double t;
+                    if (KIND == 1) {
+                        t = (A[ma[u][0] * GLR + nb[v][0]] * A[ma[u][1] * GLR + nb[v][1]] +
+                             A[ma[u][1] * GLR + nb[v][0]] * A[ma[u][0] * GLR + nb[v][1]]) *
+                            B[ma[u][2] * GLR + nb[v][2]];
+                    } else {
+                        t = A[ma[u][0] * GLR + nb[v][0]] *
+                            (B[ma[u][1] * GLR + nb[v][1]] * B[ma[u][2] * GLR + nb[v][2]] +
+                             B[ma[u][2] * GLR + nb[v][1]] * B[ma[u][1] * GLR + nb[v][2]]);
+                    }
+                    acc[u][v] += w * t;
+                }
+        }
+    }
+    const double scale = 1.0 / (16.0 * M_PI);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = IJ.x * 64 + tm + u, c = IJ.y * 64 + tn + v;
+            if (okm[u] && okn[v] && m <= c) {
+                out[(size_t)m * n + c] = scale * acc[u][v];
+                out[(size_t)c * n + m] = scale * acc[u][v];
+            }
+        }
+}
+
+// One field of the polarised Gram: Legendre table, pair products and M over
+// [lmin, lmax] with basis q[l][ldq] (first R columns) and per-leg weight v.
+struct GlField {
+    int lmin, lmax, ldq, R;
+    const double* q;
+    const double* v;
+};
+
+// Node count of the polarised Gram: 3 max(T lmax, E lmax) / 2 + 1, as
+// init_orthol_glint takes it with polarisation on (orthogonalisation_late.c:33-36).
+int gl_pol_npt(const mcmb_ctx* c)
+{
+    int lmax = c->ls.lmax;
+    if (c->es.ready && c->es.lmax > lmax) lmax = c->es.lmax;
+    return 3 * lmax / 2 + 1;
+}
+
+// Components TTE (1), TEE (2), EEE (3) into the device buffer d_gamma (n x n,
+// n = the E setup's mode count).
+int gram_gl_pol_launch(mcmb_ctx* c, int pol, int npt, double* d_gamma)
+{
+    const PolSet& E = c->es;
+    const LSpace& L = c->ls;
+    const std::vector<int>& ord = E.order[pol - 1];
+    // slot field per component (0 = T, 1 = E) and the width each field needs
+    static const int field[3][3] = {{0, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    int R[2] = {0, 0};
+    for (int m = 0; m < E.n; ++m)
+        for (int a = 0; a < 3; ++a) {
+            const int f = field[pol - 1][a], p = ord[3 * m + a];
+            if (f == 0 && p >= L.ldq)
+                return fail(MCMB_ERR_ORDER, "polarised Gram: mode %d uses T basis column %d of %d", m, p, L.ldq);
+            R[f] = std::max(R[f], p + 1);
+        }
+    const GlField F[2] = {{L.lmin, L.lmax, L.ldq, R[0], L.d_q.as<double>(), L.d_v.as<double>()},
+                          {E.lmin, E.lmax, E.ldq, R[1], E.d_q.as<double>(), E.d_v.as<double>()}};
+    const int n = E.n, T = (n + 63) / 64, ntiles = T * (T + 1) / 2;
+    std::vector<double> x(npt), w(npt);
+    MCMB_TRY(mcmb_gauss_legendre(npt, x.data(), w.data()));
+    // x, w, then per used field Pt, U, M; then the tile list
+    size_t bytes = 2 * round256(sizeof(double) * npt) + round256(sizeof(int2) * (size_t)ntiles);
+    for (int f = 0; f < 2; ++f)
+        if (F[f].R > 0) {
+            const size_t nl = F[f].lmax + 1, np = (size_t)F[f].R * (F[f].R + 1) / 2;
+            bytes += round256(sizeof(double) * nl * npt) + round256(sizeof(double) * nl * np) +
+                     round256(sizeof(double) * np * npt);
+        }
+    DevBuf buf;
+    MCMB_TRY(buf.alloc(bytes));
+    char* p = buf.as<char>();
+    double* dx = reinterpret_cast<double*>(p);
+    p += round256(sizeof(double) * npt);
+    double* dw = reinterpret_cast<double*>(p);
+    p += round256(sizeof(double) * npt);
+    int2* dtiles = reinterpret_cast<int2*>(p);
+    p += round256(sizeof(int2) * (size_t)ntiles);
+    MCMB_HIP(hipMemcpyAsync(dx, x.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(dw, w.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(gl_tiles, dim3((ntiles + 255) / 256), dim3(256), 0, c->stream, T, dtiles);
+    const double* Mf[2] = {nullptr, nullptr};
+    for (int f = 0; f < 2; ++f) {
+        if (F[f].R == 0) continue;
+        const int nl = F[f].lmax + 1, np = F[f].R * (F[f].R + 1) / 2;
+        double* Pt = reinterpret_cast<double*>(p);
+        p += round256(sizeof(double) * (size_t)nl * npt);
+        double* U = reinterpret_cast<double*>(p);
+        p += round256(sizeof(double) * (size_t)nl * np);
+        double* M = reinterpret_cast<double*>(p);
+        p += round256(sizeof(double) * (size_t)np * npt);
+        hipLaunchKernelGGL(gl_legendre, dim3((npt + 255) / 256), dim3(256), 0, c->stream, (const double*)dx, npt,
+                           F[f].lmin, F[f].lmax, F[f].v, Pt);
+        hipLaunchKernelGGL(gl_pairs, dim3((unsigned)((nl * np + 255) / 256)), dim3(256), 0, c->stream, F[f].q,
+                           F[f].ldq, F[f].R, F[f].lmax, U, np);
+        hipLaunchKernelGGL(gl_mgemm, dim3((npt + 63) / 64, (np + 31) / 32), dim3(256), 0, c->stream,
+                           (const double*)Pt, (const double*)U, npt, np, nl, M);
+        Mf[f] = M;
+    }
+    const int* modes = E.d_modes[pol - 1].as<int>();
+    if (pol == 3)
+        hipLaunchKernelGGL(gl_gamma, dim3((unsigned)ntiles), dim3(256), 0, c->stream, Mf[1], (const double*)dw, npt,
+                           R[1], modes, n, (const int2*)dtiles, d_gamma);
+    else if (pol == 1)
+        hipLaunchKernelGGL(gl_gamma_mixed<1>, dim3((unsigned)ntiles), dim3(256), 0, c->stream, Mf[0], R[0], Mf[1],
+                           R[1], (const double*)dw, npt, modes, n, (const int2*)dtiles, d_gamma);
+    else
+        hipLaunchKernelGGL(gl_gamma_mixed<2>, dim3((unsigned)ntiles), dim3(256), 0, c->stream, Mf[0], R[0], Mf[1],
+                           R[1], (const double*)dw, npt, modes, n, (const int2*)dtiles, d_gamma);
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipEventRecord(c->ev1, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    return MCMB_OK;
+}
+
 // ---- the allocation-free variant ----------------------------------------
 // Scratch of mcmb_gamma_gl_noalloc: one array per GPU, held until the process
 // ends, shared by every call on that GPU (calls that overlap on one GPU race).
@@ -299,6 +477,35 @@ extern "C" int mcmb_gamma_gl(mcmb_ctx* c, int npt, double* gamma_host, double* g
         dst = c->d_gamma_tmp.as<double>();
     }
     MCMB_TRY(gram_gl_launch(c, npt, dst));
+    if (gamma_host)
+        MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return MCMB_OK;
+}
+
+extern "C" int mcmb_gamma_gl_pol(mcmb_ctx* c, int pol, int npt, double* gamma_host, double* gamma_dev)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_gamma_gl_pol: null context");
+    if (pol < MCMB_POL_TTT || pol > MCMB_POL_EEE) return fail(MCMB_ERR_ARG, "mcmb_gamma_gl_pol: pol %d", pol);
+    if (!c->ls.ready) return fail(MCMB_ERR_STATE, "mcmb_gamma_gl_pol: call mcmb_lspace_set first");
+    if (pol != MCMB_POL_TTT && !c->es.ready)
+        return fail(MCMB_ERR_STATE, "mcmb_gamma_gl_pol: call mcmb_lspace_set_E first");
+    // degree of the node integrand: the sum of the three legs' lmax
+    const int lt = c->ls.lmax, le = c->es.ready ? c->es.lmax : lt;
+    const int deg = pol == MCMB_POL_TTT ? 3 * lt : pol == MCMB_POL_TTE ? 2 * lt + le
+                  : pol == MCMB_POL_TEE ? lt + 2 * le : 3 * le;
+    if (npt <= 0) npt = gl_pol_npt(c);
+    if (2 * npt - 1 < deg)
+        return fail(MCMB_ERR_ARG, "mcmb_gamma_gl_pol: %d nodes are not exact for degree %d", npt, deg);
+    MCMB_TRY(set_device(c));
+    const int n = pol == MCMB_POL_TTT ? c->ls.n : c->es.n;
+    double* dst = gamma_dev;
+    if (!dst) {
+        MCMB_TRY(c->d_gamma_tmp.alloc(sizeof(double) * (size_t)n * n));
+        dst = c->d_gamma_tmp.as<double>();
+    }
+    if (pol == MCMB_POL_TTT) MCMB_TRY(gram_gl_launch(c, npt, dst));
+    else MCMB_TRY(gram_gl_pol_launch(c, pol, npt, dst));
     if (gamma_host)
         MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));

@@ -19,8 +19,10 @@ static double* g_gamma3d = NULL;        /* cached FP64 tetrapyd Gram, n x n */
 static double* g_gammagl = NULL;        /* cached GL-factorised Gram, n x n */
 static double* g_cross[2] = {NULL, NULL};   /* cached GL cross-Gamma, FMM / natural spline */
 static long g_cross_gen[2] = {-1, -1};  /* q-tilde generation of each cross cache */
+static double* g_pol[4] = {NULL, NULL, NULL, NULL};   /* cached polarised GL Grams, TTT..EEE */
 static int g_n = 0;
 static long g_gen = -1;                 /* l-space generation the caches belong to */
+static long g_egen = -1;                /* E setup generation the caches belong to (-1: none) */
 static int g_prim_nblocks = 1;          /* thread count of the last decompose_tetrapyd_prim */
 
 static void drop_caches(void)
@@ -28,6 +30,7 @@ static void drop_caches(void)
     free(g_gamma3d); g_gamma3d = NULL;
     free(g_gammagl); g_gammagl = NULL;
     for (int k = 0; k < 2; ++k) { free(g_cross[k]); g_cross[k] = NULL; g_cross_gen[k] = -1; }
+    for (int k = 0; k < 4; ++k) { free(g_pol[k]); g_pol[k] = NULL; }
 }
 
 int mcmb_legacy_bind(mcmb_ctx* ctx)
@@ -37,6 +40,7 @@ int mcmb_legacy_bind(mcmb_ctx* ctx)
     drop_caches();
     g_n = 0;
     g_gen = -1;
+    g_egen = -1;
     g_status = MCMB_OK;
     pthread_mutex_unlock(&g_mu);
     return MCMB_OK;
@@ -51,9 +55,12 @@ static int refresh(void)
     if (!g_ctx) return MCMB_ERR_STATE;
     int st = mcmb_lspace_info(g_ctx, &lmin, &lmax, &n, &gen);
     if (st != MCMB_OK) return st;
-    if (gen != g_gen) {
+    long egen = -1;
+    if (mcmb_lspace_E_info(g_ctx, NULL, NULL, NULL, &egen) != MCMB_OK) egen = -1;
+    if (gen != g_gen || egen != g_egen) {
         drop_caches();
         g_gen = gen;
+        g_egen = egen;
     }
     g_lsize = lmax + 1;
     g_n = n;
@@ -149,6 +156,92 @@ void decompose_tetrapyd_XXX(tetrapyd_limits* blk, int rank, int nranks, int nthr
     for (int t = 0; t < nthreads; ++t) {
         if (t)
             for (long s = 0; s < pt[t - 1]; ++s) get_ijk_next_XXX(lmax, &i, &j, &k);
+        blk[t].i_bgn = i; blk[t].j_bgn = j; blk[t].k_bgn = k; blk[t].loops = (int)pt[t];
+    }
+    free(pr);
+    free(pt);
+}
+
+/* tetrapyd_tools.c:299-337: the mixed walk, i <= j over the X range and k
+ * over the Y range, k from max(j - i, lminY) (parity-bumped) to min(i + j,
+ * lmaxY) in steps of 2.  Kept as the reference has it: a new i restarts at
+ * k = lminY with no parity bump, a j whose k range is empty still yields a
+ * point, and the walk ends on i + j + k + 1 >= 2 lmaxX + lmaxY at the point
+ * (lmaxX, lmaxX, lmaxY rounded down to even), which for an odd lmaxY is
+ * reached early, from (lmaxX - 1, lmaxX, lmaxY).  It is the mixed tetrapyd
+ * exactly when lminX = lminY and lmaxX = lmaxY, both even. */
+int get_ijk_next_XXY(int lminX, int lminY, int lmaxX, int lmaxY, int* pi, int* pj, int* pk)
+{
+    int i = *pi, j = *pj, k = *pk;
+    const int ktop = i + j < lmaxY ? i + j : lmaxY;
+    if (j == lmaxX && k + 2 > lmaxY) {
+        ++i;
+        j = i;
+        k = lminY;   /* (i + j) % 2 == 0 */
+    } else if (k + 2 > ktop) {
+        ++j;
+        k = j - i > lminY ? j - i : lminY;
+        if ((i + j + k) & 1) ++k;
+    } else {
+        k += 2;
+    }
+    if (i + j + k + 1 >= 2 * lmaxX + lmaxY) {
+        *pi = lmaxX; *pj = lmaxX; *pk = lmaxY - (lmaxY & 1);
+        return 0;
+    }
+    *pi = i; *pj = j; *pk = k;
+    return 1;
+}
+
+/* tetrapyd_tools.c:339-381 */
+void check_ijk_min_XXY(int lminX, int lminY, int* pi, int* pj, int* pk)
+{
+    int i = *pi < lminX ? lminX : *pi, j = *pj < lminX ? lminX : *pj, k = *pk < lminY ? lminY : *pk;
+    if (i > j) { int t = i; i = j; j = t; }
+    if (k > i + j) {
+        if (i + j >= lminY) k = i + j;
+        else if (lminY > 2 * lminX) { i = lminY / 2; j = i + (lminY & 1); k = lminY; }
+        else { i = j = lminX; k = lminY + (lminY & 1); }
+    } else if (j > i + k) {
+        j = i + k;
+    } else if ((i + j + k) & 1) {
+        ++k;
+    }
+    *pi = i; *pj = j; *pk = k;
+}
+
+/* tetrapyd_tools.c:383-414 */
+void check_ijk_max_XXY(int lmaxX, int lmaxY, int* pi, int* pj, int* pk)
+{
+    int i = *pi > lmaxX ? lmaxX : *pi, j = *pj > lmaxX ? lmaxX : *pj, k = *pk > lmaxY ? lmaxY : *pk;
+    if (i > j) { int t = i; i = j; j = t; }
+    if (k > i + j) k = i + j;
+    else if (j > i + k) j = i + k;
+    else if ((i + j + k) & 1) --k;
+    *pi = i; *pj = j; *pk = k;
+}
+
+/* tetrapyd_tools.c:226-297: count the walk (capped at lmaxX^2 lmaxY steps),
+ * split it over ranks and then threads, walk forward to each slice start */
+void decompose_tetrapyd_XXY(tetrapyd_limits* blk, int rank, int nranks, int nthreads, int lminX, int lminY,
+                            int lmaxX, int lmaxY)
+{
+    int i0 = lminX, j0 = lminX, k0 = lminY;
+    check_ijk_min_XXY(lminX, lminY, &i0, &j0, &k0);
+    int i = i0, j = j0, k = k0;
+    long cap = (long)lmaxX * lmaxX * lmaxY + 1, count = 1;
+    while (count < cap && get_ijk_next_XXY(lminX, lminY, lmaxX, lmaxY, &i, &j, &k)) ++count;
+    ++count;
+    long* pr = (long*)malloc(sizeof(long) * (size_t)nranks);
+    long* pt = (long*)malloc(sizeof(long) * (size_t)nthreads);
+    divide_tasks(count, nranks, pr);
+    divide_tasks(pr[rank], nthreads, pt);
+    i = i0; j = j0; k = k0;
+    for (int r = 0; r < rank; ++r)
+        for (long s = 0; s < pr[r]; ++s) get_ijk_next_XXY(lminX, lminY, lmaxX, lmaxY, &i, &j, &k);
+    for (int t = 0; t < nthreads; ++t) {
+        if (t)
+            for (long s = 0; s < pt[t - 1]; ++s) get_ijk_next_XXY(lminX, lminY, lmaxX, lmaxY, &i, &j, &k);
         blk[t].i_bgn = i; blk[t].j_bgn = j; blk[t].k_bgn = k; blk[t].loops = (int)pt[t];
     }
     free(pr);
@@ -397,11 +490,37 @@ int calculate_orthol_3D(int m, int n, double* result)
     return g_status;
 }
 
-/* orthogonalisation_late.c:239 (TTT only) */
+/* orthogonalisation_late.c:239-336: result[0] = Gamma_TTT(m, n) of the GL
+ * Gram; with an E setup bound (mcmb_lspace_set_E, the reference's
+ * do_polarisation = 1) also result[1..3] = TTE, TEE, EEE, every component on
+ * 3 max(T lmax, E lmax)/2 + 1 nodes as init_orthol_glint takes them then.
+ * Each component is computed whole on first use and cached. */
 int calculate_orthol(int m, int n, double* result)
 {
-    g_status = cached_entry(&g_gammagl, 1, m, n, &result[0]);
-    return g_status;
+    pthread_mutex_lock(&g_mu);
+    int st = refresh();
+    const int pol = st == MCMB_OK && g_egen >= 0;
+    int ne = 0;
+    if (pol && mcmb_lspace_E_info(g_ctx, NULL, NULL, &ne, NULL) == MCMB_OK && ne != g_n) st = MCMB_ERR_ARG;
+    if (st == MCMB_OK && (m < 0 || n < 0 || m >= g_n || n >= g_n)) st = MCMB_ERR_ARG;
+    if (st == MCMB_OK && !pol) {
+        pthread_mutex_unlock(&g_mu);
+        g_status = cached_entry(&g_gammagl, 1, m, n, &result[0]);
+        return g_status;
+    }
+    for (int k = 0; k < 4 && st == MCMB_OK; ++k) {
+        if (!g_pol[k]) {
+            double* g = (double*)malloc(sizeof(double) * (size_t)g_n * g_n);
+            if (!g) { st = MCMB_ERR_NOMEM; break; }
+            st = mcmb_gamma_gl_pol(g_ctx, k, 0, g, NULL);
+            if (st != MCMB_OK) { free(g); break; }
+            g_pol[k] = g;
+        }
+        result[k] = g_pol[k][(size_t)m * g_n + n];
+    }
+    pthread_mutex_unlock(&g_mu);
+    g_status = st;
+    return st;
 }
 
 /* orthogonalisation_late.c:23-97: the GL tables are built by the first
@@ -422,6 +541,18 @@ void calculate_orthoMij_T(int i, int j)
     (void)j;
     pthread_mutex_lock(&g_mu);
     g_status = refresh();
+    pthread_mutex_unlock(&g_mu);
+}
+
+/* orthogonalisation_late.c:131-157: as calculate_orthoMij_T for the E field;
+ * MCMB_ERR_STATE without an E setup */
+void calculate_orthoMij_E(int i, int j)
+{
+    (void)i;
+    (void)j;
+    pthread_mutex_lock(&g_mu);
+    g_status = refresh();
+    if (g_status == MCMB_OK && g_egen < 0) g_status = MCMB_ERR_STATE;
     pthread_mutex_unlock(&g_mu);
 }
 
@@ -664,6 +795,67 @@ void find_perm_late_TTT(int r, int* p1, int* p2, int* p3)
     *p2 = order[3 * r + 1];
     *p3 = order[3 * r + 2];
 }
+
+/* fixed_data.c:992-1019 -- mode r of the bound E setup's TTE / TEE / EEE
+ * table, in the slot order given to mcmb_lspace_set_E (-1s outside it) */
+static void find_perm_late_E(int table, int r, int* p1, int* p2, int* p3)
+{
+    const int* t[3] = {NULL, NULL, NULL};
+    int n = 0;
+    *p1 = *p2 = *p3 = -1;
+    if (!g_ctx || mcmb_lspace_E_tables(g_ctx, NULL, NULL, &t[0], &t[1], &t[2], &n, NULL, NULL) != MCMB_OK || r < 0 ||
+        r >= n)
+        return;
+    *p1 = t[table][3 * r];
+    *p2 = t[table][3 * r + 1];
+    *p3 = t[table][3 * r + 2];
+}
+void find_perm_late_TTE(int r, int* p1, int* p2, int* p3) { find_perm_late_E(0, r, p1, p2, p3); }
+void find_perm_late_TEE(int r, int* p1, int* p2, int* p3) { find_perm_late_E(1, r, p1, p2, p3); }
+void find_perm_late_EEE(int r, int* p1, int* p2, int* p3) { find_perm_late_E(2, r, p1, p2, p3); }
+
+/* orthogonalisation_late.c:646-715 -- the mixed mode products over the T
+ * (l-space) and E setups, basis rows by multipole; 0 when a leg is outside
+ * its field's range:
+ *   plijk_TTE = (qT_p1(l1) qT_p2(l2) + qT_p2(l1) qT_p1(l2)) qE_p3(l3) / 2,
+ *   plijk_TEE = qT_p1(l1) (qE_p2(l2) qE_p3(l3) + qE_p3(l2) qE_p2(l3)) / 2,
+ *   plijk_EEE = the six-permutation mean over the E basis. */
+static double plijk_pol(int table, int r, int l1, int l2, int l3)
+{
+    const double *qT, *qE;
+    const int* t[3] = {NULL, NULL, NULL};
+    int ldT, ldE, nT, nE, tlo, thi, elo, ehi;
+    if (!g_ctx || mcmb_lspace_tables(g_ctx, &qT, &ldT, NULL, &nT, &tlo, &thi) != MCMB_OK ||
+        mcmb_lspace_E_tables(g_ctx, &qE, &ldE, &t[0], &t[1], &t[2], &nE, &elo, &ehi) != MCMB_OK)
+        return 0.0;
+    if (r < 0 || r >= nE) return 0.0;
+    const int* o = t[table] + 3 * r;
+    /* field of each leg: TTE (T, T, E), TEE (T, E, E), EEE (E, E, E) */
+    const int l[3] = {l1, l2, l3};
+    for (int a = 0; a < 3; ++a) {
+        const int e = table == 2 || (table == 1 && a > 0) || (table == 0 && a == 2);
+        if (e ? (l[a] < elo || l[a] > ehi) : (l[a] < tlo || l[a] > thi)) return 0.0;
+    }
+    if (table == 0) {
+        const double* a = qT + (long)l1 * ldT;
+        const double* b = qT + (long)l2 * ldT;
+        return (a[o[0]] * b[o[1]] + a[o[1]] * b[o[0]]) * qE[(long)l3 * ldE + o[2]] / 2.0;
+    }
+    if (table == 1) {
+        const double* b = qE + (long)l2 * ldE;
+        const double* c = qE + (long)l3 * ldE;
+        return qT[(long)l1 * ldT + o[0]] * (b[o[1]] * c[o[2]] + b[o[2]] * c[o[1]]) / 2.0;
+    }
+    const double* a = qE + (long)l1 * ldE;
+    const double* b = qE + (long)l2 * ldE;
+    const double* c = qE + (long)l3 * ldE;
+    const int p1 = o[0], p2 = o[1], p3 = o[2];
+    return (a[p1] * b[p2] * c[p3] + a[p2] * b[p3] * c[p1] + a[p3] * b[p1] * c[p2] + a[p3] * b[p2] * c[p1] +
+            a[p2] * b[p1] * c[p3] + a[p1] * b[p3] * c[p2]) / 6.0;
+}
+double plijk_TTE(int r, int l1, int l2, int l3) { return plijk_pol(0, r, l1, l2, l3); }
+double plijk_TEE(int r, int l1, int l2, int l3) { return plijk_pol(1, r, l1, l2, l3); }
+double plijk_EEE(int r, int l1, int l2, int l3) { return plijk_pol(2, r, l1, l2, l3); }
 
 /* fixed_data.c get_terms_prim / get_kmax of the uniform setup (0 if unset) */
 int get_terms_prim(void)

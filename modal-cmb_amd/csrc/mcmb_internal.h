@@ -172,6 +172,20 @@ struct CrossSet {
     DevBuf d_qt, d_x, d_modes;
 };
 
+// E-mode field of the late basis, the second field of the polarised Gram:
+// basis q[l][ldq] (row = multipole), per-leg weight v[l], and the mode tables
+// of the mixed components in find_perm_late_XXX's slot order: order[0] TTE
+// (T, T, E), order[1] TEE (T, E, E), order[2] EEE; T indices refer to the
+// l-space setup's basis.
+struct PolSet {
+    bool ready = false;
+    long generation = -1;
+    int lmin = 0, lmax = 0, ldq = 0, n = 0;
+    std::vector<double> q, v;
+    std::vector<int> order[3];
+    DevBuf d_q, d_v, d_modes[3];
+};
+
 // Per-sample weights c_x of an x rule (MCMB_SPLINE_FMM, MCMB_SPLINE_NATURAL,
 // MCMB_SPLINE_TRAPEZIUM) on the increasing grid x.
 int spline_weights(const std::vector<double>& x, int kind, std::vector<double>& cw);
@@ -190,6 +204,7 @@ struct mcmb_ctx {
     mcmb::PointSet ks;   // primordial k-space Gauss-Legendre tetrapyd
     mcmb::PointSet ku;   // primordial k-space uniform-grid tetrapyd (prim walk)
     mcmb::CrossSet xs;   // q-tilde of the cross-Gamma
+    mcmb::PolSet es;     // E-mode field of the late basis (polarised Gram)
     mcmb::DevBuf d_W;    // full-cube projection weight W[i][j][k] (k-space)
     mcmb::DevBuf pz, py, ph, pd, psc, pdm;   // projection batch buffers (Z, Y, H, partials, sin / cos, D)
     // stage-1 products Y = W Z of the last forward projection, per batch of

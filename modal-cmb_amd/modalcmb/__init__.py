@@ -21,6 +21,7 @@ MIXED = 1
 FP32 = 2
 H2_EXACT = 0
 H2_ASYMPTOTIC = 1
+POL_TTT, POL_TTE, POL_TEE, POL_EEE = 0, 1, 2, 3
 
 _STATUS = {
     0: "ok", -1: "bad argument", -2: "HIP runtime error", -3: "matrix not positive definite",
@@ -58,6 +59,20 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_gamma_tet.argtypes = [vp, ip, vp, vp]
         lib.mcmb_gamma_gl.argtypes = [vp, ip, vp, vp]
         lib.mcmb_gamma_gl_noalloc.argtypes = [vp, ip, vp]
+        lib.mcmb_lspace_set_E.argtypes = [vp, ip, ip, ip, vp, vp, ip, vp, vp, vp]
+        lib.mcmb_lspace_clear_E.argtypes = [vp]
+        lib.mcmb_lspace_E_info.argtypes = [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip), C.POINTER(C.c_long)]
+        lib.mcmb_gamma_gl_pol.argtypes = [vp, ip, ip, vp, vp]
+        lib.plijk_TTE.restype = dp
+        lib.plijk_TTE.argtypes = [ip, ip, ip, ip]
+        lib.plijk_TEE.restype = dp
+        lib.plijk_TEE.argtypes = [ip, ip, ip, ip]
+        lib.plijk_EEE.restype = dp
+        lib.plijk_EEE.argtypes = [ip, ip, ip, ip]
+        for f in ("get_ijk_next_XXY", "check_ijk_min_XXY", "check_ijk_max_XXY"):
+            n_int = 4 if f == "get_ijk_next_XXY" else 2
+            getattr(lib, f).argtypes = [ip] * n_int + [C.POINTER(ip)] * 3
+        lib.decompose_tetrapyd_XXY.argtypes = [vp, ip, ip, ip, ip, ip, ip, ip]
         lib.mcmb_gamma_tet_slice.argtypes = [vp, ip, ip, ip, vp, vp, C.POINTER(dp)]
         lib.mcmb_lspace_npoints.argtypes = [vp]
         lib.mcmb_lspace_npoints.restype = C.c_long
@@ -144,7 +159,10 @@ EXPORTED_SYMBOLS = (
     "mcmb_spline_weights", "mcmb_decompose_gamma_TTT_3D", "decompose_gamma_TTT_3D_mpi", "calculate_gamma_2d",
     "calculate_gamma", "init_gamma_glint", "precompute_gammaMij_T", "calculate_gammaMij_T", "sync_gammaMij",
     "get_qtilde_xsize", "get_qtilde_xvec", "get_qtilde_T_lsize", "init_orthol_glint", "calculate_orthoMij_T",
-    "sync_orthoMij", "mcmb_file_name", "mcmb_write_qtilde", "mcmb_read_qtilde", "mcmb_basis_late_order9",
+    "sync_orthoMij", "mcmb_lspace_set_E", "mcmb_lspace_clear_E", "mcmb_lspace_E_info", "mcmb_lspace_E_tables", "mcmb_gamma_gl_pol",
+    "calculate_orthoMij_E", "find_perm_late_TTE", "find_perm_late_TEE", "find_perm_late_EEE", "plijk_TTE",
+    "plijk_TEE", "plijk_EEE", "get_ijk_next_XXY", "check_ijk_min_XXY", "check_ijk_max_XXY",
+    "decompose_tetrapyd_XXY", "mcmb_file_name", "mcmb_write_qtilde", "mcmb_read_qtilde", "mcmb_basis_late_order9",
     "find_perm_late_TTT",
 )
 
@@ -277,6 +295,33 @@ class Context:
     def gamma_gl(self, npt: int = 0) -> np.ndarray:
         g = np.empty((self.n, self.n), dtype=np.float64)
         _check(_lib.mcmb_gamma_gl(self._h, npt, _ptr(g), None))
+        return g
+
+    def lspace_set_E(self, E_lmin: int, E_lmax: int, qE: np.ndarray, vE: np.ndarray, order_TTE: np.ndarray,
+                     order_TEE: np.ndarray, order_EEE: np.ndarray) -> None:
+        """E field of the late basis, qE[l][ldq] / vE[l] for l = 0..E_lmax, and the
+        TTE (T, T, E), TEE (T, E, E), EEE mode tables (one row count)."""
+        qE = np.ascontiguousarray(qE, dtype=np.float64)
+        vE = np.ascontiguousarray(vE, dtype=np.float64)
+        tabs = [np.ascontiguousarray(t, dtype=np.int32).reshape(-1, 3) for t in (order_TTE, order_TEE, order_EEE)]
+        if qE.ndim != 2 or qE.shape[0] != E_lmax + 1 or vE.shape[0] != E_lmax + 1:
+            raise ValueError("qE must be [(E_lmax+1), ldq] and vE [(E_lmax+1)]")
+        if len({t.shape[0] for t in tabs}) != 1:
+            raise ValueError("the three mode tables must have one row count")
+        self._keepE = (qE, vE, tabs)
+        _check(_lib.mcmb_lspace_set_E(self._h, E_lmin, E_lmax, qE.shape[1], _ptr(qE), _ptr(vE), tabs[0].shape[0],
+                                      _ptr(tabs[0]), _ptr(tabs[1]), _ptr(tabs[2])))
+        self.nE = tabs[0].shape[0]
+
+    def lspace_clear_E(self) -> None:
+        _check(_lib.mcmb_lspace_clear_E(self._h))
+        self.nE = 0
+
+    def gamma_gl_pol(self, pol: int, npt: int = 0) -> np.ndarray:
+        """Component pol (POL_TTT .. POL_EEE) of the polarised GL Gram."""
+        n = self.n if pol == POL_TTT else self.nE
+        g = np.empty((n, n), dtype=np.float64)
+        _check(_lib.mcmb_gamma_gl_pol(self._h, pol, npt, _ptr(g), None))
         return g
 
     def gamma_tet_device(self, precision: int, dev_ptr: int) -> None:

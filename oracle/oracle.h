@@ -113,6 +113,35 @@ void or_gamma_tet(int lmin, int lmax, const double* q, int ldq, const double* v,
 void or_gamma_gl(int lmin, int lmax, const double* q, int ldq, const double* v,
                  int n, const int* order, int npt, double* gamma);
 
+/* ---- polarisation (pol.c) ------------------------------------------------ */
+/* XXY walk  tetrapyd_tools.c:226-414: i <= j over [lminX, lmaxX], k over
+ * [lminY, lmaxY] with the triangle and even parity, the reference's quirks
+ * kept (a new i restarts at k = lminY with no parity bump; the walk ends at
+ * i + j + k + 1 >= 2 lmaxX + lmaxY, for an odd lmaxY already at (lmaxX - 1,
+ * lmaxX, lmaxY)).  Pinned: brute-force enumeration of
+ * the mixed tetrapyd (tests/test_oracle_pol.py). */
+int  or_get_ijk_next_XXY(int lminX, int lminY, int lmaxX, int lmaxY, int* i, int* j, int* k);
+void or_check_ijk_min_XXY(int lminX, int lminY, int* i, int* j, int* k);
+void or_check_ijk_max_XXY(int lmaxX, int lmaxY, int* i, int* j, int* k);
+long or_count_tetrapyd_XXY(int lminX, int lminY, int lmaxX, int lmaxY);
+long or_walk_XXY(int lminX, int lminY, int lmaxX, int lmaxY, int* ijk, long cap);
+void or_decompose_tetrapyd_XXY(or_tetrapyd_limits* block, int rank, int nranks, int nthreads, int lminX,
+                               int lminY, int lmaxX, int lmaxY);
+/* calculate_orthol's four components (orthogonalisation_late.c:239-336,
+ * M^T / M^E of calculate_orthoMij_T / _E 99-157): pol 0 TTT, 1 TTE, 2 TEE,
+ * 3 EEE; order in find_perm_late_XXX's slot order (TTE = T, T, E; TEE = T,
+ * E, E); q / v of each field are rows by multipole.  Pinned: the direct sum
+ * below (TTE, TEE) and the direct tetrapyd Gram on the E field (EEE). */
+void or_gamma_gl_pol(int pol, int Tlmin, int Tlmax, const double* qT, int ldqT, const double* vT, int Elmin,
+                     int Elmax, const double* qE, int ldqE, const double* vE, int n, const int* order, int npt,
+                     double* gamma);
+/* Direct sum over the mixed tetrapyd with the exact h^2, multiplicity 2 for
+ * unequal pair legs: pol 1 (plijk_TTE over i <= j in T, k in E), pol 2
+ * (plijk_TEE over i in T, j <= k in E), orthogonalisation_late.c:646-690. */
+void or_gamma_xxy_direct(int pol, int Tlmin, int Tlmax, const double* qT, int ldqT, const double* vT, int Elmin,
+                         int Elmax, const double* qE, int ldqE, const double* vE, int n, const int* order,
+                         double* gamma);
+
 /* ---- orthonormalisation (ortho_prim_mpi.c:198-284) ---------------------- */
 /* norm_i = sqrt(G_ii); C = D^-1 G D^-1; C = L L^T; ortho = symmetrised D L
  * (entry [i][j] = [j][i] = norm[max] L(max,min), update_ortho fixed_data.c:3285);

@@ -81,6 +81,16 @@ def lib() -> C.CDLL:
         L.or_decompose_tetrapyd_XXX.argtypes = [vp, ip, ip, ip, ip, ip]
         L.or_decompose_tetrapyd_prim.argtypes = [vp, ip, ip, ip, ip, ip]
         L.or_kgl_limits.argtypes = [ip, vp, vp]
+        L.or_count_tetrapyd_XXY.restype = C.c_long
+        L.or_count_tetrapyd_XXY.argtypes = [ip, ip, ip, ip]
+        L.or_walk_XXY.restype = C.c_long
+        L.or_walk_XXY.argtypes = [ip, ip, ip, ip, vp, C.c_long]
+        L.or_decompose_tetrapyd_XXY.argtypes = [vp, ip, ip, ip, ip, ip, ip, ip]
+        L.or_get_ijk_next_XXY.argtypes = [ip, ip, ip, ip] + [C.POINTER(ip)] * 3
+        L.or_check_ijk_min_XXY.argtypes = [ip, ip] + [C.POINTER(ip)] * 3
+        L.or_check_ijk_max_XXY.argtypes = [ip, ip] + [C.POINTER(ip)] * 3
+        L.or_gamma_gl_pol.argtypes = [ip, ip, ip, vp, ip, vp, ip, ip, vp, ip, vp, ip, vp, ip, vp]
+        L.or_gamma_xxy_direct.argtypes = [ip, ip, ip, vp, ip, vp, ip, ip, vp, ip, vp, ip, vp, vp]
         _lib = L
     return _lib
 
@@ -132,6 +142,73 @@ def decompose_gamma_TTT_3D(lsize, rank, numranks, workshare, nthreads) -> np.nda
     out = np.zeros((nthreads, 6), dtype=np.int32)
     lib().or_decompose_gamma_TTT_3D(lsize, rank, numranks, workshare, nthreads, _p(out))
     return out
+
+
+def walk_XXY(lminX: int, lminY: int, lmaxX: int, lmaxY: int) -> np.ndarray:
+    n = lib().or_count_tetrapyd_XXY(lminX, lminY, lmaxX, lmaxY)
+    out = np.empty((n, 3), dtype=np.int32)
+    lib().or_walk_XXY(lminX, lminY, lmaxX, lmaxY, _p(out), n)
+    return out
+
+
+def decompose_XXY(rank, nranks, nthreads, lminX, lminY, lmaxX, lmaxY) -> np.ndarray:
+    out = np.zeros((nthreads, 4), dtype=np.int32)
+    lib().or_decompose_tetrapyd_XXY(_p(out), rank, nranks, nthreads, lminX, lminY, lmaxX, lmaxY)
+    return out
+
+
+def _ijk_call(f, a, b, ijk):
+    i, j, k = (C.c_int(int(x)) for x in ijk)
+    r = f(a, b, C.byref(i), C.byref(j), C.byref(k))
+    return r, (i.value, j.value, k.value)
+
+
+def next_XXY(lminX, lminY, lmaxX, lmaxY, ijk):
+    """(more, (i, j, k)) after one get_ijk_next_XXY step."""
+    i, j, k = (C.c_int(int(x)) for x in ijk)
+    r = lib().or_get_ijk_next_XXY(lminX, lminY, lmaxX, lmaxY, C.byref(i), C.byref(j), C.byref(k))
+    return r, (i.value, j.value, k.value)
+
+
+def check_min_XXY(lminX, lminY, ijk):
+    return _ijk_call(lib().or_check_ijk_min_XXY, lminX, lminY, ijk)[1]
+
+
+def check_max_XXY(lmaxX, lmaxY, ijk):
+    return _ijk_call(lib().or_check_ijk_max_XXY, lmaxX, lmaxY, ijk)[1]
+
+
+def _pol_args(T, E):
+    """(lmin, lmax, q, v) of each field -> flattened ctypes arguments"""
+    out = []
+    for lmin, lmax, q, v in (T, E):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out.append((lmin, lmax, q, v))
+    return out
+
+
+def gamma_gl_pol(pol, T, E, order, npt=None) -> np.ndarray:
+    """calculate_orthol's component pol (0 TTT, 1 TTE, 2 TEE, 3 EEE); T and E
+    are (lmin, lmax, q[l][R], v[l]) of each field."""
+    (tl, th, qT, vT), (el, eh, qE, vE) = _pol_args(T, E)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    n = order.shape[0]
+    npt = 3 * max(th, eh) // 2 + 1 if npt is None else npt
+    g = np.empty((n, n))
+    lib().or_gamma_gl_pol(pol, tl, th, _p(qT), qT.shape[1], _p(vT), el, eh, _p(qE), qE.shape[1], _p(vE), n,
+                          _p(order), npt, _p(g))
+    return g
+
+
+def gamma_xxy_direct(pol, T, E, order) -> np.ndarray:
+    (tl, th, qT, vT), (el, eh, qE, vE) = _pol_args(T, E)
+    order = np.ascontiguousarray(order, dtype=np.int32)
+    n = order.shape[0]
+    g = np.empty((n, n))
+    lib().or_gamma_xxy_direct(pol, tl, th, _p(qT), qT.shape[1], _p(vT), el, eh, _p(qE), qE.shape[1], _p(vE), n,
+                              _p(order), _p(g))
+    return g
 
 
 def basis_legendre(values: np.ndarray, pmax: int, vmin: float, vmax: float) -> np.ndarray:

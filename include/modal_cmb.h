@@ -208,6 +208,28 @@ int mcmb_gamma_cross_3d(mcmb_ctx* ctx, int rank, int nranks, double* gamma_host,
  * 3 x 3 permanents, the same x < 10 rule, the spline integral in x, scaled by
  * dphi^2/(8 pi).  Needs lmax_q >= the late lmax. */
 int mcmb_gamma_cross_gl(mcmb_ctx* ctx, int spline, int npt, double* gamma_host, double* gamma_dev);
+/* E-mode q-tilde of the polarised cross-Gamma (read_qtilde's E files,
+ * fixed_data.c:2052-2063, 2330-2440): qt_E[p][l][x] of the T q-tilde's shape,
+ * and per multipole 0..lmax_q the spectra C_TE, C_EE, noise N_TE, N_EE, beams
+ * b_TE, b_EE and fsky_E / fsky_T.  Call after mcmb_qtilde_set (whose C_l, N_l,
+ * b_l are the TT ones); a new mcmb_qtilde_set drops it. */
+int mcmb_qtilde_set_E(mcmb_ctx* ctx, const double* qt_E, const double* cl_TE, const double* cl_EE,
+                      const double* nl_TE, const double* nl_EE, const double* bl_TE, const double* bl_EE,
+                      double fsky_ratio);
+/* fsky ratio and generation of the E-mode q-tilde setup (outputs may be NULL). */
+int mcmb_qtilde_E_info(mcmb_ctx* ctx, double* fsky_ratio, long* generation);
+/* Component pol (MCMB_POL_*) of the GL cross-Gamma, calculate_gamma_2d /
+ * calculate_gamma result[pol] (calculate_gamma.c:48-352, 511-1016): late slot
+ * b of the component's mode table reads M^T or M^E by its field (TTT: T T T,
+ * TTE: T T E, TEE: T E E, EEE: E E E), a_ab = M^{X_b}[p_a][q_b] and the node
+ * sum of the 3 x 3 permanent as above, with
+ *   M^E[x][p][q][s] = sum_{l in E range} P_l(mu_s) EEw(l) (s1 qt_E - s2 qt_T)[p][l][x] b^E_q(l),
+ *   s1 = b_TT^2 C_TT + N_TT, s2 = sqrt(fsky_E/fsky_T) b_TE^2 C_TE + N_TE,
+ *   EEw = (2l+1)^(5/6) b_EE / sqrt(s1 (s1 s4 - s3^2)),
+ *   s3 = b_TT b_EE C_TE + N_TE, s4 = b_EE^2 C_EE + N_EE;
+ * npt <= 0: 3 max(T lmax, E lmax)/2 + 1.  Gamma is n_late x n_prim with
+ * n_late the l-space mode count (TTT) or the E setup's (the others). */
+int mcmb_gamma_cross_gl_pol(mcmb_ctx* ctx, int pol, int spline, int npt, double* gamma_host, double* gamma_dev);
 /* Per-sample weights c[x] of the x rules above on the grid x[n] (host only):
  * sum_x c[x] y[x] is the rule's integral of y over [x_0, x_{n-1}]. */
 int mcmb_spline_weights(int kind, int n, const double* x, double* c);
@@ -468,7 +490,9 @@ void gamma_3d_host(double* gamma_flat, int gamma_size, int rank, int numranks,
  * cross-Gamma (FMM spline in x / natural spline in x) of the bound context,
  * computed whole on the first call and cached until a setup changes; safe to
  * call from several threads at once (the first call fills the cache under a
- * lock).  Only result[0] is written. */
+ * lock).  result[0] alone, or result[0..3] = TTT, TTE, TEE, EEE of
+ * mcmb_gamma_cross_gl_pol when mcmb_lspace_set_E and mcmb_qtilde_set_E are
+ * both bound (the reference's do_polarisation = 1). */
 void calculate_gamma_2d(int m, int n, double* result);
 void calculate_gamma(int m, int n, double* result);
 /* calculate_gamma.c:48, 255, 354, 425 -- the M_ij tables are formed on the
@@ -479,6 +503,11 @@ void init_gamma_glint(void);
 void precompute_gammaMij_T(void);
 void calculate_gammaMij_T(int i, int j);
 void sync_gammaMij(void);
+/* calculate_gamma.c:306, 388 -- the E tables come with the polarised
+ * cross-Gamma (mcmb_lspace_set_E and mcmb_qtilde_set_E bound; MCMB_ERR_STATE
+ * in mcmb_legacy_status otherwise) */
+void precompute_gammaMij_E(void);
+void calculate_gammaMij_E(int i, int j);
 /* fixed_data.c get_qtilde_xsize / get_qtilde_xvec / get_qtilde_T_lsize of the
  * bound q-tilde setup (lsize = lmax_q + 1) */
 int get_qtilde_xsize(void);

@@ -499,6 +499,96 @@ __global__ __launch_bounds__(256) void cx_gl_gamma(const double* __restrict__ M,
         }
 }
 
+// The polarised components TTE (KIND 1) and TEE (KIND 2) of the above: late
+// slot b reads M^T or M^E by its field (TTE: T, T, E; TEE: T, E, E), a_ab =
+// M^{X_b}[p_a][q_b] (gamma_2d_pt_TTE / _TEE, calculate_gamma.c:557-653);
+// both fields' M staged in LDS GS2 nodes at a time.
+constexpr int GS2 = 3;
+template <int KIND>
+__global__ __launch_bounds__(256) void cx_gl_gamma_mixed(const double* __restrict__ MT, int RT,
+                                                         const double* __restrict__ ME, int RE, int np, int nx,
+                                                         int npt, const double* __restrict__ wgt,
+                                                         const double* __restrict__ xv, const double* __restrict__ cx,
+                                                         const int* __restrict__ mlate, int nl,
+                                                         const int* __restrict__ mprim, int npm, double scale,
+                                                         double* __restrict__ out)
+{
+    __shared__ double Ts[GS2][XR * XR], Es[GS2][XR * XR];
+    __shared__ double ws[GS2];
+    const int tnp = (npm + 63) / 64;
+    const int I = blockIdx.x / tnp, J = blockIdx.x % tnp;
+    const int tm = (threadIdx.x >> 4) * 4, tn = (threadIdx.x & 15) * 4;
+    int qa[4][3], pa[4][3];
+    bool okm[4], okn[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int m = I * 64 + tm + u, c = J * 64 + tn + u;
+        okm[u] = m < nl;
+        okn[u] = c < npm;
+        const int qm = okm[u] ? mlate[m] : 0, pn = okn[u] ? mprim[c] : 0;
+        qa[u][0] = qm & 255; qa[u][1] = (qm >> 8) & 255; qa[u][2] = (qm >> 16) & 255;
+        pa[u][0] = pn & 255; pa[u][1] = (pn >> 8) & 255; pa[u][2] = (pn >> 16) & 255;
+    }
+    // LDS table of each late slot
+    const double(*S1)[XR * XR] = Ts;
+    const double(*S2)[XR * XR] = KIND == 1 ? Ts : Es;
+    const double(*S3)[XR * XR] = Es;
+    double tot[4][4] = {};
+    const int PT = np * RT, PE = np * RE;
+    for (int x = 0; x < nx; ++x) {
+        double acc[4][4] = {};
+        const double* MTx = MT + (long)x * PT * npt;
+        const double* MEx = ME + (long)x * PE * npt;
+        for (int s0 = 0; s0 < npt; s0 += GS2) {
+            __syncthreads();
+            for (int e = threadIdx.x; e < GS2 * PT; e += 256) {
+                const int s = e % GS2, pq = e / GS2;
+                Ts[s][(pq / RT) * XR + pq % RT] = (s0 + s < npt) ? MTx[(long)pq * npt + s0 + s] : 0.0;
+            }
+            for (int e = threadIdx.x; e < GS2 * PE; e += 256) {
+                const int s = e % GS2, pq = e / GS2;
+                Es[s][(pq / RE) * XR + pq % RE] = (s0 + s < npt) ? MEx[(long)pq * npt + s0 + s] : 0.0;
+            }
+            if (threadIdx.x < GS2) ws[threadIdx.x] = (s0 + threadIdx.x < npt) ? wgt[s0 + threadIdx.x] : 0.0;
+            __syncthreads();
+            for (int s = 0; s < GS2; ++s) {
+                const double *A = S1[s], *B = S2[s], *D = S3[s];
+                const double w = ws[s];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const double a11 = A[pa[v][0] * XR + qa[u][0]], a12 = B[pa[v][0] * XR + qa[u][1]],
+                                     a13 = D[pa[v][0] * XR + qa[u][2]];
+                        const double a21 = A[pa[v][1] * XR + qa[u][0]], a22 = B[pa[v][1] * XR + qa[u][1]],
+                                     a23 = D[pa[v][1] * XR + qa[u][2]];
+                        const double a31 = A[pa[v][2] * XR + qa[u][0]], a32 = B[pa[v][2] * XR + qa[u][1]],
+                                     a33 = D[pa[v][2] * XR + qa[u][2]];
+                        const double perm = a11 * (a22 * a33 + a23 * a32) + a12 * (a21 * a33 + a23 * a31) +
+                                            a13 * (a21 * a32 + a22 * a31);
+                        acc[u][v] += w * perm;
+                    }
+            }
+        }
+        const double x2 = xv[x] * xv[x], c = cx[x];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                double y = x2 * acc[u][v];
+                if (x < 10 && fabs(y) > 1e-18) y = 0.0;
+                tot[u][v] += c * y;
+            }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = I * 64 + tm + u, c = J * 64 + tn + v;
+            if (okm[u] && okn[v]) out[(long)m * npm + c] = scale * tot[u][v];
+        }
+}
+
 }  // namespace mcmb
 
 using namespace mcmb;
@@ -523,6 +613,8 @@ extern "C" int mcmb_qtilde_set(mcmb_ctx* c, int np, int lmax_q, int nx, const do
     MCMB_TRY(set_device(c));
     CrossSet& X = c->xs;
     X.ready = false;
+    X.pol = false;
+    X.d_qc.release();
     X.np = np;
     X.lq = lmax_q;
     X.nx = nx;
@@ -547,6 +639,53 @@ extern "C" int mcmb_qtilde_set(mcmb_ctx* c, int np, int lmax_q, int nx, const do
     MCMB_HIP(hipStreamSynchronize(c->stream));
     X.generation = next_generation();
     X.ready = true;
+    return MCMB_OK;
+}
+
+extern "C" int mcmb_qtilde_set_E(mcmb_ctx* c, const double* qt_E, const double* cl_TE, const double* cl_EE,
+                                 const double* nl_TE, const double* nl_EE, const double* bl_TE, const double* bl_EE,
+                                 double fsky_ratio)
+{
+    if (!c || !qt_E || !cl_TE || !cl_EE || !nl_TE || !nl_EE || !bl_TE || !bl_EE)
+        return fail(MCMB_ERR_ARG, "mcmb_qtilde_set_E: null argument");
+    if (!c->xs.ready) return fail(MCMB_ERR_STATE, "mcmb_qtilde_set_E: call mcmb_qtilde_set first");
+    if (!(fsky_ratio >= 0.0)) return fail(MCMB_ERR_ARG, "mcmb_qtilde_set_E: fsky ratio %g", fsky_ratio);
+    MCMB_TRY(set_device(c));
+    CrossSet& X = c->xs;
+    X.pol = false;
+    const int nlq = X.lq + 1;
+    X.cte.assign(cl_TE, cl_TE + nlq);
+    X.cee.assign(cl_EE, cl_EE + nlq);
+    X.nte.assign(nl_TE, nl_TE + nlq);
+    X.nee.assign(nl_EE, nl_EE + nlq);
+    X.bte.assign(bl_TE, bl_TE + nlq);
+    X.bee.assign(bl_EE, bl_EE + nlq);
+    X.fsky_ratio = fsky_ratio;
+    // qc[p][l][x] = s1 qt_E - s2 qt_T, s1 = b_TT^2 C_TT + N_TT,
+    // s2 = sqrt(fsky_E / fsky_T) b_TE^2 C_TE + N_TE (init_gamma_glint,
+    // calculate_gamma.c:171-183)
+    const size_t nq = (size_t)X.np * nlq * X.nx;
+    std::vector<double> qc(nq);
+    const double rf = std::sqrt(fsky_ratio);
+    for (int p = 0; p < X.np; ++p)
+        for (int l = 0; l < nlq; ++l) {
+            const double s1 = X.bl[l] * X.bl[l] * X.cl[l] + X.nl[l];
+            const double s2 = rf * X.bte[l] * X.bte[l] * X.cte[l] + X.nte[l];
+            const size_t o = ((size_t)p * nlq + l) * X.nx;
+            for (int x = 0; x < X.nx; ++x) qc[o + x] = s1 * qt_E[o + x] - s2 * X.qt[o + x];
+        }
+    MCMB_TRY(X.d_qc.alloc(sizeof(double) * nq));
+    MCMB_HIP(hipMemcpy(X.d_qc.p, qc.data(), sizeof(double) * nq, hipMemcpyHostToDevice));
+    X.pol = true;
+    X.pol_generation = next_generation();
+    return MCMB_OK;
+}
+
+extern "C" int mcmb_qtilde_E_info(mcmb_ctx* c, double* fsky_ratio, long* generation)
+{
+    if (!c || !c->xs.ready || !c->xs.pol) return fail(MCMB_ERR_STATE, "mcmb_qtilde_E_info: no E-mode q-tilde setup");
+    if (fsky_ratio) *fsky_ratio = c->xs.fsky_ratio;
+    if (generation) *generation = c->xs.pol_generation;
     return MCMB_OK;
 }
 
@@ -696,61 +835,158 @@ extern "C" int mcmb_gamma_cross_3d(mcmb_ctx* c, int rank, int nranks, double* ga
 
 // ---- Gauss-Legendre form ---------------------------------------------------------
 
+// Component pol (0 TTT .. 3 EEE) of the GL cross-Gamma into dst (n_late x
+// n_prim): per late field in use, the Legendre table times the field's l
+// weight, M[x][p][q][s] over the field's range, then the node and x sums.
+static int cross_gl_launch(mcmb_ctx* c, int pol, int spline, int npt, double* dst)
+{
+    static const int fld[4][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    const LSpace& L = c->ls;
+    const CrossSet& X = c->xs;
+    const PolSet& E = c->es;
+    const int* ord = pol == 0 ? L.order.data() : E.order[pol - 1].data();
+    const int nlate = pol == 0 ? L.n : E.n, np = X.np, nx = X.nx, nprim = X.n;
+    const int lo[2] = {L.lmin, E.lmin}, hi[2] = {L.lmax, E.lmax}, ldb[2] = {L.ldq, E.ldq};
+    int R[2] = {0, 0};
+    for (int m = 0; m < nlate; ++m)
+        for (int b = 0; b < 3; ++b) {
+            const int f = fld[pol][b], p = ord[3 * m + b];
+            if (p >= ldb[f])
+                return fail(MCMB_ERR_ORDER, "GL cross-Gamma: late mode %d slot %d uses column %d of %d", m, b, p, ldb[f]);
+            R[f] = std::max(R[f], p + 1);
+        }
+    for (int f = 0; f < 2; ++f) {
+        if (R[f] == 0) continue;
+        if (R[f] > XR) return fail(MCMB_ERR_ARG, "GL cross-Gamma: late basis width %d > %d", R[f], XR);
+        if (hi[f] > X.lq)
+            return fail(MCMB_ERR_ARG, "GL cross-Gamma: late lmax %d beyond the q-tilde multipoles (%d)", hi[f], X.lq);
+    }
+    std::vector<double> mu(npt), w(npt), cw;
+    MCMB_TRY(mcmb_gauss_legendre(npt, mu.data(), w.data()));
+    MCMB_TRY(spline_weights(X.xvec, spline, cw));
+    // l weights: TTw = (2l+1)^(5/6) b_TT / sqrt(b_TT^2 C_TT + N_TT), EEw =
+    // (2l+1)^(5/6) b_EE / sqrt(s1 (s1 s4 - s3^2)) (calculate_gamma.c:73-84,
+    // 143-159), 0 for l <= 1
+    std::vector<double> tw[2];
+    for (int f = 0; f < 2; ++f) {
+        if (R[f] == 0) continue;
+        tw[f].assign(hi[f] + 1, 0.0);
+        for (int l = std::max(2, lo[f]); l <= hi[f]; ++l) {
+            const double s1 = X.bl[l] * X.bl[l] * X.cl[l] + X.nl[l];
+            if (f == 0) {
+                tw[f][l] = std::pow(2.0 * l + 1.0, 5.0 / 6.0) * (X.bl[l] / std::sqrt(s1));
+            } else {
+                const double s3 = X.bl[l] * X.bee[l] * X.cte[l] + X.nte[l];
+                const double s4 = X.bee[l] * X.bee[l] * X.cee[l] + X.nee[l];
+                tw[f][l] = std::pow(2.0 * l + 1.0, 5.0 / 6.0) * (X.bee[l] / std::sqrt(s1 * (s1 * s4 - s3 * s3)));
+            }
+        }
+    }
+    DevBuf d_mu, d_w, d_cw, d_tw[2], d_Pt[2], d_M[2];
+    MCMB_TRY(d_mu.alloc(sizeof(double) * npt));
+    MCMB_TRY(d_w.alloc(sizeof(double) * npt));
+    MCMB_TRY(d_cw.alloc(sizeof(double) * nx));
+    MCMB_HIP(hipMemcpyAsync(d_mu.p, mu.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(d_w.p, w.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(d_cw.p, cw.data(), sizeof(double) * nx, hipMemcpyHostToDevice, c->stream));
+    for (int f = 0; f < 2; ++f) {
+        if (R[f] == 0) continue;
+        MCMB_TRY(d_tw[f].alloc(sizeof(double) * tw[f].size()));
+        MCMB_TRY(d_Pt[f].alloc(sizeof(double) * (size_t)(hi[f] + 1) * npt));
+        MCMB_TRY(d_M[f].alloc(sizeof(double) * (size_t)nx * np * R[f] * npt));
+        MCMB_HIP(hipMemcpyAsync(d_tw[f].p, tw[f].data(), sizeof(double) * tw[f].size(), hipMemcpyHostToDevice,
+                                c->stream));
+    }
+    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
+    const double* bq[2] = {L.d_q.as<double>(), E.d_q.as<double>()};
+    const double* qsrc[2] = {X.d_qt.as<double>(), X.d_qc.as<double>()};
+    int launches = 0;
+    for (int f = 0; f < 2; ++f) {
+        if (R[f] == 0) continue;
+        hipLaunchKernelGGL(cx_legendre, dim3((npt + 255) / 256), dim3(256), 0, c->stream, d_mu.as<double>(), npt,
+                           lo[f], hi[f], d_tw[f].as<double>(), d_Pt[f].as<double>());
+        hipLaunchKernelGGL(cx_mgemm, dim3((npt + 63) / 64, (nx + 31) / 32, np * R[f]), dim3(256), 0, c->stream,
+                           qsrc[f], X.lq + 1, nx, bq[f], ldb[f], R[f], d_Pt[f].as<double>(), npt, lo[f], hi[f], np,
+                           d_M[f].as<double>());
+        launches += 2;
+    }
+    const unsigned tiles = (unsigned)(((nlate + 63) / 64) * ((nprim + 63) / 64));
+    const double scale = X.dphi * X.dphi / (8.0 * M_PI);
+    const int* ml = pol == 0 ? L.d_modes.as<int>() : E.d_modes[pol - 1].as<int>();
+    if (pol == 0 || pol == 3) {
+        const int f = pol == 0 ? 0 : 1;
+        hipLaunchKernelGGL(cx_gl_gamma, dim3(tiles), dim3(256), 0, c->stream, (const double*)d_M[f].as<double>(), np,
+                           R[f], nx, npt, d_w.as<double>(), X.d_x.as<double>(), d_cw.as<double>(), ml, nlate,
+                           X.d_modes.as<int>(), nprim, scale, dst);
+    } else if (pol == 1) {
+        hipLaunchKernelGGL(cx_gl_gamma_mixed<1>, dim3(tiles), dim3(256), 0, c->stream,
+                           (const double*)d_M[0].as<double>(), R[0], (const double*)d_M[1].as<double>(), R[1], np, nx,
+                           npt, d_w.as<double>(), X.d_x.as<double>(), d_cw.as<double>(), ml, nlate,
+                           X.d_modes.as<int>(), nprim, scale, dst);
+    } else {
+        hipLaunchKernelGGL(cx_gl_gamma_mixed<2>, dim3(tiles), dim3(256), 0, c->stream,
+                           (const double*)d_M[0].as<double>(), R[0], (const double*)d_M[1].as<double>(), R[1], np, nx,
+                           npt, d_w.as<double>(), X.d_x.as<double>(), d_cw.as<double>(), ml, nlate,
+                           X.d_modes.as<int>(), nprim, scale, dst);
+    }
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipEventRecord(c->ev1, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = launches + 1;
+    return MCMB_OK;
+}
+
 extern "C" int mcmb_gamma_cross_gl(mcmb_ctx* c, int spline, int npt, double* gamma_host, double* gamma_dev)
 {
     MCMB_TRY(cross_checks(c, "mcmb_gamma_cross_gl"));
     if (spline != MCMB_SPLINE_FMM && spline != MCMB_SPLINE_NATURAL)
         return fail(MCMB_ERR_ARG, "mcmb_gamma_cross_gl: spline %d", spline);
-    const LSpace& L = c->ls;
-    const CrossSet& X = c->xs;
-    if (L.lmax > X.lq)
-        return fail(MCMB_ERR_ARG, "mcmb_gamma_cross_gl: late lmax %d beyond the q-tilde multipoles (%d)", L.lmax, X.lq);
-    if (L.R > XR) return fail(MCMB_ERR_ARG, "mcmb_gamma_cross_gl: late basis width %d > %d", L.R, XR);
-    const int lmin = L.lmin, lmax = L.lmax, np = X.np, Rq = L.R, nx = X.nx, nlate = L.n, nprim = X.n;
+    const int lmax = c->ls.lmax;
     if (npt <= 0) npt = 3 * lmax / 2 + 1;
     if (2 * npt - 1 < 3 * lmax)
         return fail(MCMB_ERR_ARG, "mcmb_gamma_cross_gl: npt %d < (3 lmax + 1)/2 is not exact", npt);
     MCMB_TRY(set_device(c));
-    std::vector<double> mu(npt), w(npt), tw(lmax + 1, 0.0), cw;
-    MCMB_TRY(mcmb_gauss_legendre(npt, mu.data(), w.data()));
-    for (int l = std::max(2, lmin); l <= lmax; ++l)
-        tw[l] = std::pow(2.0 * l + 1.0, 5.0 / 6.0) * (X.bl[l] / std::sqrt(X.bl[l] * X.bl[l] * X.cl[l] + X.nl[l]));
-    MCMB_TRY(spline_weights(X.xvec, spline, cw));
-    const size_t gb = sizeof(double) * (size_t)nlate * nprim;
+    const size_t gb = sizeof(double) * (size_t)c->ls.n * c->xs.n;
     double* dst = gamma_dev;
     if (!dst) {
         MCMB_TRY(c->d_gamma_tmp.alloc(gb));
         dst = c->d_gamma_tmp.as<double>();
     }
-    DevBuf d_mu, d_w, d_tw, d_cw, d_Pt, d_M;
-    MCMB_TRY(d_mu.alloc(sizeof(double) * npt));
-    MCMB_TRY(d_w.alloc(sizeof(double) * npt));
-    MCMB_TRY(d_tw.alloc(sizeof(double) * tw.size()));
-    MCMB_TRY(d_cw.alloc(sizeof(double) * nx));
-    MCMB_TRY(d_Pt.alloc(sizeof(double) * (size_t)(lmax + 1) * npt));
-    MCMB_TRY(d_M.alloc(sizeof(double) * (size_t)nx * np * Rq * npt));
-    MCMB_HIP(hipMemcpyAsync(d_mu.p, mu.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipMemcpyAsync(d_w.p, w.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipMemcpyAsync(d_tw.p, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipMemcpyAsync(d_cw.p, cw.data(), sizeof(double) * nx, hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(cx_legendre, dim3((npt + 255) / 256), dim3(256), 0, c->stream, d_mu.as<double>(), npt, lmin,
-                       lmax, d_tw.as<double>(), d_Pt.as<double>());
-    hipLaunchKernelGGL(cx_mgemm, dim3((npt + 63) / 64, (nx + 31) / 32, np * Rq), dim3(256), 0, c->stream,
-                       X.d_qt.as<double>(), X.lq + 1, nx, L.d_q.as<double>(), L.ldq, Rq, d_Pt.as<double>(), npt,
-                       lmin, lmax, np, d_M.as<double>());
-    const unsigned tiles = (unsigned)(((nlate + 63) / 64) * ((nprim + 63) / 64));
-    hipLaunchKernelGGL(cx_gl_gamma, dim3(tiles), dim3(256), 0, c->stream, (const double*)d_M.as<double>(), np, Rq, nx,
-                       npt, d_w.as<double>(), X.d_x.as<double>(), d_cw.as<double>(), L.d_modes.as<int>(), nlate,
-                       X.d_modes.as<int>(), nprim, X.dphi * X.dphi / (8.0 * M_PI), dst);
-    MCMB_HIP(hipGetLastError());
-    MCMB_HIP(hipEventRecord(c->ev1, c->stream));
-    if (gamma_host) MCMB_HIP(hipMemcpyAsync(gamma_host, dst, gb, hipMemcpyDeviceToHost, c->stream));
-    MCMB_HIP(hipStreamSynchronize(c->stream));
-    float ms = 0.f;
-    MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->last_ms = ms;
-    c->last_launches = 3;
+    MCMB_TRY(cross_gl_launch(c, 0, spline, npt, dst));
+    if (gamma_host) MCMB_HIP(hipMemcpy(gamma_host, dst, gb, hipMemcpyDeviceToHost));
+    return MCMB_OK;
+}
+
+extern "C" int mcmb_gamma_cross_gl_pol(mcmb_ctx* c, int pol, int spline, int npt, double* gamma_host,
+                                       double* gamma_dev)
+{
+    MCMB_TRY(cross_checks(c, "mcmb_gamma_cross_gl_pol"));
+    if (pol < MCMB_POL_TTT || pol > MCMB_POL_EEE) return fail(MCMB_ERR_ARG, "mcmb_gamma_cross_gl_pol: pol %d", pol);
+    if (spline != MCMB_SPLINE_FMM && spline != MCMB_SPLINE_NATURAL)
+        return fail(MCMB_ERR_ARG, "mcmb_gamma_cross_gl_pol: spline %d", spline);
+    if (pol != MCMB_POL_TTT && !c->es.ready)
+        return fail(MCMB_ERR_STATE, "mcmb_gamma_cross_gl_pol: call mcmb_lspace_set_E first");
+    if (pol != MCMB_POL_TTT && !c->xs.pol)
+        return fail(MCMB_ERR_STATE, "mcmb_gamma_cross_gl_pol: call mcmb_qtilde_set_E first");
+    const int lt = c->ls.lmax, le = c->es.ready ? c->es.lmax : lt;
+    const int deg = pol == MCMB_POL_TTT ? 3 * lt : pol == MCMB_POL_TTE ? 2 * lt + le
+                  : pol == MCMB_POL_TEE ? lt + 2 * le : 3 * le;
+    // init_gamma_glint's node count with polarisation: 3 max(T lmax, E lmax)/2 + 1
+    if (npt <= 0) npt = 3 * std::max(lt, le) / 2 + 1;
+    if (2 * npt - 1 < deg)
+        return fail(MCMB_ERR_ARG, "mcmb_gamma_cross_gl_pol: %d nodes are not exact for degree %d", npt, deg);
+    MCMB_TRY(set_device(c));
+    const size_t gb = sizeof(double) * (size_t)(pol == MCMB_POL_TTT ? c->ls.n : c->es.n) * c->xs.n;
+    double* dst = gamma_dev;
+    if (!dst) {
+        MCMB_TRY(c->d_gamma_tmp.alloc(gb));
+        dst = c->d_gamma_tmp.as<double>();
+    }
+    MCMB_TRY(cross_gl_launch(c, pol, spline, npt, dst));
+    if (gamma_host) MCMB_HIP(hipMemcpy(gamma_host, dst, gb, hipMemcpyDeviceToHost));
     return MCMB_OK;
 }
 

@@ -20,6 +20,8 @@ static double* g_gammagl = NULL;        /* cached GL-factorised Gram, n x n */
 static double* g_cross[2] = {NULL, NULL};   /* cached GL cross-Gamma, FMM / natural spline */
 static long g_cross_gen[2] = {-1, -1};  /* q-tilde generation of each cross cache */
 static double* g_pol[4] = {NULL, NULL, NULL, NULL};   /* cached polarised GL Grams, TTT..EEE */
+static double* g_crossp[2][4];          /* cached polarised GL cross-Gamma, per x rule and component */
+static long g_crossp_gen[2][2] = {{-1, -1}, {-1, -1}};   /* (q-tilde, E q-tilde) generations of each */
 static int g_n = 0;
 static long g_gen = -1;                 /* l-space generation the caches belong to */
 static long g_egen = -1;                /* E setup generation the caches belong to (-1: none) */
@@ -31,6 +33,10 @@ static void drop_caches(void)
     free(g_gammagl); g_gammagl = NULL;
     for (int k = 0; k < 2; ++k) { free(g_cross[k]); g_cross[k] = NULL; g_cross_gen[k] = -1; }
     for (int k = 0; k < 4; ++k) { free(g_pol[k]); g_pol[k] = NULL; }
+    for (int sp = 0; sp < 2; ++sp) {
+        for (int k = 0; k < 4; ++k) { free(g_crossp[sp][k]); g_crossp[sp][k] = NULL; }
+        g_crossp_gen[sp][0] = g_crossp_gen[sp][1] = -1;
+    }
 }
 
 int mcmb_legacy_bind(mcmb_ctx* ctx)
@@ -562,42 +568,63 @@ void sync_orthoMij(void) { g_status = MCMB_OK; }
 /* ---- GL cross-Gamma (calculate_gamma.c) ------------------------------------ */
 
 /* Entry (m, n) of the cached GL cross-Gamma with x rule spline (0 FMM, 1
- * natural), computed whole on first use under g_mu. */
-static int cross_entry(int spline, int m, int n, double* out)
+ * natural) into result[0]; with the polarised setups bound (mcmb_lspace_set_E
+ * and mcmb_qtilde_set_E: the reference's do_polarisation = 1) into
+ * result[0..3] = TTT, TTE, TEE, EEE.  Each is computed whole on first use
+ * under g_mu and kept until a setup changes. */
+static int cross_entry(int spline, int m, int n, double* result)
 {
     pthread_mutex_lock(&g_mu);
-    int nq = 0;
-    long qgen = -1;
+    int nq = 0, ne = 0;
+    long qgen = -1, qegen = -1;
     int st = refresh();
     if (st == MCMB_OK && mcmb_qtilde_info(g_ctx, NULL, NULL, NULL, &nq, &qgen) != MCMB_OK) st = MCMB_ERR_STATE;
-    if (st == MCMB_OK && g_cross[spline] && g_cross_gen[spline] != qgen) {
-        free(g_cross[spline]);
-        g_cross[spline] = NULL;
-    }
-    if (st == MCMB_OK && !g_cross[spline]) {
-        double* g = (double*)malloc(sizeof(double) * (size_t)g_n * nq);
-        if (!g) st = MCMB_ERR_NOMEM;
-        else if ((st = mcmb_gamma_cross_gl(g_ctx, spline, 0, g, NULL)) != MCMB_OK) free(g);
-        else { g_cross[spline] = g; g_cross_gen[spline] = qgen; }
-    }
-    double v = 0.0;
+    const int pol = st == MCMB_OK && g_egen >= 0 && mcmb_qtilde_E_info(g_ctx, NULL, &qegen) == MCMB_OK;
+    if (pol && mcmb_lspace_E_info(g_ctx, NULL, NULL, &ne, NULL) == MCMB_OK && ne != g_n) st = MCMB_ERR_ARG;
     if (st == MCMB_OK && (m < 0 || n < 0 || m >= g_n || n >= nq)) st = MCMB_ERR_ARG;
-    if (st == MCMB_OK) v = g_cross[spline][(size_t)m * nq + n];
+    if (st == MCMB_OK && !pol) {
+        if (g_cross[spline] && g_cross_gen[spline] != qgen) {
+            free(g_cross[spline]);
+            g_cross[spline] = NULL;
+        }
+        if (!g_cross[spline]) {
+            double* g = (double*)malloc(sizeof(double) * (size_t)g_n * nq);
+            if (!g) st = MCMB_ERR_NOMEM;
+            else if ((st = mcmb_gamma_cross_gl(g_ctx, spline, 0, g, NULL)) != MCMB_OK) free(g);
+            else { g_cross[spline] = g; g_cross_gen[spline] = qgen; }
+        }
+        if (st == MCMB_OK) result[0] = g_cross[spline][(size_t)m * nq + n];
+    } else if (st == MCMB_OK) {
+        if (g_crossp_gen[spline][0] != qgen || g_crossp_gen[spline][1] != qegen) {
+            for (int k = 0; k < 4; ++k) { free(g_crossp[spline][k]); g_crossp[spline][k] = NULL; }
+            g_crossp_gen[spline][0] = qgen;
+            g_crossp_gen[spline][1] = qegen;
+        }
+        for (int k = 0; k < 4 && st == MCMB_OK; ++k) {
+            if (!g_crossp[spline][k]) {
+                double* g = (double*)malloc(sizeof(double) * (size_t)g_n * nq);
+                if (!g) { st = MCMB_ERR_NOMEM; break; }
+                if ((st = mcmb_gamma_cross_gl_pol(g_ctx, k, spline, 0, g, NULL)) != MCMB_OK) { free(g); break; }
+                g_crossp[spline][k] = g;
+            }
+            result[k] = g_crossp[spline][k][(size_t)m * nq + n];
+        }
+    }
     pthread_mutex_unlock(&g_mu);
-    *out = v;
     return st;
 }
 
-/* calculate_gamma.c:772-843: result[0] = Gamma_TTT(m, n), FMM spline in x */
+/* calculate_gamma.c:772-843: result[0] = Gamma_TTT(m, n), FMM spline in x
+ * (result[1..3] = TTE, TEE, EEE with the polarised setups bound) */
 void calculate_gamma_2d(int m, int n, double* result)
 {
-    g_status = cross_entry(0, m, n, &result[0]);
+    g_status = cross_entry(0, m, n, result);
 }
 
-/* calculate_gamma.c:936-1016: result[0] = Gamma_TTT(m, n), natural spline in x */
+/* calculate_gamma.c:936-1016: the same with the natural spline in x */
 void calculate_gamma(int m, int n, double* result)
 {
-    g_status = cross_entry(1, m, n, &result[0]);
+    g_status = cross_entry(1, m, n, result);
 }
 
 /* calculate_gamma.c:48-240: the tables are built on the GPU by the first
@@ -614,8 +641,20 @@ void init_gamma_glint(void)
 /* calculate_gamma.c:255-303: forms every M_ij and the FMM-spline Gamma whole */
 void precompute_gammaMij_T(void)
 {
-    double v;
-    g_status = cross_entry(0, 0, 0, &v);
+    double v[4];
+    g_status = cross_entry(0, 0, 0, v);
+}
+
+/* calculate_gamma.c:306-352: the E tables are formed with the T ones, by the
+ * first cross-Gamma of the polarised setups; MCMB_ERR_STATE without them */
+void precompute_gammaMij_E(void)
+{
+    double v[4];
+    pthread_mutex_lock(&g_mu);
+    int st = refresh();
+    if (st == MCMB_OK && (g_egen < 0 || mcmb_qtilde_E_info(g_ctx, NULL, NULL) != MCMB_OK)) st = MCMB_ERR_STATE;
+    pthread_mutex_unlock(&g_mu);
+    g_status = st == MCMB_OK ? cross_entry(0, 0, 0, v) : st;
 }
 
 /* calculate_gamma.c:354-386: per-(i, j) M of one rank; the GPU forms them all */
@@ -624,6 +663,18 @@ void calculate_gammaMij_T(int i, int j)
     (void)i;
     (void)j;
     init_gamma_glint();
+}
+
+/* calculate_gamma.c:388-423: as calculate_gammaMij_T for the E field */
+void calculate_gammaMij_E(int i, int j)
+{
+    (void)i;
+    (void)j;
+    init_gamma_glint();
+    if (g_status != MCMB_OK) return;
+    pthread_mutex_lock(&g_mu);
+    if (g_egen < 0 || mcmb_qtilde_E_info(g_ctx, NULL, NULL) != MCMB_OK) g_status = MCMB_ERR_STATE;
+    pthread_mutex_unlock(&g_mu);
 }
 
 /* calculate_gamma.c:425-509: one GPU holds every M_ij, nothing to sum */

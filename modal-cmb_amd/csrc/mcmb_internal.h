@@ -170,6 +170,13 @@ struct CrossSet {
     std::vector<double> qt, xvec, cl, nl, bl;
     std::vector<int> order;
     DevBuf d_qt, d_x, d_modes;
+    // E-mode part (polarised cross-Gamma): the primordial side of M^E, qc =
+    // s1 qt_E - s2 qt_T, and the TE / EE spectra; dropped by a new T setup
+    bool pol = false;
+    long pol_generation = -1;
+    double fsky_ratio = 1.0;
+    std::vector<double> cte, cee, nte, nee, bte, bee;
+    DevBuf d_qc;
 };
 
 // E-mode field of the late basis, the second field of the polarised Gram:

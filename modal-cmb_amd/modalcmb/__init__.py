@@ -63,6 +63,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_lspace_clear_E.argtypes = [vp]
         lib.mcmb_lspace_E_info.argtypes = [vp, C.POINTER(ip), C.POINTER(ip), C.POINTER(ip), C.POINTER(C.c_long)]
         lib.mcmb_gamma_gl_pol.argtypes = [vp, ip, ip, vp, vp]
+        lib.mcmb_qtilde_set_E.argtypes = [vp] + [vp] * 7 + [dp]
+        lib.mcmb_qtilde_E_info.argtypes = [vp, C.POINTER(dp), C.POINTER(C.c_long)]
+        lib.mcmb_gamma_cross_gl_pol.argtypes = [vp, ip, ip, ip, vp, vp]
         lib.plijk_TTE.restype = dp
         lib.plijk_TTE.argtypes = [ip, ip, ip, ip]
         lib.plijk_TEE.restype = dp
@@ -162,7 +165,8 @@ EXPORTED_SYMBOLS = (
     "sync_orthoMij", "mcmb_lspace_set_E", "mcmb_lspace_clear_E", "mcmb_lspace_E_info", "mcmb_lspace_E_tables", "mcmb_gamma_gl_pol",
     "calculate_orthoMij_E", "find_perm_late_TTE", "find_perm_late_TEE", "find_perm_late_EEE", "plijk_TTE",
     "plijk_TEE", "plijk_EEE", "get_ijk_next_XXY", "check_ijk_min_XXY", "check_ijk_max_XXY",
-    "decompose_tetrapyd_XXY", "mcmb_file_name", "mcmb_write_qtilde", "mcmb_read_qtilde", "mcmb_basis_late_order9",
+    "decompose_tetrapyd_XXY", "mcmb_qtilde_set_E", "mcmb_qtilde_E_info", "mcmb_gamma_cross_gl_pol",
+    "precompute_gammaMij_E", "calculate_gammaMij_E", "mcmb_file_name", "mcmb_write_qtilde", "mcmb_read_qtilde", "mcmb_basis_late_order9",
     "find_perm_late_TTT",
 )
 
@@ -281,6 +285,19 @@ class Context:
         _check(_lib.mcmb_qtilde_set(self._h, np_, nlq - 1, nx, _ptr(qt), _ptr(xvec), op.shape[0], _ptr(op),
                                     _ptr(sp[0]), _ptr(sp[1]), _ptr(sp[2]), float(deltaphi)))
         self.nprim = op.shape[0]
+
+    def qtilde_set_E(self, qt_E: np.ndarray, cl_TE, cl_EE, nl_TE, nl_EE, bl_TE, bl_EE, fsky_ratio: float) -> None:
+        """E-mode q-tilde of the T q-tilde's shape and the TE / EE spectra per multipole."""
+        qt_E = np.ascontiguousarray(qt_E, dtype=np.float64)
+        sp = [np.ascontiguousarray(a, dtype=np.float64) for a in (cl_TE, cl_EE, nl_TE, nl_EE, bl_TE, bl_EE)]
+        self._keepQE = (qt_E, sp)
+        _check(_lib.mcmb_qtilde_set_E(self._h, _ptr(qt_E), *(_ptr(a) for a in sp), float(fsky_ratio)))
+
+    def gamma_cross_gl_pol(self, pol: int, spline: int = SPLINE_FMM, npt: int = 0) -> np.ndarray:
+        n = self.n if pol == POL_TTT else self.nE
+        g = np.empty((n, self.nprim))
+        _check(_lib.mcmb_gamma_cross_gl_pol(self._h, pol, spline, npt, _ptr(g), None))
+        return g
 
     def gamma_cross_3d(self, rank: int = 0, nranks: int = 1) -> np.ndarray:
         g = np.empty((self.n, self.nprim))

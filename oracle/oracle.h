@@ -142,6 +142,19 @@ void or_gamma_xxy_direct(int pol, int Tlmin, int Tlmax, const double* qT, int ld
                          int Elmax, const double* qE, int ldqE, const double* vE, int n, const int* order,
                          double* gamma);
 
+/* calculate_gamma_2d (spline 0) / calculate_gamma (spline 1) component pol
+ * (0 TTT .. 3 EEE) for all (m, n), calculate_gamma.c:48-352, 511-1016: M^T
+ * over [Tlmin, Tlmax] with TTw qtT, M^E over [Elmin, Elmax] with EEw (s1 qtE
+ * - s2 qtT), late slot b reading the field of the component; spec = 9 rows
+ * of lsize_q (C_TT, C_TE, C_EE, N_TT, N_TE, N_EE, b_TT, b_TE, b_EE).
+ * Pinned: an x-separable q-tilde reduces every component to a direct sum
+ * over the mixed cube with the exact h^2 (tests/test_oracle_pol.py). */
+void or_gamma_cross_gl_pol(int pol, int Tlmin, int Tlmax, const double* bT, int ldbT, int Elmin, int Elmax,
+                           const double* bE, int ldbE, const int* order_late, int nlate, int np, int lsize_q,
+                           const double* qtT, const double* qtE, int nx, const double* xvec, const int* order_prim,
+                           int n, const double* spec, double fsky_ratio, double dphi, int spline, int npt,
+                           double* gamma);
+
 /* ---- orthonormalisation (ortho_prim_mpi.c:198-284) ---------------------- */
 /* norm_i = sqrt(G_ii); C = D^-1 G D^-1; C = L L^T; ortho = symmetrised D L
  * (entry [i][j] = [j][i] = norm[max] L(max,min), update_ortho fixed_data.c:3285);

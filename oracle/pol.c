@@ -324,3 +324,128 @@ void or_gamma_xxy_direct(int pol, int Tlmin, int Tlmax, const double* qT, int ld
     for (int m = 0; m < n; ++m)
         for (int c = m + 1; c < n; ++c) gamma[(long)c * n + m] = gamma[(long)m * n + c];
 }
+
+/* ---- polarised cross-Gamma (calculate_gamma.c) --------------------------- */
+
+/* calculate_gamma_2d (spline 0) / calculate_gamma (spline 1), component pol,
+ * for all (m, n) (gamma_pt / gamma_2d_pt_XXX calculate_gamma.c:511-771,
+ * tables of init_gamma_glint 48-250 and precompute_gammaMij_T / _E 253-352):
+ *   M^T[p][q][x](mu_s) = sum_{l=Tlmin..Tlmax} P_l(mu_s) TTw(l) qtT[p][l][x] bT_q(l),
+ *     TTw = (2l+1)^(5/6) b_TT / sqrt(b_TT^2 C_TT + N_TT)
+ *   M^E[p][q][x](mu_s) = sum_{l=Elmin..Elmax} P_l(mu_s) EEw(l) (s1 qtE - s2 qtT)[p][l][x] bE_q(l),
+ *     s1 = b_TT^2 C_TT + N_TT, s2 = sqrt(fsky_E/fsky_T) b_TE^2 C_TE + N_TE,
+ *     s3 = b_TT b_EE C_TE + N_TE, s4 = b_EE^2 C_EE + N_EE,
+ *     EEw = (2l+1)^(5/6) b_EE / sqrt(s1 (s1 s4 - s3^2))
+ *   (both weights 0 for l <= 1);
+ *   y_x = x^2 sum_s w_s perm_3x3(M^{X_b}[p_a][q_b][x][s]), X_b = the field of
+ *   late slot b (TTT: T T T, TTE: T T E, TEE: T E E, EEE: E E E), zeroed for
+ *   x < 10 when |y| > 1e-18; Gamma[m][n] = dphi^2/(8 pi) * spline integral.
+ * order_late is the component's table (find_perm_late_XXX slot order); spec
+ * holds 9 rows of lsize_q values: C_TT, C_TE, C_EE, N_TT, N_TE, N_EE, b_TT,
+ * b_TE, b_EE; qtT / qtE are [np][lsize_q][nx]. */
+void or_gamma_cross_gl_pol(int pol, int Tlmin, int Tlmax, const double* bT, int ldbT, int Elmin, int Elmax,
+                           const double* bE, int ldbE, const int* order_late, int nlate, int np, int lsize_q,
+                           const double* qtT, const double* qtE, int nx, const double* xvec, const int* order_prim,
+                           int n, const double* spec, double fsky_ratio, double dphi, int spline, int npt,
+                           double* gamma)
+{
+    static const int fld[4][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+    const double *CTT = spec, *CTE = spec + lsize_q, *CEE = spec + 2 * lsize_q, *NTT = spec + 3 * lsize_q,
+                 *NTE = spec + 4 * lsize_q, *NEE = spec + 5 * lsize_q, *BTT = spec + 6 * lsize_q,
+                 *BTE = spec + 7 * lsize_q, *BEE = spec + 8 * lsize_q;
+    int R[2] = {0, 0};
+    for (int m = 0; m < nlate; ++m)
+        for (int b = 0; b < 3; ++b) {
+            const int f = fld[pol][b];
+            if (order_late[3 * m + b] + 1 > R[f]) R[f] = order_late[3 * m + b] + 1;
+        }
+    double* gx = (double*)malloc(sizeof(double) * npt);
+    double* gw = (double*)malloc(sizeof(double) * npt);
+    or_gauss_legendre(npt, gx, gw);
+    const int lmaxall = Tlmax > Elmax ? Tlmax : Elmax;
+    /* per field: the l weight and the x-resolved primordial side qs[p][l][x] */
+    const int lo[2] = {Tlmin, Elmin}, hi[2] = {Tlmax, Elmax}, ldb[2] = {ldbT, ldbE};
+    const double* bb[2] = {bT, bE};
+    double* M[2] = {NULL, NULL};
+    for (int f = 0; f < 2; ++f) {
+        if (R[f] == 0) continue;
+        double* wl = (double*)calloc((size_t)lmaxall + 1, sizeof(double));
+        double* qs = (double*)malloc(sizeof(double) * (size_t)np * lsize_q * nx);
+        for (int l = lo[f]; l <= hi[f]; ++l) {
+            if (l <= 1) continue;
+            const double s1 = BTT[l] * BTT[l] * CTT[l] + NTT[l];
+            if (f == 0) {
+                wl[l] = pow(2.0 * l + 1.0, 5.0 / 6.0) * (BTT[l] / sqrt(s1));
+            } else {
+                const double s3 = BTT[l] * BEE[l] * CTE[l] + NTE[l];
+                const double s4 = BEE[l] * BEE[l] * CEE[l] + NEE[l];
+                wl[l] = pow(2.0 * l + 1.0, 5.0 / 6.0) * (BEE[l] / sqrt(s1 * (s1 * s4 - s3 * s3)));
+            }
+        }
+        for (int p = 0; p < np; ++p)
+            for (int l = 0; l < lsize_q; ++l) {
+                const double s1 = BTT[l] * BTT[l] * CTT[l] + NTT[l];
+                const double s2 = sqrt(fsky_ratio) * BTE[l] * BTE[l] * CTE[l] + NTE[l];
+                for (int x = 0; x < nx; ++x) {
+                    const long e = ((long)p * lsize_q + l) * nx + x;
+                    qs[e] = f == 0 ? qtT[e] : s1 * qtE[e] - s2 * qtT[e];
+                }
+            }
+        /* M[f][((p * R + q) * nx + x) * npt + s] */
+        double* Mf = (double*)calloc((size_t)np * R[f] * nx * npt, sizeof(double));
+#pragma omp parallel
+        {
+            double* Pl = (double*)malloc(sizeof(double) * (lmaxall + 1));
+#pragma omp for schedule(dynamic, 1)
+            for (int s = 0; s < npt; ++s) {
+                Pl[0] = 1.0;
+                if (lmaxall > 0) Pl[1] = gx[s];
+                for (int l = 2; l <= lmaxall; ++l)
+                    Pl[l] = ((2.0 * l - 1.0) * gx[s] * Pl[l - 1] - (l - 1.0) * Pl[l - 2]) / l;
+                for (int p = 0; p < np; ++p)
+                    for (int q = 0; q < R[f]; ++q)
+                        for (int x = 0; x < nx; ++x) {
+                            double sum = 0.0;
+                            for (int l = lo[f]; l <= hi[f]; ++l)
+                                sum += Pl[l] * (wl[l] * qs[((long)p * lsize_q + l) * nx + x] * bb[f][(long)l * ldb[f] + q]);
+                            Mf[(((long)p * R[f] + q) * nx + x) * npt + s] = sum;
+                        }
+            }
+            free(Pl);
+        }
+        M[f] = Mf;
+        free(wl);
+        free(qs);
+    }
+    const double scale = dphi * dphi / (8.0 * M_PI);
+    const int f1 = fld[pol][0], f2 = fld[pol][1], f3 = fld[pol][2];
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int m = 0; m < nlate; ++m) {
+        double* y = (double*)malloc(sizeof(double) * nx);
+        const int q1 = order_late[3 * m], q2 = order_late[3 * m + 1], q3 = order_late[3 * m + 2];
+        for (int c = 0; c < n; ++c) {
+            const int p1 = order_prim[3 * c], p2 = order_prim[3 * c + 1], p3 = order_prim[3 * c + 2];
+            for (int x = 0; x < nx; ++x) {
+#define MM(f, p, q) (M[f] + (((long)(p) * R[f] + (q)) * nx + x) * npt)
+                /* a_ab = M^{X_b}[p_a][q_b] */
+                const double *A11 = MM(f1, p1, q1), *A12 = MM(f2, p1, q2), *A13 = MM(f3, p1, q3);
+                const double *A21 = MM(f1, p2, q1), *A22 = MM(f2, p2, q2), *A23 = MM(f3, p2, q3);
+                const double *A31 = MM(f1, p3, q1), *A32 = MM(f2, p3, q2), *A33 = MM(f3, p3, q3);
+#undef MM
+                double r = 0.0;
+                for (int s = 0; s < npt; ++s)
+                    r += gw[s] * (A11[s] * A22[s] * A33[s] + A11[s] * A23[s] * A32[s] + A12[s] * A21[s] * A33[s] +
+                                  A12[s] * A23[s] * A31[s] + A13[s] * A21[s] * A32[s] + A13[s] * A22[s] * A31[s]);
+                y[x] = xvec[x] * xvec[x] * r;
+                if (x < 10 && fabs(y[x]) > 1e-18) y[x] = 0.0;
+            }
+            const double I = spline ? or_spline_integral_natural(xvec, y, nx) : or_spline_integral_fmm(xvec, y, nx);
+            gamma[(long)m * n + c] = I * scale;
+        }
+        free(y);
+    }
+    free(M[0]);
+    free(M[1]);
+    free(gx);
+    free(gw);
+}

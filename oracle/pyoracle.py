@@ -91,6 +91,8 @@ def lib() -> C.CDLL:
         L.or_check_ijk_max_XXY.argtypes = [ip, ip] + [C.POINTER(ip)] * 3
         L.or_gamma_gl_pol.argtypes = [ip, ip, ip, vp, ip, vp, ip, ip, vp, ip, vp, ip, vp, ip, vp]
         L.or_gamma_xxy_direct.argtypes = [ip, ip, ip, vp, ip, vp, ip, ip, vp, ip, vp, ip, vp, vp]
+        L.or_gamma_cross_gl_pol.argtypes = [ip, ip, ip, vp, ip, ip, ip, vp, ip, vp, ip, ip, ip, vp, vp, ip, vp, vp,
+                                            ip, vp, dp, dp, ip, ip, vp]
         _lib = L
     return _lib
 
@@ -208,6 +210,30 @@ def gamma_xxy_direct(pol, T, E, order) -> np.ndarray:
     g = np.empty((n, n))
     lib().or_gamma_xxy_direct(pol, tl, th, _p(qT), qT.shape[1], _p(vT), el, eh, _p(qE), qE.shape[1], _p(vE), n,
                               _p(order), _p(g))
+    return g
+
+
+def gamma_cross_gl_pol(pol, T, E, order_late, qtT, qtE, xvec, order_prim, spec, fsky_ratio, dphi, spline=0,
+                       npt=None) -> np.ndarray:
+    """calculate_gamma(_2d)'s component pol; T / E = (lmin, lmax, b[l][R]) of
+    the late fields, qtT / qtE [np][lsize_q][nx], spec [9][lsize_q] (C_TT, C_TE,
+    C_EE, N_TT, N_TE, N_EE, b_TT, b_TE, b_EE)."""
+    (tl, th, bT), (el, eh, bE) = T, E
+    bT = np.ascontiguousarray(bT, dtype=np.float64)
+    bE = np.ascontiguousarray(bE, dtype=np.float64)
+    ol = np.ascontiguousarray(order_late, dtype=np.int32)
+    op = np.ascontiguousarray(order_prim, dtype=np.int32)
+    qtT = np.ascontiguousarray(qtT, dtype=np.float64)
+    qtE = np.ascontiguousarray(qtE, dtype=np.float64)
+    xvec = np.ascontiguousarray(xvec, dtype=np.float64)
+    spec = np.ascontiguousarray(spec, dtype=np.float64)
+    np_, lsq, nx = qtT.shape
+    assert qtE.shape == qtT.shape and spec.shape == (9, lsq)
+    npt = 3 * max(th, eh) // 2 + 1 if npt is None else npt
+    g = np.empty((ol.shape[0], op.shape[0]))
+    lib().or_gamma_cross_gl_pol(pol, tl, th, _p(bT), bT.shape[1], el, eh, _p(bE), bE.shape[1], _p(ol), ol.shape[0],
+                                np_, lsq, _p(qtT), _p(qtE), nx, _p(xvec), _p(op), op.shape[0], _p(spec),
+                                float(fsky_ratio), float(dphi), spline, npt, _p(g))
     return g
 
 

@@ -150,3 +150,110 @@ def test_legacy_calculate_orthol_pol(ctx):
         assert lib.calculate_orthol(0, 0, res) == -1
     finally:
         lib.mcmb_legacy_bind(None)
+
+
+# ---- polarised cross-Gamma (calculate_gamma / calculate_gamma_2d result[0..3]) ----
+
+def cross_case(Lq=40, T=(2, 40), E=(2, 36), RT=4, RE=4, npb=4, nx=26, seed=0):
+    """non-separable q-tilde of T and E, TT/TE/EE spectra with s1 s4 - s3^2 > 0;
+    the first 6 x samples scaled by 1e-12 (kept by the x < 10 rule, far below
+    its 1e-18 threshold), samples 6..9 far above it (dropped)"""
+    rng = np.random.default_rng(seed)
+    ls = np.arange(Lq + 1.0)
+    xv = np.sort(rng.uniform(0.2, 3.0, nx))
+    qt = np.zeros((2, npb, Lq + 1, nx))
+    for f in range(2):
+        A = rng.standard_normal((npb, 3))
+        for p in range(npb):
+            for a in range(3):
+                qt[f, p] += A[p, a] * np.outer(np.cos((a + 1 + f) * ls / Lq * (p + 1)), np.sin((a + 1) * xv + p + f))
+    qt[:, :, :, :6] *= 1e-12
+    ctt = np.where(ls > 0, 1000.0 / np.maximum(ls * (ls + 1), 1.0), 0.0)
+    cee = 0.05 * ctt * (1 + 0.2 * np.cos(ls / 9))
+    cte = 0.4 * np.sqrt(ctt * cee) * np.sin(ls / 5)
+    spec = np.array([ctt, cte, cee, 0.02 + 0 * ls, 0.003 * np.cos(ls / 4), 0.03 + 0 * ls,
+                     np.exp(-ls * (ls + 1) * 2e-4), np.exp(-ls * (ls + 1) * 3e-4), np.exp(-ls * (ls + 1) * 4e-4)])
+    Tf, Ef = field(*T, RT, seed + 1), field(*E, RE, seed + 2)
+    return dict(qtT=qt[0], qtE=qt[1], xv=xv, spec=spec, T=Tf, E=Ef, RT=RT, RE=RE, npb=npb, fr=0.8, dphi=1.5714e-8,
+                prim=W.order_lex(npb)[rng.permutation(len(W.order_lex(npb)))].astype(np.int32))
+
+
+def cross_setup(ctx, c):
+    T, E = c["T"], c["E"]
+    ttt, tte, tee, eee = setup(ctx, T, E, c["RT"], c["RE"], n=None)
+    s = c["spec"]
+    ctx.qtilde_set(c["qtT"], c["xv"], c["prim"], s[0], s[3], s[6], c["dphi"])
+    ctx.qtilde_set_E(c["qtE"], s[1], s[2], s[4], s[5], s[7], s[8], c["fr"])
+    return ttt, tte, tee, eee
+
+
+@pytest.mark.parametrize("kw", [{}, dict(T=(5, 33), E=(2, 40), RT=5, RE=3, npb=3, seed=3)])
+@pytest.mark.parametrize("spline", [modalcmb.SPLINE_FMM, modalcmb.SPLINE_NATURAL])
+def test_cross_gl_pol_vs_oracle(ctx, kw, spline):
+    c = cross_case(**kw)
+    tabs = cross_setup(ctx, c)
+    for pol in range(4):
+        G = ctx.gamma_cross_gl_pol(pol, spline)
+        ref = O.gamma_cross_gl_pol(pol, c["T"][:3], c["E"][:3], tabs[pol], c["qtT"], c["qtE"], c["xv"], c["prim"],
+                                   c["spec"], c["fr"], c["dphi"], spline=spline)
+        assert G.shape == ref.shape
+        assert relmax(G, ref) < 1e-12, pol
+    # TTT on the T lmax's node count is the TT cross-Gamma, bit for bit
+    lt = c["T"][1]
+    assert np.array_equal(ctx.gamma_cross_gl_pol(0, spline, 3 * lt // 2 + 1), ctx.gamma_cross_gl(spline))
+
+
+def test_legacy_calculate_gamma_pol(ctx):
+    lib = modalcmb.load_library()
+    c = cross_case(RT=4, RE=4)
+    RT, RE = 4, 4
+    T, E = c["T"], c["E"]
+    ttt = W.order_lex(RT).astype(np.int32)
+    tte, tee, eee = tables(RT, RE, n=len(ttt))
+    ctx.lspace_set(T[0], T[1], T[2], T[3], ttt, h2_mode=modalcmb.H2_ASYMPTOTIC)
+    s = c["spec"]
+    ctx.qtilde_set(c["qtT"], c["xv"], c["prim"], s[0], s[3], s[6], c["dphi"])
+    lib.mcmb_legacy_bind(ctx._h)
+    try:
+        res = (C.c_double * 4)(-1, -1, -1, -1)
+        lib.calculate_gamma_2d(2, 3, res)                  # T only: result[0]
+        assert lib.mcmb_legacy_status() == 0 and res[1] == -1
+        lib.precompute_gammaMij_E()
+        assert lib.mcmb_legacy_status() == -7
+        ctx.lspace_set_E(E[0], E[1], E[2], E[3], tte, tee, eee)
+        ctx.qtilde_set_E(c["qtE"], s[1], s[2], s[4], s[5], s[7], s[8], c["fr"])
+        lib.precompute_gammaMij_E()
+        assert lib.mcmb_legacy_status() == 0
+        lib.calculate_gammaMij_E(0, 0)
+        assert lib.mcmb_legacy_status() == 0
+        for spline, fn in ((modalcmb.SPLINE_FMM, lib.calculate_gamma_2d), (modalcmb.SPLINE_NATURAL, lib.calculate_gamma)):
+            G = [ctx.gamma_cross_gl_pol(p, spline) for p in range(4)]
+            for m, k in ((0, 0), (2, 3), (len(ttt) - 1, 5)):
+                fn(m, k, res)
+                assert lib.mcmb_legacy_status() == 0
+                assert [res[q] for q in range(4)] == [G[q][m, k] for q in range(4)]
+        # a new T q-tilde drops the E part: result[0] only again
+        ctx.qtilde_set(c["qtT"] * 2.0, c["xv"], c["prim"], s[0], s[3], s[6], c["dphi"])
+        res2 = (C.c_double * 4)(-1, -1, -1, -1)
+        lib.calculate_gamma_2d(2, 3, res2)
+        assert lib.mcmb_legacy_status() == 0 and res2[1] == -1
+    finally:
+        lib.mcmb_legacy_bind(None)
+
+
+def test_cross_gl_pol_needs_e_setups(ctx):
+    c = cross_case()
+    T = c["T"]
+    s = c["spec"]
+    ctx.lspace_set(T[0], T[1], T[2], T[3], W.order_lex(c["RT"]), h2_mode=modalcmb.H2_ASYMPTOTIC)
+    ctx.qtilde_set(c["qtT"], c["xv"], c["prim"], s[0], s[3], s[6], c["dphi"])
+    ctx.nE = 1
+    with pytest.raises(modalcmb.ModalError) as e:
+        ctx.gamma_cross_gl_pol(modalcmb.POL_TEE)
+    assert e.value.status == -7                            # no E late field
+    E = c["E"]
+    tte, tee, eee = tables(c["RT"], c["RE"])
+    ctx.lspace_set_E(E[0], E[1], E[2], E[3], tte, tee, eee)
+    with pytest.raises(modalcmb.ModalError) as e:
+        ctx.gamma_cross_gl_pol(modalcmb.POL_TEE)
+    assert e.value.status == -7                            # no E q-tilde

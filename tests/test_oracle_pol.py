@@ -161,3 +161,113 @@ def test_direct_mixed_sum_swaps_fields():
     g_tee = O.gamma_xxy_direct(2, T, E, tee)
     g_swapped = O.gamma_xxy_direct(1, E, T, tee[:, [1, 2, 0]])
     assert relmax(g_tee, g_swapped) < 1e-13
+
+
+# ---- the polarised cross-Gamma ------------------------------------------------
+
+FIELDS_OF = {0: (0, 0, 0), 1: (0, 0, 1), 2: (0, 1, 1), 3: (1, 1, 1)}
+
+
+def spectra(lsize):
+    """C_TT, C_TE, C_EE, N_TT, N_TE, N_EE, b_TT, b_TE, b_EE with
+    s1 s4 - s3^2 > 0 at every l"""
+    ls = np.arange(lsize, dtype=np.float64)
+    ctt = np.where(ls > 0, 1000.0 / np.maximum(ls * (ls + 1), 1.0), 0.0)
+    cee = 0.05 * ctt * (1 + 0.2 * np.cos(ls / 9))
+    cte = 0.4 * np.sqrt(ctt * cee) * np.sin(ls / 5)
+    ntt, nte, nee = 0.02 + 0 * ls, 0.003 * np.cos(ls / 4), 0.03 + 0 * ls
+    btt, bte, bee = (np.exp(-ls * (ls + 1) * s) for s in (2e-4, 3e-4, 4e-4))
+    return np.array([ctt, cte, cee, ntt, nte, nee, btt, bte, bee])
+
+
+def separable_qtilde(T, E, spec, np_, nx, fsky_ratio, seed):
+    """q-tilde with x dependence g(x) only, built so that TTw qt_T = g (2l+1)
+    v_T b^T_p over the T range and EEw (s1 qt_E - s2 qt_T) = g (2l+1) v_E b^E_p
+    over the E range (the weights of calculate_gamma.c:73-160).  g is scaled by
+    1e-9 on the first 10 samples so that the x < 10 rule keeps them."""
+    rng = np.random.default_rng(seed)
+    (tl, th, qT, vT), (el, eh, qE, vE) = T, E
+    lsq = spec.shape[1]
+    CTT, CTE, CEE, NTT, NTE, NEE, BTT, BTE, BEE = spec
+    x = np.sort(rng.uniform(0.5, 3.0, nx))
+    g = np.cos(x) + 1.5
+    g[:10] *= 1e-9
+    ls = np.arange(lsq, dtype=np.float64)
+    s1 = BTT ** 2 * CTT + NTT
+    s2 = np.sqrt(fsky_ratio) * BTE ** 2 * CTE + NTE
+    s3 = BTT * BEE * CTE + NTE
+    s4 = BEE ** 2 * CEE + NEE
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ttw = (2 * ls + 1) ** (5 / 6) * BTT / np.sqrt(s1)
+        eew = (2 * ls + 1) ** (5 / 6) * BEE / np.sqrt(s1 * (s1 * s4 - s3 ** 2))
+    qtT = np.zeros((np_, lsq, nx))
+    qtE = np.zeros((np_, lsq, nx))
+    for l in range(max(tl, 2), th + 1):
+        qtT[:, l, :] = np.outer((2 * l + 1) * vT[l] * qT[l, :np_] / ttw[l], g)
+    for l in range(max(el, 2), eh + 1):
+        comb = np.outer((2 * l + 1) * vE[l] * qE[l, :np_] / eew[l], g)
+        qtE[:, l, :] = (comb + s2[l] * qtT[:, l, :]) / s1[l]
+    return x, g, qtT, qtE
+
+
+def cube_sum(T, E, pol, L):
+    """D[a1, q1, a2, q2, a3, q3] = sum over the full cube (legs in their
+    fields' ranges) of h^2 prod_b v(l_b) b_{a_b}(l_b) b_{q_b}(l_b), exact h^2"""
+    H = np.zeros((L, L, L))
+    for i in range(L):
+        for j in range(L):
+            for k in range(L):
+                if (i + j + k) % 2 == 0 and abs(i - j) <= k <= i + j:
+                    H[i, j, k] = O.lib().or_h2_exact(i, j, k)
+    A = []
+    for f in FIELDS_OF[pol]:
+        lo, hi, q, v = T if f == 0 else E
+        a = np.zeros((L, q.shape[1], q.shape[1]))
+        for l in range(lo, hi + 1):
+            a[l] = v[l] * np.outer(q[l], q[l])
+        A.append(a)
+    return np.einsum("ijk,iab,jcd,kef->abcdef", H, A[0], A[1], A[2], optimize=True)
+
+
+@pytest.mark.parametrize("pol", [0, 1, 2, 3])
+@pytest.mark.parametrize("spline", [0, 1])
+def test_cross_gl_pol_reduces_to_mixed_cube_sums(pol, spline):
+    R = 3
+    T = field(2, 22, R, 11)
+    E = field(3, 26, R, 12)
+    L = 27
+    spec = spectra(L)
+    fr, dphi = 0.8, 1.5714e-8
+    x, g, qtT, qtE = separable_qtilde(T, E, spec, R, 24, fr, 13)
+    late = W.order_lex(R).astype(np.int32)            # any slot assignment of indices < R
+    rng = np.random.default_rng(14)
+    prim = W.order_lex(R).astype(np.int32)[rng.permutation(10)]
+    G = O.gamma_cross_gl_pol(pol, T[:3], E[:3], late, qtT, qtE, x, prim, spec, fr, dphi, spline=spline)
+    y = x ** 2 * g ** 3
+    Ix = O.spline_integral(x, y, "natural" if spline else "fmm")
+    D = cube_sum(T, E, pol, L)
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    ref = np.empty_like(G)
+    for m, (q1, q2, q3) in enumerate(late):
+        for c, pn in enumerate(prim):
+            ref[m, c] = sum(D[pn[s[0]], q1, pn[s[1]], q2, pn[s[2]], q3] for s in perms)
+    ref *= dphi ** 2 * Ix
+    assert relmax(G, ref) < 1e-11
+
+
+def test_cross_gl_pol_ttt_is_the_tt_cross_gamma():
+    """component 0 with no E field in use equals the TT-only restatement"""
+    R = 3
+    T = field(2, 22, R, 21)
+    E = field(2, 22, R, 22)
+    spec = spectra(23)
+    rng = np.random.default_rng(23)
+    qtT = rng.standard_normal((R, 23, 12))
+    qtE = rng.standard_normal((R, 23, 12))
+    x = np.sort(rng.uniform(0.2, 3.0, 12))
+    late = W.order_lex(R).astype(np.int32)
+    prim = late[rng.permutation(len(late))]
+    for spline in (0, 1):
+        G = O.gamma_cross_gl_pol(0, T[:3], E[:3], late, qtT, qtE, x, prim, spec, 0.7, 2e-8, spline=spline)
+        ref = O.gamma_cross_gl(2, 22, T[2], late, qtT, x, prim, spec[0], spec[3], spec[6], 2e-8, spline=spline)
+        assert relmax(G, ref) < 1e-14

@@ -501,9 +501,10 @@ def main():
     t = time.time()
     G_one = ctx.gamma_tet(modalcmb.FP64)
     t_one = time.time() - t
+    parts = {name: round(ms, 3) for name, ms in ctx.last_setup()}
     assert np.array_equal(G_one, G)
     one_shot = {"lspace_set_s": round(t_set, 4), "first_gamma_to_host_s": round(t_one, 4),
-                "total_s": round(t_set + t_one, 4),
+                "total_s": round(t_set + t_one, 4), "first_gamma_parts_ms": parts,
                 "note": "what one gamma_3d_host-style call costs on a fresh setup: tables to the device, basis "
                         "factorisation, host row tables, buffer allocation, the Gamma and its 197 MB copy "
                         "to pageable host memory"}

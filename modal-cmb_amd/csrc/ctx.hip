@@ -54,35 +54,110 @@ int set_device(mcmb_ctx* ctx)
     return MCMB_OK;
 }
 
+constexpr size_t STAGE_BYTES = (size_t)16 << 20;   // one half of the pinned staging
+
+int copy_to_host(mcmb_ctx* c, void* host, const void* dev, size_t bytes)
+{
+    if (!bytes) return MCMB_OK;
+    const char* env = getenv("MCMB_HOST_COPY");
+    if (bytes < 2 * STAGE_BYTES || (env && !strcmp(env, "direct"))) {
+        MCMB_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        return MCMB_OK;
+    }
+    if (!c->stage) {
+        MCMB_HIP(hipHostMalloc(&c->stage, 2 * STAGE_BYTES, hipHostMallocDefault));
+        for (int k = 0; k < 2; ++k) MCMB_HIP(hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming));
+    }
+    char* const stage = static_cast<char*>(c->stage);
+    const char* const src = static_cast<const char*>(dev);
+    char* const dst = static_cast<char*>(host);
+    const size_t nk = (bytes + STAGE_BYTES - 1) / STAGE_BYTES;
+    auto issue = [&](size_t k) -> int {
+        const size_t off = k * STAGE_BYTES, len = std::min(STAGE_BYTES, bytes - off);
+        MCMB_HIP(hipMemcpyAsync(stage + (k & 1) * STAGE_BYTES, src + off, len, hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipEventRecord(c->stage_ev[k & 1], c->stream));
+        return MCMB_OK;
+    };
+    const int nth = std::min(8, host_threads());
+    MCMB_TRY(issue(0));
+    for (size_t k = 0; k < nk; ++k) {
+        // half (k + 1) & 1 was emptied into host memory at step k - 1
+        if (k + 1 < nk) MCMB_TRY(issue(k + 1));
+        MCMB_HIP(hipEventSynchronize(c->stage_ev[k & 1]));
+        const size_t off = k * STAGE_BYTES, len = std::min(STAGE_BYTES, bytes - off);
+        const char* half = stage + (k & 1) * STAGE_BYTES;
+        run_threads(nth, [&](int t) {
+            const size_t a = len * t / nth, b = len * (t + 1) / nth;
+            memcpy(dst + off + a, half + a, b - a);
+        });
+    }
+    return MCMB_OK;
+}
+
 // Rows of T(lmin, lmax) in the order get_ijk_next_XXX walks them
 // (tetrapyd_tools.c:104-142): i ascending from lmin, j from i up to
-// lmax - i%2, k from j + i%2 in steps of 2 up to min(i + j, lmax).
-void build_lspace_rows(int lmin, int lmax, std::vector<RowDesc>& rows, long& npoints)
+// lmax - i%2, k from j + i%2 in steps of 2 up to min(i + j, lmax).  No row is
+// empty (j + i%2 <= i + j and <= lmax), so row (i, j) is at roff[i - lmin] +
+// j - i, roff holding the running sum of the lmax - i%2 - i + 1 rows of each
+// i; host threads fill ranges of i.
+void build_lspace_rows(int lmin, int lmax, std::vector<RowDesc>& rows, long& npoints, std::vector<long>* roff_out)
 {
-    rows.clear();
-    npoints = 0;
-    for (int i = lmin; i <= lmax; ++i) {
-        int par = i & 1;
-        for (int j = i; j <= lmax - par; ++j) {
-            int kb = j + par;
-            int ke = (i + j < lmax) ? i + j : lmax;
-            if (kb > ke) continue;
-            int nk = (ke - kb) / 2 + 1;
-            rows.push_back(RowDesc{i, j, kb, nk});
-            npoints += nk;
+    const int ni = std::max(0, lmax - lmin + 1);
+    std::vector<long> roff(ni + 1, 0);
+    for (int a = 0; a < ni; ++a) {
+        const int i = lmin + a, par = i & 1;
+        roff[a + 1] = roff[a] + std::max(0, lmax - par - i + 1);
+    }
+    rows.resize(roff[ni]);
+    const int nth = std::max(1, std::min(host_threads(), ni / 64 + 1));
+    std::vector<long> pts(nth, 0);
+    run_threads(nth, [&](int t) {
+        // i ranges balanced by rows
+        const long r0 = roff[ni] * t / nth, r1 = roff[ni] * (t + 1) / nth;
+        const int a0 = (int)(std::upper_bound(roff.begin(), roff.end(), r0) - roff.begin()) - 1;
+        long np = 0;
+        for (int a = std::max(0, a0); a < ni && roff[a] < r1; ++a) {
+            if (roff[a] < r0) continue;   // this i belongs to the previous thread
+            const int i = lmin + a, par = i & 1;
+            long r = roff[a];
+            for (int j = i; j <= lmax - par; ++j, ++r) {
+                const int kb = j + par, ke = std::min(i + j, lmax);
+                const int nk = (ke - kb) / 2 + 1;
+                rows[r] = RowDesc{i, j, kb, nk};
+                np += nk;
+            }
         }
+        pts[t] = np;
+    });
+    npoints = 0;
+    for (long v : pts) npoints += v;
+    if (roff_out) roff_out->swap(roff);
+}
+
+// The same rows on the device, from the per-i offsets: one workgroup per i.
+__global__ void lspace_rows_kernel(const long* __restrict__ roff, int lmin, int lmax, RowDesc* __restrict__ rows)
+{
+    const int a = blockIdx.x, i = lmin + a, par = i & 1;
+    const long r0 = roff[a];
+    for (int j = i + threadIdx.x; j <= lmax - par; j += blockDim.x) {
+        const int kb = j + par, ke = min(i + j, lmax);
+        rows[r0 + (j - i)] = RowDesc{i, j, kb, (ke - kb) / 2 + 1};
     }
 }
 
-// Copy rows, basis and mode table of a point set to the device.
-int upload_pointset(mcmb_ctx* c, PointSet& L, const std::vector<int>& order)
+// Copy rows (unless rows_on_device: the caller has written d_rows), basis
+// and mode table of a point set to the device.
+int upload_pointset(mcmb_ctx* c, PointSet& L, const std::vector<int>& order, bool rows_on_device)
 {
     std::vector<int> modes(L.n);
     for (int m = 0; m < L.n; ++m) modes[m] = order[3 * m] | (order[3 * m + 1] << 8) | (order[3 * m + 2] << 16);
-    MCMB_TRY(L.d_rows.alloc(sizeof(RowDesc) * L.rows.size()));
+    if (!rows_on_device) MCMB_TRY(L.d_rows.alloc(sizeof(RowDesc) * L.rows.size()));
     MCMB_TRY(L.d_q.alloc(sizeof(double) * L.q.size()));
     MCMB_TRY(L.d_modes.alloc(sizeof(int) * modes.size()));
-    MCMB_HIP(hipMemcpyAsync(L.d_rows.p, L.rows.data(), sizeof(RowDesc) * L.rows.size(), hipMemcpyHostToDevice, c->stream));
+    if (!rows_on_device)
+        MCMB_HIP(hipMemcpyAsync(L.d_rows.p, L.rows.data(), sizeof(RowDesc) * L.rows.size(), hipMemcpyHostToDevice,
+                                c->stream));
     MCMB_HIP(hipMemcpyAsync(L.d_q.p, L.q.data(), sizeof(double) * L.q.size(), hipMemcpyHostToDevice, c->stream));
     MCMB_HIP(hipMemcpyAsync(L.d_modes.p, modes.data(), sizeof(int) * modes.size(), hipMemcpyHostToDevice, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
@@ -155,6 +230,9 @@ int mcmb_destroy(mcmb_ctx* c)
     c->d_partial.release();
     c->d_gamma_tmp.release();
     c->d_aux.release();
+    if (c->stage) (void)hipHostFree(c->stage);
+    for (int k = 0; k < 2; ++k)
+        if (c->stage_ev[k]) (void)hipEventDestroy(c->stage_ev[k]);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (int k = 0; k <= MAX_PHASES; ++k)
@@ -204,7 +282,20 @@ int mcmb_lspace_set(mcmb_ctx* c, int lmin, int lmax, int ldq, const double* q, c
     L.order.assign(order, order + 3 * (size_t)nmodes);
     L.q.assign(q, q + (size_t)(lmax + 1) * ldq);
     L.v.assign(v, v + (size_t)(lmax + 1));
-    build_lspace_rows(lmin, lmax, L.rows, L.npoints);
+    std::vector<long> roff;
+    build_lspace_rows(lmin, lmax, L.rows, L.npoints, &roff);
+    {
+        // the device copy of the rows expanded on the device from the per-i offsets
+        DevBuf d_roff;
+        MCMB_TRY(d_roff.alloc(sizeof(long) * roff.size()));
+        MCMB_TRY(L.d_rows.alloc(sizeof(RowDesc) * L.rows.size()));
+        MCMB_HIP(hipMemcpy(d_roff.p, roff.data(), sizeof(long) * roff.size(), hipMemcpyHostToDevice));
+        if (lmax >= lmin)
+            hipLaunchKernelGGL(lspace_rows_kernel, dim3(lmax - lmin + 1), dim3(256), 0, c->stream,
+                               d_roff.as<long>(), lmin, lmax, L.d_rows.as<RowDesc>());
+        MCMB_HIP(hipGetLastError());
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+    }
     // f(a) = prod_{k<=a}(2k-1)/(2k) in long double, a <= 3 lmax / 2 + 1, and
     // 1/((2a+1) f(a)), interleaved: ftab[2a], ftab[2a+1]
     int amax = 3 * lmax / 2 + 2;
@@ -229,7 +320,7 @@ int mcmb_lspace_set(mcmb_ctx* c, int lmin, int lmax, int ldq, const double* q, c
         MCMB_HIP(hipMemcpy(L.d_ftp.p, fp.data(), sizeof(double) * fp.size(), hipMemcpyHostToDevice));
         MCMB_HIP(hipMemcpy(L.d_vp.p, vp.data(), sizeof(double) * vp.size(), hipMemcpyHostToDevice));
     }
-    MCMB_TRY(upload_pointset(c, L, L.order));
+    MCMB_TRY(upload_pointset(c, L, L.order, true));
     L.ready = true;
     L.generation = next_generation();
     return MCMB_OK;
@@ -314,7 +405,7 @@ int mcmb_device_to_host(mcmb_ctx* c, void* host, const void* dev, size_t bytes)
 {
     if (!c || (bytes && (!host || !dev))) return fail(MCMB_ERR_ARG, "mcmb_device_to_host: null argument");
     MCMB_TRY(set_device(c));
-    if (bytes) MCMB_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+    MCMB_TRY(copy_to_host(c, host, dev, bytes));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     return MCMB_OK;
 }
@@ -330,6 +421,17 @@ int mcmb_last_phases(mcmb_ctx* c, int cap, double* ms, const char** names)
         names[k] = c->phase_names[k];
     }
     return c->nphases;
+}
+
+int mcmb_last_setup(mcmb_ctx* c, int cap, double* ms, const char** names)
+{
+    if (!c) return fail(MCMB_ERR_ARG, "mcmb_last_setup: null context");
+    if (cap < 0 || (cap > 0 && (!ms || !names))) return fail(MCMB_ERR_ARG, "mcmb_last_setup: bad buffers");
+    for (int k = 0; k < c->nsetup && k < cap; ++k) {
+        ms[k] = c->setup_ms[k];
+        names[k] = c->setup_names[k];
+    }
+    return c->nsetup;
 }
 
 int mcmb_last_basis_rank(mcmb_ctx* c, int* rank)
@@ -354,7 +456,7 @@ int mcmb_gamma_tet(mcmb_ctx* c, int precision, double* gamma_host, double* gamma
     }
     MCMB_TRY(gram_tet_launch(c, c->ls, precision, dst, 0, (int)c->ls.rows.size()));
     if (gamma_host)
-        MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
+        MCMB_TRY(copy_to_host(c, gamma_host, dst, sizeof(double) * (size_t)n * n));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     return MCMB_OK;
 }
@@ -384,7 +486,7 @@ int mcmb_gamma_tet_slice(mcmb_ctx* c, int precision, int rank, int nranks, doubl
         MCMB_TRY(gram_tet_launch(c, c->ls, precision, dst, lo, hi));
     }
     if (gamma_host)
-        MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
+        MCMB_TRY(copy_to_host(c, gamma_host, dst, sizeof(double) * (size_t)n * n));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     return MCMB_OK;
 }
@@ -419,7 +521,7 @@ int mcmb_gamma_tet_lrange(mcmb_ctx* c, int precision, int l1_lo, int l1_hi, doub
         MCMB_TRY(gram_tet_launch(c, c->ls, precision, dst, lo, hi));
     }
     if (gamma_host)
-        MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
+        MCMB_TRY(copy_to_host(c, gamma_host, dst, sizeof(double) * (size_t)n * n));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     return MCMB_OK;
 }

@@ -828,7 +828,7 @@ extern "C" int mcmb_gamma_cross_3d(mcmb_ctx* c, int rank, int nranks, double* ga
         c->last_ms = ms;
         c->last_launches = (int)(4 * ((nck + cpb - 1) / cpb));
     }
-    if (gamma_host) MCMB_HIP(hipMemcpyAsync(gamma_host, dst, gb, hipMemcpyDeviceToHost, c->stream));
+    if (gamma_host) MCMB_TRY(copy_to_host(c, gamma_host, dst, gb));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     return MCMB_OK;
 }
@@ -956,7 +956,7 @@ extern "C" int mcmb_gamma_cross_gl(mcmb_ctx* c, int spline, int npt, double* gam
         dst = c->d_gamma_tmp.as<double>();
     }
     MCMB_TRY(cross_gl_launch(c, 0, spline, npt, dst));
-    if (gamma_host) MCMB_HIP(hipMemcpy(gamma_host, dst, gb, hipMemcpyDeviceToHost));
+    if (gamma_host) MCMB_TRY(copy_to_host(c, gamma_host, dst, gb));
     return MCMB_OK;
 }
 
@@ -986,7 +986,7 @@ extern "C" int mcmb_gamma_cross_gl_pol(mcmb_ctx* c, int pol, int spline, int npt
         dst = c->d_gamma_tmp.as<double>();
     }
     MCMB_TRY(cross_gl_launch(c, pol, spline, npt, dst));
-    if (gamma_host) MCMB_HIP(hipMemcpy(gamma_host, dst, gb, hipMemcpyDeviceToHost));
+    if (gamma_host) MCMB_TRY(copy_to_host(c, gamma_host, dst, gb));
     return MCMB_OK;
 }
 

@@ -478,7 +478,7 @@ extern "C" int mcmb_gamma_gl(mcmb_ctx* c, int npt, double* gamma_host, double* g
     }
     MCMB_TRY(gram_gl_launch(c, npt, dst));
     if (gamma_host)
-        MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
+        MCMB_TRY(copy_to_host(c, gamma_host, dst, sizeof(double) * (size_t)n * n));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     return MCMB_OK;
 }
@@ -507,7 +507,7 @@ extern "C" int mcmb_gamma_gl_pol(mcmb_ctx* c, int pol, int npt, double* gamma_ho
     if (pol == MCMB_POL_TTT) MCMB_TRY(gram_gl_launch(c, npt, dst));
     else MCMB_TRY(gram_gl_pol_launch(c, pol, npt, dst));
     if (gamma_host)
-        MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
+        MCMB_TRY(copy_to_host(c, gamma_host, dst, sizeof(double) * (size_t)n * n));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     return MCMB_OK;
 }

@@ -25,9 +25,12 @@
 #include "mcmb_internal.h"
 #include "weights.h"
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
+#include <thread>
 #include <vector>
 
 namespace mcmb {
@@ -1026,7 +1029,8 @@ struct Run {
 // Host tables of one block of runs, concatenated into one int array.
 struct BlockTables {
     SepBlock b;
-    std::vector<int> buf;
+    std::unique_ptr<int[]> buf;   // the block's tables, each 16-byte aligned
+    size_t size = 0;              // ints in buf
 };
 
 template <class T> size_t append(std::vector<int>& buf, const std::vector<T>& v)
@@ -1042,7 +1046,9 @@ template <class T> size_t append(std::vector<int>& buf, const std::vector<T>& v)
 // NP basis columns, ld = padded leading dimension of U, M and Y rows; tm x tn
 // GEMM tiles; the l1 contraction's K (the block's runs) cut into up to ksplit
 // tasks, task s writing its own partial sum at s NP ldq2 (ldq2: the leading
-// dimension of its rows).
+// dimension of its rows).  The row groups are formed per run (rows of one
+// l1), so host threads take contiguous ranges of runs (balanced by rows) and
+// their groups are concatenated in run order: the same tables as one thread.
 void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1, int NP, int ld, int gsz, int tm,
                  int tn, int ksplit, BlockTables& BT)
 {
@@ -1051,57 +1057,90 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
     T.row0 = runs[b0].r0;
     T.nrows = runs[b1 - 1].r1 - T.row0;
     T.nruns = b1 - b0;
-    std::vector<int> groups, aidx2(T.nrows), aidx3(T.nruns);
-    std::vector<int4> ginfo;     // per group: l3 start, l3 count on the group's grid
-    std::vector<long> wofs;      // per group: first staged weight
+    T.jspan = 0;
+    std::vector<int> aidx3(T.nruns);
+    // per thread: its groups (row indices, gsz per group, -1 padding), their
+    // l3 start and count, and their staged-weight sizes
+    struct Part {
+        std::vector<int> groups;
+        std::vector<int4> ginfo;
+        std::vector<long> wsz;
+        int jspan = 0;
+        bool contig2 = true;
+    };
+    const int nth = std::max(1, std::min(host_threads(), T.nruns));
+    std::vector<Part> parts(nth);
+    std::vector<long> rstart(T.nruns + 1);
+    for (int k = 0; k < T.nruns; ++k) rstart[k] = runs[b0 + k].r0 - T.row0;
+    rstart[T.nruns] = T.nrows;
+    std::vector<int> cut(nth + 1);
+    for (int t = 0; t <= nth; ++t)
+        cut[t] = (int)(std::lower_bound(rstart.begin(), rstart.end(), (long)T.nrows * t / nth) - rstart.begin());
+    cut[nth] = T.nruns;
+    run_threads(nth, [&](int t) {
+        Part& P = parts[t];
+        std::vector<std::vector<int>> cls(kstep);
+        auto flush = [&](std::vector<int>& c) {
+            if (c.empty()) return;
+            int kmin = INT_MAX, kmax = INT_MIN;
+            for (int g = 0; g < gsz; ++g) {
+                P.groups.push_back(g < (int)c.size() ? c[g] : -1);
+                if (g < (int)c.size() && L.rows[c[g]].nk > 0) {
+                    const RowDesc& d = L.rows[c[g]];
+                    kmin = std::min(kmin, d.k0);
+                    kmax = std::max(kmax, d.k0 + kstep * (d.nk - 1));
+                }
+                if (g < (int)c.size()) P.jspan = std::max(P.jspan, L.rows[c[g]].j - L.rows[c[0]].j);
+            }
+            const int nkk = kmax >= kmin ? (kmax - kmin) / kstep + 1 : 0;
+            P.ginfo.push_back(make_int4(nkk ? kmin : 0, nkk, 0, 0));
+            P.wsz.push_back((long)gsz * 16 * ((nkk + 15) / 16));
+            c.clear();
+        };
+        for (int ru = b0 + cut[t]; ru < b0 + cut[t + 1]; ++ru) {
+            const Run& R = runs[ru];
+            const int j0 = L.rows[R.r0].j;
+            for (int r = R.r0; r < R.r1; ++r) {
+                const RowDesc& d = L.rows[r];
+                if (d.j != j0 + (r - R.r0)) P.contig2 = false;
+                std::vector<int>& c = cls[((d.k0 % kstep) + kstep) % kstep];
+                c.push_back(r);
+                if ((int)c.size() == gsz) flush(c);
+            }
+            for (auto& c : cls) flush(c);
+        }
+    });
+    // concatenate in run order; wofs = running sum of the weight sizes
+    std::vector<long> gofs(nth + 1, 0);
+    T.contig2 = true;
+    for (int t = 0; t < nth; ++t) {
+        gofs[t + 1] = gofs[t] + (long)parts[t].ginfo.size();
+        T.jspan = std::max(T.jspan, parts[t].jspan);
+        T.contig2 = T.contig2 && parts[t].contig2;
+    }
+    const long ngr = gofs[nth];
+    std::vector<long> wofs(ngr);
+    long wsize = 0;
+    for (int t = 0; t < nth; ++t)
+        for (size_t g = 0; g < parts[t].wsz.size(); ++g) {
+            wofs[gofs[t] + g] = wsize;
+            wsize += parts[t].wsz[g];
+        }
     std::vector<SepTask> task2(T.nruns), task3;
     std::vector<int4> tile2, tile3;
-    std::vector<std::vector<int>> cls(kstep);
-    long wsize = 0;
-    auto flush = [&](std::vector<int>& c) {
-        if (c.empty()) return;
-        int kmin = INT_MAX, kmax = INT_MIN;
-        for (int g = 0; g < gsz; ++g) {
-            groups.push_back(g < (int)c.size() ? c[g] : -1);
-            if (g < (int)c.size() && L.rows[c[g]].nk > 0) {
-                const RowDesc& d = L.rows[c[g]];
-                kmin = std::min(kmin, d.k0);
-                kmax = std::max(kmax, d.k0 + kstep * (d.nk - 1));
-            }
-            if (g < (int)c.size()) T.jspan = std::max(T.jspan, L.rows[c[g]].j - L.rows[c[0]].j);
-        }
-        const int nkk = kmax >= kmin ? (kmax - kmin) / kstep + 1 : 0;
-        ginfo.push_back(make_int4(nkk ? kmin : 0, nkk, 0, 0));
-        wofs.push_back(wsize);
-        wsize += (long)gsz * 16 * ((nkk + 15) / 16);
-        c.clear();
-    };
-    T.contig2 = T.contig3 = true;
-    for (int ru = b0; ru < b1; ++ru) {
-        const Run& R = runs[ru];
-        const int j0 = L.rows[R.r0].j;
-        for (int r = R.r0; r < R.r1; ++r) {
-            const RowDesc& d = L.rows[r];
-            aidx2[r - T.row0] = d.j;
-            if (d.j != j0 + (r - R.r0)) T.contig2 = false;
-            std::vector<int>& c = cls[((d.k0 % kstep) + kstep) % kstep];
-            c.push_back(r);
-            if ((int)c.size() == gsz) flush(c);
-        }
-        for (auto& c : cls) flush(c);
-        const int lt = ru - b0;
+    T.contig3 = true;
+    for (int lt = 0; lt < T.nruns; ++lt) {
+        const Run& R = runs[b0 + lt];
         task2[lt].boff = R.r0 - T.row0;
         task2[lt].coff = (long)lt * NP * ld;
-        task2[lt].aoff = R.r0 - T.row0;
+        // contiguous A rows: the task's first row index replaces the list offset
+        task2[lt].aoff = T.contig2 ? L.rows[R.r0].j : R.r0 - T.row0;
         task2[lt].K = R.r1 - R.r0;
         aidx3[lt] = R.i;
         if (R.i != runs[b0].i + lt) T.contig3 = false;
         for (int n0 = 0; n0 < NP; n0 += tn)
             for (int m0 = 0; m0 < NP; m0 += tm) tile2.push_back(make_int4(lt, m0, n0, 0));
     }
-    // contiguous A rows: the task's first row index replaces the list offset
-    if (T.contig2)
-        for (int lt = 0; lt < T.nruns; ++lt) task2[lt].aoff = L.rows[runs[b0 + lt].r0].j;
     const long N3 = (long)NP * (NP + 1) / 2;   // stage 3 runs over the unordered pairs {e, f}
     const long ldq2 = N3 + (N3 & 1);
     const int ns = std::max(1, ksplit);   // every block writes all partial sums (tasks of K = 0 write zeros)
@@ -1116,20 +1155,63 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
         for (long n0 = 0; n0 < N3; n0 += tn)
             for (int m0 = 0; m0 < NP; m0 += tm) tile3.push_back(make_int4(sp, m0, (int)n0, 0));
     }
-    T.ngroups = (int)(groups.size() / gsz);
+    T.ngroups = (int)ngr;
     T.nt2 = (int)tile2.size();
     T.nt3 = (int)tile3.size();
     T.wsize = wsize;
-    BT.buf.clear();
-    T.o_groups = append(BT.buf, groups);
-    T.o_ginfo = append(BT.buf, ginfo);
-    T.o_wofs = append(BT.buf, wofs);
-    T.o_aidx2 = append(BT.buf, aidx2);
-    T.o_aidx3 = append(BT.buf, aidx3);
-    T.o_task2 = append(BT.buf, task2);
-    T.o_task3 = append(BT.buf, task3);
-    T.o_tile2 = append(BT.buf, tile2);
-    T.o_tile3 = append(BT.buf, tile3);
+    // layout: each table at a 16-byte aligned offset (ints); the A row list
+    // only when the rows of a run are not contiguous in j
+    size_t off = 0;
+    auto place = [&](size_t ints) {
+        const size_t o = (off + 3) / 4 * 4;
+        off = o + ints;
+        return o;
+    };
+    constexpr size_t I4 = sizeof(int4) / sizeof(int), IL = sizeof(long) / sizeof(int),
+                     IT = sizeof(SepTask) / sizeof(int);
+    T.o_groups = place((size_t)ngr * gsz);
+    T.o_ginfo = place((size_t)ngr * I4);
+    T.o_wofs = place((size_t)ngr * IL);
+    T.o_aidx2 = place(T.contig2 ? 0 : (size_t)T.nrows);
+    T.o_aidx3 = place(aidx3.size());
+    T.o_task2 = place(task2.size() * IT);
+    T.o_task3 = place(task3.size() * IT);
+    T.o_tile2 = place(tile2.size() * I4);
+    T.o_tile3 = place(tile3.size() * I4);
+    BT.size = (off + 3) / 4 * 4;
+    BT.buf.reset(new int[BT.size]);
+    int* const buf = BT.buf.get();
+    // alignment gaps: zero (never read)
+    auto zero_gap = [&](size_t from, size_t to) {
+        if (to > from) memset(buf + from, 0, sizeof(int) * (to - from));
+    };
+    const size_t ends[9][2] = {{T.o_groups, (size_t)ngr * gsz}, {T.o_ginfo, (size_t)ngr * I4},
+                               {T.o_wofs, (size_t)ngr * IL}, {T.o_aidx2, T.contig2 ? 0 : (size_t)T.nrows},
+                               {T.o_aidx3, aidx3.size()}, {T.o_task2, task2.size() * IT},
+                               {T.o_task3, task3.size() * IT}, {T.o_tile2, tile2.size() * I4},
+                               {T.o_tile3, tile3.size() * I4}};
+    for (int k = 0; k < 9; ++k) zero_gap(ends[k][0] + ends[k][1], k + 1 < 9 ? ends[k + 1][0] : BT.size);
+    // the large tables in parallel, from each thread's part
+    run_threads(nth, [&](int t) {
+        const Part& P = parts[t];
+        if (!P.groups.empty())
+            memcpy(buf + T.o_groups + (size_t)gofs[t] * gsz, P.groups.data(), sizeof(int) * P.groups.size());
+        if (!P.ginfo.empty())
+            memcpy(buf + T.o_ginfo + (size_t)gofs[t] * I4, P.ginfo.data(), sizeof(int4) * P.ginfo.size());
+        const long g0 = gofs[t], g1 = gofs[t + 1];
+        if (g1 > g0) memcpy(buf + T.o_wofs + (size_t)g0 * IL, wofs.data() + g0, sizeof(long) * (g1 - g0));
+        if (!T.contig2)
+            for (int ru = b0 + cut[t]; ru < b0 + cut[t + 1]; ++ru)
+                for (int r = runs[ru].r0; r < runs[ru].r1; ++r) buf[T.o_aidx2 + (r - T.row0)] = L.rows[r].j;
+    });
+    auto put = [&](size_t o, const void* src, size_t bytes) {
+        if (bytes) memcpy(buf + o, src, bytes);
+    };
+    put(T.o_aidx3, aidx3.data(), sizeof(int) * aidx3.size());
+    put(T.o_task2, task2.data(), sizeof(SepTask) * task2.size());
+    put(T.o_task3, task3.data(), sizeof(SepTask) * task3.size());
+    put(T.o_tile2, tile2.data(), sizeof(int4) * tile2.size());
+    put(T.o_tile3, tile3.data(), sizeof(int4) * tile3.size());
 }
 
 // blocks for ntiles tiles in groups of GS dealt to 8 XCDs
@@ -1158,11 +1240,21 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
     P = SepPlan();
     // runs of rows with equal l1, and blocks of runs whose stage outputs fit
     // the caps (M: 16 GiB, Y: 8 GiB)
+    // run starts found by host threads over ranges of rows, in order
     std::vector<Run> runs;
-    for (int r = row_lo; r < row_hi; ++r) {
-        const int i = L.rows[r].i;
-        if (runs.empty() || runs.back().i != i) runs.push_back({r, r + 1, i});
-        else runs.back().r1 = r + 1;
+    {
+        const int nth = std::max(1, std::min(host_threads(), (row_hi - row_lo) / 4096 + 1));
+        std::vector<std::vector<int>> starts(nth);
+        run_threads(nth, [&](int t) {
+            const int r0 = row_lo + (int)((long)(row_hi - row_lo) * t / nth);
+            const int r1 = row_lo + (int)((long)(row_hi - row_lo) * (t + 1) / nth);
+            for (int r = r0; r < r1; ++r)
+                if (r == row_lo || L.rows[r].i != L.rows[r - 1].i) starts[t].push_back(r);
+        });
+        std::vector<int> st;
+        for (auto& v : starts) st.insert(st.end(), v.begin(), v.end());
+        for (size_t k = 0; k < st.size(); ++k)
+            runs.push_back({st[k], k + 1 < st.size() ? st[k + 1] : row_hi, L.rows[st[k]].i});
     }
     const long NPL = (long)NP * ld;
     long cap_rows = std::max<long>(1, (16l << 30) / ((long)ld * 8));
@@ -1186,13 +1278,13 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
     std::vector<BlockTables> tabs(blocks.size());
     for (size_t b = 0; b < blocks.size(); ++b) {
         build_block(L, runs, blocks[b].first, blocks[b].second, NP, ld, gsz, tm, tn, ksplit, tabs[b]);
-        P.max_tab = std::max(P.max_tab, tabs[b].buf.size());
+        P.max_tab = std::max(P.max_tab, tabs[b].size);
         P.max_wsize = std::max(P.max_wsize, tabs[b].b.wsize);
     }
     P.max_tab = (P.max_tab + 3) / 4 * 4;   // every block's tables 16-byte aligned
     MCMB_TRY(c->sg_tab.alloc(sizeof(int) * P.max_tab * std::max<size_t>(1, blocks.size())));
     for (size_t b = 0; b < blocks.size(); ++b) {
-        MCMB_HIP(hipMemcpy(c->sg_tab.as<int>() + b * P.max_tab, tabs[b].buf.data(), sizeof(int) * tabs[b].buf.size(),
+        MCMB_HIP(hipMemcpy(c->sg_tab.as<int>() + b * P.max_tab, tabs[b].buf.get(), sizeof(int) * tabs[b].size,
                            hipMemcpyHostToDevice));
         P.blocks.push_back(tabs[b].b);
     }
@@ -1382,6 +1474,19 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     const int ld = NP + (NP & 1);          // even leading dimension of pair rows: 16-byte aligned
     const int nl = (int)(L.q.size() / (size_t)L.ldq);
     c->nphases = 0;
+    // host wall time of each setup part and of the whole call
+    using clk = std::chrono::steady_clock;
+    const clk::time_point t_call = clk::now();
+    clk::time_point t_mark = t_call;
+    c->nsetup = 0;
+    auto setup_done = [&](const char* name) {
+        const clk::time_point t = clk::now();
+        if (c->nsetup < MAX_PHASES) {
+            c->setup_names[c->nsetup] = name;
+            c->setup_ms[c->nsetup++] = std::chrono::duration<double, std::milli>(t - t_mark).count();
+        }
+        t_mark = t;
+    };
     if (row_hi <= row_lo) {
         MCMB_HIP(hipMemsetAsync(d_gamma, 0, sizeof(double) * (size_t)n * n, c->stream));
         MCMB_HIP(hipStreamSynchronize(c->stream));
@@ -1417,6 +1522,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         }
         MCMB_TRY(separable_compress(c, L, nl, NP, ld, pxy));
         comp = S.r > 0;
+        setup_done("basis factorisation");
     }
     const int NB = comp ? S.r : NP;
     const int ldB = NB + (NB & 1);         // even leading dimension of M, Y rows
@@ -1433,6 +1539,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     bool rowptr_ready = false;
     MCMB_TRY(separable_plan(c, L, row_lo, row_hi, NB, ldB, gsz, tm, tn, ksplit, rowptr_ready));
     const SepPlan& P = c->sep;
+    setup_done("row group tables");
 
     const long NPP = (long)NP * (NP + 1) / 2;   // unordered pairs {e, f}
     const long ldp = NPP + (NPP & 1);          // even leading dimension of Ys and G2 rows
@@ -1459,6 +1566,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         c->sg_off_np = NP;
     }
 
+    setup_done("buffers");
     const double* dU;   // basis table [l][ldu] the contractions read
     if (comp) {
         MCMB_TRY(separable_ptab(c, ldu));
@@ -1682,6 +1790,11 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     MCMB_HIP(hipEventRecord(c->evp[np], c->stream));
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
+    setup_done("launch to completion");
+    if (c->nsetup < MAX_PHASES) {
+        c->setup_names[c->nsetup] = "whole call";
+        c->setup_ms[c->nsetup++] = std::chrono::duration<double, std::milli>(clk::now() - t_call).count();
+    }
     float ms = 0.f;
     MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;

@@ -357,7 +357,7 @@ extern "C" int mcmb_kuniform_gram(mcmb_ctx* c, int precision, long p0, long p1, 
         R.ready = true;
         MCMB_TRY(gram_tet_launch(c, R, precision, d_g, 0, (int)R.rows.size()));
     }
-    if (gamma_host) MCMB_HIP(hipMemcpyAsync(gamma_host, d_g, gb, hipMemcpyDeviceToHost, c->stream));
+    if (gamma_host) MCMB_TRY(copy_to_host(c, gamma_host, d_g, gb));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     return MCMB_OK;
 }

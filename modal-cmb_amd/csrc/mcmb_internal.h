@@ -1,8 +1,11 @@
 // Internal declarations shared by the HIP translation units of libmodalcmb.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <string>
+#include <thread>
 #include <vector>
 #include "modal_cmb.h"
 
@@ -199,6 +202,27 @@ int spline_weights(const std::vector<double>& x, int kind, std::vector<double>& 
 
 constexpr int MAX_PHASES = 10;
 
+// Host threads of the table builds: up to 16, never more than the hardware
+// has; MCMB_HOST_THREADS sets the count (1..64).
+inline int host_threads()
+{
+    if (const char* e = getenv("MCMB_HOST_THREADS")) {
+        const int n = atoi(e);
+        if (n >= 1 && n <= 64) return n;
+    }
+    const unsigned hw = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(16u, hw ? hw : 1u));
+}
+
+// Run f(t) for t < n on n host threads (the calling thread takes t = 0).
+template <class F> void run_threads(int n, F f)
+{
+    std::vector<std::thread> th;
+    for (int t = 1; t < n; ++t) th.emplace_back(f, t);
+    f(0);
+    for (auto& x : th) x.join();
+}
+
 }  // namespace mcmb
 
 struct mcmb_ctx {
@@ -236,20 +260,36 @@ struct mcmb_ctx {
     int nphases = 0;
     int last_rank = -1;   // basis columns of the last separable Gram's contractions (0: the pairs)
     const char* phase_names[mcmb::MAX_PHASES] = {};
+    // host wall time of the parts of the last Gram call (setup and the whole
+    // call), in call order
+    // pinned staging of large device -> host copies: two halves of
+    // STAGE_BYTES, allocated on first use
+    void* stage = nullptr;
+    hipEvent_t stage_ev[2] = {};
+    double setup_ms[mcmb::MAX_PHASES] = {};
+    const char* setup_names[mcmb::MAX_PHASES] = {};
+    int nsetup = 0;
 };
 
 namespace mcmb {
 // Build the ordered-tetrapyd row table of T(lmin, lmax) in walk order.
-void build_lspace_rows(int lmin, int lmax, std::vector<RowDesc>& rows, long& npoints);
+void build_lspace_rows(int lmin, int lmax, std::vector<RowDesc>& rows, long& npoints,
+                       std::vector<long>* roff = nullptr);
 int gram_tet_launch(mcmb_ctx* ctx, PointSet& P, int precision, double* d_gamma, int row_lo, int row_hi);
 // contiguous share [row_lo, row_hi) of rank r of nr, balanced by chunk count
 void lspace_slice(const PointSet& L, int kc, int rank, int nranks, int* row_lo, int* row_hi, double* frac);
-int upload_pointset(mcmb_ctx* ctx, PointSet& P, const std::vector<int>& order);
+int upload_pointset(mcmb_ctx* ctx, PointSet& P, const std::vector<int>& order, bool rows_on_device = false);
 int gram_gl_launch(mcmb_ctx* ctx, int npt, double* d_gamma);
 // Look up this device's scratch array of the allocation-free GL Gram (once per
 // device, with the device current).
 int gl_scratch_setup(int device);
 int set_device(mcmb_ctx* ctx);
+// Copy bytes from device memory dev (written by work on the context stream)
+// to host memory, returning when the host copy is complete.  Copies of 32 MiB
+// and more go through the context's pinned staging, 16 MiB halves: the next
+// half is copied from the device while host threads move the previous one,
+// which is 3x to 4x the rate of a pageable hipMemcpy.
+int copy_to_host(mcmb_ctx* ctx, void* host, const void* dev, size_t bytes);
 // Per-point weights of P (walk order) into ws, scaled = sqrt(w)/6 if nonzero;
 // rowptr[r] = first point of row r (built and uploaded unless rowptr_ready).
 // Weights by the point set's kind.

@@ -800,7 +800,7 @@ int mcmb_kgram(mcmb_ctx* c, int precision, double* gamma_host, double* gamma_dev
     }
     MCMB_TRY(gram_tet_launch(c, c->ks, precision, dst, 0, (int)c->ks.rows.size()));
     if (gamma_host)
-        MCMB_HIP(hipMemcpyAsync(gamma_host, dst, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost, c->stream));
+        MCMB_TRY(copy_to_host(c, gamma_host, dst, sizeof(double) * (size_t)n * n));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     return MCMB_OK;
 }

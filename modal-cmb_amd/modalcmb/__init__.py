@@ -88,6 +88,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         lib.mcmb_lspace_weights.restype = C.c_long
         lib.mcmb_last_kernel_ms.argtypes = [vp, C.POINTER(dp), C.POINTER(ip)]
         lib.mcmb_last_phases.argtypes = [vp, ip, C.POINTER(dp), C.POINTER(C.c_char_p)]
+        lib.mcmb_last_setup.argtypes = [vp, ip, C.POINTER(dp), C.POINTER(C.c_char_p)]
         lib.mcmb_last_basis_rank.argtypes = [vp, C.POINTER(ip)]
         lib.mcmb_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
         lib.mcmb_device_free.argtypes = [vp, vp]
@@ -146,7 +147,7 @@ EXPORTED_SYMBOLS = (
     "mcmb_last_error", "mcmb_status_string", "mcmb_create", "mcmb_destroy", "mcmb_sync",
     "mcmb_lspace_set", "mcmb_gamma_tet", "mcmb_gamma_gl", "mcmb_gamma_gl_noalloc", "mcmb_lspace_npoints",
     "mcmb_lspace_points", "mcmb_lspace_weights", "mcmb_orthonormalise", "mcmb_cholesky_solve",
-    "mcmb_last_kernel_ms", "mcmb_last_phases", "mcmb_last_basis_rank", "mcmb_device_alloc", "mcmb_device_free", "mcmb_device_to_host", "mcmb_device_copy", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
+    "mcmb_last_kernel_ms", "mcmb_last_phases", "mcmb_last_setup", "mcmb_last_basis_rank", "mcmb_device_alloc", "mcmb_device_free", "mcmb_device_to_host", "mcmb_device_copy", "mcmb_legacy_bind", "mcmb_legacy_status", "mcmb_lspace_info",
     "mcmb_gauss_legendre", "mcmb_gamma_tet_slice", "mcmb_gamma_tet_lrange", "mcmb_lspace_points_at", "mcmb_kspace_set", "mcmb_kspace_npoints",
     "mcmb_kgram", "mcmb_project_fwd", "mcmb_project_bwd", "mcmb_eigen_sym", "mcmb_orthonormalise_late",
     "mcmb_kuniform_set", "mcmb_kuniform_info", "mcmb_kuniform_npoints", "mcmb_kuniform_weights",
@@ -401,6 +402,16 @@ class Context:
         ms = (C.c_double * cap)()
         names = (C.c_char_p * cap)()
         k = _lib.mcmb_last_phases(self._h, cap, ms, names)
+        if k < 0:
+            _check(k)
+        return [(names[i].decode(), float(ms[i])) for i in range(min(k, cap))]
+
+    def last_setup(self) -> list[tuple[str, float]]:
+        """(name, host wall ms) per part of the last separable Gram call."""
+        cap = 16
+        ms = (C.c_double * cap)()
+        names = (C.c_char_p * cap)()
+        k = _lib.mcmb_last_setup(self._h, cap, ms, names)
         if k < 0:
             _check(k)
         return [(names[i].decode(), float(ms[i])) for i in range(min(k, cap))]

@@ -121,6 +121,30 @@ def test_lmax2500_separable_vs_gl(ctx):
     assert np.array_equal(G, G.T)
 
 
+def test_lmax2500_host_copy_staged_equals_direct(ctx):
+    # 197 MB outputs reach the host through pinned 16 MiB halves and host
+    # threads; the bytes are those of a plain hipMemcpy, also for a length
+    # that is not a multiple of the halves
+    import os
+    c, ws = lspace(ctx, "lmax2500_r30")
+    G = ctx.gamma_tet(modalcmb.FP64)
+    n = G.shape[0]
+    dev = ctx.device_alloc(8 * n * n)
+    try:
+        ctx.gamma_tet_device(modalcmb.FP64, dev)
+        os.environ["MCMB_HOST_COPY"] = "direct"
+        try:
+            G2 = ctx.gamma_tet(modalcmb.FP64)
+            part_direct = ctx.device_to_host(dev, np.empty(n * n - 7))
+        finally:
+            del os.environ["MCMB_HOST_COPY"]
+        part = ctx.device_to_host(dev, np.empty(n * n - 7))
+    finally:
+        ctx.device_free(dev)
+    assert np.array_equal(G, G2)
+    assert np.array_equal(part, part_direct) and np.array_equal(part, G.ravel()[:n * n - 7])
+
+
 # ---- 200-node GL k-grid, basis width 20 -----------------------------------
 
 def test_gl200_projection_vs_oracle_and_central_differences(ctx):

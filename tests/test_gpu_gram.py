@@ -298,6 +298,28 @@ def test_separable_phases_fused_weights(ctx, R):
     assert rel_elem(G["separable-pairs"], G["factored"]) < 1e-12
 
 
+@pytest.mark.parametrize("threads,block_rows", [("3", None), ("16", None), ("7", "3000")])
+def test_separable_tables_any_thread_count(ctx, threads, block_rows):
+    # the row group tables are built by host threads over runs of rows and
+    # concatenated in run order: any thread count gives the same tables, so
+    # the same Gram bit for bit (one block, and blocks of 3000 rows)
+    ws = W.lspace(300, 12, True)
+    env = {"MCMB_HOST_THREADS": "1"}
+    if block_rows:
+        env["MCMB_SEP_BLOCK_ROWS"] = block_rows
+    os.environ.update(env)
+    try:
+        ctx.lspace_set(2, 300, ws["q"], ws["v"], ws["order"])
+        G1 = ctx.gamma_tet(modalcmb.FP64)
+        os.environ["MCMB_HOST_THREADS"] = threads
+        ctx.lspace_set(2, 300, ws["q"], ws["v"], ws["order"])      # a new setup: tables rebuilt
+        Gt = ctx.gamma_tet(modalcmb.FP64)
+    finally:
+        for k in env:
+            os.environ.pop(k, None)
+    assert np.array_equal(G1, Gt)
+
+
 @pytest.mark.parametrize("R,compress", [(10, "1"), (24, "1"), (10, "0"), (24, "0")])
 def test_separable_multi_block(ctx, R, compress):
     # rows cut into blocks of at most 3000 (of 21,841) by MCMB_SEP_BLOCK_ROWS:

@@ -416,9 +416,11 @@ int mcmb_device_copy(mcmb_ctx* ctx, void* dst, const void* src, size_t bytes);
  * ms[] and names[] (static strings) and returns the phase count (0 if the last
  * such call recorded none).  No reference counterpart. */
 int mcmb_last_phases(mcmb_ctx* ctx, int cap, double* ms, const char** names);
-/* Host wall time (ms) of the parts of the last separable Gram call: basis
- * factorisation, row group tables, buffers (each only when the call built
- * it), launch to completion and the whole call; returns the count. */
+/* Host wall time (ms) of the parts of the last separable Gram call, in call
+ * order: basis factorisation, the row tables' parts (row runs and blocks, row
+ * groups and tables on the host, table allocation, table upload), buffers
+ * (each only when the call built it), launch to completion and the whole
+ * call; returns the count. */
 int mcmb_last_setup(mcmb_ctx* ctx, int cap, double* ms, const char** names);
 /* *rank = the basis columns the last FP64 separable Gram contracted over: r
  * when the pair products q_x q_y had numerical rank r <= NP / 2 (NP =

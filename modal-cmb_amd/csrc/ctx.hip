@@ -56,6 +56,46 @@ int set_device(mcmb_ctx* ctx)
 
 constexpr size_t STAGE_BYTES = (size_t)16 << 20;   // one half of the pinned staging
 
+// Allocate the pinned staging halves and their events on first use.
+static int stage_ready(mcmb_ctx* c)
+{
+    if (c->stage) return MCMB_OK;
+    MCMB_HIP(hipHostMalloc(&c->stage, 2 * STAGE_BYTES, hipHostMallocDefault));
+    for (int k = 0; k < 2; ++k) MCMB_HIP(hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming));
+    return MCMB_OK;
+}
+
+int copy_to_device(mcmb_ctx* c, void* dev, const void* host, size_t bytes)
+{
+    if (!bytes) return MCMB_OK;
+    const char* env = getenv("MCMB_HOST_COPY");
+    if (bytes < STAGE_BYTES / 4 || (env && !strcmp(env, "direct"))) {
+        MCMB_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        return MCMB_OK;
+    }
+    MCMB_TRY(stage_ready(c));
+    char* const stage = static_cast<char*>(c->stage);
+    const char* const src = static_cast<const char*>(host);
+    char* const dst = static_cast<char*>(dev);
+    const size_t nk = (bytes + STAGE_BYTES - 1) / STAGE_BYTES;
+    const int nth = std::min(8, host_threads());
+    for (size_t k = 0; k < nk; ++k) {
+        // half k & 1 is free once the DMA of chunk k - 2 has read it
+        if (k >= 2) MCMB_HIP(hipEventSynchronize(c->stage_ev[k & 1]));
+        const size_t off = k * STAGE_BYTES, len = std::min(STAGE_BYTES, bytes - off);
+        char* half = stage + (k & 1) * STAGE_BYTES;
+        run_threads(nth, [&](int t) {
+            const size_t a = len * t / nth, b = len * (t + 1) / nth;
+            memcpy(half + a, src + off + a, b - a);
+        });
+        MCMB_HIP(hipMemcpyAsync(dst + off, half, len, hipMemcpyHostToDevice, c->stream));
+        MCMB_HIP(hipEventRecord(c->stage_ev[k & 1], c->stream));
+    }
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    return MCMB_OK;
+}
+
 int copy_to_host(mcmb_ctx* c, void* host, const void* dev, size_t bytes)
 {
     if (!bytes) return MCMB_OK;
@@ -65,10 +105,7 @@ int copy_to_host(mcmb_ctx* c, void* host, const void* dev, size_t bytes)
         MCMB_HIP(hipStreamSynchronize(c->stream));
         return MCMB_OK;
     }
-    if (!c->stage) {
-        MCMB_HIP(hipHostMalloc(&c->stage, 2 * STAGE_BYTES, hipHostMallocDefault));
-        for (int k = 0; k < 2; ++k) MCMB_HIP(hipEventCreateWithFlags(&c->stage_ev[k], hipEventDisableTiming));
-    }
+    MCMB_TRY(stage_ready(c));
     char* const stage = static_cast<char*>(c->stage);
     const char* const src = static_cast<const char*>(dev);
     char* const dst = static_cast<char*>(host);

@@ -302,6 +302,11 @@ __global__ __launch_bounds__(512, 2) void sep_rowmom_mfma(const int* __restrict_
 // The arithmetic is the staged kernel's, so the weights are bit-identical to
 // the staged ones.  B operands are loaded 3 K-steps ahead into 4 register
 // sets.  l-space only (l3 step 2).
+// swap = 0: the rows of a group share l1 (its constant leg) and differ in
+// l2; swap = 1: they share l2 and differ in l1.  The weight is symmetric in
+// (l1, l2), so the table windows below hold for either leg; the row factor
+// and the point where permsix changes (l3 = l2) are taken with the legs in
+// (l1, l2) order, so the weights are the staged kernel's bit for bit.
 template <int CPW, int ASYM, int NW>
 __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_rowmom_lw(const int* __restrict__ groups,
                                                         const int4* __restrict__ ginfo,
@@ -309,7 +314,7 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
                                                         const double* __restrict__ f,
                                                         const double* __restrict__ v, int row_base, int ngroups,
                                                         const double* __restrict__ U, int ldu, int NP, int ld,
-                                                        double* __restrict__ M)
+                                                        double* __restrict__ M, int swap)
 {
     constexpr int G = SR_GM, RT = SR_GM / 16;
     constexpr int WPT = 8 / NW;   // weights of a chunk per thread (64 NW threads)
@@ -328,14 +333,15 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
     const int gb = ((blockIdx.x & 7) + 8 * (slot / SR_XC)) * SR_XC + slot % SR_XC;
     if (gb >= ngroups) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    // rows of a group share l1; the first is never padding
+    // rows of a group share one leg (gi: l1, or l2 when swap); the first is
+    // never padding
     const RowDesc d0 = rows[groups[(long)gb * G]];
-    const int gi = d0.i;
+    const int gi = swap ? d0.j : d0.i;
     if (tid < G) {
         const int r = groups[(long)gb * G + tid];
-        // padding rows: no points, the first row's j (keeps window indices in range)
-        const RowDesc d = r >= 0 ? rows[r] : RowDesc{0, d0.j, 0, 0};
-        gj[tid] = d.j;
+        // padding rows: no points, the first row's legs (keeps window indices in range)
+        const RowDesc d = r >= 0 ? rows[r] : RowDesc{d0.i, d0.j, 0, 0};
+        gj[tid] = swap ? d.i : d.j;   // the row's other leg
         gk0[tid] = d.k0;
         gnk[tid] = d.nk;
     }
@@ -395,10 +401,13 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
     // checked on the host), so for l3 = L0 + 2 t (t < 16) the f indices are
     // A + dj + t, A + dh + t, A + dj + 15 - t and the 1/((2a+1) f) index A +
     // dj + t, all below 64
-    int jhi = d0.j;
+    const int jlo = swap ? d0.i : d0.j;
+    int jhi = jlo;
     if constexpr (WIN)
         for (int q = 1; q < G; ++q) jhi = max(jhi, gj[q]);
-    const int jlo = d0.j, dj = (rj - jlo) >> 1, dh = (jhi - rj) >> 1;
+    const int dj = (rj - jlo) >> 1, dh = (jhi - rj) >> 1;
+    // the legs in (l1, l2) order, and the l3 at which permsix changes (l3 = l2)
+    const int li = swap ? rj : gi, lj = swap ? gi : rj;
     auto wload = [&](int c, double (&wr)[5]) {
         const int L0 = kmin + 32 * c;
         const int A1 = (jlo + L0 - gi) >> 1, A2 = (gi + L0 - jhi) >> 1, A3 = (gi + jlo - L0 - 30) >> 1,
@@ -413,9 +422,9 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
 #pragma unroll
         for (int q = 0; q < 5; ++q) Wf[b][q][lane] = wr[q];
     };
-    // row factors of the weight at l3 = j and at l3 > j (l3 >= j >= i on the row)
-    const double re = sep_row_factor(gi, rj, permsix_d(gi, rj, rj), v),
-                 rn = sep_row_factor(gi, rj, permsix_d(gi, rj, rj + 1), v);
+    // row factors of the weight at l3 = l2 and at l3 > l2 (l3 >= l2 >= l1 on the row)
+    const double re = sep_row_factor(li, lj, permsix_d(li, lj, lj), v),
+                 rn = sep_row_factor(li, lj, permsix_d(li, lj, lj + 1), v);
     constexpr int XS = 4 * NW;   // l3 (and table index) step between a thread's elements
     struct Tab {
         double f1, f2, f3, rg, tk;   // tk: the l3 factor (2 l3 + 1) v[l3]
@@ -485,9 +494,9 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
             const Tab& tx = t[y];
             double wv;
             if (ASYM) {
-                wv = lspace_weight(tx.in ? gi : 0, tx.in ? rj : 0, tx.in ? tx.k : 0, 1, f, v);
+                wv = lspace_weight(tx.in ? li : 0, tx.in ? lj : 0, tx.in ? tx.k : 0, 1, f, v);
             } else {
-                wv = sep_weight(tx.k == rj ? re : rn, tx.tk, tx.f1, tx.f2, tx.f3, tx.rg);
+                wv = sep_weight(tx.k == lj ? re : rn, tx.tk, tx.f1, tx.f2, tx.f3, tx.rg);
             }
             Wt[buf][kk0 + 2 * NW * x][g] = tx.in ? wv : 0.0;
         }
@@ -1050,7 +1059,7 @@ template <class T> size_t append(std::vector<int>& buf, const std::vector<T>& v)
 // l1), so host threads take contiguous ranges of runs (balanced by rows) and
 // their groups are concatenated in run order: the same tables as one thread.
 void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1, int NP, int ld, int gsz, int tm,
-                 int tn, int ksplit, BlockTables& BT)
+                 int tn, int ksplit, bool by_l2, BlockTables& BT)
 {
     SepBlock& T = BT.b;
     const int kstep = std::max(1, L.kstep);
@@ -1073,12 +1082,75 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
     std::vector<long> rstart(T.nruns + 1);
     for (int k = 0; k < T.nruns; ++k) rstart[k] = runs[b0 + k].r0 - T.row0;
     rstart[T.nruns] = T.nrows;
+    // contiguous rows: each run (l1 = i) holds l2 = i, i + 1, ... in order
+    std::vector<char> rc(nth, 1);
+    run_threads(nth, [&](int t) {
+        for (int ru = b0 + (int)((long)T.nruns * t / nth); ru < b0 + (int)((long)T.nruns * (t + 1) / nth); ++ru)
+            for (int r = runs[ru].r0; r < runs[ru].r1; ++r)
+                if (L.rows[r].j != L.rows[runs[ru].r0].j + (r - runs[ru].r0)) { rc[t] = 0; break; }
+    });
+    T.contig2 = true;
+    for (char x : rc) T.contig2 = T.contig2 && x;
+    T.contig3 = true;
+    for (int lt = 0; lt < T.nruns; ++lt)
+        if (runs[b0 + lt].i != runs[b0].i + lt || L.rows[runs[b0 + lt].r0].j != runs[b0 + lt].i) T.contig3 = false;
+    // groups by l2 when asked for, the rows of (i, j) are at r0(i) + j - i, and
+    // the block spans enough l1 values (256) that the groups of one l2 and
+    // parity are mostly full (a block of few l1 would give groups of 1 or 2 rows)
+    by_l2 = by_l2 && kstep == 2 && T.contig2 && T.contig3 && T.nruns >= 256;
+    T.swap = by_l2;
+    const int ia = runs[b0].i, ib = runs[b1 - 1].i;
+    // per l1 of the block: its first row and its last l2 (small tables, so the
+    // l2-major walk below does not touch the row table)
+    std::vector<int> r0_of, jl_of;
+    if (by_l2) {
+        r0_of.resize(ib - ia + 1);
+        jl_of.resize(ib - ia + 1);
+        for (int i = ia; i <= ib; ++i) {
+            r0_of[i - ia] = runs[b0 + i - ia].r0;
+            jl_of[i - ia] = L.rows[runs[b0 + i - ia].r1 - 1].j;
+        }
+    }
+    auto last_j = [&](int i) { return jl_of[i - ia]; };
+    int jtop = ia;
+    if (by_l2)
+        for (int i = ia; i <= ib; ++i) jtop = std::max(jtop, last_j(i));
+    // per thread: a range of the grouping index (runs, or l2 values balanced by rows)
     std::vector<int> cut(nth + 1);
-    for (int t = 0; t <= nth; ++t)
-        cut[t] = (int)(std::lower_bound(rstart.begin(), rstart.end(), (long)T.nrows * t / nth) - rstart.begin());
-    cut[nth] = T.nruns;
+    if (!by_l2) {
+        for (int t = 0; t <= nth; ++t)
+            cut[t] = (int)(std::lower_bound(rstart.begin(), rstart.end(), (long)T.nrows * t / nth) - rstart.begin());
+        cut[nth] = T.nruns;
+    } else {
+        // rows at l2 = j: the i in [ia, min(j, ib)] whose runs reach j (a
+        // difference array over j: +1 at i, -1 past last_j(i)); jstart = their
+        // running sum, cut at equal shares
+        std::vector<long> cnt(jtop - ia + 2, 0);
+        for (int i = ia; i <= ib; ++i) {
+            cnt[i - ia] += 1;
+            cnt[last_j(i) - ia + 1] -= 1;
+        }
+        std::vector<long> jstart(jtop - ia + 2, 0);
+        long run = 0, acc = 0;
+        for (int j = ia; j <= jtop; ++j) {
+            run += cnt[j - ia];
+            jstart[j - ia] = acc;
+            acc += run;
+        }
+        jstart[jtop - ia + 1] = acc;
+        for (int t = 0; t <= nth; ++t)
+            cut[t] = ia + (int)(std::lower_bound(jstart.begin(), jstart.end(), acc * t / nth) - jstart.begin());
+        cut[0] = ia;
+        cut[nth] = jtop + 1;
+    }
     run_threads(nth, [&](int t) {
         Part& P = parts[t];
+        // room for the part's rows and one partial group per class and grouping index
+        const long prow = by_l2 ? T.nrows / nth + 1 : rstart[cut[t + 1]] - rstart[cut[t]];
+        const long pidx = std::max(0, cut[t + 1] - cut[t]);
+        P.groups.reserve(prow + (long)gsz * kstep * pidx);
+        P.ginfo.reserve(prow / gsz + kstep * pidx + 1);
+        P.wsz.reserve(prow / gsz + kstep * pidx + 1);
         std::vector<std::vector<int>> cls(kstep);
         auto flush = [&](std::vector<int>& c) {
             if (c.empty()) return;
@@ -1090,33 +1162,48 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
                     kmin = std::min(kmin, d.k0);
                     kmax = std::max(kmax, d.k0 + kstep * (d.nk - 1));
                 }
-                if (g < (int)c.size()) P.jspan = std::max(P.jspan, L.rows[c[g]].j - L.rows[c[0]].j);
+                if (g < (int)c.size()) {
+                    // span of the leg that varies within the group
+                    const RowDesc &d = L.rows[c[g]], &d0 = L.rows[c[0]];
+                    P.jspan = std::max(P.jspan, by_l2 ? d.i - d0.i : d.j - d0.j);
+                }
             }
             const int nkk = kmax >= kmin ? (kmax - kmin) / kstep + 1 : 0;
             P.ginfo.push_back(make_int4(nkk ? kmin : 0, nkk, 0, 0));
             P.wsz.push_back((long)gsz * 16 * ((nkk + 15) / 16));
             c.clear();
         };
-        for (int ru = b0 + cut[t]; ru < b0 + cut[t + 1]; ++ru) {
-            const Run& R = runs[ru];
-            const int j0 = L.rows[R.r0].j;
-            for (int r = R.r0; r < R.r1; ++r) {
-                const RowDesc& d = L.rows[r];
-                if (d.j != j0 + (r - R.r0)) P.contig2 = false;
-                std::vector<int>& c = cls[((d.k0 % kstep) + kstep) % kstep];
-                c.push_back(r);
-                if ((int)c.size() == gsz) flush(c);
+        if (!by_l2) {
+            for (int ru = b0 + cut[t]; ru < b0 + cut[t + 1]; ++ru) {
+                const Run& R = runs[ru];
+                for (int r = R.r0; r < R.r1; ++r) {
+                    std::vector<int>& c = cls[((L.rows[r].k0 % kstep) + kstep) % kstep];
+                    c.push_back(r);
+                    if ((int)c.size() == gsz) flush(c);
+                }
+                for (auto& c : cls) flush(c);
             }
-            for (auto& c : cls) flush(c);
+        } else {
+            // l2 = j: the rows (i, j), i of one parity at a time (one l3 grid
+            // per parity), i ascending
+            for (int j = cut[t]; j < cut[t + 1]; ++j) {
+                for (int par = 0; par < 2; ++par) {
+                    std::vector<int>& c = cls[par];
+                    for (int i = ia + ((ia & 1) != par); i <= std::min(j, ib); i += 2) {
+                        if (j > jl_of[i - ia]) continue;
+                        c.push_back(r0_of[i - ia] + (j - i));
+                        if ((int)c.size() == gsz) flush(c);
+                    }
+                    flush(c);
+                }
+            }
         }
     });
     // concatenate in run order; wofs = running sum of the weight sizes
     std::vector<long> gofs(nth + 1, 0);
-    T.contig2 = true;
     for (int t = 0; t < nth; ++t) {
         gofs[t + 1] = gofs[t] + (long)parts[t].ginfo.size();
         T.jspan = std::max(T.jspan, parts[t].jspan);
-        T.contig2 = T.contig2 && parts[t].contig2;
     }
     const long ngr = gofs[nth];
     std::vector<long> wofs(ngr);
@@ -1128,7 +1215,7 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
         }
     std::vector<SepTask> task2(T.nruns), task3;
     std::vector<int4> tile2, tile3;
-    T.contig3 = true;
+    bool c3 = true;
     for (int lt = 0; lt < T.nruns; ++lt) {
         const Run& R = runs[b0 + lt];
         task2[lt].boff = R.r0 - T.row0;
@@ -1137,10 +1224,11 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
         task2[lt].aoff = T.contig2 ? L.rows[R.r0].j : R.r0 - T.row0;
         task2[lt].K = R.r1 - R.r0;
         aidx3[lt] = R.i;
-        if (R.i != runs[b0].i + lt) T.contig3 = false;
+        if (R.i != runs[b0].i + lt) c3 = false;
         for (int n0 = 0; n0 < NP; n0 += tn)
             for (int m0 = 0; m0 < NP; m0 += tm) tile2.push_back(make_int4(lt, m0, n0, 0));
     }
+    T.contig3 = c3;
     const long N3 = (long)NP * (NP + 1) / 2;   // stage 3 runs over the unordered pairs {e, f}
     const long ldq2 = N3 + (N3 & 1);
     const int ns = std::max(1, ksplit);   // every block writes all partial sums (tasks of K = 0 write zeros)
@@ -1232,11 +1320,17 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
     SepPlan& P = c->sep;
     const char* brenv = getenv("MCMB_SEP_BLOCK_ROWS");
     const long block_rows = brenv ? atol(brenv) : 0;
+    // l-space rows are grouped by l2 (less l3-range padding per group than
+    // groups sharing l1: 3.6% against 11% at lmax 2500); MCMB_SEP_GROUPS=l1
+    // keeps the groups of one l1
+    const char* genv = getenv("MCMB_SEP_GROUPS");
+    const bool by_l2 = !L.kspace && !L.uniform && L.kstep == 2 && !(genv && !strcmp(genv, "l1"));
     const bool same_set = P.ps == &L && P.gen == L.generation;
     rowptr_ready = same_set && c->sg_rp.p;
     if (same_set && P.row_lo == row_lo && P.row_hi == row_hi && P.gsz == gsz && P.nb == NP && P.tm == tm &&
-        P.ksplit == ksplit && P.block_rows == block_rows)
+        P.ksplit == ksplit && P.block_rows == block_rows && P.by_l2 == by_l2)
         return MCMB_OK;
+    setup_part(c, "before the row tables");
     P = SepPlan();
     // runs of rows with equal l1, and blocks of runs whose stage outputs fit
     // the caps (M: 16 GiB, Y: 8 GiB)
@@ -1275,19 +1369,22 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
         P.max_runs = std::max<long>(P.max_runs, b1 - b0);
         b0 = b1;
     }
+    setup_part(c, "row runs and blocks");
     std::vector<BlockTables> tabs(blocks.size());
     for (size_t b = 0; b < blocks.size(); ++b) {
-        build_block(L, runs, blocks[b].first, blocks[b].second, NP, ld, gsz, tm, tn, ksplit, tabs[b]);
+        build_block(L, runs, blocks[b].first, blocks[b].second, NP, ld, gsz, tm, tn, ksplit, by_l2, tabs[b]);
         P.max_tab = std::max(P.max_tab, tabs[b].size);
         P.max_wsize = std::max(P.max_wsize, tabs[b].b.wsize);
     }
     P.max_tab = (P.max_tab + 3) / 4 * 4;   // every block's tables 16-byte aligned
+    setup_part(c, "row groups and tables (host)");
     MCMB_TRY(c->sg_tab.alloc(sizeof(int) * P.max_tab * std::max<size_t>(1, blocks.size())));
+    setup_part(c, "table allocation");
     for (size_t b = 0; b < blocks.size(); ++b) {
-        MCMB_HIP(hipMemcpy(c->sg_tab.as<int>() + b * P.max_tab, tabs[b].buf.get(), sizeof(int) * tabs[b].size,
-                           hipMemcpyHostToDevice));
+        MCMB_TRY(copy_to_device(c, c->sg_tab.as<int>() + b * P.max_tab, tabs[b].buf.get(), sizeof(int) * tabs[b].size));
         P.blocks.push_back(tabs[b].b);
     }
+    setup_part(c, "table upload");
     P.ps = &L;
     P.gen = L.generation;
     P.row_lo = row_lo;
@@ -1297,6 +1394,7 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
     P.tm = tm;
     P.ksplit = ksplit;
     P.block_rows = block_rows;
+    P.by_l2 = by_l2;
     return MCMB_OK;
 }
 
@@ -1475,18 +1573,8 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     const int nl = (int)(L.q.size() / (size_t)L.ldq);
     c->nphases = 0;
     // host wall time of each setup part and of the whole call
-    using clk = std::chrono::steady_clock;
-    const clk::time_point t_call = clk::now();
-    clk::time_point t_mark = t_call;
-    c->nsetup = 0;
-    auto setup_done = [&](const char* name) {
-        const clk::time_point t = clk::now();
-        if (c->nsetup < MAX_PHASES) {
-            c->setup_names[c->nsetup] = name;
-            c->setup_ms[c->nsetup++] = std::chrono::duration<double, std::milli>(t - t_mark).count();
-        }
-        t_mark = t;
-    };
+    setup_begin(c);
+    auto setup_done = [&](const char* name) { setup_part(c, name); };
     if (row_hi <= row_lo) {
         MCMB_HIP(hipMemsetAsync(d_gamma, 0, sizeof(double) * (size_t)n * n, c->stream));
         MCMB_HIP(hipStreamSynchronize(c->stream));
@@ -1539,7 +1627,6 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     bool rowptr_ready = false;
     MCMB_TRY(separable_plan(c, L, row_lo, row_hi, NB, ldB, gsz, tm, tn, ksplit, rowptr_ready));
     const SepPlan& P = c->sep;
-    setup_done("row group tables");
 
     const long NPP = (long)NP * (NP + 1) / 2;   // unordered pairs {e, f}
     const long ldp = NPP + (NPP & 1);          // even leading dimension of Ys and G2 rows
@@ -1660,7 +1747,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
                        0, c->stream, tab + T.o_groups, ginfo, (const RowDesc*)L.d_rows.as<RowDesc>(),          \
                        (const double*)L.d_ftp.as<double>() + ftab_guard(L.lmax), (const double*)L.d_vp.as<double>(), \
                        T.row0,                                                                                  \
-                       T.ngroups, dU, ldu, NB, ldB, c->sg_M.as<double>())
+                       T.ngroups, dU, ldu, NB, ldB, c->sg_M.as<double>(), (int)T.swap)
 #define SEP_LW(CC, NN)                                                                                        \
     do {                                                                                                      \
         if (wkind == 0) SEP_ROWMOM_LW(CC, 0, NN);                                                             \
@@ -1791,10 +1878,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     setup_done("launch to completion");
-    if (c->nsetup < MAX_PHASES) {
-        c->setup_names[c->nsetup] = "whole call";
-        c->setup_ms[c->nsetup++] = std::chrono::duration<double, std::milli>(clk::now() - t_call).count();
-    }
+    setup_end(c);
     float ms = 0.f;
     MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;

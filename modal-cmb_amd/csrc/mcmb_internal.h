@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <cstdlib>
 #include <string>
@@ -131,7 +132,8 @@ struct SepBlock {
     size_t o_groups = 0, o_ginfo = 0, o_wofs = 0, o_aidx2 = 0, o_aidx3 = 0, o_task2 = 0, o_task3 = 0, o_tile2 = 0,
            o_tile3 = 0;
     long wsize = 0;   // doubles of the staged weights of the block's row groups
-    int jspan = 0;    // largest j span of the rows of one group
+    int jspan = 0;    // largest span of the varying leg over the rows of one group
+    bool swap = false;   // groups of rows sharing l2 (else sharing l1)
 };
 struct SepPlan {
     const void* ps = nullptr;   // point set the tables were built for
@@ -142,6 +144,7 @@ struct SepPlan {
     int tm = 0;                 // GEMM tile rows (the tile shape follows)
     int ksplit = 0;             // K splits of the l1 contraction
     long block_rows = 0;        // row cap override the blocks were cut with (0: none)
+    bool by_l2 = false;         // row groups share l2 (else l1)
     size_t max_tab = 0;         // ints per block in the table buffer
     long max_rows = 0, max_runs = 0, max_wsize = 0;
     std::vector<SepBlock> blocks;
@@ -266,10 +269,38 @@ struct mcmb_ctx {
     // STAGE_BYTES, allocated on first use
     void* stage = nullptr;
     hipEvent_t stage_ev[2] = {};
-    double setup_ms[mcmb::MAX_PHASES] = {};
-    const char* setup_names[mcmb::MAX_PHASES] = {};
+    double setup_ms[2 * mcmb::MAX_PHASES] = {};
+    const char* setup_names[2 * mcmb::MAX_PHASES] = {};
     int nsetup = 0;
+    std::chrono::steady_clock::time_point setup_call, setup_mark;
 };
+
+namespace mcmb {
+// Host wall time of the parts of a call: setup_begin at its start, then
+// setup_part(name) at the end of each part (the time since the previous mark).
+inline void setup_begin(mcmb_ctx* c)
+{
+    c->nsetup = 0;
+    c->setup_call = c->setup_mark = std::chrono::steady_clock::now();
+}
+inline void setup_part(mcmb_ctx* c, const char* name)
+{
+    const auto t = std::chrono::steady_clock::now();
+    if (c->nsetup < 2 * MAX_PHASES) {
+        c->setup_names[c->nsetup] = name;
+        c->setup_ms[c->nsetup++] = std::chrono::duration<double, std::milli>(t - c->setup_mark).count();
+    }
+    c->setup_mark = t;
+}
+inline void setup_end(mcmb_ctx* c)
+{
+    const auto t = std::chrono::steady_clock::now();
+    if (c->nsetup < 2 * MAX_PHASES) {
+        c->setup_names[c->nsetup] = "whole call";
+        c->setup_ms[c->nsetup++] = std::chrono::duration<double, std::milli>(t - c->setup_call).count();
+    }
+}
+}  // namespace mcmb
 
 namespace mcmb {
 // Build the ordered-tetrapyd row table of T(lmin, lmax) in walk order.
@@ -290,6 +321,9 @@ int set_device(mcmb_ctx* ctx);
 // half is copied from the device while host threads move the previous one,
 // which is 3x to 4x the rate of a pageable hipMemcpy.
 int copy_to_host(mcmb_ctx* ctx, void* host, const void* dev, size_t bytes);
+// The reverse, host -> device (4 MiB and more staged: host threads fill one
+// half while the other is copied to the device).
+int copy_to_device(mcmb_ctx* ctx, void* dev, const void* host, size_t bytes);
 // Per-point weights of P (walk order) into ws, scaled = sqrt(w)/6 if nonzero;
 // rowptr[r] = first point of row r (built and uploaded unless rowptr_ready).
 // Weights by the point set's kind.

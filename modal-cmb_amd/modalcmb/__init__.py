@@ -408,7 +408,7 @@ class Context:
 
     def last_setup(self) -> list[tuple[str, float]]:
         """(name, host wall ms) per part of the last separable Gram call."""
-        cap = 16
+        cap = 20
         ms = (C.c_double * cap)()
         names = (C.c_char_p * cap)()
         k = _lib.mcmb_last_setup(self._h, cap, ms, names)

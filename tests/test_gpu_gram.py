@@ -298,6 +298,33 @@ def test_separable_phases_fused_weights(ctx, R):
     assert rel_elem(G["separable-pairs"], G["factored"]) < 1e-12
 
 
+@pytest.mark.parametrize("R,compress,h2", [(12, "1", modalcmb.H2_EXACT), (12, "0", modalcmb.H2_EXACT),
+                                            (12, "1", modalcmb.H2_ASYMPTOTIC), (20, "0", modalcmb.H2_EXACT)])
+def test_separable_row_groups_by_l2(ctx, R, compress, h2):
+    # lmax 300 has 299 l1 runs, so the l3 contraction groups rows by l2 (its
+    # fused kernel with the legs' roles swapped); its weights are the staged
+    # ones bit for bit, and the Gram is that of groups sharing l1 to rounding
+    ws = W.lspace(300, R, True)
+    ctx.lspace_set(2, 300, ws["q"], ws["v"], ws["order"], h2_mode=h2)
+    os.environ["MCMB_SEP_COMPRESS"] = compress
+    try:
+        G = ctx.gamma_tet(modalcmb.FP64)
+        os.environ["MCMB_SEP_WEIGHTS"] = "staged"
+        Gs = ctx.gamma_tet(modalcmb.FP64)
+        del os.environ["MCMB_SEP_WEIGHTS"]
+        os.environ["MCMB_SEP_GROUPS"] = "l1"
+        G1 = ctx.gamma_tet(modalcmb.FP64)
+    finally:
+        for k in ("MCMB_SEP_COMPRESS", "MCMB_SEP_WEIGHTS", "MCMB_SEP_GROUPS"):
+            os.environ.pop(k, None)
+    assert np.array_equal(G, Gs)
+    assert rel_elem(G, G1) < 1e-13
+    assert np.array_equal(G, G.T)
+    if R == 12 and compress == "1":
+        geom = 1 if h2 == modalcmb.H2_ASYMPTOTIC else 0
+        assert rel_elem(G, O.gamma_tet(2, 300, ws["q"], ws["v"], ws["order"], geom=geom)) < 1e-12
+
+
 @pytest.mark.parametrize("threads,block_rows", [("3", None), ("16", None), ("7", "3000")])
 def test_separable_tables_any_thread_count(ctx, threads, block_rows):
     # the row group tables are built by host threads over runs of rows and

@@ -580,14 +580,15 @@ static int block_jacobi_device(mcmb_ctx* c, int n, const double* d_G, int normal
     struct Side {
         hipStream_t st = nullptr;
         hipEvent_t solved[2] = {nullptr, nullptr}, rotated[2] = {nullptr, nullptr};
+        // release on every exit path; a failure here has no caller to report to
         ~Side()
         {
-            if (st) hipStreamSynchronize(st);
+            if (st) (void)hipStreamSynchronize(st);
             for (int k = 0; k < 2; ++k) {
-                if (solved[k]) hipEventDestroy(solved[k]);
-                if (rotated[k]) hipEventDestroy(rotated[k]);
+                if (solved[k]) (void)hipEventDestroy(solved[k]);
+                if (rotated[k]) (void)hipEventDestroy(rotated[k]);
             }
-            if (st) hipStreamDestroy(st);
+            if (st) (void)hipStreamDestroy(st);
         }
     } side;
     MCMB_HIP(hipStreamCreate(&side.st));
@@ -681,7 +682,7 @@ extern "C" int mcmb_eigen_sym(mcmb_ctx* c, int n, const double* a, double* evals
     MCMB_TRY(set_device(c));
     DevBuf dG, dV, dE;
     MCMB_TRY(dG.alloc(sizeof(double) * (size_t)n * n));
-    MCMB_HIP(hipMemcpyAsync(dG.p, a, sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, c->stream));
+    MCMB_TRY(copy_to_device(c, dG.p, a, sizeof(double) * (size_t)n * n));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
     MCMB_TRY(eigen_device(c, n, dG.as<double>(), 0, nullptr, evals, dV, sweeps));
     if (evecs) {
@@ -690,9 +691,9 @@ extern "C" int mcmb_eigen_sym(mcmb_ctx* c, int n, const double* a, double* evals
         hipLaunchKernelGGL(jac_transpose, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream,
                            dV.as<double>(), n, dE.as<double>());
         MCMB_HIP(hipGetLastError());
-        MCMB_HIP(hipMemcpyAsync(evecs, dE.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
     }
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
+    if (evecs) MCMB_TRY(copy_to_host(c, evecs, dE.p, sizeof(double) * (size_t)n * n));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -715,7 +716,7 @@ extern "C" int mcmb_orthonormalise_late(mcmb_ctx* c, int n, const double* gamma,
     const size_t nn = (size_t)n * n;
     MCMB_TRY(dG.alloc(sizeof(double) * nn));
     MCMB_TRY(dN.alloc(sizeof(double) * n));
-    MCMB_HIP(hipMemcpyAsync(dG.p, gamma, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+    MCMB_TRY(copy_to_device(c, dG.p, gamma, sizeof(double) * nn));
     MCMB_TRY(eigen_device(c, n, dG.as<double>(), 1, dN.as<double>(), evals, dV, nullptr));
     MCMB_HIP(hipMemcpyAsync(norm, dN.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
@@ -731,9 +732,8 @@ extern "C" int mcmb_orthonormalise_late(mcmb_ctx* c, int n, const double* gamma,
                            dV.as<double>(), dEv.as<double>(), dN.as<double>(), n, r, dL.as<double>(),
                            dLi.as<double>());
         MCMB_HIP(hipGetLastError());
-        MCMB_HIP(hipMemcpyAsync(lambda, dL.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
-        MCMB_HIP(hipMemcpyAsync(lambdainv, dLi.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
-        MCMB_HIP(hipStreamSynchronize(c->stream));
+        MCMB_TRY(copy_to_host(c, lambda, dL.p, sizeof(double) * nn));
+        MCMB_TRY(copy_to_host(c, lambdainv, dLi.p, sizeof(double) * nn));
         if (rank) *rank = r;
         return MCMB_OK;
     };

@@ -3,19 +3,21 @@
 //
 //   norm_i = sqrt(G_ii),  C = D^-1 G D^-1 = L L^T  (blocked right-looking
 //   Cholesky, block NB: one-workgroup LDS factorisation of the diagonal block,
-//   row-parallel triangular solve of the panel, tiled SYRK update of the
-//   trailing lower triangle), Linv = L^-1 by blocked forward substitution.
+//   row-parallel triangular solve of the panel, SYRK update of the trailing
+//   lower triangle on FP64 MFMA), Linv = L^-1 by blocked right-looking
+//   forward substitution on the identity (each block's update of the rows
+//   below it on FP64 MFMA, over the right-hand sides that are not yet zero).
 //   ortho  = symmetrised D L          (update_ortho, fixed_data.c:3285-3295)
 //   lambda = symmetrised (D L)^-1     (update_lambda, fixed_data.c:3576-3586)
 // A non-positive pivot sets a device flag (first failing column + 1); later
 // kernels see it and return, and the host reports MCMB_ERR_NOT_SPD.
 #include "mcmb_internal.h"
+#include <algorithm>
 #include <vector>
 
 namespace mcmb {
 
 constexpr int NB = 32;     // Cholesky / substitution block
-constexpr int GT = 64;     // SYRK / GEMM output tile
 
 __global__ void la_norm_kernel(const double* __restrict__ G, int n, double* __restrict__ norm)
 {
@@ -101,92 +103,67 @@ __global__ __launch_bounds__(256) void la_trsm_panel(double* __restrict__ C, int
         if (j < nb) cr[j] = x[j];
 }
 
-// Trailing update C[i][j] -= sum_l L[i][k0+l] L[j][k0+l] on lower tiles of
-// [k1, n)^2 (k1 = k0 + nb); tile list = (I, J) with I >= J, GT x GT.
-__global__ __launch_bounds__(256) void la_syrk(double* __restrict__ C, int n, int k0, int k1,
-                                               const int* __restrict__ flag)
-{
-    __shared__ double Pi[GT][NB + 1], Pj[GT][NB + 1];
-    if (*flag) return;
-    const int nb = k1 - k0;
-    // decode lower tile index t -> (I, J), I >= J
-    const int t = blockIdx.x;
-    int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((I + 1) * (I + 2) / 2 <= t) ++I;
-    while (I * (I + 1) / 2 > t) --I;
-    const int J = t - I * (I + 1) / 2;
-    const int i0 = k1 + I * GT, j0 = k1 + J * GT;
-    for (int e = threadIdx.x; e < GT * NB; e += 256) {
-        int r = e / NB, l = e % NB;
-        Pi[r][l] = (i0 + r < n && l < nb) ? C[(size_t)(i0 + r) * n + k0 + l] : 0.0;
-        Pj[r][l] = (j0 + r < n && l < nb) ? C[(size_t)(j0 + r) * n + k0 + l] : 0.0;
-    }
-    __syncthreads();
-    const int tr = (threadIdx.x >> 4) * 4, tc = (threadIdx.x & 15) * 4;
-    double acc[4][4] = {};
-    for (int l = 0; l < nb; ++l) {
-        double a[4], b[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) { a[u] = Pi[tr + u][l]; b[u] = Pj[tc + u][l]; }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            int i = i0 + tr + u, j = j0 + tc + v;
-            if (i < n && j < n && j <= i) C[(size_t)i * n + j] -= acc[u][v];
-        }
-}
+typedef double fd4 __attribute__((ext_vector_type(4)));
 
-// X[r][rb..rb+nb) -= sum_{k < kend} Aop(rb+m, k) X[r][k], RHS-major X (ldx),
-// Aop(i,k) = A[i*n + k] (trans == 0) or A[k*n + i] (trans == 1); k range
-// [kbeg, kend).  Tile: NB rows x 64 RHS.
-__global__ __launch_bounds__(256) void la_gemm_update(const double* __restrict__ A, int n, int trans,
-                                                      double* __restrict__ X, int ldx, int nrhs, int rb,
-                                                      int kbeg, int kend)
+// C(i, j) -= sum_{k < K} A(i, k) B(j, k) over 64 x 64 tiles of rows [i0, i1)
+// and columns [j0, j1), on FP64 MFMA (v_mfma_f64_16x16x4f64), with
+// A(i, k) = A[i ai + k ak], B(j, k) = B[j bj + k bk], C(i, j) = C[i ci + j cj].
+// LOWER: only the tiles I >= J of a square range (i0 = j0, i1 = j1), and
+// only j <= i written (the SYRK of the Cholesky).  Four waves per tile, a
+// 32 x 32 quadrant each (2 x 2 blocks of 16 x 16), operands straight from
+// global memory, K in steps of 4.  Returns at once when *flag is set.
+template <bool LOWER>
+__global__ __launch_bounds__(256) void la_mfma_update(const double* __restrict__ A, long ai, long ak,
+                                                      const double* __restrict__ B, long bj, long bk,
+                                                      double* __restrict__ C, long ci, long cj, int i0, int i1,
+                                                      int j0, int j1, int K, const int* __restrict__ flag)
 {
-    __shared__ double As[NB][NB + 1];
-    __shared__ double Bs[64][NB + 1];
-    const int r0 = blockIdx.x * 64;
-    const int tm = (threadIdx.x >> 5), tr = (threadIdx.x & 31);   // 8 x 32 threads
-    double acc[4][2] = {};
-    for (int kc = kbeg; kc < kend; kc += NB) {
-        for (int e = threadIdx.x; e < NB * NB; e += 256) {
-            int m = e / NB, k = e % NB;
-            int i = rb + m, kk = kc + k;
-            double v = 0.0;
-            if (i < n && kk < kend) v = trans ? A[(size_t)kk * n + i] : A[(size_t)i * n + kk];
-            As[m][k] = v;
-        }
-        for (int e = threadIdx.x; e < 64 * NB; e += 256) {
-            int r = e / NB, k = e % NB;
-            int rr = r0 + r, kk = kc + k;
-            Bs[r][k] = (rr < nrhs && kk < kend) ? X[(size_t)rr * ldx + kk] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int k = 0; k < NB; ++k) {
-            double a[4], b[2];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) a[u] = As[tm * 4 + u][k];
-            b[0] = Bs[tr][k];
-            b[1] = Bs[tr + 32][k];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) { acc[u][0] += a[u] * b[0]; acc[u][1] += a[u] * b[1]; }
-        }
-        __syncthreads();
+    if (flag && *flag) return;
+    int I, J;
+    if (LOWER) {
+        // lower tile index t -> (I, J), I >= J
+        const int t = blockIdx.x;
+        I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((I + 1) * (I + 2) / 2 <= t) ++I;
+        while (I * (I + 1) / 2 > t) --I;
+        J = t - I * (I + 1) / 2;
+    } else {
+        const int ntj = (j1 - j0 + 63) / 64;
+        I = blockIdx.x / ntj;
+        J = blockIdx.x % ntj;
     }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int rb = i0 + 64 * I + 32 * (w >> 1), cb = j0 + 64 * J + 32 * (w & 1);
+    const int l16 = lane & 15, kq = lane >> 4;
+    fd4 acc[2][2];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int v = 0; v < 2; ++v) {
-            int i = rb + tm * 4 + u, rr = r0 + tr + 32 * v;
-            if (i < n && rr < nrhs && tm * 4 + u < NB) X[(size_t)rr * ldx + i] -= acc[u][v];
-        }
+        for (int y = 0; y < 2; ++y) acc[x][y] = (fd4){0.0, 0.0, 0.0, 0.0};
+    const int r0 = rb + l16, r1 = rb + 16 + l16, c0 = cb + l16, c1 = cb + 16 + l16;
+    for (int k0 = 0; k0 < K; k0 += 4) {
+        const int k = k0 + kq;
+        const bool kin = k < K;
+        double a[2], b[2];
+        a[0] = (kin && r0 < i1) ? A[r0 * ai + k * ak] : 0.0;
+        a[1] = (kin && r1 < i1) ? A[r1 * ai + k * ak] : 0.0;
+        b[0] = (kin && c0 < j1) ? B[c0 * bj + k * bk] : 0.0;
+        b[1] = (kin && c1 < j1) ? B[c1 * bj + k * bk] : 0.0;
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+    }
+    // element v of a block: row 4 v + lane / 16, column lane % 16
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = rb + 16 * x + 4 * v + kq, j = cb + 16 * y + l16;
+                if (i < i1 && j < j1 && (!LOWER || j <= i)) C[i * ci + j * cj] -= acc[x][y][v];
+            }
 }
 
 // Diagonal-block substitution on every RHS (one RHS per thread):
@@ -250,19 +227,31 @@ __global__ void la_outputs(const double* __restrict__ L, const double* __restric
     lambda[e] = Xinv[(size_t)b * n + a] / norm[b];
 }
 
-// Forward (L y = b) or backward (L^T y = b) substitution on nrhs RHS-major columns.
-static int la_solve(mcmb_ctx* c, const double* dL, int n, int upper, double* dX, int nrhs)
+// Forward (L y = b) or backward (L^T y = b) substitution on nrhs RHS-major
+// columns (X[r n + i]), right-looking: per block, the diagonal block's
+// substitution, then the update of the rows after it (forward) or before it
+// (backward) on FP64 MFMA.  inverse (forward only, X = I on entry): right-hand
+// side r is zero in the rows of the blocks before its own, so block b works on
+// the right-hand sides r < rb + nb only.
+static int la_solve(mcmb_ctx* c, const double* dL, int n, int upper, double* dX, int nrhs, bool inverse = false)
 {
-    const unsigned gr = (unsigned)((nrhs + 63) / 64), gt = (unsigned)((nrhs + 255) / 256);
     const int nblk = (n + NB - 1) / NB;
     for (int b = 0; b < nblk; ++b) {
         const int kb = upper ? (nblk - 1 - b) : b;
-        const int rb = kb * NB;
-        if (!upper && rb > 0)
-            hipLaunchKernelGGL(la_gemm_update, dim3(gr), dim3(256), 0, c->stream, dL, n, 0, dX, n, nrhs, rb, 0, rb);
-        if (upper && rb + NB < n)
-            hipLaunchKernelGGL(la_gemm_update, dim3(gr), dim3(256), 0, c->stream, dL, n, 1, dX, n, nrhs, rb, rb + NB, n);
-        hipLaunchKernelGGL(la_trsm_rhs, dim3(gt), dim3(256), 0, c->stream, dL, n, upper, dX, n, nrhs, rb);
+        const int rb = kb * NB, nb = std::min(NB, n - rb);
+        const int nr = inverse ? std::min(nrhs, rb + nb) : nrhs;
+        hipLaunchKernelGGL(la_trsm_rhs, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, c->stream, dL, n, upper, dX,
+                           n, nr, rb);
+        // rows [i0, i1) of the update; columns = the right-hand sides [0, nr)
+        const int i0 = upper ? 0 : rb + nb, i1 = upper ? rb : n;
+        if (i1 > i0) {
+            const unsigned tiles = (unsigned)(((i1 - i0 + 63) / 64) * ((nr + 63) / 64));
+            // forward: A(i, k) = L(i, rb + k); backward: A(i, k) = L^T(i, rb + k) = L(rb + k, i)
+            const double* A = upper ? dL + (size_t)rb * n : dL + rb;
+            hipLaunchKernelGGL((la_mfma_update<false>), dim3(tiles), dim3(256), 0, c->stream, A,
+                               upper ? 1l : (long)n, upper ? (long)n : 1l, (const double*)(dX + rb), (long)n, 1l, dX,
+                               1l, (long)n, i0, i1, 0, nr, nb, (const int*)nullptr);
+        }
     }
     MCMB_HIP(hipGetLastError());
     return MCMB_OK;
@@ -278,9 +267,11 @@ static int la_potrf(mcmb_ctx* c, double* dC, int n, int* d_flag, int* bad)
         if (k1 < n) {
             hipLaunchKernelGGL(la_trsm_panel, dim3((unsigned)((n - k1 + 255) / 256)), dim3(256), 0, c->stream, dC, n, k0,
                                d_flag);
-            const int T = (n - k1 + GT - 1) / GT;
-            hipLaunchKernelGGL(la_syrk, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, c->stream, dC, n, k0, k1,
-                               d_flag);
+            // C(i, j) -= sum_l L(i, k0 + l) L(j, k0 + l) on the trailing lower triangle
+            const int T = (n - k1 + 63) / 64;
+            hipLaunchKernelGGL((la_mfma_update<true>), dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, c->stream,
+                               (const double*)(dC + k0), (long)n, 1l, (const double*)(dC + k0), (long)n, 1l, dC, (long)n,
+                               1l, k1, n, k1, n, k1 - k0, (const int*)d_flag);
         }
     }
     MCMB_HIP(hipGetLastError());
@@ -304,7 +295,7 @@ extern "C" int mcmb_orthonormalise(mcmb_ctx* c, int n, const double* gamma, doub
     MCMB_TRY(dC.alloc(nn * sizeof(double)));
     MCMB_TRY(dN.alloc(n * sizeof(double)));
     MCMB_TRY(dF.alloc(sizeof(int)));
-    MCMB_HIP(hipMemcpyAsync(dG.p, gamma, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    MCMB_TRY(copy_to_device(c, dG.p, gamma, nn * sizeof(double)));
     hipLaunchKernelGGL(la_norm_kernel, dim3((n + 255) / 256), dim3(256), 0, c->stream, dG.as<double>(), n, dN.as<double>());
     hipLaunchKernelGGL(la_scale_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, dG.as<double>(), n,
                        dN.as<double>(), dC.as<double>());
@@ -318,17 +309,16 @@ extern "C" int mcmb_orthonormalise(mcmb_ctx* c, int n, const double* gamma, doub
     // Linv = L^-1: solve L X = I column by column (RHS-major X)
     MCMB_TRY(dX.alloc(nn * sizeof(double)));
     hipLaunchKernelGGL(la_identity, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, dX.as<double>(), n);
-    MCMB_TRY(la_solve(c, dC.as<double>(), n, 0, dX.as<double>(), n));
+    MCMB_TRY(la_solve(c, dC.as<double>(), n, 0, dX.as<double>(), n, true));
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
     MCMB_TRY(dO.alloc(nn * sizeof(double)));
     MCMB_TRY(dL.alloc(nn * sizeof(double)));
     hipLaunchKernelGGL(la_outputs, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, dC.as<double>(),
                        dX.as<double>(), dN.as<double>(), n, dO.as<double>(), dL.as<double>());
     MCMB_HIP(hipGetLastError());
-    if (ortho) MCMB_HIP(hipMemcpyAsync(ortho, dO.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (lambda) MCMB_HIP(hipMemcpyAsync(lambda, dL.p, nn * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    if (norm) MCMB_HIP(hipMemcpyAsync(norm, dN.p, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    MCMB_HIP(hipStreamSynchronize(c->stream));
+    if (ortho) MCMB_TRY(copy_to_host(c, ortho, dO.p, nn * sizeof(double)));
+    if (lambda) MCMB_TRY(copy_to_host(c, lambda, dL.p, nn * sizeof(double)));
+    if (norm) MCMB_TRY(copy_to_host(c, norm, dN.p, n * sizeof(double)));
     float ms = 0.f;
     MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
@@ -345,11 +335,10 @@ extern "C" int mcmb_cholesky_solve(mcmb_ctx* c, int n, const double* ortho, int 
     DevBuf dF, dX;
     MCMB_TRY(dF.alloc(nn * sizeof(double)));
     MCMB_TRY(dX.alloc((size_t)n * nrhs * sizeof(double)));
-    MCMB_HIP(hipMemcpyAsync(dF.p, ortho, nn * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipMemcpyAsync(dX.p, alpha, (size_t)n * nrhs * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    MCMB_TRY(copy_to_device(c, dF.p, ortho, nn * sizeof(double)));
+    MCMB_TRY(copy_to_device(c, dX.p, alpha, (size_t)n * nrhs * sizeof(double)));
     MCMB_TRY(la_solve(c, dF.as<double>(), n, 0, dX.as<double>(), nrhs));
     MCMB_TRY(la_solve(c, dF.as<double>(), n, 1, dX.as<double>(), nrhs));
-    MCMB_HIP(hipMemcpyAsync(beta, dX.p, (size_t)n * nrhs * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    MCMB_HIP(hipStreamSynchronize(c->stream));
+    MCMB_TRY(copy_to_host(c, beta, dX.p, (size_t)n * nrhs * sizeof(double)));
     return MCMB_OK;
 }

@@ -931,8 +931,8 @@ extern "C" int mcmb_gamma_post(mcmb_ctx* c, int n, const double* lambda, const d
     MCMB_TRY(dA.alloc(sizeof(double) * np));
     MCMB_TRY(dB.alloc(sizeof(double) * np));
     MCMB_TRY(dC.alloc(sizeof(double) * nn));
-    MCMB_HIP(hipMemcpyAsync(dL.p, lambda, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipMemcpyAsync(dG.p, gamma, sizeof(double) * nn, hipMemcpyHostToDevice, c->stream));
+    MCMB_TRY(copy_to_device(c, dL.p, lambda, sizeof(double) * nn));
+    MCMB_TRY(copy_to_device(c, dG.p, gamma, sizeof(double) * nn));
     const unsigned gp = (unsigned)((np + 255) / 256);
     const dim3 grid((unsigned)((n + PT - 1) / PT), (unsigned)((n + PT - 1) / PT));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
@@ -944,12 +944,12 @@ extern "C" int mcmb_gamma_post(mcmb_ctx* c, int n, const double* lambda, const d
     // Gamma'[i][j] = sum_m orthoinv[i][m] Gamma[m][j]
     hipLaunchKernelGGL(la_pad, dim3(gp), dim3(256), 0, c->stream, dC.as<double>(), n, 0, n, Kp, dA.as<double>());
     hipLaunchKernelGGL(la_pad, dim3(gp), dim3(256), 0, c->stream, dG.as<double>(), n, 0, Kp, n, dB.as<double>());
-    if (orthoinv) MCMB_HIP(hipMemcpyAsync(orthoinv, dC.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
     hipLaunchKernelGGL(proj_gemm<double>, grid, dim3(256), 0, c->stream, dA.as<double>(), dB.as<double>(), n, Kp, n,
                        n, dG.as<double>(), n);
     MCMB_HIP(hipGetLastError());
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
-    MCMB_HIP(hipMemcpyAsync(gamma_out, dG.p, sizeof(double) * nn, hipMemcpyDeviceToHost, c->stream));
+    if (orthoinv) MCMB_TRY(copy_to_host(c, orthoinv, dC.p, sizeof(double) * nn));
+    MCMB_TRY(copy_to_host(c, gamma_out, dG.p, sizeof(double) * nn));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     float ms = 0.f;
     MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));

@@ -18,7 +18,7 @@ def spd(n, seed, cond=1e4):
     return (Q * ev) @ Q.T * np.outer(d, d)
 
 
-@pytest.mark.parametrize("n", [1, 5, 31, 32, 33, 100, 257])
+@pytest.mark.parametrize("n", [1, 5, 31, 32, 33, 63, 64, 65, 100, 257, 700, 1030])
 def test_orthonormalise_matches_oracle(ctx, n):
     G = spd(n, n)
     ortho, lam, norm = ctx.orthonormalise(G)
@@ -41,17 +41,27 @@ def test_orthonormalise_not_spd_reports_pivot(ctx):
     assert "pivot 17" in str(e.value)
 
 
-def test_cholesky_solve_many_rhs(ctx):
-    n = 300
+@pytest.mark.parametrize("n,nrhs", [(300, 7), (300, 1), (611, 70), (129, 130)])
+def test_cholesky_solve_many_rhs(ctx, n, nrhs):
     G = spd(n, 9, cond=1e3)
     ortho, _, _ = ctx.orthonormalise(G)
     rng = np.random.default_rng(2)
-    C = rng.standard_normal((n, 7))
+    C = rng.standard_normal((n, nrhs))
     A = G @ C
     B = ctx.cholesky_solve(ortho, A)
     assert np.allclose(B, C, rtol=1e-9, atol=1e-9)
-    for j in range(7):
+    for j in range(0, nrhs, max(1, nrhs // 7)):
         assert np.allclose(B[:, j], O.cholesky_solve(ortho, A[:, j]), rtol=1e-10, atol=1e-10)
+
+
+def test_orthonormalise_inverse_is_exact_triangle(ctx):
+    """tril(lambda) is (D L)^-1: lower triangular with L^-1 D^-1 on it, so
+    tril(lambda) @ tril(ortho) = I to rounding, on a size with ragged tiles."""
+    n = 517
+    G = spd(n, 5, cond=1e3)
+    ortho, lam, _ = ctx.orthonormalise(G)
+    P = np.tril(lam) @ np.tril(ortho)
+    assert np.abs(P - np.eye(n)).max() < 1e-11
 
 
 def test_lmax500_gram_orthonormalisation_status(ctx):

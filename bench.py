@@ -71,10 +71,13 @@ import re
 import sys
 import time
 
-# numpy's BLAS threads spin for a while after each call and then hold the
-# cores the library's own host threads (row tables, staged copies) run on; the
-# bench's numpy work is matrix-vector sized, so one BLAS thread costs nothing
-os.environ.setdefault("OPENBLAS_NUM_THREADS", "1")
+# numpy's BLAS threads spin for a while after each call and hold the cores
+# the library's own host threads (row tables, staged copies) run on: a
+# first Gamma right after a numpy matrix product measured 79 ms instead of
+# 22 ms, with 1.2 s of CPU time spent in that window.  The bench's numpy
+# work is small, so it runs on one BLAS thread (set before numpy loads,
+# overriding an inherited thread count).
+os.environ["OPENBLAS_NUM_THREADS"] = "1"
 
 import numpy as np
 

@@ -1029,17 +1029,114 @@ bool gram_separable_supported(const PointSet& L, int precision)
     return precision == MCMB_FP64 && L.R >= 1 && L.R <= 255 && L.R * (L.R + 1) / 2 <= SR_NPMAX;
 }
 
+// Row groups sharing l2 formed on the device, one wave per (l2, parity)
+// class cls = 2 (j - ia) + par: the rows (i, j) of the block's l1 values i of
+// parity par, i ascending, whose runs reach l2 = j (j <= jl_of[i - ia]), row
+// index r0_of[i - ia] + j - i, in groups of G from group gstart[cls] on, the
+// last one padded with -1 (the host walk's order and padding).  Then per
+// group: its l3 range (ginfo), its staged-weight size (into wofs, scanned
+// later) and the span of l1 over its rows (the largest into *jspan).
+template <int G>
+__global__ __launch_bounds__(64) void sep_groups_l2(const int* __restrict__ r0_of, const int* __restrict__ jl_of,
+                                                    const int* __restrict__ gstart, int ia, int ib,
+                                                    const RowDesc* __restrict__ rows, int kstep,
+                                                    int* __restrict__ groups, int4* __restrict__ ginfo,
+                                                    long* __restrict__ wofs, int* __restrict__ jspan)
+{
+    static_assert(G == 32, "two groups per wave pass");
+    const int cls = blockIdx.x, j = ia + (cls >> 1), par = cls & 1;
+    const int g0 = gstart[cls], g1 = gstart[cls + 1];
+    if (g1 == g0) return;   // the whole workgroup returns
+    const int lane = threadIdx.x;
+    int* const out = groups + (long)g0 * G;
+    const int ilast = min(j, ib);
+    int pos = 0;
+    for (int i0 = ia + ((ia & 1) != par); i0 <= ilast; i0 += 128) {
+        const int i = i0 + 2 * lane;
+        const bool ok = i <= ilast && j <= jl_of[i - ia];
+        const unsigned long long m = __ballot(ok);
+        if (ok) out[pos + __popcll(m & ((1ull << lane) - 1))] = r0_of[i - ia] + (j - i);
+        pos += __popcll(m);
+    }
+    for (long e = pos + lane; e < (long)(g1 - g0) * G; e += 64) out[e] = -1;
+    __syncthreads();
+    const int h = lane / G, t = lane % G;
+    for (int gp = g0; gp < g1; gp += 2) {
+        const int g = gp + h;
+        int kmin = INT_MAX, kmax = INT_MIN, di = 0;
+        if (g < g1) {
+            const int r = out[(long)(g - g0) * G + t];
+            if (r >= 0) {
+                const RowDesc d = rows[r], d0 = rows[out[(long)(g - g0) * G]];
+                di = d.i - d0.i;
+                if (d.nk > 0) {
+                    kmin = d.k0;
+                    kmax = d.k0 + kstep * (d.nk - 1);
+                }
+            }
+        }
+#pragma unroll
+        for (int o = G / 2; o >= 1; o >>= 1) {
+            kmin = min(kmin, __shfl_xor(kmin, o));
+            kmax = max(kmax, __shfl_xor(kmax, o));
+            di = max(di, __shfl_xor(di, o));
+        }
+        if (g < g1 && t == 0) {
+            const int nkk = kmax >= kmin ? (kmax - kmin) / kstep + 1 : 0;
+            ginfo[g] = make_int4(nkk ? kmin : 0, nkk, 0, 0);
+            wofs[g] = (long)G * 16 * ((nkk + 15) / 16);
+            atomicMax(jspan, di);
+        }
+    }
+}
+
+// In-place exclusive prefix sum of the n group weight sizes in wofs (one
+// workgroup: a contiguous range per thread, the range sums scanned in LDS);
+// the sum of all into *total.
+__global__ __launch_bounds__(1024) void sep_wofs_scan(long* __restrict__ wofs, long n, long* __restrict__ total)
+{
+    __shared__ long part[1024];
+    const int t = threadIdx.x;
+    const long per = (n + 1023) / 1024;
+    const long a = min(n, t * per), b = min(n, a + per);
+    long s = 0;
+    for (long e = a; e < b; ++e) s += wofs[e];
+    part[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const long x = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    long base = part[t] - s;
+    for (long e = a; e < b; ++e) {
+        const long x = wofs[e];
+        wofs[e] = base;
+        base += x;
+    }
+    if (t == 1023) *total = part[1023];
+}
+
 namespace {
 
 struct Run {
     int r0, r1, i;
 };
 
-// Host tables of one block of runs, concatenated into one int array.
+// One block's tables, one int array with each table 16-byte aligned.  Built
+// on the host (buf holds all of it, host0 = 0), or with the row groups, their
+// l3 ranges and weight offsets formed on the device (dev: buf holds only the
+// small tables [host0, size); the device part needs, per l1 of the block, its
+// first row and last l2, and the first group of each (l2, parity) class).
 struct BlockTables {
     SepBlock b;
-    std::unique_ptr<int[]> buf;   // the block's tables, each 16-byte aligned
-    size_t size = 0;              // ints in buf
+    std::unique_ptr<int[]> buf;
+    size_t size = 0;    // ints of the block's tables
+    size_t host0 = 0;   // first int that buf holds
+    bool dev = false;
+    int ia = 0, ib = 0, jtop = 0;
+    std::vector<int> r0_of, jl_of, gstart;
 };
 
 template <class T> size_t append(std::vector<int>& buf, const std::vector<T>& v)
@@ -1052,14 +1149,108 @@ template <class T> size_t append(std::vector<int>& buf, const std::vector<T>& v)
     return off;
 }
 
+// The l2 and l1 contraction tasks and tiles, and the l1 of each run.
+struct SmallTables {
+    std::vector<int> aidx3;
+    std::vector<SepTask> task2, task3;
+    std::vector<int4> tile2, tile3;
+};
+
+static void small_tables(const PointSet& L, const std::vector<Run>& runs, int b0, int NP, int ld, int tm, int tn,
+                         int ksplit, SepBlock& T, SmallTables& S)
+{
+    S.aidx3.resize(T.nruns);
+    S.task2.resize(T.nruns);
+    bool c3 = true;
+    for (int lt = 0; lt < T.nruns; ++lt) {
+        const Run& R = runs[b0 + lt];
+        S.task2[lt].boff = R.r0 - T.row0;
+        S.task2[lt].coff = (long)lt * NP * ld;
+        // contiguous A rows: the task's first row index replaces the list offset
+        S.task2[lt].aoff = T.contig2 ? L.rows[R.r0].j : R.r0 - T.row0;
+        S.task2[lt].K = R.r1 - R.r0;
+        S.aidx3[lt] = R.i;
+        if (R.i != runs[b0].i + lt) c3 = false;
+        for (int n0 = 0; n0 < NP; n0 += tn)
+            for (int m0 = 0; m0 < NP; m0 += tm) S.tile2.push_back(make_int4(lt, m0, n0, 0));
+    }
+    T.contig3 = c3;
+    const long N3 = (long)NP * (NP + 1) / 2;   // stage 3 runs over the unordered pairs {e, f}
+    const long ldq2 = N3 + (N3 & 1);
+    const int ns = std::max(1, ksplit);   // every block writes all partial sums (tasks of K = 0 write zeros)
+    for (int sp = 0; sp < ns; ++sp) {
+        const int k0 = (int)((long)T.nruns * sp / ns), k1 = (int)((long)T.nruns * (sp + 1) / ns);
+        SepTask t3;
+        t3.boff = k0;
+        t3.coff = (long)sp * NP * ldq2;
+        t3.aoff = (T.contig3 ? runs[b0].i : 0) + k0;
+        t3.K = k1 - k0;
+        S.task3.push_back(t3);
+        for (long n0 = 0; n0 < N3; n0 += tn)
+            for (int m0 = 0; m0 < NP; m0 += tm) S.tile3.push_back(make_int4(sp, m0, (int)n0, 0));
+    }
+    T.nt2 = (int)S.tile2.size();
+    T.nt3 = (int)S.tile3.size();
+}
+
+constexpr size_t I4 = sizeof(int4) / sizeof(int), IL = sizeof(long) / sizeof(int), IT = sizeof(SepTask) / sizeof(int);
+
+// Offsets of the tables for ngr groups of gsz rows (the A row list only when
+// the rows of a run are not contiguous in l2); returns the ints of all.
+static size_t block_layout(SepBlock& T, long ngr, int gsz, const SmallTables& S)
+{
+    size_t off = 0;
+    auto place = [&](size_t ints) {
+        const size_t o = (off + 3) / 4 * 4;
+        off = o + ints;
+        return o;
+    };
+    T.o_groups = place((size_t)ngr * gsz);
+    T.o_ginfo = place((size_t)ngr * I4);
+    T.o_wofs = place((size_t)ngr * IL);
+    T.o_aidx2 = place(T.contig2 ? 0 : (size_t)T.nrows);
+    T.o_aidx3 = place(S.aidx3.size());
+    T.o_task2 = place(S.task2.size() * IT);
+    T.o_task3 = place(S.task3.size() * IT);
+    T.o_tile2 = place(S.tile2.size() * I4);
+    T.o_tile3 = place(S.tile3.size() * I4);
+    return (off + 3) / 4 * 4;
+}
+
+// buf (ints [host0, size) of the layout): the small tables, and zeros in the
+// alignment gaps (never read) from host0 on.
+static void fill_small(BlockTables& BT, long ngr, int gsz, const SmallTables& S)
+{
+    const SepBlock& T = BT.b;
+    int* const buf = BT.buf.get() - BT.host0;   // indexed by layout offset, used from host0 on
+    const size_t ends[9][2] = {{T.o_groups, (size_t)ngr * gsz}, {T.o_ginfo, (size_t)ngr * I4},
+                               {T.o_wofs, (size_t)ngr * IL}, {T.o_aidx2, T.contig2 ? 0 : (size_t)T.nrows},
+                               {T.o_aidx3, S.aidx3.size()}, {T.o_task2, S.task2.size() * IT},
+                               {T.o_task3, S.task3.size() * IT}, {T.o_tile2, S.tile2.size() * I4},
+                               {T.o_tile3, S.tile3.size() * I4}};
+    for (int k = 0; k < 9; ++k) {
+        const size_t from = std::max(BT.host0, ends[k][0] + ends[k][1]), to = k + 1 < 9 ? ends[k + 1][0] : BT.size;
+        if (to > from) memset(buf + from, 0, sizeof(int) * (to - from));
+    }
+    auto put = [&](size_t o, const void* src, size_t bytes) {
+        if (bytes) memcpy(buf + o, src, bytes);
+    };
+    put(T.o_aidx3, S.aidx3.data(), sizeof(int) * S.aidx3.size());
+    put(T.o_task2, S.task2.data(), sizeof(SepTask) * S.task2.size());
+    put(T.o_task3, S.task3.data(), sizeof(SepTask) * S.task3.size());
+    put(T.o_tile2, S.tile2.data(), sizeof(int4) * S.tile2.size());
+    put(T.o_tile3, S.tile3.data(), sizeof(int4) * S.tile3.size());
+}
+
 // NP basis columns, ld = padded leading dimension of U, M and Y rows; tm x tn
 // GEMM tiles; the l1 contraction's K (the block's runs) cut into up to ksplit
 // tasks, task s writing its own partial sum at s NP ldq2 (ldq2: the leading
-// dimension of its rows).  The row groups are formed per run (rows of one
-// l1), so host threads take contiguous ranges of runs (balanced by rows) and
-// their groups are concatenated in run order: the same tables as one thread.
+// dimension of its rows).  Host threads form the row groups over ranges of
+// the grouping index (runs, or l2 values balanced by rows), concatenated in
+// order: the same tables as one thread.  dev_groups: groups sharing l2 are
+// left to the device (sep_groups_l2), which forms the same ones.
 void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1, int NP, int ld, int gsz, int tm,
-                 int tn, int ksplit, bool by_l2, BlockTables& BT)
+                 int tn, int ksplit, bool by_l2, bool dev_groups, BlockTables& BT)
 {
     SepBlock& T = BT.b;
     const int kstep = std::max(1, L.kstep);
@@ -1067,7 +1258,6 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
     T.nrows = runs[b1 - 1].r1 - T.row0;
     T.nruns = b1 - b0;
     T.jspan = 0;
-    std::vector<int> aidx3(T.nruns);
     // per thread: its groups (row indices, gsz per group, -1 padding), their
     // l3 start and count, and their staged-weight sizes
     struct Part {
@@ -1075,10 +1265,8 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
         std::vector<int4> ginfo;
         std::vector<long> wsz;
         int jspan = 0;
-        bool contig2 = true;
     };
     const int nth = std::max(1, std::min(host_threads(), T.nruns));
-    std::vector<Part> parts(nth);
     std::vector<long> rstart(T.nruns + 1);
     for (int k = 0; k < T.nruns; ++k) rstart[k] = runs[b0 + k].r0 - T.row0;
     rstart[T.nruns] = T.nrows;
@@ -1103,18 +1291,50 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
     // per l1 of the block: its first row and its last l2 (small tables, so the
     // l2-major walk below does not touch the row table)
     std::vector<int> r0_of, jl_of;
+    int jtop = ia;
     if (by_l2) {
         r0_of.resize(ib - ia + 1);
         jl_of.resize(ib - ia + 1);
         for (int i = ia; i <= ib; ++i) {
             r0_of[i - ia] = runs[b0 + i - ia].r0;
             jl_of[i - ia] = L.rows[runs[b0 + i - ia].r1 - 1].j;
+            jtop = std::max(jtop, jl_of[i - ia]);
         }
     }
+    SmallTables S;
+    small_tables(L, runs, b0, NP, ld, tm, tn, ksplit, T, S);
+    if (by_l2 && dev_groups && gsz == 32) {
+        // group counts per (l2, parity) class: rows at l2 = j of the l1 values
+        // i of that parity with i <= j <= jl_of(i) (difference arrays over j)
+        std::vector<int> d[2];
+        for (auto& x : d) x.assign(jtop - ia + 2, 0);
+        for (int i = ia; i <= ib; ++i) {
+            d[i & 1][i - ia] += 1;
+            d[i & 1][jl_of[i - ia] - ia + 1] -= 1;
+        }
+        BT.gstart.assign(2 * (jtop - ia + 1) + 1, 0);
+        int c[2] = {0, 0};
+        for (int j = ia; j <= jtop; ++j)
+            for (int par = 0; par < 2; ++par) {
+                c[par] += d[par][j - ia];
+                const int cls = 2 * (j - ia) + par;
+                BT.gstart[cls + 1] = BT.gstart[cls] + (c[par] + gsz - 1) / gsz;
+            }
+        const long ngr = BT.gstart.back();
+        T.ngroups = (int)ngr;
+        BT.size = block_layout(T, ngr, gsz, S);
+        BT.dev = true;
+        BT.host0 = T.o_aidx2;
+        BT.ia = ia;
+        BT.ib = ib;
+        BT.jtop = jtop;
+        BT.r0_of = std::move(r0_of);
+        BT.jl_of = std::move(jl_of);
+        BT.buf.reset(new int[BT.size - BT.host0]);
+        fill_small(BT, ngr, gsz, S);
+        return;   // jspan, wsize: from the device
+    }
     auto last_j = [&](int i) { return jl_of[i - ia]; };
-    int jtop = ia;
-    if (by_l2)
-        for (int i = ia; i <= ib; ++i) jtop = std::max(jtop, last_j(i));
     // per thread: a range of the grouping index (runs, or l2 values balanced by rows)
     std::vector<int> cut(nth + 1);
     if (!by_l2) {
@@ -1143,6 +1363,7 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
         cut[0] = ia;
         cut[nth] = jtop + 1;
     }
+    std::vector<Part> parts(nth);
     run_threads(nth, [&](int t) {
         Part& P = parts[t];
         // room for the part's rows and one partial group per class and grouping index
@@ -1199,7 +1420,7 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
             }
         }
     });
-    // concatenate in run order; wofs = running sum of the weight sizes
+    // concatenate in order; wofs = running sum of the weight sizes
     std::vector<long> gofs(nth + 1, 0);
     for (int t = 0; t < nth; ++t) {
         gofs[t + 1] = gofs[t] + (long)parts[t].ginfo.size();
@@ -1213,72 +1434,13 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
             wofs[gofs[t] + g] = wsize;
             wsize += parts[t].wsz[g];
         }
-    std::vector<SepTask> task2(T.nruns), task3;
-    std::vector<int4> tile2, tile3;
-    bool c3 = true;
-    for (int lt = 0; lt < T.nruns; ++lt) {
-        const Run& R = runs[b0 + lt];
-        task2[lt].boff = R.r0 - T.row0;
-        task2[lt].coff = (long)lt * NP * ld;
-        // contiguous A rows: the task's first row index replaces the list offset
-        task2[lt].aoff = T.contig2 ? L.rows[R.r0].j : R.r0 - T.row0;
-        task2[lt].K = R.r1 - R.r0;
-        aidx3[lt] = R.i;
-        if (R.i != runs[b0].i + lt) c3 = false;
-        for (int n0 = 0; n0 < NP; n0 += tn)
-            for (int m0 = 0; m0 < NP; m0 += tm) tile2.push_back(make_int4(lt, m0, n0, 0));
-    }
-    T.contig3 = c3;
-    const long N3 = (long)NP * (NP + 1) / 2;   // stage 3 runs over the unordered pairs {e, f}
-    const long ldq2 = N3 + (N3 & 1);
-    const int ns = std::max(1, ksplit);   // every block writes all partial sums (tasks of K = 0 write zeros)
-    for (int sp = 0; sp < ns; ++sp) {
-        const int k0 = (int)((long)T.nruns * sp / ns), k1 = (int)((long)T.nruns * (sp + 1) / ns);
-        SepTask t3;
-        t3.boff = k0;
-        t3.coff = (long)sp * NP * ldq2;
-        t3.aoff = (T.contig3 ? runs[b0].i : 0) + k0;
-        t3.K = k1 - k0;
-        task3.push_back(t3);
-        for (long n0 = 0; n0 < N3; n0 += tn)
-            for (int m0 = 0; m0 < NP; m0 += tm) tile3.push_back(make_int4(sp, m0, (int)n0, 0));
-    }
     T.ngroups = (int)ngr;
-    T.nt2 = (int)tile2.size();
-    T.nt3 = (int)tile3.size();
     T.wsize = wsize;
-    // layout: each table at a 16-byte aligned offset (ints); the A row list
-    // only when the rows of a run are not contiguous in j
-    size_t off = 0;
-    auto place = [&](size_t ints) {
-        const size_t o = (off + 3) / 4 * 4;
-        off = o + ints;
-        return o;
-    };
-    constexpr size_t I4 = sizeof(int4) / sizeof(int), IL = sizeof(long) / sizeof(int),
-                     IT = sizeof(SepTask) / sizeof(int);
-    T.o_groups = place((size_t)ngr * gsz);
-    T.o_ginfo = place((size_t)ngr * I4);
-    T.o_wofs = place((size_t)ngr * IL);
-    T.o_aidx2 = place(T.contig2 ? 0 : (size_t)T.nrows);
-    T.o_aidx3 = place(aidx3.size());
-    T.o_task2 = place(task2.size() * IT);
-    T.o_task3 = place(task3.size() * IT);
-    T.o_tile2 = place(tile2.size() * I4);
-    T.o_tile3 = place(tile3.size() * I4);
-    BT.size = (off + 3) / 4 * 4;
+    BT.size = block_layout(T, ngr, gsz, S);
+    BT.host0 = 0;
     BT.buf.reset(new int[BT.size]);
     int* const buf = BT.buf.get();
-    // alignment gaps: zero (never read)
-    auto zero_gap = [&](size_t from, size_t to) {
-        if (to > from) memset(buf + from, 0, sizeof(int) * (to - from));
-    };
-    const size_t ends[9][2] = {{T.o_groups, (size_t)ngr * gsz}, {T.o_ginfo, (size_t)ngr * I4},
-                               {T.o_wofs, (size_t)ngr * IL}, {T.o_aidx2, T.contig2 ? 0 : (size_t)T.nrows},
-                               {T.o_aidx3, aidx3.size()}, {T.o_task2, task2.size() * IT},
-                               {T.o_task3, task3.size() * IT}, {T.o_tile2, tile2.size() * I4},
-                               {T.o_tile3, tile3.size() * I4}};
-    for (int k = 0; k < 9; ++k) zero_gap(ends[k][0] + ends[k][1], k + 1 < 9 ? ends[k + 1][0] : BT.size);
+    fill_small(BT, ngr, gsz, S);
     // the large tables in parallel, from each thread's part
     run_threads(nth, [&](int t) {
         const Part& P = parts[t];
@@ -1292,14 +1454,6 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
             for (int ru = b0 + cut[t]; ru < b0 + cut[t + 1]; ++ru)
                 for (int r = runs[ru].r0; r < runs[ru].r1; ++r) buf[T.o_aidx2 + (r - T.row0)] = L.rows[r].j;
     });
-    auto put = [&](size_t o, const void* src, size_t bytes) {
-        if (bytes) memcpy(buf + o, src, bytes);
-    };
-    put(T.o_aidx3, aidx3.data(), sizeof(int) * aidx3.size());
-    put(T.o_task2, task2.data(), sizeof(SepTask) * task2.size());
-    put(T.o_task3, task3.data(), sizeof(SepTask) * task3.size());
-    put(T.o_tile2, tile2.data(), sizeof(int4) * tile2.size());
-    put(T.o_tile3, tile3.data(), sizeof(int4) * tile3.size());
 }
 
 // blocks for ntiles tiles in groups of GS dealt to 8 XCDs
@@ -1370,21 +1524,69 @@ static int separable_plan(mcmb_ctx* c, PointSet& L, int row_lo, int row_hi, int 
         b0 = b1;
     }
     setup_part(c, "row runs and blocks");
+    // row groups sharing l2 are formed on the device unless MCMB_SEP_TABLES=host
+    // (the host walk, same tables)
+    const char* tenv = getenv("MCMB_SEP_TABLES");
+    const bool dev_groups = !(tenv && !strcmp(tenv, "host"));
     std::vector<BlockTables> tabs(blocks.size());
     for (size_t b = 0; b < blocks.size(); ++b) {
-        build_block(L, runs, blocks[b].first, blocks[b].second, NP, ld, gsz, tm, tn, ksplit, by_l2, tabs[b]);
+        build_block(L, runs, blocks[b].first, blocks[b].second, NP, ld, gsz, tm, tn, ksplit, by_l2, dev_groups, tabs[b]);
         P.max_tab = std::max(P.max_tab, tabs[b].size);
-        P.max_wsize = std::max(P.max_wsize, tabs[b].b.wsize);
     }
     P.max_tab = (P.max_tab + 3) / 4 * 4;   // every block's tables 16-byte aligned
     setup_part(c, "row groups and tables (host)");
     MCMB_TRY(c->sg_tab.alloc(sizeof(int) * P.max_tab * std::max<size_t>(1, blocks.size())));
     setup_part(c, "table allocation");
+    // device-formed blocks: inputs (per l1 first row and last l2, class
+    // starts) and results (jspan, weight total) in one buffer per block
+    std::vector<DevBuf> aux(blocks.size());
+    bool any_dev = false;
     for (size_t b = 0; b < blocks.size(); ++b) {
-        MCMB_TRY(copy_to_device(c, c->sg_tab.as<int>() + b * P.max_tab, tabs[b].buf.get(), sizeof(int) * tabs[b].size));
+        BlockTables& BT = tabs[b];
+        int* const dst = c->sg_tab.as<int>() + b * P.max_tab;
+        MCMB_TRY(copy_to_device(c, dst + BT.host0, BT.buf.get(), sizeof(int) * (BT.size - BT.host0)));
+        if (!BT.dev) continue;
+        any_dev = true;
+        const size_t nl1 = BT.r0_of.size(), ncl = BT.gstart.size();
+        // [jspan, pad, total (long)] then r0_of, jl_of, gstart
+        std::vector<int> in(4 + 2 * nl1 + ncl, 0);
+        std::copy(BT.r0_of.begin(), BT.r0_of.end(), in.begin() + 4);
+        std::copy(BT.jl_of.begin(), BT.jl_of.end(), in.begin() + 4 + nl1);
+        std::copy(BT.gstart.begin(), BT.gstart.end(), in.begin() + 4 + 2 * nl1);
+        MCMB_TRY(aux[b].alloc(sizeof(int) * in.size()));
+        MCMB_HIP(hipMemcpyAsync(aux[b].p, in.data(), sizeof(int) * in.size(), hipMemcpyHostToDevice, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));   // in is pageable and goes out of scope
+        // alignment gaps of the device part: zero
+        MCMB_HIP(hipMemsetAsync(dst, 0, sizeof(int) * BT.host0, c->stream));
+        const SepBlock& T = BT.b;
+        int* const ax = aux[b].as<int>();
+        const unsigned ncls = (unsigned)(ncl - 1);
+        MCMB_TRY(launch_guard("row groups", ncls, 64));
+        hipLaunchKernelGGL((sep_groups_l2<32>), dim3(ncls), dim3(64), 0, c->stream, (const int*)(ax + 4),
+                           (const int*)(ax + 4 + nl1), (const int*)(ax + 4 + 2 * nl1), BT.ia, BT.ib,
+                           (const RowDesc*)L.d_rows.as<RowDesc>(), L.kstep, dst + T.o_groups,
+                           reinterpret_cast<int4*>(dst + T.o_ginfo), reinterpret_cast<long*>(dst + T.o_wofs), ax);
+        hipLaunchKernelGGL(sep_wofs_scan, dim3(1), dim3(1024), 0, c->stream, reinterpret_cast<long*>(dst + T.o_wofs),
+                           (long)T.ngroups, reinterpret_cast<long*>(ax + 2));
+        MCMB_HIP(hipGetLastError());
+    }
+    if (any_dev) {
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        for (size_t b = 0; b < blocks.size(); ++b) {
+            if (!tabs[b].dev) continue;
+            int res[4];
+            MCMB_HIP(hipMemcpy(res, aux[b].p, sizeof(res), hipMemcpyDeviceToHost));
+            tabs[b].b.jspan = res[0];
+            long tot;
+            memcpy(&tot, res + 2, sizeof(long));
+            tabs[b].b.wsize = tot;
+        }
+    }
+    for (size_t b = 0; b < blocks.size(); ++b) {
+        P.max_wsize = std::max(P.max_wsize, tabs[b].b.wsize);
         P.blocks.push_back(tabs[b].b);
     }
-    setup_part(c, "table upload");
+    setup_part(c, any_dev ? "row groups (device) and table upload" : "table upload");
     P.ps = &L;
     P.gen = L.generation;
     P.row_lo = row_lo;

@@ -331,7 +331,7 @@ def test_separable_tables_any_thread_count(ctx, threads, block_rows):
     # concatenated in run order: any thread count gives the same tables, so
     # the same Gram bit for bit (one block, and blocks of 3000 rows)
     ws = W.lspace(300, 12, True)
-    env = {"MCMB_HOST_THREADS": "1"}
+    env = {"MCMB_HOST_THREADS": "1", "MCMB_SEP_TABLES": "host"}
     if block_rows:
         env["MCMB_SEP_BLOCK_ROWS"] = block_rows
     os.environ.update(env)
@@ -471,3 +471,32 @@ def test_separable_rank_needs_a_gap(ctx, kind, rank):
     ref = O.gamma_tet(2, L, q, ws["v"], ws["order"])
     assert rel_elem(G["separable"], ref) < 1e-12
     assert rel_elem(G["separable-pairs"], ref) < 1e-12
+
+
+@pytest.mark.parametrize("lmax,R,block_rows,weights", [(300, 12, None, None), (300, 12, None, "staged"),
+                                                       (2500, 30, None, None), (2500, 30, "1500000", None),
+                                                       (2500, 30, None, "staged")])
+def test_separable_device_row_groups_equal_host(ctx, lmax, R, block_rows, weights):
+    # row groups sharing l2 formed on the device (one wave per l2 and parity)
+    # are the host walk's groups, l3 ranges and weight offsets: the same Gram
+    # bit for bit, with fused weights and with staged ones (which read the
+    # offsets), in one block and in several
+    ws = W.lspace(lmax, R, True)
+    env = {}
+    if block_rows:
+        env["MCMB_SEP_BLOCK_ROWS"] = block_rows
+    if weights:
+        env["MCMB_SEP_WEIGHTS"] = weights
+    os.environ.update(env)
+    try:
+        ctx.lspace_set(2, lmax, ws["q"], ws["v"], ws["order"])
+        Gd = ctx.gamma_tet(modalcmb.FP64)
+        os.environ["MCMB_SEP_TABLES"] = "host"
+        ctx.lspace_set(2, lmax, ws["q"], ws["v"], ws["order"])      # a new setup: tables rebuilt on the host
+        Gh = ctx.gamma_tet(modalcmb.FP64)
+    finally:
+        for k in list(env) + ["MCMB_SEP_TABLES"]:
+            os.environ.pop(k, None)
+    assert np.array_equal(Gd, Gh)
+    names = [n for n, _ in ctx.last_setup()]
+    assert "row groups and tables (host)" in names

@@ -267,6 +267,7 @@ int mcmb_destroy(mcmb_ctx* c)
     c->d_partial.release();
     c->d_gamma_tmp.release();
     c->d_aux.release();
+    tridiag_release(c);
     if (c->stage) (void)hipHostFree(c->stage);
     for (int k = 0; k < 2; ++k)
         if (c->stage_ev[k]) (void)hipEventDestroy(c->stage_ev[k]);

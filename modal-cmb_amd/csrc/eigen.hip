@@ -663,13 +663,18 @@ static int block_jacobi_device(mcmb_ctx* c, int n, const double* d_G, int normal
     return MCMB_OK;
 }
 
-// Eigen-decomposition: block Jacobi, or the scalar Jacobi when MCMB_EIGEN=scalar.
+// Eigen-decomposition: Householder tridiagonalisation (sweeps 0) up to its
+// size limit, else block Jacobi; MCMB_EIGEN=block or scalar selects the block
+// or the scalar Jacobi.
 static int eigen_device(mcmb_ctx* c, int n, const double* d_G, int normalise, double* d_norm, double* evals,
                         DevBuf& d_V, int* sweeps_out)
 {
     const char* e = getenv("MCMB_EIGEN");
     if (e && !strcmp(e, "scalar")) return jacobi_device(c, n, d_G, normalise, d_norm, evals, d_V, sweeps_out);
-    return block_jacobi_device(c, n, d_G, normalise, d_norm, evals, d_V, sweeps_out);
+    if ((e && !strcmp(e, "block")) || !tridiag_supported(n))
+        return block_jacobi_device(c, n, d_G, normalise, d_norm, evals, d_V, sweeps_out);
+    if (sweeps_out) *sweeps_out = 0;
+    return tridiag_eigen(c, n, d_G, normalise, d_norm, evals, d_V);
 }
 
 }  // namespace mcmb

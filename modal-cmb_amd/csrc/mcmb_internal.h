@@ -273,6 +273,9 @@ struct mcmb_ctx {
     const char* setup_names[2 * mcmb::MAX_PHASES] = {};
     int nsetup = 0;
     std::chrono::steady_clock::time_point setup_call, setup_mark;
+    // rocBLAS handle of the tridiagonal eigensolver's divide and conquer
+    // step (rocsolver_dstedc), created on first use
+    void* rb_handle = nullptr;
 };
 
 namespace mcmb {
@@ -324,6 +327,13 @@ int copy_to_host(mcmb_ctx* ctx, void* host, const void* dev, size_t bytes);
 // The reverse, host -> device (4 MiB and more staged: host threads fill one
 // half while the other is copied to the device).
 int copy_to_device(mcmb_ctx* ctx, void* dev, const void* host, size_t bytes);
+// Symmetric eigen-decomposition by Householder tridiagonalisation (n up to
+// 8000): evals (host) descending, d_V rows the sign-fixed eigenvectors; with
+// normalise, of D^-1 G D^-1 (d_norm[i] = sqrt G_ii).  tridiag_release frees
+// the context's rocBLAS handle.
+bool tridiag_supported(int n);
+int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* d_norm, double* evals, DevBuf& d_V);
+void tridiag_release(mcmb_ctx* c);
 // Per-point weights of P (walk order) into ws, scaled = sqrt(w)/6 if nonzero;
 // rowptr[r] = first point of row r (built and uploaded unless rowptr_ready).
 // Weights by the point set's kind.

@@ -1,0 +1,702 @@
+// FP64 symmetric eigen-decomposition by Householder tridiagonalisation on
+// the GPU (the eigen-decomposition of ortho_late_mpi.c:768-776 and
+// ortho_prim_mpi.c:223-224, gsl_eigen_symmv + sort descending).
+//
+//   A = Q T Q^T (T tridiagonal, Q = H_0 ... H_{n-2}, H_j = I - tau_j v_j v_j^T)
+//   T = Z diag(lambda) Z^T   (rocSOLVER dstedc, divide and conquer)
+//   eigenvectors X = Q Z, applied as blocks of reflectors.
+//
+// Tridiagonalisation (the lower form of LAPACK dsytrd / dlatrd, in panels of
+// TRD_NB columns).  A is kept full and symmetric, row-major; row j stands for
+// column j.  Column j = k0 + i of a panel takes two kernels:
+//   trd_row   x = A[j][j..n) - sum_{p<i} (V_p[c] W_p[j] + W_p[c] V_p[j])
+//             (the panel's earlier reflectors, not yet in A), d_j = x_j,
+//             the reflector of x_{j+1..n): beta = -sign(x_{j+1}) |x|,
+//             tau = (beta - x_{j+1}) / beta, v = x / (x_{j+1} - beta) with
+//             v_{j+1} = 1 (tau = 0 when x_{j+2..n) = 0), e_j = beta, and the
+//             dots V_p^T v, W_p^T v the next kernel needs;
+//   trd_symv  w = tau (A v - V (W^T v) - W (V^T v)) over rows j+1.., then
+//             w += -tau/2 (w^T v) v (kept as alpha2 until the panel ends).
+// Each ends with a reduction over its workgroups: every workgroup writes its
+// partial sums, and the last to finish (an atomic ticket after a device-scope
+// fence) adds them in workgroup order, so results do not depend on timing.
+// No workgroup waits for another.  At the end of a panel the trailing matrix
+// takes A -= V W^T + W V^T on FP64 MFMA (lower tiles, mirrored, so A stays
+// exactly symmetric).
+//
+// Back-transformation: per panel, H_k0 ... H_{k0+nb-1} = I - V T V^T (T upper
+// triangular: T_ii = tau_i, T[0:i, i] = -tau_i T[0:i, 0:i] V^T v_i, LAPACK
+// dlarft), and with the eigenvectors as rows E (row k = z_k^T), X^T =
+// Z^T H_{n-2} ... H_0, so panels from the last to the first apply
+// E <- E - (E V) (T^T V^T), both products on FP64 MFMA.
+#include "mcmb_internal.h"
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace mcmb {
+
+constexpr int TRD_NB = 32;           // reflectors per panel
+constexpr int TRD_MAXN = 8000;       // v of a column staged in LDS (under 64 KiB with the static part)
+constexpr int TRD_ROWT = 256;        // trd_row threads
+constexpr int TRD_SYMT = 512;        // trd_symv threads
+constexpr int TRD_SYMG = 1024;       // most trd_symv workgroups
+
+typedef double fd4 __attribute__((ext_vector_type(4)));
+
+// Device-side state of the current column and panel.
+struct TrdState {
+    double scale;               // 1 / (x_{j+1} - beta), 0 when tau = 0
+    double tau;
+    double alpha;               // x_{j+1}
+    double c1[TRD_NB];          // V_p^T v
+    double c2[TRD_NB];          // W_p^T v
+    double alpha2[TRD_NB];      // w_p += alpha2[p] v_p, p of the panel
+    unsigned int ticket[2];
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// A = G (or D^-1 G D^-1 with norm[i] = sqrt G_ii).
+__global__ void trd_init(const double* __restrict__ G, int n, int lda, int normalise, double* __restrict__ norm,
+                         double* __restrict__ A)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < (long)n * n) {
+        const int i = (int)(e / n), j = (int)(e % n);
+        double v = G[e];
+        if (normalise) v /= sqrt(G[(long)i * n + i]) * sqrt(G[(long)j * n + j]);
+        A[(long)i * lda + j] = v;
+    }
+    if (norm && e < n) norm[e] = sqrt(G[e * n + e]);
+}
+
+// W_i[c] of the panel, final: tau_i (y_i[c] - sum_{p<i} (V_p[c] c2[p] +
+// W_p[c] c1[p])) + alpha2_i V_i[c], from y_i = A v_i as trd_symv stored it in
+// W_i and the final W_p (p < i).
+__device__ __forceinline__ double trd_wfin(const double* __restrict__ Vp, const double* __restrict__ Wp, int n, int i,
+                                          long c, double tau, const double* c1, const double* c2, double a2)
+{
+    double s = Wp[(long)i * n + c];
+    for (int p = 0; p < i; ++p) s -= Vp[(long)p * n + c] * c2[p] + Wp[(long)p * n + c] * c1[p];
+    return tau * s + a2 * Vp[(long)i * n + c];
+}
+
+// Column j = k0 + i.  W_{i-1} is made final on columns j+1.. (column j itself
+// stays as trd_symv left it: nothing reads it again), then the row update
+// x = A[j][j..) - sum_{p<i} (V_p W_p[j] + W_p V_p[j]), d_j, and |x_{j+2..}|^2
+// summed over the workgroups; the last workgroup forms the reflector.
+// Vp, Wp: the panel's rows (V_p at Vp + p n, W_p at Wp + p n).
+__global__ __launch_bounds__(TRD_ROWT) void trd_row(const double* __restrict__ A, int lda, int n, int j, int i,
+                                                   const double* __restrict__ Vp, double* __restrict__ Wp,
+                                                   const double* __restrict__ tt, double* __restrict__ xbuf,
+                                                   double* __restrict__ dd, double* __restrict__ ee,
+                                                   double* __restrict__ tto, TrdState* __restrict__ st,
+                                                   double* __restrict__ part)
+{
+    __shared__ double sV[TRD_NB], sW[TRD_NB], c1[TRD_NB], c2[TRD_NB];
+    __shared__ double red[TRD_ROWT / 64];
+    __shared__ int last;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int q = i - 1;   // the column whose W becomes final here (none when i = 0)
+    double tq = 0.0, a2q = 0.0;
+    if (q >= 0) {
+        tq = tt[j - 1];
+        a2q = st->alpha2[q];
+        if (tid < q) {
+            c1[tid] = st->c1[tid];
+            c2[tid] = st->c2[tid];
+        }
+    }
+    if (tid < i) sV[tid] = Vp[(long)tid * n + j];
+    if (tid < q) sW[tid] = Wp[(long)tid * n + j];
+    __syncthreads();
+    if (q >= 0 && w == 0) {
+        // W_q[j] final: lane p < q adds its correction term, then the wave tree
+        double t = 0.0;
+        if (lane < q) t = Vp[(long)lane * n + j] * c2[lane] + sW[lane] * c1[lane];
+        t = wave_sum(t);
+        if (lane == 0) sW[q] = tq * (Wp[(long)q * n + j] - t) + a2q * sV[q];
+    }
+    __syncthreads();
+    const int c = j + blockIdx.x * TRD_ROWT + tid;
+    double x = 0.0;
+    if (c < n) {
+        x = A[(long)j * lda + c];
+        // one pass over the final W_p, V_p (p < q): the row update and W_q's
+        // correction (unrolled with guards: every load issued before the sums)
+        double vp[TRD_NB - 1], wp[TRD_NB - 1];
+#pragma unroll
+        for (int p = 0; p < TRD_NB - 1; ++p) {
+            vp[p] = p < q ? Vp[(long)p * n + c] : 0.0;
+            wp[p] = p < q ? Wp[(long)p * n + c] : 0.0;
+        }
+        double corr = 0.0;
+#pragma unroll
+        for (int p = 0; p < TRD_NB - 1; ++p) {
+            if (p < q) {
+                x -= vp[p] * sW[p] + wp[p] * sV[p];
+                corr += vp[p] * c2[p] + wp[p] * c1[p];
+            }
+        }
+        if (q >= 0) {
+            const double vq = Vp[(long)q * n + c];
+            const double wq = tq * (Wp[(long)q * n + c] - corr) + a2q * vq;
+            if (c > j) Wp[(long)q * n + c] = wq;
+            x -= vq * sW[q] + wq * sV[q];
+        }
+        if (c == j) dd[j] = x;
+        if (c == j + 1) st->alpha = x;
+        if (c >= j + 1) xbuf[c] = x;
+    }
+    const double xs = (c < n && c >= j + 2) ? x : 0.0;
+    {
+        const double s = wave_sum(xs * xs);
+        if (lane == 0) red[w] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int k = 0; k < TRD_ROWT / 64; ++k) s += red[k];
+        part[blockIdx.x] = s;
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) last = atomicAdd(&st->ticket[0], 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    if (w == 0) {
+        double s = 0.0;
+        for (unsigned b = lane; b < gridDim.x; b += 64) s += part[b];
+        s = wave_sum(s);
+        if (lane == 0) red[0] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const double ss = red[0];
+        const double alpha = st->alpha;
+        double tau = 0.0, beta = alpha, scale = 0.0;
+        if (ss > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+            tau = (beta - alpha) / beta;
+            scale = 1.0 / (alpha - beta);
+        }
+        ee[j] = beta;
+        tto[j] = tau;
+        st->tau = tau;
+        st->scale = scale;
+        st->ticket[0] = 0;
+    }
+}
+
+// y = A v on rows j+1.. (the trailing matrix as of the panel's start; v in
+// LDS; four rows per wave at a time, 16-byte loads kept four steps ahead in
+// registers: lda is even, so the
+// columns j+1+peel, j+2+peel of every row (peel = (j+1) & 1) are one aligned
+// pair), stored in W_i, v stored as V_j; and the sums c1[p] = V_p^T v,
+// c2[p] = W_p^T v (p < i, lane p of each wave) and y^T v over the
+// workgroups.  The last workgroup forms alpha2 = -tau/2 w^T v with
+// w^T v = tau (y^T v - 2 c1 . c2) (w = tau (y - V c2 - W c1) is made final
+// by the next trd_row).
+__global__ __launch_bounds__(TRD_SYMT) void trd_symv(const double* __restrict__ A, int lda, int n, int j, int i,
+                                                    const double* __restrict__ xbuf, const double* __restrict__ Vp,
+                                                    double* __restrict__ Wp, double* __restrict__ Vj,
+                                                    TrdState* __restrict__ st, double* __restrict__ part)
+{
+    constexpr int NWV = TRD_SYMT / 64, NQ = 2 * TRD_NB + 1, RW = 4, DP = 4;
+    extern __shared__ double vs[];
+    __shared__ double red[NWV][NQ];
+    __shared__ double tot[NQ];
+    __shared__ int last;
+    const int m = n - j - 1, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const double scale = st->scale;
+    for (int e = tid; e < m; e += TRD_SYMT) vs[e] = e == 0 ? 1.0 : scale * xbuf[j + 1 + e];
+    __syncthreads();
+    const int peel = (j + 1) & 1, npair = (m - peel) >> 1, tail = (m - peel) & 1;
+    double a1 = 0.0, a2 = 0.0, yv = 0.0;   // lane p < i: V_p . v, W_p . v; lane 0: y . v
+    const int nw = gridDim.x * NWV;
+    for (int r0 = RW * (blockIdx.x * NWV + w); r0 < m; r0 += RW * nw) {
+        const double2* Ap[RW];
+        double y[RW];
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int rr = min(r0 + k, m - 1);
+            const double* const Ar = A + (long)(j + 1 + rr) * lda + (j + 1);
+            Ap[k] = reinterpret_cast<const double2*>(Ar + peel);
+            y[k] = 0.0;
+            if (lane == 0) {
+                if (peel) y[k] = Ar[0] * vs[0];
+                if (tail) y[k] = fma(Ar[m - 1], vs[m - 1], y[k]);
+            }
+        }
+        // steps s of the wave (t = lane + 64 s; lanes past npair read the
+        // last pair and weigh it by zero, so the loop is uniform), loaded DP
+        // steps ahead into a ring of registers (slot d holds step s0 + d)
+        const int ns = (npair + 63) >> 6;
+        double2 ring[DP][RW];
+        if (ns > 0) {
+#pragma unroll
+            for (int d = 0; d < DP; ++d)
+#pragma unroll
+                for (int k = 0; k < RW; ++k) ring[d][k] = Ap[k][min(lane + 64 * d, npair - 1)];
+        }
+        for (int s0 = 0; s0 < ns; s0 += DP) {
+#pragma unroll
+            for (int d = 0; d < DP; ++d) {
+                const int st = s0 + d;
+                if (st < ns) {
+                    const int t = lane + 64 * st;
+                    const bool ok = t < npair;
+                    const int e = peel + 2 * min(t, npair - 1);
+                    const double v0 = ok ? vs[e] : 0.0, v1 = ok ? vs[e + 1] : 0.0;
+#pragma unroll
+                    for (int k = 0; k < RW; ++k) {
+                        y[k] = fma(ring[d][k].x, v0, y[k]);
+                        y[k] = fma(ring[d][k].y, v1, y[k]);
+                    }
+                    if (st + DP < ns)
+#pragma unroll
+                        for (int k = 0; k < RW; ++k) ring[d][k] = Ap[k][min(t + 64 * DP, npair - 1)];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RW; ++k) y[k] = wave_sum(y[k]);
+#pragma unroll
+        for (int k = 0; k < RW; ++k) {
+            const int rr = r0 + k, r = j + 1 + rr;
+            if (rr < m) {
+                const double vr = vs[rr];
+                if (lane < i) {
+                    a1 = fma(Vp[(long)lane * n + r], vr, a1);
+                    a2 = fma(Wp[(long)lane * n + r], vr, a2);
+                }
+                if (lane == 0) {
+                    Wp[(long)i * n + r] = y[k];
+                    Vj[r] = vr;
+                    yv = fma(y[k], vr, yv);
+                }
+            }
+        }
+    }
+    if (lane < i) {
+        red[w][1 + lane] = a1;
+        red[w][1 + i + lane] = a2;
+    }
+    if (lane == 0) red[w][0] = yv;
+    __syncthreads();
+    const int nq = 1 + 2 * i;
+    // partial q of workgroup b at part[q TRD_SYMG + b]
+    if (tid < nq) {
+        double s = 0.0;
+        for (int k = 0; k < NWV; ++k) s += red[k][tid];
+        part[(long)tid * TRD_SYMG + blockIdx.x] = s;
+    }
+    __threadfence();
+    __syncthreads();
+    if (tid == 0) last = atomicAdd(&st->ticket[1], 1u) == gridDim.x - 1;
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // wave w adds the partials q = w, w + NWV, ...: lane-strided sums, then the wave tree
+    for (int qq = w; qq < nq; qq += NWV) {
+        double s = 0.0;
+        for (unsigned b = lane; b < gridDim.x; b += 64) s += part[(long)qq * TRD_SYMG + b];
+        s = wave_sum(s);
+        if (lane == 0) tot[qq] = s;
+    }
+    __syncthreads();
+    if (tid < i) {
+        st->c1[tid] = tot[1 + tid];
+        st->c2[tid] = tot[1 + i + tid];
+    }
+    if (tid == 0) {
+        double cc = 0.0;
+        for (int p = 0; p < i; ++p) cc = fma(tot[1 + p], tot[1 + i + p], cc);
+        const double tau = st->tau;
+        st->alpha2[i] = -0.5 * tau * (tau * (tot[0] - 2.0 * cc));
+        st->ticket[1] = 0;
+    }
+}
+
+// W_q final (q = nb - 1, the panel's last column) on columns [c0, n).
+__global__ void trd_wfinal(double* __restrict__ Wp, const double* __restrict__ Vp, int n, int q, int c0,
+                           const double* __restrict__ tt, const TrdState* __restrict__ st)
+{
+    __shared__ double c1[TRD_NB], c2[TRD_NB];
+    if (threadIdx.x < q) {
+        c1[threadIdx.x] = st->c1[threadIdx.x];
+        c2[threadIdx.x] = st->c2[threadIdx.x];
+    }
+    __syncthreads();
+    const long c = c0 + (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    Wp[(long)q * n + c] = trd_wfin(Vp, Wp, n, q, c, tt[0], c1, c2, st->alpha2[q]);
+}
+
+// A[k1.., k1..] -= V W^T + W V^T (nb panel rows), 64 x 64 lower tiles I >= J
+// on FP64 MFMA (four waves of 32 x 32), K = 2 nb; each tile subtracted from
+// the lower block and, mirrored through LDS, from the upper one.
+__global__ __launch_bounds__(256) void trd_syr2k(double* __restrict__ A, int lda, int n, int k1, const double* __restrict__ Vp,
+                                                 const double* __restrict__ Wp, int nb)
+{
+    __shared__ double S[64][65];
+    const int t = blockIdx.x;
+    int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((I + 1) * (I + 2) / 2 <= t) ++I;
+    while (I * (I + 1) / 2 > t) --I;
+    const int J = t - I * (I + 1) / 2;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, l16 = lane & 15, kq = lane >> 4;
+    const int rl = 32 * (w >> 1), cl = 32 * (w & 1);                 // quadrant within the tile
+    const int rb = k1 + 64 * I + rl, cb = k1 + 64 * J + cl;
+    const int K = 2 * nb;
+    fd4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = (fd4){0.0, 0.0, 0.0, 0.0};
+    for (int q0 = 0; q0 < K; q0 += 4) {
+        const int q = q0 + kq;
+        const bool qin = q < K;
+        // A'(i, q) = V_q[i] | W_{q-nb}[i];  B'(j, q) = W_q[j] | V_{q-nb}[j]
+        const double* const Ar = q < nb ? Vp + (long)q * n : Wp + (long)(q - nb) * n;
+        const double* const Br = q < nb ? Wp + (long)q * n : Vp + (long)(q - nb) * n;
+        double a[2], b[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            const int r = rb + 16 * x + l16, cc = cb + 16 * x + l16;
+            a[x] = (qin && r < n) ? Ar[r] : 0.0;
+            b[x] = (qin && cc < n) ? Br[cc] : 0.0;
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+    }
+    // element v of a 16 x 16 block: row 4 v + lane / 16, column lane % 16
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) S[rl + 16 * x + 4 * v + kq][cl + 16 * y + l16] = acc[x][y][v];
+    __syncthreads();
+    const int r0 = k1 + 64 * I, c0 = k1 + 64 * J;
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int ii = e >> 6, jj = e & 63;
+        const int gi = r0 + ii, gj = c0 + jj;
+        if (gi < n && gj < n && (I > J || jj <= ii)) A[(long)gi * lda + gj] -= S[ii][jj];
+    }
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+        const int jj = e >> 6, ii = e & 63;
+        const int gi = r0 + ii, gj = c0 + jj;
+        if (gi < n && gj < n && (I > J || jj < ii)) A[(long)gj * lda + gi] -= S[ii][jj];
+    }
+}
+
+__global__ void trd_identity(double* __restrict__ X, int n)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < (long)n * n) X[e] = (e / n == e % n) ? 1.0 : 0.0;
+}
+
+__global__ void trd_lastd(const double* __restrict__ A, int lda, int n, double* __restrict__ dd)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) dd[n - 1] = A[(long)(n - 1) * lda + n - 1];
+}
+
+// Per panel b (one workgroup each): G = V V^T of its reflectors, T (dlarft,
+// forward, column-wise) and Vt = T^T V (rows p: sum_{q <= p} T[q][p] v_q),
+// over columns k0 + 1 .. n - 1 (the reflectors are zero before).
+__global__ __launch_bounds__(1024) void trd_tmat(const double* __restrict__ V, const double* __restrict__ tt, int n,
+                                                 double* __restrict__ Vt)
+{
+    __shared__ double Gs[TRD_NB][TRD_NB + 1], Ts[TRD_NB][TRD_NB + 1];
+    const int k0 = blockIdx.x * TRD_NB, nb = min(TRD_NB, n - 1 - k0), c0 = k0 + 1;
+    const int tid = threadIdx.x, a = tid / TRD_NB, b = tid % TRD_NB;
+    if (a < nb && b < nb && b >= a) {
+        double s = 0.0;
+        const double* const va = V + (long)(k0 + a) * n;
+        const double* const vb = V + (long)(k0 + b) * n;
+        for (int c = k0 + b + 1; c < n; ++c) s = fma(va[c], vb[c], s);
+        Gs[a][b] = s;
+    }
+    __syncthreads();
+    if (tid < TRD_NB * TRD_NB) Ts[a][b] = 0.0;
+    __syncthreads();
+    if (tid < nb) {
+        const int q = tid;
+        Ts[q][q] = tt[k0 + q];
+        for (int i = q + 1; i < nb; ++i) {
+            double s = 0.0;
+            for (int r = q; r < i; ++r) s = fma(Ts[q][r], Gs[r][i], s);   // G[r][i], r < i
+            Ts[q][i] = -tt[k0 + i] * s;
+        }
+    }
+    __syncthreads();
+    for (int c = c0 + tid; c < n; c += 1024) {
+        double s[TRD_NB];
+#pragma unroll
+        for (int p = 0; p < TRD_NB; ++p) s[p] = 0.0;
+#pragma unroll
+        for (int q = 0; q < TRD_NB; ++q) {
+            const double vq = q < nb ? V[(long)(k0 + q) * n + c] : 0.0;
+#pragma unroll
+            for (int p = q; p < TRD_NB; ++p) s[p] = fma(Ts[q][p], vq, s[p]);
+        }
+#pragma unroll
+        for (int p = 0; p < TRD_NB; ++p)
+            if (p < nb) Vt[(long)(k0 + p) * n + c] = s[p];
+    }
+}
+
+// M[r][p] = sum_{c >= c0} E[r][c] V_p[c] for 32 rows per workgroup and the
+// panel's nb <= 32 reflectors (FP64 MFMA; the four waves split the columns c
+// and add their 32 x 32 results in LDS in wave order).
+__global__ __launch_bounds__(256) void trd_bt_proj(const double* __restrict__ E, int n, const double* __restrict__ Vp,
+                                                   int nb, int c0, double* __restrict__ M)
+{
+    __shared__ double R[4][32][33];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, l16 = lane & 15, kq = lane >> 4;
+    const int r0 = blockIdx.x * 32;
+    const int m = n - c0, seg = ((m + 3) / 4 + 3) / 4 * 4;   // columns per wave: a quarter, a multiple of 4
+    const int ca = c0 + w * seg, cz = min(n, ca + seg);
+    fd4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = (fd4){0.0, 0.0, 0.0, 0.0};
+    for (int k = ca; k < cz; k += 4) {
+        const int c = k + kq;
+        const bool cin = c < cz;
+        double a[2], b[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            const int r = r0 + 16 * x + l16, p = 16 * x + l16;
+            a[x] = (cin && r < n) ? E[(long)r * n + c] : 0.0;
+            b[x] = (cin && p < nb) ? Vp[(long)p * n + c] : 0.0;
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) R[w][16 * x + 4 * v + kq][16 * y + l16] = acc[x][y][v];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+        const int rr = e >> 5, p = e & 31;
+        const int r = r0 + rr;
+        if (r < n && p < nb) M[(long)r * TRD_NB + p] = ((R[0][rr][p] + R[1][rr][p]) + R[2][rr][p]) + R[3][rr][p];
+    }
+}
+
+// E[r][c] -= sum_p M[r][p] Vt_p[c] for c >= c0 (64 x 64 tiles, FP64 MFMA, K = nb).
+__global__ __launch_bounds__(256) void trd_bt_update(double* __restrict__ E, int n, const double* __restrict__ M,
+                                                     const double* __restrict__ Vt, int nb, int c0)
+{
+    const int m = n - c0, ntc = (m + 63) / 64;
+    const int I = blockIdx.x / ntc, J = blockIdx.x % ntc;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, l16 = lane & 15, kq = lane >> 4;
+    const int rb = 64 * I + 32 * (w >> 1), cb = c0 + 64 * J + 32 * (w & 1);
+    fd4 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) acc[x][y] = (fd4){0.0, 0.0, 0.0, 0.0};
+    for (int p0 = 0; p0 < nb; p0 += 4) {
+        const int p = p0 + kq;
+        const bool pin = p < nb;
+        double a[2], b[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+            const int r = rb + 16 * x + l16, c = cb + 16 * x + l16;
+            a[x] = (pin && r < n) ? M[(long)r * TRD_NB + p] : 0.0;
+            b[x] = (pin && c < n) ? Vt[(long)p * n + c] : 0.0;
+        }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[x], b[y], acc[x][y], 0, 0, 0);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = rb + 16 * x + 4 * v + kq, c = cb + 16 * y + l16;
+                if (r < n && c < n) E[(long)r * n + c] -= acc[x][y][v];
+            }
+}
+
+// Rows of E in the order perm, each with its largest-magnitude component
+// (the first on ties) made positive.
+__global__ __launch_bounds__(256) void trd_sort_rows(const double* __restrict__ E, int n, const int* __restrict__ perm,
+                                                     double* __restrict__ V)
+{
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    const int k = blockIdx.x;
+    const double* row = E + (long)perm[k] * n;
+    double best = -1.0;
+    int bi = 0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const double a = fabs(row[i]);
+        if (a > best) { best = a; bi = i; }
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            const double b = sv[threadIdx.x + s];
+            const int j = si[threadIdx.x + s];
+            if (b > sv[threadIdx.x] || (b == sv[threadIdx.x] && j < si[threadIdx.x])) {
+                sv[threadIdx.x] = b;
+                si[threadIdx.x] = j;
+            }
+        }
+        __syncthreads();
+    }
+    const double sg = row[si[0]] < 0.0 ? -1.0 : 1.0;
+    for (int i = threadIdx.x; i < n; i += 256) V[(long)k * n + i] = sg * row[i];
+}
+
+static int rb_status(rocblas_status s, const char* what)
+{
+    if (s == rocblas_status_success) return MCMB_OK;
+    return fail(MCMB_ERR_HIP, "%s: rocBLAS/rocSOLVER status %d", what, (int)s);
+}
+
+bool tridiag_supported(int n) { return n >= 1 && n <= TRD_MAXN; }
+
+int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* d_norm, double* evals, DevBuf& d_V)
+{
+    if (!tridiag_supported(n)) return fail(MCMB_ERR_ARG, "tridiagonal eigensolver: n = %d outside 1..%d", n, TRD_MAXN);
+    if (!c->rb_handle) {
+        rocblas_handle h = nullptr;
+        MCMB_TRY(rb_status(rocblas_create_handle(&h), "rocblas_create_handle"));
+        c->rb_handle = h;
+    }
+    rocblas_handle h = static_cast<rocblas_handle>(c->rb_handle);
+    MCMB_TRY(rb_status(rocblas_set_stream(h, c->stream), "rocblas_set_stream"));
+    const size_t nn = (size_t)n * n;
+    const int lda = n + (n & 1);   // even: 16-byte aligned column pairs in every row
+    const int npan = std::max(1, (n - 1 + TRD_NB - 1) / TRD_NB);
+    DevBuf A, V, Wp, xb, dd, ee, tt, st, part, E, Vt, M, perm, info;
+    MCMB_TRY(A.alloc(sizeof(double) * (size_t)n * lda));
+    MCMB_TRY(V.alloc(sizeof(double) * nn));
+    MCMB_TRY(Wp.alloc(sizeof(double) * TRD_NB * (size_t)n));
+    MCMB_TRY(xb.alloc(sizeof(double) * n));
+    MCMB_TRY(dd.alloc(sizeof(double) * n));
+    MCMB_TRY(ee.alloc(sizeof(double) * n));
+    MCMB_TRY(tt.alloc(sizeof(double) * n));
+    MCMB_TRY(st.alloc(sizeof(TrdState)));
+    MCMB_TRY(part.alloc(sizeof(double) * (2 * TRD_NB + 1) * std::max(TRD_SYMG, (n + TRD_ROWT - 1) / TRD_ROWT)));
+    MCMB_TRY(info.alloc(sizeof(int)));
+    MCMB_HIP(hipMemsetAsync(V.p, 0, sizeof(double) * nn, c->stream));
+    MCMB_HIP(hipMemsetAsync(Wp.p, 0, sizeof(double) * TRD_NB * (size_t)n, c->stream));
+    MCMB_HIP(hipMemsetAsync(ee.p, 0, sizeof(double) * n, c->stream));
+    MCMB_HIP(hipMemsetAsync(tt.p, 0, sizeof(double) * n, c->stream));
+    MCMB_HIP(hipMemsetAsync(st.p, 0, sizeof(TrdState), c->stream));
+    MCMB_TRY(launch_guard("tridiagonal init", (nn + 255) / 256, 256));
+    hipLaunchKernelGGL(trd_init, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, d_G, n, lda, normalise,
+                       d_norm, A.as<double>());
+    MCMB_HIP(hipGetLastError());
+    double* const Ad = A.as<double>();
+    TrdState* const sd = reinterpret_cast<TrdState*>(st.p);
+    for (int k0 = 0; k0 <= n - 2; k0 += TRD_NB) {
+        const int nb = std::min(TRD_NB, n - 1 - k0), k1 = k0 + nb;
+        const double* const Vp = V.as<double>() + (size_t)k0 * n;
+        for (int i = 0; i < nb; ++i) {
+            const int j = k0 + i, m = n - j - 1;
+            const unsigned g1 = (unsigned)((n - j + TRD_ROWT - 1) / TRD_ROWT);
+            hipLaunchKernelGGL(trd_row, dim3(g1), dim3(TRD_ROWT), 0, c->stream, (const double*)Ad, lda, n, j, i, Vp,
+                               Wp.as<double>(), (const double*)tt.as<double>(), xb.as<double>(), dd.as<double>(),
+                               ee.as<double>(), tt.as<double>(), sd, part.as<double>());
+            const unsigned g2 = (unsigned)std::min(TRD_SYMG, (m + 4 * (TRD_SYMT / 64) - 1) / (4 * (TRD_SYMT / 64)));
+            hipLaunchKernelGGL(trd_symv, dim3(g2), dim3(TRD_SYMT), sizeof(double) * (size_t)m, c->stream,
+                               (const double*)Ad, lda, n, j, i, (const double*)xb.as<double>(), Vp, Wp.as<double>(),
+                               V.as<double>() + (size_t)j * n, sd, part.as<double>());
+        }
+        if (k1 < n) {
+            hipLaunchKernelGGL(trd_wfinal, dim3((unsigned)((n - k1 + 255) / 256)), dim3(256), 0, c->stream,
+                               Wp.as<double>(), Vp, n, nb - 1, k1, (const double*)(tt.as<double>() + k1 - 1),
+                               (const TrdState*)sd);
+            const int T = (n - k1 + 63) / 64;
+            hipLaunchKernelGGL(trd_syr2k, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, c->stream, Ad, lda, n, k1, Vp,
+                               (const double*)Wp.as<double>(), nb);
+        }
+        // alpha2 of the next panel starts from zero
+        MCMB_HIP(hipMemsetAsync(sd->alpha2, 0, sizeof(double) * TRD_NB, c->stream));
+    }
+    hipLaunchKernelGGL(trd_lastd, dim3(1), dim3(64), 0, c->stream, (const double*)Ad, lda, n, dd.as<double>());
+    MCMB_HIP(hipGetLastError());
+    // eigenpairs of T: eigenvalues ascending in dd, eigenvectors as the
+    // columns of a column-major n x n matrix, i.e. the rows of E
+    MCMB_TRY(E.alloc(sizeof(double) * nn));
+    hipLaunchKernelGGL(trd_identity, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, E.as<double>(), n);
+    MCMB_HIP(hipGetLastError());
+    MCMB_TRY(rb_status(rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, dd.as<double>(), ee.as<double>(),
+                                        E.as<double>(), n, info.as<int>()),
+                       "rocsolver_dstedc"));
+    int inf = 0;
+    MCMB_HIP(hipMemcpyAsync(&inf, info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    if (inf != 0) return fail(MCMB_ERR_ARG, "tridiagonal eigensolver (dstedc) failed: info = %d at n = %d", inf, n);
+    // back-transformation, panels from the last to the first
+    if (n >= 2) {
+        MCMB_TRY(Vt.alloc(sizeof(double) * nn));
+        MCMB_TRY(M.alloc(sizeof(double) * TRD_NB * (size_t)n));
+        hipLaunchKernelGGL(trd_tmat, dim3((unsigned)npan), dim3(1024), 0, c->stream, (const double*)V.as<double>(),
+                           (const double*)tt.as<double>(), n, Vt.as<double>());
+        for (int b = npan - 1; b >= 0; --b) {
+            const int k0 = b * TRD_NB, nb = std::min(TRD_NB, n - 1 - k0), c0 = k0 + 1;
+            hipLaunchKernelGGL(trd_bt_proj, dim3((unsigned)((n + 31) / 32)), dim3(256), 0, c->stream,
+                               (const double*)E.as<double>(), n, (const double*)(V.as<double>() + (size_t)k0 * n), nb,
+                               c0, M.as<double>());
+            const unsigned tiles = (unsigned)(((n + 63) / 64) * ((n - c0 + 63) / 64));
+            hipLaunchKernelGGL(trd_bt_update, dim3(tiles), dim3(256), 0, c->stream, E.as<double>(), n,
+                               (const double*)M.as<double>(), (const double*)(Vt.as<double>() + (size_t)k0 * n), nb, c0);
+        }
+        MCMB_HIP(hipGetLastError());
+    }
+    // descending order: eigenvalue k of the output is dd[n - 1 - k]
+    std::vector<double> w(n);
+    MCMB_HIP(hipMemcpyAsync(w.data(), dd.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));
+    std::vector<int> pm(n);
+    std::iota(pm.begin(), pm.end(), 0);
+    std::stable_sort(pm.begin(), pm.end(), [&](int a, int b) { return w[a] > w[b]; });
+    for (int k = 0; k < n; ++k) evals[k] = w[pm[k]];
+    MCMB_TRY(perm.alloc(sizeof(int) * n));
+    MCMB_HIP(hipMemcpyAsync(perm.p, pm.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    MCMB_TRY(d_V.alloc(sizeof(double) * nn));
+    hipLaunchKernelGGL(trd_sort_rows, dim3(n), dim3(256), 0, c->stream, (const double*)E.as<double>(), n,
+                       perm.as<int>(), d_V.as<double>());
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipStreamSynchronize(c->stream));   // pm and the work buffers go out of scope
+    return MCMB_OK;
+}
+
+void tridiag_release(mcmb_ctx* c)
+{
+    if (c->rb_handle) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(c->rb_handle));
+    c->rb_handle = nullptr;
+}
+
+}  // namespace mcmb

@@ -1,5 +1,6 @@
-"""GPU: FP64 Jacobi eigen-decomposition and the late-side orthonormalisation
-with its PCA fallback, against LAPACK (numpy eigh) and the oracle."""
+"""GPU: FP64 eigen-decomposition (Householder tridiagonalisation, block and
+scalar Jacobi) and the late-side orthonormalisation with its PCA fallback,
+against LAPACK (numpy eigh) and the oracle."""
 import os
 
 import numpy as np
@@ -18,14 +19,15 @@ def _sym(n, seed):
     return 0.5 * (a + a.T)
 
 
-@pytest.fixture(params=["block", "scalar"])
+@pytest.fixture(params=["tridiagonal", "block", "scalar"])
 def eig_mode(request):
-    """Block Jacobi (the default) or the scalar Jacobi (MCMB_EIGEN=scalar)."""
+    """Householder tridiagonalisation (the default), block Jacobi
+    (MCMB_EIGEN=block) or the scalar Jacobi (MCMB_EIGEN=scalar)."""
     old = os.environ.get("MCMB_EIGEN")
-    if request.param == "scalar":
-        os.environ["MCMB_EIGEN"] = "scalar"
-    else:
+    if request.param == "tridiagonal":
         os.environ.pop("MCMB_EIGEN", None)
+    else:
+        os.environ["MCMB_EIGEN"] = request.param
     yield request.param
     if old is None:
         os.environ.pop("MCMB_EIGEN", None)
@@ -51,7 +53,10 @@ def test_eigen_sym_against_lapack(ctx, n, eig_mode):
         sep = min(gaps[k - 1] if k > 0 else np.inf, gaps[k] if k < n - 1 else np.inf)
         if sep > 1e-4 * nrm:
             assert abs(abs(V[:, k] @ Wl[:, k]) - 1.0) < 1e-9
-    assert 0 < sweeps < 20 or n == 1
+    if eig_mode == "tridiagonal":
+        assert sweeps == 0
+    else:
+        assert 0 < sweeps < 20 or n == 1
 
 
 @pytest.mark.parametrize("n", [97, 480])
@@ -71,6 +76,25 @@ def test_eigen_sym_graded_singular_spectrum(ctx, n, eig_mode):
     # the large eigenvalues to normwise accuracy, relative to themselves
     assert np.abs(ev[:big] / ref[:big] - 1).max() < 1e-10
     assert sweeps < 60
+
+
+@pytest.mark.parametrize("n", [4000, 4960])
+def test_eigen_sym_large_against_lapack(ctx, n):
+    """Householder tridiagonalisation at the sizes of the lmax-2000 and
+    lmax-2500 Grams: eigenvalues against LAPACK to normwise accuracy,
+    residual and orthogonality of the vectors (checked on 64 random
+    combinations of them, so the check itself stays O(n^2))."""
+    A = _sym(n, n)
+    ev, V, sweeps = ctx.eigen_sym(A)
+    ref = np.linalg.eigvalsh(A)[::-1]
+    nrm = np.abs(ref).max()
+    assert sweeps == 0
+    assert np.all(np.diff(ev) <= 0)
+    assert np.abs(ev - ref).max() < 1e-13 * nrm * 8
+    X = np.random.default_rng(1).standard_normal((n, 64))
+    Y = V @ X
+    assert np.abs(A @ Y - V @ (ev[:, None] * X)).max() < 1e-12 * nrm * np.sqrt(n) * 8
+    assert np.abs(V.T @ Y - X).max() < 1e-12 * np.sqrt(n) * 8
 
 
 def test_eigen_sym_matches_oracle_vectors(ctx):

@@ -44,6 +44,7 @@ constexpr int SG_TN = 32 * SG_FN;     // GEMM tile columns (96)
 constexpr int SG_SMALL = 64;          // basis widths up to this use 64 x 64 tiles (2 x 2 blocks per wave)
 constexpr int SG_KC = 16;             // K rows per LDS stage
 constexpr int SG_KSPLIT = 16;         // K splits of the l1 contraction over a compressed basis
+constexpr int SR_UPAD = 8;          // zero rows after the last l of the basis tables
 constexpr int SR_G = 16;            // rows per row-moment group
 constexpr int SR_KC = 16;           // l3 values per weight stage (SR_G * SR_KC = 256 threads)
 constexpr int SR_NPMAX = 768;       // pair columns of stage 1 (3 per thread of 256)
@@ -61,9 +62,9 @@ __global__ void sep_utab(const double* __restrict__ q, int ldq, int nl, const in
                          double* __restrict__ U)
 {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long)nl * ld) return;
+    if (e >= (long)(nl + SR_UPAD) * ld) return;
     const int l = (int)(e / ld), f = (int)(e % ld);
-    if (f >= NP) {
+    if (f >= NP || l >= nl) {
         U[e] = 0.0;
         return;
     }
@@ -307,15 +308,16 @@ __global__ __launch_bounds__(512, 2) void sep_rowmom_mfma(const int* __restrict_
 // (l1, l2), so the table windows below hold for either leg; the row factor
 // and the point where permsix changes (l3 = l2) are taken with the legs in
 // (l1, l2) order, so the weights are the staged kernel's bit for bit.
-template <int CPW, int ASYM, int NW>
+template <int CPW, int ASYM, int NW, bool SWAP>
 __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_rowmom_lw(const int* __restrict__ groups,
                                                         const int4* __restrict__ ginfo,
                                                         const RowDesc* __restrict__ rows,
                                                         const double* __restrict__ f,
                                                         const double* __restrict__ v, int row_base, int ngroups,
                                                         const double* __restrict__ U, int ldu, int NP, int ld,
-                                                        double* __restrict__ M, int swap)
+                                                        double* __restrict__ M)
 {
+    constexpr bool swap = SWAP;
     constexpr int G = SR_GM, RT = SR_GM / 16;
     constexpr int WPT = 8 / NW;   // weights of a chunk per thread (64 NW threads)
     static_assert(WPT * 64 * NW == 16 * G, "a chunk's weights split evenly over the threads");
@@ -356,11 +358,14 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
     // at 8 tiles).  Each output column sums in l3 order either way.
     constexpr bool CONS = CPW == 4;
     const int col0 = 16 * CPW * w + (CONS ? CPW * (lane & 15) : (lane & 15));
-    const double* const Ub = U + (long)kmin * ldu + col0;
-    const long ustep = 2l * ldu;
+    // K-step s reads U row kmin + 2 (4 s + lane / 16); s stays below nks, so
+    // rows reach at most 6 past the group's last l3, into the table's zero
+    // padding (SR_UPAD rows after lmax), and meet zero weights there
+    const double* const Ubl = U + (long)(kmin + 2 * (lane >> 4)) * ldu + col0;
+    const long ustep4 = 8l * ldu;
 
     auto loadb = [&](int s, double (&b)[CPW]) {
-        const double* Ur = Ub + ustep * min(4 * s + (lane >> 4), nkk - 1);
+        const double* Ur = Ubl + ustep4 * s;
         if constexpr (CONS) {
 #pragma unroll
             for (int ct = 0; ct < CPW; ct += 2) {
@@ -501,6 +506,24 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
             Wt[buf][kk0 + 2 * NW * x][g] = tx.in ? wv : 0.0;
         }
     };
+    // chunks after the first of a group sharing l2 (exact h^2): every row
+    // starts at l3 = kmin (its l1 parity is the group's), so no element lies
+    // below its row; above it the tables' zero guards make the weight +0
+    // exactly (f(g - l3) with g < l3, or v past lmax), as the range flag
+    // would; and l3 = l2 only at the first element of chunk 0, so the row
+    // factor is rn.  Same products, same bits as weigh.
+    auto weigh_tail = [&](const Tab (&t)[2], int gr, int buf) {
+        if constexpr (SWAP && !ASYM) {
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const int x = 2 * gr + y;
+                const Tab& tx = t[y];
+                Wt[buf][kk0 + 2 * NW * x][g] = sep_weight(rn, tx.tk, tx.f1, tx.f2, tx.f3, tx.rg);
+            }
+        } else {
+            weigh(t, gr, buf);
+        }
+    };
     if constexpr (WIN) {
         if (nks > 0) {
             double wr[5];
@@ -547,7 +570,7 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
             const int gr = NG == 1 ? (ks == 2 ? 0 : -1) : NG == 2 ? (ks == 0 ? 0 : ks == 2 ? 1 : -1) : ks;
             if (gr >= 0) {
                 __builtin_amdgcn_sched_barrier(0);
-                weigh(t, gr, buf ^ 1);
+                weigh_tail(t, gr, buf ^ 1);
                 if (gr + 1 < NG) gather(c + 1, gr + 1, t);
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -907,9 +930,9 @@ __global__ __launch_bounds__(256) void sep_mgs_step(double* __restrict__ Uc, int
 __global__ void sep_ptab(const double* __restrict__ Pc, int nlp, int nl, int r, int ldu, double* __restrict__ P)
 {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long)nl * ldu) return;
+    if (e >= (long)(nl + SR_UPAD) * ldu) return;
     const int l = (int)(e / ldu), a = (int)(e % ldu);
-    P[e] = a < r ? Pc[(long)a * nlp + l] : 0.0;
+    P[e] = (a < r && l < nl) ? Pc[(long)a * nlp + l] : 0.0;
 }
 
 // H[g][a r + b] = Hs(sorted (g, a, b)) with Hs(x, y, z) = G2[x][{y,z}] +
@@ -1756,8 +1779,9 @@ static int separable_ptab(mcmb_ctx* c, int ldu)
 {
     SepComp& S = c->sepc;
     if (S.ldu == ldu) return MCMB_OK;
-    MCMB_TRY(S.pt.alloc(sizeof(double) * (size_t)S.nl * ldu));
-    const long e = (long)S.nl * ldu;
+    // SR_UPAD zero rows after the last l (read by the fused l3 contraction)
+    MCMB_TRY(S.pt.alloc(sizeof(double) * (size_t)(S.nl + SR_UPAD) * ldu));
+    const long e = (long)(S.nl + SR_UPAD) * ldu;
     MCMB_TRY(launch_guard("compressed basis table", (unsigned long long)((e + 255) / 256), 256));
     hipLaunchKernelGGL(sep_ptab, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, c->stream,
                        (const double*)S.pc.as<double>(), S.nlp, S.nl, S.r, ldu, S.pt.as<double>());
@@ -1861,12 +1885,12 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         MCMB_TRY(separable_ptab(c, ldu));
         dU = S.pt.as<double>();
     } else {
-        MCMB_TRY(c->sg_U.alloc(sizeof(double) * (size_t)nl * ldu + sizeof(int2) * NP));
+        MCMB_TRY(c->sg_U.alloc(sizeof(double) * (size_t)(nl + SR_UPAD) * ldu + sizeof(int2) * NP));
         double* const dUw = c->sg_U.as<double>();
-        int2* d_pxy = reinterpret_cast<int2*>(dUw + (size_t)nl * ldu);
+        int2* d_pxy = reinterpret_cast<int2*>(dUw + (size_t)(nl + SR_UPAD) * ldu);
         MCMB_HIP(hipMemcpy(d_pxy, pxy.data(), sizeof(int2) * NP, hipMemcpyHostToDevice));
         MCMB_TRY(phase("pair products"));
-        const long e = (long)nl * ldu;
+        const long e = (long)(nl + SR_UPAD) * ldu;
         MCMB_TRY(launch_guard("pair products", (unsigned long long)((e + 255) / 256), 256));
         hipLaunchKernelGGL(sep_utab, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, c->stream, L.d_q.as<double>(),
                            L.ldq, nl, d_pxy, NP, ldu, dUw);
@@ -1943,17 +1967,22 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     hipLaunchKernelGGL((sep_rowmom_mfma<CC, SR_GM / 16>), dim3((unsigned)T.ngroups), dim3(64 * nwv), 0, c->stream,       \
                        tab + T.o_groups, ginfo, wofs, (const double*)c->sg_wd.as<double>(), T.row0, L.kstep,  \
                        dU, ldu, NB, ldB, c->sg_M.as<double>())
-#define SEP_ROWMOM_LW(CC, AA, NN)                                                                             \
-    hipLaunchKernelGGL((sep_rowmom_lw<CC, AA, NN>),                                                         \
+#define SEP_ROWMOM_LW(CC, AA, NN, SS)                                                                             \
+    hipLaunchKernelGGL((sep_rowmom_lw<CC, AA, NN, SS>),                                                     \
                        dim3((unsigned)(8 * SR_XC * ((T.ngroups + 8 * SR_XC - 1) / (8 * SR_XC)))), dim3(64 * NN), \
                        0, c->stream, tab + T.o_groups, ginfo, (const RowDesc*)L.d_rows.as<RowDesc>(),          \
                        (const double*)L.d_ftp.as<double>() + ftab_guard(L.lmax), (const double*)L.d_vp.as<double>(), \
                        T.row0,                                                                                  \
-                       T.ngroups, dU, ldu, NB, ldB, c->sg_M.as<double>(), (int)T.swap)
+                       T.ngroups, dU, ldu, NB, ldB, c->sg_M.as<double>())
 #define SEP_LW(CC, NN)                                                                                        \
     do {                                                                                                      \
-        if (wkind == 0) SEP_ROWMOM_LW(CC, 0, NN);                                                             \
-        else SEP_ROWMOM_LW(CC, 1, NN);                                                                        \
+        if (T.swap) {                                                                                         \
+            if (wkind == 0) SEP_ROWMOM_LW(CC, 0, NN, true);                                                   \
+            else SEP_ROWMOM_LW(CC, 1, NN, true);                                                              \
+        } else {                                                                                              \
+            if (wkind == 0) SEP_ROWMOM_LW(CC, 0, NN, false);                                                  \
+            else SEP_ROWMOM_LW(CC, 1, NN, false);                                                             \
+        }                                                                                                     \
     } while (0)
         if (wfused) {
             // the (column tiles, waves) pairs the width selection above produces

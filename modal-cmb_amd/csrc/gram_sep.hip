@@ -335,20 +335,21 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
     const int gb = ((blockIdx.x & 7) + 8 * (slot / SR_XC)) * SR_XC + slot % SR_XC;
     if (gb >= ngroups) return;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    // rows of a group share one leg (gi: l1, or l2 when swap); the first is
-    // never padding
-    const RowDesc d0 = rows[groups[(long)gb * G]];
-    const int gi = swap ? d0.j : d0.i;
+    // the group's l3 range, the leg its rows share (gi: l1, or l2 when swap)
+    // and the range [jlo, jhi] of their other leg, from its table entry: the
+    // table windows and the first basis rows are read without waiting for the
+    // row descriptors
+    const int4 info = ginfo[gb];
+    const int kmin = info.x, nkk = info.y;
+    const int gi = info.z, jlo = info.w & 0xffff, jhi = info.w >> 16;
     if (tid < G) {
         const int r = groups[(long)gb * G + tid];
         // padding rows: no points, the first row's legs (keeps window indices in range)
-        const RowDesc d = r >= 0 ? rows[r] : RowDesc{d0.i, d0.j, 0, 0};
-        gj[tid] = swap ? d.i : d.j;   // the row's other leg
+        const RowDesc d = r >= 0 ? rows[r] : RowDesc{0, 0, 0, 0};
+        gj[tid] = r >= 0 ? (swap ? d.i : d.j) : jlo;   // the row's other leg
         gk0[tid] = d.k0;
         gnk[tid] = d.nk;
     }
-    const int4 info = ginfo[gb];
-    const int kmin = info.x, nkk = info.y;
     const int nks = (nkk + 3) >> 2;           // K-steps
     const int nch = (nkk + 15) >> 4;          // chunks of 4 K-steps
     // 4 column tiles: lane l of wave w holds columns 16 CPW w + CPW (l % 16)
@@ -392,27 +393,8 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
                 acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
     };
     double bs[4][CPW];
-    __syncthreads();   // row info
-    // weights of chunk c into Wt[buf]: thread tid forms the elements
-    // tid + 64 NW x (x < 8 / NW), element e = (kk = e / 32, g = e % 32),
-    // l3 = kmin + 2 (16 c + kk);
-    // all are of row g = tid % 32, whose constants stay in registers.  The
-    // table reads of a group of two weights (gather) are issued one or more
-    // K-steps of MFMAs before their use (weigh); exact weights by
-    // sep_weight, as the staged ones.
-    const int g = tid & (G - 1), kk0 = tid / G;   // element x: l3 index kk0 + 2 NW x of the chunk
-    const int rj = gj[g], rk0 = gk0[g], rnk = gnk[g];
-    // window geometry: the rows' j lie in [jlo, jhi] (one parity, span <= 96:
-    // checked on the host), so for l3 = L0 + 2 t (t < 16) the f indices are
-    // A + dj + t, A + dh + t, A + dj + 15 - t and the 1/((2a+1) f) index A +
-    // dj + t, all below 64
-    const int jlo = swap ? d0.i : d0.j;
-    int jhi = jlo;
-    if constexpr (WIN)
-        for (int q = 1; q < G; ++q) jhi = max(jhi, gj[q]);
-    const int dj = (rj - jlo) >> 1, dh = (jhi - rj) >> 1;
-    // the legs in (l1, l2) order, and the l3 at which permsix changes (l3 = l2)
-    const int li = swap ? rj : gi, lj = swap ? gi : rj;
+    // the first three K-steps' basis rows and chunks 0 and 1's windows, in
+    // flight while the row descriptors arrive
     auto wload = [&](int c, double (&wr)[5]) {
         const int L0 = kmin + 32 * c;
         const int A1 = (jlo + L0 - gi) >> 1, A2 = (gi + L0 - jhi) >> 1, A3 = (gi + jlo - L0 - 30) >> 1,
@@ -427,6 +409,32 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
 #pragma unroll
         for (int q = 0; q < 5; ++q) Wf[b][q][lane] = wr[q];
     };
+    double wr0[WIN ? 5 : 1], wr1[WIN ? 5 : 1];
+    if (nks > 0) {
+#pragma unroll
+        for (int x = 0; x < 3; ++x) loadb(min(x, nks - 1), bs[x]);
+        if constexpr (WIN) {
+            wload(0, wr0);
+            wload(min(1, nch - 1), wr1);
+        }
+    }
+    __syncthreads();   // row info
+    // weights of chunk c into Wt[buf]: thread tid forms the elements
+    // tid + 64 NW x (x < 8 / NW), element e = (kk = e / 32, g = e % 32),
+    // l3 = kmin + 2 (16 c + kk);
+    // all are of row g = tid % 32, whose constants stay in registers.  The
+    // table reads of a group of two weights (gather) are issued one or more
+    // K-steps of MFMAs before their use (weigh); exact weights by
+    // sep_weight, as the staged ones.
+    const int g = tid & (G - 1), kk0 = tid / G;   // element x: l3 index kk0 + 2 NW x of the chunk
+    const int rj = gj[g], rk0 = gk0[g], rnk = gnk[g];
+    // window geometry: the rows' j lie in [jlo, jhi] (one parity, span <= 96:
+    // checked on the host), so for l3 = L0 + 2 t (t < 16) the f indices are
+    // A + dj + t, A + dh + t, A + dj + 15 - t and the 1/((2a+1) f) index A +
+    // dj + t, all below 64
+    const int dj = (rj - jlo) >> 1, dh = (jhi - rj) >> 1;
+    // the legs in (l1, l2) order, and the l3 at which permsix changes (l3 = l2)
+    const int li = swap ? rj : gi, lj = swap ? gi : rj;
     // row factors of the weight at l3 = l2 and at l3 > l2 (l3 >= l2 >= l1 on the row)
     const double re = sep_row_factor(li, lj, permsix_d(li, lj, lj), v),
                  rn = sep_row_factor(li, lj, permsix_d(li, lj, lj + 1), v);
@@ -526,11 +534,8 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
     };
     if constexpr (WIN) {
         if (nks > 0) {
-            double wr[5];
-            wload(0, wr);
-            wstore(0, wr);
-            wload(min(1, nch - 1), wr);
-            wstore(1, wr);
+            wstore(0, wr0);
+            wstore(1, wr1);
         }
         __syncthreads();
     }
@@ -541,8 +546,6 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
             gather(0, gr, t0);
             weigh(t0, gr, 0);
         }
-#pragma unroll
-        for (int x = 0; x < 3; ++x) loadb(min(x, nks - 1), bs[x]);
     }
     __syncthreads();
     // all chunks but the last: 4 K-steps each; the next chunk's table reads
@@ -1049,7 +1052,8 @@ __global__ __launch_bounds__(256) void sep_assemble_sym(const double* __restrict
 
 bool gram_separable_supported(const PointSet& L, int precision)
 {
-    return precision == MCMB_FP64 && L.R >= 1 && L.R <= 255 && L.R * (L.R + 1) / 2 <= SR_NPMAX;
+    // (group table entries keep two legs in 16 bits each)
+    return precision == MCMB_FP64 && L.R >= 1 && L.R <= 255 && L.R * (L.R + 1) / 2 <= SR_NPMAX && L.lmax < 32768;
 }
 
 // Row groups sharing l2 formed on the device, one wave per (l2, parity)
@@ -1086,12 +1090,13 @@ __global__ __launch_bounds__(64) void sep_groups_l2(const int* __restrict__ r0_o
     const int h = lane / G, t = lane % G;
     for (int gp = g0; gp < g1; gp += 2) {
         const int g = gp + h;
-        int kmin = INT_MAX, kmax = INT_MIN, di = 0;
+        int kmin = INT_MAX, kmax = INT_MIN, di = 0, d0i = 0;
         if (g < g1) {
             const int r = out[(long)(g - g0) * G + t];
             if (r >= 0) {
                 const RowDesc d = rows[r], d0 = rows[out[(long)(g - g0) * G]];
                 di = d.i - d0.i;
+                d0i = d0.i;
                 if (d.nk > 0) {
                     kmin = d.k0;
                     kmax = d.k0 + kstep * (d.nk - 1);
@@ -1106,7 +1111,8 @@ __global__ __launch_bounds__(64) void sep_groups_l2(const int* __restrict__ r0_o
         }
         if (g < g1 && t == 0) {
             const int nkk = kmax >= kmin ? (kmax - kmin) / kstep + 1 : 0;
-            ginfo[g] = make_int4(nkk ? kmin : 0, nkk, 0, 0);
+            // shared leg l2 = j; l1 from the first row's to the largest
+            ginfo[g] = make_int4(nkk ? kmin : 0, nkk, j, d0i | ((d0i + di) << 16));
             wofs[g] = (long)G * 16 * ((nkk + 15) / 16);
             atomicMax(jspan, di);
         }
@@ -1398,7 +1404,7 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
         std::vector<std::vector<int>> cls(kstep);
         auto flush = [&](std::vector<int>& c) {
             if (c.empty()) return;
-            int kmin = INT_MAX, kmax = INT_MIN;
+            int kmin = INT_MAX, kmax = INT_MIN, jhi = INT_MIN;
             for (int g = 0; g < gsz; ++g) {
                 P.groups.push_back(g < (int)c.size() ? c[g] : -1);
                 if (g < (int)c.size() && L.rows[c[g]].nk > 0) {
@@ -1410,10 +1416,14 @@ void build_block(const PointSet& L, const std::vector<Run>& runs, int b0, int b1
                     // span of the leg that varies within the group
                     const RowDesc &d = L.rows[c[g]], &d0 = L.rows[c[0]];
                     P.jspan = std::max(P.jspan, by_l2 ? d.i - d0.i : d.j - d0.j);
+                    jhi = std::max(jhi, by_l2 ? d.i : d.j);
                 }
             }
             const int nkk = kmax >= kmin ? (kmax - kmin) / kstep + 1 : 0;
-            P.ginfo.push_back(make_int4(nkk ? kmin : 0, nkk, 0, 0));
+            // the shared leg, and the first row's and the largest other leg
+            const RowDesc& d0 = L.rows[c[0]];
+            const int gi = by_l2 ? d0.j : d0.i, jlo = by_l2 ? d0.i : d0.j;
+            P.ginfo.push_back(make_int4(nkk ? kmin : 0, nkk, gi, jlo | (jhi << 16)));
             P.wsz.push_back((long)gsz * 16 * ((nkk + 15) / 16));
             c.clear();
         };

@@ -58,7 +58,7 @@ Reported beside it:
                       (direct_extrapolated) the direct tetrapyd sum timed on
                       the top i-rows and extrapolated by point count
 
-Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--slices 32]
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--slices 32]
                        [--cpu-seconds 20] [--no-cpu] [--no-projection] [--no-other-workloads]
                        [--workload lmax2500_r30]
 """
@@ -382,8 +382,10 @@ def with_path(path, fn):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    # a step is about 3 ms: 5 warm-up steps bring the clocks to their steady state, and the
+    # mean over 20 steps (and its per-phase HIP-event times) is stable to about 1%
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="lmax2500_r30",
                     help="l-space workload of the headline Gamma (a key of modalcmb.workloads.WORKLOADS)")
     ap.add_argument("--slices", type=int, default=32,

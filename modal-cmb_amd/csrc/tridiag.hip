@@ -32,7 +32,9 @@
 #include "mcmb_internal.h"
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
+#include <dlfcn.h>
 #include <algorithm>
+#include <mutex>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -582,18 +584,53 @@ static int rb_status(rocblas_status s, const char* what)
     return fail(MCMB_ERR_HIP, "%s: rocBLAS/rocSOLVER status %d", what, (int)s);
 }
 
+// rocBLAS and rocSOLVER are loaded on the first eigen-decomposition (the
+// solver library is large; processes that never call it do not map it).
+struct RocSolver {
+    bool ok = false;
+    char err[256] = "";
+    rocblas_status (*create_handle)(rocblas_handle*) = nullptr;
+    rocblas_status (*destroy_handle)(rocblas_handle) = nullptr;
+    rocblas_status (*set_stream)(rocblas_handle, hipStream_t) = nullptr;
+    rocblas_status (*dstedc)(rocblas_handle, const rocblas_evect, const rocblas_int, double*, double*, double*,
+                             const rocblas_int, rocblas_int*) = nullptr;
+};
+
+static const RocSolver& rocsolver_lib()
+{
+    static RocSolver R;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        void* hb = dlopen("librocblas.so.5", RTLD_NOW | RTLD_GLOBAL);
+        void* hs = hb ? dlopen("librocsolver.so.0", RTLD_NOW | RTLD_GLOBAL) : nullptr;
+        if (!hb || !hs) {
+            snprintf(R.err, sizeof(R.err), "%s", dlerror());
+            return;
+        }
+        R.create_handle = reinterpret_cast<decltype(R.create_handle)>(dlsym(hb, "rocblas_create_handle"));
+        R.destroy_handle = reinterpret_cast<decltype(R.destroy_handle)>(dlsym(hb, "rocblas_destroy_handle"));
+        R.set_stream = reinterpret_cast<decltype(R.set_stream)>(dlsym(hb, "rocblas_set_stream"));
+        R.dstedc = reinterpret_cast<decltype(R.dstedc)>(dlsym(hs, "rocsolver_dstedc"));
+        R.ok = R.create_handle && R.destroy_handle && R.set_stream && R.dstedc;
+        if (!R.ok) snprintf(R.err, sizeof(R.err), "rocBLAS/rocSOLVER symbols missing");
+    });
+    return R;
+}
+
 bool tridiag_supported(int n) { return n >= 1 && n <= TRD_MAXN; }
 
 int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* d_norm, double* evals, DevBuf& d_V)
 {
     if (!tridiag_supported(n)) return fail(MCMB_ERR_ARG, "tridiagonal eigensolver: n = %d outside 1..%d", n, TRD_MAXN);
+    const RocSolver& RS = rocsolver_lib();
+    if (!RS.ok) return fail(MCMB_ERR_HIP, "tridiagonal eigensolver: cannot load rocBLAS/rocSOLVER (%s)", RS.err);
     if (!c->rb_handle) {
         rocblas_handle h = nullptr;
-        MCMB_TRY(rb_status(rocblas_create_handle(&h), "rocblas_create_handle"));
+        MCMB_TRY(rb_status(RS.create_handle(&h), "rocblas_create_handle"));
         c->rb_handle = h;
     }
     rocblas_handle h = static_cast<rocblas_handle>(c->rb_handle);
-    MCMB_TRY(rb_status(rocblas_set_stream(h, c->stream), "rocblas_set_stream"));
+    MCMB_TRY(rb_status(RS.set_stream(h, c->stream), "rocblas_set_stream"));
     const size_t nn = (size_t)n * n;
     const int lda = n + (n & 1);   // even: 16-byte aligned column pairs in every row
     const int npan = std::max(1, (n - 1 + TRD_NB - 1) / TRD_NB);
@@ -651,8 +688,8 @@ int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* 
     MCMB_TRY(E.alloc(sizeof(double) * nn));
     hipLaunchKernelGGL(trd_identity, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, E.as<double>(), n);
     MCMB_HIP(hipGetLastError());
-    MCMB_TRY(rb_status(rocsolver_dstedc(h, rocblas_evect_tridiagonal, n, dd.as<double>(), ee.as<double>(),
-                                        E.as<double>(), n, info.as<int>()),
+    MCMB_TRY(rb_status(RS.dstedc(h, rocblas_evect_tridiagonal, n, dd.as<double>(), ee.as<double>(), E.as<double>(),
+                                  n, info.as<int>()),
                        "rocsolver_dstedc"));
     int inf = 0;
     MCMB_HIP(hipMemcpyAsync(&inf, info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -695,7 +732,8 @@ int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* 
 
 void tridiag_release(mcmb_ctx* c)
 {
-    if (c->rb_handle) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(c->rb_handle));
+    // a handle exists only if the library was loaded
+    if (c->rb_handle) (void)rocsolver_lib().destroy_handle(static_cast<rocblas_handle>(c->rb_handle));
     c->rb_handle = nullptr;
 }
 

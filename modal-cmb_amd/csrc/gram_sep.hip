@@ -75,9 +75,10 @@ __global__ void sep_utab(const double* __restrict__ q, int ldq, int nl, const in
 // Weights of the row groups, staged densely for stage 1: for group gb of G
 // rows, l3 index kg of the group's grid (l3 = kmin + kstep kg) and member row
 // g, Wd[wofs[gb] + G kg + g] = w (0 outside the row's range), kg padded to a
-// multiple of 16.  KIND 0 / 1: the l-space weight permsix h^2 v v v with the
-// exact / asymptotic h^2, formed here; KIND 2: copied from the walk-order
-// weights ws.
+// multiple of 16.  KIND 0: the l-space weight with the exact h^2 less its l3
+// factor (sep_weight: the contraction's basis table carries (2 l3 + 1) v_l3);
+// KIND 1: the whole weight permsix h^2 v v v with the asymptotic h^2; both
+// formed here; KIND 2: copied from the walk-order weights ws.
 template <int KIND, int G>
 __global__ __launch_bounds__(256) void sep_wstage(const RowDesc* __restrict__ rows, const long* __restrict__ rowptr,
                                                   const double* __restrict__ ws, const int* __restrict__ groups,
@@ -122,8 +123,8 @@ __global__ __launch_bounds__(256) void sep_wstage(const RowDesc* __restrict__ ro
                     w = lspace_weight(gi[g], gj[g], k, 1, f, v);
                 } else {
                     const int i = gi[g], j = gj[g], gg = (i + j + k) >> 1;
-                    w = sep_weight(sep_row_factor(i, j, permsix_d(i, j, k), v), sep_l3_factor(k, v),
-                                   f[2 * (gg - i)], f[2 * (gg - j)], f[2 * (gg - k)], f[2 * gg + 1]);
+                    w = sep_weight(sep_row_factor(i, j, permsix_d(i, j, k), v), f[2 * (gg - i)], f[2 * (gg - j)],
+                                   f[2 * (gg - k)], f[2 * gg + 1]);
                 }
             }
         }
@@ -323,11 +324,11 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
     static_assert(WPT * 64 * NW == 16 * G, "a chunk's weights split evenly over the threads");
     __shared__ double Wt[2][16][G];
     __shared__ int gj[G], gk0[G], gnk[G];
-    // one wave, exact h^2: a chunk's table reads come from five 64-entry
-    // windows in LDS (f at A1.., A2.., A3.., 1/((2a+1) f) at A4.., v at every
-    // other l3), loaded by one coalesced read each, double-buffered by chunk
+    // one wave, exact h^2: a chunk's table reads come from four 64-entry
+    // windows in LDS (f at A1.., A2.., A3.., 1/((2a+1) f) at A4..), loaded by
+    // one coalesced read each, double-buffered by chunk
     constexpr bool WIN = !ASYM && NW == 1;
-    __shared__ double Wf[WIN ? 2 : 1][5][64];
+    __shared__ double Wf[WIN ? 2 : 1][4][64];
     // runs of SR_XC consecutive groups (overlapping l3 windows, so shared pair
     // products in L2) are dealt to the 8 XCDs in turn: blocks b, b + 8, ...
     // share an XCD
@@ -395,7 +396,7 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
     double bs[4][CPW];
     // the first three K-steps' basis rows and chunks 0 and 1's windows, in
     // flight while the row descriptors arrive
-    auto wload = [&](int c, double (&wr)[5]) {
+    auto wload = [&](int c, double (&wr)[4]) {
         const int L0 = kmin + 32 * c;
         const int A1 = (jlo + L0 - gi) >> 1, A2 = (gi + L0 - jhi) >> 1, A3 = (gi + jlo - L0 - 30) >> 1,
                   A4 = (gi + jlo + L0) >> 1;
@@ -403,13 +404,12 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
         wr[1] = f[2 * (A2 + lane)];
         wr[2] = f[2 * (A3 + lane)];
         wr[3] = f[2 * (A4 + lane) + 1];
-        wr[4] = sep_l3_factor(L0 + 2 * min(lane, 15), v);
     };
-    auto wstore = [&](int b, const double (&wr)[5]) {
+    auto wstore = [&](int b, const double (&wr)[4]) {
 #pragma unroll
-        for (int q = 0; q < 5; ++q) Wf[b][q][lane] = wr[q];
+        for (int q = 0; q < 4; ++q) Wf[b][q][lane] = wr[q];
     };
-    double wr0[WIN ? 5 : 1], wr1[WIN ? 5 : 1];
+    double wr0[WIN ? 4 : 1], wr1[WIN ? 4 : 1];
     if (nks > 0) {
 #pragma unroll
         for (int x = 0; x < 3; ++x) loadb(min(x, nks - 1), bs[x]);
@@ -440,7 +440,7 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
                  rn = sep_row_factor(li, lj, permsix_d(li, lj, lj + 1), v);
     constexpr int XS = 4 * NW;   // l3 (and table index) step between a thread's elements
     struct Tab {
-        double f1, f2, f3, rg, tk;   // tk: the l3 factor (2 l3 + 1) v[l3]
+        double f1, f2, f3, rg;
         int k;
         bool in;
     };
@@ -475,7 +475,6 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
                 tx.f2 = W[64 + dh + tt];
                 tx.f3 = W[128 + dj + 15 - tt];
                 tx.rg = W[192 + dj + tt];
-                tx.tk = W[256 + tt];
                 tx.k = l30 + XS * x;
                 tx.in = (unsigned)(u0 + 2 * NW * x) < (unsigned)rnk;
             }
@@ -485,7 +484,6 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
             const double* const q2 = f + 2 * (g0 - rj);
             const double* const q3 = f + 2 * (g0 - l30);
             const double* const q4 = f + 2 * g0 + 1;
-            const double* const q5 = v + l30;
 #pragma unroll
             for (int y = 0; y < 2; ++y) {
                 const int x = 2 * gr + y;
@@ -495,7 +493,6 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
                 tx.f3 = q3[-XS * x];
                 tx.rg = q4[XS * x];
                 tx.k = l30 + XS * x;
-                tx.tk = (double)(2 * tx.k + 1) * q5[XS * x];
                 tx.in = (unsigned)(u0 + 2 * NW * x) < (unsigned)rnk;
             }
         }
@@ -509,7 +506,7 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
             if (ASYM) {
                 wv = lspace_weight(tx.in ? li : 0, tx.in ? lj : 0, tx.in ? tx.k : 0, 1, f, v);
             } else {
-                wv = sep_weight(tx.k == lj ? re : rn, tx.tk, tx.f1, tx.f2, tx.f3, tx.rg);
+                wv = sep_weight(tx.k == lj ? re : rn, tx.f1, tx.f2, tx.f3, tx.rg);
             }
             Wt[buf][kk0 + 2 * NW * x][g] = tx.in ? wv : 0.0;
         }
@@ -526,7 +523,7 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
             for (int y = 0; y < 2; ++y) {
                 const int x = 2 * gr + y;
                 const Tab& tx = t[y];
-                Wt[buf][kk0 + 2 * NW * x][g] = sep_weight(rn, tx.tk, tx.f1, tx.f2, tx.f3, tx.rg);
+                Wt[buf][kk0 + 2 * NW * x][g] = sep_weight(rn, tx.f1, tx.f2, tx.f3, tx.rg);
             }
         } else {
             weigh(t, gr, buf);
@@ -559,7 +556,7 @@ __global__ __launch_bounds__(64 * NW, (NW == 1 && CPW == 4) ? 3 : 2) void sep_ro
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) a[ks][rt] = Wt[buf][4 * ks + (lane >> 4)][16 * rt + (lane & 15)];
         // windows of chunk c + 2, stored into this chunk's buffer at its end
-        double wr[WIN ? 5 : 1];
+        double wr[WIN ? 4 : 1];
         if constexpr (WIN) wload(min(c + 2, nch - 1), wr);
         Tab t[2];
         gather(c + 1, 0, t);
@@ -930,6 +927,17 @@ __global__ __launch_bounds__(256) void sep_mgs_step(double* __restrict__ Uc, int
 
 // P[l][a] = Pc[a][l] for a < r, 0 for r <= a < ldu: the compressed basis in
 // the row layout of the pair products
+// The l3 stage's basis table for the exact l-space weight: U3[l] = (2l + 1)
+// v_l U[l] (the weight's l3 factor, sep_l3_factor), zero past the last l.
+__global__ void sep_u3tab(const double* __restrict__ U, int ldu, int nl, const double* __restrict__ v,
+                          double* __restrict__ U3)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)(nl + SR_UPAD) * ldu) return;
+    const int l = (int)(e / ldu);
+    U3[e] = l < nl ? sep_l3_factor(l, v) * U[e] : 0.0;
+}
+
 __global__ void sep_ptab(const double* __restrict__ Pc, int nlp, int nl, int r, int ldu, double* __restrict__ P)
 {
     const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1913,6 +1921,27 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     // l-space weights are formed where they are staged; other point sets stage
     // their walk-order weights
     const int wkind = (L.kspace || L.uniform) ? 2 : (L.h2mode == MCMB_H2_ASYMPTOTIC ? 1 : 0);
+    // the l3 stage reads (2 l3 + 1) v_l3 U(l3) for the exact l-space weight,
+    // which leaves that factor out (sep_weight); U itself otherwise.  Kept for
+    // the same table, setup and width.
+    const double* dU1 = dU;
+    if (wkind == 0) {
+        if (!(c->sg_u3_src == (const void*)dU && c->sg_u3_gen == L.generation && c->sg_u3_ldu == ldu &&
+              c->sg_u3_nl == nl)) {
+            MCMB_TRY(c->sg_U3.alloc(sizeof(double) * (size_t)(nl + SR_UPAD) * ldu));
+            const long e = (long)(nl + SR_UPAD) * ldu;
+            MCMB_TRY(launch_guard("l3 basis table", (unsigned long long)((e + 255) / 256), 256));
+            hipLaunchKernelGGL(sep_u3tab, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, c->stream, dU, ldu, nl,
+                               (const double*)L.d_vp.as<double>(), c->sg_U3.as<double>());
+            MCMB_HIP(hipGetLastError());
+            ++launches;
+            c->sg_u3_src = dU;
+            c->sg_u3_gen = L.generation;
+            c->sg_u3_ldu = ldu;
+            c->sg_u3_nl = nl;
+        }
+        dU1 = c->sg_U3.as<double>();
+    }
     // the MFMA l3 contraction forms l-space weights itself (1, 2 or 4 waves:
     // up to 512 columns); MCMB_SEP_WEIGHTS=staged stages them instead
     const char* wenv = getenv("MCMB_SEP_WEIGHTS");
@@ -1972,18 +2001,18 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
 #define SEP_ROWMOM(NN, GG, FF, HH)                                                                            \
     hipLaunchKernelGGL((sep_rowmom<NN, GG, FF, HH>), dim3((unsigned)T.ngroups), dim3(NN), 0, c->stream,      \
                        tab + T.o_groups, ginfo, wofs, (const double*)c->sg_wd.as<double>(), T.row0, L.kstep,  \
-                       dU, ldu, NB, ldB, c->sg_M.as<double>())
+                       dU1, ldu, NB, ldB, c->sg_M.as<double>())
 #define SEP_ROWMOM_MFMA(CC)                                                                                   \
     hipLaunchKernelGGL((sep_rowmom_mfma<CC, SR_GM / 16>), dim3((unsigned)T.ngroups), dim3(64 * nwv), 0, c->stream,       \
                        tab + T.o_groups, ginfo, wofs, (const double*)c->sg_wd.as<double>(), T.row0, L.kstep,  \
-                       dU, ldu, NB, ldB, c->sg_M.as<double>())
+                       dU1, ldu, NB, ldB, c->sg_M.as<double>())
 #define SEP_ROWMOM_LW(CC, AA, NN, SS)                                                                             \
     hipLaunchKernelGGL((sep_rowmom_lw<CC, AA, NN, SS>),                                                     \
                        dim3((unsigned)(8 * SR_XC * ((T.ngroups + 8 * SR_XC - 1) / (8 * SR_XC)))), dim3(64 * NN), \
                        0, c->stream, tab + T.o_groups, ginfo, (const RowDesc*)L.d_rows.as<RowDesc>(),          \
                        (const double*)L.d_ftp.as<double>() + ftab_guard(L.lmax), (const double*)L.d_vp.as<double>(), \
                        T.row0,                                                                                  \
-                       T.ngroups, dU, ldu, NB, ldB, c->sg_M.as<double>())
+                       T.ngroups, dU1, ldu, NB, ldB, c->sg_M.as<double>())
 #define SEP_LW(CC, NN)                                                                                        \
     do {                                                                                                      \
         if (T.swap) {                                                                                         \

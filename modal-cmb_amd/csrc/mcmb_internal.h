@@ -256,6 +256,11 @@ struct mcmb_ctx {
     mcmb::DevBuf sg_ws, sg_rp, sg_wd, sg_U, sg_M, sg_Y, sg_G, sg_tab;
     mcmb::DevBuf sg_Ys, sg_Gs, sg_off;   // symmetrised Y, symmetrised G, its row offsets
     int sg_off_np = 0;                   // pair count the offsets were built for
+    // the l3 stage's basis table with the exact weight's l3 factor, and what it was built from
+    mcmb::DevBuf sg_U3;
+    const void* sg_u3_src = nullptr;
+    long sg_u3_gen = -1;
+    int sg_u3_ldu = 0, sg_u3_nl = 0;
     mcmb::SepPlan sep;
     mcmb::SepComp sepc;
     // per-launch timing of the last call that records phases (event k+1 - event k)

@@ -46,10 +46,14 @@ __device__ __forceinline__ double lspace_weight(int i, int j, int k, int asym,
 // The exact l-space weight in the separable Gram's l3 contraction, regrouped
 // as a row factor, an l3 factor and the 3j part:
 //   w = [permsix (1/4pi) (2i+1)(2j+1) v_i v_j] [(2k+1) v_k]
-//       [f(g-i) f(g-j)] [f(g-k) / ((2g+1) f(g))],
-// 4 multiplies per point where lspace_weight makes 9.  It differs from
-// lspace_weight by a few ulps (another rounding order); the staged and fused
-// forms of the contraction both use it, so they agree bit for bit.
+//       [f(g-i) f(g-j)] [f(g-k) / ((2g+1) f(g))].
+// The l3 factor (2k+1) v_k depends on k alone, so the contraction takes it
+// with the basis (its l3-stage table holds (2k+1) v_k U(k)), and the weight
+// it forms is the rest: sep_weight = row [f(g-i) f(g-j)] [f(g-k) rg], 3
+// multiplies per point where lspace_weight makes 9.  It differs from
+// lspace_weight by a few ulps (another rounding order); every form of the
+// contraction (staged or fused weights, MFMA or VALU) uses it, so they agree
+// bit for bit.
 __device__ __forceinline__ double sep_row_factor(int i, int j, double pk, const double* __restrict__ v)
 {
     return pk * 0.079577471545947667884 * (double)((2 * i + 1) * (2 * j + 1)) * v[i] * v[j];
@@ -58,9 +62,9 @@ __device__ __forceinline__ double sep_l3_factor(int k, const double* __restrict_
 {
     return (double)(2 * k + 1) * v[k];
 }
-__device__ __forceinline__ double sep_weight(double row, double tk, double f1, double f2, double f3, double rg)
+__device__ __forceinline__ double sep_weight(double row, double f1, double f2, double f3, double rg)
 {
-    return row * (tk * ((f1 * f2) * (f3 * rg)));
+    return row * ((f1 * f2) * (f3 * rg));
 }
 
 // k-space weight of the ordered GL point (a, b, c): permsix w_a w_b w_c / K

@@ -8,21 +8,22 @@
 //
 // Tridiagonalisation (the lower form of LAPACK dsytrd / dlatrd, in panels of
 // TRD_NB columns).  A is kept full and symmetric, row-major; row j stands for
-// column j.  Column j = k0 + i of a panel takes two kernels:
+// column j.  Column j = k0 + i of a panel takes three kernels:
 //   trd_row   x = A[j][j..n) - sum_{p<i} (V_p[c] W_p[j] + W_p[c] V_p[j])
-//             (the panel's earlier reflectors, not yet in A), d_j = x_j,
-//             the reflector of x_{j+1..n): beta = -sign(x_{j+1}) |x|,
-//             tau = (beta - x_{j+1}) / beta, v = x / (x_{j+1} - beta) with
-//             v_{j+1} = 1 (tau = 0 when x_{j+2..n) = 0), e_j = beta, and the
-//             dots V_p^T v, W_p^T v the next kernel needs;
-//   trd_symv  w = tau (A v - V (W^T v) - W (V^T v)) over rows j+1.., then
-//             w += -tau/2 (w^T v) v (kept as alpha2 until the panel ends).
-// Each ends with a reduction over its workgroups: every workgroup writes its
-// partial sums, and the last to finish (an atomic ticket after a device-scope
-// fence) adds them in workgroup order, so results do not depend on timing.
-// No workgroup waits for another.  At the end of a panel the trailing matrix
-// takes A -= V W^T + W V^T on FP64 MFMA (lower tiles, mirrored, so A stays
-// exactly symmetric).
+//             (the panel's earlier reflectors, not yet in A), d_j = x_j, and
+//             per workgroup its part of |x_{j+2..}|^2;
+//   trd_symv  the reflector of x_{j+1..n), formed alike in every workgroup:
+//             beta = -sign(x_{j+1}) |x|, tau = (beta - x_{j+1}) / beta,
+//             v = x / (x_{j+1} - beta) with v_{j+1} = 1 (tau = 0 when
+//             x_{j+2..n) = 0), e_j = beta; then y = A v over rows j+1.. and
+//             per workgroup its parts of V_p^T v, W_p^T v and y^T v;
+//   trd_symv_fin  those parts added: w = tau (y - V (W^T v) - W (V^T v)) +
+//             alpha2 v with alpha2 = -tau/2 (w^T v), kept as y, the two dot
+//             vectors and alpha2 until the next trd_row makes w final.
+// Partial sums are added in workgroup order, so results do not depend on
+// timing, and no workgroup waits for another.  At the end of a panel the
+// trailing matrix takes A -= V W^T + W V^T on FP64 MFMA (lower tiles,
+// mirrored, so A stays exactly symmetric).
 //
 // Back-transformation: per panel, H_k0 ... H_{k0+nb-1} = I - V T V^T (T upper
 // triangular: T_ii = tau_i, T[0:i, i] = -tau_i T[0:i, 0:i] V^T v_i, LAPACK
@@ -51,13 +52,11 @@ typedef double fd4 __attribute__((ext_vector_type(4)));
 
 // Device-side state of the current column and panel.
 struct TrdState {
-    double scale;               // 1 / (x_{j+1} - beta), 0 when tau = 0
     double tau;
     double alpha;               // x_{j+1}
     double c1[TRD_NB];          // V_p^T v
     double c2[TRD_NB];          // W_p^T v
     double alpha2[TRD_NB];      // w_p += alpha2[p] v_p, p of the panel
-    unsigned int ticket[2];
 };
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -94,19 +93,18 @@ __device__ __forceinline__ double trd_wfin(const double* __restrict__ Vp, const 
 
 // Column j = k0 + i.  W_{i-1} is made final on columns j+1.. (column j itself
 // stays as trd_symv left it: nothing reads it again), then the row update
-// x = A[j][j..) - sum_{p<i} (V_p W_p[j] + W_p V_p[j]), d_j, and |x_{j+2..}|^2
-// summed over the workgroups; the last workgroup forms the reflector.
+// x = A[j][j..) - sum_{p<i} (V_p W_p[j] + W_p V_p[j]), d_j, and each
+// workgroup's part of |x_{j+2..}|^2 (trd_symv adds them and forms the
+// reflector).
 // Vp, Wp: the panel's rows (V_p at Vp + p n, W_p at Wp + p n).
 __global__ __launch_bounds__(TRD_ROWT) void trd_row(const double* __restrict__ A, int lda, int n, int j, int i,
                                                    const double* __restrict__ Vp, double* __restrict__ Wp,
                                                    const double* __restrict__ tt, double* __restrict__ xbuf,
-                                                   double* __restrict__ dd, double* __restrict__ ee,
-                                                   double* __restrict__ tto, TrdState* __restrict__ st,
+                                                   double* __restrict__ dd, TrdState* __restrict__ st,
                                                    double* __restrict__ part)
 {
     __shared__ double sV[TRD_NB], sW[TRD_NB], c1[TRD_NB], c2[TRD_NB];
     __shared__ double red[TRD_ROWT / 64];
-    __shared__ int last;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int q = i - 1;   // the column whose W becomes final here (none when i = 0)
     double tq = 0.0, a2q = 0.0;
@@ -170,34 +168,6 @@ __global__ __launch_bounds__(TRD_ROWT) void trd_row(const double* __restrict__ A
         for (int k = 0; k < TRD_ROWT / 64; ++k) s += red[k];
         part[blockIdx.x] = s;
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) last = atomicAdd(&st->ticket[0], 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    if (w == 0) {
-        double s = 0.0;
-        for (unsigned b = lane; b < gridDim.x; b += 64) s += part[b];
-        s = wave_sum(s);
-        if (lane == 0) red[0] = s;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const double ss = red[0];
-        const double alpha = st->alpha;
-        double tau = 0.0, beta = alpha, scale = 0.0;
-        if (ss > 0.0) {
-            beta = -copysign(sqrt(alpha * alpha + ss), alpha);
-            tau = (beta - alpha) / beta;
-            scale = 1.0 / (alpha - beta);
-        }
-        ee[j] = beta;
-        tto[j] = tau;
-        st->tau = tau;
-        st->scale = scale;
-        st->ticket[0] = 0;
-    }
 }
 
 // y = A v on rows j+1.. (the trailing matrix as of the panel's start; v in
@@ -212,15 +182,37 @@ __global__ __launch_bounds__(TRD_ROWT) void trd_row(const double* __restrict__ A
 __global__ __launch_bounds__(TRD_SYMT) void trd_symv(const double* __restrict__ A, int lda, int n, int j, int i,
                                                     const double* __restrict__ xbuf, const double* __restrict__ Vp,
                                                     double* __restrict__ Wp, double* __restrict__ Vj,
-                                                    TrdState* __restrict__ st, double* __restrict__ part)
+                                                    const double* __restrict__ rpart, int nr, double* __restrict__ ee,
+                                                    double* __restrict__ tt, TrdState* __restrict__ st,
+                                                    double* __restrict__ part)
 {
     constexpr int NWV = TRD_SYMT / 64, NQ = 2 * TRD_NB + 1, RW = 4, DP = 4;
     extern __shared__ double vs[];
     __shared__ double red[NWV][NQ];
-    __shared__ double tot[NQ];
-    __shared__ int last;
+    __shared__ double rs;
     const int m = n - j - 1, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const double scale = st->scale;
+    if (w == 0) {
+        double s = 0.0;
+        for (int b = lane; b < nr; b += 64) s += rpart[b];
+        s = wave_sum(s);
+        if (lane == 0) {
+            const double alpha = st->alpha;
+            double tau = 0.0, beta = alpha, scale = 0.0;
+            if (s > 0.0) {
+                beta = -copysign(sqrt(alpha * alpha + s), alpha);
+                tau = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+            rs = scale;
+            if (blockIdx.x == 0) {
+                ee[j] = beta;
+                tt[j] = tau;
+                st->tau = tau;
+            }
+        }
+    }
+    __syncthreads();
+    const double scale = rs;
     for (int e = tid; e < m; e += TRD_SYMT) vs[e] = e == 0 ? 1.0 : scale * xbuf[j + 1 + e];
     __syncthreads();
     const int peel = (j + 1) & 1, npair = (m - peel) >> 1, tail = (m - peel) & 1;
@@ -303,16 +295,22 @@ __global__ __launch_bounds__(TRD_SYMT) void trd_symv(const double* __restrict__ 
         for (int k = 0; k < NWV; ++k) s += red[k][tid];
         part[(long)tid * TRD_SYMG + blockIdx.x] = s;
     }
-    __threadfence();
-    __syncthreads();
-    if (tid == 0) last = atomicAdd(&st->ticket[1], 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // wave w adds the partials q = w, w + NWV, ...: lane-strided sums, then the wave tree
-    for (int qq = w; qq < nq; qq += NWV) {
+}
+
+// trd_symv's nb workgroup partials added (wave w takes the values q = w, w +
+// 8, ..: lane-strided sums in workgroup order, then the wave tree): c1, c2
+// and alpha2_i = -tau/2 w^T v with w^T v = tau (y^T v - 2 c1 . c2) (w = tau
+// (y - V c2 - W c1) is made final by the next trd_row).
+__global__ __launch_bounds__(512) void trd_symv_fin(int i, int nb, const double* __restrict__ part,
+                                                   TrdState* __restrict__ st)
+{
+    constexpr int NQ = 2 * TRD_NB + 1;
+    __shared__ double tot[NQ];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int nq = 1 + 2 * i;
+    for (int qq = w; qq < nq; qq += 8) {
         double s = 0.0;
-        for (unsigned b = lane; b < gridDim.x; b += 64) s += part[(long)qq * TRD_SYMG + b];
+        for (int b = lane; b < nb; b += 64) s += part[(long)qq * TRD_SYMG + b];
         s = wave_sum(s);
         if (lane == 0) tot[qq] = s;
     }
@@ -326,7 +324,6 @@ __global__ __launch_bounds__(TRD_SYMT) void trd_symv(const double* __restrict__ 
         for (int p = 0; p < i; ++p) cc = fma(tot[1 + p], tot[1 + i + p], cc);
         const double tau = st->tau;
         st->alpha2[i] = -0.5 * tau * (tau * (tot[0] - 2.0 * cc));
-        st->ticket[1] = 0;
     }
 }
 
@@ -634,7 +631,7 @@ int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* 
     const size_t nn = (size_t)n * n;
     const int lda = n + (n & 1);   // even: 16-byte aligned column pairs in every row
     const int npan = std::max(1, (n - 1 + TRD_NB - 1) / TRD_NB);
-    DevBuf A, V, Wp, xb, dd, ee, tt, st, part, E, Vt, M, perm, info;
+    DevBuf A, V, Wp, xb, dd, ee, tt, st, part, rpart, E, Vt, M, perm, info;
     MCMB_TRY(A.alloc(sizeof(double) * (size_t)n * lda));
     MCMB_TRY(V.alloc(sizeof(double) * nn));
     MCMB_TRY(Wp.alloc(sizeof(double) * TRD_NB * (size_t)n));
@@ -643,7 +640,8 @@ int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* 
     MCMB_TRY(ee.alloc(sizeof(double) * n));
     MCMB_TRY(tt.alloc(sizeof(double) * n));
     MCMB_TRY(st.alloc(sizeof(TrdState)));
-    MCMB_TRY(part.alloc(sizeof(double) * (2 * TRD_NB + 1) * std::max(TRD_SYMG, (n + TRD_ROWT - 1) / TRD_ROWT)));
+    MCMB_TRY(part.alloc(sizeof(double) * (2 * TRD_NB + 1) * TRD_SYMG));
+    MCMB_TRY(rpart.alloc(sizeof(double) * ((n + TRD_ROWT - 1) / TRD_ROWT)));
     MCMB_TRY(info.alloc(sizeof(int)));
     MCMB_HIP(hipMemsetAsync(V.p, 0, sizeof(double) * nn, c->stream));
     MCMB_HIP(hipMemsetAsync(Wp.p, 0, sizeof(double) * TRD_NB * (size_t)n, c->stream));
@@ -663,12 +661,15 @@ int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* 
             const int j = k0 + i, m = n - j - 1;
             const unsigned g1 = (unsigned)((n - j + TRD_ROWT - 1) / TRD_ROWT);
             hipLaunchKernelGGL(trd_row, dim3(g1), dim3(TRD_ROWT), 0, c->stream, (const double*)Ad, lda, n, j, i, Vp,
-                               Wp.as<double>(), (const double*)tt.as<double>(), xb.as<double>(), dd.as<double>(),
-                               ee.as<double>(), tt.as<double>(), sd, part.as<double>());
+                               Wp.as<double>(), (const double*)tt.as<double>(), xb.as<double>(), dd.as<double>(), sd,
+                               rpart.as<double>());
             const unsigned g2 = (unsigned)std::min(TRD_SYMG, (m + 4 * (TRD_SYMT / 64) - 1) / (4 * (TRD_SYMT / 64)));
             hipLaunchKernelGGL(trd_symv, dim3(g2), dim3(TRD_SYMT), sizeof(double) * (size_t)m, c->stream,
                                (const double*)Ad, lda, n, j, i, (const double*)xb.as<double>(), Vp, Wp.as<double>(),
-                               V.as<double>() + (size_t)j * n, sd, part.as<double>());
+                               V.as<double>() + (size_t)j * n, (const double*)rpart.as<double>(), (int)g1,
+                               ee.as<double>(), tt.as<double>(), sd, part.as<double>());
+            hipLaunchKernelGGL(trd_symv_fin, dim3(1), dim3(512), 0, c->stream, i, (int)g2,
+                               (const double*)part.as<double>(), sd);
         }
         if (k1 < n) {
             hipLaunchKernelGGL(trd_wfinal, dim3((unsigned)((n - k1 + 255) / 256)), dim3(256), 0, c->stream,

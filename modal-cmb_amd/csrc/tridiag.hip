@@ -91,6 +91,28 @@ __device__ __forceinline__ double trd_wfin(const double* __restrict__ Vp, const 
     return tau * s + a2 * Vp[(long)i * n + c];
 }
 
+// trd_symv's nb workgroup partials of column i's 1 + 2 i values (y^T v, then
+// V_p^T v and W_p^T v) added in a fixed order by 256 threads: value q by the
+// four threads 4 q .. 4 q + 3, each over the workgroups b = its index mod 4,
+// then the four in a fixed pairing.  tot[q] (LDS) for q < 1 + 2 i; alpha2 =
+// -tau/2 w^T v with w^T v = tau (y^T v - 2 c1 . c2).
+__device__ __forceinline__ void trd_sum_parts(int i, int nb, const double* __restrict__ part, int tid, double* tot)
+{
+    const int q = tid >> 2, sub = tid & 3;
+    double s = 0.0;
+    if (q < 1 + 2 * i)
+        for (int b = sub; b < nb; b += 4) s += part[(long)q * TRD_SYMG + b];
+    s += __shfl_xor(s, 1);
+    s += __shfl_xor(s, 2);
+    if (q < 1 + 2 * i && sub == 0) tot[q] = s;
+}
+__device__ __forceinline__ double trd_alpha2(int i, double tau, const double* tot)
+{
+    double cc = 0.0;
+    for (int p = 0; p < i; ++p) cc = fma(tot[1 + p], tot[1 + i + p], cc);
+    return -0.5 * tau * (tau * (tot[0] - 2.0 * cc));
+}
+
 // Column j = k0 + i.  W_{i-1} is made final on columns j+1.. (column j itself
 // stays as trd_symv left it: nothing reads it again), then the row update
 // x = A[j][j..) - sum_{p<i} (V_p W_p[j] + W_p V_p[j]), d_j, and each
@@ -101,24 +123,27 @@ __global__ __launch_bounds__(TRD_ROWT) void trd_row(const double* __restrict__ A
                                                    const double* __restrict__ Vp, double* __restrict__ Wp,
                                                    const double* __restrict__ tt, double* __restrict__ xbuf,
                                                    double* __restrict__ dd, TrdState* __restrict__ st,
+                                                   const double* __restrict__ spart, int snb,
                                                    double* __restrict__ part)
 {
-    __shared__ double sV[TRD_NB], sW[TRD_NB], c1[TRD_NB], c2[TRD_NB];
+    static_assert(TRD_ROWT == 256, "trd_sum_parts takes 256 threads");
+    __shared__ double sV[TRD_NB], sW[TRD_NB];
+    __shared__ double tot[2 * TRD_NB + 1];
     __shared__ double red[TRD_ROWT / 64];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int q = i - 1;   // the column whose W becomes final here (none when i = 0)
-    double tq = 0.0, a2q = 0.0;
-    if (q >= 0) {
-        tq = tt[j - 1];
-        a2q = st->alpha2[q];
-        if (tid < q) {
-            c1[tid] = st->c1[tid];
-            c2[tid] = st->c2[tid];
-        }
-    }
+    // its sums, from trd_symv's partials (the same in every workgroup)
+    if (q >= 0) trd_sum_parts(q, snb, spart, tid, tot);
     if (tid < i) sV[tid] = Vp[(long)tid * n + j];
     if (tid < q) sW[tid] = Wp[(long)tid * n + j];
     __syncthreads();
+    const double* const c1 = tot + 1;
+    const double* const c2 = tot + 1 + (q > 0 ? q : 0);
+    double tq = 0.0, a2q = 0.0;
+    if (q >= 0) {
+        tq = tt[j - 1];
+        a2q = trd_alpha2(q, tq, tot);
+    }
     if (q >= 0 && w == 0) {
         // W_q[j] final: lane p < q adds its correction term, then the wave tree
         double t = 0.0;
@@ -297,34 +322,21 @@ __global__ __launch_bounds__(TRD_SYMT) void trd_symv(const double* __restrict__ 
     }
 }
 
-// trd_symv's nb workgroup partials added (wave w takes the values q = w, w +
-// 8, ..: lane-strided sums in workgroup order, then the wave tree): c1, c2
-// and alpha2_i = -tau/2 w^T v with w^T v = tau (y^T v - 2 c1 . c2) (w = tau
-// (y - V c2 - W c1) is made final by the next trd_row).
-__global__ __launch_bounds__(512) void trd_symv_fin(int i, int nb, const double* __restrict__ part,
-                                                   TrdState* __restrict__ st)
+// The sums of a panel's last column (the next trd_row, which adds them itself
+// for the other columns, belongs to the next panel): c1, c2 and alpha2 into
+// the state for trd_wfinal.  tt: tau of the column.
+__global__ __launch_bounds__(256) void trd_symv_fin(int i, int nb, const double* __restrict__ part,
+                                                   TrdState* __restrict__ st, const double* __restrict__ tt)
 {
-    constexpr int NQ = 2 * TRD_NB + 1;
-    __shared__ double tot[NQ];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int nq = 1 + 2 * i;
-    for (int qq = w; qq < nq; qq += 8) {
-        double s = 0.0;
-        for (int b = lane; b < nb; b += 64) s += part[(long)qq * TRD_SYMG + b];
-        s = wave_sum(s);
-        if (lane == 0) tot[qq] = s;
-    }
+    __shared__ double tot[2 * TRD_NB + 1];
+    const int tid = threadIdx.x;
+    trd_sum_parts(i, nb, part, tid, tot);
     __syncthreads();
     if (tid < i) {
         st->c1[tid] = tot[1 + tid];
         st->c2[tid] = tot[1 + i + tid];
     }
-    if (tid == 0) {
-        double cc = 0.0;
-        for (int p = 0; p < i; ++p) cc = fma(tot[1 + p], tot[1 + i + p], cc);
-        const double tau = st->tau;
-        st->alpha2[i] = -0.5 * tau * (tau * (tot[0] - 2.0 * cc));
-    }
+    if (tid == 0) st->alpha2[i] = trd_alpha2(i, tt[0], tot);
 }
 
 // W_q final (q = nb - 1, the panel's last column) on columns [c0, n).
@@ -654,6 +666,7 @@ int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* 
     MCMB_HIP(hipGetLastError());
     double* const Ad = A.as<double>();
     TrdState* const sd = reinterpret_cast<TrdState*>(st.p);
+    int prev_g2 = 0;   // workgroups of the last trd_symv (its partials feed the next trd_row)
     for (int k0 = 0; k0 <= n - 2; k0 += TRD_NB) {
         const int nb = std::min(TRD_NB, n - 1 - k0), k1 = k0 + nb;
         const double* const Vp = V.as<double>() + (size_t)k0 * n;
@@ -662,14 +675,16 @@ int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* 
             const unsigned g1 = (unsigned)((n - j + TRD_ROWT - 1) / TRD_ROWT);
             hipLaunchKernelGGL(trd_row, dim3(g1), dim3(TRD_ROWT), 0, c->stream, (const double*)Ad, lda, n, j, i, Vp,
                                Wp.as<double>(), (const double*)tt.as<double>(), xb.as<double>(), dd.as<double>(), sd,
-                               rpart.as<double>());
+                               (const double*)part.as<double>(), prev_g2, rpart.as<double>());
             const unsigned g2 = (unsigned)std::min(TRD_SYMG, (m + 4 * (TRD_SYMT / 64) - 1) / (4 * (TRD_SYMT / 64)));
             hipLaunchKernelGGL(trd_symv, dim3(g2), dim3(TRD_SYMT), sizeof(double) * (size_t)m, c->stream,
                                (const double*)Ad, lda, n, j, i, (const double*)xb.as<double>(), Vp, Wp.as<double>(),
                                V.as<double>() + (size_t)j * n, (const double*)rpart.as<double>(), (int)g1,
                                ee.as<double>(), tt.as<double>(), sd, part.as<double>());
-            hipLaunchKernelGGL(trd_symv_fin, dim3(1), dim3(512), 0, c->stream, i, (int)g2,
-                               (const double*)part.as<double>(), sd);
+            prev_g2 = (int)g2;
+            if (i == nb - 1)
+                hipLaunchKernelGGL(trd_symv_fin, dim3(1), dim3(256), 0, c->stream, i, (int)g2,
+                                   (const double*)part.as<double>(), sd, (const double*)(tt.as<double>() + j));
         }
         if (k1 < n) {
             hipLaunchKernelGGL(trd_wfinal, dim3((unsigned)((n - k1 + 255) / 256)), dim3(256), 0, c->stream,

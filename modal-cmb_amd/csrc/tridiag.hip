@@ -132,7 +132,19 @@ __global__ __launch_bounds__(TRD_ROWT) void trd_row(const double* __restrict__ A
     __shared__ double red[TRD_ROWT / 64];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int q = i - 1;   // the column whose W becomes final here (none when i = 0)
-    // its sums, from trd_symv's partials (the same in every workgroup)
+    // this thread's column: its loads issued first, in flight during the sums below
+    const int c = j + blockIdx.x * TRD_ROWT + tid;
+    const bool cin = c < n;
+    double x = cin ? A[(long)j * lda + c] : 0.0;
+    double vp[TRD_NB - 1], wp[TRD_NB - 1];
+#pragma unroll
+    for (int p = 0; p < TRD_NB - 1; ++p) {
+        vp[p] = (cin && p < q) ? Vp[(long)p * n + c] : 0.0;
+        wp[p] = (cin && p < q) ? Wp[(long)p * n + c] : 0.0;
+    }
+    const double vq = (cin && q >= 0) ? Vp[(long)q * n + c] : 0.0;
+    const double wqr = (cin && q >= 0) ? Wp[(long)q * n + c] : 0.0;
+    // column q's sums, from trd_symv's partials (the same in every workgroup)
     if (q >= 0) trd_sum_parts(q, snb, spart, tid, tot);
     if (tid < i) sV[tid] = Vp[(long)tid * n + j];
     if (tid < q) sW[tid] = Wp[(long)tid * n + j];
@@ -152,18 +164,8 @@ __global__ __launch_bounds__(TRD_ROWT) void trd_row(const double* __restrict__ A
         if (lane == 0) sW[q] = tq * (Wp[(long)q * n + j] - t) + a2q * sV[q];
     }
     __syncthreads();
-    const int c = j + blockIdx.x * TRD_ROWT + tid;
-    double x = 0.0;
-    if (c < n) {
-        x = A[(long)j * lda + c];
-        // one pass over the final W_p, V_p (p < q): the row update and W_q's
-        // correction (unrolled with guards: every load issued before the sums)
-        double vp[TRD_NB - 1], wp[TRD_NB - 1];
-#pragma unroll
-        for (int p = 0; p < TRD_NB - 1; ++p) {
-            vp[p] = p < q ? Vp[(long)p * n + c] : 0.0;
-            wp[p] = p < q ? Wp[(long)p * n + c] : 0.0;
-        }
+    if (cin) {
+        // one pass over the final W_p, V_p (p < q): the row update and W_q's correction
         double corr = 0.0;
 #pragma unroll
         for (int p = 0; p < TRD_NB - 1; ++p) {
@@ -173,8 +175,7 @@ __global__ __launch_bounds__(TRD_ROWT) void trd_row(const double* __restrict__ A
             }
         }
         if (q >= 0) {
-            const double vq = Vp[(long)q * n + c];
-            const double wq = tq * (Wp[(long)q * n + c] - corr) + a2q * vq;
+            const double wq = tq * (wqr - corr) + a2q * vq;
             if (c > j) Wp[(long)q * n + c] = wq;
             x -= vq * sW[q] + wq * sV[q];
         }
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(TRD_ROWT) void trd_row(const double* __restrict__ A
         if (c == j + 1) st->alpha = x;
         if (c >= j + 1) xbuf[c] = x;
     }
-    const double xs = (c < n && c >= j + 2) ? x : 0.0;
+    const double xs = (cin && c >= j + 2) ? x : 0.0;
     {
         const double s = wave_sum(xs * xs);
         if (lane == 0) red[w] = s;

@@ -54,10 +54,8 @@ int set_device(mcmb_ctx* ctx)
     return MCMB_OK;
 }
 
-constexpr size_t STAGE_BYTES = (size_t)16 << 20;   // one half of the pinned staging
-
 // Allocate the pinned staging halves and their events on first use.
-static int stage_ready(mcmb_ctx* c)
+int stage_ready(mcmb_ctx* c)
 {
     if (c->stage) return MCMB_OK;
     MCMB_HIP(hipHostMalloc(&c->stage, 2 * STAGE_BYTES, hipHostMallocDefault));

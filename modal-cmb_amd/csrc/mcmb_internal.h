@@ -81,6 +81,14 @@ struct DevBuf {
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Work buffers of the tridiagonal eigensolver (tridiag.hip, dc_tridiag.hip),
+// kept by the context between calls: about 7 n^2 doubles after a
+// decomposition of order n, freed with the context.
+struct TrdWork {
+    DevBuf A, V, Wp, xb, dd, ee, tt, st, part, rpart, E, Vt, M, perm, info;
+    DevBuf Qb, Ab, Ub, zb, up, org, tau, zh;   // divide and conquer
+};
+
 // Weight of a tetrapyd point.
 enum WeightKind : int { W_LSPACE_EXACT = 0, W_LSPACE_ASYM = 1, W_KSPACE_GL = 2 };
 
@@ -281,6 +289,7 @@ struct mcmb_ctx {
     // rocBLAS handle of the tridiagonal eigensolver's divide and conquer
     // step (rocsolver_dstedc), created on first use
     void* rb_handle = nullptr;
+    mcmb::TrdWork trd;
 };
 
 namespace mcmb {
@@ -332,13 +341,23 @@ int copy_to_host(mcmb_ctx* ctx, void* host, const void* dev, size_t bytes);
 // The reverse, host -> device (4 MiB and more staged: host threads fill one
 // half while the other is copied to the device).
 int copy_to_device(mcmb_ctx* ctx, void* dev, const void* host, size_t bytes);
+// The context's pinned staging (2 x STAGE_BYTES at ctx->stage), allocated on
+// first use.  Callers other than the two copies above may use it between
+// their own stream synchronisations.
+constexpr size_t STAGE_BYTES = (size_t)16 << 20;
+int stage_ready(mcmb_ctx* ctx);
 // Symmetric eigen-decomposition by Householder tridiagonalisation (n up to
 // 8000): evals (host) descending, d_V rows the sign-fixed eigenvectors; with
 // normalise, of D^-1 G D^-1 (d_norm[i] = sqrt G_ii).  tridiag_release frees
-// the context's rocBLAS handle.
+// the context's work buffers (TrdWork) and rocBLAS handle (made only when
+// MCMB_TRIDIAG=rocsolver).
 bool tridiag_supported(int n);
 int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* d_norm, double* evals, DevBuf& d_V);
 void tridiag_release(mcmb_ctx* c);
+// Eigenpairs of the symmetric tridiagonal matrix with diagonal d and
+// off-diagonal e (host arrays of n and n - 1) by divide and conquer: w (host)
+// ascending, dQ (device, column-major n x n) column k the eigenvector of w[k].
+int dc_tridiag(mcmb_ctx* c, int n, const double* d, const double* e, double* dQ, double* w);
 // Per-point weights of P (walk order) into ws, scaled = sqrt(w)/6 if nonzero;
 // rowptr[r] = first point of row r (built and uploaded unless rowptr_ready).
 // Weights by the point set's kind.

@@ -3,7 +3,8 @@
 // ortho_prim_mpi.c:223-224, gsl_eigen_symmv + sort descending).
 //
 //   A = Q T Q^T (T tridiagonal, Q = H_0 ... H_{n-2}, H_j = I - tau_j v_j v_j^T)
-//   T = Z diag(lambda) Z^T   (rocSOLVER dstedc, divide and conquer)
+//   T = Z diag(lambda) Z^T   (divide and conquer, dc_tridiag.hip; rocSOLVER
+//                             dstedc when MCMB_TRIDIAG=rocsolver)
 //   eigenvectors X = Q Z, applied as blocks of reflectors.
 //
 // Tridiagonalisation (the lower form of LAPACK dsytrd / dlatrd, in panels of
@@ -627,24 +628,61 @@ static const RocSolver& rocsolver_lib()
     return R;
 }
 
+// The reduction to tridiagonal form after trd_init: every column's and
+// panel's launches on stream st_, then the last diagonal entry.
+static int trd_reduce(TrdWork& W, int n, int lda, hipStream_t st_)
+{
+    DevBuf &A = W.A, &V = W.V, &Wp = W.Wp, &xb = W.xb, &dd = W.dd, &ee = W.ee, &tt = W.tt, &st = W.st,
+           &part = W.part, &rpart = W.rpart;
+    double* const Ad = A.as<double>();
+    TrdState* const sd = reinterpret_cast<TrdState*>(st.p);
+    int prev_g2 = 0;   // workgroups of the last trd_symv (its partials feed the next trd_row)
+    for (int k0 = 0; k0 <= n - 2; k0 += TRD_NB) {
+        const int nb = std::min(TRD_NB, n - 1 - k0), k1 = k0 + nb;
+        const double* const Vp = V.as<double>() + (size_t)k0 * n;
+        for (int i = 0; i < nb; ++i) {
+            const int j = k0 + i, m = n - j - 1;
+            const unsigned g1 = (unsigned)((n - j + TRD_ROWT - 1) / TRD_ROWT);
+            hipLaunchKernelGGL(trd_row, dim3(g1), dim3(TRD_ROWT), 0, st_, (const double*)Ad, lda, n, j, i, Vp,
+                               Wp.as<double>(), (const double*)tt.as<double>(), xb.as<double>(), dd.as<double>(), sd,
+                               (const double*)part.as<double>(), prev_g2, rpart.as<double>());
+            const unsigned g2 = (unsigned)std::min(TRD_SYMG, (m + 4 * (TRD_SYMT / 64) - 1) / (4 * (TRD_SYMT / 64)));
+            hipLaunchKernelGGL(trd_symv, dim3(g2), dim3(TRD_SYMT), sizeof(double) * (size_t)m, st_,
+                               (const double*)Ad, lda, n, j, i, (const double*)xb.as<double>(), Vp, Wp.as<double>(),
+                               V.as<double>() + (size_t)j * n, (const double*)rpart.as<double>(), (int)g1,
+                               ee.as<double>(), tt.as<double>(), sd, part.as<double>());
+            prev_g2 = (int)g2;
+            if (i == nb - 1)
+                hipLaunchKernelGGL(trd_symv_fin, dim3(1), dim3(256), 0, st_, i, (int)g2,
+                                   (const double*)part.as<double>(), sd, (const double*)(tt.as<double>() + j));
+        }
+        if (k1 < n) {
+            hipLaunchKernelGGL(trd_wfinal, dim3((unsigned)((n - k1 + 255) / 256)), dim3(256), 0, st_,
+                               Wp.as<double>(), Vp, n, nb - 1, k1, (const double*)(tt.as<double>() + k1 - 1),
+                               (const TrdState*)sd);
+            const int T = (n - k1 + 63) / 64;
+            hipLaunchKernelGGL(trd_syr2k, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, st_, Ad, lda, n, k1, Vp,
+                               (const double*)Wp.as<double>(), nb);
+        }
+        // alpha2 of the next panel starts from zero
+        MCMB_HIP(hipMemsetAsync(sd->alpha2, 0, sizeof(double) * TRD_NB, st_));
+    }
+    hipLaunchKernelGGL(trd_lastd, dim3(1), dim3(64), 0, st_, (const double*)Ad, lda, n, dd.as<double>());
+    MCMB_HIP(hipGetLastError());
+    return MCMB_OK;
+}
+
 bool tridiag_supported(int n) { return n >= 1 && n <= TRD_MAXN; }
 
 int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* d_norm, double* evals, DevBuf& d_V)
 {
     if (!tridiag_supported(n)) return fail(MCMB_ERR_ARG, "tridiagonal eigensolver: n = %d outside 1..%d", n, TRD_MAXN);
-    const RocSolver& RS = rocsolver_lib();
-    if (!RS.ok) return fail(MCMB_ERR_HIP, "tridiagonal eigensolver: cannot load rocBLAS/rocSOLVER (%s)", RS.err);
-    if (!c->rb_handle) {
-        rocblas_handle h = nullptr;
-        MCMB_TRY(rb_status(RS.create_handle(&h), "rocblas_create_handle"));
-        c->rb_handle = h;
-    }
-    rocblas_handle h = static_cast<rocblas_handle>(c->rb_handle);
-    MCMB_TRY(rb_status(RS.set_stream(h, c->stream), "rocblas_set_stream"));
     const size_t nn = (size_t)n * n;
     const int lda = n + (n & 1);   // even: 16-byte aligned column pairs in every row
     const int npan = std::max(1, (n - 1 + TRD_NB - 1) / TRD_NB);
-    DevBuf A, V, Wp, xb, dd, ee, tt, st, part, rpart, E, Vt, M, perm, info;
+    TrdWork& W = c->trd;
+    DevBuf &A = W.A, &V = W.V, &Wp = W.Wp, &xb = W.xb, &dd = W.dd, &ee = W.ee, &tt = W.tt, &st = W.st,
+           &part = W.part, &rpart = W.rpart, &E = W.E, &Vt = W.Vt, &M = W.M, &perm = W.perm, &info = W.info;
     MCMB_TRY(A.alloc(sizeof(double) * (size_t)n * lda));
     MCMB_TRY(V.alloc(sizeof(double) * nn));
     MCMB_TRY(Wp.alloc(sizeof(double) * TRD_NB * (size_t)n));
@@ -665,53 +703,41 @@ int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* 
     hipLaunchKernelGGL(trd_init, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, d_G, n, lda, normalise,
                        d_norm, A.as<double>());
     MCMB_HIP(hipGetLastError());
-    double* const Ad = A.as<double>();
-    TrdState* const sd = reinterpret_cast<TrdState*>(st.p);
-    int prev_g2 = 0;   // workgroups of the last trd_symv (its partials feed the next trd_row)
-    for (int k0 = 0; k0 <= n - 2; k0 += TRD_NB) {
-        const int nb = std::min(TRD_NB, n - 1 - k0), k1 = k0 + nb;
-        const double* const Vp = V.as<double>() + (size_t)k0 * n;
-        for (int i = 0; i < nb; ++i) {
-            const int j = k0 + i, m = n - j - 1;
-            const unsigned g1 = (unsigned)((n - j + TRD_ROWT - 1) / TRD_ROWT);
-            hipLaunchKernelGGL(trd_row, dim3(g1), dim3(TRD_ROWT), 0, c->stream, (const double*)Ad, lda, n, j, i, Vp,
-                               Wp.as<double>(), (const double*)tt.as<double>(), xb.as<double>(), dd.as<double>(), sd,
-                               (const double*)part.as<double>(), prev_g2, rpart.as<double>());
-            const unsigned g2 = (unsigned)std::min(TRD_SYMG, (m + 4 * (TRD_SYMT / 64) - 1) / (4 * (TRD_SYMT / 64)));
-            hipLaunchKernelGGL(trd_symv, dim3(g2), dim3(TRD_SYMT), sizeof(double) * (size_t)m, c->stream,
-                               (const double*)Ad, lda, n, j, i, (const double*)xb.as<double>(), Vp, Wp.as<double>(),
-                               V.as<double>() + (size_t)j * n, (const double*)rpart.as<double>(), (int)g1,
-                               ee.as<double>(), tt.as<double>(), sd, part.as<double>());
-            prev_g2 = (int)g2;
-            if (i == nb - 1)
-                hipLaunchKernelGGL(trd_symv_fin, dim3(1), dim3(256), 0, c->stream, i, (int)g2,
-                                   (const double*)part.as<double>(), sd, (const double*)(tt.as<double>() + j));
-        }
-        if (k1 < n) {
-            hipLaunchKernelGGL(trd_wfinal, dim3((unsigned)((n - k1 + 255) / 256)), dim3(256), 0, c->stream,
-                               Wp.as<double>(), Vp, n, nb - 1, k1, (const double*)(tt.as<double>() + k1 - 1),
-                               (const TrdState*)sd);
-            const int T = (n - k1 + 63) / 64;
-            hipLaunchKernelGGL(trd_syr2k, dim3((unsigned)(T * (T + 1) / 2)), dim3(256), 0, c->stream, Ad, lda, n, k1, Vp,
-                               (const double*)Wp.as<double>(), nb);
-        }
-        // alpha2 of the next panel starts from zero
-        MCMB_HIP(hipMemsetAsync(sd->alpha2, 0, sizeof(double) * TRD_NB, c->stream));
-    }
-    hipLaunchKernelGGL(trd_lastd, dim3(1), dim3(64), 0, c->stream, (const double*)Ad, lda, n, dd.as<double>());
-    MCMB_HIP(hipGetLastError());
+    MCMB_TRY(trd_reduce(W, n, lda, c->stream));
     // eigenpairs of T: eigenvalues ascending in dd, eigenvectors as the
     // columns of a column-major n x n matrix, i.e. the rows of E
     MCMB_TRY(E.alloc(sizeof(double) * nn));
-    hipLaunchKernelGGL(trd_identity, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, E.as<double>(), n);
-    MCMB_HIP(hipGetLastError());
-    MCMB_TRY(rb_status(RS.dstedc(h, rocblas_evect_tridiagonal, n, dd.as<double>(), ee.as<double>(), E.as<double>(),
-                                  n, info.as<int>()),
-                       "rocsolver_dstedc"));
-    int inf = 0;
-    MCMB_HIP(hipMemcpyAsync(&inf, info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    MCMB_HIP(hipStreamSynchronize(c->stream));
-    if (inf != 0) return fail(MCMB_ERR_ARG, "tridiagonal eigensolver (dstedc) failed: info = %d at n = %d", inf, n);
+    std::vector<double> w(n);
+    const char* solver = getenv("MCMB_TRIDIAG");
+    if (solver && !strcmp(solver, "rocsolver")) {
+        const RocSolver& RS = rocsolver_lib();
+        if (!RS.ok) return fail(MCMB_ERR_HIP, "tridiagonal eigensolver: cannot load rocBLAS/rocSOLVER (%s)", RS.err);
+        if (!c->rb_handle) {
+            rocblas_handle h = nullptr;
+            MCMB_TRY(rb_status(RS.create_handle(&h), "rocblas_create_handle"));
+            c->rb_handle = h;
+        }
+        rocblas_handle h = static_cast<rocblas_handle>(c->rb_handle);
+        MCMB_TRY(rb_status(RS.set_stream(h, c->stream), "rocblas_set_stream"));
+        hipLaunchKernelGGL(trd_identity, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, c->stream, E.as<double>(), n);
+        MCMB_HIP(hipGetLastError());
+        MCMB_TRY(rb_status(RS.dstedc(h, rocblas_evect_tridiagonal, n, dd.as<double>(), ee.as<double>(),
+                                      E.as<double>(), n, info.as<int>()),
+                           "rocsolver_dstedc"));
+        int inf = 0;
+        MCMB_HIP(hipMemcpyAsync(&inf, info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipMemcpyAsync(w.data(), dd.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        if (inf != 0) return fail(MCMB_ERR_ARG, "tridiagonal eigensolver (dstedc) failed: info = %d at n = %d", inf, n);
+    } else if (solver && strcmp(solver, "dc")) {
+        return fail(MCMB_ERR_ARG, "MCMB_TRIDIAG=%s: expected dc or rocsolver", solver);
+    } else {
+        std::vector<double> dh(n), eh(n);
+        MCMB_HIP(hipMemcpyAsync(dh.data(), dd.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipMemcpyAsync(eh.data(), ee.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+        MCMB_HIP(hipStreamSynchronize(c->stream));
+        MCMB_TRY(dc_tridiag(c, n, dh.data(), eh.data(), E.as<double>(), w.data()));
+    }
     // back-transformation, panels from the last to the first
     if (n >= 2) {
         MCMB_TRY(Vt.alloc(sizeof(double) * nn));
@@ -729,10 +755,7 @@ int tridiag_eigen(mcmb_ctx* c, int n, const double* d_G, int normalise, double* 
         }
         MCMB_HIP(hipGetLastError());
     }
-    // descending order: eigenvalue k of the output is dd[n - 1 - k]
-    std::vector<double> w(n);
-    MCMB_HIP(hipMemcpyAsync(w.data(), dd.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
-    MCMB_HIP(hipStreamSynchronize(c->stream));
+    // descending order
     std::vector<int> pm(n);
     std::iota(pm.begin(), pm.end(), 0);
     std::stable_sort(pm.begin(), pm.end(), [&](int a, int b) { return w[a] > w[b]; });
@@ -752,6 +775,7 @@ void tridiag_release(mcmb_ctx* c)
     // a handle exists only if the library was loaded
     if (c->rb_handle) (void)rocsolver_lib().destroy_handle(static_cast<rocblas_handle>(c->rb_handle));
     c->rb_handle = nullptr;
+    c->trd = TrdWork();
 }
 
 }  // namespace mcmb

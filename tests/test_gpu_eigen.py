@@ -1,5 +1,5 @@
-"""GPU: FP64 eigen-decomposition (Householder tridiagonalisation, block and
-scalar Jacobi) and the late-side orthonormalisation with its PCA fallback,
+"""GPU: FP64 eigen-decomposition (Householder tridiagonalisation with the
+own divide and conquer or rocSOLVER's, block and scalar Jacobi) and the late-side orthonormalisation with its PCA fallback,
 against LAPACK (numpy eigh) and the oracle."""
 import os
 
@@ -19,20 +19,25 @@ def _sym(n, seed):
     return 0.5 * (a + a.T)
 
 
-@pytest.fixture(params=["tridiagonal", "block", "scalar"])
+def _setenv(name, value):
+    if value is None:
+        os.environ.pop(name, None)
+    else:
+        os.environ[name] = value
+
+
+@pytest.fixture(params=["tridiagonal", "tridiagonal-rocsolver", "block", "scalar"])
 def eig_mode(request):
-    """Householder tridiagonalisation (the default), block Jacobi
+    """Householder tridiagonalisation with the own divide and conquer (the
+    default) or with rocSOLVER's (MCMB_TRIDIAG=rocsolver), block Jacobi
     (MCMB_EIGEN=block) or the scalar Jacobi (MCMB_EIGEN=scalar)."""
-    old = os.environ.get("MCMB_EIGEN")
-    if request.param == "tridiagonal":
-        os.environ.pop("MCMB_EIGEN", None)
-    else:
-        os.environ["MCMB_EIGEN"] = request.param
+    old = os.environ.get("MCMB_EIGEN"), os.environ.get("MCMB_TRIDIAG")
+    tri = request.param.startswith("tridiagonal")
+    _setenv("MCMB_EIGEN", None if tri else request.param)
+    _setenv("MCMB_TRIDIAG", "rocsolver" if request.param == "tridiagonal-rocsolver" else None)
     yield request.param
-    if old is None:
-        os.environ.pop("MCMB_EIGEN", None)
-    else:
-        os.environ["MCMB_EIGEN"] = old
+    _setenv("MCMB_EIGEN", old[0])
+    _setenv("MCMB_TRIDIAG", old[1])
 
 
 @pytest.mark.parametrize("n", [1, 2, 3, 8, 33, 64, 257, 600])
@@ -53,7 +58,7 @@ def test_eigen_sym_against_lapack(ctx, n, eig_mode):
         sep = min(gaps[k - 1] if k > 0 else np.inf, gaps[k] if k < n - 1 else np.inf)
         if sep > 1e-4 * nrm:
             assert abs(abs(V[:, k] @ Wl[:, k]) - 1.0) < 1e-9
-    if eig_mode == "tridiagonal":
+    if eig_mode.startswith("tridiagonal"):
         assert sweeps == 0
     else:
         assert 0 < sweeps < 20 or n == 1
@@ -95,6 +100,72 @@ def test_eigen_sym_large_against_lapack(ctx, n):
     Y = V @ X
     assert np.abs(A @ Y - V @ (ev[:, None] * X)).max() < 1e-12 * nrm * np.sqrt(n) * 8
     assert np.abs(V.T @ Y - X).max() < 1e-12 * np.sqrt(n) * 8
+
+
+def _wilkinson(m):
+    """W_{2m+1}^+: diagonal |m - i|, unit off-diagonal (pairs of eigenvalues
+    that agree to many digits)."""
+    n = 2 * m + 1
+    return np.diag(np.abs(np.arange(n) - m).astype(float)) + np.eye(n, k=1) + np.eye(n, k=-1)
+
+
+def _hard_spectra():
+    rng = np.random.default_rng(5)
+    out = {"wilkinson101": _wilkinson(50)}
+    # twelve W_21^+ glued by 1e-10 off-diagonals: clusters of 12 eigenvalues
+    # within ~1e-10 of each other
+    W21 = _wilkinson(10)
+    G = np.zeros((252, 252))
+    for b in range(12):
+        G[21 * b:21 * b + 21, 21 * b:21 * b + 21] = W21
+        if b:
+            G[21 * b, 21 * b - 1] = G[21 * b - 1, 21 * b] = 1e-10
+    out["glued_wilkinson252"] = G
+    out["identity300"] = np.eye(300)
+    # two tight clusters around 1 and 2 in a random basis
+    Q, _ = np.linalg.qr(rng.standard_normal((400, 400)))
+    lam = np.r_[np.ones(200), 2 * np.ones(200)] + 1e-13 * rng.standard_normal(400)
+    out["clusters400"] = 0.5 * ((Q * lam) @ Q.T + ((Q * lam) @ Q.T).T)
+    return out
+
+
+@pytest.mark.parametrize("name", ["wilkinson101", "glued_wilkinson252", "identity300", "clusters400"])
+def test_eigen_sym_tridiagonal_hard_spectra(ctx, name):
+    """Spectra that exercise the divide and conquer's deflation: eigenvalue
+    pairs closer than the tolerance (a Givens rotation deflates one of
+    each), whole clusters and a multiple eigenvalue (the rank-one
+    update's weights vanish).  Vectors orthonormal and accurate to
+    working precision whatever the gaps."""
+    A = _hard_spectra()[name]
+    n = A.shape[0]
+    for solver in ("dc", "rocsolver"):
+        old = os.environ.get("MCMB_TRIDIAG")
+        os.environ["MCMB_TRIDIAG"] = solver
+        try:
+            ev, V, sweeps = ctx.eigen_sym(A)
+        finally:
+            _setenv("MCMB_TRIDIAG", old)
+        ref = np.linalg.eigvalsh(A)[::-1]
+        nrm = np.abs(ref).max()
+        assert sweeps == 0
+        assert np.all(np.diff(ev) <= 0)
+        assert np.abs(ev - ref).max() < 1e-14 * nrm * 16, solver
+        assert np.abs(A @ V - V * ev).max() < 1e-14 * nrm * 16 * np.sqrt(n), solver
+        assert np.abs(V.T @ V - np.eye(n)).max() < 1e-14 * 16 * np.sqrt(n), solver
+
+
+def test_eigen_sym_repeated_calls_bitwise_equal(ctx):
+    """The work buffers stay with the context between calls, and a call with
+    another order reallocates them; every call gives the same bits (the
+    reductions add their partial sums in a fixed order)."""
+    A, B = _sym(600, 11), _sym(257, 12)
+    ref = ctx.eigen_sym(A)
+    runs = [ctx.eigen_sym(A), ctx.eigen_sym(B), ctx.eigen_sym(B), ctx.eigen_sym(A)]
+    for r in (runs[0], runs[3]):
+        assert np.array_equal(r[0], ref[0]) and np.array_equal(r[1], ref[1])
+    assert np.array_equal(runs[1][0], runs[2][0]) and np.array_equal(runs[1][1], runs[2][1])
+    ev, V = runs[1][0], runs[1][1]
+    assert np.abs(B @ V - V * ev).max() < 1e-12 * np.abs(ev).max() * np.sqrt(257)
 
 
 def test_eigen_sym_matches_oracle_vectors(ctx):

@@ -414,7 +414,9 @@ int mcmb_device_copy(mcmb_ctx* ctx, void* dst, const void* src, size_t bytes);
  * products or weights, l3 / l2 / l1 contractions, expansion, assembly),
  * each between HIP events on the context stream.  Writes up to cap entries of
  * ms[] and names[] (static strings) and returns the phase count (0 if the last
- * such call recorded none).  No reference counterpart. */
+ * such call recorded none; with MCMB_PHASES=0 in the environment no call
+ * records them, which saves about 25 us per Gram at 4960 modes).  No
+ * reference counterpart. */
 int mcmb_last_phases(mcmb_ctx* ctx, int cap, double* ms, const char** names);
 /* Host wall time (ms) of the parts of the last separable Gram call, in call
  * order: basis factorisation, the row tables' parts (row runs and blocks, row

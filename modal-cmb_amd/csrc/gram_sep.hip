@@ -1006,18 +1006,33 @@ __global__ void sep_assemble(const double* __restrict__ Gs, const long* __restri
 // output, computed where x <= y), over 32 x 32 tiles (bm <= bn) of Gamma:
 // for each matching sigma the sorted triple x <= y <= z of its pairs is read.
 // Entries m <= nn are computed; the tile and (off the diagonal) its mirror
-// are written from LDS along rows, so Gamma is exactly symmetric.
+// are written from LDS along rows, so Gamma is exactly symmetric.  The reads
+// of G are gathers (about 0.6 distinct 128-byte lines per entry of a tile at
+// 4960 modes), so neighbouring tiles are kept on one XCD (S x S blocks, see
+// below) to share its L2.
+template <int S>
 __global__ __launch_bounds__(256) void sep_assemble_sym(const double* __restrict__ G, long ldg, int R,
                                                         const int* __restrict__ modes, int n, int nt,
                                                         double* __restrict__ gamma)
 {
     __shared__ double T[32][33];
-    int bm = 0, rem = blockIdx.x;   // tile (bm <= bn) in row-major upper-triangle order
-    while (rem >= nt - bm) {
-        rem -= nt - bm;
-        ++bm;
+    // tile (bm <= bn) of this workgroup: S x S blocks of tiles, blocks in
+    // row-major upper-triangle order, block k on XCD k % 8 with its S^2 tiles
+    // in consecutive dispatch slots there (workgroups b, b + 8, ... share an
+    // XCD), so the tiles that read largely the same lines of G run together
+    // and share that XCD's L2
+    const int nb = (nt + S - 1) / S;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    int rem = (slot / (S * S)) * 8 + xcd;
+    const int sub = slot % (S * S);
+    if (rem >= nb * (nb + 1) / 2) return;
+    int sm = 0;
+    while (rem >= nb - sm) {
+        rem -= nb - sm;
+        ++sm;
     }
-    const int bn = bm + rem;
+    const int bm = S * sm + sub / S, bn = S * (sm + rem) + sub % S;
+    if (bm > bn || bn >= nt) return;
     const int c = threadIdx.x & 31, r0 = threadIdx.x >> 5;
     const int nn = 32 * bn + c;
     const int pn = nn < n ? modes[nn] : 0;
@@ -1044,16 +1059,18 @@ __global__ __launch_bounds__(256) void sep_assemble_sym(const double* __restrict
         T[r][c] = acc;
     }
     __syncthreads();
+    // Gamma is written once and not read here: streaming stores keep the
+    // caches for G
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int r = r0 + 8 * i, m = 32 * bm + r;
-        if (m < n && nn < n) gamma[(long)m * n + nn] = m <= nn ? T[r][c] : T[c][r];
+        if (m < n && nn < n) __builtin_nontemporal_store(m <= nn ? T[r][c] : T[c][r], &gamma[(long)m * n + nn]);
     }
     if (bm != bn) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int r = r0 + 8 * i, row = 32 * bn + r, col = 32 * bm + c;
-            if (row < n && col < n) gamma[(long)row * n + col] = T[c][r];
+            if (row < n && col < n) __builtin_nontemporal_store(T[c][r], &gamma[(long)row * n + col]);
         }
     }
 }
@@ -1833,7 +1850,10 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     // phases: [basis compression], pair products, [weights], per block the
     // three contractions (times summed over blocks), [expansion], assembly
     int np = 0;   // phases started; phase k runs from event k to event k + 1
+    const char* penv = getenv("MCMB_PHASES");
+    const bool phases = !(penv && !strcmp(penv, "0"));
     auto phase = [&](const char* name) -> int {
+        if (!phases) return MCMB_OK;
         c->phase_names[np] = name;
         MCMB_HIP(hipEventRecord(c->evp[np], c->stream));
         ++np;
@@ -2123,10 +2143,13 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
     MCMB_TRY(phase("assembly"));
     if (comp) {
         // the 6-term sums straight from the expanded tensor, by 32 x 32 tiles
-        const int nt = (n + 31) / 32;
-        const unsigned long long ntile = (unsigned long long)nt * (nt + 1) / 2;
+        // blocks of 6 x 6 tiles per XCD (assembly 0.205 -> 0.146 ms at 4960 modes;
+        // 2 x 2: 0.165, 4 x 4: 0.152, 8 x 8: 0.147)
+        constexpr int S = 6;
+        const int nt = (n + 31) / 32, nbk = (nt + S - 1) / S;
+        const unsigned long long ntile = ((unsigned long long)nbk * (nbk + 1) / 2 + 7) / 8 * 8 * S * S;
         MCMB_TRY(launch_guard("assembly", ntile, 256));
-        hipLaunchKernelGGL(sep_assemble_sym, dim3((unsigned)ntile), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL(sep_assemble_sym<S>, dim3((unsigned)ntile), dim3(256), 0, c->stream,
                            (const double*)c->sg_G.as<double>(), ldp, R, L.d_modes.as<int>(), n, nt, d_gamma);
         MCMB_HIP(hipGetLastError());
         ++launches;
@@ -2144,7 +2167,7 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
         MCMB_HIP(hipGetLastError());
         launches += 2;
     }
-    MCMB_HIP(hipEventRecord(c->evp[np], c->stream));
+    if (np > 0) MCMB_HIP(hipEventRecord(c->evp[np], c->stream));
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     setup_done("launch to completion");

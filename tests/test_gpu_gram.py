@@ -251,6 +251,14 @@ def test_separable_phases_and_repeat_calls(ctx):
     part, _ = ctx.gamma_tet_slice(modalcmb.FP64, 1, 3)
     assert np.array_equal(ctx.gamma_tet(modalcmb.FP64), G1)
     assert [p for p, _ in ctx.last_phases()] == body + ["expansion to pairs", "assembly"]
+    # without the per-phase events: the same bits, no phases, the call still timed
+    os.environ["MCMB_PHASES"] = "0"
+    try:
+        assert np.array_equal(ctx.gamma_tet(modalcmb.FP64), G1)
+        assert ctx.last_phases() == []
+        assert ctx.last_kernel_ms()[0] > 0
+    finally:
+        del os.environ["MCMB_PHASES"]
     os.environ["MCMB_SEP_COMPRESS"] = "0"
     try:
         Gp = ctx.gamma_tet(modalcmb.FP64)

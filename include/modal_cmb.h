@@ -151,7 +151,10 @@ int mcmb_gamma_gl_pol(mcmb_ctx* ctx, int pol, int npt, double* gamma_host, doubl
  * MiB of tables) are rejected with MCMB_ERR_ARG.  Every call on a GPU shares
  * that GPU's array, so calls that overlap on one GPU (other threads, other
  * contexts on the same device) race; the call returns when its kernels have
- * finished.  No reference counterpart: for callers that cannot allocate
+ * finished.  It always takes the direct node sum (mcmb_gamma_gl with
+ * MCMB_GL_PATH=direct, bit for bit), since the compressed pair basis of the
+ * default path is built in allocated memory; the two paths agree to
+ * rounding.  No reference counterpart: for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, harnesses
  * that forbid hipMalloc). */
 int mcmb_gamma_gl_noalloc(mcmb_ctx* ctx, int npt, double* gamma_dev);
@@ -424,11 +427,12 @@ int mcmb_last_phases(mcmb_ctx* ctx, int cap, double* ms, const char** names);
  * (each only when the call built it), launch to completion and the whole
  * call; returns the count. */
 int mcmb_last_setup(mcmb_ctx* ctx, int cap, double* ms, const char** names);
-/* *rank = the basis columns the last FP64 separable Gram contracted over: r
- * when the pair products q_x q_y had numerical rank r <= NP / 2 (NP =
- * R(R+1)/2) and were replaced by an orthonormal basis of their span, 0 when
- * the NP pair products themselves were contracted, -1 when the last Gram took
- * another contraction order.  No reference counterpart. */
+/* *rank = the basis columns the last FP64 separable or Gauss-Legendre Gram
+ * contracted over: r when the pair products q_x q_y had numerical rank r <=
+ * NP / 2 (NP = R(R+1)/2) and were replaced by an orthonormal basis of their
+ * span, 0 when the NP pair products themselves were contracted (for the GL
+ * Gram: the direct node sum), -1 when the last Gram took another contraction
+ * order.  No reference counterpart. */
 int mcmb_last_basis_rank(mcmb_ctx* ctx, int* rank);
 
 /* ---- reference-named entry points (legacy shims over the bound context) --- */

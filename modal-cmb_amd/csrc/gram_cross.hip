@@ -861,8 +861,9 @@ static int cross_gl_launch(mcmb_ctx* c, int pol, int spline, int npt, double* ds
         if (hi[f] > X.lq)
             return fail(MCMB_ERR_ARG, "GL cross-Gamma: late lmax %d beyond the q-tilde multipoles (%d)", hi[f], X.lq);
     }
-    std::vector<double> mu(npt), w(npt), cw;
-    MCMB_TRY(mcmb_gauss_legendre(npt, mu.data(), w.data()));
+    std::vector<double> cw;
+    const double *mu, *w;
+    MCMB_TRY(gl_nodes(c, npt, &mu, &w));
     MCMB_TRY(spline_weights(X.xvec, spline, cw));
     // l weights: TTw = (2l+1)^(5/6) b_TT / sqrt(b_TT^2 C_TT + N_TT), EEw =
     // (2l+1)^(5/6) b_EE / sqrt(s1 (s1 s4 - s3^2)) (calculate_gamma.c:73-84,
@@ -886,8 +887,8 @@ static int cross_gl_launch(mcmb_ctx* c, int pol, int spline, int npt, double* ds
     MCMB_TRY(d_mu.alloc(sizeof(double) * npt));
     MCMB_TRY(d_w.alloc(sizeof(double) * npt));
     MCMB_TRY(d_cw.alloc(sizeof(double) * nx));
-    MCMB_HIP(hipMemcpyAsync(d_mu.p, mu.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipMemcpyAsync(d_w.p, w.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(d_mu.p, mu, sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(d_w.p, w, sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
     MCMB_HIP(hipMemcpyAsync(d_cw.p, cw.data(), sizeof(double) * nx, hipMemcpyHostToDevice, c->stream));
     for (int f = 0; f < 2; ++f) {
         if (R[f] == 0) continue;

@@ -227,6 +227,26 @@ static int gl_run(mcmb_ctx* c, int npt, const double* x, const double* w, const 
     MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
     c->last_launches = 5;
+    c->last_rank = 0;   // the direct node sum over the pair products
+    return MCMB_OK;
+}
+
+int gl_nodes(mcmb_ctx* c, int n, const double** x, const double** w)
+{
+    if (n < 1) return fail(MCMB_ERR_ARG, "Gauss-Legendre rule of order %d", n);
+    if ((int)c->gl_x.size() != n) {
+        c->gl_x.assign(n, 0.0);
+        c->gl_w.assign(n, 0.0);
+        // roots dealt round-robin in blocks of 16 (the cost per root is even)
+        const int nr = (n + 1) / 2, nth = std::max(1, std::min(host_threads(), (nr + 15) / 16));
+        double* const xs = c->gl_x.data();
+        double* const ws = c->gl_w.data();
+        run_threads(nth, [&](int t) {
+            for (int b = 16 * t; b < nr; b += 16 * nth) mcmb_gl_roots(n, b, std::min(nr, b + 16), xs, ws);
+        });
+    }
+    *x = c->gl_x.data();
+    *w = c->gl_w.data();
     return MCMB_OK;
 }
 
@@ -244,11 +264,17 @@ int gram_gl_launch(mcmb_ctx* c, int npt, double* d_gamma)
     MCMB_TRY(gl_check(c, &npt));
     const int n = L.n, R = L.R, nl = L.lmax + 1, npair = R * (R + 1) / 2;
     const int T = (n + 63) / 64;
-    std::vector<double> x(npt), w(npt);
-    MCMB_TRY(mcmb_gauss_legendre(npt, x.data(), w.data()));
+    const double *x, *w;
+    MCMB_TRY(gl_nodes(c, npt, &x, &w));
+    // the pair products of Legendre-like bases have a low-rank basis: the
+    // node sum then runs over it (r^3 core instead of a permanent per mode
+    // pair and node)
+    bool used = false;
+    MCMB_TRY(gram_gl_compressed(c, npt, x, w, d_gamma, &used));
+    if (used) return MCMB_OK;
     DevBuf buf;
     MCMB_TRY(buf.alloc(gl_bytes(npt, nl, npair, T * (T + 1) / 2)));
-    return gl_run(c, npt, x.data(), w.data(), gl_carve(buf.as<char>(), npt, nl, npair), d_gamma);
+    return gl_run(c, npt, x, w, gl_carve(buf.as<char>(), npt, nl, npair), d_gamma);
 }
 
 // ---- polarised components (calculate_orthol result[1..3]) -----------------
@@ -369,8 +395,8 @@ int gram_gl_pol_launch(mcmb_ctx* c, int pol, int npt, double* d_gamma)
     const GlField F[2] = {{L.lmin, L.lmax, L.ldq, R[0], L.d_q.as<double>(), L.d_v.as<double>()},
                           {E.lmin, E.lmax, E.ldq, R[1], E.d_q.as<double>(), E.d_v.as<double>()}};
     const int n = E.n, T = (n + 63) / 64, ntiles = T * (T + 1) / 2;
-    std::vector<double> x(npt), w(npt);
-    MCMB_TRY(mcmb_gauss_legendre(npt, x.data(), w.data()));
+    const double *x, *w;
+    MCMB_TRY(gl_nodes(c, npt, &x, &w));
     // x, w, then per used field Pt, U, M; then the tile list
     size_t bytes = 2 * round256(sizeof(double) * npt) + round256(sizeof(int2) * (size_t)ntiles);
     for (int f = 0; f < 2; ++f)
@@ -388,8 +414,8 @@ int gram_gl_pol_launch(mcmb_ctx* c, int pol, int npt, double* d_gamma)
     p += round256(sizeof(double) * npt);
     int2* dtiles = reinterpret_cast<int2*>(p);
     p += round256(sizeof(int2) * (size_t)ntiles);
-    MCMB_HIP(hipMemcpyAsync(dx, x.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipMemcpyAsync(dw, w.data(), sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(dx, x, sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(dw, w, sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(gl_tiles, dim3((ntiles + 255) / 256), dim3(256), 0, c->stream, T, dtiles);
     const double* Mf[2] = {nullptr, nullptr};

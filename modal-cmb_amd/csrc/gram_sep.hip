@@ -1825,6 +1825,177 @@ static int separable_ptab(mcmb_ctx* c, int ldu)
     return MCMB_OK;
 }
 
+// ---- Gauss-Legendre Gram over the compressed pair basis ---------------------
+//
+// The GL-factorised Gram (calculate_orthol, orthogonalisation_late.c:239-336)
+// is Gamma_mn = sum_s w_s / (48 pi) perm[M_{m_a n_b}(x_s)] with M_p(x) =
+// sum_l (2l+1) v_l P_l(x) U[l][p] over the pair products U.  With U = P Ct
+// (the compressed basis of the separable Gram), M_p(x_s) = sum_a Ct[a][p]
+// Phi_a(s), Phi_a(s) = sum_l P[l][a] (2l+1) v_l P_l(x_s), so the node sum
+// becomes the core H[g][a][b] = (3 / 4 pi) sum_s w_s Phi_g Phi_a Phi_b (r^3
+// entries instead of a permanent per mode pair and node), expanded to pairs
+// and assembled exactly as the separable Gram's core (whose assembly takes
+// 1/36 of the six matchings: 36 / (48 pi) = 3 / (4 pi)).
+
+// Phi[a][s] (row stride npt) for a < r: thread (node s, group of GL_AG
+// columns) runs the Legendre recurrence over l and accumulates its columns.
+constexpr int GL_AG = 8;
+__global__ __launch_bounds__(256) void sep_gl_phi(const double* __restrict__ Pc, int nlp, int nl, int r,
+                                                  const double* __restrict__ x, int npt, int lmin,
+                                                  const double* __restrict__ v, double* __restrict__ Phi)
+{
+    const int s = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int a0 = GL_AG * (blockIdx.y * 4 + (threadIdx.x >> 6));
+    if (a0 >= r) return;   // the whole wave
+    const double xs = s < npt ? x[s] : 0.0;
+    double acc[GL_AG];
+#pragma unroll
+    for (int k = 0; k < GL_AG; ++k) acc[k] = 0.0;
+    double p0 = 1.0, p1 = xs;   // P_{l-2}, P_{l-1}
+    for (int l = 0; l < nl; ++l) {
+        double pl;
+        if (l == 0) pl = 1.0;
+        else if (l == 1) pl = xs;
+        else {
+            pl = ((2.0 * l - 1.0) * xs * p1 - (l - 1.0) * p0) / l;
+            p0 = p1;
+            p1 = pl;
+        }
+        if (l < lmin) continue;
+        const double t = (2.0 * l + 1.0) * v[l] * pl;
+#pragma unroll
+        for (int k = 0; k < GL_AG; ++k)
+            if (a0 + k < r) acc[k] = fma(Pc[(long)(a0 + k) * nlp + l], t, acc[k]);
+    }
+    if (s < npt)
+#pragma unroll
+        for (int k = 0; k < GL_AG; ++k)
+            if (a0 + k < r) Phi[(long)(a0 + k) * npt + s] = acc[k];
+}
+
+// H[g][a r + b] = H[g][b r + a] = scale sum_s w_s Phi_g(s) Phi_a(s) Phi_b(s),
+// one workgroup per a <= b: t = w Phi_a Phi_b in LDS chunks of GL_SC nodes,
+// wave q sums the columns g = q, q + 4, ... over its lanes (fixed order).
+constexpr int GL_SC = 1024;
+constexpr int GL_GW = 16;   // columns g per wave (r <= 64)
+__global__ __launch_bounds__(256) void sep_gl_core(const double* __restrict__ Phi, int npt,
+                                                   const double* __restrict__ w, int r, double scale,
+                                                   double* __restrict__ H, long ldh)
+{
+    __shared__ double ts[GL_SC];
+    int a = 0, rem = blockIdx.x;   // pair (a, b), a <= b, row-major upper triangle
+    while (rem >= r - a) {
+        rem -= r - a;
+        ++a;
+    }
+    const int b = a + rem;
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    double acc[GL_GW];
+#pragma unroll
+    for (int k = 0; k < GL_GW; ++k) acc[k] = 0.0;
+    for (int s0 = 0; s0 < npt; s0 += GL_SC) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < GL_SC; e += 256) {
+            const int s = s0 + e;
+            ts[e] = s < npt ? w[s] * Phi[(long)a * npt + s] * Phi[(long)b * npt + s] : 0.0;
+        }
+        __syncthreads();
+        const int ns = min(GL_SC, npt - s0);
+#pragma unroll
+        for (int k = 0; k < GL_GW; ++k) {
+            const int g = q + 4 * k;
+            if (g < r)
+                for (int e = lane; e < ns; e += 64) acc[k] = fma(Phi[(long)g * npt + s0 + e], ts[e], acc[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < GL_GW; ++k) {
+        const int g = q + 4 * k;
+        double t = acc[k];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o);
+        if (g < r && lane == 0) {
+            H[(long)g * ldh + (long)a * r + b] = scale * t;
+            H[(long)g * ldh + (long)b * r + a] = scale * t;
+        }
+    }
+}
+
+int gram_gl_compressed(mcmb_ctx* c, int npt, const double* x, const double* w, double* d_gamma, bool* used)
+{
+    *used = false;
+    PointSet& L = c->ls;
+    const char* genv = getenv("MCMB_GL_PATH");
+    if (genv && !strcmp(genv, "direct")) return MCMB_OK;
+    if (!gram_separable_supported(L, MCMB_FP64)) return MCMB_OK;
+    const int n = L.n, R = L.R, NP = R * (R + 1) / 2;
+    const int ld = NP + (NP & 1);
+    const int nl = (int)(L.q.size() / (size_t)L.ldq);
+    std::vector<int2> pxy(NP);
+    for (int a = 0; a < R; ++a)
+        for (int b = a; b < R; ++b) pxy[pair_index(a, b, R)] = make_int2(a, b);
+    MCMB_TRY(separable_compress(c, L, nl, NP, ld, pxy));
+    const SepComp& S = c->sepc;
+    const int r = S.r;
+    if (r <= 0 || r > 4 * GL_GW) return MCMB_OK;   // full rank: the direct node sum
+    const long ldh = (long)r * r + ((r * r) & 1);
+    const long NPP = (long)NP * (NP + 1) / 2, ldp = NPP + (NPP & 1);
+    MCMB_TRY(c->gl_xw.alloc(sizeof(double) * (2 * (size_t)npt + (size_t)r * npt)));
+    double* const dx = c->gl_xw.as<double>();
+    double* const dw = dx + npt;
+    double* const dphi = dw + npt;
+    MCMB_TRY(c->sg_M.alloc(sizeof(double) * (size_t)(ldh * ld)));
+    MCMB_TRY(c->sg_Ys.alloc(sizeof(double) * (size_t)((long)r * ldp)));
+    MCMB_TRY(c->sg_G.alloc(sizeof(double) * (size_t)((long)NP * ldp)));
+    MCMB_HIP(hipMemcpyAsync(dx, x, sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(dw, w, sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(sep_gl_phi, dim3((unsigned)((npt + 63) / 64), (unsigned)((r + 4 * GL_AG - 1) / (4 * GL_AG))),
+                       dim3(256), 0, c->stream, (const double*)S.pc.as<double>(), S.nlp, nl, r, (const double*)dx, npt,
+                       L.lmin, L.d_v.as<double>(), dphi);
+    MCMB_HIP(hipMemsetAsync(S.h.p, 0, sizeof(double) * (size_t)r * ldh, c->stream));
+    hipLaunchKernelGGL(sep_gl_core, dim3((unsigned)(r * (r + 1) / 2)), dim3(256), 0, c->stream,
+                       (const double*)dphi, npt, (const double*)dw, r, 3.0 / (4.0 * M_PI), S.h.as<double>(), ldh);
+    MCMB_HIP(hipGetLastError());
+    // the expansion and the assembly of the separable Gram
+    const int* tb = S.tab.as<int>();
+    double* const T1 = c->sg_M.as<double>();
+    double* const Y2s = c->sg_Ys.as<double>();
+    double* const G = c->sg_G.as<double>();
+    MCMB_TRY(launch_guard("GL expansion", sep_blocks(S.nt[0], S.gs[0]), 256));
+    hipLaunchKernelGGL((sep_gemm<true, 4, SG_FM, SG_FN>), dim3(sep_blocks(S.nt[0], S.gs[0])), dim3(256), 0, c->stream,
+                       (const double*)S.h.as<double>(), (int)ldh, tb, (const double*)S.ct.as<double>(), (long)ld, T1,
+                       (long)ld, r * r, NP, reinterpret_cast<const SepTask*>(tb + S.o_task[0]),
+                       reinterpret_cast<const int4*>(tb + S.o_tile[0]), S.nt[0], S.gs[0], 0);
+    MCMB_TRY(launch_guard("GL expansion", sep_blocks(S.nt[1], S.gs[1]), 256));
+    hipLaunchKernelGGL((sep_gemm<true, 4, SG_FM, SG_FN, true>), dim3(sep_blocks(S.nt[1], S.gs[1])), dim3(256), 0,
+                       c->stream, (const double*)S.ct.as<double>(), ld, tb, (const double*)T1, (long)ld, Y2s, ldp, NP,
+                       NP, reinterpret_cast<const SepTask*>(tb + S.o_task[1]),
+                       reinterpret_cast<const int4*>(tb + S.o_tile[1]), S.nt[1], S.gs[1], 0);
+    MCMB_TRY(launch_guard("GL expansion", sep_blocks(S.nt[2], S.gs[2]), 256));
+    hipLaunchKernelGGL((sep_gemm<true, 4, SG_FM, SG_FN>), dim3(sep_blocks(S.nt[2], S.gs[2])), dim3(256), 0, c->stream,
+                       (const double*)S.ct.as<double>(), ld, tb, (const double*)Y2s, (long)ldp, G, (long)ldp, NP,
+                       (int)NPP, reinterpret_cast<const SepTask*>(tb + S.o_task[2]),
+                       reinterpret_cast<const int4*>(tb + S.o_tile[2]), S.nt[2], S.gs[2], 0);
+    constexpr int SB = 6;   // blocks of SB x SB assembly tiles per XCD
+    const int nt = (n + 31) / 32, nbk = (nt + SB - 1) / SB;
+    const unsigned long long ntile = ((unsigned long long)nbk * (nbk + 1) / 2 + 7) / 8 * 8 * SB * SB;
+    MCMB_TRY(launch_guard("GL assembly", ntile, 256));
+    hipLaunchKernelGGL(sep_assemble_sym<SB>, dim3((unsigned)ntile), dim3(256), 0, c->stream, (const double*)G, ldp, R,
+                       L.d_modes.as<int>(), n, nt, d_gamma);
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipEventRecord(c->ev1, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));   // x and w may live on the caller's stack
+    float ms = 0.f;
+    MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = 7;
+    c->nphases = 0;
+    c->last_rank = r;
+    *used = true;
+    return MCMB_OK;
+}
+
 int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gamma, int row_lo, int row_hi)
 {
     if (!gram_separable_supported(L, precision))

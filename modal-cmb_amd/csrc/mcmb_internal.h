@@ -236,6 +236,9 @@ template <class F> void run_threads(int n, F f)
 
 }  // namespace mcmb
 
+// Roots i0 <= i < i1 of the Gauss-Legendre rule of order n (quadrature.c).
+extern "C" void mcmb_gl_roots(int n, int i0, int i1, double* x, double* w);
+
 struct mcmb_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -290,7 +293,18 @@ struct mcmb_ctx {
     // step (rocsolver_dstedc), created on first use
     void* rb_handle = nullptr;
     mcmb::TrdWork trd;
+    // the GL Gram over the compressed basis: node table and Phi
+    mcmb::DevBuf gl_xw;
+    // the last Gauss-Legendre rule the Grams asked for (gl_nodes)
+    std::vector<double> gl_x, gl_w;
 };
+
+namespace mcmb {
+// The Gauss-Legendre rule of order n, kept by the context (the last order
+// asked for) and computed on host threads; *x, *w stay valid until the next
+// call with another order.
+int gl_nodes(mcmb_ctx* ctx, int n, const double** x, const double** w);
+}  // namespace mcmb
 
 namespace mcmb {
 // Host wall time of the parts of a call: setup_begin at its start, then
@@ -328,6 +342,12 @@ int gram_tet_launch(mcmb_ctx* ctx, PointSet& P, int precision, double* d_gamma, 
 void lspace_slice(const PointSet& L, int kc, int rank, int nranks, int* row_lo, int* row_hi, double* frac);
 int upload_pointset(mcmb_ctx* ctx, PointSet& P, const std::vector<int>& order, bool rows_on_device = false);
 int gram_gl_launch(mcmb_ctx* ctx, int npt, double* d_gamma);
+// The GL Gram of the bound l-space setup through the separable Gram's
+// compressed pair basis (node table x, w on the host, npt entries).  *used
+// stays false, and nothing is computed, when the pair products have no
+// compressed basis (or MCMB_GL_PATH=direct): the caller then takes the direct
+// node sum.
+int gram_gl_compressed(mcmb_ctx* ctx, int npt, const double* x, const double* w, double* d_gamma, bool* used);
 // Look up this device's scratch array of the allocation-free GL Gram (once per
 // device, with the device current).
 int gl_scratch_setup(int device);

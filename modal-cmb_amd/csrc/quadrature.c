@@ -19,11 +19,13 @@ static void legendre_pn(int n, long double x, long double* pn, long double* dpn)
     *dpn = (n == 0) ? 0.0L : n * (x * p1 - p0) / (x * x - 1.0L);
 }
 
-int mcmb_gauss_legendre(int n, double* x, double* w)
+/* Roots i0 <= i < i1 of the (n + 1) / 2 in [0, 1) (i counted from -1 up):
+ * x[i] = -r, x[n - 1 - i] = r and their weights.  Each root is independent,
+ * so ranges may run on separate threads. */
+void mcmb_gl_roots(int n, int i0, int i1, double* x, double* w)
 {
-    if (n < 1 || !x || !w) return MCMB_ERR_ARG;
     const long double pi = 3.14159265358979323846264338327950288L;
-    for (int i = 0; i < (n + 1) / 2; ++i) {
+    for (int i = i0; i < i1; ++i) {
         long double t = cosl(pi * (4.0L * i + 3.0L) / (4.0L * n + 2.0L));
         long double r = t * (1.0L - (n - 1.0L) / (8.0L * n * (long double)n * n));
         long double pn, dp;
@@ -39,7 +41,13 @@ int mcmb_gauss_legendre(int n, double* x, double* w)
         x[i] = (double)(-r);
         w[n - 1 - i] = (double)wt;
         w[i] = (double)wt;
+        if (i == n - 1 - i) x[i] = 0.0;   /* the middle node of an odd order */
     }
-    if (n & 1) x[n / 2] = 0.0;
+}
+
+int mcmb_gauss_legendre(int n, double* x, double* w)
+{
+    if (n < 1 || !x || !w) return MCMB_ERR_ARG;
+    mcmb_gl_roots(n, 0, (n + 1) / 2, x, w);
     return MCMB_OK;
 }

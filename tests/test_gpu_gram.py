@@ -124,6 +124,46 @@ def test_gamma_gl_gpu_vs_oracle(ctx, L, R, mod):
     assert np.array_equal(G, G.T)
 
 
+@pytest.mark.parametrize("L,R,mod", [(300, 12, True), (2500, 30, False)])
+def test_gamma_gl_compressed_equals_direct_node_sum(ctx, L, R, mod):
+    """Legendre bases: the GL Gram runs over the compressed pair basis (rank
+    2R - 1); the direct node sum (a permanent per mode pair and node,
+    MCMB_GL_PATH=direct) gives the same Gram to rounding."""
+    ws = W.lspace(L, R, mod)
+    ctx.lspace_set(2, L, ws["q"], ws["v"], ws["order"])
+    G = ctx.gamma_gl()
+    assert ctx.last_basis_rank() == 2 * R - 1
+    os.environ["MCMB_GL_PATH"] = "direct"
+    try:
+        Gd = ctx.gamma_gl()
+    finally:
+        del os.environ["MCMB_GL_PATH"]
+    assert rel_elem(G, Gd) < 1e-13
+    assert np.array_equal(G, G.T)
+    assert np.array_equal(G, ctx.gamma_gl())
+
+
+def test_gamma_gl_full_rank_basis_takes_direct_node_sum(ctx):
+    """A random basis has full-rank pair products: no compressed basis, and
+    the GL Gram is the direct node sum bit for bit."""
+    rng = np.random.default_rng(17)
+    L, R = 120, 8
+    q = rng.standard_normal((L + 1, R))
+    v = 1.0 / np.sqrt(1.0 + np.arange(L + 1))
+    order = W.order_lex(R)
+    ctx.lspace_set(2, L, q, v, order)
+    G = ctx.gamma_gl()
+    assert ctx.last_basis_rank() == 0
+    os.environ["MCMB_GL_PATH"] = "direct"
+    try:
+        Gd = ctx.gamma_gl()
+    finally:
+        del os.environ["MCMB_GL_PATH"]
+    assert np.array_equal(G, Gd)
+    ref = O.gamma_gl(2, L, q, v, order)
+    assert rel_elem(G, ref) < 1e-12
+
+
 def test_gamma_gl_equals_tetrapyd_on_gpu(ctx):
     # two independent GPU evaluations of the same sum at lmax 500
     ws = W.lspace(500, 10, True)

@@ -1,7 +1,12 @@
 """mcmb_gamma_gl_noalloc (the GL Gram without device allocation, scratch in a
 per-GPU static array) against mcmb_gamma_gl and the oracle, through the C ABI:
-two calls in a row on one context, the second on a smaller setup."""
+two calls in a row on one context, the second on a smaller setup.  The
+allocation-free entry always takes the direct node sum; mcmb_gamma_gl takes
+the compressed pair basis where the pair products have one (both settings of
+these tests), so the two agree bit for bit under MCMB_GL_PATH=direct and to
+rounding otherwise."""
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -35,8 +40,13 @@ def test_gamma_gl_noalloc_matches_allocating_entry_point():
                 assert st == 0, lib.mcmb_last_error().decode()
                 buf = ctx.device_to_host(dev, np.empty(cap))
                 G = buf[:n * n].reshape(n, n)
-                ref = ctx.gamma_gl()
+                os.environ["MCMB_GL_PATH"] = "direct"
+                try:
+                    ref = ctx.gamma_gl()
+                finally:
+                    del os.environ["MCMB_GL_PATH"]
                 assert np.array_equal(G, ref)           # same kernels, same nodes
+                assert rel_elem(G, ctx.gamma_gl()) < 1e-13
                 assert rel_elem(G, O.gamma_gl(2, L, ws["q"], ws["v"], ws["order"])) < 1e-12
                 if prev is not None:
                     # nothing written past the smaller call's n x n output

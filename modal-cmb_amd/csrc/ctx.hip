@@ -63,6 +63,16 @@ int stage_ready(mcmb_ctx* c)
     return MCMB_OK;
 }
 
+int copy_stream_ready(mcmb_ctx* c)
+{
+    MCMB_TRY(stage_ready(c));
+    if (c->cstream) return MCMB_OK;
+    for (int h = 0; h < 2; ++h)
+        for (int k = 0; k < PSLOTS; ++k) MCMB_HIP(hipEventCreateWithFlags(&c->slot_ev[h][k], hipEventDisableTiming));
+    MCMB_HIP(hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
+    return MCMB_OK;
+}
+
 int copy_to_device(mcmb_ctx* c, void* dev, const void* host, size_t bytes)
 {
     if (!bytes) return MCMB_OK;
@@ -269,6 +279,10 @@ int mcmb_destroy(mcmb_ctx* c)
     if (c->stage) (void)hipHostFree(c->stage);
     for (int k = 0; k < 2; ++k)
         if (c->stage_ev[k]) (void)hipEventDestroy(c->stage_ev[k]);
+    for (int h = 0; h < 2; ++h)
+        for (int k = 0; k < PSLOTS; ++k)
+            if (c->slot_ev[h][k]) (void)hipEventDestroy(c->slot_ev[h][k]);
+    if (c->cstream) (void)hipStreamDestroy(c->cstream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     for (int k = 0; k <= MAX_PHASES; ++k)

@@ -1700,10 +1700,19 @@ constexpr double SEP_RANK_GAP = 1e6;
 // than NP / 2 steps (too little saving to pay for the expansion) or the
 // products vanish.  Kept per point set; the expansion tables are built with
 // it.
-static int separable_compress(mcmb_ctx* c, PointSet& L, int nl, int NP, int ld, const std::vector<int2>& pxy)
+// The basis a compression is made of: its cache key (the owner's address and
+// generation) and the table q[l][col] (row stride ldq, nl rows).
+struct BasisRef {
+    const void* key;
+    long gen;
+    const double* q;
+    int ldq;
+};
+
+static int separable_compress(mcmb_ctx* c, SepComp& S, const BasisRef& Bq, int nl, int NP, int ld,
+                              const std::vector<int2>& pxy)
 {
-    SepComp& S = c->sepc;
-    if (S.ps == L.basis_key() && S.gen == L.basis_generation() && S.NP == NP && S.nl == nl) return MCMB_OK;
+    if (S.ps == Bq.key && S.gen == Bq.gen && S.NP == NP && S.nl == nl) return MCMB_OK;
     S.ps = nullptr;
     S.r = -1;
     S.ldu = 0;
@@ -1724,8 +1733,8 @@ static int separable_compress(mcmb_ctx* c, PointSet& L, int nl, int NP, int ld, 
     MCMB_HIP(hipMemsetAsync(rank, 0xff, sizeof(int), c->stream));   // -1: running
     const long e = (long)NP * nlp;
     MCMB_TRY(launch_guard("pair columns", (unsigned long long)((e + 255) / 256), 256));
-    hipLaunchKernelGGL(sep_ucols, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, c->stream, L.d_q.as<double>(),
-                       L.ldq, nl, nlp, dpxy.as<int2>(), NP, uc.as<double>());
+    hipLaunchKernelGGL(sep_ucols, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, c->stream, Bq.q, Bq.ldq, nl, nlp,
+                       dpxy.as<int2>(), NP, uc.as<double>());
     const unsigned ncb = (unsigned)((NP + 3) / 4);
     hipLaunchKernelGGL(sep_mgs_norms, dim3(ncb), dim3(256), 0, c->stream, (const double*)uc.as<double>(), nlp, NP,
                        dn);
@@ -1799,8 +1808,8 @@ static int separable_compress(mcmb_ctx* c, PointSet& L, int nl, int NP, int ld, 
         MCMB_HIP(hipMemcpy(S.tab.p, buf.data(), sizeof(int) * buf.size(), hipMemcpyHostToDevice));
         MCMB_TRY(S.h.alloc(sizeof(double) * (size_t)r * ldh));
     }
-    S.ps = L.basis_key();
-    S.gen = L.basis_generation();
+    S.ps = Bq.key;
+    S.gen = Bq.gen;
     S.NP = NP;
     S.nl = nl;
     S.nlp = nlp;
@@ -1949,7 +1958,8 @@ int gram_gl_compressed(mcmb_ctx* c, int npt, const double* x, const double* w, d
     std::vector<int2> pxy(NP);
     for (int a = 0; a < R; ++a)
         for (int b = a; b < R; ++b) pxy[pair_index(a, b, R)] = make_int2(a, b);
-    MCMB_TRY(separable_compress(c, L, nl, NP, ld, pxy));
+    MCMB_TRY(separable_compress(c, c->sepc, BasisRef{L.basis_key(), L.basis_generation(), L.d_q.as<double>(), L.ldq},
+                                nl, NP, ld, pxy));
     const SepComp& S = c->sepc;
     const int r = S.r;
     if (r <= 0 || r > 4 * GL_GW) return MCMB_OK;   // full rank: the direct node sum
@@ -2061,7 +2071,9 @@ int gram_separable_launch(mcmb_ctx* c, PointSet& L, int precision, double* d_gam
             MCMB_TRY(phase("basis compression"));
             launches += 2 + NP / 2;
         }
-        MCMB_TRY(separable_compress(c, L, nl, NP, ld, pxy));
+        MCMB_TRY(separable_compress(c, c->sepc,
+                                    BasisRef{L.basis_key(), L.basis_generation(), L.d_q.as<double>(), L.ldq}, nl, NP,
+                                    ld, pxy));
         comp = S.r > 0;
         setup_done("basis factorisation");
     }

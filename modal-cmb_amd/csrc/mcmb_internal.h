@@ -212,6 +212,8 @@ struct PolSet {
 int spline_weights(const std::vector<double>& x, int kind, std::vector<double>& cw);
 
 constexpr int MAX_PHASES = 10;
+// Per-batch slots of the pinned staging used by the projection's transfers.
+constexpr int PSLOTS = 8;
 
 // Host threads of the table builds: up to 16, never more than the hardware
 // has; MCMB_HOST_THREADS sets the count (1..64).
@@ -252,6 +254,7 @@ struct mcmb_ctx {
     mcmb::PolSet es;     // E-mode field of the late basis (polarised Gram)
     mcmb::DevBuf d_W;    // full-cube projection weight W[i][j][k] (k-space)
     mcmb::DevBuf pz, py, ph, pd, psc, pdm;   // projection batch buffers (Z, Y, H, partials, sin / cos, D)
+    mcmb::DevBuf pio;   // a projection call's shapes, inputs and results on the device
     // stage-1 products Y = W Z of the last forward projection, per batch of
     // shapes, reused by a backward call on the same shapes
     std::vector<mcmb::DevBuf> ycache;
@@ -285,6 +288,10 @@ struct mcmb_ctx {
     // STAGE_BYTES, allocated on first use
     void* stage = nullptr;
     hipEvent_t stage_ev[2] = {};
+    // copies that overlap the context stream's work, and the events of the
+    // projection's per-batch slots of the staging (made on first use)
+    hipStream_t cstream = nullptr;
+    hipEvent_t slot_ev[2][mcmb::PSLOTS] = {};
     double setup_ms[2 * mcmb::MAX_PHASES] = {};
     const char* setup_names[2 * mcmb::MAX_PHASES] = {};
     int nsetup = 0;
@@ -366,6 +373,9 @@ int copy_to_device(mcmb_ctx* ctx, void* dev, const void* host, size_t bytes);
 // their own stream synchronisations.
 constexpr size_t STAGE_BYTES = (size_t)16 << 20;
 int stage_ready(mcmb_ctx* ctx);
+// The staging, the context's copy stream (default priority) and the slot
+// events, made on first use.
+int copy_stream_ready(mcmb_ctx* ctx);
 // Symmetric eigen-decomposition by Householder tridiagonalisation (n up to
 // 8000): evals (host) descending, d_V rows the sign-fixed eigenvectors; with
 // normalise, of D^-1 G D^-1 (d_norm[i] = sqrt G_ii).  tridiag_release frees

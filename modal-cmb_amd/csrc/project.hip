@@ -818,28 +818,69 @@ int mcmb_project_fwd(mcmb_ctx* c, int precision, int ntheta, const double* theta
     const bool cache = (size_t)nbat * ybytes(c, precision) <= PCACHE_BYTES;
     c->ycache_gen = -1;
     if (cache && (int)c->ycache.size() < nbat) c->ycache.resize(nbat);
-    DevBuf dth, dal, ddal;
-    MCMB_TRY(dth.alloc(sizeof(double) * 2 * (size_t)ntheta));
-    MCMB_TRY(dal.alloc(sizeof(double) * (size_t)ntheta * n));
-    MCMB_TRY(ddal.alloc(sizeof(double) * (size_t)ntheta * 2 * n));
+    // device arrays of the call: theta [ntheta][2], alpha [ntheta][n], dalpha [ntheta][2][n]
+    MCMB_TRY(c->pio.alloc(sizeof(double) * (2 + 3 * (size_t)n) * ntheta));
+    double* const dth = c->pio.as<double>();
+    double* const dal = dth + 2 * (size_t)ntheta;
+    double* const ddal = dal + (size_t)ntheta * n;
+    // Each batch's results go to the host while later batches compute: once
+    // the batch's kernels are done, the copy stream moves its alpha (and
+    // dalpha) into slot bi % nslot of the pinned staging, and host threads
+    // move the slots on to the caller's arrays in batch order, so only the
+    // last batch's transfer follows the GPU work.
+    const int nout = dalpha ? 3 : 1;
+    const size_t slot_bytes = sizeof(double) * (size_t)PBATCH * n * nout;
+    const int nslot = (int)std::min<size_t>(PSLOTS, 2 * STAGE_BYTES / slot_bytes);
+    const char* cenv = getenv("MCMB_HOST_COPY");
+    const bool staged = nslot >= 1 && !(cenv && !strcmp(cenv, "direct"));
+    if (staged) MCMB_TRY(copy_stream_ready(c));
+    const int nth = std::min(8, host_threads());
+    auto drain = [&](int bi) -> int {
+        MCMB_HIP(hipEventSynchronize(c->slot_ev[1][bi % nslot]));
+        const int b0 = bi * PBATCH, nb = std::min(PBATCH, ntheta - b0);
+        const double* sl = reinterpret_cast<const double*>(static_cast<char*>(c->stage) + (bi % nslot) * slot_bytes);
+        const size_t na = (size_t)nb * n;
+        run_threads(nth, [&](int t) {
+            const size_t a = na * t / nth, b = na * (t + 1) / nth;
+            memcpy(alpha + (size_t)b0 * n + a, sl + a, sizeof(double) * (b - a));
+            if (dalpha) memcpy(dalpha + (size_t)b0 * 2 * n + 2 * a, sl + na + 2 * a, sizeof(double) * 2 * (b - a));
+        });
+        return MCMB_OK;
+    };
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
-    MCMB_HIP(hipMemcpyAsync(dth.p, theta, sizeof(double) * 2 * (size_t)ntheta, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(dth, theta, sizeof(double) * 2 * (size_t)ntheta, hipMemcpyHostToDevice, c->stream));
     for (int b0 = 0, bi = 0; b0 < ntheta; b0 += PBATCH, ++bi) {
         const int nb = std::min(PBATCH, ntheta - b0);
         DevBuf& Yb = cache ? c->ycache[bi] : c->py;
-        double* d_al = dal.as<double>() + (size_t)b0 * n;
-        double* d_dal = ddal.as<double>() + (size_t)b0 * 2 * n;
+        double* d_al = dal + (size_t)b0 * n;
+        double* d_dal = ddal + (size_t)b0 * 2 * n;
         if (precision == MCMB_FP64)
-            MCMB_TRY(project_fwd_batch<double>(c, dth.as<double>() + 2 * b0, nb, d_al, d_dal, Yb));
+            MCMB_TRY(project_fwd_batch<double>(c, dth + 2 * b0, nb, d_al, d_dal, Yb));
         else
-            MCMB_TRY(project_fwd_batch<float>(c, dth.as<double>() + 2 * b0, nb, d_al, d_dal, Yb));
+            MCMB_TRY(project_fwd_batch<float>(c, dth + 2 * b0, nb, d_al, d_dal, Yb));
+        if (staged) {
+            if (bi >= nslot) MCMB_TRY(drain(bi - nslot));   // the slot's previous batch
+            const int k = bi % nslot;
+            char* sl = static_cast<char*>(c->stage) + k * slot_bytes;
+            MCMB_HIP(hipEventRecord(c->slot_ev[0][k], c->stream));
+            MCMB_HIP(hipStreamWaitEvent(c->cstream, c->slot_ev[0][k], 0));
+            MCMB_HIP(hipMemcpyAsync(sl, d_al, sizeof(double) * (size_t)nb * n, hipMemcpyDeviceToHost, c->cstream));
+            if (dalpha)
+                MCMB_HIP(hipMemcpyAsync(sl + sizeof(double) * (size_t)nb * n, d_dal, sizeof(double) * 2 * (size_t)nb * n,
+                                        hipMemcpyDeviceToHost, c->cstream));
+            MCMB_HIP(hipEventRecord(c->slot_ev[1][k], c->cstream));
+        }
     }
-    // the device arrays already have the caller's layout: one contiguous copy each
-    MCMB_HIP(hipMemcpyAsync(alpha, dal.p, sizeof(double) * (size_t)ntheta * n, hipMemcpyDeviceToHost, c->stream));
-    if (dalpha)
-        MCMB_HIP(hipMemcpyAsync(dalpha, ddal.p, sizeof(double) * (size_t)ntheta * 2 * n, hipMemcpyDeviceToHost,
-                                c->stream));
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
+    if (staged) {
+        for (int bi = std::max(0, nbat - nslot); bi < nbat; ++bi) MCMB_TRY(drain(bi));
+    } else {
+        // the device arrays already have the caller's layout: one contiguous copy each
+        MCMB_HIP(hipMemcpyAsync(alpha, dal, sizeof(double) * (size_t)ntheta * n, hipMemcpyDeviceToHost, c->stream));
+        if (dalpha)
+            MCMB_HIP(hipMemcpyAsync(dalpha, ddal, sizeof(double) * (size_t)ntheta * 2 * n, hipMemcpyDeviceToHost,
+                                    c->stream));
+    }
     MCMB_HIP(hipStreamSynchronize(c->stream));
     if (cache) {
         c->ycache_theta.assign(theta, theta + 2 * (size_t)ntheta);
@@ -870,31 +911,55 @@ int mcmb_project_bwd(mcmb_ctx* c, int precision, int ntheta, const double* theta
         if (sl >= 0) return fail(MCMB_ERR_ORDER, "mcmb_project_bwd: repeated mode (%d,%d,%d)", r, s, t);
         sl = m;
     }
-    DevBuf dslot, dth, dg, dd;
-    MCMB_TRY(dslot.alloc(sizeof(int) * slot.size()));
-    MCMB_TRY(dth.alloc(sizeof(double) * 2 * (size_t)ntheta));
-    MCMB_TRY(dg.alloc(sizeof(double) * (size_t)ntheta * n));
-    MCMB_TRY(dd.alloc(sizeof(double) * 2 * (size_t)ntheta));
+    // device arrays of the call: theta [ntheta][2], gbar [ntheta][n], dtheta
+    // [ntheta][2], then the slot table
+    MCMB_TRY(c->pio.alloc(sizeof(double) * (4 + (size_t)n) * ntheta + sizeof(int) * slot.size()));
+    double* const dth = c->pio.as<double>();
+    double* const dg = dth + 2 * (size_t)ntheta;
+    double* const dd = dg + (size_t)ntheta * n;
+    int* const dslot = reinterpret_cast<int*>(dd + 2 * (size_t)ntheta);
     // the last forward's stage-1 products, when it ran on exactly these shapes
     const bool cached = c->ycache_gen == K.generation && c->ycache_prec == precision &&
                         c->ycache_theta.size() == 2 * (size_t)ntheta &&
                         !memcmp(c->ycache_theta.data(), theta, sizeof(double) * 2 * (size_t)ntheta);
-    MCMB_HIP(hipMemcpyAsync(dslot.p, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice, c->stream));
+    // gbar goes up batch by batch through slots of the pinned staging: host
+    // threads fill slot bi % nslot while the GPU works on earlier batches
+    const size_t slot_bytes = sizeof(double) * (size_t)PBATCH * n;
+    const int nslot = (int)std::min<size_t>(PSLOTS, 2 * STAGE_BYTES / slot_bytes);
+    const char* cenv = getenv("MCMB_HOST_COPY");
+    const bool staged = nslot >= 1 && !(cenv && !strcmp(cenv, "direct"));
+    if (staged) MCMB_TRY(copy_stream_ready(c));
+    const int nth = std::min(8, host_threads());
+    MCMB_HIP(hipMemcpyAsync(dslot, slot.data(), sizeof(int) * slot.size(), hipMemcpyHostToDevice, c->stream));
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
-    MCMB_HIP(hipMemcpyAsync(dth.p, theta, sizeof(double) * 2 * (size_t)ntheta, hipMemcpyHostToDevice, c->stream));
-    MCMB_HIP(hipMemcpyAsync(dg.p, gbar, sizeof(double) * (size_t)ntheta * n, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(dth, theta, sizeof(double) * 2 * (size_t)ntheta, hipMemcpyHostToDevice, c->stream));
+    if (!staged)
+        MCMB_HIP(hipMemcpyAsync(dg, gbar, sizeof(double) * (size_t)ntheta * n, hipMemcpyHostToDevice, c->stream));
     for (int b0 = 0, bi = 0; b0 < ntheta; b0 += PBATCH, ++bi) {
         const int nb = std::min(PBATCH, ntheta - b0);
         DevBuf* Yc = cached ? &c->ycache[bi] : nullptr;
-        const double* th = dth.as<double>() + 2 * b0;
-        const double* gb = dg.as<double>() + (size_t)b0 * n;
-        double* dt = dd.as<double>() + 2 * b0;
+        const double* th = dth + 2 * b0;
+        double* gb = dg + (size_t)b0 * n;
+        double* dt = dd + 2 * b0;
+        if (staged) {
+            const int k = bi % nslot;
+            // the slot's previous upload has been read
+            if (bi >= nslot) MCMB_HIP(hipEventSynchronize(c->slot_ev[0][k]));
+            double* sl = reinterpret_cast<double*>(static_cast<char*>(c->stage) + k * slot_bytes);
+            const size_t na = (size_t)nb * n;
+            run_threads(nth, [&](int t) {
+                const size_t a = na * t / nth, b = na * (t + 1) / nth;
+                memcpy(sl + a, gbar + (size_t)b0 * n + a, sizeof(double) * (b - a));
+            });
+            MCMB_HIP(hipMemcpyAsync(gb, sl, sizeof(double) * na, hipMemcpyHostToDevice, c->stream));
+            MCMB_HIP(hipEventRecord(c->slot_ev[0][k], c->stream));
+        }
         if (precision == MCMB_FP64)
-            MCMB_TRY(project_bwd_batch<double>(c, th, nb, gb, dslot.as<int>(), dt, Yc));
+            MCMB_TRY(project_bwd_batch<double>(c, th, nb, gb, dslot, dt, Yc));
         else
-            MCMB_TRY(project_bwd_batch<float>(c, th, nb, gb, dslot.as<int>(), dt, Yc));
+            MCMB_TRY(project_bwd_batch<float>(c, th, nb, gb, dslot, dt, Yc));
     }
-    MCMB_HIP(hipMemcpyAsync(dtheta, dd.p, sizeof(double) * 2 * (size_t)ntheta, hipMemcpyDeviceToHost, c->stream));
+    MCMB_HIP(hipMemcpyAsync(dtheta, dd, sizeof(double) * 2 * (size_t)ntheta, hipMemcpyDeviceToHost, c->stream));
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));
     float ms = 0.f;

@@ -125,3 +125,26 @@ def test_projection_adjoint_reuses_forward_products(ctx, prec):
     ctx.project_fwd(other, prec)
     dt = ctx.project_bwd(th, g2, prec)
     assert np.abs(dt - ref).max() < (1e-11 if prec == modalcmb.FP64 else 1e-5) * np.abs(ref).max()
+
+
+def test_projection_transfers_over_more_batches_than_slots(ctx, monkeypatch):
+    # 300 shapes are 10 batches of 32, more than the 8 pinned slots that carry
+    # the results out and gbar in while later batches compute: the values
+    # equal those of plain whole-array copies bit for bit, and the last
+    # batches' shapes match the oracle
+    ks = W.kspace(24, 4)
+    ctx.kspace_set(ks["kn"], ks["kw"], ks["q"], ks["order"])
+    th = W.shape_batch(300, seed=21)
+    gbar = np.random.default_rng(2).standard_normal((300, ks["order"].shape[0]))
+    a, d = ctx.project_fwd(th, modalcmb.FP64)
+    g = ctx.project_bwd(th, gbar, modalcmb.FP64)
+    a_only = ctx.project_fwd(th, modalcmb.FP64, derivatives=False)
+    monkeypatch.setenv("MCMB_HOST_COPY", "direct")
+    a2, d2 = ctx.project_fwd(th, modalcmb.FP64)
+    g2 = ctx.project_bwd(th, gbar, modalcmb.FP64)
+    assert np.array_equal(a, a2) and np.array_equal(d, d2) and np.array_equal(g, g2)
+    assert np.array_equal(a_only, a)
+    for b in (0, 255, 256, 299):
+        ra, rom, rph = O.project_kgl(ks["kn"], ks["kw"], ks["q"], ks["order"], th[b, 0], th[b, 1])
+        assert relmax(a[b], ra) < 1e-12
+        assert relmax(d[b, 0], rom) < 1e-12 and relmax(d[b, 1], rph) < 1e-12

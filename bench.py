@@ -307,8 +307,9 @@ def bench_projection(modalcmb, W, args):
             "alpha": max(rel(a32[b], a64[b]) for b in range(B)),
             "dalpha_dtheta": max(max(rel(d32[b, 0], d64[b, 0]), rel(d32[b, 1], d64[b, 1])) for b in range(B)),
             "norm": "max over shapes of max |x32 - x64| / max |x64| within the shape"},
-        "fwd_stage1_flops": 2.0 * (N * (N + 1) // 2) * Kp * 4 * R * B,
-        "fwd_tflops_stage1_over_fwd_time": round(2.0 * (N * (N + 1) // 2) * Kp * 4 * R * B / (k_fwd * 1e-3) / 1e12, 2),
+        # stage 1 (W times the cos / sin omega k columns) over the pair rows i <= j
+        "fwd_stage1_flops": 2.0 * (N * (N + 1) // 2) * Kp * 2 * R * B,
+        "fwd_tflops_stage1_over_fwd_time": round(2.0 * (N * (N + 1) // 2) * Kp * 2 * R * B / (k_fwd * 1e-3) / 1e12, 2),
         "fp32_mfma_peak_tflops": PEAK_FP32_TFLOPS,
     }
     if not args.no_cpu:

@@ -67,7 +67,8 @@ __global__ void proj_build_W(const double* __restrict__ kn, const double* __rest
     W[e] = (T)w;
 }
 
-// Z[k][col], col = (b*4 + c)*R + t for shapes b < nb of the batch; Kp x ldz.
+// Z[k][col], col = (b*2 + c)*R + t for shapes b < nb of the batch: c = 0
+// cos(omega_b k) q_t(k), c = 1 sin(omega_b k) q_t(k); Kp x ldz.
 template <class T>
 __global__ void proj_build_Z(const double* __restrict__ kn, const double* __restrict__ q, int ldq, int N, int R,
                              const double* __restrict__ theta, int nb, int Kp, int ldz, T* __restrict__ Z)
@@ -76,14 +77,12 @@ __global__ void proj_build_Z(const double* __restrict__ kn, const double* __rest
     if (e >= (size_t)Kp * ldz) return;
     int k = (int)(e / ldz), col = (int)(e % ldz);
     double z = 0.0;
-    if (k < N && col < nb * 4 * R) {
-        int t = col % R, c = (col / R) % 4, b = col / (4 * R);
+    if (k < N && col < nb * 2 * R) {
+        int t = col % R, c = (col / R) % 2, b = col / (2 * R);
         double om = theta[2 * b];
         double sb, cb;
         sincos(om * kn[k], &sb, &cb);
-        double v = (c & 1) ? sb : cb;
-        if (c >= 2) v *= kn[k];
-        z = v * q[(size_t)k * ldq + t];
+        z = (c ? sb : cb) * q[(size_t)k * ldq + t];
     }
     Z[e] = (T)z;
 }
@@ -195,7 +194,11 @@ __global__ void proj_sincos_tab(const double* __restrict__ kn, const double* __r
 }
 
 // ---- forward stage 2: per (shape b, i) -------------------------------------
-// H[b][x][i][s][t] = sum_j q_s(j) G_x(i, j, t), x = value, d/domega, d/dphi.
+// H[b][x][i][s][t] = sum_j q_s(j) G_x(i, j, t) with G_x(i, j, t) = sum_k
+// W_ijk {sin, cos}(omega K + phi) q_t(k), x = 0 (sin: the value) and x = 1
+// (cos: the derivatives).  From stage 1's Yc, Ys (cos / sin omega k_k
+// columns) by angle addition: G_0 = sin A Yc + cos A Ys, G_1 = cos A Yc -
+// sin A Ys, A = omega (k_i + k_j) + phi.
 template <class T>
 __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int ldy, const double* __restrict__ kn,
                                                    const double* __restrict__ q, int ldq, int N, int R,
@@ -204,12 +207,11 @@ __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int 
 {
     // one workgroup per (shape b, node i); H_x[s][t] = sum_j q_s(j) G_x[j][t]
     // over chunks of 32 nodes j, on FP64 MFMA 16x16x4: A = Q^T (s x j), B =
-    // G_x (j x t).  Wave w owns the s tile w / 2 and the t tile w % 2 of all
-    // three x (3 accumulators, the A operand shared).  The thread's 4
-    // elements (j = j0 + tid / 32 + 8 u, t = tid % 32) of the next chunk are
-    // loaded (Y, q, k_j, sin / cos omega k_j) while the current chunk's
-    // products run.
-    __shared__ double G[3][32][32];     // [x][j in chunk][t]
+    // G_x (j x t).  Wave w owns the s tile w / 2 and the t tile w % 2 of both
+    // x (2 accumulators, the A operand shared).  The thread's 4 elements (j =
+    // j0 + tid / 32 + 8 u, t = tid % 32) of the next chunk are loaded (Y, q,
+    // sin / cos omega k_j) while the current chunk's products run.
+    __shared__ double G[2][32][32];     // [x][j in chunk][t]
     __shared__ double Qs[32][32];       // [j in chunk][s]
     const int b = blockIdx.x % nb, i = blockIdx.x / nb;
     const int tid = threadIdx.x, t = tid & 31, jr = tid >> 5;
@@ -218,21 +220,20 @@ __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int 
     double sa, ca;   // sin, cos of omega k_i + phi
     sincos(theta[2 * b] * ki + theta[2 * b + 1], &sa, &ca);
     const double* const sc = scb + 2 * (size_t)b * N;
-    d4 acc[3];
+    d4 acc[2];
 #pragma unroll
-    for (int x = 0; x < 3; ++x) acc[x] = (d4){0.0, 0.0, 0.0, 0.0};
-    T yv[4][4];
-    double qv[4], kv[4], sv[4], cv[4];
+    for (int x = 0; x < 2; ++x) acc[x] = (d4){0.0, 0.0, 0.0, 0.0};
+    T yv[4][2];
+    double qv[4], sv[4], cv[4];
     // loads of chunk j0 (nodes past N repeat the last one and are zeroed in stage())
     auto fetch = [&](int j0) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int j = min(j0 + jr + 8 * u, N - 1), tt = min(t, R - 1);
-            const T* y = Y + (size_t)pair_row(i, j, N) * ldy + (size_t)b * 4 * R + tt;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) yv[u][c] = y[c * R];
+            const T* y = Y + (size_t)pair_row(i, j, N) * ldy + (size_t)b * 2 * R + tt;
+            yv[u][0] = y[0];
+            yv[u][1] = y[R];
             qv[u] = q[(size_t)j * ldq + tt];
-            kv[u] = kn[j];
             sv[u] = sc[2 * j];
             cv[u] = sc[2 * j + 1];
         }
@@ -244,13 +245,10 @@ __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int 
             const bool in = j0 + jj < N && t < R;
             const double sA = sa * cv[u] + ca * sv[u], cA = ca * cv[u] - sa * sv[u];
             const double yc = (double)yv[u][0], ys = (double)yv[u][1];
-            const double ycp = (double)yv[u][2], ysp = (double)yv[u][3];
             const double g0 = sA * yc + cA * ys;
-            const double g2 = cA * yc - sA * ys;
-            const double g1 = (ki + kv[u]) * g2 + cA * ycp - sA * ysp;
+            const double g1 = cA * yc - sA * ys;
             G[0][jj][t] = in ? g0 : 0.0;
             G[1][jj][t] = in ? g1 : 0.0;
-            G[2][jj][t] = in ? g2 : 0.0;
             Qs[jj][t] = in ? qv[u] : 0.0;
         }
     };
@@ -266,7 +264,7 @@ __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int 
             const int jj = 4 * ks + (lane >> 4);
             const double a = Qs[jj][sm + (lane & 15)];
 #pragma unroll
-            for (int x = 0; x < 3; ++x)
+            for (int x = 0; x < 2; ++x)
                 acc[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, G[x][jj][tn + (lane & 15)], acc[x], 0, 0, 0);
         }
     }
@@ -274,24 +272,34 @@ __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int 
     const int tt = tn + (lane & 15);
     if (tt < R)
 #pragma unroll
-        for (int x = 0; x < 3; ++x)
+        for (int x = 0; x < 2; ++x)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int ss = sm + 4 * v + (lane >> 4);
-                if (ss < R) H[(((size_t)b * 3 + x) * N + i) * R * R + ss * R + tt] = acc[x][v];
+                if (ss < R) H[(((size_t)b * 2 + x) * N + i) * R * R + ss * R + tt] = acc[x][v];
             }
 }
 
-// D[b][x][r][st] = sum_i q_r(i) H[b][x][i][st] (st = s R + t) on FP64 MFMA
-// 16x16x4: one wave per (shape b, x, block of 16 columns st), the two r
-// tiles sharing each B load (H, read once); K runs over the nodes i.
+// D[b][x][r][st] (st = s R + t) on FP64 MFMA 16x16x4:
+//   x = 0: sum_i q_r(i) H[b][0][i][st]       (alpha)
+//   x = 1: sum_i q_r(i) H[b][1][i][st]       (d alpha / d phi)
+//   x = 2: sum_i k_i q_r(i) H[b][1][i][st]   (E, the k_i-weighted cos form)
+// One wave per (shape b, x, block of 16 columns st), the two r tiles sharing
+// each B load (H, read once per x); K runs over the nodes i.
+//
+// E gives d alpha / d omega = sum W K cos(omega K + phi) q_r q_s q_t: W and
+// the cos term are symmetric in the three legs, so each of k_j and k_k can
+// be moved onto leg i by relabelling, and d alpha / d omega [r][s][t] =
+// E[r][s][t] + E[s][r][t] + E[t][s][r].
 __global__ __launch_bounds__(64) void proj_stage3(const double* __restrict__ H, const double* __restrict__ q, int ldq,
-                                                  int N, int R, double* __restrict__ D)
+                                                  const double* __restrict__ kn, int N, int R, double* __restrict__ D)
 {
     const int lane = threadIdx.x;
     const int RR = R * R, nblk = (RR + 15) / 16;
     const int bx = blockIdx.x / nblk, st = 16 * (blockIdx.x % nblk) + (lane & 15);
-    const double* const h = H + (size_t)bx * N * RR + min(st, RR - 1);
+    const int b = bx / 3, x = bx % 3;
+    const bool kw = x == 2;   // uniform over the wave
+    const double* const h = H + ((size_t)b * 2 + min(x, 1)) * N * RR + min(st, RR - 1);
     d4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
     const int nks = (N + 3) / 4;
 #pragma unroll 5
@@ -301,8 +309,9 @@ __global__ __launch_bounds__(64) void proj_stage3(const double* __restrict__ H, 
         const int ic = min(i, N - 1);
         const double bv = in ? h[(size_t)ic * RR] : 0.0;
         const int r0 = lane & 15, r1 = 16 + (lane & 15);
-        const double a0 = r0 < R ? q[(size_t)ic * ldq + r0] : 0.0;
-        const double a1 = r1 < R ? q[(size_t)ic * ldq + r1] : 0.0;
+        const double kf = kw ? kn[ic] : 1.0;
+        const double a0 = r0 < R ? kf * q[(size_t)ic * ldq + r0] : 0.0;
+        const double a1 = r1 < R ? kf * q[(size_t)ic * ldq + r1] : 0.0;
         acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[0], 0, 0, 0);
         acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[1], 0, 0, 0);
     }
@@ -317,8 +326,10 @@ __global__ __launch_bounds__(64) void proj_stage3(const double* __restrict__ H, 
             }
 }
 
-// alpha_x[b][m] = D[b][x][r_m][s_m R + t_m], written in the caller's layout:
-// alpha[b][m] (x = 0) and dalpha[b][x-1][m] (x = 1, 2)
+// alpha[b][m] = D[b][0][r][s R + t], d alpha / d omega = E[r][s][t] +
+// E[s][r][t] + E[t][s][r] (E = D[b][2]), d alpha / d phi = D[b][1][r][s R +
+// t], (r, s, t) = order[m], written in the caller's layout: alpha[b][m] and
+// dalpha[b][x][m] (x = 0: omega, 1: phi).
 __global__ void proj_stage3_modes(const double* __restrict__ D, int R, const int* __restrict__ modes, int n, int nb,
                                   double* __restrict__ alpha, double* __restrict__ dalpha)
 {
@@ -326,16 +337,30 @@ __global__ void proj_stage3_modes(const double* __restrict__ D, int R, const int
     if (e >= nb * 3 * n) return;
     const int m = e % n, x = (e / n) % 3, b = e / (3 * n);
     const int pk = modes[m], r = pk & 255, s = (pk >> 8) & 255, t = (pk >> 16) & 255;
-    const double a = D[(((size_t)b * 3 + x) * R + r) * R * R + s * R + t];
-    if (x == 0) alpha[(size_t)b * n + m] = a;
-    else dalpha[((size_t)b * 2 + x - 1) * n + m] = a;
+    const int RR = R * R;
+    const double* const Db = D + (size_t)b * 3 * R * RR;
+    if (x == 0) alpha[(size_t)b * n + m] = Db[(size_t)r * RR + s * R + t];
+    else if (x == 1) dalpha[((size_t)b * 2 + 1) * n + m] = Db[(size_t)R * RR + (size_t)r * RR + s * R + t];
+    else {
+        const double* const E = Db + 2 * (size_t)R * RR;
+        dalpha[(size_t)b * 2 * n + m] = E[(size_t)r * RR + s * R + t] + E[(size_t)s * RR + r * R + t] +
+                                        E[(size_t)t * RR + s * R + r];
+    }
 }
 
 // ---- adjoint ---------------------------------------------------------------
-// Hbar[b][i][s][t] = sum over canonical triples (r, s, t) = order[m] of
-// gbar[b][m] q_r(i): per shape a product Q (N x R) . Gb (R x R^2), Gb[r][st]
-// = gbar[b][slot[r][st]] (0 where slot is -1), on FP64 MFMA 16x16x4.  One
-// wave per (shape b, block of 16 columns st): its B operand (Gb, R padded to
+// With Gb[r][s][t] = gbar[b][m] at (r, s, t) = order[m] (0 elsewhere) and
+// Gh[r][s][t] = Gb[r][s][t] + Gb[s][r][t] + Gb[t][s][r]:
+//   dL/domega = sum Gb (E[r][s][t] + E[s][r][t] + E[t][s][r]) = sum Gh E,
+//   dL/dphi   = sum Gb Dphi = sum Gh Dphi / 3
+// (Dphi is symmetric in r, s, t, and Gh / 3 has the symmetric part of Gb).
+// Both are then sum_i {k_i, 1/3} P_i with P_i = sum_{j,t} Gbar(i,j,t)
+// G_1(i,j,t), Gbar(i,j,t) = sum_s q_s(j) Hbar[i][s][t], Hbar[i][s][t] =
+// sum_r q_r(i) Gh[r][s][t].
+//
+// Hbar[b][i][s][t]: per shape a product Q (N x R) . Gh (R x R^2) on FP64
+// MFMA 16x16x4, Gh gathered through slot[r][s][t] = m (-1: no mode).  One
+// wave per (shape b, block of 16 columns st): its B operand (Gh, R padded to
 // 32) is gathered once into registers, then the N / 16 row tiles of Q run
 // through 8 K-steps each.
 __global__ __launch_bounds__(64) void proj_hbar(const double* __restrict__ gbar, const int* __restrict__ slot,
@@ -346,11 +371,14 @@ __global__ __launch_bounds__(64) void proj_hbar(const double* __restrict__ gbar,
     const int RR = R * R, nblk = (RR + 15) / 16;
     const int b = blockIdx.x / nblk, st = 16 * (blockIdx.x % nblk) + (lane & 15);
     double bv[8];
+    const int s = min(st, RR - 1) / R, t = min(st, RR - 1) % R;
+    auto g = [&](int m) { return m >= 0 ? gbar[(size_t)b * n + m] : 0.0; };
 #pragma unroll
     for (int ks = 0; ks < 8; ++ks) {
         const int r = 4 * ks + (lane >> 4);
-        const int m = (r < R && st < RR) ? slot[r * RR + st] : -1;
-        bv[ks] = m >= 0 ? gbar[(size_t)b * n + m] : 0.0;
+        bv[ks] = 0.0;
+        if (r < R && st < RR)
+            bv[ks] = g(slot[r * RR + s * R + t]) + g(slot[s * RR + r * R + t]) + g(slot[t * RR + s * R + r]);
     }
     for (int i0 = 0; i0 < N; i0 += 16) {
         const int ia = min(i0 + (lane & 15), N - 1);   // A row (rows past N: the last, not stored)
@@ -371,7 +399,8 @@ __global__ __launch_bounds__(64) void proj_hbar(const double* __restrict__ gbar,
     }
 }
 
-// dtheta[b] += sum_{j,t} Gbar(i,j,t) (G1, G2)(i,j,t) for one (b, i) per workgroup
+// dpart[b][i] = (k_i P_i, P_i / 3), P_i = sum_{j,t} Gbar(i,j,t) G_1(i,j,t), for
+// one (b, i) per workgroup
 template <class T>
 __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int ldy, const double* __restrict__ kn,
                                                     const double* __restrict__ q, int ldq, int N, int R,
@@ -382,12 +411,12 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
     // one workgroup per (shape b, node i), over chunks of 32 nodes j:
     // Gbar[j][t] = sum_s q_s(j) Hbar[s][t] on FP64 MFMA 16x16x4 (A = Q, j x s;
     // B = Hbar, s x t; wave w owns the j tile w / 2 and the t tile w % 2),
-    // then each lane adds Gbar G1 and Gbar G2 at its 4 output elements.  The
-    // Y, k_j and sin / cos omega k_j of those elements in the next chunk are
-    // loaded while the current chunk's products run.
+    // then each lane adds Gbar G_1 at its 4 output elements.  The Y and sin /
+    // cos omega k_j of those elements in the next chunk are loaded while the
+    // current chunk's products run.
     __shared__ double Hs[32][32];   // [s][t]
     __shared__ double Qs[32][32];   // [s][j in chunk]
-    __shared__ double red[2][256];
+    __shared__ double red[256];
     const int b = blockIdx.x % nb, i = blockIdx.x / nb;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int jm = 16 * (w >> 1), tn = 16 * (w & 1);
@@ -400,23 +429,22 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
         const int ss = e >> 5, tt = e & 31;
         Hs[ss][tt] = (ss < R && tt < R) ? Hb[(((size_t)b * N) + i) * R * R + ss * R + tt] : 0.0;
     }
-    T yv[4][4];
-    double kv[4], sv[4], cv[4];
+    T yv[4][2];
+    double sv[4], cv[4];
     // the lane's elements (j0 + jm + 4 v + lane / 16, t) of a chunk (nodes
     // past N: the last node; columns past R: the last column; both unused)
     auto fetch = [&](int j0) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int j = min(j0 + jm + 4 * v + (lane >> 4), N - 1), tt = min(t, R - 1);
-            const T* y = Y + (size_t)pair_row(i, j, N) * ldy + (size_t)b * 4 * R + tt;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) yv[v][c] = y[c * R];
-            kv[v] = kn[j];
+            const T* y = Y + (size_t)pair_row(i, j, N) * ldy + (size_t)b * 2 * R + tt;
+            yv[v][0] = y[0];
+            yv[v][1] = y[R];
             sv[v] = sc[2 * j];
             cv[v] = sc[2 * j + 1];
         }
     };
-    double dom = 0.0, dph = 0.0;
+    double pacc = 0.0;
     fetch(0);
     for (int j0 = 0; j0 < N; j0 += 32) {
         __syncthreads();   // Hs written / the previous chunk is done with Qs
@@ -426,13 +454,12 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
         }
         __syncthreads();
         // this chunk's element data, then the next chunk's loads
-        T yc[4][4];
-        double kc[4], scs[4], ccs[4];
+        T yc[4][2];
+        double scs[4], ccs[4];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-#pragma unroll
-            for (int c = 0; c < 4; ++c) yc[v][c] = yv[v][c];
-            kc[v] = kv[v];
+            yc[v][0] = yv[v][0];
+            yc[v][1] = yv[v][1];
             scs[v] = sv[v];
             ccs[v] = cv[v];
         }
@@ -449,24 +476,19 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
                 if (j0 + jm + 4 * v + (lane >> 4) < N) {
                     const double sA = sa * ccs[v] + ca * scs[v], cA = ca * ccs[v] - sa * scs[v];
                     const double yr = (double)yc[v][0], ys = (double)yc[v][1];
-                    const double ycp = (double)yc[v][2], ysp = (double)yc[v][3];
-                    const double g2 = cA * yr - sA * ys;
-                    const double g1 = (ki + kc[v]) * g2 + cA * ycp - sA * ysp;
-                    dom += gb[v] * g1;
-                    dph += gb[v] * g2;
+                    pacc += gb[v] * (cA * yr - sA * ys);
                 }
             }
     }
-    red[0][tid] = dom;
-    red[1][tid] = dph;
+    red[tid] = pacc;
     __syncthreads();
     for (int wd = 128; wd > 0; wd >>= 1) {
-        if (tid < wd) { red[0][tid] += red[0][tid + wd]; red[1][tid] += red[1][tid + wd]; }
+        if (tid < wd) red[tid] += red[tid + wd];
         __syncthreads();
     }
     if (tid == 0) {
-        dpart[((size_t)b * N + i) * 2] = red[0][0];
-        dpart[((size_t)b * N + i) * 2 + 1] = red[1][0];
+        dpart[((size_t)b * N + i) * 2] = ki * red[0];
+        dpart[((size_t)b * N + i) * 2 + 1] = red[0] * (1.0 / 3.0);
     }
 }
 
@@ -519,7 +541,7 @@ static int ensure_W(mcmb_ctx* c, int prec)
 // columns per shape, rounded up to whole GEMM tiles).
 static int stage1_ld(int nb, int R)
 {
-    return (nb * 4 * R + PT - 1) / PT * PT;
+    return (nb * 2 * R + PT - 1) / PT * PT;
 }
 
 // FP32 stage-1 GEMM C[M][ldc] = A[M][Kp] B[Kp][ldb] (B zero-padded to ldb
@@ -620,7 +642,7 @@ static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, int& ldy, DevB
     DevBuf& Zb = c->pz;
     PointSet& K = c->ks;
     const int N = K.lmax + 1, R = K.R, Kp = (N + PK - 1) / PK * PK;
-    const int Nc = nb * 4 * R;
+    const int Nc = nb * 2 * R;
     const int ldz = stage1_ld(nb, R);
     ldy = ldz;
     MCMB_TRY(Zb.alloc(sizeof(T) * (size_t)Kp * ldz));
@@ -662,14 +684,15 @@ static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double*
     DevBuf& Hb = c->ph;
     int ldy = 0;
     MCMB_TRY(run_stage1<T>(c, d_theta, nb, ldy, Yb));
-    MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)PBATCH * 3 * N * R * R));
+    MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)PBATCH * 2 * N * R * R));
     MCMB_TRY(sincos_table(c, d_theta, nb));
     hipLaunchKernelGGL(proj_stage2<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
                        K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb,
                        (const double*)c->psc.as<double>(), Hb.as<double>());
     MCMB_TRY(c->pdm.alloc(sizeof(double) * (size_t)PBATCH * 3 * R * R * R));
     hipLaunchKernelGGL(proj_stage3, dim3((unsigned)(nb * 3 * ((R * R + 15) / 16))), dim3(64), 0, c->stream,
-                       (const double*)Hb.as<double>(), K.d_q.as<double>(), K.ldq, N, R, c->pdm.as<double>());
+                       (const double*)Hb.as<double>(), K.d_q.as<double>(), K.ldq, K.d_kn.as<double>(), N, R,
+                       c->pdm.as<double>());
     const int ne = nb * 3 * n;
     hipLaunchKernelGGL(proj_stage3_modes, dim3((ne + 255) / 256), dim3(256), 0, c->stream,
                        (const double*)c->pdm.as<double>(), R, K.d_modes.as<int>(), n, nb, d_alpha, d_dalpha);
@@ -694,7 +717,7 @@ static int project_bwd_batch(mcmb_ctx* c, const double* d_theta, int nb, const d
     if (Ycached && Ycached->bytes < sizeof(T) * (size_t)M * ldy) Ycached = nullptr;
     DevBuf& Yb = Ycached ? *Ycached : c->py;
     if (!Ycached) MCMB_TRY(run_stage1<T>(c, d_theta, nb, ldy, Yb));
-    MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)PBATCH * 3 * N * R * R));
+    MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)PBATCH * N * R * R));
     MCMB_TRY(Dp.alloc(sizeof(double) * (size_t)PBATCH * N * 2));
     const unsigned hblocks = (unsigned)(nb * ((R * R + 15) / 16));
     hipLaunchKernelGGL(proj_hbar, dim3(hblocks), dim3(64), 0, c->stream, d_gbar, d_slot, K.d_q.as<double>(), K.ldq,

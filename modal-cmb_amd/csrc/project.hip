@@ -193,6 +193,20 @@ __global__ void proj_sincos_tab(const double* __restrict__ kn, const double* __r
     scb[2 * e + 1] = cb;
 }
 
+// Workgroup -> (shape b, node i) of the per-node kernels, 8 ceil(nb / 8) N
+// workgroups: workgroups k, k + 8, ... share an XCD (and its L2), so XCD x
+// takes the shapes x G .. x G + G - 1 (G = ceil(nb / 8)) of each node in
+// turn, whose stage-1 columns are adjacent in every pair row they read.
+// False for the padding workgroups of a partial batch.
+__device__ __forceinline__ bool shape_node(int nb, int& b, int& i)
+{
+    const int G = (nb + 7) >> 3, xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    i = slot / G;
+    b = xcd * G + slot % G;
+    return b < nb;
+}
+__host__ __forceinline__ unsigned shape_node_blocks(int nb, int N) { return 8u * (unsigned)((nb + 7) / 8) * N; }
+
 // ---- forward stage 2: per (shape b, i) -------------------------------------
 // H[b][x][i][s][t] = sum_j q_s(j) G_x(i, j, t) with G_x(i, j, t) = sum_k
 // W_ijk {sin, cos}(omega K + phi) q_t(k), x = 0 (sin: the value) and x = 1
@@ -213,7 +227,8 @@ __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int 
     // sin / cos omega k_j) while the current chunk's products run.
     __shared__ double G[2][32][32];     // [x][j in chunk][t]
     __shared__ double Qs[32][32];       // [j in chunk][s]
-    const int b = blockIdx.x % nb, i = blockIdx.x / nb;
+    int b, i;
+    if (!shape_node(nb, b, i)) return;   // the whole workgroup
     const int tid = threadIdx.x, t = tid & 31, jr = tid >> 5;
     const int lane = tid & 63, w = tid >> 6, sm = 16 * (w >> 1), tn = 16 * (w & 1);
     const double ki = kn[i];
@@ -284,8 +299,10 @@ __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int 
 //   x = 0: sum_i q_r(i) H[b][0][i][st]       (alpha)
 //   x = 1: sum_i q_r(i) H[b][1][i][st]       (d alpha / d phi)
 //   x = 2: sum_i k_i q_r(i) H[b][1][i][st]   (E, the k_i-weighted cos form)
-// One wave per (shape b, x, block of 16 columns st), the two r tiles sharing
-// each B load (H, read once per x); K runs over the nodes i.
+// One wave per (shape b, H block y, block of 16 columns st): y = 0 makes x =
+// 0, y = 1 makes x = 1 and 2 from the same B loads (each H element is read
+// once); the two r tiles share them too.  K runs over the nodes i, the loads
+// of 4 K-steps issued before their products.
 //
 // E gives d alpha / d omega = sum W K cos(omega K + phi) q_r q_s q_t: W and
 // the cos term are symmetric in the three legs, so each of k_j and k_k can
@@ -296,34 +313,51 @@ __global__ __launch_bounds__(64) void proj_stage3(const double* __restrict__ H, 
 {
     const int lane = threadIdx.x;
     const int RR = R * R, nblk = (RR + 15) / 16;
-    const int bx = blockIdx.x / nblk, st = 16 * (blockIdx.x % nblk) + (lane & 15);
-    const int b = bx / 3, x = bx % 3;
-    const bool kw = x == 2;   // uniform over the wave
-    const double* const h = H + ((size_t)b * 2 + min(x, 1)) * N * RR + min(st, RR - 1);
-    d4 acc[2] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    const int by = blockIdx.x / nblk, st = 16 * (blockIdx.x % nblk) + (lane & 15);
+    const int b = by >> 1, y = by & 1;   // y uniform over the wave
+    const double* const h = H + ((size_t)b * 2 + y) * N * RR + min(st, RR - 1);
+    const int r0 = lane & 15, r1 = 16 + (lane & 15);
+    d4 acc[2][2];   // [w: 0 plain, 1 k-weighted][r tile]
+#pragma unroll
+    for (int w = 0; w < 2; ++w)
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) acc[w][rt] = (d4){0.0, 0.0, 0.0, 0.0};
     const int nks = (N + 3) / 4;
-#pragma unroll 5
-    for (int ks = 0; ks < nks; ++ks) {
-        const int i = 4 * ks + (lane >> 4);
-        const bool in = i < N;
-        const int ic = min(i, N - 1);
-        const double bv = in ? h[(size_t)ic * RR] : 0.0;
-        const int r0 = lane & 15, r1 = 16 + (lane & 15);
-        const double kf = kw ? kn[ic] : 1.0;
-        const double a0 = r0 < R ? kf * q[(size_t)ic * ldq + r0] : 0.0;
-        const double a1 = r1 < R ? kf * q[(size_t)ic * ldq + r1] : 0.0;
-        acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, bv, acc[0], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, bv, acc[1], 0, 0, 0);
+    constexpr int U = 4;
+    for (int k0 = 0; k0 < nks; k0 += U) {
+        double bv[U], a0[U], a1[U], kf[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            // K-steps past the last are zero through bv (i >= N)
+            const int i = 4 * (k0 + u) + (lane >> 4);
+            const int ic = min(i, N - 1);
+            bv[u] = i < N ? h[(size_t)ic * RR] : 0.0;
+            a0[u] = r0 < R ? q[(size_t)ic * ldq + r0] : 0.0;
+            a1[u] = r1 < R ? q[(size_t)ic * ldq + r1] : 0.0;
+            kf[u] = kn[ic];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[u], bv[u], acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[u], bv[u], acc[0][1], 0, 0, 0);
+            if (y) {
+                acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(kf[u] * a0[u], bv[u], acc[1][0], 0, 0, 0);
+                acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(kf[u] * a1[u], bv[u], acc[1][1], 0, 0, 0);
+            }
+        }
     }
-    // element v of tile rt: r = 16 rt + 4 v + lane / 16, column st
+    // element v of tile rt: r = 16 rt + 4 v + lane / 16, column st; x = y + w
     if (st < RR)
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
+        for (int w = 0; w < 2; ++w)
+            if (w <= y)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int r = 16 * rt + 4 * v + (lane >> 4);
-                if (r < R) D[((size_t)bx * R + r) * RR + st] = acc[rt][v];
-            }
+                for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = 16 * rt + 4 * v + (lane >> 4);
+                        if (r < R) D[(((size_t)b * 3 + y + w) * R + r) * RR + st] = acc[w][rt][v];
+                    }
 }
 
 // alpha[b][m] = D[b][0][r][s R + t], d alpha / d omega = E[r][s][t] +
@@ -417,7 +451,8 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
     __shared__ double Hs[32][32];   // [s][t]
     __shared__ double Qs[32][32];   // [s][j in chunk]
     __shared__ double red[256];
-    const int b = blockIdx.x % nb, i = blockIdx.x / nb;
+    int b, i;
+    if (!shape_node(nb, b, i)) return;   // the whole workgroup
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int jm = 16 * (w >> 1), tn = 16 * (w & 1);
     const int t = tn + (lane & 15);   // the lane's output column; rows jm + 4 v + lane / 16
@@ -492,15 +527,17 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
     }
 }
 
-// dtheta[b][x] = sum_i dpart[b][i][x] (fixed order)
-__global__ void proj_sum_parts(const double* __restrict__ dpart, int N, int nb, double* __restrict__ out)
+// dtheta[b][x] = sum_i dpart[b][i][x]: one wave per (b, x), lane l summing
+// i = l, l + 64, ..., then a butterfly over the lanes (a fixed order)
+__global__ __launch_bounds__(64) void proj_sum_parts(const double* __restrict__ dpart, int N, int nb,
+                                                     double* __restrict__ out)
 {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nb * 2) return;
-    int b = e / 2, x = e % 2;
+    const int e = blockIdx.x, b = e / 2, x = e % 2, lane = threadIdx.x;
     double s = 0.0;
-    for (int i = 0; i < N; ++i) s += dpart[((size_t)b * N + i) * 2 + x];
-    out[e] = s;
+    for (int i = lane; i < N; i += 64) s += dpart[((size_t)b * N + i) * 2 + x];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) out[e] = s;
 }
 
 // ---- host side ---------------------------------------------------------------
@@ -686,11 +723,11 @@ static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double*
     MCMB_TRY(run_stage1<T>(c, d_theta, nb, ldy, Yb));
     MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)PBATCH * 2 * N * R * R));
     MCMB_TRY(sincos_table(c, d_theta, nb));
-    hipLaunchKernelGGL(proj_stage2<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
+    hipLaunchKernelGGL(proj_stage2<T>, dim3(shape_node_blocks(nb, N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
                        K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb,
                        (const double*)c->psc.as<double>(), Hb.as<double>());
     MCMB_TRY(c->pdm.alloc(sizeof(double) * (size_t)PBATCH * 3 * R * R * R));
-    hipLaunchKernelGGL(proj_stage3, dim3((unsigned)(nb * 3 * ((R * R + 15) / 16))), dim3(64), 0, c->stream,
+    hipLaunchKernelGGL(proj_stage3, dim3((unsigned)(nb * 2 * ((R * R + 15) / 16))), dim3(64), 0, c->stream,
                        (const double*)Hb.as<double>(), K.d_q.as<double>(), K.ldq, K.d_kn.as<double>(), N, R,
                        c->pdm.as<double>());
     const int ne = nb * 3 * n;
@@ -723,10 +760,10 @@ static int project_bwd_batch(mcmb_ctx* c, const double* d_theta, int nb, const d
     hipLaunchKernelGGL(proj_hbar, dim3(hblocks), dim3(64), 0, c->stream, d_gbar, d_slot, K.d_q.as<double>(), K.ldq,
                        N, R, n, nb, Hb.as<double>());
     MCMB_TRY(sincos_table(c, d_theta, nb));
-    hipLaunchKernelGGL(proj_adjoint<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
+    hipLaunchKernelGGL(proj_adjoint<T>, dim3(shape_node_blocks(nb, N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
                        K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb,
                        (const double*)c->psc.as<double>(), Hb.as<double>(), Dp.as<double>());
-    hipLaunchKernelGGL(proj_sum_parts, dim3((nb * 2 + 255) / 256), dim3(256), 0, c->stream, Dp.as<double>(), N, nb,
+    hipLaunchKernelGGL(proj_sum_parts, dim3((unsigned)(nb * 2)), dim3(64), 0, c->stream, Dp.as<double>(), N, nb,
                        d_dtheta);
     MCMB_HIP(hipGetLastError());
     return MCMB_OK;

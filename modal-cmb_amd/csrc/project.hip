@@ -193,20 +193,6 @@ __global__ void proj_sincos_tab(const double* __restrict__ kn, const double* __r
     scb[2 * e + 1] = cb;
 }
 
-// Workgroup -> (shape b, node i) of the per-node kernels, 8 ceil(nb / 8) N
-// workgroups: workgroups k, k + 8, ... share an XCD (and its L2), so XCD x
-// takes the shapes x G .. x G + G - 1 (G = ceil(nb / 8)) of each node in
-// turn, whose stage-1 columns are adjacent in every pair row they read.
-// False for the padding workgroups of a partial batch.
-__device__ __forceinline__ bool shape_node(int nb, int& b, int& i)
-{
-    const int G = (nb + 7) >> 3, xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    i = slot / G;
-    b = xcd * G + slot % G;
-    return b < nb;
-}
-__host__ __forceinline__ unsigned shape_node_blocks(int nb, int N) { return 8u * (unsigned)((nb + 7) / 8) * N; }
-
 // ---- forward stage 2: per (shape b, i) -------------------------------------
 // H[b][x][i][s][t] = sum_j q_s(j) G_x(i, j, t) with G_x(i, j, t) = sum_k
 // W_ijk {sin, cos}(omega K + phi) q_t(k), x = 0 (sin: the value) and x = 1
@@ -227,8 +213,7 @@ __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int 
     // sin / cos omega k_j) while the current chunk's products run.
     __shared__ double G[2][32][32];     // [x][j in chunk][t]
     __shared__ double Qs[32][32];       // [j in chunk][s]
-    int b, i;
-    if (!shape_node(nb, b, i)) return;   // the whole workgroup
+    const int b = blockIdx.x % nb, i = blockIdx.x / nb;
     const int tid = threadIdx.x, t = tid & 31, jr = tid >> 5;
     const int lane = tid & 63, w = tid >> 6, sm = 16 * (w >> 1), tn = 16 * (w & 1);
     const double ki = kn[i];
@@ -451,8 +436,7 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
     __shared__ double Hs[32][32];   // [s][t]
     __shared__ double Qs[32][32];   // [s][j in chunk]
     __shared__ double red[256];
-    int b, i;
-    if (!shape_node(nb, b, i)) return;   // the whole workgroup
+    const int b = blockIdx.x % nb, i = blockIdx.x / nb;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int jm = 16 * (w >> 1), tn = 16 * (w & 1);
     const int t = tn + (lane & 15);   // the lane's output column; rows jm + 4 v + lane / 16
@@ -673,13 +657,100 @@ __global__ __launch_bounds__(256, 2) void proj_gemm_f32(const float* __restrict_
             }
 }
 
+// FP64 stage-1 GEMM, the same scheme with 16-byte (two-double) operand
+// loads two chunks ahead: a chunk is 1024 double2 of A (row r = e / 8, k = 2
+// (e % 8)) and 1024 of B (k = e / 64, n = 2 (e % 64)), e = tid + 256 u, u <
+// 4.  A is stored [m][k] with a row stride of 18 doubles (16-byte aligned
+// rows), B [k][n] with 136.  It issues the MFMA sequence of proj_gemm<double>
+// (same operands, same K order).
+constexpr int PD_LDA = PK + 2, PD_LDB = PT + 8;
+typedef double d2v __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256, 2) void proj_gemm_f64(const double* __restrict__ A, const double* __restrict__ B,
+                                                        int M, int Kp, int ldb, double* __restrict__ C, int ldc)
+{
+    __shared__ __align__(16) double As[2][PT][PD_LDA];
+    __shared__ __align__(16) double Bs[2][PK][PD_LDB];
+    const int m0 = blockIdx.y * PT, n0 = blockIdx.x * PT;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    d4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = (d4){0.0, 0.0, 0.0, 0.0};
+    const d2v z2 = {0.0, 0.0};
+    auto gload = [&](int k0, d2v (&ra)[4], d2v (&rb)[4]) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            const int r = e >> 3, kq = e & 7, gm = m0 + r;
+            ra[u] = gm < M ? *reinterpret_cast<const d2v*>(A + (size_t)gm * Kp + k0 + 2 * kq) : z2;
+            const int kb = e >> 6, c2 = e & 63;
+            rb[u] = *reinterpret_cast<const d2v*>(B + (size_t)(k0 + kb) * ldb + n0 + 2 * c2);
+        }
+    };
+    auto sstore = [&](int buf, const d2v (&ra)[4], const d2v (&rb)[4]) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            *reinterpret_cast<d2v*>(&As[buf][e >> 3][2 * (e & 7)]) = ra[u];
+            *reinterpret_cast<d2v*>(&Bs[buf][e >> 6][2 * (e & 63)]) = rb[u];
+        }
+    };
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int ks = 0; ks < PK / 4; ++ks) {
+            const int kk = ks * 4 + (lane >> 4);
+            double av[4], bv[4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) av[a] = As[buf][wm * 64 + a * 16 + (lane & 15)][kk];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) bv[b] = Bs[buf][kk][wn * 64 + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+    };
+    // chunk kc is in LDS buffer kc & 1, chunk kc + 1 in registers and chunk
+    // kc + 2 loading; loads past the last chunk repeat it (unused)
+    const int nk = Kp / PK;
+    d2v ra0[4], rb0[4], ra1[4], rb1[4];
+    gload(0, ra0, rb0);
+    gload(min(1, nk - 1) * PK, ra1, rb1);
+    sstore(0, ra0, rb0);
+    __syncthreads();
+    for (int kc = 0;; kc += 2) {
+        gload(min(kc + 2, nk - 1) * PK, ra0, rb0);
+        compute(0);
+        sstore(1, ra1, rb1);
+        __syncthreads();
+        if (kc + 1 >= nk) break;
+        gload(min(kc + 3, nk - 1) * PK, ra1, rb1);
+        compute(1);
+        sstore(0, ra0, rb0);
+        __syncthreads();
+        if (kc + 2 >= nk) break;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gm = m0 + wm * 64 + a * 16 + MF<double>::row(v, lane);
+                const int gn = n0 + wn * 64 + b * 16 + (lane & 15);
+                if (gm < M) C[(size_t)gm * ldc + gn] = acc[a][b][v];
+            }
+}
+
 template <class T>
 static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, int& ldy, DevBuf& Yb)
 {
     DevBuf& Zb = c->pz;
     PointSet& K = c->ks;
     const int N = K.lmax + 1, R = K.R, Kp = (N + PK - 1) / PK * PK;
-    const int Nc = nb * 2 * R;
     const int ldz = stage1_ld(nb, R);
     ldy = ldz;
     MCMB_TRY(Zb.alloc(sizeof(T) * (size_t)Kp * ldz));
@@ -693,8 +764,8 @@ static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, int& ldy, DevB
         hipLaunchKernelGGL(proj_gemm_f32, grid, dim3(256), 0, c->stream, (const float*)c->d_W.as<T>(),
                            (const float*)Zb.as<T>(), (int)M, Kp, ldz, (float*)Yb.as<T>(), ldy);
     else
-        hipLaunchKernelGGL(proj_gemm<T>, grid, dim3(256), 0, c->stream, c->d_W.as<T>(), Zb.as<T>(), M, Kp, Nc, ldz,
-                           Yb.as<T>(), ldy);
+        hipLaunchKernelGGL(proj_gemm_f64, grid, dim3(256), 0, c->stream, (const double*)c->d_W.as<T>(),
+                           (const double*)Zb.as<T>(), (int)M, Kp, ldz, (double*)Yb.as<T>(), ldy);
     MCMB_HIP(hipGetLastError());
     return MCMB_OK;
 }
@@ -723,7 +794,7 @@ static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double*
     MCMB_TRY(run_stage1<T>(c, d_theta, nb, ldy, Yb));
     MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)PBATCH * 2 * N * R * R));
     MCMB_TRY(sincos_table(c, d_theta, nb));
-    hipLaunchKernelGGL(proj_stage2<T>, dim3(shape_node_blocks(nb, N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
+    hipLaunchKernelGGL(proj_stage2<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
                        K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb,
                        (const double*)c->psc.as<double>(), Hb.as<double>());
     MCMB_TRY(c->pdm.alloc(sizeof(double) * (size_t)PBATCH * 3 * R * R * R));
@@ -760,7 +831,7 @@ static int project_bwd_batch(mcmb_ctx* c, const double* d_theta, int nb, const d
     hipLaunchKernelGGL(proj_hbar, dim3(hblocks), dim3(64), 0, c->stream, d_gbar, d_slot, K.d_q.as<double>(), K.ldq,
                        N, R, n, nb, Hb.as<double>());
     MCMB_TRY(sincos_table(c, d_theta, nb));
-    hipLaunchKernelGGL(proj_adjoint<T>, dim3(shape_node_blocks(nb, N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
+    hipLaunchKernelGGL(proj_adjoint<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
                        K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb,
                        (const double*)c->psc.as<double>(), Hb.as<double>(), Dp.as<double>());
     hipLaunchKernelGGL(proj_sum_parts, dim3((unsigned)(nb * 2)), dim3(64), 0, c->stream, Dp.as<double>(), N, nb,

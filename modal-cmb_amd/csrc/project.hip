@@ -431,8 +431,8 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
     // Gbar[j][t] = sum_s q_s(j) Hbar[s][t] on FP64 MFMA 16x16x4 (A = Q, j x s;
     // B = Hbar, s x t; wave w owns the j tile w / 2 and the t tile w % 2),
     // then each lane adds Gbar G_1 at its 4 output elements.  The Y and sin /
-    // cos omega k_j of those elements in the next chunk are loaded while the
-    // current chunk's products run.
+    // cos omega k_j of those elements and the basis entries of Qs in the next
+    // chunk are loaded while the current chunk's products run.
     __shared__ double Hs[32][32];   // [s][t]
     __shared__ double Qs[32][32];   // [s][j in chunk]
     __shared__ double red[256];
@@ -449,9 +449,10 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
         Hs[ss][tt] = (ss < R && tt < R) ? Hb[(((size_t)b * N) + i) * R * R + ss * R + tt] : 0.0;
     }
     T yv[4][2];
-    double sv[4], cv[4];
-    // the lane's elements (j0 + jm + 4 v + lane / 16, t) of a chunk (nodes
-    // past N: the last node; columns past R: the last column; both unused)
+    double sv[4], cv[4], qn[4];
+    // the lane's elements (j0 + jm + 4 v + lane / 16, t) of a chunk and the
+    // thread's 4 entries of its Qs (nodes past N: the last node; columns past
+    // R: the last column; all unused)
     auto fetch = [&](int j0) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
@@ -462,14 +463,20 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
             sv[v] = sc[2 * j];
             cv[v] = sc[2 * j + 1];
         }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int e = tid + 256 * k, ss = e >> 5, jj = e & 31;
+            qn[k] = q[(size_t)min(j0 + jj, N - 1) * ldq + min(ss, R - 1)];
+        }
     };
     double pacc = 0.0;
     fetch(0);
     for (int j0 = 0; j0 < N; j0 += 32) {
         __syncthreads();   // Hs written / the previous chunk is done with Qs
-        for (int e = tid; e < 32 * 32; e += 256) {
-            const int ss = e >> 5, jj = e & 31, j = j0 + jj;
-            Qs[ss][jj] = (ss < R && j < N) ? q[(size_t)j * ldq + ss] : 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int e = tid + 256 * k, ss = e >> 5, jj = e & 31;
+            Qs[ss][jj] = (ss < R && j0 + jj < N) ? qn[k] : 0.0;
         }
         __syncthreads();
         // this chunk's element data, then the next chunk's loads

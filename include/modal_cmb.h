@@ -140,7 +140,10 @@ int mcmb_lspace_E_info(mcmb_ctx* ctx, int* E_lmin, int* E_lmax, int* nmodes, lon
  *   TEE: sum_s w_s/(16 pi) M^T_{m0 n0} (M^E_{m1 n1} M^E_{m2 n2} + M^E_{m2 n1} M^E_{m1 n2}).
  * npt <= 0: 3 max(T lmax, E lmax)/2 + 1 (init_orthol_glint with
  * polarisation); npt must be exact for the component's degree.  n x n with n
- * = the l-space mode count (TTT) or the E setup's (the others). */
+ * = the l-space mode count (TTT) or the E setup's (the others).  When the
+ * pair products of the bases have a low numerical rank (see
+ * mcmb_last_basis_rank) the node sum runs over the compressed bases, the
+ * same Gram to rounding; MCMB_GL_PATH=direct keeps the direct node sum. */
 int mcmb_gamma_gl_pol(mcmb_ctx* ctx, int pol, int npt, double* gamma_host, double* gamma_dev);
 
 /* The same GL Gram into the caller's device buffer gamma_dev (n x n doubles)
@@ -432,7 +435,10 @@ int mcmb_last_setup(mcmb_ctx* ctx, int cap, double* ms, const char** names);
  * NP / 2 (NP = R(R+1)/2) and were replaced by an orthonormal basis of their
  * span, 0 when the NP pair products themselves were contracted (for the GL
  * Gram: the direct node sum), -1 when the last Gram took another contraction
- * order.  No reference counterpart. */
+ * order.  For the polarised GL components (mcmb_gamma_gl_pol) it is the rank
+ * of the basis on two legs (T for TTE, E for TEE and EEE); the mixed
+ * components contract over both bases only when both have such a rank.  No
+ * reference counterpart. */
 int mcmb_last_basis_rank(mcmb_ctx* ctx, int* rank);
 
 /* ---- reference-named entry points (legacy shims over the bound context) --- */

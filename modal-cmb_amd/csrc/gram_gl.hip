@@ -397,6 +397,14 @@ int gram_gl_pol_launch(mcmb_ctx* c, int pol, int npt, double* d_gamma)
     const int n = E.n, T = (n + 63) / 64, ntiles = T * (T + 1) / 2;
     const double *x, *w;
     MCMB_TRY(gl_nodes(c, npt, &x, &w));
+    {
+        // over the compressed bases when their pair products have a low rank
+        bool used = false;
+        if (pol == 3) MCMB_TRY(gram_gl_eee_compressed(c, R[1], npt, x, w, d_gamma, &used));
+        else MCMB_TRY(gram_gl_mixed_compressed(c, pol, R[0], R[1], npt, x, w, d_gamma, &used));
+        if (used) return MCMB_OK;
+    }
+    c->last_rank = 0;
     // x, w, then per used field Pt, U, M; then the tile list
     size_t bytes = 2 * round256(sizeof(double) * npt) + round256(sizeof(int2) * (size_t)ntiles);
     for (int f = 0; f < 2; ++f)

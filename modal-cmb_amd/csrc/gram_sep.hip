@@ -1075,6 +1075,79 @@ __global__ __launch_bounds__(256) void sep_assemble_sym(const double* __restrict
     }
 }
 
+// A polarised Gamma from the mixed GL expansion K[x][{y, z}] (x a pair of
+// the field on one leg, {y, z} two pairs of the field on the other two legs,
+// in pair order), over 32 x 32 tiles (bm <= bn) placed as in
+// sep_assemble_sym, with p_xy = pair(a_x, b_y) of T and u_xy of E:
+//   KIND 1 (TTE, legs T T E): K[u22][{p00, p11}] + K[u22][{p10, p01}]
+//   KIND 2 (TEE, legs T E E): K[p00][{u11, u22}] + K[p00][{u21, u12}]
+// (the two matchings of orthogonalisation_late.c:302-316; the 1/(16 pi) is
+// in the core).  Both sums are symmetric in m and n.
+template <int KIND, int S>
+__global__ __launch_bounds__(256) void sep_assemble_mix(const double* __restrict__ K, long ldk, int RT, int RE,
+                                                        const int* __restrict__ modes, int n, int nt,
+                                                        double* __restrict__ gamma)
+{
+    __shared__ double T[32][33];
+    const int nb = (nt + S - 1) / S;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    int rem = (slot / (S * S)) * 8 + xcd;
+    const int sub = slot % (S * S);
+    if (rem >= nb * (nb + 1) / 2) return;
+    int sm = 0;
+    while (rem >= nb - sm) {
+        rem -= nb - sm;
+        ++sm;
+    }
+    const int bm = S * sm + sub / S, bn = S * (sm + rem) + sub % S;
+    if (bm > bn || bn >= nt) return;
+    const int c = threadIdx.x & 31, r0 = threadIdx.x >> 5;
+    const int nn = 32 * bn + c;
+    const int pn = nn < n ? modes[nn] : 0;
+    const int b0 = pn & 255, b1 = (pn >> 8) & 255, b2 = (pn >> 16) & 255;
+    auto pr = [](int x, int y, int R) { return pair_index(min(x, y), max(x, y), R); };
+    const int NPP = KIND == 1 ? RT * (RT + 1) / 2 : RE * (RE + 1) / 2;   // pairs of the paired field
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = r0 + 8 * i, m = 32 * bm + r;
+        double acc = 0.0;
+        if (m < n && nn < n && m <= nn) {
+            const int pm = modes[m];
+            const int a0 = pm & 255, a1 = (pm >> 8) & 255, a2 = (pm >> 16) & 255;
+            int x, y1, z1, y2, z2;
+            if (KIND == 1) {
+                x = pr(a2, b2, RE);
+                y1 = pr(a0, b0, RT);
+                z1 = pr(a1, b1, RT);
+                y2 = pr(a1, b0, RT);
+                z2 = pr(a0, b1, RT);
+            } else {
+                x = pr(a0, b0, RT);
+                y1 = pr(a1, b1, RE);
+                z1 = pr(a2, b2, RE);
+                y2 = pr(a2, b1, RE);
+                z2 = pr(a1, b2, RE);
+            }
+            const double* const Kx = K + (long)x * ldk;
+            acc = Kx[pr(y1, z1, NPP)] + Kx[pr(y2, z2, NPP)];
+        }
+        T[r][c] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = r0 + 8 * i, m = 32 * bm + r;
+        if (m < n && nn < n) __builtin_nontemporal_store(m <= nn ? T[r][c] : T[c][r], &gamma[(long)m * n + nn]);
+    }
+    if (bm != bn) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int r = r0 + 8 * i, row = 32 * bn + r, col = 32 * bm + c;
+            if (row < n && col < n) __builtin_nontemporal_store(T[c][r], &gamma[(long)row * n + col]);
+        }
+    }
+}
+
 bool gram_separable_supported(const PointSet& L, int precision)
 {
     // (group table entries keep two legs in 16 bits each)
@@ -1900,11 +1973,14 @@ __global__ __launch_bounds__(256) void sep_gl_phi(const double* __restrict__ Pw,
 // H[g][a r + b] = H[g][b r + a] = scale sum_s w_s Phi_g(s) Phi_a(s) Phi_b(s),
 // one workgroup per a <= b: t = w Phi_a Phi_b in LDS chunks of GL_SC nodes,
 // wave q sums the columns g = q, q + 4, ... over its lanes (fixed order).
+// MIX (two fields): Phi_g from PhiG (rg columns) and the entries go to
+// H[b][g r + a] = H[a][g r + b] (the paired field's index first).
 constexpr int GL_SC = 1024;
-constexpr int GL_GW = 16;   // columns g per wave (r <= 64)
-__global__ __launch_bounds__(256) void sep_gl_core(const double* __restrict__ Phi, int npt,
-                                                   const double* __restrict__ w, int r, double scale,
-                                                   double* __restrict__ H, long ldh)
+constexpr int GL_GW = 16;   // columns g per wave (r, rg <= 64)
+template <bool MIX>
+__global__ __launch_bounds__(256) void sep_gl_core(const double* __restrict__ Phi, const double* __restrict__ PhiG,
+                                                   int rg, int npt, const double* __restrict__ w, int r,
+                                                   double scale, double* __restrict__ H, long ldh)
 {
     __shared__ double ts[GL_SC];
     int a = 0, rem = blockIdx.x;   // pair (a, b), a <= b, row-major upper triangle
@@ -1928,8 +2004,8 @@ __global__ __launch_bounds__(256) void sep_gl_core(const double* __restrict__ Ph
 #pragma unroll
         for (int k = 0; k < GL_GW; ++k) {
             const int g = q + 4 * k;
-            if (g < r)
-                for (int e = lane; e < ns; e += 64) acc[k] = fma(Phi[(long)g * npt + s0 + e], ts[e], acc[k]);
+            if (g < rg)
+                for (int e = lane; e < ns; e += 64) acc[k] = fma(PhiG[(long)g * npt + s0 + e], ts[e], acc[k]);
         }
     }
 #pragma unroll
@@ -1938,29 +2014,70 @@ __global__ __launch_bounds__(256) void sep_gl_core(const double* __restrict__ Ph
         double t = acc[k];
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) t += __shfl_xor(t, o);
-        if (g < r && lane == 0) {
-            H[(long)g * ldh + (long)a * r + b] = scale * t;
-            H[(long)g * ldh + (long)b * r + a] = scale * t;
+        if (g < rg && lane == 0) {
+            if (MIX) {
+                H[(long)b * ldh + (long)g * r + a] = scale * t;
+                H[(long)a * ldh + (long)g * r + b] = scale * t;
+            } else {
+                H[(long)g * ldh + (long)a * r + b] = scale * t;
+                H[(long)g * ldh + (long)b * r + a] = scale * t;
+            }
         }
     }
 }
+
+// One field of a compressed GL Gram whose three legs are that field: its
+// basis (the first R columns of q[l][col], nl rows from l = 0), per-leg
+// weight v, lmin, the mode table (legs packed 8 bits each) and the
+// compression cache kept for that basis.
+struct GlField3 {
+    BasisRef B;
+    int nl, R, lmin;
+    const double* d_v;
+    int n;
+    const int* d_modes;
+    SepComp* S;
+};
+
+static int gl_compressed_run(mcmb_ctx* c, const GlField3& F, int npt, const double* x, const double* w,
+                             double* d_gamma, bool* used);
 
 int gram_gl_compressed(mcmb_ctx* c, int npt, const double* x, const double* w, double* d_gamma, bool* used)
 {
     *used = false;
     PointSet& L = c->ls;
+    if (!gram_separable_supported(L, MCMB_FP64)) return MCMB_OK;
+    const GlField3 F{BasisRef{L.basis_key(), L.basis_generation(), L.d_q.as<double>(), L.ldq},
+                     (int)(L.q.size() / (size_t)L.ldq), L.R, L.lmin, L.d_v.as<double>(), L.n,
+                     L.d_modes.as<int>(), &c->sepc};
+    return gl_compressed_run(c, F, npt, x, w, d_gamma, used);
+}
+
+int gram_gl_eee_compressed(mcmb_ctx* c, int R, int npt, const double* x, const double* w, double* d_gamma,
+                           bool* used)
+{
+    *used = false;
+    PolSet& E = c->es;
+    if (R < 1 || R > 255 || R * (R + 1) / 2 > SR_NPMAX || E.lmax >= 32768) return MCMB_OK;
+    const GlField3 F{BasisRef{&c->es, E.generation, E.d_q.as<double>(), E.ldq}, E.lmax + 1, R, E.lmin,
+                     E.d_v.as<double>(), E.n, E.d_modes[2].as<int>(), &c->sepc_e};
+    return gl_compressed_run(c, F, npt, x, w, d_gamma, used);
+}
+
+static int gl_compressed_run(mcmb_ctx* c, const GlField3& F, int npt, const double* x, const double* w,
+                             double* d_gamma, bool* used)
+{
+    *used = false;
     const char* genv = getenv("MCMB_GL_PATH");
     if (genv && !strcmp(genv, "direct")) return MCMB_OK;
-    if (!gram_separable_supported(L, MCMB_FP64)) return MCMB_OK;
-    const int n = L.n, R = L.R, NP = R * (R + 1) / 2;
+    const int n = F.n, R = F.R, NP = R * (R + 1) / 2;
     const int ld = NP + (NP & 1);
-    const int nl = (int)(L.q.size() / (size_t)L.ldq);
+    const int nl = F.nl;
     std::vector<int2> pxy(NP);
     for (int a = 0; a < R; ++a)
         for (int b = a; b < R; ++b) pxy[pair_index(a, b, R)] = make_int2(a, b);
-    MCMB_TRY(separable_compress(c, c->sepc, BasisRef{L.basis_key(), L.basis_generation(), L.d_q.as<double>(), L.ldq},
-                                nl, NP, ld, pxy));
-    const SepComp& S = c->sepc;
+    MCMB_TRY(separable_compress(c, *F.S, F.B, nl, NP, ld, pxy));
+    const SepComp& S = *F.S;
     const int r = S.r;
     if (r <= 0 || r > 4 * GL_GW) return MCMB_OK;   // full rank: the direct node sum
     const long ldh = (long)r * r + ((r * r) & 1);
@@ -1978,12 +2095,13 @@ int gram_gl_compressed(mcmb_ctx* c, int npt, const double* x, const double* w, d
     MCMB_HIP(hipEventRecord(c->ev0, c->stream));
     const long ntab = (long)r * S.nlp;
     hipLaunchKernelGGL(sep_gl_tabs, dim3((unsigned)((ntab + 255) / 256)), dim3(256), 0, c->stream,
-                       (const double*)S.pc.as<double>(), S.nlp, nl, r, L.lmin, L.d_v.as<double>(), dpw);
+                       (const double*)S.pc.as<double>(), S.nlp, nl, r, F.lmin, F.d_v, dpw);
     hipLaunchKernelGGL(sep_gl_phi, dim3((unsigned)((npt + 63) / 64), (unsigned)((r + 4 * GL_AG - 1) / (4 * GL_AG))),
                        dim3(256), 0, c->stream, (const double*)dpw, S.nlp, nl, r, (const double*)dx, npt, dphi);
     MCMB_HIP(hipMemsetAsync(S.h.p, 0, sizeof(double) * (size_t)r * ldh, c->stream));
-    hipLaunchKernelGGL(sep_gl_core, dim3((unsigned)(r * (r + 1) / 2)), dim3(256), 0, c->stream,
-                       (const double*)dphi, npt, (const double*)dw, r, 3.0 / (4.0 * M_PI), S.h.as<double>(), ldh);
+    hipLaunchKernelGGL(sep_gl_core<false>, dim3((unsigned)(r * (r + 1) / 2)), dim3(256), 0, c->stream,
+                       (const double*)dphi, (const double*)dphi, r, npt, (const double*)dw, r, 3.0 / (4.0 * M_PI),
+                       S.h.as<double>(), ldh);
     MCMB_HIP(hipGetLastError());
     // the expansion and the assembly of the separable Gram
     const int* tb = S.tab.as<int>();
@@ -2010,7 +2128,7 @@ int gram_gl_compressed(mcmb_ctx* c, int npt, const double* x, const double* w, d
     const unsigned long long ntile = ((unsigned long long)nbk * (nbk + 1) / 2 + 7) / 8 * 8 * SB * SB;
     MCMB_TRY(launch_guard("GL assembly", ntile, 256));
     hipLaunchKernelGGL(sep_assemble_sym<SB>, dim3((unsigned)ntile), dim3(256), 0, c->stream, (const double*)G, ldp, R,
-                       L.d_modes.as<int>(), n, nt, d_gamma);
+                       F.d_modes, n, nt, d_gamma);
     MCMB_HIP(hipGetLastError());
     MCMB_HIP(hipEventRecord(c->ev1, c->stream));
     MCMB_HIP(hipStreamSynchronize(c->stream));   // x and w may live on the caller's stack
@@ -2020,6 +2138,156 @@ int gram_gl_compressed(mcmb_ctx* c, int npt, const double* x, const double* w, d
     c->last_launches = 8;
     c->nphases = 0;
     c->last_rank = r;
+    *used = true;
+    return MCMB_OK;
+}
+
+// The mixed polarised components TTE (pol 1) and TEE (pol 2) over the two
+// compressed bases.  With Phi^T, Phi^E the compressed bases evaluated at the
+// nodes as above, the field P on two legs (T for TTE, E for TEE) and S on
+// the third:
+//   H[b][g rP + a] = 1/(16 pi) sum_s w_s PhiP_a PhiP_b PhiS_g   (a, b < rP; g < rS)
+//   T1[(g, a)][q]  = sum_b H[b][g rP + a] CtP[b][q]
+//   Y[g][{p, q}]   = sum_a CtP[a][p] T1[(g, a)][q]     (symmetric: pair order)
+//   K[u][{p, q}]   = sum_g CtS[g][u] Y[g][{p, q}]
+// so K[u][{p, q}] = 1/(16 pi) sum_s w_s MP_p MP_q MS_u, assembled by
+// sep_assemble_mix.  RT, RE: the basis columns the component's modes use.
+int gram_gl_mixed_compressed(mcmb_ctx* c, int pol, int RT, int RE, int npt, const double* x, const double* w,
+                             double* d_gamma, bool* used)
+{
+    *used = false;
+    const char* genv = getenv("MCMB_GL_PATH");
+    if (genv && !strcmp(genv, "direct")) return MCMB_OK;
+    PointSet& L = c->ls;
+    PolSet& E = c->es;
+    auto fits = [](int R) { return R >= 1 && R <= 255 && R * (R + 1) / 2 <= SR_NPMAX; };
+    if (!fits(RT) || !fits(RE) || L.lmax >= 32768 || E.lmax >= 32768) return MCMB_OK;
+    const int NPT = RT * (RT + 1) / 2, NPE = RE * (RE + 1) / 2;
+    const int ldT = NPT + (NPT & 1), ldE = NPE + (NPE & 1);
+    const int nlT = (int)(L.q.size() / (size_t)L.ldq), nlE = E.lmax + 1;
+    auto pairs = [](int R) {
+        std::vector<int2> v(R * (R + 1) / 2);
+        for (int a = 0; a < R; ++a)
+            for (int b = a; b < R; ++b) v[pair_index(a, b, R)] = make_int2(a, b);
+        return v;
+    };
+    MCMB_TRY(separable_compress(c, c->sepc_pt,
+                                BasisRef{L.basis_key(), L.basis_generation(), L.d_q.as<double>(), L.ldq}, nlT, NPT,
+                                ldT, pairs(RT)));
+    MCMB_TRY(separable_compress(c, c->sepc_e, BasisRef{&c->es, E.generation, E.d_q.as<double>(), E.ldq}, nlE, NPE,
+                                ldE, pairs(RE)));
+    const SepComp& ST = c->sepc_pt;
+    const SepComp& SE = c->sepc_e;
+    const int rT = ST.r, rE = SE.r;
+    if (rT <= 0 || rT > 4 * GL_GW || rE <= 0 || rE > 4 * GL_GW) return MCMB_OK;   // the direct node sum
+    // P: the field on two legs, S: the other
+    const bool tte = pol == 1;
+    const SepComp& SP = tte ? ST : SE;
+    const SepComp& SS = tte ? SE : ST;
+    const int rP = SP.r, rS = SS.r, NPP_ = tte ? NPT : NPE, NPS = tte ? NPE : NPT;
+    const int ldP = tte ? ldT : ldE, ldS = tte ? ldE : ldT;
+    const long NQ = (long)NPP_ * (NPP_ + 1) / 2, ldq = NQ + (NQ & 1);   // pair pairs of P
+    const long ldh = (long)rS * rP + (((long)rS * rP) & 1);
+    // device work: x, w, Phi^T, Phi^E, the two weighted tables, H
+    const size_t nd = 2 * (size_t)npt + (size_t)(rT + rE) * npt + (size_t)rT * ST.nlp + (size_t)rE * SE.nlp +
+                      (size_t)rP * ldh;
+    MCMB_TRY(c->gl_xw.alloc(sizeof(double) * nd));
+    double* const dx = c->gl_xw.as<double>();
+    double* const dw = dx + npt;
+    double* const phT = dw + npt;
+    double* const phE = phT + (size_t)rT * npt;
+    double* const pwT = phE + (size_t)rE * npt;
+    double* const pwE = pwT + (size_t)rT * ST.nlp;
+    double* const H = pwE + (size_t)rE * SE.nlp;
+    MCMB_TRY(c->sg_M.alloc(sizeof(double) * (size_t)((long)rS * rP * ldP)));
+    MCMB_TRY(c->sg_Ys.alloc(sizeof(double) * (size_t)((long)rS * ldq)));
+    MCMB_TRY(c->sg_G.alloc(sizeof(double) * (size_t)((long)NPS * ldq)));
+    // the three GEMMs' tasks and tiles (groups: the m tiles of one B panel)
+    std::vector<SepTask> t1(1), t2(rS), t3(1);
+    std::vector<int4> g1, g2, g3;
+    t1[0] = SepTask{0, 0, 0, rP};
+    for (int n0 = 0; n0 < NPP_; n0 += SG_TN)
+        for (int m0 = 0; m0 < rS * rP; m0 += SG_TM) g1.push_back(make_int4(0, m0, n0, 0));
+    for (int g = 0; g < rS; ++g) {
+        t2[g] = SepTask{(long)g * rP, (long)g * ldq, 0, rP};
+        for (int n0 = 0; n0 < NPP_; n0 += SG_TN)
+            for (int m0 = 0; m0 < NPP_ && m0 < n0 + SG_TN; m0 += SG_TM) g2.push_back(make_int4(g, m0, n0, 0));
+    }
+    t3[0] = SepTask{0, 0, 0, rS};
+    for (long n0 = 0; n0 < NQ; n0 += SG_TN)
+        for (int m0 = 0; m0 < NPS; m0 += SG_TM) g3.push_back(make_int4(0, m0, (int)n0, 0));
+    std::vector<int> buf;
+    const size_t ot1 = append(buf, t1), og1 = append(buf, g1), ot2 = append(buf, t2), og2 = append(buf, g2),
+                 ot3 = append(buf, t3), og3 = append(buf, g3);
+    MCMB_TRY(c->gl_tab.alloc(sizeof(int) * buf.size()));
+    const int* tb = c->gl_tab.as<int>();
+    MCMB_HIP(hipMemcpyAsync(c->gl_tab.p, buf.data(), sizeof(int) * buf.size(), hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(dx, x, sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(dw, w, sizeof(double) * npt, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipEventRecord(c->ev0, c->stream));
+    // the compressed bases at the nodes
+    struct Fd {
+        const SepComp* S;
+        int nl, lmin;
+        const double* v;
+        double *pw, *ph;
+    };
+    const Fd fd[2] = {{&ST, nlT, L.lmin, L.d_v.as<double>(), pwT, phT}, {&SE, nlE, E.lmin, E.d_v.as<double>(), pwE, phE}};
+    for (const Fd& f : fd) {
+        const int r = f.S->r;
+        const long ntab = (long)r * f.S->nlp;
+        hipLaunchKernelGGL(sep_gl_tabs, dim3((unsigned)((ntab + 255) / 256)), dim3(256), 0, c->stream,
+                           (const double*)f.S->pc.as<double>(), f.S->nlp, f.nl, r, f.lmin, f.v, f.pw);
+        hipLaunchKernelGGL(sep_gl_phi, dim3((unsigned)((npt + 63) / 64), (unsigned)((r + 4 * GL_AG - 1) / (4 * GL_AG))),
+                           dim3(256), 0, c->stream, (const double*)f.pw, f.S->nlp, f.nl, r, (const double*)dx, npt,
+                           f.ph);
+    }
+    const double* const phP = tte ? phT : phE;
+    const double* const phS = tte ? phE : phT;
+    MCMB_HIP(hipMemsetAsync(H, 0, sizeof(double) * (size_t)rP * ldh, c->stream));
+    hipLaunchKernelGGL(sep_gl_core<true>, dim3((unsigned)(rP * (rP + 1) / 2)), dim3(256), 0, c->stream, phP, phS, rS,
+                       npt, (const double*)dw, rP, 1.0 / (16.0 * M_PI), H, ldh);
+    MCMB_HIP(hipGetLastError());
+    double* const T1 = c->sg_M.as<double>();
+    double* const Y = c->sg_Ys.as<double>();
+    double* const K = c->sg_G.as<double>();
+    const int gs1 = (rS * rP + SG_TM - 1) / SG_TM, gs2 = (NPP_ + SG_TM - 1) / SG_TM, gs3 = (NPS + SG_TM - 1) / SG_TM;
+    const int nt1 = (int)g1.size(), nt2 = (int)g2.size(), nt3 = (int)g3.size();
+    MCMB_TRY(launch_guard("GL expansion", sep_blocks(nt1, gs1), 256));
+    hipLaunchKernelGGL((sep_gemm<true, 4, SG_FM, SG_FN>), dim3(sep_blocks(nt1, gs1)), dim3(256), 0, c->stream,
+                       (const double*)H, (int)ldh, tb, (const double*)SP.ct.as<double>(), (long)ldP, T1, (long)ldP,
+                       rS * rP, NPP_, reinterpret_cast<const SepTask*>(tb + ot1),
+                       reinterpret_cast<const int4*>(tb + og1), nt1, gs1, 0);
+    MCMB_TRY(launch_guard("GL expansion", sep_blocks(nt2, gs2), 256));
+    hipLaunchKernelGGL((sep_gemm<true, 4, SG_FM, SG_FN, true>), dim3(sep_blocks(nt2, gs2)), dim3(256), 0, c->stream,
+                       (const double*)SP.ct.as<double>(), ldP, tb, (const double*)T1, (long)ldP, Y, ldq, NPP_, NPP_,
+                       reinterpret_cast<const SepTask*>(tb + ot2), reinterpret_cast<const int4*>(tb + og2), nt2, gs2,
+                       0);
+    MCMB_TRY(launch_guard("GL expansion", sep_blocks(nt3, gs3), 256));
+    hipLaunchKernelGGL((sep_gemm<true, 4, SG_FM, SG_FN>), dim3(sep_blocks(nt3, gs3)), dim3(256), 0, c->stream,
+                       (const double*)SS.ct.as<double>(), ldS, tb, (const double*)Y, (long)ldq, K, (long)ldq, NPS,
+                       (int)NQ, reinterpret_cast<const SepTask*>(tb + ot3), reinterpret_cast<const int4*>(tb + og3),
+                       nt3, gs3, 0);
+    const int n = E.n;
+    constexpr int SB = 6;
+    const int nt = (n + 31) / 32, nbk = (nt + SB - 1) / SB;
+    const unsigned long long ntile = ((unsigned long long)nbk * (nbk + 1) / 2 + 7) / 8 * 8 * SB * SB;
+    MCMB_TRY(launch_guard("GL assembly", ntile, 256));
+    if (tte)
+        hipLaunchKernelGGL((sep_assemble_mix<1, SB>), dim3((unsigned)ntile), dim3(256), 0, c->stream,
+                           (const double*)K, ldq, RT, RE, E.d_modes[0].as<int>(), n, nt, d_gamma);
+    else
+        hipLaunchKernelGGL((sep_assemble_mix<2, SB>), dim3((unsigned)ntile), dim3(256), 0, c->stream,
+                           (const double*)K, ldq, RT, RE, E.d_modes[1].as<int>(), n, nt, d_gamma);
+    MCMB_HIP(hipGetLastError());
+    MCMB_HIP(hipEventRecord(c->ev1, c->stream));
+    MCMB_HIP(hipStreamSynchronize(c->stream));   // x, w and the tables may live on the host stack
+    float ms = 0.f;
+    MCMB_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = 11;
+    c->nphases = 0;
+    c->last_rank = rP;   // the basis on two legs
     *used = true;
     return MCMB_OK;
 }

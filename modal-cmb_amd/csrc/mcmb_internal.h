@@ -277,6 +277,9 @@ struct mcmb_ctx {
     int sg_u3_ldu = 0, sg_u3_nl = 0;
     mcmb::SepPlan sep;
     mcmb::SepComp sepc;
+    mcmb::SepComp sepc_e;    // the same for the E field's basis (polarised GL Gram)
+    mcmb::SepComp sepc_pt;   // the T basis over the columns the mixed polarised modes use
+    mcmb::DevBuf gl_tab;     // task and tile tables of the mixed polarised GL expansion
     // per-launch timing of the last call that records phases (event k+1 - event k)
     hipEvent_t evp[mcmb::MAX_PHASES + 1] = {};
     int nphases = 0;
@@ -355,6 +358,15 @@ int gram_gl_launch(mcmb_ctx* ctx, int npt, double* d_gamma);
 // compressed basis (or MCMB_GL_PATH=direct): the caller then takes the direct
 // node sum.
 int gram_gl_compressed(mcmb_ctx* ctx, int npt, const double* x, const double* w, double* d_gamma, bool* used);
+// The EEE component of the polarised GL Gram over the compressed E basis
+// (its first R columns): as gram_gl_compressed, on the E setup.
+int gram_gl_eee_compressed(mcmb_ctx* ctx, int R, int npt, const double* x, const double* w, double* d_gamma,
+                           bool* used);
+// The TTE (pol 1) and TEE (pol 2) components over the compressed T and E
+// bases (RT, RE: the columns their modes use); *used false when either
+// basis has no low-rank factorisation (then the direct node sum applies).
+int gram_gl_mixed_compressed(mcmb_ctx* ctx, int pol, int RT, int RE, int npt, const double* x, const double* w,
+                             double* d_gamma, bool* used);
 // Look up this device's scratch array of the allocation-free GL Gram (once per
 // device, with the device current).
 int gl_scratch_setup(int device);

@@ -4,6 +4,7 @@ restatement of calculate_orthol, TTE / TEE also against the direct sum over
 the mixed tetrapyd; the reference-named shims (calculate_orthol,
 find_perm_late_TTE/TEE/EEE, plijk_TTE/TEE/EEE) over a bound context."""
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
@@ -70,6 +71,54 @@ def test_gl_pol_components_vs_oracle(ctx, Tr, Er, RT, RE):
             assert relmax(G, O.gamma_xxy_direct(pol, T, E, order)) < 1e-12
     # TTT on its own node count is the auto GL Gram
     assert relmax(ctx.gamma_gl_pol(0, 3 * Tr[1] // 2 + 1), ctx.gamma_gl()) < 1e-14
+
+
+def test_gl_eee_compressed_equals_direct_node_sum(ctx):
+    """A Legendre-type E basis at lmax 2500 with 30 columns and all 4960 EEE
+    modes: the EEE component runs over the compressed pair basis (rank 2R - 1
+    = 59) and equals the direct node sum (MCMB_GL_PATH=direct) to rounding;
+    the E basis has its own cached factorisation beside the T basis's."""
+    T, E = field(2, 2500, 30, 5), field(2, 2500, 30, 6)
+    setup(ctx, T, E, 30, 30, n=4960)
+    G = ctx.gamma_gl_pol(modalcmb.POL_EEE)
+    assert ctx.last_basis_rank() == 59
+    os.environ["MCMB_GL_PATH"] = "direct"
+    try:
+        Gd = ctx.gamma_gl_pol(modalcmb.POL_EEE)
+        assert ctx.last_basis_rank() == 0
+    finally:
+        del os.environ["MCMB_GL_PATH"]
+    d = np.sqrt(np.outer(np.abs(np.diag(Gd)), np.abs(np.diag(Gd))))
+    assert float((np.abs(G - Gd) / d).max()) < 1e-13
+    assert np.array_equal(G, G.T)
+    # the T basis's compression is kept apart: TTT and EEE alternate, each bit-stable
+    Gt = ctx.gamma_gl()
+    assert ctx.last_basis_rank() == 59
+    assert np.array_equal(ctx.gamma_gl_pol(modalcmb.POL_EEE), G)
+    assert np.array_equal(ctx.gamma_gl(), Gt)
+
+
+@pytest.mark.parametrize("RT,RE,n", [(30, 30, 4960), (12, 20, 600), (20, 9, 160)])
+def test_gl_mixed_compressed_equals_direct_node_sum(ctx, RT, RE, n):
+    """TTE and TEE over the compressed T and E bases (ranks 2 RT - 1 and 2 RE
+    - 1) against the direct node sum (MCMB_GL_PATH=direct): the full lmax
+    2500 / 30-column size with 4960 modes, and unequal widths either way round
+    (the field on two legs wider or narrower than the other)."""
+    T, E = field(2, 2500, RT, 7), field(10, 2400, RE, 8)
+    setup(ctx, T, E, RT, RE, n=n)
+    for pol, rank in ((modalcmb.POL_TTE, 2 * RT - 1), (modalcmb.POL_TEE, 2 * RE - 1)):
+        G = ctx.gamma_gl_pol(pol)
+        assert ctx.last_basis_rank() == rank, pol
+        os.environ["MCMB_GL_PATH"] = "direct"
+        try:
+            Gd = ctx.gamma_gl_pol(pol)
+            assert ctx.last_basis_rank() == 0
+        finally:
+            del os.environ["MCMB_GL_PATH"]
+        d = np.sqrt(np.outer(np.abs(np.diag(Gd)), np.abs(np.diag(Gd))))
+        assert float((np.abs(G - Gd) / d).max()) < 1e-13, pol
+        assert np.array_equal(G, G.T)
+        assert np.array_equal(ctx.gamma_gl_pol(pol), G)
 
 
 def test_gl_pol_lmax1000_vs_oracle(ctx):

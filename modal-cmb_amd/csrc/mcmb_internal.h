@@ -253,6 +253,7 @@ struct mcmb_ctx {
     mcmb::CrossSet xs;   // q-tilde of the cross-Gamma
     mcmb::PolSet es;     // E-mode field of the late basis (polarised Gram)
     mcmb::DevBuf d_W;    // full-cube projection weight W[i][j][k] (k-space)
+    mcmb::DevBuf d_wrow, d_wtile;   // W's row of each pair (i, j); each W row tile's nonzero K chunks
     mcmb::DevBuf pz, py, ph, pd, psc, pdm;   // projection batch buffers (Z, Y, H, partials, sin / cos, D)
     mcmb::DevBuf pio;   // a projection call's shapes, inputs and results on the device
     // stage-1 products Y = W Z of the last forward projection, per batch of

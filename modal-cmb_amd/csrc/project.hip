@@ -67,6 +67,37 @@ __global__ void proj_build_W(const double* __restrict__ kn, const double* __rest
     W[e] = (T)w;
 }
 
+// The K chunks of each PT-row tile of W that hold a nonzero weight: tiles[t]
+// = (first chunk, chunk count) over the tile's rows (an all-zero tile keeps
+// chunk 0).  The stage-1 GEMMs run only over those chunks: the others add
+// exact zeros.
+template <class T>
+__global__ __launch_bounds__(256) void proj_w_tiles(const T* __restrict__ W, long M, int Kp,
+                                                    int2* __restrict__ tiles)
+{
+    __shared__ int lo, hi;
+    if (threadIdx.x == 0) {
+        lo = Kp;
+        hi = -1;
+    }
+    __syncthreads();
+    const long r0 = (long)blockIdx.x * PT, ne = (long)min((long)PT, M - r0) * Kp;
+    int mlo = Kp, mhi = -1;
+    for (long e = threadIdx.x; e < ne; e += 256)
+        if (W[r0 * Kp + e] != (T)0) {
+            const int k = (int)(e % Kp);
+            mlo = min(mlo, k);
+            mhi = max(mhi, k);
+        }
+    atomicMin(&lo, mlo);
+    atomicMax(&hi, mhi);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int c0 = hi < 0 ? 0 : lo / PK, c1 = hi < 0 ? 0 : hi / PK;
+        tiles[blockIdx.x] = make_int2(c0, c1 - c0 + 1);
+    }
+}
+
 // Z[k][col], col = (b*2 + c)*R + t for shapes b < nb of the batch: c = 0
 // cos(omega_b k) q_t(k), c = 1 sin(omega_b k) q_t(k); Kp x ldz.
 template <class T>
@@ -198,9 +229,11 @@ __global__ void proj_sincos_tab(const double* __restrict__ kn, const double* __r
 // W_ijk {sin, cos}(omega K + phi) q_t(k), x = 0 (sin: the value) and x = 1
 // (cos: the derivatives).  From stage 1's Yc, Ys (cos / sin omega k_k
 // columns) by angle addition: G_0 = sin A Yc + cos A Ys, G_1 = cos A Yc -
-// sin A Ys, A = omega (k_i + k_j) + phi.
+// sin A Ys, A = omega (k_i + k_j) + phi.  Y's row of the pair (i, j) is
+// rowof[pair_row(i, j)] (stage 1 runs over W's rows in their own order).
 template <class T>
-__global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int ldy, const double* __restrict__ kn,
+__global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int ldy, const int* __restrict__ rowof,
+                                                   const double* __restrict__ kn,
                                                    const double* __restrict__ q, int ldq, int N, int R,
                                                    const double* __restrict__ theta, int nb,
                                                    const double* __restrict__ scb, double* __restrict__ H)
@@ -230,7 +263,7 @@ __global__ __launch_bounds__(256) void proj_stage2(const T* __restrict__ Y, int 
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const int j = min(j0 + jr + 8 * u, N - 1), tt = min(t, R - 1);
-            const T* y = Y + (size_t)pair_row(i, j, N) * ldy + (size_t)b * 2 * R + tt;
+            const T* y = Y + (size_t)rowof[pair_row(i, j, N)] * ldy + (size_t)b * 2 * R + tt;
             yv[u][0] = y[0];
             yv[u][1] = y[R];
             qv[u] = q[(size_t)j * ldq + tt];
@@ -421,7 +454,8 @@ __global__ __launch_bounds__(64) void proj_hbar(const double* __restrict__ gbar,
 // dpart[b][i] = (k_i P_i, P_i / 3), P_i = sum_{j,t} Gbar(i,j,t) G_1(i,j,t), for
 // one (b, i) per workgroup
 template <class T>
-__global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int ldy, const double* __restrict__ kn,
+__global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int ldy, const int* __restrict__ rowof,
+                                                    const double* __restrict__ kn,
                                                     const double* __restrict__ q, int ldq, int N, int R,
                                                     const double* __restrict__ theta, int nb,
                                                     const double* __restrict__ scb,
@@ -457,7 +491,7 @@ __global__ __launch_bounds__(256) void proj_adjoint(const T* __restrict__ Y, int
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int j = min(j0 + jm + 4 * v + (lane >> 4), N - 1), tt = min(t, R - 1);
-            const T* y = Y + (size_t)pair_row(i, j, N) * ldy + (size_t)b * 2 * R + tt;
+            const T* y = Y + (size_t)rowof[pair_row(i, j, N)] * ldy + (size_t)b * 2 * R + tt;
             yv[v][0] = y[0];
             yv[v][1] = y[R];
             sv[v] = sc[2 * j];
@@ -539,13 +573,41 @@ static int ensure_W(mcmb_ctx* c, int prec)
     const int N = K.lmax + 1, Kp = (N + PK - 1) / PK * PK;
     if (c->w_ready == prec + 1) return MCMB_OK;
     const long M = (long)N * (N + 1) / 2;
+    // The pair rows i <= j in an order that groups rows of like nonzero k
+    // range: W_ijk is nonzero only for k_k in [k_j - k_i, k_i + k_j] (the
+    // triangle), so with the rows sorted by (upper end / PK, lower end) most
+    // K chunks of a PT-row tile are zero throughout, and the GEMMs skip them
+    // (43% of the chunks remain at 300 nodes, against 87% in i-major pair
+    // order).  Each row's k-sum is unchanged, so Y holds the same values in
+    // W's row order; wrow[pair_row(i, j)] is the row of (i, j).
+    const std::vector<double>& kn = K.kn;
     std::vector<int2> rowij;
+    std::vector<long> key;
     rowij.reserve(M);
+    key.reserve(M);
     for (int i = 0; i < N; ++i)
-        for (int j = i; j < N; ++j) rowij.push_back(make_int2(i, j));
+        for (int j = i; j < N; ++j) {
+            const long lo = std::lower_bound(kn.begin(), kn.end(), kn[j] - kn[i]) - kn.begin();
+            const long hi = std::upper_bound(kn.begin(), kn.end(), kn[i] + kn[j]) - kn.begin();
+            key.push_back((hi / PK) * (N + 1) + lo);
+            rowij.push_back(make_int2(i, j));
+        }
+    std::vector<long> ord(M);
+    for (long r = 0; r < M; ++r) ord[r] = r;
+    std::stable_sort(ord.begin(), ord.end(), [&](long a, long b) { return key[a] < key[b]; });
+    std::vector<int2> rows(M);
+    std::vector<int> wrow(M);
+    for (long r = 0; r < M; ++r) {
+        rows[r] = rowij[ord[r]];
+        wrow[ord[r]] = (int)r;   // rowij is in pair_row order
+    }
     DevBuf d_ij;
     MCMB_TRY(d_ij.alloc(sizeof(int2) * (size_t)M));
-    MCMB_HIP(hipMemcpyAsync(d_ij.p, rowij.data(), sizeof(int2) * (size_t)M, hipMemcpyHostToDevice, c->stream));
+    MCMB_HIP(hipMemcpyAsync(d_ij.p, rows.data(), sizeof(int2) * (size_t)M, hipMemcpyHostToDevice, c->stream));
+    MCMB_TRY(c->d_wrow.alloc(sizeof(int) * (size_t)M));
+    MCMB_HIP(hipMemcpyAsync(c->d_wrow.p, wrow.data(), sizeof(int) * (size_t)M, hipMemcpyHostToDevice, c->stream));
+    const unsigned ntile = (unsigned)((M + PT - 1) / PT);
+    MCMB_TRY(c->d_wtile.alloc(sizeof(int2) * ntile));
     size_t tot = (size_t)M * Kp;
     MCMB_TRY(launch_guard("projection weight cube", (tot + 255) / 256, 256));
     if (prec == MCMB_FP64) {
@@ -553,11 +615,15 @@ static int ensure_W(mcmb_ctx* c, int prec)
         hipLaunchKernelGGL(proj_build_W<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
                            K.d_kn.as<double>(), K.d_kw.as<double>(), N, Kp, d_ij.as<int2>(), M,
                            c->d_W.as<double>());
+        hipLaunchKernelGGL(proj_w_tiles<double>, dim3(ntile), dim3(256), 0, c->stream,
+                           (const double*)c->d_W.as<double>(), M, Kp, c->d_wtile.as<int2>());
     } else {
         MCMB_TRY(c->d_W.alloc(tot * sizeof(float)));
         hipLaunchKernelGGL(proj_build_W<float>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream,
                            K.d_kn.as<double>(), K.d_kw.as<double>(), N, Kp, d_ij.as<int2>(), M,
                            c->d_W.as<float>());
+        hipLaunchKernelGGL(proj_w_tiles<float>, dim3(ntile), dim3(256), 0, c->stream,
+                           (const float*)c->d_W.as<float>(), M, Kp, c->d_wtile.as<int2>());
     }
     MCMB_HIP(hipGetLastError());
     MCMB_HIP(hipStreamSynchronize(c->stream));   // the pair table goes out of scope
@@ -572,17 +638,18 @@ static int stage1_ld(int nb, int R)
     return (nb * 2 * R + PT - 1) / PT * PT;
 }
 
-// FP32 stage-1 GEMM C[M][ldc] = A[M][Kp] B[Kp][ldb] (B zero-padded to ldb
-// columns and Kp rows, A to Kp columns): 128 x 128 tiles, 4 waves of 64 x 64
-// on v_mfma_f32_16x16x4f32, K chunks of 16 in double-buffered LDS.  Operands
-// come in as 16-byte loads two chunks ahead of their use (two register
-// sets), A stored [m][k] and B [k][n] with row strides (20 and 144 floats)
-// that keep both the 16-byte stores and the MFMA operand reads free of bank
-// conflicts.  The K order is that of proj_gemm<float>, so the results are
-// the same bit for bit.
+// FP32 stage-1 GEMM C[m][ldc] = A[m][Kp] B[Kp][ldb] (B zero-padded to ldb
+// columns and Kp rows, A to Kp columns): 128 x 128 tiles, 4 waves of
+// 64 x 64 on v_mfma_f32_16x16x4f32, K chunks of 16 in double-buffered LDS,
+// over the tile's chunks ktiles[tile] = (first, count) only (A is zero in
+// the others).  Operands come in as 16-byte loads two chunks ahead of their
+// use (two register sets), A stored [m][k] and B [k][n] with row strides (20
+// and 144 floats) that keep both the 16-byte stores and the MFMA operand
+// reads free of bank conflicts.  The chunks run in K order.
 constexpr int PF_LDA = PK + 4, PF_LDB = PT + 16;
 __global__ __launch_bounds__(256, 2) void proj_gemm_f32(const float* __restrict__ A, const float* __restrict__ B,
-                                                        int M, int Kp, int ldb, float* __restrict__ C, int ldc)
+                                                        int M, int Kp, int ldb, float* __restrict__ C, int ldc,
+                                                        const int2* __restrict__ ktiles)
 {
     __shared__ __align__(16) float As[2][PT][PF_LDA];
     __shared__ __align__(16) float Bs[2][PK][PF_LDB];
@@ -631,22 +698,24 @@ __global__ __launch_bounds__(256, 2) void proj_gemm_f32(const float* __restrict_
                     acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[a], bv[b], acc[a][b], 0, 0, 0);
         }
     };
-    // chunk kc is in LDS buffer kc & 1, chunk kc + 1 in registers and chunk
-    // kc + 2 loading; loads past the last chunk repeat it (unused), so every
-    // load is unconditional
-    const int nk = Kp / PK;
+    // the tile's chunks c0 .. c0 + nk - 1 (the rest of W's rows here are
+    // zero); chunk kc is in LDS buffer kc & 1, chunk kc + 1 in registers and
+    // chunk kc + 2 loading; loads past the last chunk repeat it (unused), so
+    // every load is unconditional
+    const int2 kr = ktiles[blockIdx.y];
+    const int c0 = kr.x, nk = kr.y;
     f4 ra0[2], rb0[2], ra1[2], rb1[2];
-    gload(0, ra0, rb0);
-    gload(min(1, nk - 1) * PK, ra1, rb1);
+    gload(c0 * PK, ra0, rb0);
+    gload((c0 + min(1, nk - 1)) * PK, ra1, rb1);
     sstore(0, ra0, rb0);
     __syncthreads();
     for (int kc = 0;; kc += 2) {
-        gload(min(kc + 2, nk - 1) * PK, ra0, rb0);
+        gload((c0 + min(kc + 2, nk - 1)) * PK, ra0, rb0);
         compute(0);
         sstore(1, ra1, rb1);
         __syncthreads();
         if (kc + 1 >= nk) break;
-        gload(min(kc + 3, nk - 1) * PK, ra1, rb1);
+        gload((c0 + min(kc + 3, nk - 1)) * PK, ra1, rb1);
         compute(1);
         sstore(0, ra0, rb0);
         __syncthreads();
@@ -673,7 +742,8 @@ __global__ __launch_bounds__(256, 2) void proj_gemm_f32(const float* __restrict_
 constexpr int PD_LDA = PK + 2, PD_LDB = PT + 8;
 typedef double d2v __attribute__((ext_vector_type(2)));
 __global__ __launch_bounds__(256, 2) void proj_gemm_f64(const double* __restrict__ A, const double* __restrict__ B,
-                                                        int M, int Kp, int ldb, double* __restrict__ C, int ldc)
+                                                        int M, int Kp, int ldb, double* __restrict__ C, int ldc,
+                                                        const int2* __restrict__ ktiles)
 {
     __shared__ __align__(16) double As[2][PT][PD_LDA];
     __shared__ __align__(16) double Bs[2][PK][PD_LDB];
@@ -720,21 +790,21 @@ __global__ __launch_bounds__(256, 2) void proj_gemm_f64(const double* __restrict
                     acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
         }
     };
-    // chunk kc is in LDS buffer kc & 1, chunk kc + 1 in registers and chunk
-    // kc + 2 loading; loads past the last chunk repeat it (unused)
-    const int nk = Kp / PK;
+    // the tile's chunks c0 .. c0 + nk - 1, as proj_gemm_f32
+    const int2 kr = ktiles[blockIdx.y];
+    const int c0 = kr.x, nk = kr.y;
     d2v ra0[4], rb0[4], ra1[4], rb1[4];
-    gload(0, ra0, rb0);
-    gload(min(1, nk - 1) * PK, ra1, rb1);
+    gload(c0 * PK, ra0, rb0);
+    gload((c0 + min(1, nk - 1)) * PK, ra1, rb1);
     sstore(0, ra0, rb0);
     __syncthreads();
     for (int kc = 0;; kc += 2) {
-        gload(min(kc + 2, nk - 1) * PK, ra0, rb0);
+        gload((c0 + min(kc + 2, nk - 1)) * PK, ra0, rb0);
         compute(0);
         sstore(1, ra1, rb1);
         __syncthreads();
         if (kc + 1 >= nk) break;
-        gload(min(kc + 3, nk - 1) * PK, ra1, rb1);
+        gload((c0 + min(kc + 3, nk - 1)) * PK, ra1, rb1);
         compute(1);
         sstore(0, ra0, rb0);
         __syncthreads();
@@ -769,10 +839,12 @@ static int run_stage1(mcmb_ctx* c, const double* d_theta, int nb, int& ldy, DevB
     dim3 grid((unsigned)(ldz / PT), (unsigned)((M + PT - 1) / PT));
     if (std::is_same<T, float>::value)
         hipLaunchKernelGGL(proj_gemm_f32, grid, dim3(256), 0, c->stream, (const float*)c->d_W.as<T>(),
-                           (const float*)Zb.as<T>(), (int)M, Kp, ldz, (float*)Yb.as<T>(), ldy);
+                           (const float*)Zb.as<T>(), (int)M, Kp, ldz, (float*)Yb.as<T>(), ldy,
+                           (const int2*)c->d_wtile.as<int2>());
     else
         hipLaunchKernelGGL(proj_gemm_f64, grid, dim3(256), 0, c->stream, (const double*)c->d_W.as<T>(),
-                           (const double*)Zb.as<T>(), (int)M, Kp, ldz, (double*)Yb.as<T>(), ldy);
+                           (const double*)Zb.as<T>(), (int)M, Kp, ldz, (double*)Yb.as<T>(), ldy,
+                           (const int2*)c->d_wtile.as<int2>());
     MCMB_HIP(hipGetLastError());
     return MCMB_OK;
 }
@@ -802,7 +874,7 @@ static int project_fwd_batch(mcmb_ctx* c, const double* d_theta, int nb, double*
     MCMB_TRY(Hb.alloc(sizeof(double) * (size_t)PBATCH * 2 * N * R * R));
     MCMB_TRY(sincos_table(c, d_theta, nb));
     hipLaunchKernelGGL(proj_stage2<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
-                       K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb,
+                       (const int*)c->d_wrow.as<int>(), K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb,
                        (const double*)c->psc.as<double>(), Hb.as<double>());
     MCMB_TRY(c->pdm.alloc(sizeof(double) * (size_t)PBATCH * 3 * R * R * R));
     hipLaunchKernelGGL(proj_stage3, dim3((unsigned)(nb * 2 * ((R * R + 15) / 16))), dim3(64), 0, c->stream,
@@ -839,7 +911,7 @@ static int project_bwd_batch(mcmb_ctx* c, const double* d_theta, int nb, const d
                        N, R, n, nb, Hb.as<double>());
     MCMB_TRY(sincos_table(c, d_theta, nb));
     hipLaunchKernelGGL(proj_adjoint<T>, dim3((unsigned)(nb * N)), dim3(256), 0, c->stream, Yb.as<T>(), ldy,
-                       K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb,
+                       (const int*)c->d_wrow.as<int>(), K.d_kn.as<double>(), K.d_q.as<double>(), K.ldq, N, R, d_theta, nb,
                        (const double*)c->psc.as<double>(), Hb.as<double>(), Dp.as<double>());
     hipLaunchKernelGGL(proj_sum_parts, dim3((unsigned)(nb * 2)), dim3(64), 0, c->stream, Dp.as<double>(), N, nb,
                        d_dtheta);
